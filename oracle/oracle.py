@@ -1,0 +1,283 @@
+"""ctypes front end of the oracle libraries (TEST INFRASTRUCTURE ONLY).
+
+``Oracle`` drives oracle/libramoracle.so, the C restatement of the RAM split
+step; ``Reference`` drives oracle/_ref/libramref.so, the reference's own
+Fortran drift operators behind oracle/ref_harness.f90.  Both take the same
+host arrays in the reference's Fortran layout.  Only tests/, the smoke check
+and the bench's CPU-baseline leg use this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ORACLE_SO = HERE / "libramoracle.so"
+REF_SO = HERE / "_ref" / "libramref.so"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class _State(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("ns", "nr", "nt", "ne", "npa")] + \
+        [(n, C.c_double) for n in ("dts", "betalim", "fraccfl", "mdr", "dphi", "conf1", "conf2")] + \
+        [("charge", _ip), ("cex_kind", _ip), ("rmas", _dp), ("rlz", _dp),
+         ("ekev", _dp), ("ebnd", _dp), ("we", _dp), ("de", _dp),
+         ("grel", _dp), ("grbnd", _dp), ("v", _dp), ("mu", _dp), ("wmu", _dp), ("dmu", _dp),
+         ("upa", _dp), ("fnhs", _dp), ("fnis", _dp), ("bounhs", _dp), ("bounis", _dp),
+         ("hdns", _dp), ("didt", _dp), ("dibndt", _dp), ("bnes", _dp), ("dbdt", _dp),
+         ("vt", _dp), ("eir", _dp), ("eip", _dp), ("outside_mgnp", _ip), ("fgeos", _dp),
+         ("f2", _dp), ("dtdrift_r", _dp), ("dtdrift_p", _dp), ("dtdrift_e", _dp),
+         ("dtdrift_mu", _dp), ("setrc", _dp), ("elorc", _dp), ("lsdr", _dp), ("lscha", _dp),
+         ("lsatm", _dp)]
+
+
+class _Drift(C.Structure):
+    _fields_ = [("qs", C.c_int)] + [(n, _dp) for n in
+                                   ("vr", "p1", "p2", "edot", "mudot", "cdr", "cdp", "cde", "cdmu")]
+
+
+class _Loss(C.Structure):
+    _fields_ = [("charge", _dp), ("atlos", _dp)]
+
+
+def _fd(a):
+    a = np.asfortranarray(a, dtype=np.float64)
+    return a
+
+
+def _fi(a):
+    return np.asfortranarray(a, dtype=np.int32)
+
+
+def _ptr(a, typ=_dp):
+    return a.ctypes.data_as(typ)
+
+
+class Oracle:
+    """The C restatement, holding its own copies of one problem's arrays."""
+
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            if not ORACLE_SO.exists():
+                raise FileNotFoundError(f"{ORACLE_SO} is not built (make -C oracle)")
+            lib = C.CDLL(str(ORACLE_SO))
+            sp = C.POINTER(_State)
+            for name in ("driftpara", "driftr", "driftp", "drifte", "driftmu"):
+                f = getattr(lib, "oracle_" + name)
+                f.argtypes = [sp, C.c_int, C.POINTER(_Drift)]
+                f.restype = None
+            lib.oracle_cepara.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
+            lib.oracle_charexchange.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
+            lib.oracle_atmol.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
+            lib.oracle_sumrc.argtypes = [sp, C.c_int]
+            lib.oracle_ram_run.argtypes = [sp, C.c_int, _dp]
+            lib.oracle_ram_run.restype = C.c_int
+            lib.oracle_drift_alloc.argtypes = [sp, C.POINTER(_Drift)]
+            lib.oracle_drift_alloc.restype = C.c_int
+            lib.oracle_drift_free.argtypes = [C.POINTER(_Drift)]
+            lib.oracle_loss_alloc.argtypes = [sp, C.POINTER(_Loss)]
+            lib.oracle_loss_alloc.restype = C.c_int
+            lib.oracle_loss_free.argtypes = [C.POINTER(_Loss)]
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, problem):
+        g, geo = problem.grid, problem.geom
+        self.ns, self.nr, self.nt, self.ne, self.npa = g.ns, g.nr, g.nt, g.ne, g.npa
+        keep = {}
+
+        def d(name, a):
+            keep[name] = _fd(a)
+            return _ptr(keep[name])
+
+        def i(name, a):
+            keep[name] = _fi(a)
+            return _ptr(keep[name], _ip)
+
+        st = _State()
+        st.ns, st.nr, st.nt, st.ne, st.npa = g.ns, g.nr, g.nt, g.ne, g.npa
+        st.dts, st.betalim, st.fraccfl = problem.dts, problem.betalim, problem.fraccfl
+        st.mdr, st.dphi, st.conf1, st.conf2 = g.mdr, g.dphi, g.conf1, g.conf2
+        st.charge = i("charge", g.charge)
+        st.cex_kind = i("cex_kind", g.cex_kind)
+        st.rmas = d("rmas", g.rmas)
+        st.rlz = d("rlz", g.rlz)
+        for n in ("ekev", "ebnd", "we", "de", "grel", "grbnd", "v", "mu", "wmu", "dmu", "upa"):
+            setattr(st, n, d(n, getattr(g, n)))
+        for n in ("fnhs", "fnis", "bounhs", "bounis", "hdns", "didt", "dibndt", "bnes",
+                  "dbdt", "eir", "eip"):
+            setattr(st, n, d(n, getattr(geo, n)))
+        st.outside_mgnp = i("outside_mgnp", geo.outside_mgnp)
+        st.vt = d("vt", problem.vt)
+        st.fgeos = d("fgeos", problem.fgeos)
+        st.f2 = d("f2", problem.f2.copy(order="F"))
+        for n in ("dtdrift_r", "dtdrift_p", "dtdrift_e", "dtdrift_mu", "setrc", "elorc",
+                  "lsdr", "lscha", "lsatm"):
+            setattr(st, n, d(n, np.zeros(g.ns)))
+        self._keep = keep
+        self.st = st
+        self._drift = {}
+        self._loss = {}
+
+    # -- state access -------------------------------------------------------
+    @property
+    def f2(self) -> np.ndarray:
+        return self._keep["f2"]
+
+    def set_f2(self, f2):
+        self._keep["f2"][...] = f2
+
+    def set_vt(self, vt):
+        self._keep["vt"][...] = vt
+
+    def set_fgeos(self, fgeos):
+        self._keep["fgeos"][...] = fgeos
+
+    def arr(self, name) -> np.ndarray:
+        return self._keep[name]
+
+    def dtdrift(self) -> np.ndarray:
+        """(NS, 4) DtDriftR, DtDriftP, DtDriftE, DtDriftMu."""
+        return np.stack([self._keep[n] for n in ("dtdrift_r", "dtdrift_p", "dtdrift_e",
+                                                 "dtdrift_mu")], axis=1)
+
+    # -- operators (species s is 1-based) -----------------------------------
+    def _d(self, s):
+        if s not in self._drift:
+            dr = _Drift()
+            if self.lib().oracle_drift_alloc(C.byref(self.st), C.byref(dr)):
+                raise MemoryError
+            self._drift[s] = dr
+        return self._drift[s]
+
+    def _l(self, s):
+        if s not in self._loss:
+            ls = _Loss()
+            if self.lib().oracle_loss_alloc(C.byref(self.st), C.byref(ls)):
+                raise MemoryError
+            self._loss[s] = ls
+        return self._loss[s]
+
+    def call(self, op: str, s: int):
+        lib = self.lib()
+        if op in ("driftpara", "driftr", "driftp", "drifte", "driftmu"):
+            getattr(lib, "oracle_" + op)(C.byref(self.st), s, C.byref(self._d(s)))
+        elif op in ("cepara", "charexchange", "atmol"):
+            getattr(lib, "oracle_" + op)(C.byref(self.st), s, C.byref(self._l(s)))
+        elif op == "sumrc":
+            lib.oracle_sumrc(C.byref(self.st), s)
+        else:
+            raise ValueError(op)
+
+    def drift_table(self, s: int, name: str) -> np.ndarray:
+        """A DRIFTPARA/drift coefficient array of species s as numpy (copy)."""
+        dr = self._d(s)
+        shapes = {"vr": (self.nr,), "p1": (self.nr,), "p2": (self.nr, self.ne),
+                  "edot": (self.nr, self.ne), "mudot": (self.nr, self.npa)}
+        shp = shapes.get(name, (self.nr, self.nt, self.ne, self.npa))
+        n = int(np.prod(shp))
+        return np.ctypeslib.as_array(getattr(dr, name), shape=(n,)).reshape(shp, order="F").copy()
+
+    def loss_table(self, s: int, name: str) -> np.ndarray:
+        ls = self._l(s)
+        shp = (self.nr, self.nt, self.ne, self.npa) if name == "charge" else (self.nr, self.ne)
+        n = int(np.prod(shp))
+        return np.ctypeslib.as_array(getattr(ls, name), shape=(n,)).reshape(shp, order="F").copy()
+
+    def ram_run(self, nthreads: int = 1) -> float:
+        out = C.c_double(0.0)
+        if self.lib().oracle_ram_run(C.byref(self.st), nthreads, C.byref(out)):
+            raise MemoryError("oracle_ram_run failed")
+        return out.value
+
+    def close(self):
+        lib = self.lib()
+        for dr in self._drift.values():
+            lib.oracle_drift_free(C.byref(dr))
+        for ls in self._loss.values():
+            lib.oracle_loss_free(C.byref(ls))
+        self._drift.clear()
+        self._loss.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Reference:
+    """The reference's own DRIFTPARA/DRIFTR/DRIFTP/DRIFTE/DRIFTMU (Fortran)."""
+
+    _lib = None
+    OPS = {"driftpara": 0, "driftr": 1, "driftp": 2, "drifte": 3, "driftmu": 4}
+
+    @classmethod
+    def available(cls) -> bool:
+        return REF_SO.exists()
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            # unresolved MPI entry points are never called: bind lazily
+            lib = C.CDLL(str(REF_SO), mode=os.RTLD_LAZY)
+            lib.ref_call.argtypes = [C.c_int, C.c_int]
+            lib.ref_get_f2.argtypes = [_dp]
+            lib.ref_put_f2.argtypes = [_dp]
+            lib.ref_get_dtdrift.argtypes = [_dp]
+            lib.ref_funt.argtypes = [C.c_double]
+            lib.ref_funt.restype = C.c_double
+            lib.ref_funi.argtypes = [C.c_double]
+            lib.ref_funi.restype = C.c_double
+            cls._lib = lib
+        return cls._lib
+
+    def __init__(self, problem):
+        g, geo = problem.grid, problem.geom
+        self.shape = (g.ns, g.nr, g.nt, g.ne, g.npa)
+        self.ns = g.ns
+        keep = [_fi(g.charge), _fd(g.rmas), _fd(g.rlz), _fd(g.ekev), _fd(g.ebnd), _fd(g.we),
+                _fd(g.de), _fd(g.grel), _fd(g.grbnd), _fd(g.v), _fd(g.mu), _fd(g.wmu),
+                _fd(g.dmu), _fd(g.upa), _fd(geo.fnhs), _fd(geo.fnis), _fd(geo.bounhs),
+                _fd(geo.bounis), _fd(geo.hdns), _fd(geo.didt), _fd(geo.dibndt), _fd(geo.bnes),
+                _fd(geo.dbdt), _fd(problem.vt), _fd(geo.eir), _fd(geo.eip),
+                _fi(geo.outside_mgnp), _fd(problem.fgeos), _fd(problem.f2)]
+        args = [C.c_int(v) for v in (g.ns, g.nr, g.nt, g.ne, g.npa)]
+        args += [C.c_double(v) for v in (problem.dts, problem.betalim, problem.fraccfl, g.mdr,
+                                         g.dphi, g.conf1, g.conf2)]
+        args += [_ptr(a, _ip if a.dtype == np.int32 else _dp) for a in keep]
+        self.lib().ref_setup(*args)
+
+    def call(self, op: str, s: int):
+        self.lib().ref_call(self.OPS[op], s)
+
+    @property
+    def f2(self) -> np.ndarray:
+        out = np.zeros(self.shape, order="F")
+        self.lib().ref_get_f2(_ptr(out))
+        return out
+
+    def set_f2(self, f2):
+        a = _fd(f2)
+        self.lib().ref_put_f2(_ptr(a))
+
+    def dtdrift(self) -> np.ndarray:
+        out = np.zeros((self.ns, 4), order="F")
+        self.lib().ref_get_dtdrift(_ptr(out))
+        return out
+
+    @classmethod
+    def funt(cls, x):
+        return cls.lib().ref_funt(x)
+
+    @classmethod
+    def funi(cls, x):
+        return cls.lib().ref_funi(x)

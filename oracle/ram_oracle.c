@@ -1,0 +1,621 @@
+/*
+ * ram_oracle.c -- CPU restatement of RAM's kinetic split step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path and the timed CPU
+ * baseline.  The product never loads it.
+ *
+ * Each routine follows the reference's loop order and expression order
+ * (left-to-right Fortran evaluation, integer powers as repeated products),
+ * so that, compiled without FMA contraction, it reproduces the reference's
+ * arithmetic.  The reference files are cited per routine.  Indices written
+ * in comments are the reference's 1-based ones; the macros below map them to
+ * 0-based storage in Fortran (column-major) order.
+ */
+#include "ram_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* src/ModRamConst.f90:13-22 (decimal literals rounded to double) */
+static const double RAM_Q = 1.602E-19;
+static const double RAM_CS = 2.998E8;
+/* Earth's angular velocity, src/ModRamDrift.f90:224,309,405 */
+static const double RAM_OME = 7.3E-5;
+
+#define NR1 (st->nr + 1)
+/* F2(S,I,J,K,L) */
+#define F2(S, I, J, K, L)                                                     \
+  st->f2[((S)-1) + (size_t)st->ns *                                           \
+                       (((I)-1) + (size_t)st->nr *                            \
+                                      (((J)-1) + (size_t)st->nt *             \
+                                                     (((K)-1) + (size_t)st->ne * ((L)-1))))]
+/* geometry (NR+1,NT,NPA) */
+#define G3(A, I, J, L) (A)[((I)-1) + (size_t)NR1 * (((J)-1) + (size_t)st->nt * ((L)-1))]
+/* fields (NR+1,NT) */
+#define G2(A, I, J) (A)[((I)-1) + (size_t)NR1 * ((J)-1)]
+#define MGNP(I, J) st->outside_mgnp[((I)-1) + (size_t)st->nr * ((J)-1)]
+#define FGEOS(S, J, K, L)                                                     \
+  st->fgeos[((S)-1) + (size_t)st->ns *                                        \
+                          (((J)-1) + (size_t)st->nt * (((K)-1) + (size_t)st->ne * ((L)-1)))]
+#define SK(A, S, K) (A)[((S)-1) + (size_t)st->ns * ((K)-1)]
+#define V1(A, I) (A)[(I)-1]
+/* drift scratch (NR,NT,NE,NPA), (NR,NE), (NR,NPA) */
+#define C4(A, I, J, K, L)                                                     \
+  (A)[((I)-1) + (size_t)st->nr * (((J)-1) + (size_t)st->nt * (((K)-1) + (size_t)st->ne * ((L)-1)))]
+#define RK(A, I, K) (A)[((I)-1) + (size_t)st->nr * ((K)-1)]
+
+static double dmax(double a, double b) { return a > b ? a : b; }
+static double dmin(double a, double b) { return a < b ? a : b; }
+
+/* The reference's beta limiter, LIMITER=MAX(MIN(BetaLim*R,1.),MIN(R,BetaLim))
+ * (src/ModRamDrift.f90:178). */
+static double limiter(double r, double beta) {
+  return dmax(dmin(beta * r, 1.0), dmin(r, beta));
+}
+
+int oracle_drift_alloc(const oracle_state *st, oracle_drift_scratch *d) {
+  size_t n4 = (size_t)st->nr * st->nt * st->ne * st->npa;
+  memset(d, 0, sizeof(*d));
+  d->vr = calloc(st->nr, sizeof(double));
+  d->p1 = calloc(st->nr, sizeof(double));
+  d->p2 = calloc((size_t)st->nr * st->ne, sizeof(double));
+  d->edot = calloc((size_t)st->nr * st->ne, sizeof(double));
+  d->mudot = calloc((size_t)st->nr * st->npa, sizeof(double));
+  d->cdr = calloc(n4, sizeof(double));
+  d->cdp = calloc(n4, sizeof(double));
+  d->cde = calloc(n4, sizeof(double));
+  d->cdmu = calloc(n4, sizeof(double));
+  if (!d->vr || !d->p1 || !d->p2 || !d->edot || !d->mudot || !d->cdr || !d->cdp ||
+      !d->cde || !d->cdmu) {
+    oracle_drift_free(d);
+    return -1;
+  }
+  return 0;
+}
+
+void oracle_drift_free(oracle_drift_scratch *d) {
+  free(d->vr); free(d->p1); free(d->p2); free(d->edot); free(d->mudot);
+  free(d->cdr); free(d->cdp); free(d->cde); free(d->cdmu);
+  memset(d, 0, sizeof(*d));
+}
+
+int oracle_loss_alloc(const oracle_state *st, oracle_loss_scratch *l) {
+  size_t n4 = (size_t)st->nr * st->nt * st->ne * st->npa;
+  l->charge = calloc(n4, sizeof(double));
+  l->atlos = calloc((size_t)st->nr * st->ne, sizeof(double));
+  if (!l->charge || !l->atlos) {
+    oracle_loss_free(l);
+    return -1;
+  }
+  return 0;
+}
+
+void oracle_loss_free(oracle_loss_scratch *l) {
+  free(l->charge); free(l->atlos);
+  l->charge = l->atlos = 0;
+}
+
+/* DRIFTPARA, src/ModRamDrift.f90:36-88. */
+void oracle_driftpara(const oracle_state *st, int s, oracle_drift_scratch *d) {
+  const double dts = st->dts, mdr = st->mdr, dphi = st->dphi;
+  d->qs = st->charge[s - 1];
+  const double qs = (double)d->qs;
+  for (int i = 1; i <= st->nr; ++i) {
+    const double rlz = V1(st->rlz, i);
+    /* :65-69 -- the inner J loop rewrites the same value NT times */
+    d->vr[i - 1] = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;
+    d->p1[i - 1] = dts / dphi / 2.0 / mdr / rlz;
+    for (int k = 1; k <= st->ne; ++k) {
+      const double g = SK(st->grel, s, k);
+      RK(d->p2, i, k) = dts * V1(st->ekev, k) * 1000.0 * (g + 1.0) / g / (rlz * rlz) / dphi / qs;
+    }
+  }
+  for (int i = 1; i <= st->nr; ++i) {
+    const double rlz = V1(st->rlz, i);
+    for (int l = 1; l <= st->npa - 1; ++l) { /* :77-80 */
+      const double muboun = V1(st->mu, l) + 0.5 * V1(st->wmu, l);
+      RK(d->mudot, i, l) = (1.0 - muboun * muboun) * dts / 2.0 / muboun / rlz;
+    }
+    RK(d->mudot, i, st->npa) = 0.0; /* :81 */
+    for (int k = 1; k <= st->ne; ++k) { /* :82-84 */
+      const double gb = SK(st->grbnd, s, k);
+      RK(d->edot, i, k) = V1(st->ebnd, k) * dts / rlz * (gb + 1.0) / gb / 2.0;
+    }
+  }
+}
+
+/* DRIFTR, src/ModRamDrift.f90:95-198.  The ghost cells F(NR+1), F(NR+2) of
+ * the line buffer are NOT reset between lines: in the reference they keep
+ * the values of the last inflow line (F is allocated and zeroed once per
+ * call, :112-113), and an outflow line whose last interior face is upwinded
+ * from outside (sgn(NR-1)=-1, :174-175) reads that stale F(NR+1).  The
+ * restatement keeps this behaviour. */
+void oracle_driftr(oracle_state *st, int s, oracle_drift_scratch *d) {
+  const int nr = st->nr, nt = st->nt, ne = st->ne, npa = st->npa;
+  const double dts = st->dts, mdr = st->mdr, dphi = st->dphi, beta = st->betalim;
+  const double qs = (double)d->qs;
+  double *f = calloc((size_t)nr + 3, sizeof(double)); /* F(1..NR+2) at f[1..] */
+  double *fbnd = calloc((size_t)nr + 1, sizeof(double));
+  double *cr = calloc((size_t)nr * nt, sizeof(double));
+  int *sgn = calloc((size_t)nr + 1, sizeof(int));
+  double dtr = 100000.0; /* :115 */
+
+  for (int i = 1; i <= nr; ++i) { /* :118-127 ExB radial drift */
+    for (int j = 1; j <= nt; ++j) {
+      const int j0 = (j == 1) ? nt - 1 : j - 1;
+      const int j1 = (j == nt) ? 2 : j + 1;
+      const double bsum = G2(st->bnes, i, j) + G2(st->bnes, i + 1, j);
+      const double dv = G2(st->vt, i, j0) + G2(st->vt, i + 1, j0) - G2(st->vt, i, j1) -
+                        G2(st->vt, i + 1, j1);
+      cr[(i - 1) + (size_t)nr * (j - 1)] =
+          d->vr[i - 1] * dv / bsum +
+          (G2(st->eip, i, j) + G2(st->eip, i + 1, j)) / bsum * dts / mdr;
+    }
+  }
+
+  for (int k = 1; k <= ne; ++k) {
+    const double g = SK(st->grel, s, k);
+    const double p4 = dts * V1(st->ekev, k) * 1000.0 * (g + 1.0) / g / dphi / mdr / qs; /* :131 */
+    for (int l = 1; l <= npa; ++l) {
+      for (int j = 1; j <= nt; ++j) {
+        for (int i = 1; i <= nr; ++i) f[i] = F2(s, i, j, k, l);
+        const int j0 = (j == 1) ? nt - 1 : j - 1;
+        const int j1 = (j == nt) ? 2 : j + 1;
+        for (int i = 1; i <= nr; ++i) { /* :139-153 */
+          const double cgr1 = G3(st->fnis, i + 1, j1, l) + G3(st->fnis, i, j1, l) -
+                              G3(st->fnis, i + 1, j0, l) - G3(st->fnis, i, j0, l);
+          const double cgr2 = G2(st->bnes, i + 1, j1) + G2(st->bnes, i, j1) -
+                              G2(st->bnes, i + 1, j0) - G2(st->bnes, i, j0);
+          const double cgr3 =
+              cgr1 + (G3(st->fnis, i + 1, j, l) + G3(st->fnis, i, j, l) -
+                      2.0 * G3(st->fnhs, i + 1, j, l) - 2.0 * G3(st->fnhs, i, j, l)) *
+                         cgr2 / 2.0 / (G2(st->bnes, i + 1, j) + G2(st->bnes, i, j));
+          const double cgr = cgr3 / (G3(st->fnhs, i, j, l) + G3(st->fnhs, i + 1, j, l)) * p4 /
+                             2.0 / (G2(st->bnes, i, j) + G2(st->bnes, i + 1, j)) /
+                             (V1(st->rlz, i) + 0.5 * mdr);
+          const double c = cr[(i - 1) + (size_t)nr * (j - 1)] + cgr;
+          C4(d->cdr, i, j, k, l) = c;
+          if (MGNP(i, j) == 0) {
+            const double ctemp = dmax(fabs(c), 1E-10);
+            dtr = dmin(dtr, st->fraccfl * dts / ctemp);
+          }
+          sgn[i] = (c < 0) ? -1 : 1;
+        }
+        int ur;
+        if (sgn[nr] == 1) { /* :154-168 outflow / inflow at the outer edge */
+          fbnd[1] = 0.0;
+          fbnd[nr] = f[nr];
+          ur = nr - 1;
+        } else {
+          fbnd[1] = f[2];
+          ur = nr;
+          if (MGNP(nr, j) == 1) {
+            f[nr + 1] = 0.0;
+            f[nr + 2] = 0.0;
+          } else {
+            f[nr + 1] = FGEOS(s, j, k, l) * st->conf1 * G3(st->fnhs, nr, j, l);
+            f[nr + 2] = FGEOS(s, j, k, l) * st->conf2 * G3(st->fnhs, nr, j, l);
+          }
+        }
+        for (int i = 2; i <= ur; ++i) { /* :169-183 limited face value */
+          const double sg = (double)sgn[i];
+          const double x = f[i + 1] - f[i];
+          const double fup = 0.5 * (f[i] + f[i + 1] - sg * x);
+          if (fabs(x) <= 1.E-27) {
+            fbnd[i] = fup;
+          } else {
+            const int n = i + 1 - sgn[i];
+            const double r = (f[n] - f[n - 1]) / x;
+            if (r <= 0) {
+              fbnd[i] = fup;
+            } else {
+              const double corr = -0.5 * (C4(d->cdr, i, j, k, l) - sg) * x;
+              fbnd[i] = fup + limiter(r, beta) * corr;
+            }
+          }
+        }
+        for (int i = 2; i <= nr; ++i) { /* :185-191 conservative update */
+          double v = F2(s, i, j, k, l) - C4(d->cdr, i, j, k, l) * fbnd[i] +
+                     C4(d->cdr, i - 1, j, k, l) * fbnd[i - 1];
+          if (v < 0) v = 1E-15;
+          F2(s, i, j, k, l) = v;
+        }
+      }
+    }
+  }
+  st->dtdrift_r[s - 1] = dtr;
+  free(f); free(fbnd); free(cr); free(sgn);
+}
+
+/* DRIFTP, src/ModRamDrift.f90:204-279 (periodic in local time). */
+void oracle_driftp(oracle_state *st, int s, oracle_drift_scratch *d) {
+  const int nr = st->nr, nt = st->nt, ne = st->ne, npa = st->npa;
+  const double dts = st->dts, mdr = st->mdr, dphi = st->dphi, beta = st->betalim;
+  double *f = calloc((size_t)nt + 1, sizeof(double));
+  double *fbnd = calloc((size_t)nt + 1, sizeof(double));
+  double dtp = 100000.0; /* :223 */
+  for (int l = 1; l <= npa; ++l) {
+    for (int k = 1; k <= ne; ++k) {
+      for (int i = 2; i <= nr; ++i) {
+        const double rlz = V1(st->rlz, i);
+        for (int j = 1; j <= nt; ++j) f[j] = F2(s, i, j, k, l);
+        for (int j = 2; j <= nt; ++j) { /* :229-260 */
+          const int j1 = (j == nt) ? 2 : j + 1;
+          const double gpa1 =
+              G3(st->fnis, i, j, l) + G3(st->fnis, i, j1, l) +
+              (G3(st->fnis, i + 1, j1, l) + G3(st->fnis, i + 1, j, l) - G3(st->fnis, i - 1, j, l) -
+               G3(st->fnis, i - 1, j1, l)) * rlz / 2.0 / mdr;
+          const double gpa2 =
+              rlz / 4.0 / mdr *
+              (G3(st->fnis, i, j, l) + G3(st->fnis, i, j1, l) - 2.0 * G3(st->fnhs, i, j, l) -
+               2.0 * G3(st->fnhs, i, j1, l)) *
+              (G2(st->bnes, i + 1, j1) + G2(st->bnes, i + 1, j) - G2(st->bnes, i - 1, j) -
+               G2(st->bnes, i - 1, j1)) /
+              (G2(st->bnes, i, j) + G2(st->bnes, i, j1));
+          const double c =
+              ((G2(st->vt, i + 1, j) + G2(st->vt, i + 1, j1) - G2(st->vt, i - 1, j) -
+                G2(st->vt, i - 1, j1)) * d->p1[i - 1] -
+               RK(d->p2, i, k) * (gpa1 + gpa2) /
+                   (G3(st->fnhs, i, j, l) + G3(st->fnhs, i, j1, l)) -
+               (G2(st->eir, i, j1) + G2(st->eir, i, j)) / rlz * dts / dphi) /
+                  (G2(st->bnes, i, j) + G2(st->bnes, i, j1)) +
+              RAM_OME * dts / dphi;
+          C4(d->cdp, i, j, k, l) = c;
+          if (MGNP(i, j) == 0) {
+            const double ctemp = dmax(fabs(c), 1E-10);
+            dtp = dmin(dtp, st->fraccfl * dts / ctemp);
+          }
+          const int sgn = (c < 0) ? -1 : 1;
+          const double sg = (double)sgn;
+          const double x = f[j1] - f[j];
+          const double fup = 0.5 * (f[j] + f[j1] - sg * x);
+          if (fabs(x) <= 1.E-27) {
+            fbnd[j] = fup;
+          } else {
+            int n = j + 1 - sgn;
+            if (n > nt) n = n - nt + 1; /* :251 */
+            const double r = (f[n] - f[n - 1]) / x;
+            if (r <= 0) {
+              fbnd[j] = fup;
+            } else {
+              const double corr = -0.5 * (c - sg) * x;
+              fbnd[j] = fup + limiter(r, beta) * corr;
+            }
+          }
+        }
+        C4(d->cdp, i, 1, k, l) = C4(d->cdp, i, nt, k, l); /* :261-262 */
+        fbnd[1] = fbnd[nt];
+        for (int j = 2; j <= nt; ++j) { /* :263-271 */
+          double v = F2(s, i, j, k, l) - C4(d->cdp, i, j, k, l) * fbnd[j] +
+                     C4(d->cdp, i, j - 1, k, l) * fbnd[j - 1];
+          if (v < 0) v = 1E-15;
+          F2(s, i, j, k, l) = v;
+        }
+        F2(s, i, 1, k, l) = F2(s, i, nt, k, l); /* :272 */
+      }
+    }
+  }
+  st->dtdrift_p[s - 1] = dtp;
+  free(f); free(fbnd);
+}
+
+/* DRIFTE, src/ModRamDrift.f90:285-376. */
+void oracle_drifte(oracle_state *st, int s, oracle_drift_scratch *d) {
+  const int nr = st->nr, nt = st->nt, ne = st->ne, npa = st->npa;
+  const double dts = st->dts, mdr = st->mdr, dphi = st->dphi, beta = st->betalim;
+  const double qs = (double)d->qs;
+  double *fb = calloc((size_t)ne + 3, sizeof(double)); /* F(0..NE+2) */
+  double *f = fb;                                       /* f[k] == F(k) */
+  double *fbnd = calloc((size_t)ne + 1, sizeof(double));
+  double dte = 10000.0; /* :308 */
+  const double ezero = V1(st->ekev, 1) - V1(st->we, 1);                     /* :310 */
+  const double grzero = 1.0 + ezero * 1000.0 * RAM_Q / st->rmas[s - 1] / RAM_CS / RAM_CS; /* :311 */
+  const double g1 = SK(st->grel, s, 1), g2 = SK(st->grel, s, 2);
+  f[ne + 1] = 0.0; /* :312-313 */
+  f[ne + 2] = 0.0;
+  for (int j = 1; j <= nt; ++j) {
+    const int j0 = (j == 1) ? nt - 1 : j - 1;
+    const int j2 = (j == nt) ? 2 : j + 1;
+    for (int i = 2; i <= nr; ++i) {
+      const double rlz = V1(st->rlz, i);
+      const double bij = G2(st->bnes, i, j);
+      const double drd1 = (G2(st->eip, i, j) * rlz - (G2(st->vt, i, j2) - G2(st->vt, i, j0)) / 2.0 / dphi) / bij;
+      const double dpd1 = RAM_OME * rlz + ((G2(st->vt, i + 1, j) - G2(st->vt, i - 1, j)) / 2.0 / mdr - G2(st->eir, i, j)) / bij;
+      for (int l = 1; l <= npa; ++l) {
+        const double fnis = G3(st->fnis, i, j, l), fnhs = G3(st->fnhs, i, j, l);
+        const double gpa = (1.0 - fnis / 2.0 / fnhs) / bij; /* :323-332 */
+        const double gpr1 = gpa * (G2(st->bnes, i + 1, j) - G2(st->bnes, i - 1, j)) / 2.0 / mdr;
+        const double gpr2 = -fnis / fnhs / rlz;
+        const double gpr3 = -(G3(st->fnis, i + 1, j, l) - G3(st->fnis, i - 1, j, l)) / 2.0 / mdr / fnhs;
+        const double gpp1 = gpa * (G2(st->bnes, i, j2) - G2(st->bnes, i, j0)) / 2.0 / dphi;
+        const double gpp2 = -(G3(st->fnis, i, j2, l) - G3(st->fnis, i, j0, l)) / 2.0 / dphi / fnhs;
+        const double drd2 = (G3(st->fnis, i, j2, l) - G3(st->fnis, i, j0, l)) / 2.0 / dphi +
+                            (fnis - 2.0 * fnhs) * (G2(st->bnes, i, j2) - G2(st->bnes, i, j0)) / 4.0 / bij / dphi;
+        const double dpd2 = fnis + (G3(st->fnis, i + 1, j, l) - G3(st->fnis, i - 1, j, l)) * rlz / 2.0 / mdr +
+                            rlz * (fnis - 2.0 * fnhs) / 4.0 / mdr *
+                                (G2(st->bnes, i + 1, j) - G2(st->bnes, i - 1, j)) / bij;
+        for (int k = 1; k <= ne; ++k) f[k] = F2(s, i, j, k, l); /* :333-335 */
+        f[1] = f[2] * g1 / g2 * sqrt((g2 * g2 - 1.0) / (g1 * g1 - 1.0));
+        f[0] = f[1] * grzero / g1 * sqrt((g1 * g1 - 1.0) / (grzero * grzero - 1.0));
+        for (int k = 1; k <= ne; ++k) { /* :336-362 */
+          const double gb = SK(st->grbnd, s, k);
+          const double edt1 = V1(st->ebnd, k) * 1e3 * (gb + 1.0) / 2.0 / gb / fnhs / rlz / bij / qs;
+          const double drdt = drd1 + edt1 * drd2 * rlz;
+          const double dpdt = dpd1 - edt1 * dpd2;
+          const double dbdt1 = G2(st->dbdt, i, j) * (1.0 - fnis / 2.0 / fnhs) * rlz / bij;
+          const double didt1 = -G3(st->didt, i, j, l) * rlz / fnhs;
+          const double c = RK(d->edot, i, k) *
+                           ((gpr1 + gpr2 + gpr3) * drdt + (gpp1 + gpp2) * dpdt + dbdt1 + didt1);
+          C4(d->cde, i, j, k, l) = c;
+          if (MGNP(i, j) == 0) {
+            const double ctemp = dmax(fabs(c), 1E-10);
+            dte = dmin(dte, st->fraccfl * dts * V1(st->de, k) / ctemp);
+          }
+          const int sgn = (c < 0) ? -1 : 1;
+          const double sg = (double)sgn;
+          const double x = f[k + 1] - f[k];
+          const double fup = 0.5 * (f[k] + f[k + 1] - sg * x);
+          if (fabs(x) <= 1.E-27) {
+            fbnd[k] = fup;
+          } else {
+            const int n = k + 1 - sgn;
+            const double r = (f[n] - f[n - 1]) / x;
+            if (r <= 0) {
+              fbnd[k] = fup;
+            } else {
+              const double corr = -0.5 * (c / V1(st->de, k) - sg) * x;
+              fbnd[k] = fup + limiter(r, beta) * corr;
+            }
+          }
+        }
+        for (int k = 2; k <= ne; ++k) { /* :363-369 */
+          const double wek = V1(st->we, k);
+          double v = F2(s, i, j, k, l) - C4(d->cde, i, j, k, l) / wek * fbnd[k] +
+                     C4(d->cde, i, j, k - 1, l) / wek * fbnd[k - 1];
+          if (v < 0) v = 1E-15;
+          F2(s, i, j, k, l) = v;
+        }
+      }
+    }
+  }
+  st->dtdrift_e[s - 1] = dte;
+  free(fb); free(fbnd);
+}
+
+/* DRIFTMU, src/ModRamDrift.f90:382-473. */
+void oracle_driftmu(oracle_state *st, int s, oracle_drift_scratch *d) {
+  const int nr = st->nr, nt = st->nt, ne = st->ne, npa = st->npa;
+  const double dts = st->dts, mdr = st->mdr, dphi = st->dphi, beta = st->betalim;
+  const double qs = (double)d->qs;
+  double *f = calloc((size_t)npa + 1, sizeof(double));
+  double *fbnd = calloc((size_t)npa + 1, sizeof(double));
+  double dtmu = 10000.0; /* :404 */
+  for (int k = 1; k <= ne; ++k) {
+    const double g = SK(st->grel, s, k);
+    for (int j = 1; j <= nt; ++j) {
+      const int j0 = (j == 1) ? nt - 1 : j - 1;
+      const int j1 = (j == nt) ? 2 : j + 1;
+      for (int i = 2; i <= nr; ++i) {
+        const double rlz = V1(st->rlz, i);
+        const double bij = G2(st->bnes, i, j);
+        for (int l = 1; l <= npa; ++l) f[l] = F2(s, i, j, k, l); /* :413-414 */
+        f[1] = f[2];
+        const double drm1 = (G2(st->eip, i, j) * rlz - (G2(st->vt, i, j1) - G2(st->vt, i, j0)) / 2.0 / dphi) / bij;
+        const double dpm1 = RAM_OME * rlz + ((G2(st->vt, i + 1, j) - G2(st->vt, i - 1, j)) / 2.0 / mdr - G2(st->eir, i, j)) / bij;
+        for (int l = 2; l <= npa; ++l) { /* :417-455 */
+          const double bis = G3(st->bounis, i, j, l), bhs = G3(st->bounhs, i, j, l);
+          const double cmudot = RK(d->mudot, i, l) * bis / bhs;
+          const double gmr1 = (G2(st->bnes, i + 1, j) - G2(st->bnes, i - 1, j)) / 4.0 / mdr / bij;
+          const double gmr2 = 1.0 / rlz;
+          const double gmr3 = (G3(st->bounis, i + 1, j, l) - G3(st->bounis, i - 1, j, l)) / 2.0 / mdr / bis;
+          const double gmp1 = (G2(st->bnes, i, j1) - G2(st->bnes, i, j0)) / 4.0 / dphi / bij;
+          const double gmp2 = (G3(st->bounis, i, j1, l) - G3(st->bounis, i, j0, l)) / 2.0 / dphi / bis;
+          const double edt = V1(st->ekev, k) * 1e3 * (g + 1.0) / 2.0 / g / bhs / rlz / bij / qs;
+          const double drm2 = (G3(st->bounis, i, j1, l) - G3(st->bounis, i, j0, l)) / 2.0 / dphi +
+                              (bis - 2.0 * bhs) * (G2(st->bnes, i, j1) - G2(st->bnes, i, j0)) / 4.0 / bij / dphi;
+          const double dpm2 = bis + (G3(st->bounis, i + 1, j, l) - G3(st->bounis, i - 1, j, l)) * rlz / 2.0 / mdr +
+                              (bis - 2.0 * bhs) * rlz / 4.0 / mdr *
+                                  (G2(st->bnes, i + 1, j) - G2(st->bnes, i - 1, j)) / bij;
+          const double drdm = drm1 + edt * drm2 * rlz;
+          const double dpdm = dpm1 - edt * dpm2;
+          const double dbdt2 = G2(st->dbdt, i, j) / 2.0 / bij * rlz;
+          const double dibndt2 = G3(st->dibndt, i, j, l) * rlz / bis;
+          const double c = -cmudot * ((gmr1 + gmr2 + gmr3) * drdm + (gmp1 + gmp2) * dpdm + dbdt2 + dibndt2);
+          C4(d->cdmu, i, j, k, l) = c;
+          if (MGNP(i, j) == 0) {
+            const double ctemp = dmax(fabs(c), 1E-32);
+            dtmu = dmin(dtmu, st->fraccfl * dts * V1(st->dmu, l) / ctemp);
+          }
+          const int isgm = (c < 0.0) ? -1 : 1;
+          const double sg = (double)isgm;
+          if (l <= npa - 2) {
+            const double x = f[l + 1] - f[l];
+            const double fup = 0.5 * (f[l] + f[l + 1] - sg * x);
+            if (fabs(x) <= 1.E-27) {
+              fbnd[l] = fup;
+            } else {
+              const int n = l + 1 - isgm;
+              const double r = (f[n] - f[n - 1]) / x;
+              if (r <= 0) {
+                fbnd[l] = fup;
+              } else {
+                const double corr = -0.5 * (c / V1(st->dmu, l) - sg) * x;
+                fbnd[l] = fup + limiter(r, beta) * corr;
+              }
+            }
+          }
+        }
+        C4(d->cdmu, i, j, k, 1) = 0.0; /* :456-458 */
+        fbnd[1] = 0.0;
+        fbnd[npa - 1] = f[npa];
+        for (int l = 2; l <= npa - 1; ++l) { /* :459-465 */
+          const double wmul = V1(st->wmu, l);
+          double v = F2(s, i, j, k, l) - C4(d->cdmu, i, j, k, l) / wmul * fbnd[l] +
+                     C4(d->cdmu, i, j, k, l - 1) / wmul * fbnd[l - 1];
+          if (v < 0) v = 1E-15;
+          F2(s, i, j, k, l) = v;
+        }
+        F2(s, i, j, k, npa) = F2(s, i, j, k, npa - 1) * G3(st->fnhs, i, j, npa) * V1(st->mu, npa) /
+                              G3(st->fnhs, i, j, npa - 1) / V1(st->mu, npa - 1); /* :466 */
+      }
+    }
+  }
+  st->dtdrift_mu[s - 1] = dtmu;
+  free(f); free(fbnd);
+}
+
+/* CEPARA, src/ModRamLoss.f90:19-170: charge-exchange factors for the three
+ * analytic cross-section fits (H+ :39-51, He+ :54-66, O+ :69-81) and the
+ * atmospheric-loss factor ATLOS (:162-167).  The file-based branch
+ * (:84-158) is not part of this path (no species here uses it). */
+void oracle_cepara(const oracle_state *st, int s, oracle_loss_scratch *ls) {
+  const int nr = st->nr, nt = st->nt, ne = st->ne, npa = st->npa;
+  const double dts = st->dts;
+  const size_t n4 = (size_t)nr * nt * ne * npa;
+  for (size_t n = 0; n < n4; ++n) ls->charge[n] = 1.0; /* :34 */
+  const int kind = st->cex_kind[s - 1];
+  if (kind != ORACLE_CEX_NONE) {
+    for (int l = 2; l <= npa; ++l)
+      for (int k = 2; k <= ne; ++k)
+        for (int i = 2; i <= nr; ++i)
+          for (int j = 1; j <= nt; ++j) {
+            double x = log10(V1(st->ekev, k));
+            if (x < -2.0) x = -2.0;
+            const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+            double y;
+            if (kind == ORACLE_CEX_H)
+              y = -18.767 - 0.11017 * x - 3.8173e-2 * x2 - 0.1232 * x3 - 5.0488e-2 * x4;
+            else if (kind == ORACLE_CEX_HE)
+              y = -20.789 + 0.92316 * x - 0.68017 * x2 + 0.66153 * x3 - 0.20998 * x4;
+            else
+              y = -18.987 - 0.10613 * x - 5.4841E-3 * x2 - 1.6262E-2 * x3 - 7.0554E-3 * x4;
+            const double alpha = pow(10.0, y) * SK(st->v, s, k) * G3(st->hdns, i, j, l) * dts;
+            C4(ls->charge, i, j, k, l) = exp(-alpha);
+          }
+  }
+  for (int k = 2; k <= ne; ++k)
+    for (int i = 2; i <= nr; ++i) {
+      const double taub = 2.0 * V1(st->rlz, i) / SK(st->v, s, k);
+      RK(ls->atlos, i, k) = exp(-dts / taub);
+    }
+}
+
+/* CHAREXCHANGE, src/ModRamLoss.f90:457-478. */
+void oracle_charexchange(oracle_state *st, int s, const oracle_loss_scratch *ls) {
+  for (int k = 2; k <= st->ne; ++k)
+    for (int j = 1; j <= st->nt; ++j)
+      for (int i = 2; i <= st->nr; ++i)
+        for (int l = 2; l <= st->npa; ++l)
+          F2(s, i, j, k, l) = F2(s, i, j, k, l) * C4(ls->charge, i, j, k, l);
+}
+
+/* ATMOL, src/ModRamLoss.f90:485-507. */
+void oracle_atmol(oracle_state *st, int s, const oracle_loss_scratch *ls) {
+  for (int k = 2; k <= st->ne; ++k)
+    for (int j = 1; j <= st->nt; ++j)
+      for (int i = 2; i <= st->nr; ++i) {
+        const int u = (int)V1(st->upa, i);
+        for (int l = u; l <= st->npa; ++l)
+          F2(s, i, j, k, l) = F2(s, i, j, k, l) * pow(RK(ls->atlos, i, k), 1.0 / G3(st->fnhs, i, j, l));
+      }
+}
+
+/* SUMRC, src/ModRamRun.f90:231-259 (sequential sum in the I,K,L,J order). */
+void oracle_sumrc(oracle_state *st, int s) {
+  st->elorc[s - 1] = 0.0;
+  const double enold = st->setrc[s - 1];
+  double sum = 0.0;
+  for (int i = 2; i <= st->nr; ++i)
+    for (int k = 2; k <= st->ne; ++k)
+      for (int l = 2; l <= st->npa; ++l)
+        for (int j = 1; j <= st->nt - 1; ++j) {
+          const double weight = F2(s, i, j, k, l) * V1(st->we, k) * V1(st->wmu, l);
+          sum = sum + V1(st->ekev, k) * weight;
+        }
+  st->setrc[s - 1] = sum;
+  st->elorc[s - 1] = enold - sum;
+}
+
+/* The per-species body of ram_run, src/ModRamRun.f90:66-184, optional
+ * processes off. */
+static int ram_run_species(oracle_state *st, int s) {
+  oracle_drift_scratch d;
+  oracle_loss_scratch ls;
+  if (oracle_drift_alloc(st, &d)) return -1;
+  if (oracle_loss_alloc(st, &ls)) {
+    oracle_drift_free(&d);
+    return -1;
+  }
+  const int cex = st->cex_kind[s - 1] != ORACLE_CEX_NONE;
+  oracle_cepara(st, s, &ls);
+  oracle_driftpara(st, s, &d);
+  oracle_driftr(st, s, &d);
+  oracle_driftp(st, s, &d);
+  oracle_drifte(st, s, &d);
+  oracle_driftmu(st, s, &d);
+  oracle_sumrc(st, s);
+  st->lsdr[s - 1] += st->elorc[s - 1];
+  if (cex) {
+    oracle_charexchange(st, s, &ls);
+    oracle_sumrc(st, s);
+    st->lscha[s - 1] += st->elorc[s - 1];
+  }
+  oracle_atmol(st, s, &ls);
+  oracle_sumrc(st, s);
+  st->lsatm[s - 1] += st->elorc[s - 1];
+  oracle_atmol(st, s, &ls);
+  oracle_sumrc(st, s);
+  st->lsatm[s - 1] += st->elorc[s - 1];
+  if (cex) {
+    oracle_charexchange(st, s, &ls);
+    oracle_sumrc(st, s);
+    st->lscha[s - 1] += st->elorc[s - 1];
+  }
+  oracle_driftmu(st, s, &d);
+  oracle_drifte(st, s, &d);
+  oracle_driftp(st, s, &d);
+  oracle_driftr(st, s, &d);
+  oracle_sumrc(st, s);
+  st->lsdr[s - 1] += st->elorc[s - 1];
+  oracle_drift_free(&d);
+  oracle_loss_free(&ls);
+  return 0;
+}
+
+/* ram_run, src/ModRamRun.f90:64-205. */
+int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next) {
+  int err = 0;
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1) reduction(| : err)
+#endif
+  for (int s = 1; s <= st->ns; ++s) err |= ram_run_species(st, s) ? 1 : 0;
+  if (err) return -1;
+  /* :186 periodic copy F2(:,:,NT,:,:)=F2(:,:,1,:,:) */
+  for (int l = 1; l <= st->npa; ++l)
+    for (int k = 1; k <= st->ne; ++k)
+      for (int i = 1; i <= st->nr; ++i)
+        for (int s = 1; s <= st->ns; ++s) F2(s, i, st->nt, k, l) = F2(s, i, 1, k, l);
+  /* :197-201 outside-magnetopause cells */
+  for (int i = 1; i <= st->nr; ++i)
+    for (int j = 1; j <= st->nt; ++j)
+      if (MGNP(i, j) == 1)
+        for (int l = 1; l <= st->npa; ++l)
+          for (int k = 1; k <= st->ne; ++k)
+            for (int s = 1; s <= st->ns; ++s) F2(s, i, j, k, l) = 1e-31;
+  if (dts_next) { /* :204 (the DtsMin floor of :205 is the caller's) */
+    double m = 1e300;
+    for (int s = 0; s < st->ns; ++s) {
+      m = dmin(m, st->dtdrift_r[s]);
+      m = dmin(m, st->dtdrift_p[s]);
+      m = dmin(m, st->dtdrift_e[s]);
+      m = dmin(m, st->dtdrift_mu[s]);
+    }
+    *dts_next = m;
+  }
+  return 0;
+}

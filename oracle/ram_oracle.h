@@ -1,0 +1,96 @@
+/*
+ * ram_oracle.h -- CPU restatement of the RAM kinetic split step (TEST
+ * INFRASTRUCTURE ONLY).
+ *
+ * This is the oracle the HIP path is checked against.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker or the timed CPU baseline -- never as the product.
+ *
+ * Every array uses the reference's Fortran (column-major, 1-based) layout so
+ * that the same host buffers can be handed to this oracle, to the reference's
+ * own Fortran (oracle/_ref) and to the C-ABI of the GPU library:
+ *   F2(NS,NR,NT,NE,NPA)                    src/ModRamInit.f90:72
+ *   FNHS,FNIS,BOUNHS,BOUNIS,HDNS,dIdt,dIbndt (NR+1,NT,NPA)   :75-77
+ *   BNES,dBdt,VT,EIR,EIP (NR+1,NT)                           :129
+ *   FGEOS(NS,NT,NE,NPA)                                       :133
+ *   outsideMGNP(NR,NT) (int)
+ *   GREL,GRBND,V (NS,NE)
+ */
+#ifndef RAM_ORACLE_H
+#define RAM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Charge-exchange cross-section family of a species
+ * (src/ModRamLoss.f90:37-83, selected by species%s_name). */
+enum { ORACLE_CEX_NONE = 0, ORACLE_CEX_H = 1, ORACLE_CEX_HE = 2, ORACLE_CEX_O = 3 };
+
+typedef struct {
+  int ns, nr, nt, ne, npa;
+  /* scalar parameters: ModRamTiming::DTs, ModRamParams::BetaLim,
+   * ModRamVariables::FracCFL, MDR, DPHI, CONF1, CONF2 */
+  double dts, betalim, fraccfl, mdr, dphi, conf1, conf2;
+  /* species */
+  const int *charge;     /* (NS)  species%s_charge                  */
+  const int *cex_kind;   /* (NS)  ORACLE_CEX_* ; 0 => species%CEX false */
+  const double *rmas;    /* (NS)  RMAS                              */
+  /* 1-D grids */
+  const double *rlz;     /* (NR+1) */
+  const double *ekev, *ebnd, *we, *de;   /* (NE) */
+  const double *grel, *grbnd, *v;        /* (NS,NE) */
+  const double *mu, *wmu, *dmu;          /* (NPA) */
+  const double *upa;                     /* (NR) real-valued index */
+  /* geometry and fields */
+  const double *fnhs, *fnis, *bounhs, *bounis, *hdns, *didt, *dibndt;
+  const double *bnes, *dbdt, *vt, *eir, *eip;
+  const int *outside_mgnp;
+  const double *fgeos;
+  /* state */
+  double *f2;
+  double *dtdrift_r, *dtdrift_p, *dtdrift_e, *dtdrift_mu; /* (NS) */
+  double *setrc, *elorc, *lsdr, *lscha, *lsatm;          /* (NS) */
+} oracle_state;
+
+/* Per-species drift tables built by DRIFTPARA (src/ModRamDrift.f90:15-19). */
+typedef struct {
+  int qs;
+  double *vr, *p1;           /* (NR)      */
+  double *p2, *edot;         /* (NR,NE)   */
+  double *mudot;             /* (NR,NPA)  */
+  double *cdr, *cdp, *cde, *cdmu; /* (NR,NT,NE,NPA) */
+} oracle_drift_scratch;
+
+/* Per-species loss tables built by CEPARA (src/ModRamLoss.f90:19-170). */
+typedef struct {
+  double *charge;  /* (NR,NT,NE,NPA) for this species */
+  double *atlos;   /* (NR,NE) */
+} oracle_loss_scratch;
+
+int  oracle_drift_alloc(const oracle_state *st, oracle_drift_scratch *d);
+void oracle_drift_free(oracle_drift_scratch *d);
+int  oracle_loss_alloc(const oracle_state *st, oracle_loss_scratch *l);
+void oracle_loss_free(oracle_loss_scratch *l);
+
+/* Species index s is 1-based, as in the reference. */
+void oracle_driftpara(const oracle_state *st, int s, oracle_drift_scratch *d);
+void oracle_driftr(oracle_state *st, int s, oracle_drift_scratch *d);
+void oracle_driftp(oracle_state *st, int s, oracle_drift_scratch *d);
+void oracle_drifte(oracle_state *st, int s, oracle_drift_scratch *d);
+void oracle_driftmu(oracle_state *st, int s, oracle_drift_scratch *d);
+void oracle_cepara(const oracle_state *st, int s, oracle_loss_scratch *l);
+void oracle_charexchange(oracle_state *st, int s, const oracle_loss_scratch *l);
+void oracle_atmol(oracle_state *st, int s, const oracle_loss_scratch *l);
+void oracle_sumrc(oracle_state *st, int s);
+
+/* One ram_run call (src/ModRamRun.f90:64-205) with the optional processes
+ * (Coulomb, WPI, EMIC, FLC, plasmasphere) off.  nthreads>1 runs the species
+ * loop with OpenMP as the reference does (src/ModRamRun.f90:64).  Returns
+ * DtsNext before the DtsMin floor in *dts_next (may be NULL). */
+int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
