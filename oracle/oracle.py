@@ -46,12 +46,12 @@ class _Loss(C.Structure):
 
 
 def _fd(a):
-    a = np.asfortranarray(a, dtype=np.float64)
-    return a
+    # always a private copy: the oracle mutates its arrays in place
+    return np.array(a, dtype=np.float64, order="F", copy=True)
 
 
 def _fi(a):
-    return np.asfortranarray(a, dtype=np.int32)
+    return np.array(a, dtype=np.int32, order="F", copy=True)
 
 
 def _ptr(a, typ=_dp):

@@ -189,7 +189,9 @@ def build_grid(nr: int, nt: int, ne: int, npa: int, species, elb: float = 0.1) -
     wmu = np.zeros(npa)
     dmu = np.zeros(npa)
     pabn = np.zeros(npa)
-    if npa == 90:
+    if npa != 72:
+        # the reference's uniform branch (written for NPA=90), used here for
+        # any NPA other than 72
         pa[0], mu[0], pa[npa - 1], mu[npa - 1] = 90.0, 0.0, 0.0, 1.0
         wmu[0] = (mu[npa - 1] - mu[0]) / npa
         for l in range(npa - 1):
@@ -201,9 +203,9 @@ def build_grid(nr: int, nt: int, ne: int, npa: int, species, elb: float = 0.1) -
         dmu[npa - 1] = 0.5 * (wmu[npa - 1] + wmu[npa - 2])
         pabn[npa - 1] = 0.0
     else:
-        if npa != 72 or nr != 20:
-            raise ValueError("the non-uniform pitch-angle grid of ARRAYS is defined "
-                             "for NPA=72 with NR=20; use NPA=90 otherwise")
+        if nr != 20:
+            raise ValueError("the non-uniform NPA=72 pitch-angle grid of ARRAYS is "
+                             "defined for NR=20")
         rwu = 0.98
         pa[0], mu[0], pa[npa - 1], mu[npa - 1] = 90.0, 0.0, 0.0, 1.0
         wmu[0] = (mu[npa - 1] - mu[0]) / 32
