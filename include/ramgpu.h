@@ -112,8 +112,8 @@ int ramgpu_get_energy_state(ramgpu_ctx *ctx, double *setrc, double *lsdr, double
 
 /* ---- operators, one per reference subroutine (S is 1-based) ------------- */
 
-/* DRIFTPARA(S), src/ModRamDrift.f90:36-88: builds the species' drift tables
- * (and the CFL limits, which depend on tables only). */
+/* DRIFTPARA(S), src/ModRamDrift.f90:36-88: builds the drift coefficient
+ * tables (for every species; they are cached until an input changes). */
 int ramgpu_driftpara(ramgpu_ctx *ctx, int s);
 /* DRIFTR/DRIFTP/DRIFTE/DRIFTMU(S), src/ModRamDrift.f90:95,204,285,382.
  * dtdrift (may be NULL) receives DtDriftR/P/E/Mu(S). */
@@ -150,6 +150,23 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *ctx, int nsteps, const double *kp,
 /* Synchronise and return the device time (ms) of the last ram_run call,
  * measured with HIP events on the library's stream. */
 int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
+
+/* Opt-in per-kernel timing (no counterpart in the reference: measurement
+ * support).  While enabled, every launch is bracketed by HIP events on the
+ * library's stream; ramgpu_kernel_times returns, per kernel kind, the summed
+ * device time and the launch count since the last query, then resets. */
+#define RAMGPU_KERNEL_PREPARE 0  /* species tables, Courant terms, ghost carry */
+#define RAMGPU_KERNEL_IJ 1       /* radial / azimuthal sweeps                 */
+#define RAMGPU_KERNEL_KL 2       /* energy / pitch-angle sweeps, losses       */
+#define RAMGPU_KERNEL_FINALIZE 3 /* DtDrift* and SUMRC bookkeeping            */
+#define RAMGPU_KERNEL_VS 4       /* Volland-Stern VT from Kp                  */
+#define RAMGPU_KERNEL_KINDS 5
+int ramgpu_kernel_timing(ramgpu_ctx *ctx, int enable);
+/* Measurement support: a plain device copy of `bytes` (read + write) run
+ * `reps` times on device `device`; gbps receives the achieved read+write
+ * bandwidth.  Used as the measured HBM peak and to calibrate counters. */
+int ramgpu_copy_probe(int device, size_t bytes, int reps, double *gbps);
+int ramgpu_kernel_times(ramgpu_ctx *ctx, double *total_ms, int *launches);
 
 #ifdef __cplusplus
 }

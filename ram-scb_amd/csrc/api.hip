@@ -19,15 +19,16 @@
 
 namespace ramgpu {
 __global__ void k_species(Tab t);
+__global__ void k_coef_ij(Tab t);
 __global__ void k_coef(Tab t);
 __global__ void k_ghost(Tab t);
-__global__ void k_ij(Tab t, Program prog, int s0, int ntiles, int kw);
-template <int MAXC>
+__global__ void k_ij(Tab t, Program prog, int s0, int ntiles, int kws);
 __global__ void k_kl(Tab t, Program prog, int s0);
 __global__ void k_finalize(Tab t, int cflmask, int summode, int s0, int nblk_kl, int nblk_ij,
                            double *elorc_out);
 __global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
                      double kp_val, int nr1, int nt);
+__global__ void k_copy(double2 *dst, const double2 *src, size_t n2);
 __global__ void k_f2_in(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
 __global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
 }  // namespace ramgpu
@@ -78,9 +79,15 @@ struct ramgpu_ctx {
   bool have_species[kMaxSpecies] = {};
   bool have_lzphi = false;
   bool tables_dirty = true, ghost_dirty = true;
+  bool efield_dirty = true;  // only the VT-dependent per-(I,J) terms are stale
   bool cfl_valid = false;  // DtDrift* current for the present tables
-  int kw = 8;             // IJ tile width (consecutive (L,K) positions)
-  int ij_threads = 256, kl_threads = 256;
+  // opt-in per-kernel timing: event pairs around every launch, by kind
+  bool timing = false;
+  std::vector<hipEvent_t> tev;  // 2 per recorded launch
+  std::vector<int> tkind;
+  size_t tused = 0;
+  int kws = 3;            // IJ tile width 2^kws (consecutive (L,K) positions)
+  int ij_threads = 256, kl_threads = 512;
 };
 
 namespace {
@@ -124,9 +131,35 @@ int launch_check(const char *what) {
   return RAMGPU_OK;
 }
 
-size_t ij_lds(const ramgpu_ctx *c) { return (2 * (size_t)c->nr * c->nt * c->kw + 16) * sizeof(double); }
+void tbeg(ramgpu_ctx *c, int kind) {
+  if (!c->timing) return;
+  if (2 * c->tused + 2 > c->tev.size()) {
+    for (int n = 0; n < 2; ++n) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        c->timing = false;
+        return;
+      }
+      c->tev.push_back(e);
+    }
+  }
+  if (c->tkind.size() < c->tused + 1) c->tkind.resize(c->tused + 1);
+  c->tkind[c->tused] = kind;
+  (void)hipEventRecord(c->tev[2 * c->tused], c->stream);
+}
+
+void tend(ramgpu_ctx *c) {
+  if (!c->timing) return;
+  (void)hipEventRecord(c->tev[2 * c->tused + 1], c->stream);
+  ++c->tused;
+}
+
+size_t ij_lds(const ramgpu_ctx *c) {
+  const size_t kw = (size_t)1 << c->kws;
+  return (2 * (size_t)c->nr * c->nt * kw + 16) * sizeof(double) + (size_t)c->nt * kw * sizeof(int);
+}
 size_t kl_lds(const ramgpu_ctx *c) {
-  return (2 * (size_t)c->ne * c->npa + 16 + c->ne + c->npa) * sizeof(double);
+  return (3 * (size_t)c->ne * c->npa + 16 + c->ne + c->npa) * sizeof(double);
 }
 
 // Rebuild derived tables (species tables, Courant-number terms) and the
@@ -137,52 +170,62 @@ int prepare(ramgpu_ctx *c) {
   Tab &t = c->t;
   if (c->tables_dirty) {
     t.omet = 7.3E-5 * t.dts / t.dphi;  // OME*DTs/DPHI, src/ModRamDrift.f90:239
+    tbeg(c, RAMGPU_KERNEL_PREPARE);
     hipLaunchKernelGGL(k_species, dim3(c->ns), dim3(256), 0, c->stream, t);
     if ((rc = launch_check("k_species"))) return rc;
     const int bt = ((c->npa + 63) / 64) * 64;
     hipLaunchKernelGGL(k_coef, dim3(c->nr * c->nt), dim3(bt), 0, c->stream, t);
     if ((rc = launch_check("k_coef"))) return rc;
+    tend(c);
     c->tables_dirty = false;
+    c->efield_dirty = true;
+  }
+  if (c->efield_dirty) {
+    tbeg(c, RAMGPU_KERNEL_PREPARE);
+    hipLaunchKernelGGL(k_coef_ij, dim3((c->nr * c->nt + 255) / 256), dim3(256), 0, c->stream, t);
+    tend(c);
+    if ((rc = launch_check("k_coef_ij"))) return rc;
+    c->efield_dirty = false;
     c->ghost_dirty = true;
     c->cfl_valid = false;
   }
   if (c->ghost_dirty) {
+    tbeg(c, RAMGPU_KERNEL_PREPARE);
     hipLaunchKernelGGL(k_ghost, dim3(c->ns), dim3(1024), 0, c->stream, t);
+    tend(c);
     if ((rc = launch_check("k_ghost"))) return rc;
     c->ghost_dirty = false;
   }
   return RAMGPU_OK;
 }
 
-int ntiles(const ramgpu_ctx *c) { return (c->ne * c->npa + c->kw - 1) / c->kw; }
+int ntiles(const ramgpu_ctx *c) {
+  const int kw = 1 << c->kws;
+  return (c->ne * c->npa + kw - 1) / kw;
+}
 
 int run_ij(ramgpu_ctx *c, const Program &p, int s0, int nsp) {
   const int nt = ntiles(c);
+  tbeg(c, RAMGPU_KERNEL_IJ);
   hipLaunchKernelGGL(k_ij, dim3(nsp * nt), dim3(c->ij_threads), ij_lds(c), c->stream, c->t, p, s0,
-                     nt, c->kw);
+                     nt, c->kws);
+  tend(c);
   return launch_check("k_ij");
 }
 
 int run_kl(ramgpu_ctx *c, const Program &p, int s0, int nsp) {
   const int nb = nsp * (c->nr - 1) * c->nt;
-  const int cpt = (c->ne * c->npa + 255) / 256;  // cells per thread
-  const size_t lds = kl_lds(c);
-  if (cpt <= 8)
-    hipLaunchKernelGGL(k_kl<8>, dim3(nb), dim3(256), lds, c->stream, c->t, p, s0);
-  else if (cpt <= 16)
-    hipLaunchKernelGGL(k_kl<16>, dim3(nb), dim3(256), lds, c->stream, c->t, p, s0);
-  else if (cpt <= 24)
-    hipLaunchKernelGGL(k_kl<24>, dim3(nb), dim3(256), lds, c->stream, c->t, p, s0);
-  else if (cpt <= 32)
-    hipLaunchKernelGGL(k_kl<32>, dim3(nb), dim3(256), lds, c->stream, c->t, p, s0);
-  else
-    return fail(RAMGPU_EINVAL, "NE*NPA=%d above the 8192-cell plane limit", c->ne * c->npa);
+  tbeg(c, RAMGPU_KERNEL_KL);
+  hipLaunchKernelGGL(k_kl, dim3(nb), dim3(c->kl_threads), kl_lds(c), c->stream, c->t, p, s0);
+  tend(c);
   return launch_check("k_kl");
 }
 
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp, double *elorc) {
+  tbeg(c, RAMGPU_KERNEL_FINALIZE);
   hipLaunchKernelGGL(k_finalize, dim3(nsp), dim3(256), 0, c->stream, c->t, cflmask, summode, s0,
                      (c->nr - 1) * c->nt, ntiles(c), elorc);
+  tend(c);
   return launch_check("k_finalize");
 }
 
@@ -280,10 +323,9 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   A(t.p2, (size_t)ns * nr * ne); A(t.edot, (size_t)ns * nr * ne); A(t.sigv, (size_t)ns * ne);
   A(t.atlos, (size_t)ns * nr * ne); A(t.rel, (size_t)ns * 8);
   A(t.carry, (size_t)ns * ne * npa);
+  A(t.latl, (size_t)ns * nr * ne);
   t.maxblk = std::max((nr - 1) * nt, ne * npa);  // covers KL planes and IJ tiles (kw>=1)
-  A(t.cflr, (size_t)ns * t.maxblk); A(t.cflp, (size_t)ns * t.maxblk);
-  A(t.cfle, (size_t)ns * ((nr - 1) * nt) * ne + (size_t)ns * t.maxblk * ne);
-  A(t.cflm, (size_t)ns * ((nr - 1) * nt) * npa + (size_t)ns * t.maxblk * npa);
+  A(t.cflmax, (size_t)ns * (2 + ne + npa));
   A(t.dtdrift, (size_t)ns * 4);
   A(t.f2, nf2);
   A(c->d_stage, nf2);
@@ -300,22 +342,19 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   t.dbdt = c->d_dbdt; t.vt = c->d_vt; t.eir = c->d_eir; t.eip = c->d_eip; t.mgnp = c->d_mgnp;
   t.fgeos = c->d_fgeos; t.bscale = c->d_bscale;
   t.dts = 5.0; t.betalim = 1.5; t.fraccfl = 0.8;
+  t.rnt = 1.0f / (float)nt;
+  t.rne = 1.0f / (float)ne;
   std::vector<double> ones(ns, 1.0);
   if (hipMemcpy(c->d_bscale, ones.data(), ns * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(RAMGPU_EHIP, "hipMemcpy failed"));
-  // launch shapes: IJ tile width chosen so the tile fits the LDS budget
-  while (c->kw > 1 && ij_lds(c) > std::min<size_t>(c->lds_limit, 64 * 1024)) c->kw /= 2;
+  // launch shapes: IJ tile width 2^kws chosen so the tile fits the LDS budget
+  while (c->kws > 0 && ij_lds(c) > std::min<size_t>(c->lds_limit, 64 * 1024)) --c->kws;
   if (ij_lds(c) > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NR*NT=%d plane does not fit the device LDS", nr * nt));
   if (kl_lds(c) > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS", ne * npa));
   (void)hipFuncSetAttribute((const void *)k_ij, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  (void)hipFuncSetAttribute((const void *)k_kl<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  (void)hipFuncSetAttribute((const void *)k_kl<16>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  (void)hipFuncSetAttribute((const void *)k_kl<24>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  (void)hipFuncSetAttribute((const void *)k_kl<32>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  if (ne * npa > 32 * 256)
-    return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d above the 8192-cell plane limit", ne * npa));
+  (void)hipFuncSetAttribute((const void *)k_kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
   *out = c;
   return RAMGPU_OK;
 }
@@ -325,6 +364,7 @@ int ramgpu_destroy(ramgpu_ctx *c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) (void)hipFree(p);
+  for (hipEvent_t e : c->tev) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -416,7 +456,7 @@ int ramgpu_set_efield(ramgpu_ctx *c, const double *vt) {
   if ((rc = set_f64(c, c->d_vt, vt, (size_t)(c->nr + 1) * c->nt, "vt"))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->have_vt = true;
-  c->tables_dirty = true;
+  c->efield_dirty = true;
   return RAMGPU_OK;
 }
 
@@ -433,7 +473,7 @@ int ramgpu_set_kp(ramgpu_ctx *c, double kp, const double *lz, const double *phi)
                      (const double *)c->d_phi, (const double *)nullptr, kp, c->nr + 1, c->nt);
   if ((rc = launch_check("k_vs"))) return rc;
   c->have_vt = true;
-  c->tables_dirty = true;
+  c->efield_dirty = true;
   return RAMGPU_OK;
 }
 
@@ -650,10 +690,12 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (int n = 0; n < nsteps && rc == RAMGPU_OK; ++n) {
+    tbeg(c, RAMGPU_KERNEL_VS);
     hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
                        (const double *)c->d_phi, (const double *)(d_kp + n), 0.0, c->nr + 1, c->nt);
+    tend(c);
     if ((rc = launch_check("k_vs"))) break;
-    c->tables_dirty = true;
+    c->efield_dirty = true;
     if (d_bs) {
       if (hipMemcpyAsync(c->d_bscale, d_bs + (size_t)n * c->ns, c->ns * sizeof(double),
                          hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
@@ -680,6 +722,64 @@ int ramgpu_last_run_ms(ramgpu_ctx *c, double *ms) {
   float f = 0.f;
   HIP_TRY(hipEventElapsedTime(&f, c->ev0, c->ev1));
   *ms = f;
+  return RAMGPU_OK;
+}
+
+int ramgpu_copy_probe(int device, size_t bytes, int reps, double *gbps) {
+  if (!gbps || reps < 1 || bytes < 4096) return fail(RAMGPU_EINVAL, "bad copy probe arguments");
+  HIP_TRY(hipSetDevice(device));
+  const size_t n2 = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  HIP_TRY(hipMalloc(&a, n2 * 16));
+  if (hipMalloc(&b, n2 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return fail(RAMGPU_ENOMEM, "copy probe allocation failed");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  (void)hipMemset(a, 0, n2 * 16);
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, b, (const double2 *)a, n2);  // warm
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, b, (const double2 *)a, n2);
+  (void)hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (err != hipSuccess) return fail(RAMGPU_EHIP, "copy probe: %s", hipGetErrorString(err));
+  *gbps = 2.0 * (double)(n2 * 16) * reps / (ms * 1e-3) / 1e9;
+  return RAMGPU_OK;
+}
+
+int ramgpu_kernel_timing(ramgpu_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->timing = enable != 0;
+  c->tused = 0;
+  return RAMGPU_OK;
+}
+
+int ramgpu_kernel_times(ramgpu_ctx *c, double *total_ms, int *launches) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!total_ms || !launches) return fail(RAMGPU_EINVAL, "null output array");
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < RAMGPU_KERNEL_KINDS; ++k) {
+    total_ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (size_t n = 0; n < c->tused; ++n) {
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, c->tev[2 * n], c->tev[2 * n + 1]));
+    total_ms[c->tkind[n]] += f;
+    launches[c->tkind[n]] += 1;
+  }
+  c->tused = 0;
   return RAMGPU_OK;
 }
 

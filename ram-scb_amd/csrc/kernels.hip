@@ -48,6 +48,16 @@ __device__ __forceinline__ size_t ijl_of(const Tab &t, int i, int j, int l) {
   return ((size_t)i * t.nt + j) * t.npa + l;
 }
 
+// n/d for 0 <= n < 2^22, d >= 1, from rd = RN_float(1/d): the float
+// quotient is within one of the true one, and one remainder test fixes it.
+__device__ __forceinline__ int fast_div(int n, int d, float rd) {
+  int q = __float2int_rz((float)n * rd);
+  const int r = n - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
+
 // a/d, correctly rounded, from rd = RN(1/d): q = RN(a*rd) lies within one ulp
 // of a/d, the residual a - d*q is exact under FMA, and one correction step
 // rounds to RN(a/d) (Markstein, IBM J. Res. Dev. 34(1), 1990).
@@ -123,6 +133,20 @@ __device__ double block_max(double v, double *red) {
   return m;  // valid in thread 0
 }
 
+__device__ double block_min(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double m = v;
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; ++w) m = fmin(m, red[w]);
+  }
+  return m;  // valid in thread 0
+}
+
 __device__ __forceinline__ void lds_max(unsigned long long *slot, double v) {
   atomicMax(slot, (unsigned long long)__double_as_longlong(v));  // v >= 0
 }
@@ -151,6 +175,7 @@ __global__ void k_species(Tab t) {
       atl = exp(-dts / taub);
     }
     t.atlos[sik] = atl;
+    t.latl[sik] = log(atl);
     if (i == 0) {
       t.p4[s * t.ne + k] = dts * ekev * 1000.0 * (g + 1.0) / g / dphi / mdr / qs;
       t.ekp[s * t.ne + k] = ebnd * 1e3 * (gb + 1.0) / 2.0 / gb;
@@ -197,13 +222,73 @@ __global__ void k_species(Tab t) {
 }
 
 // ---------------------------------------------------------------------------
-// Species-independent intermediate terms, one block per (I,J), one thread per
-// L, each written with the reference's expression:
+// Species-independent intermediate terms, each written with the reference's
+// expression: k_coef_ij the per-(I,J) and per-I terms (the ones the
+// convection potential VT enters), k_coef one block per (I,J), one thread per
+// L, the per-(I,J,L) terms:
 //   DRIFTR  src/ModRamDrift.f90:118-144   CR, CGR3/(FNHS+FNHS')
 //   DRIFTP  :232-239                       GPA1+GPA2, FNHS sums, VT and EIR terms
 //   DRIFTE  :320-341                       DRD1, DPD1, GPR*, GPP*, DRD2, DPD2, dBdt1, dIdt1
 //   DRIFTMU :415-432                       DRM1, DPM1, CMUDOT, GMR*, GMP*, DRM2, DPM2, dBdt2, dIbndt2
 //   ATMOL   src/ModRamLoss.f90:501         1/FNHS
+__global__ void k_coef_ij(Tab t) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= t.nr * t.nt) return;
+  const int i = n / t.nt, j = n % t.nt;
+  const int I = i + 1, J = j + 1;
+  const int NT = t.nt;
+  const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
+  const double rlz = t.rlz[i];
+  const size_t ij = ij_of(t, i, j);
+  if (j == 0) {
+    t.rrlz[i] = 1.0 / rlz;
+    const double rs = rlz + 0.5 * mdr;
+    t.rsum[i] = rs;
+    t.rrsum[i] = 1.0 / rs;
+  }
+  {
+    const int J0 = (J == 1) ? NT - 1 : J - 1;
+    const int J1 = (J == NT) ? 2 : J + 1;
+    const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
+    const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
+    const double dv = GEO2(t.vt, I, J0) + GEO2(t.vt, I + 1, J0) - GEO2(t.vt, I, J1) - GEO2(t.vt, I + 1, J1);
+    t.cr[ij] = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
+    t.bsr[ij] = bsum;
+    t.rbsr[ij] = 1.0 / bsum;
+  }
+  double pa = 0.0, pce = 0.0, pbs = 1.0;
+  double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
+  const double bij = GEO2(t.bnes, I, J);
+  if (I >= 2) {
+    if (J >= 2) {
+      const int J1 = (J == NT) ? 2 : J + 1;
+      const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
+      pa = (GEO2(t.vt, I + 1, J) + GEO2(t.vt, I + 1, J1) - GEO2(t.vt, I - 1, J) - GEO2(t.vt, I - 1, J1)) * p1;
+      pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
+      pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
+    }
+    const int J0 = (J == 1) ? NT - 1 : J - 1;
+    const int J2 = (J == NT) ? 2 : J + 1;
+    drd1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
+    dpd1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+    // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
+    drm1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
+    dpm1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+    db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
+  }
+  t.pa[ij] = pa;
+  t.pce[ij] = pce;
+  t.pbs[ij] = pbs;
+  t.rpbs[ij] = 1.0 / pbs;
+  t.drd1[ij] = drd1;
+  t.dpd1[ij] = dpd1;
+  t.bij[ij] = bij;
+  t.rbij[ij] = 1.0 / bij;
+  t.drm1[ij] = drm1;
+  t.dpm1[ij] = dpm1;
+  t.db2[ij] = db2;
+}
+
 __global__ void k_coef(Tab t) {
   const int b = blockIdx.x;
   const int i = b / t.nt, j = b % t.nt;
@@ -213,57 +298,6 @@ __global__ void k_coef(Tab t) {
   const int L = l + 1;
   const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
   const double rlz = t.rlz[i];
-  const size_t ij = ij_of(t, i, j);
-
-  if (l == 0 && j == 0) {
-    t.rrlz[i] = 1.0 / rlz;
-    const double rs = rlz + 0.5 * mdr;
-    t.rsum[i] = rs;
-    t.rrsum[i] = 1.0 / rs;
-  }
-  if (l == 0) {  // per-(I,J) terms
-    {
-      const int J0 = (J == 1) ? NT - 1 : J - 1;
-      const int J1 = (J == NT) ? 2 : J + 1;
-      const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
-      const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
-      const double dv = GEO2(t.vt, I, J0) + GEO2(t.vt, I + 1, J0) - GEO2(t.vt, I, J1) - GEO2(t.vt, I + 1, J1);
-      t.cr[ij] = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
-      t.bsr[ij] = bsum;
-      t.rbsr[ij] = 1.0 / bsum;
-    }
-    double pa = 0.0, pce = 0.0, pbs = 1.0;
-    double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
-    const double bij = GEO2(t.bnes, I, J);
-    if (I >= 2) {
-      if (J >= 2) {
-        const int J1 = (J == NT) ? 2 : J + 1;
-        const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
-        pa = (GEO2(t.vt, I + 1, J) + GEO2(t.vt, I + 1, J1) - GEO2(t.vt, I - 1, J) - GEO2(t.vt, I - 1, J1)) * p1;
-        pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
-        pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
-      }
-      const int J0 = (J == 1) ? NT - 1 : J - 1;
-      const int J2 = (J == NT) ? 2 : J + 1;
-      drd1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-      dpd1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-      // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
-      drm1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-      dpm1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-      db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
-    }
-    t.pa[ij] = pa;
-    t.pce[ij] = pce;
-    t.pbs[ij] = pbs;
-    t.rpbs[ij] = 1.0 / pbs;
-    t.drd1[ij] = drd1;
-    t.dpd1[ij] = dpd1;
-    t.bij[ij] = bij;
-    t.rbij[ij] = 1.0 / bij;
-    t.drm1[ij] = drm1;
-    t.dpm1[ij] = dpm1;
-    t.db2[ij] = db2;
-  }
   if (l >= NPA) return;
 
   // ---- DRIFTR, all I (:140-144)
@@ -445,15 +479,22 @@ __device__ double stale_ghost(const Tab &t, int s, int j, int k, int l) {
 }
 
 // ---------------------------------------------------------------------------
-// IJ kernel: one block per (species, tile of KW consecutive (L,K) positions),
-// holding F[(I,J)][w] and the face fluxes G = C*FBND in LDS.  Runs a program
-// of OP_R / OP_P / OP_SUM / OP_SEAM.
-__global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int ntiles, int kw) {
+// IJ kernel: one block per (species, tile of KW = 2^kws consecutive (L,K)
+// positions), holding F[(I,J)][w] and the face fluxes G = C*FBND in LDS, plus
+// the inflow flag of every radial line.  Runs a program of OP_R / OP_P /
+// OP_SUM / OP_SEAM.
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *slot, double v) {
+  atomicMax(slot, (unsigned long long)__double_as_longlong(v));  // v >= 0
+}
+
+__global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int ntiles, int kws) {
   extern __shared__ double sm[];
+  const int kw = 1 << kws;
   const int nij = t.nr * t.nt;
   double *F = sm;
   double *G = sm + (size_t)nij * kw;
   double *red = G + (size_t)nij * kw;  // 16 doubles
+  int *inflow = (int *)(red + 16);      // NT*kw line flags
   const int s = s0 + blockIdx.x / ntiles;
   const int tile = blockIdx.x % ntiles;
   const int P = t.ne * t.npa;
@@ -461,11 +502,11 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
   const int nw = min(kw, P - q0);
   const int ncell = nij * kw;
   double *gF = t.f2 + (size_t)s * f2_species_stride(t);
-  const int NR = t.nr, NT = t.nt;
+  const int NR = t.nr, NT = t.nt, NE = t.ne;
   const double beta = t.betalim;
 
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const int ij = c / kw, w = c % kw;
+    const int ij = c >> kws, w = c & (kw - 1);
     F[c] = (w < nw) ? gF[(size_t)ij * P + q0 + w] : 0.0;
   }
   __syncthreads();
@@ -474,22 +515,32 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
     const int op = prog.op[o];
     if (op == OP_R) {
       // DRIFTR, src/ModRamDrift.f90:139-191: face I of line (J,w) per cell
+      for (int n = threadIdx.x; n < NT * kw; n += blockDim.x) {  // sgn(NR) per line (:154)
+        const int j = n >> kws, w = n & (kw - 1);
+        int f = 0;
+        if (w < nw) {
+          const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
+          f = line_inflow(t, s, j, k, l) ? 1 : 0;
+        }
+        inflow[n] = f;
+      }
+      __syncthreads();
       double cmax = 0.0;
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
+        const int ij = c >> kws, w = c & (kw - 1);
         if (w >= nw) continue;
-        const int i = ij / NT, j = ij % NT;
-        const int q = q0 + w, k = q % t.ne, l = q / t.ne;
+        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
+        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
         const int I = i + 1;
         const double ci = c_radial(t, s, i, j, k, l);
         if (MGNP(I, j + 1) == 0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));
-        const bool inflow = line_inflow(t, s, j, k, l);
-        const int base = j * kw + w;  // F(I') at base + (I'-1)*NT*kw
-#define FR(II) F[base + ((II)-1) * NT * kw]
+        const bool in = inflow[(j << kws) + w] != 0;
+        const int base = (j << kws) + w;  // F(I') at base + (I'-1)*NT*kw
+#define FR(II) F[base + (((II)-1) * NT << kws)]
         double fb;
         if (I == 1) {
-          fb = inflow ? FR(2) : 0.0;
-        } else if (!inflow && I == NR) {
+          fb = in ? FR(2) : 0.0;
+        } else if (!in && I == NR) {
           fb = FR(NR);
         } else {
           const double f0 = FR(I);
@@ -510,7 +561,7 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
               if (I + 2 <= NR)
                 fn = FR(I + 2);
               else  // I == NR-1: the ghost F(NR+1)
-                fn = inflow ? ghost_value(t, s, j, k, l, t.conf1) : stale_ghost(t, s, j, k, l);
+                fn = in ? ghost_value(t, s, j, k, l, t.conf1) : stale_ghost(t, s, j, k, l);
             } else {
               fn = f0;
               fnm1 = FR(I - 1);
@@ -523,13 +574,13 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
       }
       if (prog.arg[o]) {
         const double m = block_max(cmax, red);
-        if (threadIdx.x == 0) t.cflr[(size_t)s * t.maxblk + tile] = m;
+        if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 0], m);
       }
       __syncthreads();
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
+        const int ij = c >> kws, w = c & (kw - 1);
         if (w >= nw || ij < NT) continue;  // I>=2
-        double v = F[c] - G[c] + G[c - NT * kw];
+        double v = F[c] - G[c] + G[c - (NT << kws)];
         if (v < 0) v = 1E-15;
         F[c] = v;
       }
@@ -538,16 +589,16 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
       // DRIFTP, src/ModRamDrift.f90:229-272 (periodic in J)
       double cmax = 0.0;
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
+        const int ij = c >> kws, w = c & (kw - 1);
         if (w >= nw) continue;
-        const int i = ij / NT, j = ij % NT;
+        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
         const int J = j + 1;
         if (i < 1 || J < 2) continue;
-        const int q = q0 + w, k = q % t.ne, l = q / t.ne;
+        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
         const double cp = c_azim(t, s, i, j, k, l);
         if (MGNP(i + 1, J) == 0) cmax = fmax(cmax, fmax(fabs(cp), 1E-10));
-        const int base = i * NT * kw + w;  // F(J') at base + (J'-1)*kw
-#define FP(JJ) F[base + ((JJ)-1) * kw]
+        const int base = ((i * NT) << kws) + w;  // F(J') at base + (J'-1)*kw
+#define FP(JJ) F[base + (((JJ)-1) << kws)]
         const int J1 = (J == NT) ? 2 : J + 1;
         const double f0 = FP(J), fp1 = FP(J1);
         double fn, fnm1;
@@ -565,35 +616,34 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
       }
       if (prog.arg[o]) {
         const double m = block_max(cmax, red);
-        if (threadIdx.x == 0) t.cflp[(size_t)s * t.maxblk + tile] = m;
+        if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 1], m);
       }
       __syncthreads();
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
+        const int ij = c >> kws, w = c & (kw - 1);
         if (w >= nw) continue;
-        const int i = ij / NT, j = ij % NT;
+        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
         if (i < 1 || j < 1) continue;
         // FBND(1)=FBND(NT), CDriftP(I,1)=CDriftP(I,NT)  (:261-262)
-        const double gm = (j == 1) ? G[(i * NT + NT - 1) * kw + w] : G[c - kw];
+        const double gm = (j == 1) ? G[((i * NT + NT - 1) << kws) + w] : G[c - kw];
         double v = F[c] - G[c] + gm;
         if (v < 0) v = 1E-15;
         F[c] = v;
       }
       __syncthreads();
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {  // F2(S,I,1)=F2(S,I,NT)
-        const int ij = c / kw, w = c % kw;
-        const int i = ij / NT, j = ij % NT;
-        if (w < nw && i >= 1 && j == 0) F[c] = F[(i * NT + NT - 1) * kw + w];
+      for (int n = threadIdx.x; n < (NR - 1) * kw; n += blockDim.x) {  // F2(S,I,1)=F2(S,I,NT)
+        const int i = 1 + (n >> kws), w = n & (kw - 1);
+        if (w < nw) F[((i * NT) << kws) + w] = F[((i * NT + NT - 1) << kws) + w];
       }
       __syncthreads();
     } else if (op == OP_SUM) {
       // SUMRC over I>=2, K>=2, L>=2, J<=NT-1 (src/ModRamRun.f90:246-255)
       double acc = 0.0;
       for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
+        const int ij = c >> kws, w = c & (kw - 1);
         if (w >= nw) continue;
-        const int i = ij / NT, j = ij % NT;
-        const int q = q0 + w, k = q % t.ne, l = q / t.ne;
+        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
+        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
         if (i < 1 || k < 1 || l < 1 || j > NT - 2) continue;
         acc += t.ekev[k] * (F[c] * t.we[k] * t.wmu[l]);
       }
@@ -603,50 +653,51 @@ __global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int nti
     } else if (op == OP_SEAM) {
       // F2(:,:,NT,:,:)=F2(:,:,1,:,:) then outside-magnetopause cells to 1e-31
       // (src/ModRamRun.f90:186,197-201)
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c / kw, w = c % kw;
-        if (w >= nw) continue;
-        const int i = ij / NT, j = ij % NT;
-        double v = F[c];
-        if (j == NT - 1) v = F[(i * NT) * kw + w];
-        if (MGNP(i + 1, j + 1) == 1) v = 1e-31;
-        G[c] = v;
+      for (int n = threadIdx.x; n < NR * kw; n += blockDim.x) {
+        const int i = n >> kws, w = n & (kw - 1);
+        if (w < nw) F[((i * NT + NT - 1) << kws) + w] = F[((i * NT) << kws) + w];
       }
       __syncthreads();
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) F[c] = G[c];
+      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int ij = c >> kws, w = c & (kw - 1);
+        if (w >= nw) continue;
+        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
+        if (MGNP(i + 1, j + 1) == 1) F[c] = 1e-31;
+      }
       __syncthreads();
     }
   }
 
   for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const int ij = c / kw, w = c % kw;
+    const int ij = c >> kws, w = c & (kw - 1);
     if (w < nw) gF[(size_t)ij * P + q0 + w] = F[c];
   }
 }
 
 // ---------------------------------------------------------------------------
-// KL kernel: one block of 256 threads per (species, I>=2, J), holding the
-// whole (K,L) plane F[l*NE+k] in LDS.  A face-owning thread keeps its own
-// cell's flux term in registers (own[]) and publishes the term its upper
-// neighbour needs in H[]: the reference divides both face fluxes of cell K by
-// that cell's width, CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*FBND(K-1)
-// (src/ModRamDrift.f90:364; :460 for Mu).  MAXC = cells per thread.
+// KL kernel: one block per (species, I>=2, J), holding the whole (K,L) plane
+// F[l*NE+k] in LDS.  The reference divides both face fluxes of cell K by that
+// cell's width, CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*FBND(K-1)
+// (src/ModRamDrift.f90:364; :460 for Mu), so the owner of face K writes the
+// cell's own partial update O = F(K) - C(K)/WE(K)*FBND(K) and, in H, the term
+// C(K)/WE(K+1)*FBND(K) its upper neighbour adds.
 __device__ __forceinline__ double cex_factor(const Tab &t, int s, int k, size_t q) {
   // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49)
   return exp(-(t.sigv[s * t.ne + k] * t.hdns[q] * t.dts));
 }
 __device__ __forceinline__ double atm_factor(const Tab &t, int s, int i, int k, size_t q) {
-  // ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501)
-  return pow(t.atlos[((size_t)s * t.nr + i) * t.ne + k], t.invfnhs[q]);
+  // ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501) as
+  // exp(log(ATLOS)*(1/FNHS))
+  return exp(t.latl[((size_t)s * t.nr + i) * t.ne + k] * t.invfnhs[q]);
 }
 
-template <int MAXC>
-__global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
+__global__ void __launch_bounds__(512) k_kl(Tab t, Program prog, int s0) {
   extern __shared__ double sm[];
   const int NE = t.ne, NPA = t.npa, NT = t.nt;
   const int P = NE * NPA;
   double *F = sm;
-  double *H = F + P;
+  double *O = F + P;
+  double *H = O + P;
   double *red = H + P;  // 16
   unsigned long long *cmx = (unsigned long long *)(red + 16);  // NE + NPA
   const int nplane = (t.nr - 1) * NT;
@@ -665,14 +716,15 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
   const double qs = (double)t.charge[s];  // x/QS == x*QS for QS = +-1
   const double rlz = t.rlz[i], rrlz = t.rrlz[i];
   const double bij = t.bij[ij], rbij = t.rbij[ij];
-  double own[MAXC];
+  const float rne = t.rne;
+  unsigned long long *gcfl = t.cflmax + (size_t)s * (2 + NE + NPA);
 
   for (int c = threadIdx.x; c < P; c += blockDim.x) F[c] = gF[c];
   __syncthreads();
 
   for (int o = 0; o < prog.n; ++o) {
     const int op = prog.op[o];
-    const bool rec = prog.arg[o] != 0;
+    const bool rec = prog.arg[o] != 0 && inside;
     if (op == OP_E) {
       // DRIFTE, src/ModRamDrift.f90:333-369.  F(1),F(0) relativistic ghosts
       // from F(2); F(NE+1)=F(NE+2)=0.
@@ -682,39 +734,34 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
       }
       const double *rel = t.rel + s * 8;
       const double drd1 = t.drd1[ij], dpd1 = t.dpd1[ij];
-#pragma unroll
-      for (int n = 0; n < MAXC; ++n) {
-        const int c = threadIdx.x + n * 256;
-        if (c < P) {
-          const int l = c / NE, k = c % NE;
-          const int K = k + 1;
-          const size_t q = qb + l;
-          // CDriftE(I,J,K,L), :337-342
-          const double edt1 = div_rn(div_rn(div_rn(t.ekp[s * NE + k], t.hs[q], t.rhs[q]), rlz, rrlz), bij, rbij) * qs;
-          const double drdt = drd1 + edt1 * t.drd2[q] * rlz;
-          const double dpdt = dpd1 - edt1 * t.dpd2[q];
-          const double ce = t.edot[sib + k] * (t.gr[q] * drdt + t.gp[q] * dpdt + t.db1[q] + t.di1[q]);
-          if (rec && inside) lds_max(&cmx[k], fmax(fabs(ce), 1E-10));
-          const double *row = F + l * NE;
-          const double f2v = row[1];
-          const double g1 = f2v * rel[0] / rel[1] * rel[2];
-          const double g0 = g1 * rel[3] / rel[0] * rel[4];
+      for (int c = threadIdx.x; c < P; c += blockDim.x) {
+        const int l = fast_div(c, NE, rne), k = c - l * NE;
+        const int K = k + 1;
+        const size_t q = qb + l;
+        // CDriftE(I,J,K,L), :337-342
+        const double edt1 = div_rn(div_rn(div_rn(t.ekp[s * NE + k], t.hs[q], t.rhs[q]), rlz, rrlz), bij, rbij) * qs;
+        const double drdt = drd1 + edt1 * t.drd2[q] * rlz;
+        const double dpdt = dpd1 - edt1 * t.dpd2[q];
+        const double ce = t.edot[sib + k] * (t.gr[q] * drdt + t.gp[q] * dpdt + t.db1[q] + t.di1[q]);
+        if (rec) lds_max(&cmx[k], fmax(fabs(ce), 1E-10));
+        const double *row = F + l * NE;
+        const double f2v = row[1];
+        const double g1 = f2v * rel[0] / rel[1] * rel[2];
+        const double g0 = g1 * rel[3] / rel[0] * rel[4];
 #define FE(KK) ((KK) == 0 ? g0 : (KK) == 1 ? g1 : (KK) <= NE ? row[(KK)-1] : 0.0)
-          const double f0 = FE(K), fp1 = FE(K + 1);
-          const double fn = (ce < 0) ? FE(K + 2) : f0;
-          const double fnm1 = (ce < 0) ? fp1 : FE(K - 1);
+        const double f0 = FE(K), fp1 = FE(K + 1);
+        const double fn = (ce < 0) ? FE(K + 2) : f0;
+        const double fnm1 = (ce < 0) ? fp1 : FE(K - 1);
 #undef FE
-          const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, t.de[k], t.rde[k]), beta);
-          own[n] = div_rn(ce, t.we[k], t.rwe[k]) * fb;
-          H[c] = (k + 1 < NE) ? div_rn(ce, t.we[k + 1], t.rwe[k + 1]) * fb : 0.0;
-        }
+        const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, t.de[k], t.rde[k]), beta);
+        O[c] = row[k] - div_rn(ce, t.we[k], t.rwe[k]) * fb;
+        H[c] = (k + 1 < NE) ? div_rn(ce, t.we[k + 1], t.rwe[k + 1]) * fb : 0.0;
       }
       __syncthreads();
-#pragma unroll
-      for (int n = 0; n < MAXC; ++n) {
-        const int c = threadIdx.x + n * 256;
-        if (c < P && (c % NE) >= 1) {
-          double v = F[c] - own[n] + H[c - 1];
+      for (int c = threadIdx.x; c < P; c += blockDim.x) {
+        const int l = fast_div(c, NE, rne), k = c - l * NE;
+        if (k >= 1) {
+          double v = O[c] + H[c - 1];
           if (v < 0) v = 1E-15;
           F[c] = v;
         }
@@ -722,7 +769,7 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
       if (rec) {
         __syncthreads();
         for (int k = threadIdx.x; k < NE; k += blockDim.x)
-          t.cfle[((size_t)s * t.maxblk + pl) * NE + k] = __longlong_as_double((long long)cmx[k]);
+          if (cmx[k]) atomicMax(&gcfl[2 + k], cmx[k]);
       }
       __syncthreads();
     } else if (op == OP_MU) {
@@ -733,55 +780,45 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
         __syncthreads();
       }
       const double drm1 = t.drm1[ij], dpm1 = t.dpm1[ij], db2 = t.db2[ij];
-#pragma unroll
-      for (int n = 0; n < MAXC; ++n) {
-        const int c = threadIdx.x + n * 256;
-        if (c < P) {
-          const int l = c / NE, k = c % NE;
-          const int L = l + 1;
-          double on = 0.0, h = 0.0;
-          if (L >= 2) {
-            const size_t q = qb + l;
-            // CDriftMu(I,J,K,L), :418-433
-            const double edt = div_rn(div_rn(div_rn(t.mkp[s * NE + k], t.bhs[q], t.rbhs[q]), rlz, rrlz), bij, rbij) * qs;
-            const double drdm = drm1 + edt * t.drm2[q] * rlz;
-            const double dpdm = dpm1 - edt * t.dpm2[q];
-            const double cm = -t.cmudot[q] * (t.gmr[q] * drdm + t.gmp[q] * dpdm + db2 + t.di2[q]);
-            if (rec && inside) lds_max(&cmx[NE + l], fmax(fabs(cm), 1E-32));
-            if (L <= NPA - 1) {
+      for (int c = threadIdx.x; c < P; c += blockDim.x) {
+        const int l = fast_div(c, NE, rne), k = c - l * NE;
+        const int L = l + 1;
+        double on = 0.0, h = 0.0;
+        if (L >= 2) {
+          const size_t q = qb + l;
+          // CDriftMu(I,J,K,L), :418-433
+          const double edt = div_rn(div_rn(div_rn(t.mkp[s * NE + k], t.bhs[q], t.rbhs[q]), rlz, rrlz), bij, rbij) * qs;
+          const double drdm = drm1 + edt * t.drm2[q] * rlz;
+          const double dpdm = dpm1 - edt * t.dpm2[q];
+          const double cm = -t.cmudot[q] * (t.gmr[q] * drdm + t.gmp[q] * dpdm + db2 + t.di2[q]);
+          if (rec) lds_max(&cmx[NE + l], fmax(fabs(cm), 1E-32));
+          if (L <= NPA - 1) {
 #define FM(LL) F[(((LL) == 1 ? 2 : (LL)) - 1) * NE + k]
-              double fb;
-              if (L == NPA - 1) {
-                fb = FM(NPA);
-              } else {
-                const double f0 = FM(L), fp1 = FM(L + 1);
-                const double fn = (cm < 0) ? FM(L + 2) : f0;
-                const double fnm1 = (cm < 0) ? fp1 : FM(L - 1);
-                fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, t.dmu[l], t.rdmu[l]), beta);
-              }
-#undef FM
-              on = div_rn(cm, t.wmu[l], t.rwmu[l]) * fb;
-              h = div_rn(cm, t.wmu[l + 1], t.rwmu[l + 1]) * fb;
+            double fb;
+            if (L == NPA - 1) {
+              fb = FM(NPA);
+            } else {
+              const double f0 = FM(L), fp1 = FM(L + 1);
+              const double fn = (cm < 0) ? FM(L + 2) : f0;
+              const double fnm1 = (cm < 0) ? fp1 : FM(L - 1);
+              fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, t.dmu[l], t.rdmu[l]), beta);
             }
-          } else {
-            // face 1: CDriftMu(1)=0, FBND(1)=0 -> 0/WMU(2)*0
-            h = div_rn(0.0, t.wmu[1], t.rwmu[1]) * 0.0;
+#undef FM
+            on = div_rn(cm, t.wmu[l], t.rwmu[l]) * fb;
+            h = div_rn(cm, t.wmu[l + 1], t.rwmu[l + 1]) * fb;
           }
-          own[n] = on;
-          H[c] = h;
         }
+        // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
+        O[c] = F[c] - on;
+        H[c] = h;
       }
       __syncthreads();
-#pragma unroll
-      for (int n = 0; n < MAXC; ++n) {
-        const int c = threadIdx.x + n * 256;
-        if (c < P) {
-          const int l = c / NE;
-          if (l >= 1 && l <= NPA - 2) {
-            double v = F[c] - own[n] + H[c - NE];
-            if (v < 0) v = 1E-15;
-            F[c] = v;
-          }
+      for (int c = threadIdx.x; c < P; c += blockDim.x) {
+        const int l = fast_div(c, NE, rne);
+        if (l >= 1 && l <= NPA - 2) {
+          double v = O[c] + H[c - NE];
+          if (v < 0) v = 1E-15;
+          F[c] = v;
         }
       }
       __syncthreads();
@@ -791,12 +828,12 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
         F[(NPA - 1) * NE + k] = F[(NPA - 2) * NE + k] * hN * t.mu[NPA - 1] / hN1 / t.mu[NPA - 2];
       if (rec)
         for (int l = threadIdx.x; l < NPA; l += blockDim.x)
-          t.cflm[((size_t)s * t.maxblk + pl) * NPA + l] = __longlong_as_double((long long)cmx[NE + l]);
+          if (cmx[NE + l]) atomicMax(&gcfl[2 + NE + l], cmx[NE + l]);
       __syncthreads();
     } else if (op == OP_CEX) {
       if (cex) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
         for (int c = threadIdx.x; c < P; c += blockDim.x) {
-          const int l = c / NE, k = c % NE;
+          const int l = fast_div(c, NE, rne), k = c - l * NE;
           if (k >= 1 && l >= 1) F[c] = F[c] * cex_factor(t, s, k, qb + l);
         }
       }
@@ -804,7 +841,7 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
     } else if (op == OP_ATM) {
       // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
       for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = c / NE, k = c % NE;
+        const int l = fast_div(c, NE, rne), k = c - l * NE;
         if (k >= 1 && l + 1 >= upa) F[c] = F[c] * atm_factor(t, s, i, k, qb + l);
       }
       __syncthreads();
@@ -812,7 +849,7 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
       double acc = 0.0;
       if (sum_row)
         for (int c = threadIdx.x; c < P; c += blockDim.x) {
-          const int l = c / NE, k = c % NE;
+          const int l = fast_div(c, NE, rne), k = c - l * NE;
           if (k >= 1 && l >= 1) acc += t.ekev[k] * (F[c] * t.we[k] * t.wmu[l]);
         }
       const double tot = block_sum(acc, red);
@@ -823,7 +860,7 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
       // ATMOL, SUMRC, ATMOL, SUMRC, [CEX, SUMRC].  Slots arg..arg+4.
       double a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
       for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = c / NE, k = c % NE;
+        const int l = fast_div(c, NE, rne), k = c - l * NE;
         const bool w_on = sum_row && k >= 1 && l >= 1;
         const bool cx = cex && k >= 1 && l >= 1;
         const bool atm = k >= 1 && l + 1 >= upa;
@@ -854,16 +891,12 @@ __global__ void __launch_bounds__(256) k_kl(Tab t, Program prog, int s0) {
   for (int c = threadIdx.x; c < P; c += blockDim.x) gF[c] = F[c];
 }
 
-template __global__ void k_kl<8>(Tab, Program, int);
-template __global__ void k_kl<16>(Tab, Program, int);
-template __global__ void k_kl<24>(Tab, Program, int);
-template __global__ void k_kl<32>(Tab, Program, int);
-
 // ---------------------------------------------------------------------------
-// DtDrift*(S) from the CFL partial maxima (cflmask bits 1 R, 2 P, 4 E, 8 Mu):
-// min over cells of FracCFL*DTs*w/ctemp equals FracCFL*DTs*w / max(ctemp)
-// because correctly rounded division is monotone; initial values 1e5 (R,P)
-// and 1e4 (E,Mu) as the reference's (:115,223,308,404).
+// DtDrift*(S) from the CFL maxima (cflmask bits 1 R, 2 P, 4 E, 8 Mu): min
+// over cells of FracCFL*DTs*w/ctemp equals FracCFL*DTs*w / max(ctemp) because
+// correctly rounded division is monotone; initial values 1e5 (R,P) and 1e4
+// (E,Mu) as the reference's (:115,223,308,404).  The maxima are cleared after
+// use for the next recording.
 // SUMRC bookkeeping (src/ModRamRun.f90:77-175,239-257), summode 1: one SUMRC
 // from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
 // slots 0..5 (slot 5 from the IJ tiles).
@@ -874,43 +907,36 @@ __global__ void k_finalize(Tab t, int cflmask, int summode, int s0, int nblk_kl,
   const int s = s0 + blockIdx.x;
   const int NE = t.ne, NPA = t.npa;
   const double a = t.fraccfl * t.dts;
-  if (cflmask & 1) {
-    double m = 0.0;
-    for (int b = threadIdx.x; b < nblk_ij; b += blockDim.x) m = fmax(m, t.cflr[(size_t)s * t.maxblk + b]);
-    m = block_max(m, red);
-    if (threadIdx.x == 0) t.dtdrift[s * 4 + 0] = (m > 0) ? fmin(100000.0, a / m) : 100000.0;
+  unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
+  if (cflmask) {
+    double de = 10000.0, dm = 10000.0;
+    if (cflmask & 4)
+      for (int k = threadIdx.x; k < NE; k += blockDim.x) {
+        const double v = __longlong_as_double((long long)m[2 + k]);
+        if (v > 0) de = fmin(de, a * t.de[k] / v);
+      }
+    if (cflmask & 8)
+      for (int l = 1 + threadIdx.x; l < NPA; l += blockDim.x) {
+        const double v = __longlong_as_double((long long)m[2 + NE + l]);
+        if (v > 0) dm = fmin(dm, a * t.dmu[l] / v);
+      }
+    de = block_min(de, red);
     __syncthreads();
-  }
-  if (cflmask & 2) {
-    double m = 0.0;
-    for (int b = threadIdx.x; b < nblk_ij; b += blockDim.x) m = fmax(m, t.cflp[(size_t)s * t.maxblk + b]);
-    m = block_max(m, red);
-    if (threadIdx.x == 0) t.dtdrift[s * 4 + 1] = (m > 0) ? fmin(100000.0, a / m) : 100000.0;
+    dm = block_min(dm, red);
+    if (threadIdx.x == 0) {
+      if (cflmask & 1) {
+        const double v = __longlong_as_double((long long)m[0]);
+        t.dtdrift[s * 4 + 0] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
+      }
+      if (cflmask & 2) {
+        const double v = __longlong_as_double((long long)m[1]);
+        t.dtdrift[s * 4 + 1] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
+      }
+      if (cflmask & 4) t.dtdrift[s * 4 + 2] = de;
+      if (cflmask & 8) t.dtdrift[s * 4 + 3] = dm;
+    }
     __syncthreads();
-  }
-  if (cflmask & 4) {
-    double dt = 10000.0;
-    for (int k = 0; k < NE; ++k) {
-      double m = 0.0;
-      for (int b = threadIdx.x; b < nblk_kl; b += blockDim.x)
-        m = fmax(m, t.cfle[((size_t)s * t.maxblk + b) * NE + k]);
-      m = block_max(m, red);
-      if (threadIdx.x == 0 && m > 0) dt = fmin(dt, a * t.de[k] / m);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) t.dtdrift[s * 4 + 2] = dt;
-  }
-  if (cflmask & 8) {
-    double dt = 10000.0;
-    for (int l = 1; l < NPA; ++l) {
-      double m = 0.0;
-      for (int b = threadIdx.x; b < nblk_kl; b += blockDim.x)
-        m = fmax(m, t.cflm[((size_t)s * t.maxblk + b) * NPA + l]);
-      m = block_max(m, red);
-      if (threadIdx.x == 0 && m > 0) dt = fmin(dt, a * t.dmu[l] / m);
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) t.dtdrift[s * 4 + 3] = dt;
+    for (int e = threadIdx.x; e < 2 + NE + NPA; e += blockDim.x) m[e] = 0ull;
   }
   if (summode == 0) return;
   const int nslot = (summode == 1) ? 1 : 6;
@@ -962,6 +988,13 @@ __global__ void k_vs(double *vt, const double *lz, const double *phi, const doub
     const double lre = lz[i] * kRE;
     vt[n] = avs * (lre * lre) * sin(phi[j] - 0.0);
   }
+}
+
+// Plain streaming copy (16-byte accesses), the bandwidth and counter
+// calibration probe.
+__global__ void k_copy(double2 *__restrict__ dst, const double2 *__restrict__ src, size_t n2) {
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n2; x += (size_t)gridDim.x * blockDim.x)
+    dst[x] = src[x];
 }
 
 // Fortran F2(NS,NR,NT,NE,NPA) <-> device [S][I][J][L][K]

@@ -88,11 +88,14 @@ struct Tab {
   double *atlos;  // [s][i][k]
   double *rel;    // [s][8]: GREL1, GREL2, sq1, GRZERO, sq0
   int *carry;     // [s][NE*NPA] DRIFTR ghost carry
-  // CFL partial maxima
-  double *cflr, *cflp;  // [s][maxblk]
-  double *cfle;         // [s][maxblk][NE]
-  double *cflm;         // [s][maxblk][NPA]
+  double *latl;   // [s][i][k] log(ATLOS): ATLOS**(1/FNHS) = exp(log(ATLOS)/FNHS)
+  // CFL maxima of max(|C|,floor) over cells inside the magnetopause, as the
+  // bit patterns of non-negative doubles (atomicMax order = double order):
+  // [s][0] R, [s][1] P, [s][2+K] E per K, [s][2+NE+L] Mu per L
+  unsigned long long *cflmax;
   double *dtdrift;      // [s][4]
+  // float reciprocals for exact small-integer division (fast_div)
+  float rnt, rne;
   // state
   double *f2;
   double *partials;  // [slot][s][maxblk]

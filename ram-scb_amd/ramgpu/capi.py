@@ -26,8 +26,11 @@ EXPORTS = (
     "ramgpu_get_f2", "ramgpu_set_energy_state", "ramgpu_get_energy_state", "ramgpu_driftpara",
     "ramgpu_driftr", "ramgpu_driftp", "ramgpu_drifte", "ramgpu_driftmu", "ramgpu_driftend",
     "ramgpu_cepara", "ramgpu_charexchange", "ramgpu_atmol", "ramgpu_sumrc", "ramgpu_ram_run",
-    "ramgpu_ram_run_schedule", "ramgpu_last_run_ms",
+    "ramgpu_ram_run_schedule", "ramgpu_last_run_ms", "ramgpu_kernel_timing",
+    "ramgpu_kernel_times", "ramgpu_copy_probe",
 )
+
+KERNEL_KINDS = ("prepare", "ij", "kl", "finalize", "vs")
 
 
 class SpeciesOut(C.Structure):
@@ -81,6 +84,9 @@ def load_library(path: Path | None = None):
     lib.ramgpu_ram_run.argtypes = [v, C.c_int, C.POINTER(SpeciesOut)]
     lib.ramgpu_ram_run_schedule.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
+    lib.ramgpu_kernel_timing.argtypes = [v, C.c_int]
+    lib.ramgpu_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
+    lib.ramgpu_copy_probe.argtypes = [C.c_int, C.c_size_t, C.c_int, _dp]
     for n in EXPORTS:
         if n != "ramgpu_last_error":
             getattr(lib, n).restype = C.c_int
@@ -97,6 +103,16 @@ def _fd(a, n=None):
 
 def _p(a):
     return a.ctypes.data_as(_dp)
+
+
+def copy_probe(nbytes: int = 1 << 30, reps: int = 20, device: int = 0) -> float:
+    """Read+write GB/s of a plain device copy of nbytes (measured HBM peak)."""
+    lib = load_library()
+    out = C.c_double(0.0)
+    rc = lib.ramgpu_copy_probe(device, nbytes, reps, C.byref(out))
+    if rc:
+        raise RamGpuError(lib.ramgpu_last_error().decode())
+    return out.value
 
 
 class RamGpu:
@@ -264,6 +280,16 @@ class RamGpu:
         self._check(self.lib.ramgpu_ram_run_schedule(self.h, n, _p(kp),
                                                      None if bs is None else _p(bs), out))
         return [o.as_dict() for o in out]
+
+    def kernel_timing(self, enable: bool = True):
+        self._check(self.lib.ramgpu_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        """{kind: (total_ms, launches)} since the last call (then reset)."""
+        ms = (C.c_double * len(KERNEL_KINDS))()
+        n = (C.c_int * len(KERNEL_KINDS))()
+        self._check(self.lib.ramgpu_kernel_times(self.h, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_KINDS)}
 
     def last_run_ms(self) -> float:
         ms = C.c_double(0.0)
