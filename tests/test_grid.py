@@ -37,8 +37,12 @@ def test_relativistic_factors(g72):
     for s in range(g72.ns):
         mc2_kev = g72.rmas[s] * G.CS ** 2 / G.Q / 1e3
         np.testing.assert_allclose(g72.grel[s] - 1, g72.ekev / mc2_kev, rtol=1e-12, atol=2.3e-16)
-        # (the reference forms sqrt(GREL**2-1), which cancels at low energy)
-        np.testing.assert_allclose(g72.v[s], G.CS * np.sqrt(1 - 1 / g72.grel[s] ** 2), rtol=1e-9)
+        # the reference forms sqrt(GREL**2-1), which cancels when GREL-1 is
+        # tiny: compare with the cancellation-free form to eps/(GREL-1)
+        gm1 = g72.ekev / mc2_kev
+        exact = G.CS * np.sqrt(gm1 * (gm1 + 2)) / (1 + gm1)
+        tol = 4 * np.finfo(float).eps / gm1 + 1e-14
+        assert np.all(np.abs(g72.v[s] / exact - 1) <= tol)
 
 
 def test_pitch_angle_grid_72(g72):
