@@ -1,0 +1,216 @@
+!  ramgpu_mod.f90 -- ISO_C_BINDING interface to the C ABI in include/ramgpu.h.
+!
+!  A Fortran host (the reference's ram_run, src/ModRamRun.f90:16-259) uses
+!  this module to drive the MI355X split step.  Arrays are passed as they are
+!  stored in the reference's modules (Fortran order), sizes by value, one
+!  opaque context pointer per problem.  Every function returns the C status
+!  (0 = RAMGPU_OK); ramgpu_error() returns the last message as a Fortran
+!  string.
+module ramgpu_mod
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: RAMGPU_OK = 0, RAMGPU_EINVAL = 1, RAMGPU_ENOMEM = 2, &
+       RAMGPU_EHIP = 3, RAMGPU_ESTATE = 4
+  integer(c_int), parameter :: RAMGPU_CEX_NONE = 0, RAMGPU_CEX_H = 1, RAMGPU_CEX_HE = 2, &
+       RAMGPU_CEX_O = 3
+
+  type, bind(C) :: ramgpu_species_out
+     real(c_double) :: setrc, lsdr, lscha, lsatm
+     real(c_double) :: dtdrift_r, dtdrift_p, dtdrift_e, dtdrift_mu
+  end type ramgpu_species_out
+
+  interface
+     function ramgpu_last_error() bind(C, name="ramgpu_last_error") result(p)
+       import :: c_ptr
+       type(c_ptr) :: p
+     end function ramgpu_last_error
+
+     integer(c_int) function ramgpu_create(ctx, ns, nr, nt, ne, npa, device) &
+          bind(C, name="ramgpu_create")
+       import :: c_int, c_ptr
+       type(c_ptr), intent(out) :: ctx
+       integer(c_int), value :: ns, nr, nt, ne, npa, device
+     end function ramgpu_create
+
+     integer(c_int) function ramgpu_destroy(ctx) bind(C, name="ramgpu_destroy")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function ramgpu_destroy
+
+     integer(c_int) function ramgpu_set_species(ctx, s, charge, cex_kind, rmas) &
+          bind(C, name="ramgpu_set_species")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s, charge, cex_kind
+       real(c_double), value :: rmas
+     end function ramgpu_set_species
+
+     integer(c_int) function ramgpu_set_params(ctx, dts, betalim, fraccfl) &
+          bind(C, name="ramgpu_set_params")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: dts, betalim, fraccfl
+     end function ramgpu_set_params
+
+     integer(c_int) function ramgpu_set_grid(ctx, mdr, dphi, conf1, conf2, rlz, ekev, ebnd, &
+          we, de, grel, grbnd, v, mu, wmu, dmu, upa) bind(C, name="ramgpu_set_grid")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: mdr, dphi, conf1, conf2
+       real(c_double), intent(in) :: rlz(*), ekev(*), ebnd(*), we(*), de(*), grel(*), &
+            grbnd(*), v(*), mu(*), wmu(*), dmu(*), upa(*)
+     end function ramgpu_set_grid
+
+     integer(c_int) function ramgpu_set_geometry(ctx, fnhs, fnis, bounhs, bounis, hdns, &
+          didt, dibndt, bnes, dbdt, eir, eip, outside_mgnp) bind(C, name="ramgpu_set_geometry")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: fnhs(*), fnis(*), bounhs(*), bounis(*), hdns(*), &
+            didt(*), dibndt(*), bnes(*), dbdt(*), eir(*), eip(*)
+       integer(c_int), intent(in) :: outside_mgnp(*)
+     end function ramgpu_set_geometry
+
+     integer(c_int) function ramgpu_set_efield(ctx, vt) bind(C, name="ramgpu_set_efield")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: vt(*)
+     end function ramgpu_set_efield
+
+     integer(c_int) function ramgpu_set_kp(ctx, kp, lz, phi) bind(C, name="ramgpu_set_kp")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: kp
+       real(c_double), intent(in) :: lz(*), phi(*)
+     end function ramgpu_set_kp
+
+     integer(c_int) function ramgpu_set_boundary(ctx, fgeos) bind(C, name="ramgpu_set_boundary")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: fgeos(*)
+     end function ramgpu_set_boundary
+
+     integer(c_int) function ramgpu_set_boundary_scale(ctx, scale) &
+          bind(C, name="ramgpu_set_boundary_scale")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: scale(*)
+     end function ramgpu_set_boundary_scale
+
+     integer(c_int) function ramgpu_put_f2(ctx, f2) bind(C, name="ramgpu_put_f2")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: f2(*)
+     end function ramgpu_put_f2
+
+     integer(c_int) function ramgpu_get_f2(ctx, f2) bind(C, name="ramgpu_get_f2")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: f2(*)
+     end function ramgpu_get_f2
+
+     integer(c_int) function ramgpu_set_energy_state(ctx, setrc, lsdr, lscha, lsatm) &
+          bind(C, name="ramgpu_set_energy_state")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: setrc(*), lsdr(*), lscha(*), lsatm(*)
+     end function ramgpu_set_energy_state
+
+     integer(c_int) function ramgpu_get_energy_state(ctx, setrc, lsdr, lscha, lsatm) &
+          bind(C, name="ramgpu_get_energy_state")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: setrc(*), lsdr(*), lscha(*), lsatm(*)
+     end function ramgpu_get_energy_state
+
+     integer(c_int) function ramgpu_driftpara(ctx, s) bind(C, name="ramgpu_driftpara")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_driftpara
+
+     integer(c_int) function ramgpu_driftr(ctx, s, dtdrift) bind(C, name="ramgpu_driftr")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: dtdrift
+     end function ramgpu_driftr
+
+     integer(c_int) function ramgpu_driftp(ctx, s, dtdrift) bind(C, name="ramgpu_driftp")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: dtdrift
+     end function ramgpu_driftp
+
+     integer(c_int) function ramgpu_drifte(ctx, s, dtdrift) bind(C, name="ramgpu_drifte")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: dtdrift
+     end function ramgpu_drifte
+
+     integer(c_int) function ramgpu_driftmu(ctx, s, dtdrift) bind(C, name="ramgpu_driftmu")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: dtdrift
+     end function ramgpu_driftmu
+
+     integer(c_int) function ramgpu_driftend(ctx) bind(C, name="ramgpu_driftend")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function ramgpu_driftend
+
+     integer(c_int) function ramgpu_cepara(ctx, s) bind(C, name="ramgpu_cepara")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_cepara
+
+     integer(c_int) function ramgpu_charexchange(ctx, s) bind(C, name="ramgpu_charexchange")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_charexchange
+
+     integer(c_int) function ramgpu_atmol(ctx, s) bind(C, name="ramgpu_atmol")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_atmol
+
+     integer(c_int) function ramgpu_sumrc(ctx, s, setrc, elorc) bind(C, name="ramgpu_sumrc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: setrc, elorc
+     end function ramgpu_sumrc
+
+     integer(c_int) function ramgpu_ram_run(ctx, nsteps, out) bind(C, name="ramgpu_ram_run")
+       import :: c_int, c_ptr, ramgpu_species_out
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nsteps
+       type(ramgpu_species_out), intent(out) :: out(*)
+     end function ramgpu_ram_run
+  end interface
+
+contains
+
+  function ramgpu_error() result(msg)
+    character(len=:), allocatable :: msg
+    character(kind=c_char), pointer :: p(:)
+    integer :: n
+    type(c_ptr) :: cp
+    cp = ramgpu_last_error()
+    call c_f_pointer(cp, p, [4096])
+    n = 0
+    do while (n < 4096)
+       if (p(n + 1) == c_null_char) exit
+       n = n + 1
+    end do
+    allocate(character(len=n) :: msg)
+    msg = transfer(p(1:n), msg)
+  end function ramgpu_error
+
+end module ramgpu_mod

@@ -50,5 +50,27 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+FORTRAN = ROOT / "fortran"
+FDRIVER = FORTRAN / "ramgpu_driver"
+
+
+def build_fortran(verbose: bool = False) -> Path:
+    """The ISO_C_BINDING module and the example Fortran host program, linked
+    against libramgpu.so (rpath to its directory)."""
+    fc = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
+    obj = FORTRAN / "_obj"
+    obj.mkdir(exist_ok=True)
+    cmd = [fc, "-O2", f"-module-dir", str(obj), str(FORTRAN / "ramgpu_mod.f90"),
+           str(FORTRAN / "ramgpu_driver.f90"), "-o", str(FDRIVER), f"-L{PKG}", "-lramgpu",
+           f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../ramgpu"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(obj))
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed ({r.returncode}):\n{r.stderr}")
+    return FDRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
+    print(build_fortran(verbose="-v" in sys.argv))

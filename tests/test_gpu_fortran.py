@@ -1,0 +1,59 @@
+"""The Fortran host path: ram-scb_amd/fortran/ramgpu_driver (built from
+ramgpu_mod.f90, the ISO_C_BINDING interface) runs split steps on the GPU from
+Fortran-ordered module arrays; the result must match the oracle."""
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import max_rel_err, random_problem, rel_err
+from oracle.oracle import Oracle
+from ramgpu import build, inputs
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not gpu_available(), reason="no GPU")]
+
+
+def write_bundle(path, p, nsteps):
+    g, geo = p.grid, p.geom
+    with open(path, "wb") as f:
+        np.array([g.ns, g.nr, g.nt, g.ne, g.npa, nsteps], dtype=np.int32).tofile(f)
+        np.array([p.dts, p.betalim, p.fraccfl, g.mdr, g.dphi, g.conf1, g.conf2]).tofile(f)
+        np.asarray(g.charge, dtype=np.int32).tofile(f)
+        np.asarray(g.cex_kind, dtype=np.int32).tofile(f)
+        for a in (g.rmas, g.rlz, g.ekev, g.ebnd, g.we, g.de, g.grel, g.grbnd, g.v, g.mu,
+                  g.wmu, g.dmu, g.upa, geo.fnhs, geo.fnis, geo.bounhs, geo.bounis, geo.hdns,
+                  geo.didt, geo.dibndt, geo.bnes, geo.dbdt, p.vt, geo.eir, geo.eip):
+            np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
+        np.asarray(geo.outside_mgnp, dtype=np.int32).ravel(order="F").tofile(f)
+        for a in (p.fgeos, p.f2):
+            np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
+
+
+@pytest.mark.parametrize("which", ["dipole4", "random2"])
+def test_fortran_host_matches_oracle(tmp_path, which):
+    if which == "dipole4":
+        p = inputs.make_problem(nr=10, nt=13, ne=12, npa=16, species=inputs.SPECIES4,
+                                composition=inputs.COMPOSITION4, kp=4.0, seed=8)
+    else:
+        p = random_problem(seed=7, spiky=True)
+    nsteps = 2
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_bundle(fin, p, nsteps)
+    print(f"fortran driver: {which} {p.grid.ns}x{p.grid.nr}x{p.grid.nt}x{p.grid.ne}x{p.grid.npa}",
+          flush=True)
+    r = subprocess.run([str(build.FDRIVER), str(fin), str(fout)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    g = p.grid
+    n = g.ns * g.nr * g.nt * g.ne * g.npa
+    raw = np.fromfile(fout, dtype=np.float64)
+    f2 = raw[:n].reshape((g.ns, g.nr, g.nt, g.ne, g.npa), order="F")
+    energy = raw[n:].reshape(4, g.ns)
+    o = Oracle(p)
+    for _ in range(nsteps):
+        o.ram_run(4)
+    assert rel_err(f2, o.f2) <= 1e-12 and max_rel_err(f2, o.f2) <= 1e-12
+    for row, name in enumerate(("setrc", "lsdr", "lscha", "lsatm")):
+        ref = o.arr(name)
+        assert np.all(np.abs(energy[row] - ref) <= 1e-12 * np.abs(o.arr("setrc"))), name
