@@ -156,17 +156,98 @@ int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
  * library's stream; ramgpu_kernel_times returns, per kernel kind, the summed
  * device time and the launch count since the last query, then resets. */
 #define RAMGPU_KERNEL_PREPARE 0  /* species tables, Courant terms, ghost carry */
-#define RAMGPU_KERNEL_IJ 1       /* radial / azimuthal sweeps                 */
-#define RAMGPU_KERNEL_KL 2       /* energy / pitch-angle sweeps, losses       */
+#define RAMGPU_KERNEL_R 1        /* radial sweeps (DRIFTR)                    */
+#define RAMGPU_KERNEL_PLANE 2    /* energy / pitch-angle sweeps, losses, SUMRC */
 #define RAMGPU_KERNEL_FINALIZE 3 /* DtDrift* and SUMRC bookkeeping            */
 #define RAMGPU_KERNEL_VS 4       /* Volland-Stern VT from Kp                  */
-#define RAMGPU_KERNEL_KINDS 5
+#define RAMGPU_KERNEL_P 5        /* azimuthal sweeps (DRIFTP)                 */
+#define RAMGPU_KERNEL_KINDS 6
 int ramgpu_kernel_timing(ramgpu_ctx *ctx, int enable);
 /* Measurement support: a plain device copy of `bytes` (read + write) run
  * `reps` times on device `device`; gbps receives the achieved read+write
  * bandwidth.  Used as the measured HBM peak and to calibrate counters. */
 int ramgpu_copy_probe(int device, size_t bytes, int reps, double *gbps);
 int ramgpu_kernel_times(ramgpu_ctx *ctx, double *total_ms, int *launches);
+
+/* ---- reverse-mode differentiation ------------------------------------ */
+
+/* The split step is piecewise linear in (F2, FGEOS): every Courant number,
+ * CHARGE and ATLOS factor depends on fields, geometry and grid only, so the
+ * limiter branch, upwind side and clamp F2<0 -> 1e-15 of each cell are fixed
+ * by the operator's input.  The backward entry points are exact
+ * vector-Jacobian products at that input.  Subgradient conventions: a
+ * clamped cell passes no derivative (a cell exactly at 0 is not clamped);
+ * at a kink of the limiter max(min(BetaLim*R,1), min(R,BetaLim)) a tie of
+ * min or max takes the derivative of its first argument; R <= 0 and
+ * |F(I+1)-F(I)| <= 1e-27 take the upwind value.  (No counterpart in the
+ * reference, which has no derivatives: these serve data assimilation.)
+ *
+ * Operator pairs: ramgpu_<op> (forward, above) and ramgpu_<op>_bwd
+ * (backward).  The backward is evaluated at the context's current F2, which
+ * must be the operator's input; F2 is not changed.  adj (NS,NR,NT,NE,NPA)
+ * holds dL/d(output) on entry and dL/d(input) on return; fgeos_adj
+ * (NS,NT,NE,NPA, may be NULL) accumulates dL/dFGEOS (DRIFTR only). */
+int ramgpu_driftr_bwd(ramgpu_ctx *ctx, int s, double *adj, double *fgeos_adj);
+int ramgpu_driftp_bwd(ramgpu_ctx *ctx, int s, double *adj);
+int ramgpu_drifte_bwd(ramgpu_ctx *ctx, int s, double *adj);
+int ramgpu_driftmu_bwd(ramgpu_ctx *ctx, int s, double *adj);
+int ramgpu_charexchange_bwd(ramgpu_ctx *ctx, int s, double *adj);
+int ramgpu_atmol_bwd(ramgpu_ctx *ctx, int s, double *adj);
+/* Backward of one ram_run step at the current F2 (the step's input; F2 and
+ * the SUMRC state are left unchanged). */
+int ramgpu_ram_run_bwd(ramgpu_ctx *ctx, double *adj, double *fgeos_adj);
+
+/* Gradient of a virtual-satellite flux misfit over an nsteps ram_run run
+ * starting from the current F2, with the schedule of ramgpu_ram_run_schedule
+ * (kp, bscale may be NULL):
+ *   L = sum_n sum_p ( log(omega_p(F2 after step n)) - target[n*npts+p] )^2,
+ *   omega_p = sum_L w[p*NPA + L-1] * F2(S_p,I_p,J_p,K_p,L),
+ * with pts[4p..4p+3] = S_p,I_p,J_p,K_p (1-based).  (omega is the
+ * omnidirectional flux of src/ModRamRun.f90:214 and src/ModRamSats.f90:661
+ * when w = 4*pi*WMU/(FFACTOR*FNHS).)  loss receives L, grad_f2
+ * (NS,NR,NT,NE,NPA) dL/dF2 at the start, grad_fgeos (may be NULL)
+ * [nsteps](NS,NT,NE,NPA) dL/dFGEOS of each step.  Checkpoints every
+ * ckpt_every steps; each segment is recomputed once with every operator's
+ * input on a tape.  The context is left in the state after the run. */
+int ramgpu_adjoint_run(ramgpu_ctx *ctx, int nsteps, int ckpt_every, const double *kp,
+                       const double *bscale, int npts, const int *pts, const double *w,
+                       const double *target, double *loss, double *grad_f2, double *grad_fgeos);
+
+/* ---- ensembles ------------------------------------------------------- */
+
+/* An ensemble of independent problems stepped together (no counterpart in the
+ * reference, which runs one problem per process: it serves ensemble
+ * forecasts and ensemble data assimilation).  Members are contexts created
+ * on one device, each with its own grid sizes, species, fields and F2; the
+ * ensemble does not own them and must be destroyed before them.  Every kernel
+ * of a step is one launch over all members, and a step is one captured HIP
+ * graph, replayed per step.  A member's results are bit-identical to those
+ * of stepping it alone with ramgpu_ram_run / ramgpu_ram_run_schedule. */
+typedef struct ramgpu_batch ramgpu_batch;
+int ramgpu_batch_create(ramgpu_batch **batch, ramgpu_ctx *const *members, int nmembers);
+int ramgpu_batch_destroy(ramgpu_batch *batch);
+/* nsteps ram_run steps of every member; out (may be NULL) receives the
+ * members' ramgpu_species_out records, member after member (NS_m each). */
+int ramgpu_batch_ram_run(ramgpu_batch *batch, int nsteps, ramgpu_species_out *out);
+/* With per-member schedules: before step n member m gets VT for
+ * kp[n*M + m] (as ramgpu_set_kp) and boundary scales
+ * bscale[n*sum(NS) + off_m + s-1], off_m = NS_0+..+NS_(m-1) (may be NULL:
+ * the members' current scales are kept). */
+int ramgpu_batch_ram_run_schedule(ramgpu_batch *batch, int nsteps, const double *kp,
+                                  const double *bscale, ramgpu_species_out *out);
+/* Device time (ms) of the last ensemble run, HIP events on its stream. */
+int ramgpu_batch_last_run_ms(ramgpu_batch *batch, double *ms);
+/* 1 (default): replay a captured graph per step; 0: launch eagerly. */
+int ramgpu_batch_use_graph(ramgpu_batch *batch, int enable);
+/* Autotuner: times every candidate launch grid of the R, P and plane
+ * kernels (one block per work unit, or 1/2/4/8 resident blocks per CU
+ * looping over the units) with `reps` timed launches each and keeps the
+ * fastest.  The members' state is restored afterwards.  ms_best (may be
+ * NULL) receives the best per-launch times of R, P and plane. */
+int ramgpu_batch_autotune(ramgpu_batch *batch, int reps, double *ms_best /* [3] */);
+/* Launch grid caps of R, P, plane (0 = one block per work unit). */
+int ramgpu_batch_get_shapes(ramgpu_batch *batch, int *caps /* [3] */);
+int ramgpu_batch_set_shapes(ramgpu_batch *batch, const int *caps /* [3] */);
 
 #ifdef __cplusplus
 }

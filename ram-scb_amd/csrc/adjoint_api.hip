@@ -1,0 +1,344 @@
+// adjoint_api.hip -- C ABI of reverse-mode differentiation (include/ramgpu.h).
+//
+// Operator level: ramgpu_<op>_bwd is the backward of the autograd-style pair
+// (ramgpu_<op>, ramgpu_<op>_bwd): evaluated at the context's current F2 (the
+// operator's input), it maps dL/d(output) to dL/d(input) in place and
+// accumulates dL/dFGEOS (DRIFTR).  Step level: ramgpu_ram_run_bwd, the same
+// for one whole ram_run step.  Run level: ramgpu_adjoint_run, the gradient of
+// a virtual-satellite flux misfit over a multi-step run with respect to the
+// initial F2 and to the boundary FGEOS of every step, with checkpoint-every-k
+// recomputation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace ramgpu;
+using namespace ramgpu_host;
+
+namespace {
+
+// tape slots: the inputs of the operators of one ram_run step, in order
+enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
+
+size_t nf2(const ramgpu_ctx *c) { return (size_t)c->ns * c->nr * c->nt * c->ne * c->npa; }
+size_t nfg(const ramgpu_ctx *c) { return (size_t)c->ns * c->nt * c->ne * c->npa; }
+
+struct DevBuf {
+  std::vector<void *> p;
+  ~DevBuf() {
+    for (void *q : p) (void)hipFree(q);
+  }
+  template <class T>
+  int alloc(T **out, size_t n) {
+    void *q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess)
+      return fail(RAMGPU_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+    p.push_back(q);
+    *out = (T *)q;
+    return RAMGPU_OK;
+  }
+};
+
+int fill(ramgpu_ctx *c, double *y, double v, size_t n) {
+  hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, y, v, n);
+  return launch_check("k_fill");
+}
+
+int copy_d2d(ramgpu_ctx *c, double *dst, const double *src, size_t n) {
+  HIP_TRY(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  return RAMGPU_OK;
+}
+
+int set_bwd_lds_limit(size_t limit) {
+  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E>, (const void *)k_plane_bwd<SEQ_MU>,
+                       (const void *)k_plane_bwd<SEQ_CEX>, (const void *)k_plane_bwd<SEQ_ATM>,
+                       (const void *)k_plane_bwd<SEQ_LOSS>};
+  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
+  return RAMGPU_OK;
+}
+
+size_t bwd_lds(const ramgpu_ctx *c) {
+  return plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double);
+}
+
+// backward kernels over species s0..s0+nsp-1
+int bwd_r(ramgpu_ctx *c, int final_sweep, const double *x, double *adj, double *bbar, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
+  tbeg(c, RAMGPU_KERNEL_R);
+  if (final_sweep)
+    hipLaunchKernelGGL(k_r_bwd<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj, bbar);
+  else
+    hipLaunchKernelGGL(k_r_bwd<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj, bbar);
+  tend(c);
+  return launch_check("k_r_bwd");
+}
+
+int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, blocks_p(c->t));
+  tbeg(c, RAMGPU_KERNEL_P);
+  hipLaunchKernelGGL(k_p_bwd, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj);
+  tend(c);
+  return launch_check("k_p_bwd");
+}
+
+int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int nsp) {
+  if (bwd_lds(c) > c->lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
+  set_bwd_lds_limit(c->lds_limit);
+  const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
+  const size_t lds = bwd_lds(c);
+  tbeg(c, RAMGPU_KERNEL_PLANE);
+  switch (seq) {
+    case SEQ_E: hipLaunchKernelGGL(k_plane_bwd<SEQ_E>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
+    case SEQ_MU: hipLaunchKernelGGL(k_plane_bwd<SEQ_MU>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
+    case SEQ_CEX: hipLaunchKernelGGL(k_plane_bwd<SEQ_CEX>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
+    case SEQ_ATM: hipLaunchKernelGGL(k_plane_bwd<SEQ_ATM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
+    default: hipLaunchKernelGGL(k_plane_bwd<SEQ_LOSS>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
+  }
+  tend(c);
+  return launch_check("k_plane_bwd");
+}
+
+// One ram_run step (all species) with the input of every operator saved to
+// tape[slot*nf2]: the fused step's kernels run one operator at a time, with
+// the same arithmetic, so the state, the SUMRC bookkeeping and the CFL
+// limits are those of step_once.
+int taped_step(ramgpu_ctx *c, double *tape) {
+  int rc;
+  if ((rc = prepare(c))) return rc;
+  const int rec = c->cfl_valid ? 0 : 1, ns = c->ns;
+  const size_t n = nf2(c);
+  double *f = c->t.f2;
+  auto save = [&](int slot) { return copy_d2d(c, tape + (size_t)slot * n, f, n); };
+  if ((rc = save(T_R1)) || (rc = run_r(c, rec, 0, ns)) || (rc = save(T_P1)) || (rc = run_p(c, rec, 0, ns)) ||
+      (rc = save(T_E1)) || (rc = run_plane(c, SEQ_E, rec, 0, ns)) || (rc = save(T_MU1)) ||
+      (rc = run_plane(c, SEQ_MU, rec, 0, ns)) || (rc = save(T_LOSS)) || (rc = run_plane(c, SEQ_LOSS, 0, 0, ns)) ||
+      (rc = save(T_MU2)) || (rc = run_plane(c, SEQ_MU, 0, 0, ns)) || (rc = save(T_E2)) ||
+      (rc = run_plane(c, SEQ_E, 0, 0, ns)) || (rc = save(T_P2)) || (rc = run_p(c, 0, 0, ns)) ||
+      (rc = save(T_R2)) || (rc = run_r(c, 2, 0, ns)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+    return rc;
+  c->cfl_valid = true;
+  return RAMGPU_OK;
+}
+
+// Adjoint of one taped step: adj (dL/d output) -> dL/d input, in place;
+// bbar (may be NULL) accumulates dL/dFGEOS of the step.  The tables must be
+// those of the step (its VT and boundary scale).
+int step_bwd(ramgpu_ctx *c, const double *tape, double *adj, double *bbar) {
+  const size_t n = nf2(c);
+  const int ns = c->ns;
+  auto x = [&](int slot) { return tape + (size_t)slot * n; };
+  int rc;
+  if ((rc = bwd_r(c, 1, x(T_R2), adj, bbar, 0, ns)) || (rc = bwd_p(c, x(T_P2), adj, 0, ns)) ||
+      (rc = bwd_plane(c, SEQ_E, x(T_E2), adj, 0, ns)) || (rc = bwd_plane(c, SEQ_MU, x(T_MU2), adj, 0, ns)) ||
+      (rc = bwd_plane(c, SEQ_LOSS, x(T_LOSS), adj, 0, ns)) || (rc = bwd_plane(c, SEQ_MU, x(T_MU1), adj, 0, ns)) ||
+      (rc = bwd_plane(c, SEQ_E, x(T_E1), adj, 0, ns)) || (rc = bwd_p(c, x(T_P1), adj, 0, ns)) ||
+      (rc = bwd_r(c, 0, x(T_R1), adj, bbar, 0, ns)))
+    return rc;
+  return RAMGPU_OK;
+}
+
+// host Fortran-layout F2 <-> device-layout buffer (through the context's stage)
+int upload_f2(ramgpu_ctx *c, double *dst, const double *host) {
+  const size_t n = nf2(c);
+  HIP_TRY(hipMemcpyAsync(c->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_f2_in, dim3(1024), dim3(256), 0, c->stream, dst, (const double *)c->d_stage, c->ns, c->nr,
+                     c->nt, c->ne, c->npa);
+  return launch_check("k_f2_in");
+}
+
+int download_f2(ramgpu_ctx *c, double *host, const double *src) {
+  const size_t n = nf2(c);
+  hipLaunchKernelGGL(k_f2_out, dim3(1024), dim3(256), 0, c->stream, c->d_stage, src, c->ns, c->nr, c->nt, c->ne,
+                     c->npa);
+  int rc = launch_check("k_f2_out");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+// Operator-level backward at the current F2: which 0 R, 1 P, 2 E, 3 Mu,
+// 4 CEX, 5 ATMOL.
+int op_bwd(ramgpu_ctx *c, int s, int which, double *adj, double *fgeos_adj) {
+  int rc = check_ctx(c);
+  if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
+  if (!adj) return fail(RAMGPU_EINVAL, "adj is null");
+  DevBuf b;
+  double *da = nullptr, *db = nullptr;
+  if ((rc = b.alloc(&da, nf2(c)))) return rc;
+  if ((rc = upload_f2(c, da, adj))) return rc;
+  if (fgeos_adj && which == 0) {
+    if ((rc = b.alloc(&db, nfg(c)))) return rc;
+    HIP_TRY(hipMemcpyAsync(db, fgeos_adj, nfg(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  const double *x = c->t.f2;
+  switch (which) {
+    case 0: rc = bwd_r(c, 0, x, da, db, s - 1, 1); break;
+    case 1: rc = bwd_p(c, x, da, s - 1, 1); break;
+    case 2: rc = bwd_plane(c, SEQ_E, x, da, s - 1, 1); break;
+    case 3: rc = bwd_plane(c, SEQ_MU, x, da, s - 1, 1); break;
+    case 4: rc = bwd_plane(c, SEQ_CEX, x, da, s - 1, 1); break;
+    default: rc = bwd_plane(c, SEQ_ATM, x, da, s - 1, 1); break;
+  }
+  if (rc || (rc = download_f2(c, adj, da))) return rc;
+  if (db) {
+    HIP_TRY(hipMemcpyAsync(fgeos_adj, db, nfg(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return RAMGPU_OK;
+}
+
+// The context's per-step inputs of a schedule (as ramgpu_ram_run_schedule).
+int set_step_inputs(ramgpu_ctx *c, const double *d_kp, const double *d_bs, int n) {
+  if (d_kp) {
+    hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
+                       (const double *)c->d_phi, d_kp + n, 0.0, c->nr + 1, c->nt);
+    int rc = launch_check("k_vs");
+    if (rc) return rc;
+    c->efield_dirty = true;
+  }
+  if (d_bs) {
+    HIP_TRY(hipMemcpyAsync(c->d_bscale, d_bs + (size_t)n * c->ns, c->ns * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+    c->ghost_dirty = true;
+  }
+  return RAMGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ramgpu_driftr_bwd(ramgpu_ctx *c, int s, double *adj, double *fgeos_adj) { return op_bwd(c, s, 0, adj, fgeos_adj); }
+int ramgpu_driftp_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 1, adj, nullptr); }
+int ramgpu_drifte_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 2, adj, nullptr); }
+int ramgpu_driftmu_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 3, adj, nullptr); }
+int ramgpu_charexchange_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 4, adj, nullptr); }
+int ramgpu_atmol_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 5, adj, nullptr); }
+
+int ramgpu_ram_run_bwd(ramgpu_ctx *c, double *adj, double *fgeos_adj) {
+  int rc = check_ctx(c);
+  if (rc || (rc = ready(c)) || (rc = prepare(c))) return rc;
+  if (!adj) return fail(RAMGPU_EINVAL, "adj is null");
+  const size_t n = nf2(c);
+  DevBuf b;
+  double *tape = nullptr, *da = nullptr, *db = nullptr, *save_f2 = nullptr, *save_e = nullptr;
+  if ((rc = b.alloc(&tape, T_N * n)) || (rc = b.alloc(&da, n)) || (rc = b.alloc(&save_f2, n)) ||
+      (rc = b.alloc(&save_e, (size_t)c->ns * 4)))
+    return rc;
+  if (fgeos_adj) {
+    if ((rc = b.alloc(&db, nfg(c)))) return rc;
+    HIP_TRY(hipMemcpyAsync(db, fgeos_adj, nfg(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  // the step runs on the state and is undone afterwards
+  if ((rc = copy_d2d(c, save_f2, c->t.f2, n)) || (rc = copy_d2d(c, save_e, c->t.energy, (size_t)c->ns * 4)) ||
+      (rc = taped_step(c, tape)) || (rc = copy_d2d(c, c->t.f2, save_f2, n)) ||
+      (rc = copy_d2d(c, c->t.energy, save_e, (size_t)c->ns * 4)) || (rc = upload_f2(c, da, adj)) ||
+      (rc = step_bwd(c, tape, da, db)) || (rc = download_f2(c, adj, da)))
+    return rc;
+  if (db) {
+    HIP_TRY(hipMemcpyAsync(fgeos_adj, db, nfg(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return RAMGPU_OK;
+}
+
+int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *kp, const double *bscale,
+                       int npts, const int *pts, const double *w, const double *target, double *loss,
+                       double *grad_f2, double *grad_fgeos) {
+  int rc = check_ctx(c);
+  if (rc || (rc = ready(c))) return rc;
+  if (nsteps < 1 || ckpt_every < 1) return fail(RAMGPU_EINVAL, "nsteps and ckpt_every must be >= 1");
+  if (npts < 1 || !pts || !w || !target) return fail(RAMGPU_EINVAL, "misfit points, weights and targets required");
+  if (kp && !c->have_lzphi) return fail(RAMGPU_ESTATE, "ramgpu_set_kp(lz, phi) must precede a Kp schedule");
+  for (int p = 0; p < npts; ++p) {
+    const int *q = pts + 4 * p;
+    if (q[0] < 1 || q[0] > c->ns || q[1] < 1 || q[1] > c->nr || q[2] < 1 || q[2] > c->nt || q[3] < 1 ||
+        q[3] > c->ne)
+      return fail(RAMGPU_EINVAL, "misfit point %d outside the grid", p);
+  }
+  const size_t n = nf2(c), ng = nfg(c);
+  const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
+  const int seglen = std::min(ckpt_every, nsteps);
+  DevBuf b;
+  double *ckpt, *tape, *lam, *bbar = nullptr, *dw, *dtarget, *lossv, *scratch_loss, *final_f2, *final_e;
+  double *d_kp = nullptr, *d_bs = nullptr;
+  int *dpts;
+  if ((rc = b.alloc(&ckpt, (size_t)nseg * n)) || (rc = b.alloc(&tape, (size_t)seglen * T_N * n)) ||
+      (rc = b.alloc(&lam, n)) || (rc = b.alloc(&dw, (size_t)npts * c->npa)) ||
+      (rc = b.alloc(&dtarget, (size_t)nsteps * npts)) || (rc = b.alloc(&lossv, (size_t)nsteps * npts)) ||
+      (rc = b.alloc(&scratch_loss, npts)) || (rc = b.alloc(&final_f2, n)) ||
+      (rc = b.alloc(&final_e, (size_t)c->ns * 4)) || (rc = b.alloc(&dpts, (size_t)4 * npts)))
+    return rc;
+  if (grad_fgeos && (rc = b.alloc(&bbar, (size_t)nsteps * ng))) return rc;
+  if (kp) {
+    if ((rc = b.alloc(&d_kp, nsteps))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_kp, kp, nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  if (bscale) {
+    if ((rc = b.alloc(&d_bs, (size_t)nsteps * c->ns))) return rc;
+    HIP_TRY(hipMemcpyAsync(d_bs, bscale, (size_t)nsteps * c->ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  }
+  HIP_TRY(hipMemcpyAsync(dpts, pts, (size_t)4 * npts * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dw, w, (size_t)npts * c->npa * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(dtarget, target, (size_t)nsteps * npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int mb = (npts + 63) / 64;
+  auto misfit = [&](const double *state, int step, double *lout, double *lm) {
+    hipLaunchKernelGGL(k_misfit, dim3(mb), dim3(64), 0, c->stream, state, c->nr, c->nt, c->ne, c->npa, npts,
+                       (const int *)dpts, (const double *)dw, (const double *)(dtarget + (size_t)step * npts), lout,
+                       lm);
+    return launch_check("k_misfit");
+  };
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  // forward, keeping a checkpoint at the start of every segment
+  for (int s = 0; s < nsteps; ++s) {
+    if (s % ckpt_every == 0 && (rc = copy_d2d(c, ckpt + (size_t)(s / ckpt_every) * n, c->t.f2, n))) return rc;
+    if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = step_once(c)) ||
+        (rc = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr)))
+      return rc;
+  }
+  if ((rc = copy_d2d(c, final_f2, c->t.f2, n)) || (rc = copy_d2d(c, final_e, c->t.energy, (size_t)c->ns * 4)))
+    return rc;
+  // backward, segment by segment: recompute the segment with its tape, then
+  // walk its steps in reverse
+  if ((rc = fill(c, lam, 0.0, n))) return rc;
+  if (bbar && (rc = fill(c, bbar, 0.0, (size_t)nsteps * ng))) return rc;
+  for (int g = nseg - 1; g >= 0; --g) {
+    const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
+    if ((rc = copy_d2d(c, c->t.f2, ckpt + (size_t)g * n, n))) return rc;
+    for (int s = n0; s < n1; ++s)
+      if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n)))
+        return rc;
+    for (int s = n1 - 1; s >= n0; --s) {
+      const double *y = (s == n1 - 1) ? c->t.f2 : tape + (size_t)(s + 1 - n0) * T_N * n + (size_t)T_R1 * n;
+      if ((rc = misfit(y, s, scratch_loss, lam)) || (rc = set_step_inputs(c, d_kp, d_bs, s)) ||
+          (rc = prepare(c)) ||
+          (rc = step_bwd(c, tape + (size_t)(s - n0) * T_N * n, lam, bbar ? bbar + (size_t)s * ng : nullptr)))
+        return rc;
+    }
+  }
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  // results; the context is left in the state after the run
+  if ((rc = copy_d2d(c, c->t.f2, final_f2, n)) || (rc = copy_d2d(c, c->t.energy, final_e, (size_t)c->ns * 4)))
+    return rc;
+  if (nsteps > 0 && (rc = set_step_inputs(c, d_kp, d_bs, nsteps - 1))) return rc;
+  std::vector<double> lv((size_t)nsteps * npts);
+  HIP_TRY(hipMemcpyAsync(lv.data(), lossv, lv.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (loss) {
+    double tot = 0.0;
+    for (double v : lv) tot += v;
+    *loss = tot;
+  }
+  if (grad_f2 && (rc = download_f2(c, grad_f2, lam))) return rc;
+  if (grad_fgeos) {
+    HIP_TRY(hipMemcpyAsync(grad_fgeos, bbar, (size_t)nsteps * ng * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return RAMGPU_OK;
+}
+
+}  // extern "C"

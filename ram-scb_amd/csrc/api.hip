@@ -14,30 +14,15 @@
 #include <string>
 #include <vector>
 
-#include "ramgpu.h"
-#include "ramgpu_dev.h"
-
-namespace ramgpu {
-__global__ void k_species(Tab t);
-__global__ void k_coef_ij(Tab t);
-__global__ void k_coef(Tab t);
-__global__ void k_ghost(Tab t);
-__global__ void k_ij(Tab t, Program prog, int s0, int ntiles, int kws);
-__global__ void k_kl(Tab t, Program prog, int s0);
-__global__ void k_finalize(Tab t, int cflmask, int summode, int s0, int nblk_kl, int nblk_ij,
-                           double *elorc_out);
-__global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
-                     double kp_val, int nr1, int nt);
-__global__ void k_copy(double2 *dst, const double2 *src, size_t n2);
-__global__ void k_f2_in(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
-__global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
-}  // namespace ramgpu
+#include "ctx.h"
 
 using namespace ramgpu;
 
-namespace {
+namespace ramgpu_host {
 
 thread_local std::string g_err;
+
+const std::string &last_error() { return g_err; }
 
 int fail(int code, const char *fmt, ...) {
   char buf[512];
@@ -49,48 +34,12 @@ int fail(int code, const char *fmt, ...) {
   return code;
 }
 
-#define HIP_TRY(x)                                                                      \
-  do {                                                                                  \
-    hipError_t e_ = (x);                                                                \
-    if (e_ != hipSuccess) return fail(RAMGPU_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace ramgpu_host
 
-}  // namespace
+using namespace ramgpu_host;
 
-struct ramgpu_ctx {
-  int device = 0;
-  int ns = 0, nr = 0, nt = 0, ne = 0, npa = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double last_ms = 0.0;
-  size_t lds_limit = 65536;
-  Tab t{};
-  std::vector<void *> allocs;
-  // device buffers with host-visible handles
-  double *d_rlz, *d_ekev, *d_ebnd, *d_we, *d_de, *d_grel, *d_grbnd, *d_v, *d_mu, *d_wmu, *d_dmu;
-  int *d_upa;
-  double *d_fnhs, *d_fnis, *d_bounhs, *d_bounis, *d_hdnsf, *d_didt, *d_dibndt;
-  double *d_bnes, *d_dbdt, *d_vt, *d_eir, *d_eip, *d_fgeos, *d_bscale;
-  int *d_mgnp;
-  double *d_lz, *d_phi, *d_kp;
-  double *d_stage;  // Fortran-layout staging buffer for F2 transfers
-  double *d_elorc;
-  bool have_grid = false, have_geom = false, have_vt = false, have_bnd = false, have_f2 = false;
-  bool have_species[kMaxSpecies] = {};
-  bool have_lzphi = false;
-  bool tables_dirty = true, ghost_dirty = true;
-  bool efield_dirty = true;  // only the VT-dependent per-(I,J) terms are stale
-  bool cfl_valid = false;  // DtDrift* current for the present tables
-  // opt-in per-kernel timing: event pairs around every launch, by kind
-  bool timing = false;
-  std::vector<hipEvent_t> tev;  // 2 per recorded launch
-  std::vector<int> tkind;
-  size_t tused = 0;
-  int kws = 3;            // IJ tile width 2^kws (consecutive (L,K) positions)
-  int ij_threads = 256, kl_threads = 512;
-};
 
-namespace {
+namespace ramgpu_host {
 
 template <class T>
 int dev_alloc(ramgpu_ctx *c, T **p, size_t n) {
@@ -154,12 +103,27 @@ void tend(ramgpu_ctx *c) {
   ++c->tused;
 }
 
-size_t ij_lds(const ramgpu_ctx *c) {
-  const size_t kw = (size_t)1 << c->kws;
-  return (2 * (size_t)c->nr * c->nt * kw + 16) * sizeof(double) + (size_t)c->nt * kw * sizeof(int);
+// The device copy of the descriptor follows every change of the host image
+// (species, parameters, grid scalars); uploaded synchronously, before any
+// launch that reads it.
+int sync_tab(ramgpu_ctx *c) {
+  if (!c->tab_dirty) return RAMGPU_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(c->d_tab, &c->t, sizeof(Tab), hipMemcpyHostToDevice));
+  c->tab_dirty = false;
+  return RAMGPU_OK;
 }
-size_t kl_lds(const ramgpu_ctx *c) {
-  return (3 * (size_t)c->ne * c->npa + 16 + c->ne + c->npa) * sizeof(double);
+
+// species s0..s0+nsp-1 (0-based) of this context, nb blocks each
+Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb) {
+  Launch L{};
+  L.tabs = c->d_tab;
+  L.jobs = nullptr;
+  L.vmap = nullptr;
+  L.nb = nb;
+  L.s0 = s0;
+  L.nv = nsp * nb;
+  return L;
 }
 
 // Rebuild derived tables (species tables, Courant-number terms) and the
@@ -170,11 +134,17 @@ int prepare(ramgpu_ctx *c) {
   Tab &t = c->t;
   if (c->tables_dirty) {
     t.omet = 7.3E-5 * t.dts / t.dphi;  // OME*DTs/DPHI, src/ModRamDrift.f90:239
+    c->tab_dirty = true;
+  }
+  if ((rc = sync_tab(c))) return rc;
+  if (c->tables_dirty) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
-    hipLaunchKernelGGL(k_species, dim3(c->ns), dim3(256), 0, c->stream, t);
+    const Launch ls = launch_of(c, 0, c->ns, 1);
+    hipLaunchKernelGGL(k_species, dim3(ls.nv), dim3(256), 0, c->stream, ls);
     if ((rc = launch_check("k_species"))) return rc;
-    const int bt = ((c->npa + 63) / 64) * 64;
-    hipLaunchKernelGGL(k_coef, dim3(c->nr * c->nt), dim3(bt), 0, c->stream, t);
+    const Launch lc = launch_of(c, 0, 1, c->nr * c->nt);
+    hipLaunchKernelGGL(k_coef, dim3(lc.nv), dim3(((c->npa + 63) / 64) * 64 > 256 ? 256 : ((c->npa + 63) / 64) * 64),
+                       0, c->stream, lc);
     if ((rc = launch_check("k_coef"))) return rc;
     tend(c);
     c->tables_dirty = false;
@@ -182,7 +152,8 @@ int prepare(ramgpu_ctx *c) {
   }
   if (c->efield_dirty) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
-    hipLaunchKernelGGL(k_coef_ij, dim3((c->nr * c->nt + 255) / 256), dim3(256), 0, c->stream, t);
+    const Launch l = launch_of(c, 0, 1, (c->nr * c->nt + 255) / 256);
+    hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), 0, c->stream, l);
     tend(c);
     if ((rc = launch_check("k_coef_ij"))) return rc;
     c->efield_dirty = false;
@@ -191,7 +162,8 @@ int prepare(ramgpu_ctx *c) {
   }
   if (c->ghost_dirty) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
-    hipLaunchKernelGGL(k_ghost, dim3(c->ns), dim3(1024), 0, c->stream, t);
+    const Launch l = launch_of(c, 0, c->ns, 1);
+    hipLaunchKernelGGL(k_ghost, dim3(l.nv), dim3(1024), 0, c->stream, l);
     tend(c);
     if ((rc = launch_check("k_ghost"))) return rc;
     c->ghost_dirty = false;
@@ -199,42 +171,55 @@ int prepare(ramgpu_ctx *c) {
   return RAMGPU_OK;
 }
 
-int ntiles(const ramgpu_ctx *c) {
-  const int kw = 1 << c->kws;
-  return (c->ne * c->npa + kw - 1) / kw;
-}
-
-int run_ij(ramgpu_ctx *c, const Program &p, int s0, int nsp) {
-  const int nt = ntiles(c);
-  tbeg(c, RAMGPU_KERNEL_IJ);
-  hipLaunchKernelGGL(k_ij, dim3(nsp * nt), dim3(c->ij_threads), ij_lds(c), c->stream, c->t, p, s0,
-                     nt, c->kws);
+// mode: 0 sweep, 1 sweep + CFL record, 2 last sweep of ram_run (SUMRC slot 5,
+// seam, mask)
+int run_r(ramgpu_ctx *c, int mode, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
+  tbeg(c, RAMGPU_KERNEL_R);
+  if (mode == 0)
+    hipLaunchKernelGGL(k_r<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+  else if (mode == 1)
+    hipLaunchKernelGGL(k_r<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+  else
+    hipLaunchKernelGGL(k_r<2>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
   tend(c);
-  return launch_check("k_ij");
+  return launch_check("k_r");
 }
 
-int run_kl(ramgpu_ctx *c, const Program &p, int s0, int nsp) {
-  const int nb = nsp * (c->nr - 1) * c->nt;
-  tbeg(c, RAMGPU_KERNEL_KL);
-  hipLaunchKernelGGL(k_kl, dim3(nb), dim3(c->kl_threads), kl_lds(c), c->stream, c->t, p, s0);
+int run_p(ramgpu_ctx *c, int rec, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, blocks_p(c->t));
+  tbeg(c, RAMGPU_KERNEL_P);
+  if (rec)
+    hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+  else
+    hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
   tend(c);
-  return launch_check("k_kl");
+  return launch_check("k_p");
 }
 
-int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp, double *elorc) {
+int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
+  const size_t lds = plane_lds_bytes(c->ne, c->npa);
+  tbeg(c, RAMGPU_KERNEL_PLANE);
+  switch (seq) {
+    case SEQ_E: hipLaunchKernelGGL(k_plane<SEQ_E>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    case SEQ_MU: hipLaunchKernelGGL(k_plane<SEQ_MU>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    case SEQ_CEX: hipLaunchKernelGGL(k_plane<SEQ_CEX>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    case SEQ_ATM: hipLaunchKernelGGL(k_plane<SEQ_ATM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    case SEQ_SUM: hipLaunchKernelGGL(k_plane<SEQ_SUM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    case SEQ_LOSS: hipLaunchKernelGGL(k_plane<SEQ_LOSS>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+    default: hipLaunchKernelGGL(k_plane<SEQ_STEP>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
+  }
+  tend(c);
+  return launch_check("k_plane");
+}
+
+int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp) {
+  const Launch L = launch_of(c, s0, nsp, 1);
   tbeg(c, RAMGPU_KERNEL_FINALIZE);
-  hipLaunchKernelGGL(k_finalize, dim3(nsp), dim3(256), 0, c->stream, c->t, cflmask, summode, s0,
-                     (c->nr - 1) * c->nt, ntiles(c), elorc);
+  hipLaunchKernelGGL(k_finalize, dim3(L.nv), dim3(256), 0, c->stream, L, cflmask, summode);
   tend(c);
   return launch_check("k_finalize");
-}
-
-Program prog1(int op, int arg = 0) {
-  Program p{};
-  p.n = 1;
-  p.op[0] = op;
-  p.arg[0] = arg;
-  return p;
 }
 
 int copy_dtdrift(ramgpu_ctx *c, int s, int which, double *out) {
@@ -245,12 +230,18 @@ int copy_dtdrift(ramgpu_ctx *c, int s, int which, double *out) {
   return RAMGPU_OK;
 }
 
-int drift_op(ramgpu_ctx *c, int s, int op, int which, double *dtdrift) {
+// which: 0 R, 1 P, 2 E, 3 Mu.  Each call records its CFL limit, as the
+// reference's operators do.
+int drift_op(ramgpu_ctx *c, int s, int which, double *dtdrift) {
   int rc = check_ctx(c);
   if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
-  const Program p = prog1(op, 1);  // each call records its CFL limit, as the reference's
-  rc = (op == OP_R || op == OP_P) ? run_ij(c, p, s - 1, 1) : run_kl(c, p, s - 1, 1);
-  if (rc || (rc = run_finalize(c, 1 << which, 0, s - 1, 1, nullptr))) return rc;
+  switch (which) {
+    case 0: rc = run_r(c, 1, s - 1, 1); break;
+    case 1: rc = run_p(c, 1, s - 1, 1); break;
+    case 2: rc = run_plane(c, SEQ_E, 1, s - 1, 1); break;
+    default: rc = run_plane(c, SEQ_MU, 1, s - 1, 1); break;
+  }
+  if (rc || (rc = run_finalize(c, 1 << which, 0, s - 1, 1))) return rc;
   return copy_dtdrift(c, s, which, dtdrift);
 }
 
@@ -260,19 +251,77 @@ int set_f64(ramgpu_ctx *c, double *dst, const double *src, size_t n, const char 
   return RAMGPU_OK;
 }
 
-}  // namespace
+int check_dims(int nr, int nt, int ne, int npa) {
+  const long long P = (long long)ne * npa;
+  if (P > (long long)kPlaneThreads * kMaxCellsPerThread)
+    return fail(RAMGPU_EINVAL, "NE*NPA=%lld above %d", P, kPlaneThreads * kMaxCellsPerThread);
+  if ((long long)nt * P >= (1LL << 22) || (long long)nr * nt >= (1LL << 22))
+    return fail(RAMGPU_EINVAL, "grid %dx%dx%dx%d: NT*NE*NPA must stay below 2^22", nr, nt, ne, npa);
+  return RAMGPU_OK;
+}
+
+// Let the plane kernels use the device's whole LDS; refuse planes that do
+// not fit it.
+int set_plane_lds_limit(size_t limit) {
+  const void *fns[] = {(const void *)k_plane<SEQ_E>, (const void *)k_plane<SEQ_MU>,
+                       (const void *)k_plane<SEQ_CEX>, (const void *)k_plane<SEQ_ATM>,
+                       (const void *)k_plane<SEQ_SUM>, (const void *)k_plane<SEQ_STEP>,
+                       (const void *)k_plane<SEQ_LOSS>};
+  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
+  return RAMGPU_OK;
+}
+
+}  // namespace ramgpu_host
+
+int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
+  if (!out) return RAMGPU_OK;
+  std::vector<double> e(c->ns * 4), d(c->ns * 4);
+  HIP_TRY(hipMemcpyAsync(e.data(), c->t.energy, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(d.data(), c->t.dtdrift, d.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int s = 0; s < c->ns; ++s) {
+    out[s].setrc = e[s * 4 + 0];
+    out[s].lsdr = e[s * 4 + 1];
+    out[s].lscha = e[s * 4 + 2];
+    out[s].lsatm = e[s * 4 + 3];
+    out[s].dtdrift_r = d[s * 4 + 0];
+    out[s].dtdrift_p = d[s * 4 + 1];
+    out[s].dtdrift_e = d[s * 4 + 2];
+    out[s].dtdrift_mu = d[s * 4 + 3];
+  }
+  return RAMGPU_OK;
+}
+
+// One ram_run call for all species: R, P, plane(E, Mu, loss chain, Mu, E),
+// P, R(+SUMRC, seam, mask), finalize.  The CFL limits depend on the tables
+// only, so they are recorded in the first sweep of each kind after a table
+// change and reused until the next one.
+int ramgpu_host::step_once(ramgpu_ctx *c) {
+  int rc;
+  if ((rc = prepare(c))) return rc;
+  const int rec = c->cfl_valid ? 0 : 1;
+  const int ns = c->ns;
+  if ((rc = run_r(c, rec, 0, ns)) || (rc = run_p(c, rec, 0, ns)) ||
+      (rc = run_plane(c, SEQ_STEP, rec, 0, ns)) || (rc = run_p(c, 0, 0, ns)) ||
+      (rc = run_r(c, 2, 0, ns)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+    return rc;
+  c->cfl_valid = true;
+  return RAMGPU_OK;
+}
 
 extern "C" {
 
-const char *ramgpu_last_error(void) { return g_err.c_str(); }
+const char *ramgpu_last_error(void) { return ramgpu_host::last_error().c_str(); }
 
 int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int device) {
   if (!out) return fail(RAMGPU_EINVAL, "null output pointer");
   *out = nullptr;
+  int rc0;
   if (ns < 1 || ns > kMaxSpecies) return fail(RAMGPU_EINVAL, "NS=%d outside 1..%d", ns, kMaxSpecies);
   if (nr < 4 || nt < 4 || ne < 4 || npa < 4)
     return fail(RAMGPU_EINVAL, "grid %dx%dx%dx%d: every axis needs at least 4 points", nr, nt, ne, npa);
   if (npa > 1024) return fail(RAMGPU_EINVAL, "NPA=%d above 1024", npa);
+  if ((rc0 = check_dims(nr, nt, ne, npa))) return rc0;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(RAMGPU_EHIP, "no HIP device available");
@@ -301,6 +350,11 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   const size_t n3 = (size_t)nr * nt * npa, n2 = (size_t)nr * nt;
   const size_t nf2 = (size_t)ns * nr * nt * ne * npa;
   Tab &t = c->t;
+  t.ns = ns; t.nr = nr; t.nt = nt; t.ne = ne; t.npa = npa;
+  t.maxblk = std::max(blocks_kl(t), blocks_r(t));  // partial sums per (slot, species)
+  if (plane_lds_bytes(ne, npa) > c->lds_limit)
+    return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS (%zu bytes)", ne * npa,
+                        c->lds_limit));
 #define A(p, n) \
   if ((rc = dev_alloc(c, &p, n))) return cleanup(rc)
   A(c->d_rlz, nr1); A(c->d_ekev, ne); A(c->d_ebnd, ne); A(c->d_we, ne); A(c->d_de, ne);
@@ -312,49 +366,36 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   A(c->d_mgnp, n2); A(c->d_fgeos, (size_t)ns * nt * ne * npa); A(c->d_bscale, ns);
   A(c->d_lz, nr1); A(c->d_phi, nt); A(c->d_kp, 1);
   A(t.rde, ne); A(t.rwe, ne); A(t.rdmu, npa); A(t.rwmu, npa);
-  A(t.rrlz, nr); A(t.rsum, nr); A(t.rrsum, nr);
-  A(t.cr, n2); A(t.bsr, n2); A(t.rbsr, n2); A(t.pa, n2); A(t.pce, n2); A(t.pbs, n2); A(t.rpbs, n2);
-  A(t.drd1, n2); A(t.dpd1, n2); A(t.bij, n2); A(t.rbij, n2); A(t.drm1, n2); A(t.dpm1, n2); A(t.db2, n2);
-  A(t.q1, n3); A(t.gpa, n3); A(t.fhs, n3); A(t.rfhs, n3);
-  A(t.gr, n3); A(t.gp, n3); A(t.drd2, n3); A(t.dpd2, n3); A(t.db1, n3); A(t.di1, n3); A(t.hs, n3);
-  A(t.rhs, n3); A(t.cmudot, n3); A(t.gmr, n3); A(t.gmp, n3); A(t.drm2, n3); A(t.dpm2, n3);
-  A(t.di2, n3); A(t.bhs, n3); A(t.rbhs, n3); A(t.hdns, n3); A(t.invfnhs, n3);
+  A(t.rrlz, nr); A(t.rsi, nr); A(t.rij, n2); A(t.pij, n2); A(t.kij, 8 * n2);
+  A(t.q1, n3); A(t.pl, n3); A(t.kle, 8 * n3); A(t.klm, 8 * n3); A(t.kll, n3);
   A(t.p4, (size_t)ns * ne); A(t.ekp, (size_t)ns * ne); A(t.mkp, (size_t)ns * ne);
   A(t.p2, (size_t)ns * nr * ne); A(t.edot, (size_t)ns * nr * ne); A(t.sigv, (size_t)ns * ne);
   A(t.atlos, (size_t)ns * nr * ne); A(t.rel, (size_t)ns * 8);
   A(t.carry, (size_t)ns * ne * npa);
   A(t.latl, (size_t)ns * nr * ne);
-  t.maxblk = std::max((nr - 1) * nt, ne * npa);  // covers KL planes and IJ tiles (kw>=1)
+  A(c->d_tab, 1);
   A(t.cflmax, (size_t)ns * (2 + ne + npa));
   A(t.dtdrift, (size_t)ns * 4);
   A(t.f2, nf2);
   A(c->d_stage, nf2);
-  A(t.partials, (size_t)8 * ns * t.maxblk);
+  A(t.partials, (size_t)(kSumSlots + 1) * ns * t.maxblk);  // + ELORC slot of ramgpu_sumrc
   A(t.energy, (size_t)ns * 4);
-  A(c->d_elorc, ns);
 #undef A
-  t.ns = ns; t.nr = nr; t.nt = nt; t.ne = ne; t.npa = npa;
   t.rlz = c->d_rlz; t.ekev = c->d_ekev; t.ebnd = c->d_ebnd; t.we = c->d_we; t.de = c->d_de;
   t.grel = c->d_grel; t.grbnd = c->d_grbnd; t.v = c->d_v; t.mu = c->d_mu; t.wmu = c->d_wmu;
   t.dmu = c->d_dmu; t.upa = c->d_upa;
   t.fnhs = c->d_fnhs; t.fnis = c->d_fnis; t.bounhs = c->d_bounhs; t.bounis = c->d_bounis;
   t.hdns_f = c->d_hdnsf; t.didt = c->d_didt; t.dibndt = c->d_dibndt; t.bnes = c->d_bnes;
   t.dbdt = c->d_dbdt; t.vt = c->d_vt; t.eir = c->d_eir; t.eip = c->d_eip; t.mgnp = c->d_mgnp;
-  t.fgeos = c->d_fgeos; t.bscale = c->d_bscale;
+  t.fgeos = c->d_fgeos; t.bscale = c->d_bscale; t.lz = c->d_lz; t.phi = c->d_phi;
   t.dts = 5.0; t.betalim = 1.5; t.fraccfl = 0.8;
   t.rnt = 1.0f / (float)nt;
   t.rne = 1.0f / (float)ne;
+  t.rp = 1.0f / (float)(ne * npa);
   std::vector<double> ones(ns, 1.0);
   if (hipMemcpy(c->d_bscale, ones.data(), ns * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(RAMGPU_EHIP, "hipMemcpy failed"));
-  // launch shapes: IJ tile width 2^kws chosen so the tile fits the LDS budget
-  while (c->kws > 0 && ij_lds(c) > std::min<size_t>(c->lds_limit, 64 * 1024)) --c->kws;
-  if (ij_lds(c) > c->lds_limit)
-    return cleanup(fail(RAMGPU_EINVAL, "NR*NT=%d plane does not fit the device LDS", nr * nt));
-  if (kl_lds(c) > c->lds_limit)
-    return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS", ne * npa));
-  (void)hipFuncSetAttribute((const void *)k_ij, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
-  (void)hipFuncSetAttribute((const void *)k_kl, hipFuncAttributeMaxDynamicSharedMemorySize, (int)c->lds_limit);
+  if ((rc = set_plane_lds_limit(c->lds_limit))) return cleanup(rc);
   *out = c;
   return RAMGPU_OK;
 }
@@ -384,6 +425,7 @@ int ramgpu_set_species(ramgpu_ctx *c, int s, int charge, int cex_kind, double rm
   c->t.rmas[s - 1] = rmas;
   c->have_species[s - 1] = true;
   c->tables_dirty = true;
+  c->tab_dirty = true;
   return RAMGPU_OK;
 }
 
@@ -395,6 +437,7 @@ int ramgpu_set_params(ramgpu_ctx *c, double dts, double betalim, double fraccfl)
   c->t.betalim = betalim;
   c->t.fraccfl = fraccfl;
   c->tables_dirty = true;
+  c->tab_dirty = true;
   return RAMGPU_OK;
 }
 
@@ -423,6 +466,7 @@ int ramgpu_set_grid(ramgpu_ctx *c, double mdr, double dphi, double conf1, double
   c->t.conf2 = conf2;
   c->have_grid = true;
   c->tables_dirty = true;
+  c->tab_dirty = true;
   return RAMGPU_OK;
 }
 
@@ -560,10 +604,10 @@ int ramgpu_driftpara(ramgpu_ctx *c, int s) {
   return RAMGPU_OK;
 }
 
-int ramgpu_driftr(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, OP_R, 0, dt); }
-int ramgpu_driftp(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, OP_P, 1, dt); }
-int ramgpu_drifte(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, OP_E, 2, dt); }
-int ramgpu_driftmu(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, OP_MU, 3, dt); }
+int ramgpu_driftr(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, 0, dt); }
+int ramgpu_driftp(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, 1, dt); }
+int ramgpu_drifte(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, 2, dt); }
+int ramgpu_driftmu(ramgpu_ctx *c, int s, double *dt) { return drift_op(c, s, 3, dt); }
 
 int ramgpu_driftend(ramgpu_ctx *c) { return check_ctx(c); }
 
@@ -572,7 +616,7 @@ int ramgpu_cepara(ramgpu_ctx *c, int s) { return ramgpu_driftpara(c, s); }
 int ramgpu_charexchange(ramgpu_ctx *c, int s) {
   int rc = check_ctx(c);
   if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
-  if ((rc = run_kl(c, prog1(OP_CEX), s - 1, 1))) return rc;
+  if ((rc = run_plane(c, SEQ_CEX, 0, s - 1, 1))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return RAMGPU_OK;
 }
@@ -580,85 +624,28 @@ int ramgpu_charexchange(ramgpu_ctx *c, int s) {
 int ramgpu_atmol(ramgpu_ctx *c, int s) {
   int rc = check_ctx(c);
   if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
-  if ((rc = run_kl(c, prog1(OP_ATM), s - 1, 1))) return rc;
+  if ((rc = run_plane(c, SEQ_ATM, 0, s - 1, 1))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
   return RAMGPU_OK;
 }
 
 int ramgpu_sumrc(ramgpu_ctx *c, int s, double *setrc, double *elorc) {
   int rc = check_ctx(c);
-  if (rc || (rc = check_species(c, s)) || (rc = ready(c))) return rc;
-  // slot 0 partials: every KL block writes one (rows I>=2); zero the rest
-  HIP_TRY(hipMemsetAsync(c->t.partials + (size_t)(s - 1) * c->t.maxblk, 0,
-                         c->t.maxblk * sizeof(double), c->stream));
-  if ((rc = run_kl(c, prog1(OP_SUM, 0), s - 1, 1))) return rc;
-  if ((rc = run_finalize(c, 0, 1, s - 1, 1, c->d_elorc))) return rc;
+  if (rc || (rc = check_species(c, s)) || (rc = ready(c)) || (rc = sync_tab(c))) return rc;
+  // every plane block (rows I>=2) writes its slot-0 partial
+  if ((rc = run_plane(c, SEQ_SUM, 0, s - 1, 1))) return rc;
+  if ((rc = run_finalize(c, 0, 1, s - 1, 1))) return rc;
   double e[4], el;
   HIP_TRY(hipMemcpyAsync(e, c->t.energy + (s - 1) * 4, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(&el, c->d_elorc + (s - 1), sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(&el, c->t.partials + ((size_t)kSumSlots * c->ns + (s - 1)) * c->t.maxblk,
+                         sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (setrc) *setrc = e[0];
   if (elorc) *elorc = el;
   return RAMGPU_OK;
 }
 
-// One ram_run call for all species: IJ(R,P) -> KL(E,Mu,loss chain,Mu,E) ->
-// IJ(P,R,SUMRC,seam/mask) -> finalize.  The CFL limits depend on the tables
-// only, so they are recorded in the first sweep of each kind after a table
-// change and reused until the next one.
-static int step_once(ramgpu_ctx *c) {
-  int rc;
-  if ((rc = prepare(c))) return rc;
-  const int rec = c->cfl_valid ? 0 : 1;
-  Program a{};
-  a.n = 2;
-  a.op[0] = OP_R;
-  a.arg[0] = rec;
-  a.op[1] = OP_P;
-  a.arg[1] = rec;
-  if ((rc = run_ij(c, a, 0, c->ns))) return rc;
-  Program k{};
-  k.n = 5;
-  k.op[0] = OP_E;
-  k.arg[0] = rec;
-  k.op[1] = OP_MU;
-  k.arg[1] = rec;
-  k.op[2] = OP_LOSS;
-  k.arg[2] = 0;
-  k.op[3] = OP_MU;
-  k.op[4] = OP_E;
-  if ((rc = run_kl(c, k, 0, c->ns))) return rc;
-  Program b{};
-  b.n = 4;
-  b.op[0] = OP_P;
-  b.op[1] = OP_R;
-  b.op[2] = OP_SUM;
-  b.arg[2] = 5;
-  b.op[3] = OP_SEAM;
-  if ((rc = run_ij(c, b, 0, c->ns))) return rc;
-  if ((rc = run_finalize(c, rec ? 15 : 0, 2, 0, c->ns, nullptr))) return rc;
-  c->cfl_valid = true;
-  return RAMGPU_OK;
-}
 
-static int fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
-  if (!out) return RAMGPU_OK;
-  std::vector<double> e(c->ns * 4), d(c->ns * 4);
-  HIP_TRY(hipMemcpyAsync(e.data(), c->t.energy, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(d.data(), c->t.dtdrift, d.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int s = 0; s < c->ns; ++s) {
-    out[s].setrc = e[s * 4 + 0];
-    out[s].lsdr = e[s * 4 + 1];
-    out[s].lscha = e[s * 4 + 2];
-    out[s].lsatm = e[s * 4 + 3];
-    out[s].dtdrift_r = d[s * 4 + 0];
-    out[s].dtdrift_p = d[s * 4 + 1];
-    out[s].dtdrift_e = d[s * 4 + 2];
-    out[s].dtdrift_mu = d[s * 4 + 3];
-  }
-  return RAMGPU_OK;
-}
 
 int ramgpu_ram_run(ramgpu_ctx *c, int nsteps, ramgpu_species_out *out) {
   int rc = check_ctx(c);

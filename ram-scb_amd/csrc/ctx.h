@@ -1,0 +1,103 @@
+// ctx.h -- host-side context of the gfx950 RAM split step (internal to the
+// library: api.hip, batch.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "ramgpu.h"
+#include "ramgpu_dev.h"
+#include "sweep_common.h"
+
+namespace ramgpu {
+__global__ void k_sched(Launch L, const double *kp, const double *bscale, const int *bs_off, const int *step,
+                        int nmembers, int stride_bs);
+__global__ void k_tick(int *step);
+template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
+__global__ void k_p_bwd(Launch L, const double *xin, double *adj);
+template <int SEQ> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
+__global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
+                         const double *w, const double *target, double *loss_out, double *lam);
+__global__ void k_axpy(double *y, const double *x, double a, size_t n);
+__global__ void k_fill(double *y, double v, size_t n);
+__global__ void k_species(Launch L);
+__global__ void k_coef_ij(Launch L);
+__global__ void k_coef(Launch L);
+__global__ void k_ghost(Launch L);
+template <int MODE> __global__ void k_r(Launch L);
+template <int MODE> __global__ void k_p(Launch L);
+template <int SEQ> __global__ void k_plane(Launch L, int rec);
+__global__ void k_finalize(Launch L, int cflmask, int summode);
+__global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
+                     double kp_val, int nr1, int nt);
+__global__ void k_copy(double2 *dst, const double2 *src, size_t n2);
+__global__ void k_f2_in(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
+__global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
+}  // namespace ramgpu
+
+
+using ramgpu::Tab;
+using ramgpu::kMaxSpecies;
+
+struct ramgpu_ctx {
+  int device = 0;
+  int ns = 0, nr = 0, nt = 0, ne = 0, npa = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  size_t lds_limit = 65536;
+  Tab t{};            // host image of the problem descriptor
+  Tab *d_tab = nullptr;  // its device copy, read by every kernel
+  bool tab_dirty = true;
+  std::vector<void *> allocs;
+  // device buffers with host-visible handles
+  double *d_rlz, *d_ekev, *d_ebnd, *d_we, *d_de, *d_grel, *d_grbnd, *d_v, *d_mu, *d_wmu, *d_dmu;
+  int *d_upa;
+  double *d_fnhs, *d_fnis, *d_bounhs, *d_bounis, *d_hdnsf, *d_didt, *d_dibndt;
+  double *d_bnes, *d_dbdt, *d_vt, *d_eir, *d_eip, *d_fgeos, *d_bscale;
+  int *d_mgnp;
+  double *d_lz, *d_phi, *d_kp;
+  double *d_stage;  // Fortran-layout staging buffer for F2 transfers
+  bool have_grid = false, have_geom = false, have_vt = false, have_bnd = false, have_f2 = false;
+  bool have_species[kMaxSpecies] = {};
+  bool have_lzphi = false;
+  bool tables_dirty = true, ghost_dirty = true;
+  bool efield_dirty = true;  // only the VT-dependent per-(I,J) terms are stale
+  bool cfl_valid = false;  // DtDrift* current for the present tables
+  // opt-in per-kernel timing: event pairs around every launch, by kind
+  bool timing = false;
+  std::vector<hipEvent_t> tev;  // 2 per recorded launch
+  std::vector<int> tkind;
+  size_t tused = 0;
+};
+
+namespace ramgpu_host {
+
+int fail(int code, const char *fmt, ...);
+const std::string &last_error();
+
+#define HIP_TRY(x)                                                                      \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) return fail(RAMGPU_EHIP, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+int check_ctx(ramgpu_ctx *c);
+int ready(ramgpu_ctx *c);
+int launch_check(const char *what);
+void tbeg(ramgpu_ctx *c, int kind);
+void tend(ramgpu_ctx *c);
+int sync_tab(ramgpu_ctx *c);
+ramgpu::Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb);
+int prepare(ramgpu_ctx *c);
+int fill_out(ramgpu_ctx *c, ramgpu_species_out *out);
+int check_species(ramgpu_ctx *c, int s);
+int run_r(ramgpu_ctx *c, int mode, int s0, int nsp);
+int run_p(ramgpu_ctx *c, int rec, int s0, int nsp);
+int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp);
+int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
+int step_once(ramgpu_ctx *c);
+
+}  // namespace ramgpu_host

@@ -7,13 +7,19 @@
 //
 // Kernel map
 //   k_species   per-species 1-D tables of DRIFTPARA/CEPARA, grid reciprocals
-//   k_coef      per-(I,J,L) / per-(I,J) / per-I intermediate terms of the
-//               four Courant numbers
+//   k_coef      per-(I,J,L) intermediate terms of the four Courant numbers
+//   k_coef_ij   per-(I,J) / per-I terms (the ones VT enters)
 //   k_ghost     DRIFTR outer-ghost carry between lines (see k_ghost)
-//   k_ij        R/P sweeps on (I,J) planes x KW consecutive (L,K) positions
-//   k_kl        E/Mu sweeps and loss operators on whole (K,L) planes
+//   k_r         DRIFTR, one thread per radial line (J,K,L)
+//   k_p         DRIFTP, one thread per azimuthal ring (I,K,L)
+//   k_plane     DRIFTE / DRIFTMU sweeps, loss operators and SUMRC on whole
+//               (K,L) planes in LDS
 //   k_finalize  DtDrift* from the CFL partial maxima; SUMRC bookkeeping
 //   k_vs        Volland-Stern VT from Kp
+//
+// Every kernel takes a Launch: a list of (problem, species) units, so one
+// launch can cover all species of one problem or all members of an
+// ensemble.
 //
 // Arithmetic: FP64 throughout, compiled with -ffp-contract=off.  Every
 // Courant number, limited face value and flux update is evaluated with the
@@ -24,199 +30,83 @@
 // rounded quotient, i.e. the same double as a/d.  The drift operators are
 // therefore bit-identical to the reference on identical inputs; the loss
 // operators differ only through the libm/ocml exp, pow and log10.
-#include "ramgpu_dev.h"
+#include "sweep_common.h"
 
 #include <hip/hip_runtime.h>
 
 namespace ramgpu {
-
-static constexpr double kOME = 7.3E-5;      // src/ModRamDrift.f90:224
-static constexpr double kQ = 1.602E-19;     // src/ModRamConst.f90:19
-static constexpr double kCS = 2.998E8;      // src/ModRamConst.f90:20
-static constexpr double kRE = 6.371E6;      // src/ModRamConst.f90:14
-
-// geometry accessors, 1-based as the reference
-#define GEO3(A, I, J, L) (A)[((I)-1) + (size_t)(t.nr + 1) * (((J)-1) + (size_t)t.nt * ((L)-1))]
-#define GEO2(A, I, J) (A)[((I)-1) + (size_t)(t.nr + 1) * ((J)-1)]
-#define MGNP(I, J) t.mgnp[((I)-1) + (size_t)t.nr * ((J)-1)]
-
-__device__ __forceinline__ size_t ij_of(const Tab &t, int i, int j) { return (size_t)i * t.nt + j; }
-__device__ __forceinline__ size_t lij_of(const Tab &t, int l, int i, int j) {
-  return ((size_t)l * t.nr + i) * t.nt + j;
-}
-__device__ __forceinline__ size_t ijl_of(const Tab &t, int i, int j, int l) {
-  return ((size_t)i * t.nt + j) * t.npa + l;
-}
-
-// n/d for 0 <= n < 2^22, d >= 1, from rd = RN_float(1/d): the float
-// quotient is within one of the true one, and one remainder test fixes it.
-__device__ __forceinline__ int fast_div(int n, int d, float rd) {
-  int q = __float2int_rz((float)n * rd);
-  const int r = n - q * d;
-  if (r < 0) --q;
-  else if (r >= d) ++q;
-  return q;
-}
-
-// a/d, correctly rounded, from rd = RN(1/d): q = RN(a*rd) lies within one ulp
-// of a/d, the residual a - d*q is exact under FMA, and one correction step
-// rounds to RN(a/d) (Markstein, IBM J. Res. Dev. 34(1), 1990).
-__device__ __forceinline__ double div_rn(double a, double d, double rd) {
-  const double q = a * rd;
-  const double r = fma(-q, d, a);
-  return fma(r, rd, q);
-}
-
-// Radial Courant number CDriftR(I,J,K,L) = CR(I,J) + CGR with
-// CGR = CGR3/(FNHS+FNHS')*P4/2./(BNES+BNES')/(RLZ+0.5*MDR)
-// (src/ModRamDrift.f90:140-146).  One function for the sweep, the CFL and the
-// ghost carry, so the inflow/outflow decision is the same everywhere.
-__device__ __forceinline__ double c_radial(const Tab &t, int s, int i, int j, int k, int l) {
-  const size_t ij = ij_of(t, i, j);
-  const double x = (t.q1[lij_of(t, l, i, j)] * t.p4[s * t.ne + k]) * 0.5;
-  return t.cr[ij] + div_rn(div_rn(x, t.bsr[ij], t.rbsr[ij]), t.rsum[i], t.rrsum[i]);
-}
-
-// Azimuthal Courant number (src/ModRamDrift.f90:236-239).
-__device__ __forceinline__ double c_azim(const Tab &t, int s, int i, int j, int k, int l) {
-  const size_t ij = ij_of(t, i, j), q = lij_of(t, l, i, j);
-  const double u = div_rn(t.p2[((size_t)s * t.nr + i) * t.ne + k] * t.gpa[q], t.fhs[q], t.rfhs[q]);
-  const double num = t.pa[ij] - u - t.pce[ij];
-  return div_rn(num, t.pbs[ij], t.rpbs[ij]) + t.omet;
-}
-
-// Limited face value of the reference's upwind scheme (src/ModRamDrift.f90:
-// 169-183) for the face between F(I) = f0 and F(I+1) = fp1.  fn and fnm1 are
-// F(N) and F(N-1) with N = I+1-sgn as the reference forms it (including its
-// periodic wrap in DRIFTP); cn is the Courant number as the limiter's CORR
-// term sees it (C, C/DE(K) or C/DMU(L)).
-__device__ __forceinline__ double face_value(double f0, double fp1, double fn, double fnm1,
-                                             double c, double cn, double beta) {
-  const double sg = (c < 0) ? -1.0 : 1.0;
-  const double x = fp1 - f0;
-  const double fup = 0.5 * (f0 + fp1 - sg * x);
-  if (fabs(x) <= 1.E-27) return fup;
-  const double r = (fn - fnm1) / x;
-  if (r <= 0) return fup;
-  const double lim = fmax(fmin(beta * r, 1.0), fmin(r, beta));
-  const double corr = -0.5 * (cn - sg) * x;
-  return fup + lim * corr;
-}
-
-// ---------------------------------------------------------------------------
-// Block reductions (fixed order: deterministic for a given launch shape).
-__device__ double block_sum(double v, double *red) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double tot = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) tot += red[w];
-  }
-  return tot;  // valid in thread 0
-}
-
-__device__ double block_max(double v, double *red) {
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double m = 0.0;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) m = fmax(m, red[w]);
-  }
-  return m;  // valid in thread 0
-}
-
-__device__ double block_min(double v, double *red) {
-  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wid] = v;
-  __syncthreads();
-  double m = v;
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + 63) >> 6;
-    for (int w = 0; w < nw; ++w) m = fmin(m, red[w]);
-  }
-  return m;  // valid in thread 0
-}
-
-__device__ __forceinline__ void lds_max(unsigned long long *slot, double v) {
-  atomicMax(slot, (unsigned long long)__double_as_longlong(v));  // v >= 0
-}
 
 // ---------------------------------------------------------------------------
 // Per-species tables.  DRIFTPARA (src/ModRamDrift.f90:64-85): P2, EDOT; the
 // K-only factor P4 of DRIFTR (:131); the K-only prefixes of DRIFTE's EDT1
 // (:337) and DRIFTMU's EDT (:424); CEPARA's cross-section x velocity
 // (src/ModRamLoss.f90:39-81) and ATLOS (:162-167); DRIFTE's relativistic
-// ghost factors (:310-311,334-335).  Block 0 also forms the grid reciprocals.
-__global__ void k_species(Tab t) {
-  const int s = blockIdx.x;
-  const double qs = (double)t.charge[s];
-  const double dts = t.dts, dphi = t.dphi, mdr = t.mdr;
-  for (int n = threadIdx.x; n < t.nr * t.ne; n += blockDim.x) {
-    const int i = n / t.ne, k = n % t.ne;
-    const double rlz = t.rlz[i];
-    const double g = t.grel[s + t.ns * k], gb = t.grbnd[s + t.ns * k];
-    const double ekev = t.ekev[k], ebnd = t.ebnd[k];
-    const size_t sik = ((size_t)s * t.nr + i) * t.ne + k;
-    t.p2[sik] = dts * ekev * 1000.0 * (g + 1.0) / g / (rlz * rlz) / dphi / qs;
-    t.edot[sik] = ebnd * dts / rlz * (gb + 1.0) / gb / 2.0;
-    double atl = 1.0;  // ATLOS(S,I,K)=EXP(-DTs/TAUB), TAUB=2*RLZ(I)/V(S,K), I,K>=2
-    if (i >= 1 && k >= 1) {
-      const double taub = 2.0 * rlz / t.v[s + t.ns * k];
-      atl = exp(-dts / taub);
-    }
-    t.atlos[sik] = atl;
-    t.latl[sik] = log(atl);
-    if (i == 0) {
-      t.p4[s * t.ne + k] = dts * ekev * 1000.0 * (g + 1.0) / g / dphi / mdr / qs;
-      t.ekp[s * t.ne + k] = ebnd * 1e3 * (gb + 1.0) / 2.0 / gb;
-      t.mkp[s * t.ne + k] = ekev * 1e3 * (g + 1.0) / 2.0 / g;
-      double sv = 0.0;
-      const int kind = t.cex[s];
-      if (kind != 0 && k >= 1) {
-        double x = log10(ekev);
-        if (x < -2.0) x = -2.0;
-        const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
-        double y;
-        if (kind == 1)
-          y = -18.767 - 0.11017 * x - 3.8173e-2 * x2 - 0.1232 * x3 - 5.0488e-2 * x4;
-        else if (kind == 2)
-          y = -20.789 + 0.92316 * x - 0.68017 * x2 + 0.66153 * x3 - 0.20998 * x4;
-        else
-          y = -18.987 - 0.10613 * x - 5.4841E-3 * x2 - 1.6262E-2 * x3 - 7.0554E-3 * x4;
-        sv = pow(10.0, y) * t.v[s + t.ns * k];
+// ghost factors (:310-311,334-335).  Species 0's block also forms the grid
+// reciprocals.  One block per (problem, species).
+__global__ void k_species(Launch L) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int s = u.s;
+    const double qs = (double)t.charge[s];
+    const double dts = t.dts, dphi = t.dphi, mdr = t.mdr;
+    for (int n = threadIdx.x; n < t.nr * t.ne; n += blockDim.x) {
+      const int i = n / t.ne, k = n % t.ne;
+      const double rlz = t.rlz[i];
+      const double g = t.grel[s + t.ns * k], gb = t.grbnd[s + t.ns * k];
+      const double ekev = t.ekev[k], ebnd = t.ebnd[k];
+      const size_t sik = ((size_t)s * t.nr + i) * t.ne + k;
+      t.p2[sik] = dts * ekev * 1000.0 * (g + 1.0) / g / (rlz * rlz) / dphi / qs;
+      t.edot[sik] = ebnd * dts / rlz * (gb + 1.0) / gb / 2.0;
+      double atl = 1.0;  // ATLOS(S,I,K)=EXP(-DTs/TAUB), TAUB=2*RLZ(I)/V(S,K), I,K>=2
+      if (i >= 1 && k >= 1) {
+        const double taub = 2.0 * rlz / t.v[s + t.ns * k];
+        atl = exp(-dts / taub);
       }
-      t.sigv[s * t.ne + k] = sv;
+      t.atlos[sik] = atl;
+      t.latl[sik] = log(atl);
+      if (i == 0) {
+        t.p4[s * t.ne + k] = dts * ekev * 1000.0 * (g + 1.0) / g / dphi / mdr / qs;
+        t.ekp[s * t.ne + k] = ebnd * 1e3 * (gb + 1.0) / 2.0 / gb;
+        t.mkp[s * t.ne + k] = ekev * 1e3 * (g + 1.0) / 2.0 / g;
+        double sv = 0.0;
+        const int kind = t.cex[s];
+        if (kind != 0 && k >= 1) {
+          double x = log10(ekev);
+          if (x < -2.0) x = -2.0;
+          const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+          double y;
+          if (kind == 1)
+            y = -18.767 - 0.11017 * x - 3.8173e-2 * x2 - 0.1232 * x3 - 5.0488e-2 * x4;
+          else if (kind == 2)
+            y = -20.789 + 0.92316 * x - 0.68017 * x2 + 0.66153 * x3 - 0.20998 * x4;
+          else
+            y = -18.987 - 0.10613 * x - 5.4841E-3 * x2 - 1.6262E-2 * x3 - 7.0554E-3 * x4;
+          sv = pow(10.0, y) * t.v[s + t.ns * k];
+        }
+        t.sigv[s * t.ne + k] = sv;
+      }
     }
-  }
-  if (threadIdx.x == 0) {
-    const double g1 = t.grel[s], g2 = t.grel[s + t.ns];
-    const double ezero = t.ekev[0] - t.we[0];
-    const double grzero = 1.0 + ezero * 1000.0 * kQ / t.rmas[s] / kCS / kCS;
-    double *rel = t.rel + s * 8;
-    rel[0] = g1;
-    rel[1] = g2;
-    rel[2] = sqrt((g2 * g2 - 1.0) / (g1 * g1 - 1.0));
-    rel[3] = grzero;
-    rel[4] = sqrt((g1 * g1 - 1.0) / (grzero * grzero - 1.0));
-  }
-  if (s == 0) {
-    for (int k = threadIdx.x; k < t.ne; k += blockDim.x) {
-      t.rde[k] = 1.0 / t.de[k];
-      t.rwe[k] = 1.0 / t.we[k];
+    if (threadIdx.x == 0) {
+      const double g1 = t.grel[s], g2 = t.grel[s + t.ns];
+      const double ezero = t.ekev[0] - t.we[0];
+      const double grzero = 1.0 + ezero * 1000.0 * kQ / t.rmas[s] / kCS / kCS;
+      double *rel = t.rel + s * 8;
+      rel[0] = g1;
+      rel[1] = g2;
+      rel[2] = sqrt((g2 * g2 - 1.0) / (g1 * g1 - 1.0));
+      rel[3] = grzero;
+      rel[4] = sqrt((g1 * g1 - 1.0) / (grzero * grzero - 1.0));
     }
-    for (int l = threadIdx.x; l < t.npa; l += blockDim.x) {
-      t.rdmu[l] = 1.0 / t.dmu[l];
-      t.rwmu[l] = 1.0 / t.wmu[l];
+    if (s == 0) {
+      for (int k = threadIdx.x; k < t.ne; k += blockDim.x) {
+        t.rde[k] = 1.0 / t.de[k];
+        t.rwe[k] = 1.0 / t.we[k];
+      }
+      for (int l = threadIdx.x; l < t.npa; l += blockDim.x) {
+        t.rdmu[l] = 1.0 / t.dmu[l];
+        t.rwmu[l] = 1.0 / t.wmu[l];
+      }
     }
   }
 }
@@ -231,183 +121,190 @@ __global__ void k_species(Tab t) {
 //   DRIFTE  :320-341                       DRD1, DPD1, GPR*, GPP*, DRD2, DPD2, dBdt1, dIdt1
 //   DRIFTMU :415-432                       DRM1, DPM1, CMUDOT, GMR*, GMP*, DRM2, DPM2, dBdt2, dIbndt2
 //   ATMOL   src/ModRamLoss.f90:501         1/FNHS
-__global__ void k_coef_ij(Tab t) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= t.nr * t.nt) return;
-  const int i = n / t.nt, j = n % t.nt;
-  const int I = i + 1, J = j + 1;
-  const int NT = t.nt;
-  const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
-  const double rlz = t.rlz[i];
-  const size_t ij = ij_of(t, i, j);
-  if (j == 0) {
-    t.rrlz[i] = 1.0 / rlz;
-    const double rs = rlz + 0.5 * mdr;
-    t.rsum[i] = rs;
-    t.rrsum[i] = 1.0 / rs;
-  }
-  {
-    const int J0 = (J == 1) ? NT - 1 : J - 1;
-    const int J1 = (J == NT) ? 2 : J + 1;
-    const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
-    const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
-    const double dv = GEO2(t.vt, I, J0) + GEO2(t.vt, I + 1, J0) - GEO2(t.vt, I, J1) - GEO2(t.vt, I + 1, J1);
-    t.cr[ij] = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
-    t.bsr[ij] = bsum;
-    t.rbsr[ij] = 1.0 / bsum;
-  }
-  double pa = 0.0, pce = 0.0, pbs = 1.0;
-  double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
-  const double bij = GEO2(t.bnes, I, J);
-  if (I >= 2) {
-    if (J >= 2) {
-      const int J1 = (J == NT) ? 2 : J + 1;
-      const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
-      pa = (GEO2(t.vt, I + 1, J) + GEO2(t.vt, I + 1, J1) - GEO2(t.vt, I - 1, J) - GEO2(t.vt, I - 1, J1)) * p1;
-      pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
-      pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
+// One unit per problem; k_coef_ij: one thread per (I,J), k_coef: one block per
+// (I,J).
+__global__ void k_coef_ij(Launch L) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int n = u.b * blockDim.x + threadIdx.x;
+    if (n >= t.nr * t.nt) continue;
+    const int i = n / t.nt, j = n % t.nt;
+    const int I = i + 1, J = j + 1;
+    const int NT = t.nt;
+    const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
+    const double rlz = t.rlz[i];
+    const size_t ij = (size_t)i * NT + j;
+    const double mg = (double)MGNP(I, J);
+    if (j == 0) {
+      t.rrlz[i] = 1.0 / rlz;
+      const double rs = rlz + 0.5 * mdr;
+      t.rsi[i] = make_double2(rs, 1.0 / rs);
     }
-    const int J0 = (J == 1) ? NT - 1 : J - 1;
-    const int J2 = (J == NT) ? 2 : J + 1;
-    drd1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-    dpd1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-    // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
-    drm1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-    dpm1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-    db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
-  }
-  t.pa[ij] = pa;
-  t.pce[ij] = pce;
-  t.pbs[ij] = pbs;
-  t.rpbs[ij] = 1.0 / pbs;
-  t.drd1[ij] = drd1;
-  t.dpd1[ij] = dpd1;
-  t.bij[ij] = bij;
-  t.rbij[ij] = 1.0 / bij;
-  t.drm1[ij] = drm1;
-  t.dpm1[ij] = dpm1;
-  t.db2[ij] = db2;
-}
-
-__global__ void k_coef(Tab t) {
-  const int b = blockIdx.x;
-  const int i = b / t.nt, j = b % t.nt;
-  const int I = i + 1, J = j + 1;
-  const int NT = t.nt, NPA = t.npa;
-  const int l = threadIdx.x;
-  const int L = l + 1;
-  const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
-  const double rlz = t.rlz[i];
-  if (l >= NPA) return;
-
-  // ---- DRIFTR, all I (:140-144)
-  {
-    const int J0 = (J == 1) ? NT - 1 : J - 1;
-    const int J1 = (J == NT) ? 2 : J + 1;
-    const double cgr1 = GEO3(t.fnis, I + 1, J1, L) + GEO3(t.fnis, I, J1, L) -
-                        GEO3(t.fnis, I + 1, J0, L) - GEO3(t.fnis, I, J0, L);
-    const double cgr2 = GEO2(t.bnes, I + 1, J1) + GEO2(t.bnes, I, J1) - GEO2(t.bnes, I + 1, J0) -
-                        GEO2(t.bnes, I, J0);
-    const double cgr3 = cgr1 + (GEO3(t.fnis, I + 1, J, L) + GEO3(t.fnis, I, J, L) -
-                                2.0 * GEO3(t.fnhs, I + 1, J, L) - 2.0 * GEO3(t.fnhs, I, J, L)) *
-                                   cgr2 / 2.0 / (GEO2(t.bnes, I + 1, J) + GEO2(t.bnes, I, J));
-    t.q1[lij_of(t, l, i, j)] = cgr3 / (GEO3(t.fnhs, I, J, L) + GEO3(t.fnhs, I + 1, J, L));
-  }
-  // ---- DRIFTP, I>=2, J>=2 (:232-235)
-  {
-    double gpa = 0.0, fhs = 1.0;
-    if (I >= 2 && J >= 2) {
+    {
+      const int J0 = (J == 1) ? NT - 1 : J - 1;
       const int J1 = (J == NT) ? 2 : J + 1;
-      const double gpa1 = GEO3(t.fnis, I, J, L) + GEO3(t.fnis, I, J1, L) +
-                          (GEO3(t.fnis, I + 1, J1, L) + GEO3(t.fnis, I + 1, J, L) - GEO3(t.fnis, I - 1, J, L) -
-                           GEO3(t.fnis, I - 1, J1, L)) * rlz / 2.0 / mdr;
-      const double gpa2 = rlz / 4.0 / mdr *
-                          (GEO3(t.fnis, I, J, L) + GEO3(t.fnis, I, J1, L) - 2.0 * GEO3(t.fnhs, I, J, L) -
-                           2.0 * GEO3(t.fnhs, I, J1, L)) *
-                          (GEO2(t.bnes, I + 1, J1) + GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J) -
-                           GEO2(t.bnes, I - 1, J1)) /
-                          (GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1));
-      gpa = gpa1 + gpa2;
-      fhs = GEO3(t.fnhs, I, J, L) + GEO3(t.fnhs, I, J1, L);
+      const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
+      const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
+      const double dv = GEO2(t.vt, I, J0) + GEO2(t.vt, I + 1, J0) - GEO2(t.vt, I, J1) - GEO2(t.vt, I + 1, J1);
+      const double cr = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
+      t.rij[ij] = make_double4(cr, bsum, 1.0 / bsum, mg);
     }
-    const size_t q = lij_of(t, l, i, j);
-    t.gpa[q] = gpa;
-    t.fhs[q] = fhs;
-    t.rfhs[q] = 1.0 / fhs;
-  }
-  const size_t q = ijl_of(t, i, j, l);
-  // ---- DRIFTE, I>=2 (:323-341)
-  {
-    double gr = 0, gp = 0, drd2 = 0, dpd2 = 0, db1 = 0, di1 = 0;
-    const double fnhs = GEO3(t.fnhs, I, J, L);
+    double pa = 0.0, pce = 0.0, pbs = 1.0;
+    double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
+    const double bij = GEO2(t.bnes, I, J);
     if (I >= 2) {
+      if (J >= 2) {
+        const int J1 = (J == NT) ? 2 : J + 1;
+        const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
+        pa = (GEO2(t.vt, I + 1, J) + GEO2(t.vt, I + 1, J1) - GEO2(t.vt, I - 1, J) - GEO2(t.vt, I - 1, J1)) * p1;
+        pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
+        pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
+      }
       const int J0 = (J == 1) ? NT - 1 : J - 1;
       const int J2 = (J == NT) ? 2 : J + 1;
-      const double bij = GEO2(t.bnes, I, J);
-      const double fnis = GEO3(t.fnis, I, J, L);
-      const double gpa = (1.0 - fnis / 2.0 / fnhs) / bij;
-      const double gpr1 = gpa * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / 2.0 / mdr;
-      const double gpr2 = -fnis / fnhs / rlz;
-      const double gpr3 = -(GEO3(t.fnis, I + 1, J, L) - GEO3(t.fnis, I - 1, J, L)) / 2.0 / mdr / fnhs;
-      const double gpp1 = gpa * (GEO2(t.bnes, I, J2) - GEO2(t.bnes, I, J0)) / 2.0 / dphi;
-      const double gpp2 = -(GEO3(t.fnis, I, J2, L) - GEO3(t.fnis, I, J0, L)) / 2.0 / dphi / fnhs;
-      drd2 = (GEO3(t.fnis, I, J2, L) - GEO3(t.fnis, I, J0, L)) / 2.0 / dphi +
-             (fnis - 2.0 * fnhs) * (GEO2(t.bnes, I, J2) - GEO2(t.bnes, I, J0)) / 4.0 / bij / dphi;
-      dpd2 = fnis + (GEO3(t.fnis, I + 1, J, L) - GEO3(t.fnis, I - 1, J, L)) * rlz / 2.0 / mdr +
-             rlz * (fnis - 2.0 * fnhs) / 4.0 / mdr * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / bij;
-      db1 = GEO2(t.dbdt, I, J) * (1.0 - fnis / 2.0 / fnhs) * rlz / bij;
-      di1 = -GEO3(t.didt, I, J, L) * rlz / fnhs;
-      gr = gpr1 + gpr2 + gpr3;
-      gp = gpp1 + gpp2;
+      drd1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
+      dpd1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+      // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
+      drm1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
+      dpm1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+      db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
     }
-    t.gr[q] = gr;
-    t.gp[q] = gp;
-    t.drd2[q] = drd2;
-    t.dpd2[q] = dpd2;
-    t.db1[q] = db1;
-    t.di1[q] = di1;
-    t.hs[q] = fnhs;
-    t.rhs[q] = 1.0 / fnhs;
+    t.pij[ij] = make_double4(pa, pce, pbs, 1.0 / pbs);
+    double *k8 = t.kij + ij * 8;
+    k8[KIJ_DRD1] = drd1;
+    k8[KIJ_DPD1] = dpd1;
+    k8[KIJ_BIJ] = bij;
+    k8[KIJ_RBIJ] = 1.0 / bij;
+    k8[KIJ_DRM1] = drm1;
+    k8[KIJ_DPM1] = dpm1;
+    k8[KIJ_DB2] = db2;
+    k8[KIJ_MG] = mg;
   }
-  // ---- DRIFTMU, I>=2, L>=2 (:418-432; MUDOT from DRIFTPARA :77-81)
-  {
-    double cmudot = 0, gmr = 0, gmp = 0, drm2 = 0, dpm2 = 0, di2 = 0;
-    const double bhs = GEO3(t.bounhs, I, J, L);
-    if (I >= 2 && L >= 2) {
-      const int J0 = (J == 1) ? NT - 1 : J - 1;
-      const int J1 = (J == NT) ? 2 : J + 1;
-      const double bij = GEO2(t.bnes, I, J);
-      double mudot = 0.0;
-      if (L < NPA) {
-        const double muboun = t.mu[l] + 0.5 * t.wmu[l];
-        mudot = (1.0 - muboun * muboun) * dts / 2.0 / muboun / rlz;
+}
+
+__global__ void k_coef(Launch L) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int b = u.b;
+    const int i = b / t.nt, j = b % t.nt;
+    const int I = i + 1, J = j + 1;
+    const int NT = t.nt, NPA = t.npa, NR = t.nr;
+    const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
+    const double rlz = t.rlz[i];
+    const double mg = (double)MGNP(I, J);
+    for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
+      const int L1 = l + 1;  // 1-based L
+      // ---- DRIFTR, all I (:140-144)
+      {
+        const int J0 = (J == 1) ? NT - 1 : J - 1;
+        const int J1 = (J == NT) ? 2 : J + 1;
+        const double cgr1 = GEO3(t.fnis, I + 1, J1, L1) + GEO3(t.fnis, I, J1, L1) -
+                            GEO3(t.fnis, I + 1, J0, L1) - GEO3(t.fnis, I, J0, L1);
+        const double cgr2 = GEO2(t.bnes, I + 1, J1) + GEO2(t.bnes, I, J1) - GEO2(t.bnes, I + 1, J0) -
+                            GEO2(t.bnes, I, J0);
+        const double cgr3 = cgr1 + (GEO3(t.fnis, I + 1, J, L1) + GEO3(t.fnis, I, J, L1) -
+                                    2.0 * GEO3(t.fnhs, I + 1, J, L1) - 2.0 * GEO3(t.fnhs, I, J, L1)) *
+                                       cgr2 / 2.0 / (GEO2(t.bnes, I + 1, J) + GEO2(t.bnes, I, J));
+        t.q1[((size_t)l * NR + i) * NT + j] = cgr3 / (GEO3(t.fnhs, I, J, L1) + GEO3(t.fnhs, I + 1, J, L1));
       }
-      const double bis = GEO3(t.bounis, I, J, L);
-      cmudot = mudot * bis / bhs;
-      const double gmr1 = (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / 4.0 / mdr / bij;
-      const double gmr2 = 1.0 / rlz;
-      const double gmr3 = (GEO3(t.bounis, I + 1, J, L) - GEO3(t.bounis, I - 1, J, L)) / 2.0 / mdr / bis;
-      const double gmp1 = (GEO2(t.bnes, I, J1) - GEO2(t.bnes, I, J0)) / 4.0 / dphi / bij;
-      const double gmp2 = (GEO3(t.bounis, I, J1, L) - GEO3(t.bounis, I, J0, L)) / 2.0 / dphi / bis;
-      drm2 = (GEO3(t.bounis, I, J1, L) - GEO3(t.bounis, I, J0, L)) / 2.0 / dphi +
-             (bis - 2.0 * bhs) * (GEO2(t.bnes, I, J1) - GEO2(t.bnes, I, J0)) / 4.0 / bij / dphi;
-      dpm2 = bis + (GEO3(t.bounis, I + 1, J, L) - GEO3(t.bounis, I - 1, J, L)) * rlz / 2.0 / mdr +
-             (bis - 2.0 * bhs) * rlz / 4.0 / mdr * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / bij;
-      di2 = GEO3(t.dibndt, I, J, L) * rlz / bis;
-      gmr = gmr1 + gmr2 + gmr3;
-      gmp = gmp1 + gmp2;
+      // ---- DRIFTP, I>=2, J>=2 (:232-235)
+      {
+        double gpa = 0.0, fhs = 1.0;
+        if (I >= 2 && J >= 2) {
+          const int J1 = (J == NT) ? 2 : J + 1;
+          const double gpa1 = GEO3(t.fnis, I, J, L1) + GEO3(t.fnis, I, J1, L1) +
+                              (GEO3(t.fnis, I + 1, J1, L1) + GEO3(t.fnis, I + 1, J, L1) -
+                               GEO3(t.fnis, I - 1, J, L1) - GEO3(t.fnis, I - 1, J1, L1)) *
+                                  rlz / 2.0 / mdr;
+          const double gpa2 = rlz / 4.0 / mdr *
+                              (GEO3(t.fnis, I, J, L1) + GEO3(t.fnis, I, J1, L1) - 2.0 * GEO3(t.fnhs, I, J, L1) -
+                               2.0 * GEO3(t.fnhs, I, J1, L1)) *
+                              (GEO2(t.bnes, I + 1, J1) + GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J) -
+                               GEO2(t.bnes, I - 1, J1)) /
+                              (GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1));
+          gpa = gpa1 + gpa2;
+          fhs = GEO3(t.fnhs, I, J, L1) + GEO3(t.fnhs, I, J1, L1);
+        }
+        t.pl[((size_t)l * NR + i) * NT + j] = make_double4(gpa, fhs, 1.0 / fhs, mg);
+      }
+      const size_t q = ((size_t)i * NT + j) * NPA + l;
+      // ---- DRIFTE, I>=2 (:323-341)
+      {
+        double gr = 0, gp = 0, drd2 = 0, dpd2 = 0, db1 = 0, di1 = 0;
+        const double fnhs = GEO3(t.fnhs, I, J, L1);
+        if (I >= 2) {
+          const int J0 = (J == 1) ? NT - 1 : J - 1;
+          const int J2 = (J == NT) ? 2 : J + 1;
+          const double bij = GEO2(t.bnes, I, J);
+          const double fnis = GEO3(t.fnis, I, J, L1);
+          const double gpa = (1.0 - fnis / 2.0 / fnhs) / bij;
+          const double gpr1 = gpa * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / 2.0 / mdr;
+          const double gpr2 = -fnis / fnhs / rlz;
+          const double gpr3 = -(GEO3(t.fnis, I + 1, J, L1) - GEO3(t.fnis, I - 1, J, L1)) / 2.0 / mdr / fnhs;
+          const double gpp1 = gpa * (GEO2(t.bnes, I, J2) - GEO2(t.bnes, I, J0)) / 2.0 / dphi;
+          const double gpp2 = -(GEO3(t.fnis, I, J2, L1) - GEO3(t.fnis, I, J0, L1)) / 2.0 / dphi / fnhs;
+          drd2 = (GEO3(t.fnis, I, J2, L1) - GEO3(t.fnis, I, J0, L1)) / 2.0 / dphi +
+                 (fnis - 2.0 * fnhs) * (GEO2(t.bnes, I, J2) - GEO2(t.bnes, I, J0)) / 4.0 / bij / dphi;
+          dpd2 = fnis + (GEO3(t.fnis, I + 1, J, L1) - GEO3(t.fnis, I - 1, J, L1)) * rlz / 2.0 / mdr +
+                 rlz * (fnis - 2.0 * fnhs) / 4.0 / mdr * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / bij;
+          db1 = GEO2(t.dbdt, I, J) * (1.0 - fnis / 2.0 / fnhs) * rlz / bij;
+          di1 = -GEO3(t.didt, I, J, L1) * rlz / fnhs;
+          gr = gpr1 + gpr2 + gpr3;
+          gp = gpp1 + gpp2;
+        }
+        double *e = t.kle + q * 8;
+        e[KLE_HS] = fnhs;
+        e[KLE_RHS] = 1.0 / fnhs;
+        e[KLE_DRD2] = drd2;
+        e[KLE_DPD2] = dpd2;
+        e[KLE_GR] = gr;
+        e[KLE_GP] = gp;
+        e[KLE_DB1] = db1;
+        e[KLE_DI1] = di1;
+      }
+      // ---- DRIFTMU, I>=2, L>=2 (:418-432; MUDOT from DRIFTPARA :77-81)
+      {
+        double cmudot = 0, gmr = 0, gmp = 0, drm2 = 0, dpm2 = 0, di2 = 0;
+        const double bhs = GEO3(t.bounhs, I, J, L1);
+        if (I >= 2 && L1 >= 2) {
+          const int J0 = (J == 1) ? NT - 1 : J - 1;
+          const int J1 = (J == NT) ? 2 : J + 1;
+          const double bij = GEO2(t.bnes, I, J);
+          double mudot = 0.0;
+          if (L1 < NPA) {
+            const double muboun = t.mu[l] + 0.5 * t.wmu[l];
+            mudot = (1.0 - muboun * muboun) * dts / 2.0 / muboun / rlz;
+          }
+          const double bis = GEO3(t.bounis, I, J, L1);
+          cmudot = mudot * bis / bhs;
+          const double gmr1 = (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / 4.0 / mdr / bij;
+          const double gmr2 = 1.0 / rlz;
+          const double gmr3 = (GEO3(t.bounis, I + 1, J, L1) - GEO3(t.bounis, I - 1, J, L1)) / 2.0 / mdr / bis;
+          const double gmp1 = (GEO2(t.bnes, I, J1) - GEO2(t.bnes, I, J0)) / 4.0 / dphi / bij;
+          const double gmp2 = (GEO3(t.bounis, I, J1, L1) - GEO3(t.bounis, I, J0, L1)) / 2.0 / dphi / bis;
+          drm2 = (GEO3(t.bounis, I, J1, L1) - GEO3(t.bounis, I, J0, L1)) / 2.0 / dphi +
+                 (bis - 2.0 * bhs) * (GEO2(t.bnes, I, J1) - GEO2(t.bnes, I, J0)) / 4.0 / bij / dphi;
+          dpm2 = bis + (GEO3(t.bounis, I + 1, J, L1) - GEO3(t.bounis, I - 1, J, L1)) * rlz / 2.0 / mdr +
+                 (bis - 2.0 * bhs) * rlz / 4.0 / mdr * (GEO2(t.bnes, I + 1, J) - GEO2(t.bnes, I - 1, J)) / bij;
+          di2 = GEO3(t.dibndt, I, J, L1) * rlz / bis;
+          gmr = gmr1 + gmr2 + gmr3;
+          gmp = gmp1 + gmp2;
+        }
+        double *m = t.klm + q * 8;
+        m[KLM_BHS] = bhs;
+        m[KLM_RBHS] = 1.0 / bhs;
+        m[KLM_DRM2] = drm2;
+        m[KLM_DPM2] = dpm2;
+        m[KLM_CMUDOT] = cmudot;
+        m[KLM_GMR] = gmr;
+        m[KLM_GMP] = gmp;
+        m[KLM_DI2] = di2;
+      }
+      t.kll[q] = make_double2(GEO3(t.hdns_f, I, J, L1), 1.0 / GEO3(t.fnhs, I, J, L1));
     }
-    t.cmudot[q] = cmudot;
-    t.gmr[q] = gmr;
-    t.gmp[q] = gmp;
-    t.drm2[q] = drm2;
-    t.dpm2[q] = dpm2;
-    t.di2[q] = di2;
-    t.bhs[q] = bhs;
-    t.rbhs[q] = 1.0 / bhs;
   }
-  t.hdns[q] = GEO3(t.hdns_f, I, J, L);
-  t.invfnhs[q] = 1.0 / GEO3(t.fnhs, I, J, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -418,477 +315,524 @@ __global__ void k_coef(Tab t) {
 // order, or 0 when no inflow line came before.  Whether a line is inflow
 // depends on geometry, fields and grid only, so the carry is precomputed:
 // carry[s][p] is the index p'*NT+(J'-1) of the last inflow line before plane
-// p=(K-1)*NPA+(L-1), or -1.  One block per species; block-wide max-scan.
-__device__ __forceinline__ bool line_inflow(const Tab &t, int s, int j, int k, int l) {
-  return c_radial(t, s, t.nr - 1, j, k, l) < 0;
-}
-
-__global__ void k_ghost(Tab t) {
+// p=(K-1)*NPA+(L-1), or -1.  One block per (problem, species); block-wide
+// max-scan.
+__global__ void k_ghost(Launch L) {
   __shared__ int s_scan[1024];
-  const int s = blockIdx.x;
-  const int P = t.ne * t.npa;
-  const int nthr = blockDim.x;
-  const int chunk = (P + nthr - 1) / nthr;
-  const int p0 = threadIdx.x * chunk, p1 = min(P, p0 + chunk);
-  int last = -1;
-  for (int p = p0; p < p1; ++p) {
-    const int k = p / t.npa, l = p % t.npa;
-    for (int j = t.nt - 1; j >= 0; --j)
-      if (line_inflow(t, s, j, k, l)) {
-        last = p * t.nt + j;
-        break;
-      }
-  }
-  s_scan[threadIdx.x] = last;
-  __syncthreads();
-  for (int off = 1; off < nthr; off <<= 1) {  // inclusive max-scan
-    const int v = (threadIdx.x >= off) ? s_scan[threadIdx.x - off] : -1;
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int s = u.s;
+    const int P = t.ne * t.npa;
+    const int nthr = blockDim.x;
+    const int chunk = (P + nthr - 1) / nthr;
+    const int p0 = threadIdx.x * chunk, p1 = min(P, p0 + chunk);
+    int last = -1;
+    for (int p = p0; p < p1; ++p) {
+      const int k = p / t.npa, l = p % t.npa;
+      for (int j = t.nt - 1; j >= 0; --j)
+        if (line_inflow(t, s, j, k, l)) {
+          last = p * t.nt + j;
+          break;
+        }
+    }
+    s_scan[threadIdx.x] = last;
     __syncthreads();
-    s_scan[threadIdx.x] = max(s_scan[threadIdx.x], v);
+    for (int off = 1; off < nthr; off <<= 1) {  // inclusive max-scan
+      const int v = (threadIdx.x >= off) ? s_scan[threadIdx.x - off] : -1;
+      __syncthreads();
+      s_scan[threadIdx.x] = max(s_scan[threadIdx.x], v);
+      __syncthreads();
+    }
+    int run = (threadIdx.x > 0) ? s_scan[threadIdx.x - 1] : -1;
+    for (int p = p0; p < p1; ++p) {
+      t.carry[(size_t)s * P + p] = run;
+      const int k = p / t.npa, l = p % t.npa;
+      for (int j = t.nt - 1; j >= 0; --j)
+        if (line_inflow(t, s, j, k, l)) {
+          run = p * t.nt + j;
+          break;
+        }
+    }
     __syncthreads();
   }
-  int run = (threadIdx.x > 0) ? s_scan[threadIdx.x - 1] : -1;
-  for (int p = p0; p < p1; ++p) {
-    t.carry[(size_t)s * P + p] = run;
-    const int k = p / t.npa, l = p % t.npa;
-    for (int j = t.nt - 1; j >= 0; --j)
-      if (line_inflow(t, s, j, k, l)) {
-        run = p * t.nt + j;
-        break;
-      }
-  }
-}
-
-// F(NR+1) (conf=CONF1) or F(NR+2) (conf=CONF2) of inflow line (J,K,L):
-// 0 outside the magnetopause, else FGEOS*CONF*FNHS(NR,J,L) (:161-167).
-__device__ __forceinline__ double ghost_value(const Tab &t, int s, int j, int k, int l, double conf) {
-  const int NR = t.nr;
-  if (t.mgnp[(NR - 1) + (size_t)NR * j] == 1) return 0.0;
-  const double fg = t.fgeos[s + (size_t)t.ns * (j + (size_t)t.nt * (k + (size_t)t.ne * l))] * t.bscale[s];
-  return fg * conf * t.fnhs[(NR - 1) + (size_t)(NR + 1) * (j + (size_t)t.nt * l)];
-}
-
-__device__ double stale_ghost(const Tab &t, int s, int j, int k, int l) {
-  for (int jj = j - 1; jj >= 0; --jj)
-    if (line_inflow(t, s, jj, k, l)) return ghost_value(t, s, jj, k, l, t.conf1);
-  const int p = k * t.npa + l;
-  const int c = t.carry[(size_t)s * t.ne * t.npa + p];
-  if (c < 0) return 0.0;
-  const int pp = c / t.nt, jj = c % t.nt;
-  return ghost_value(t, s, jj, pp / t.npa, pp % t.npa, t.conf1);
 }
 
 // ---------------------------------------------------------------------------
-// IJ kernel: one block per (species, tile of KW = 2^kws consecutive (L,K)
-// positions), holding F[(I,J)][w] and the face fluxes G = C*FBND in LDS, plus
-// the inflow flag of every radial line.  Runs a program of OP_R / OP_P /
-// OP_SUM / OP_SEAM.
-__device__ __forceinline__ void atomic_max_pos(unsigned long long *slot, double v) {
-  atomicMax(slot, (unsigned long long)__double_as_longlong(v));  // v >= 0
+// DRIFTR, src/ModRamDrift.f90:139-191: one thread per radial line (J,K,L),
+// consecutive lanes on consecutive (L,K) positions.  The thread walks I with
+// a register window F(I-1..I+2) of old values (F(I+3) loaded one step
+// ahead), forms face I, and updates cell I from faces I and I-1.
+// MODE 0: sweep; 1: sweep and record the CFL maximum; 2: the last sweep of
+// ram_run: also the SUMRC partial sum of src/ModRamRun.f90:246-255 (slot 5),
+// then F2(:,:,NT)=F2(:,:,1) and the magnetopause mask (:186,197-201).  In
+// mode 2 the J=NT line is not swept: the seam overwrites it with line J=1,
+// whose thread stores both.
+template <int MODE>
+__global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
+  __shared__ double red[kLineThreads / 64];
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int s = u.s;
+    const int NR = t.nr, NT = t.nt, NE = t.ne;
+    const int P = NE * t.npa;
+    const int line = u.b * kLineThreads + threadIdx.x;
+    const int j = fast_div(line, P, t.rp);
+    const bool act = line < NT * P && !(MODE == 2 && j == NT - 1);
+    double cmax = 0.0, acc = 0.0;
+    if (act) {
+      const int q = line - j * P;
+      const int l = fast_div(q, NE, t.rne), k = q - l * NE;
+      const double p4k = t.p4[s * NE + k];
+      const double beta = t.betalim;
+      const size_t di = (size_t)NT * P;  // stride between I and I+1
+      double *fl = t.f2 + (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
+      double *fs = fl + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
+      const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
+      double gh1 = 0.0, gh2 = 0.0;
+      if (in) {
+        gh1 = ghost_value(t, s, j, k, l, t.conf1);
+        gh2 = ghost_value(t, s, j, k, l, t.conf2);
+      }
+      double wek = 0, wml = 0, ekk = 0;
+      if (MODE == 2) {
+        wek = t.we[k];
+        wml = t.wmu[l];
+        ekk = t.ekev[k];
+      }
+      const bool sum_line = MODE == 2 && k >= 1 && l >= 1 && j <= NT - 2;
+      // face 1: FBND(1) = F(2) on inflow lines, 0 otherwise (:156,164)
+      const double f1 = fl[0], f2v = fl[di], f3 = fl[2 * di];
+      double fnext = fl[3 * di];  // F(4); NR >= 4
+      const double c1 = c_radial(t, p4k, 0, j, l);
+      if (MODE == 1 && t.rij[j].w == 0.0) cmax = fmax(cmax, fmax(fabs(c1), 1E-10));
+      double gprev = c1 * (in ? f2v : 0.0);
+      if (MODE == 2) {
+        const double mg1 = t.rij[j].w, mgs = t.rij[NT - 1].w;
+        if (mg1 == 1.0) fl[0] = 1e-31;
+        if (j == 0) fs[0] = (mgs == 1.0) ? 1e-31 : f1;
+      }
+      double wm1 = f1, w0 = f2v, w1 = f3, w2 = fnext;  // F(I-1), F(I), F(I+1), F(I+2) at I=2
+      for (int I = 2; I <= NR; ++I) {
+        const int i = I - 1;
+        fnext = (I + 3 <= NR) ? fl[(size_t)(I + 2) * di] : 0.0;
+        const double4 r = t.rij[(size_t)i * NT + j];
+        const double2 rs = t.rsi[i];
+        const double x = (t.q1[((size_t)l * NR + i) * NT + j] * p4k) * 0.5;
+        const double ci = r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
+        if (MODE == 1 && r.w == 0.0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));
+        double fb;
+        if (I == NR && !in) {
+          fb = w0;  // FBND(NR) = F(NR) on outflow lines
+        } else {
+          const double fp1 = (I == NR) ? gh1 : w1;
+          double fn, fnm1;
+          if (ci < 0) {
+            fnm1 = fp1;
+            if (I == NR)
+              fn = gh2;
+            else if (I + 2 <= NR)
+              fn = w2;
+            else  // I == NR-1: the ghost F(NR+1)
+              fn = in ? gh1 : stale_ghost(t, s, j, k, l);
+          } else {
+            fn = w0;
+            fnm1 = wm1;
+          }
+          fb = face_value(w0, fp1, fn, fnm1, ci, ci, beta);
+        }
+        const double g = ci * fb;
+        double v = w0 - g + gprev;
+        if (v < 0) v = 1E-15;
+        gprev = g;
+        if (MODE == 2) {
+          if (sum_line) acc += ekk * (v * wek * wml);
+          fl[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;
+          if (j == 0) fs[(size_t)i * di] = (t.rij[(size_t)i * NT + NT - 1].w == 1.0) ? 1e-31 : v;
+        } else {
+          fl[(size_t)i * di] = v;
+        }
+        wm1 = w0;
+        w0 = w1;
+        w1 = w2;
+        w2 = fnext;
+      }
+    }
+    if (MODE == 1) {
+      const double m = block_max(cmax, red);
+      if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 0], m);
+    }
+    if (MODE == 2) {
+      double a[1] = {acc}, tot[1];
+      block_sum_n<1>(a, red, tot);
+      if (threadIdx.x == 0) t.partials[((size_t)5 * t.ns + s) * t.maxblk + u.b] = tot[0];
+    }
+  }
 }
 
-__global__ void __launch_bounds__(256) k_ij(Tab t, Program prog, int s0, int ntiles, int kws) {
-  extern __shared__ double sm[];
-  const int kw = 1 << kws;
-  const int nij = t.nr * t.nt;
-  double *F = sm;
-  double *G = sm + (size_t)nij * kw;
-  double *red = G + (size_t)nij * kw;  // 16 doubles
-  int *inflow = (int *)(red + 16);      // NT*kw line flags
-  const int s = s0 + blockIdx.x / ntiles;
-  const int tile = blockIdx.x % ntiles;
-  const int P = t.ne * t.npa;
-  const int q0 = tile * kw;
-  const int nw = min(kw, P - q0);
-  const int ncell = nij * kw;
-  double *gF = t.f2 + (size_t)s * f2_species_stride(t);
-  const int NR = t.nr, NT = t.nt, NE = t.ne;
-  const double beta = t.betalim;
-
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const int ij = c >> kws, w = c & (kw - 1);
-    F[c] = (w < nw) ? gF[(size_t)ij * P + q0 + w] : 0.0;
+// ---------------------------------------------------------------------------
+// DRIFTP, src/ModRamDrift.f90:229-272: one thread per ring (I>=2,K,L),
+// periodic in J.  Face NT is formed first (FBND(1)=FBND(NT),
+// CDriftP(1)=CDriftP(NT), :261-262), then faces and cells J=2..NT-1 in order
+// with a register window, then cell NT and F2(I,1)=F2(I,NT) (:272).
+// MODE 0: sweep; 1: sweep and record the CFL maximum.
+template <int MODE>
+__global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
+  __shared__ double red[kLineThreads / 64];
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int s = u.s;
+    const int NR = t.nr, NT = t.nt, NE = t.ne;
+    const int P = NE * t.npa;
+    const int line = u.b * kLineThreads + threadIdx.x;
+    double cmax = 0.0;
+    if (line < (NR - 1) * P) {
+      const int i1 = fast_div(line, P, t.rp);
+      const int q = line - i1 * P;
+      const int i = i1 + 1;
+      const int l = fast_div(q, NE, t.rne), k = q - l * NE;
+      const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
+      const double beta = t.betalim, omet = t.omet;
+      double *fl = t.f2 + (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;  // F(J) at (J-1)*P
+      const double4 *pij = t.pij + (size_t)i * NT;
+      const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
+      // Courant number of face J (src/ModRamDrift.f90:236-239)
+#define CAZIM(JJ, OUT, MG)                                                 \
+  {                                                                        \
+    const double4 a_ = pij[(JJ)-1];                                        \
+    const double4 g_ = pl[(JJ)-1];                                         \
+    const double u_ = div_rn(p2k * g_.x, g_.y, g_.z);                      \
+    const double num_ = a_.x - u_ - a_.y;                                  \
+    OUT = div_rn(num_, a_.z, a_.w) + omet;                                 \
+    MG = g_.w;                                                             \
   }
-  __syncthreads();
-
-  for (int o = 0; o < prog.n; ++o) {
-    const int op = prog.op[o];
-    if (op == OP_R) {
-      // DRIFTR, src/ModRamDrift.f90:139-191: face I of line (J,w) per cell
-      for (int n = threadIdx.x; n < NT * kw; n += blockDim.x) {  // sgn(NR) per line (:154)
-        const int j = n >> kws, w = n & (kw - 1);
-        int f = 0;
-        if (w < nw) {
-          const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
-          f = line_inflow(t, s, j, k, l) ? 1 : 0;
-        }
-        inflow[n] = f;
-      }
-      __syncthreads();
-      double cmax = 0.0;
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw) continue;
-        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
-        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
-        const int I = i + 1;
-        const double ci = c_radial(t, s, i, j, k, l);
-        if (MGNP(I, j + 1) == 0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));
-        const bool in = inflow[(j << kws) + w] != 0;
-        const int base = (j << kws) + w;  // F(I') at base + (I'-1)*NT*kw
-#define FR(II) F[base + (((II)-1) * NT << kws)]
-        double fb;
-        if (I == 1) {
-          fb = in ? FR(2) : 0.0;
-        } else if (!in && I == NR) {
-          fb = FR(NR);
-        } else {
-          const double f0 = FR(I);
-          double fp1, fn, fnm1;
-          if (I == NR) {  // inflow only: ghosts F(NR+1), F(NR+2)
-            fp1 = ghost_value(t, s, j, k, l, t.conf1);
-            if (ci < 0) {
-              fn = ghost_value(t, s, j, k, l, t.conf2);
-              fnm1 = fp1;
-            } else {
-              fn = f0;
-              fnm1 = FR(I - 1);
-            }
-          } else {
-            fp1 = FR(I + 1);
-            if (ci < 0) {
-              fnm1 = fp1;
-              if (I + 2 <= NR)
-                fn = FR(I + 2);
-              else  // I == NR-1: the ghost F(NR+1)
-                fn = in ? ghost_value(t, s, j, k, l, t.conf1) : stale_ghost(t, s, j, k, l);
-            } else {
-              fn = f0;
-              fnm1 = FR(I - 1);
-            }
-          }
-          fb = face_value(f0, fp1, fn, fnm1, ci, ci, beta);
-        }
-#undef FR
-        G[c] = ci * fb;
-      }
-      if (prog.arg[o]) {
-        const double m = block_max(cmax, red);
-        if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 0], m);
-      }
-      __syncthreads();
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw || ij < NT) continue;  // I>=2
-        double v = F[c] - G[c] + G[c - (NT << kws)];
-        if (v < 0) v = 1E-15;
-        F[c] = v;
-      }
-      __syncthreads();
-    } else if (op == OP_P) {
-      // DRIFTP, src/ModRamDrift.f90:229-272 (periodic in J)
-      double cmax = 0.0;
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw) continue;
-        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
-        const int J = j + 1;
-        if (i < 1 || J < 2) continue;
-        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
-        const double cp = c_azim(t, s, i, j, k, l);
-        if (MGNP(i + 1, J) == 0) cmax = fmax(cmax, fmax(fabs(cp), 1E-10));
-        const int base = ((i * NT) << kws) + w;  // F(J') at base + (J'-1)*kw
-#define FP(JJ) F[base + (((JJ)-1) << kws)]
-        const int J1 = (J == NT) ? 2 : J + 1;
-        const double f0 = FP(J), fp1 = FP(J1);
+      const double F1 = fl[0], F2o = fl[P], F3 = fl[2 * (size_t)P];
+      const double FNm1 = fl[(size_t)(NT - 2) * P], FN = fl[(size_t)(NT - 1) * P];
+      double cN, mgN;
+      CAZIM(NT, cN, mgN);
+      if (MODE == 1 && mgN == 0.0) cmax = fmax(cmax, fmax(fabs(cN), 1E-10));
+      // face NT: J1 = 2; upwind N = NT+2 -> 3 (:250-251)
+      const double gN = cN * face_value(FN, F2o, (cN < 0) ? F3 : FN, (cN < 0) ? F2o : FNm1, cN, cN, beta);
+      double gprev = gN;
+      double wm1 = F1, w0 = F2o, w1 = F3, w2 = fl[3 * (size_t)P];  // F(J-1..J+2) at J=2; NT >= 4
+      for (int J = 2; J <= NT - 1; ++J) {
+        const double fnext = (J + 3 <= NT) ? fl[(size_t)(J + 2) * P] : 0.0;
+        double cp, mg;
+        CAZIM(J, cp, mg);
+        if (MODE == 1 && mg == 0.0) cmax = fmax(cmax, fmax(fabs(cp), 1E-10));
         double fn, fnm1;
         if (cp < 0) {
-          int N = J + 2;
-          if (N > NT) N = N - NT + 1;  // :251
-          fn = FP(N);
-          fnm1 = FP(N - 1);
+          if (J + 2 <= NT) {
+            fn = w2;
+            fnm1 = w1;
+          } else {  // J = NT-1: N = NT+1 -> 2, N-1 = 1
+            fn = F2o;
+            fnm1 = F1;
+          }
         } else {
-          fn = f0;
-          fnm1 = FP(J - 1);
+          fn = w0;
+          fnm1 = wm1;
         }
-#undef FP
-        G[c] = cp * face_value(f0, fp1, fn, fnm1, cp, cp, beta);
-      }
-      if (prog.arg[o]) {
-        const double m = block_max(cmax, red);
-        if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 1], m);
-      }
-      __syncthreads();
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw) continue;
-        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
-        if (i < 1 || j < 1) continue;
-        // FBND(1)=FBND(NT), CDriftP(I,1)=CDriftP(I,NT)  (:261-262)
-        const double gm = (j == 1) ? G[((i * NT + NT - 1) << kws) + w] : G[c - kw];
-        double v = F[c] - G[c] + gm;
+        const double g = cp * face_value(w0, w1, fn, fnm1, cp, cp, beta);
+        double v = w0 - g + gprev;
         if (v < 0) v = 1E-15;
-        F[c] = v;
+        fl[(size_t)(J - 1) * P] = v;
+        gprev = g;
+        wm1 = w0;
+        w0 = w1;
+        w1 = w2;
+        w2 = fnext;
       }
-      __syncthreads();
-      for (int n = threadIdx.x; n < (NR - 1) * kw; n += blockDim.x) {  // F2(S,I,1)=F2(S,I,NT)
-        const int i = 1 + (n >> kws), w = n & (kw - 1);
-        if (w < nw) F[((i * NT) << kws) + w] = F[((i * NT + NT - 1) << kws) + w];
-      }
-      __syncthreads();
-    } else if (op == OP_SUM) {
-      // SUMRC over I>=2, K>=2, L>=2, J<=NT-1 (src/ModRamRun.f90:246-255)
-      double acc = 0.0;
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw) continue;
-        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
-        const int q = q0 + w, l = fast_div(q, NE, t.rne), k = q - l * NE;
-        if (i < 1 || k < 1 || l < 1 || j > NT - 2) continue;
-        acc += t.ekev[k] * (F[c] * t.we[k] * t.wmu[l]);
-      }
-      const double tot = block_sum(acc, red);
-      if (threadIdx.x == 0) t.partials[((size_t)prog.arg[o] * t.ns + s) * t.maxblk + tile] = tot;
-      __syncthreads();
-    } else if (op == OP_SEAM) {
-      // F2(:,:,NT,:,:)=F2(:,:,1,:,:) then outside-magnetopause cells to 1e-31
-      // (src/ModRamRun.f90:186,197-201)
-      for (int n = threadIdx.x; n < NR * kw; n += blockDim.x) {
-        const int i = n >> kws, w = n & (kw - 1);
-        if (w < nw) F[((i * NT + NT - 1) << kws) + w] = F[((i * NT) << kws) + w];
-      }
-      __syncthreads();
-      for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int ij = c >> kws, w = c & (kw - 1);
-        if (w >= nw) continue;
-        const int i = fast_div(ij, NT, t.rnt), j = ij - i * NT;
-        if (MGNP(i + 1, j + 1) == 1) F[c] = 1e-31;
-      }
-      __syncthreads();
+#undef CAZIM
+      double v = FN - gN + gprev;
+      if (v < 0) v = 1E-15;
+      fl[(size_t)(NT - 1) * P] = v;
+      fl[0] = v;
     }
-  }
-
-  for (int c = threadIdx.x; c < ncell; c += blockDim.x) {
-    const int ij = c >> kws, w = c & (kw - 1);
-    if (w < nw) gF[(size_t)ij * P + q0 + w] = F[c];
+    if (MODE == 1) {
+      const double m = block_max(cmax, red);
+      if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 1], m);
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
-// KL kernel: one block per (species, I>=2, J), holding the whole (K,L) plane
-// F[l*NE+k] in LDS.  The reference divides both face fluxes of cell K by that
-// cell's width, CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*FBND(K-1)
-// (src/ModRamDrift.f90:364; :460 for Mu), so the owner of face K writes the
-// cell's own partial update O = F(K) - C(K)/WE(K)*FBND(K) and, in H, the term
-// C(K)/WE(K+1)*FBND(K) its upper neighbour adds.
-__device__ __forceinline__ double cex_factor(const Tab &t, int s, int k, size_t q) {
-  // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49)
-  return exp(-(t.sigv[s * t.ne + k] * t.hdns[q] * t.dts));
-}
-__device__ __forceinline__ double atm_factor(const Tab &t, int s, int i, int k, size_t q) {
-  // ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501) as
-  // exp(log(ATLOS)*(1/FNHS))
-  return exp(t.latl[((size_t)s * t.nr + i) * t.ne + k] * t.invfnhs[q]);
-}
+// Plane kernel: one block per (species, I>=2, J), holding the (K,L) plane
+// F[l*NE+k] in LDS; thread n owns cells n, n+512, .. and keeps their
+// partial updates in registers.  The reference divides both face fluxes of
+// cell K by that cell's width, CDriftE(K)/WE(K)*FBND(K) and
+// CDriftE(K-1)/WE(K)*FBND(K-1) (src/ModRamDrift.f90:364; :460 for Mu), so
+// the owner of face K keeps O = F(K) - C(K)/WE(K)*FBND(K) and writes to H
+// the term C(K)/WE(K+1)*FBND(K) its upper neighbour adds.
+//
+// Sequences (template SEQ):
+// (plane layout helpers: sweep_common.h)
 
-__global__ void __launch_bounds__(512) k_kl(Tab t, Program prog, int s0) {
-  extern __shared__ double sm[];
-  const int NE = t.ne, NPA = t.npa, NT = t.nt;
-  const int P = NE * NPA;
-  double *F = sm;
-  double *O = F + P;
-  double *H = O + P;
-  double *red = H + P;  // 16
-  unsigned long long *cmx = (unsigned long long *)(red + 16);  // NE + NPA
-  const int nplane = (t.nr - 1) * NT;
-  const int s = s0 + blockIdx.x / nplane;
-  const int pl = blockIdx.x % nplane;
-  const int i = 1 + pl / NT, j = pl % NT;  // I=i+1>=2
-  double *gF = t.f2 + (size_t)s * f2_species_stride(t) + ((size_t)i * NT + j) * P;
-  const double beta = t.betalim;
-  const size_t qb = ijl_of(t, i, j, 0);
-  const size_t ij = ij_of(t, i, j);
-  const size_t sib = ((size_t)s * t.nr + i) * NE;
-  const bool cex = t.cex[s] != 0;
-  const int upa = t.upa[i];  // int(UPA(I))
-  const bool sum_row = j <= NT - 2;
-  const bool inside = MGNP(i + 1, j + 1) == 0;
-  const double qs = (double)t.charge[s];  // x/QS == x*QS for QS = +-1
-  const double rlz = t.rlz[i], rrlz = t.rrlz[i];
-  const double bij = t.bij[ij], rbij = t.rbij[ij];
-  const float rne = t.rne;
-  unsigned long long *gcfl = t.cflmax + (size_t)s * (2 + NE + NPA);
-
-  for (int c = threadIdx.x; c < P; c += blockDim.x) F[c] = gF[c];
+// DRIFTE on the plane (src/ModRamDrift.f90:333-369).  F(1),F(0) relativistic
+// ghosts from F(2); F(NE+1)=F(NE+2)=0.
+__device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
+  const int NE = c.NE, NPA = c.NPA;
+  {
+    const double *src = t.kle + c.ij * NPA * 8;
+    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
+    if (rec)
+      for (int k = threadIdx.x; k < NE; k += blockDim.x) d.cmx[k] = 0ull;
+  }
   __syncthreads();
-
-  for (int o = 0; o < prog.n; ++o) {
-    const int op = prog.op[o];
-    const bool rec = prog.arg[o] != 0 && inside;
-    if (op == OP_E) {
-      // DRIFTE, src/ModRamDrift.f90:333-369.  F(1),F(0) relativistic ghosts
-      // from F(2); F(NE+1)=F(NE+2)=0.
-      if (rec) {
-        for (int k = threadIdx.x; k < NE; k += blockDim.x) cmx[k] = 0ull;
-        __syncthreads();
+  const double *k8 = t.kij + c.ij * 8;
+  const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
+  const double *rel = t.rel + c.s * 8;
+  const double r0 = rel[0], r1 = rel[1], r2 = rel[2], r3 = rel[3], r4 = rel[4];
+  const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
+  double O[kMaxCellsPerThread];
+#pragma unroll
+  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+    if (n < c.nmine) {
+      CELL_KL(n);
+      const double *e = d.cf + l * 8;
+      // CDriftE(I,J,K,L), :337-342
+      const double edt1 =
+          div_rn(div_rn(div_rn(d.kt[KT_EKP * NE + k], e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
+      const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
+      const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
+      const double ce =
+          d.kt[KT_EDOT * NE + k] * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
+      if (rec) atomic_max_pos(&d.cmx[k], fmax(fabs(ce), 1E-10));
+      const double *row = d.F + l * NE;
+      const double fk = row[k];
+      const double g1 = row[1] * r0 / r1 * r2;
+      const double g0 = g1 * r3 / r0 * r4;
+      const double f0 = (k == 0) ? g1 : fk;
+      const double fp1 = (k + 1 < NE) ? row[k + 1] : 0.0;
+      double fn, fnm1;
+      if (ce < 0) {
+        fn = (k + 2 < NE) ? row[k + 2] : 0.0;
+        fnm1 = fp1;
+      } else {
+        fn = f0;
+        fnm1 = (k == 0) ? g0 : (k == 1) ? g1 : row[k - 1];
       }
-      const double *rel = t.rel + s * 8;
-      const double drd1 = t.drd1[ij], dpd1 = t.dpd1[ij];
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne), k = c - l * NE;
-        const int K = k + 1;
-        const size_t q = qb + l;
-        // CDriftE(I,J,K,L), :337-342
-        const double edt1 = div_rn(div_rn(div_rn(t.ekp[s * NE + k], t.hs[q], t.rhs[q]), rlz, rrlz), bij, rbij) * qs;
-        const double drdt = drd1 + edt1 * t.drd2[q] * rlz;
-        const double dpdt = dpd1 - edt1 * t.dpd2[q];
-        const double ce = t.edot[sib + k] * (t.gr[q] * drdt + t.gp[q] * dpdt + t.db1[q] + t.di1[q]);
-        if (rec) lds_max(&cmx[k], fmax(fabs(ce), 1E-10));
-        const double *row = F + l * NE;
-        const double f2v = row[1];
-        const double g1 = f2v * rel[0] / rel[1] * rel[2];
-        const double g0 = g1 * rel[3] / rel[0] * rel[4];
-#define FE(KK) ((KK) == 0 ? g0 : (KK) == 1 ? g1 : (KK) <= NE ? row[(KK)-1] : 0.0)
-        const double f0 = FE(K), fp1 = FE(K + 1);
-        const double fn = (ce < 0) ? FE(K + 2) : f0;
-        const double fnm1 = (ce < 0) ? fp1 : FE(K - 1);
-#undef FE
-        const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, t.de[k], t.rde[k]), beta);
-        O[c] = row[k] - div_rn(ce, t.we[k], t.rwe[k]) * fb;
-        H[c] = (k + 1 < NE) ? div_rn(ce, t.we[k + 1], t.rwe[k + 1]) * fb : 0.0;
-      }
-      __syncthreads();
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne), k = c - l * NE;
-        if (k >= 1) {
-          double v = O[c] + H[c - 1];
-          if (v < 0) v = 1E-15;
-          F[c] = v;
-        }
-      }
-      if (rec) {
-        __syncthreads();
-        for (int k = threadIdx.x; k < NE; k += blockDim.x)
-          if (cmx[k]) atomicMax(&gcfl[2 + k], cmx[k]);
-      }
-      __syncthreads();
-    } else if (op == OP_MU) {
-      // DRIFTMU, src/ModRamDrift.f90:413-466.  F(1)=F(2) for the stencil;
-      // limiter faces L=2..NPA-2; CDriftMu(1)=0, FBND(1)=0, FBND(NPA-1)=F(NPA).
-      if (rec) {
-        for (int l = threadIdx.x; l < NPA; l += blockDim.x) cmx[NE + l] = 0ull;
-        __syncthreads();
-      }
-      const double drm1 = t.drm1[ij], dpm1 = t.dpm1[ij], db2 = t.db2[ij];
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne), k = c - l * NE;
-        const int L = l + 1;
-        double on = 0.0, h = 0.0;
-        if (L >= 2) {
-          const size_t q = qb + l;
-          // CDriftMu(I,J,K,L), :418-433
-          const double edt = div_rn(div_rn(div_rn(t.mkp[s * NE + k], t.bhs[q], t.rbhs[q]), rlz, rrlz), bij, rbij) * qs;
-          const double drdm = drm1 + edt * t.drm2[q] * rlz;
-          const double dpdm = dpm1 - edt * t.dpm2[q];
-          const double cm = -t.cmudot[q] * (t.gmr[q] * drdm + t.gmp[q] * dpdm + db2 + t.di2[q]);
-          if (rec) lds_max(&cmx[NE + l], fmax(fabs(cm), 1E-32));
-          if (L <= NPA - 1) {
-#define FM(LL) F[(((LL) == 1 ? 2 : (LL)) - 1) * NE + k]
-            double fb;
-            if (L == NPA - 1) {
-              fb = FM(NPA);
-            } else {
-              const double f0 = FM(L), fp1 = FM(L + 1);
-              const double fn = (cm < 0) ? FM(L + 2) : f0;
-              const double fnm1 = (cm < 0) ? fp1 : FM(L - 1);
-              fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, t.dmu[l], t.rdmu[l]), beta);
-            }
-#undef FM
-            on = div_rn(cm, t.wmu[l], t.rwmu[l]) * fb;
-            h = div_rn(cm, t.wmu[l + 1], t.rwmu[l + 1]) * fb;
-          }
-        }
-        // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
-        O[c] = F[c] - on;
-        H[c] = h;
-      }
-      __syncthreads();
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne);
-        if (l >= 1 && l <= NPA - 2) {
-          double v = O[c] + H[c - NE];
-          if (v < 0) v = 1E-15;
-          F[c] = v;
-        }
-      }
-      __syncthreads();
-      // F2(NPA)=F2(NPA-1)*FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1)  (:466)
-      const double hN = GEO3(t.fnhs, i + 1, j + 1, NPA), hN1 = GEO3(t.fnhs, i + 1, j + 1, NPA - 1);
-      for (int k = threadIdx.x; k < NE; k += blockDim.x)
-        F[(NPA - 1) * NE + k] = F[(NPA - 2) * NE + k] * hN * t.mu[NPA - 1] / hN1 / t.mu[NPA - 2];
-      if (rec)
-        for (int l = threadIdx.x; l < NPA; l += blockDim.x)
-          if (cmx[NE + l]) atomicMax(&gcfl[2 + NE + l], cmx[NE + l]);
-      __syncthreads();
-    } else if (op == OP_CEX) {
-      if (cex) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
-        for (int c = threadIdx.x; c < P; c += blockDim.x) {
-          const int l = fast_div(c, NE, rne), k = c - l * NE;
-          if (k >= 1 && l >= 1) F[c] = F[c] * cex_factor(t, s, k, qb + l);
-        }
-      }
-      __syncthreads();
-    } else if (op == OP_ATM) {
-      // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne), k = c - l * NE;
-        if (k >= 1 && l + 1 >= upa) F[c] = F[c] * atm_factor(t, s, i, k, qb + l);
-      }
-      __syncthreads();
-    } else if (op == OP_SUM) {
-      double acc = 0.0;
-      if (sum_row)
-        for (int c = threadIdx.x; c < P; c += blockDim.x) {
-          const int l = fast_div(c, NE, rne), k = c - l * NE;
-          if (k >= 1 && l >= 1) acc += t.ekev[k] * (F[c] * t.we[k] * t.wmu[l]);
-        }
-      const double tot = block_sum(acc, red);
-      if (threadIdx.x == 0) t.partials[((size_t)prog.arg[o] * t.ns + s) * t.maxblk + pl] = tot;
-      __syncthreads();
-    } else if (op == OP_LOSS) {
-      // ram_run :77-141 between the two drift halves: SUMRC, [CEX, SUMRC],
-      // ATMOL, SUMRC, ATMOL, SUMRC, [CEX, SUMRC].  Slots arg..arg+4.
-      double a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
-      for (int c = threadIdx.x; c < P; c += blockDim.x) {
-        const int l = fast_div(c, NE, rne), k = c - l * NE;
-        const bool w_on = sum_row && k >= 1 && l >= 1;
-        const bool cx = cex && k >= 1 && l >= 1;
-        const bool atm = k >= 1 && l + 1 >= upa;
-        double f = F[c];
-        const double cxf = cx ? cex_factor(t, s, k, qb + l) : 1.0;
-        const double af = atm ? atm_factor(t, s, i, k, qb + l) : 1.0;
-        const double wk = t.we[k], wl = t.wmu[l], ek = t.ekev[k];
-        if (w_on) a1 += ek * (f * wk * wl);
-        if (cx) f = f * cxf;
-        if (w_on) a2 += ek * (f * wk * wl);
-        if (atm) f = f * af;
-        if (w_on) a3 += ek * (f * wk * wl);
-        if (atm) f = f * af;
-        if (w_on) a4 += ek * (f * wk * wl);
-        if (cx) f = f * cxf;
-        if (w_on) a5 += ek * (f * wk * wl);
-        F[c] = f;
-      }
-      const double acc[5] = {a1, a2, a3, a4, a5};
-      for (int n = 0; n < 5; ++n) {
-        const double tot = block_sum(acc[n], red);
-        if (threadIdx.x == 0) t.partials[((size_t)(prog.arg[o] + n) * t.ns + s) * t.maxblk + pl] = tot;
-      }
-      __syncthreads();
+      const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, d.kt[KT_DE * NE + k], d.kt[KT_RDE * NE + k]), beta);
+      O[n] = fk - div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * fb;
+      d.H[cc] = (k + 1 < NE) ? div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * fb : 0.0;
     }
   }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+    if (n < c.nmine) {
+      CELL_KL(n);
+      if (k >= 1) {
+        double v = O[n] + d.H[cc - 1];
+        if (v < 0) v = 1E-15;
+        d.F[cc] = v;
+      }
+    }
+  }
+  if (rec) {
+    __syncthreads();
+    unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
+    for (int k = threadIdx.x; k < NE; k += blockDim.x)
+      if (d.cmx[k]) atomicMax(&g[2 + k], d.cmx[k]);
+  }
+  __syncthreads();
+}
 
-  for (int c = threadIdx.x; c < P; c += blockDim.x) gF[c] = F[c];
+// DRIFTMU on the plane (src/ModRamDrift.f90:413-466).  F(1)=F(2) for the
+// stencil; limiter faces L=2..NPA-2; CDriftMu(1)=0, FBND(1)=0,
+// FBND(NPA-1)=F(NPA); top cell F2(NPA)=F2(NPA-1)*FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1).
+__device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
+  const int NE = c.NE, NPA = c.NPA;
+  {
+    const double *src = t.klm + c.ij * NPA * 8;
+    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
+    if (rec)
+      for (int l = threadIdx.x; l < NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
+  }
+  __syncthreads();
+  const double *k8 = t.kij + c.ij * 8;
+  const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
+  const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
+  const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
+  double O[kMaxCellsPerThread];
+#pragma unroll
+  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+    if (n < c.nmine) {
+      CELL_KL(n);
+      const int L1 = l + 1;
+      double on = 0.0, h = 0.0;
+      if (L1 >= 2) {
+        const double *m = d.cf + l * 8;
+        // CDriftMu(I,J,K,L), :418-433
+        const double edt =
+            div_rn(div_rn(div_rn(d.kt[KT_MKP * NE + k], m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij) * qs;
+        const double drdm = drm1 + edt * m[KLM_DRM2] * rlz;
+        const double dpdm = dpm1 - edt * m[KLM_DPM2];
+        const double cm = -m[KLM_CMUDOT] * (m[KLM_GMR] * drdm + m[KLM_GMP] * dpdm + db2 + m[KLM_DI2]);
+        if (rec) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
+        if (L1 <= NPA - 1) {
+          double fb;
+          const double f0 = d.F[cc];
+          if (L1 == NPA - 1) {
+            fb = d.F[cc + NE];  // FBND(NPA-1) = F(NPA)
+          } else {
+            const double fp1 = d.F[cc + NE];
+            double fn, fnm1;
+            if (cm < 0) {
+              fn = d.F[cc + 2 * NE];
+              fnm1 = fp1;
+            } else {
+              fn = f0;
+              fnm1 = (L1 == 2) ? f0 : d.F[cc - NE];  // F(1) = F(2)
+            }
+            fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
+          }
+          on = div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb;
+          h = div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * fb;
+        }
+      }
+      // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
+      O[n] = d.F[cc] - on;
+      d.H[cc] = h;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+    if (n < c.nmine) {
+      CELL_KL(n);
+      if (l >= 1 && l <= NPA - 2) {
+        double v = O[n] + d.H[cc - NE];
+        if (v < 0) v = 1E-15;
+        d.F[cc] = v;
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const double hN = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA), hN1 = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
+    const double muN = t.mu[NPA - 1], muN1 = t.mu[NPA - 2];
+    for (int k = threadIdx.x; k < NE; k += blockDim.x)
+      d.F[(NPA - 1) * NE + k] = d.F[(NPA - 2) * NE + k] * hN * muN / hN1 / muN1;
+  }
+  if (rec) {
+    unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
+    for (int l = threadIdx.x; l < NPA; l += blockDim.x)
+      if (d.cmx[NE + l]) atomicMax(&g[2 + NE + l], d.cmx[NE + l]);
+  }
+  __syncthreads();
+}
+
+// The loss chain of ram_run between the two drift halves (src/ModRamRun.f90:
+// 77-141): SUMRC, [CEX, SUMRC], ATMOL, SUMRC, ATMOL, SUMRC, [CEX, SUMRC], with
+// the SUMRC partials of this plane block b in slots 0..4.
+__device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, int b) {
+  const int NE = c.NE, NPA = c.NPA, NT = t.nt;
+  const bool cex = t.cex[c.s] != 0;
+  const int upa = t.upa[c.i];
+  const bool sum_row = c.j <= NT - 2;
+  double a[5] = {0, 0, 0, 0, 0}, tot[5];
+#pragma unroll
+  for (int n = 0; n < kMaxCellsPerThread; ++n)
+    if (n < c.nmine) {
+      CELL_KL(n);
+      const bool w_on = sum_row && k >= 1 && l >= 1;
+      const bool cx = cex && k >= 1 && l >= 1;
+      const bool atm = k >= 1 && l + 1 >= upa;
+      double f = d.F[cc];
+      const double cxf = cx ? cex_factor(d, NE, k, l, t.dts) : 1.0;
+      const double af = atm ? atm_factor(d, NE, k, l) : 1.0;
+      const double wk = d.kt[KT_WE * NE + k], wl = d.lt[LT_WMU * NPA + l], ek = d.kt[KT_EKEV * NE + k];
+      if (w_on) a[0] += ek * (f * wk * wl);
+      if (cx) f = f * cxf;
+      if (w_on) a[1] += ek * (f * wk * wl);
+      if (atm) f = f * af;
+      if (w_on) a[2] += ek * (f * wk * wl);
+      if (atm) f = f * af;
+      if (w_on) a[3] += ek * (f * wk * wl);
+      if (cx) f = f * cxf;
+      if (w_on) a[4] += ek * (f * wk * wl);
+      d.F[cc] = f;
+    }
+  block_sum_n<5>(a, d.red, tot);
+  if (threadIdx.x < 5) {
+    double v = tot[0];
+    for (int m = 1; m < 5; ++m)
+      if (threadIdx.x == m) v = tot[m];
+    t.partials[((size_t)threadIdx.x * t.ns + c.s) * t.maxblk + b] = v;
+  }
+}
+
+template <int SEQ>
+__global__ void __launch_bounds__(kPlaneThreads, 4) k_plane(Launch L, int rec) {
+  extern __shared__ double sm[];
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int NE = t.ne, NPA = t.npa, NT = t.nt;
+    constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS;
+    PlaneCtx c;
+    PlaneLds d;
+    plane_setup(t, u, sm, t.f2, need_loss, c, d);
+    double *gF = t.f2 + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+    const bool inside = t.kij[c.ij * 8 + KIJ_MG] == 0.0;
+    const bool rec_here = rec && inside;
+
+    if constexpr (SEQ == SEQ_E) {
+      plane_e(t, c, d, rec_here);
+    } else if constexpr (SEQ == SEQ_MU) {
+      plane_mu(t, c, d, rec_here);
+    } else if constexpr (SEQ == SEQ_CEX) {
+      if (t.cex[c.s] != 0) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
+#pragma unroll
+        for (int n = 0; n < kMaxCellsPerThread; ++n)
+          if (n < c.nmine) {
+            CELL_KL(n);
+            if (k >= 1 && l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, k, l, t.dts);
+          }
+      }
+      __syncthreads();
+    } else if constexpr (SEQ == SEQ_ATM) {
+      // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
+      const int upa = t.upa[c.i];
+#pragma unroll
+      for (int n = 0; n < kMaxCellsPerThread; ++n)
+        if (n < c.nmine) {
+          CELL_KL(n);
+          if (k >= 1 && l + 1 >= upa) d.F[cc] = d.F[cc] * atm_factor(d, NE, k, l);
+        }
+      __syncthreads();
+    } else if constexpr (SEQ == SEQ_SUM) {
+      // SUMRC over I>=2, K>=2, L>=2, J<=NT-1 (src/ModRamRun.f90:246-255), slot 0
+      double a[1] = {0.0}, tot[1];
+      if (c.j <= NT - 2) {
+#pragma unroll
+        for (int n = 0; n < kMaxCellsPerThread; ++n)
+          if (n < c.nmine) {
+            CELL_KL(n);
+            if (k >= 1 && l >= 1)
+              a[0] += d.kt[KT_EKEV * NE + k] * (d.F[cc] * d.kt[KT_WE * NE + k] * d.lt[LT_WMU * NPA + l]);
+          }
+      }
+      block_sum_n<1>(a, d.red, tot);
+      if (threadIdx.x == 0) t.partials[((size_t)0 * t.ns + c.s) * t.maxblk + u.b] = tot[0];
+    } else if constexpr (SEQ == SEQ_LOSS) {
+      plane_loss(t, c, d, u.b);
+    } else {  // SEQ_STEP
+      plane_e(t, c, d, rec_here);
+      plane_mu(t, c, d, rec_here);
+      plane_loss(t, c, d, u.b);
+      plane_mu(t, c, d, false);
+      plane_e(t, c, d, false);
+    }
+
+    if constexpr (SEQ != SEQ_SUM) {
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) gF[n] = d.F[n];
+    }
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -899,80 +843,85 @@ __global__ void __launch_bounds__(512) k_kl(Tab t, Program prog, int s0) {
 // use for the next recording.
 // SUMRC bookkeeping (src/ModRamRun.f90:77-175,239-257), summode 1: one SUMRC
 // from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
-// slots 0..5 (slot 5 from the IJ tiles).
-__global__ void k_finalize(Tab t, int cflmask, int summode, int s0, int nblk_kl, int nblk_ij,
-                           double *elorc_out) {
+// slots 0..5 (slots 0..4 from the plane blocks, slot 5 from the R blocks).
+// One block per (problem, species).
+__global__ void k_finalize(Launch L, int cflmask, int summode) {
   __shared__ double red[16];
-  __shared__ double S[6];
-  const int s = s0 + blockIdx.x;
-  const int NE = t.ne, NPA = t.npa;
-  const double a = t.fraccfl * t.dts;
-  unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
-  if (cflmask) {
-    double de = 10000.0, dm = 10000.0;
-    if (cflmask & 4)
-      for (int k = threadIdx.x; k < NE; k += blockDim.x) {
-        const double v = __longlong_as_double((long long)m[2 + k]);
-        if (v > 0) de = fmin(de, a * t.de[k] / v);
+  __shared__ double S[kSumSlots];
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int s = u.s;
+    const int NE = t.ne, NPA = t.npa;
+    const double a = t.fraccfl * t.dts;
+    unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
+    if (cflmask) {
+      double de = 10000.0, dm = 10000.0;
+      if (cflmask & 4)
+        for (int k = threadIdx.x; k < NE; k += blockDim.x) {
+          const double v = __longlong_as_double((long long)m[2 + k]);
+          if (v > 0) de = fmin(de, a * t.de[k] / v);
+        }
+      if (cflmask & 8)
+        for (int l = 1 + threadIdx.x; l < NPA; l += blockDim.x) {
+          const double v = __longlong_as_double((long long)m[2 + NE + l]);
+          if (v > 0) dm = fmin(dm, a * t.dmu[l] / v);
+        }
+      de = block_min(de, red);
+      dm = block_min(dm, red);
+      if (threadIdx.x == 0) {
+        if (cflmask & 1) {
+          const double v = __longlong_as_double((long long)m[0]);
+          t.dtdrift[s * 4 + 0] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
+        }
+        if (cflmask & 2) {
+          const double v = __longlong_as_double((long long)m[1]);
+          t.dtdrift[s * 4 + 1] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
+        }
+        if (cflmask & 4) t.dtdrift[s * 4 + 2] = de;
+        if (cflmask & 8) t.dtdrift[s * 4 + 3] = dm;
       }
-    if (cflmask & 8)
-      for (int l = 1 + threadIdx.x; l < NPA; l += blockDim.x) {
-        const double v = __longlong_as_double((long long)m[2 + NE + l]);
-        if (v > 0) dm = fmin(dm, a * t.dmu[l] / v);
-      }
-    de = block_min(de, red);
-    __syncthreads();
-    dm = block_min(dm, red);
-    if (threadIdx.x == 0) {
-      if (cflmask & 1) {
-        const double v = __longlong_as_double((long long)m[0]);
-        t.dtdrift[s * 4 + 0] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
-      }
-      if (cflmask & 2) {
-        const double v = __longlong_as_double((long long)m[1]);
-        t.dtdrift[s * 4 + 1] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
-      }
-      if (cflmask & 4) t.dtdrift[s * 4 + 2] = de;
-      if (cflmask & 8) t.dtdrift[s * 4 + 3] = dm;
+      __syncthreads();
+      for (int e = threadIdx.x; e < 2 + NE + NPA; e += blockDim.x) m[e] = 0ull;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 2 + NE + NPA; e += blockDim.x) m[e] = 0ull;
-  }
-  if (summode == 0) return;
-  const int nslot = (summode == 1) ? 1 : 6;
-  for (int sl = 0; sl < nslot; ++sl) {
-    const int nb = (summode == 2 && sl == 5) ? nblk_ij : nblk_kl;
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += t.partials[((size_t)sl * t.ns + s) * t.maxblk + b];
-    const double tot = block_sum(acc, red);
-    if (threadIdx.x == 0) S[sl] = tot;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    double *e = t.energy + s * 4;
-    const double old = e[0];
-    if (summode == 1) {
-      e[0] = S[0];
-      if (elorc_out) elorc_out[s] = old - S[0];
-    } else {
-      const bool cex = t.cex[s] != 0;
-      double prev = old;
-      e[1] += prev - S[0];  // LSDR
-      prev = S[0];
-      if (cex) {
-        e[2] += prev - S[1];  // LSCHA
-        prev = S[1];
+    if (summode != 0) {
+      const int nslot = (summode == 1) ? 1 : kSumSlots;
+      for (int sl = 0; sl < nslot; ++sl) {
+        const int nb = (summode == 2 && sl == 5) ? blocks_r(t) : blocks_kl(t);
+        double acc[1] = {0.0}, tot[1];
+        for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += t.partials[((size_t)sl * t.ns + s) * t.maxblk + b];
+        block_sum_n<1>(acc, red, tot);
+        if (threadIdx.x == 0) S[sl] = tot[0];
       }
-      e[3] += prev - S[2];  // LSATM
-      prev = S[2];
-      e[3] += prev - S[3];  // LSATM
-      prev = S[3];
-      if (cex) {
-        e[2] += prev - S[4];  // LSCHA
-        prev = S[4];
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double *e = t.energy + s * 4;
+        const double old = e[0];
+        if (summode == 1) {
+          e[0] = S[0];
+          t.partials[((size_t)kSumSlots * t.ns + s) * t.maxblk] = old - S[0];  // ELORC
+        } else {
+          const bool cex = t.cex[s] != 0;
+          double prev = old;
+          e[1] += prev - S[0];  // LSDR
+          prev = S[0];
+          if (cex) {
+            e[2] += prev - S[1];  // LSCHA
+            prev = S[1];
+          }
+          e[3] += prev - S[2];  // LSATM
+          prev = S[2];
+          e[3] += prev - S[3];  // LSATM
+          prev = S[3];
+          if (cex) {
+            e[2] += prev - S[4];  // LSCHA
+            prev = S[4];
+          }
+          e[1] += prev - S[5];  // LSDR
+          e[0] = S[5];
+        }
       }
-      e[1] += prev - S[5];  // LSDR
-      e[0] = S[5];
+      __syncthreads();
     }
   }
 }
@@ -988,6 +937,36 @@ __global__ void k_vs(double *vt, const double *lz, const double *phi, const doub
     const double lre = lz[i] * kRE;
     vt[n] = avs * (lre * lre) * sin(phi[j] - 0.0);
   }
+}
+
+// One step of an ensemble's schedule: member m gets VT for kp[step*M+m] (the
+// formula of k_vs) and boundary scales bscale[step*stride + bs_off[m] + s].
+// One unit per member.
+__global__ void k_sched(Launch L, const double *kp, const double *bscale, const int *bs_off, const int *step,
+                        int nmembers, int stride_bs) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int m = (int)(u.t - L.tabs);
+    const int n = *step;
+    const double kpv = kp[(size_t)n * nmembers + m];
+    const double d = 1.0 - 0.159 * kpv + 0.0093 * (kpv * kpv);
+    const double avs = 7.05E-6 / (d * d * d) / kRE;
+    double *vt = const_cast<double *>(t.vt);
+    const int nr1 = t.nr + 1;
+    for (int x = threadIdx.x; x < nr1 * t.nt; x += blockDim.x) {
+      const int i = x % nr1, j = x / nr1;
+      const double lre = t.lz[i] * kRE;
+      vt[x] = avs * (lre * lre) * sin(t.phi[j] - 0.0);
+    }
+    if (bscale)
+      for (int s = threadIdx.x; s < t.ns; s += blockDim.x)
+        t.bscale[s] = bscale[(size_t)n * stride_bs + bs_off[m] + s];
+  }
+}
+
+__global__ void k_tick(int *step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = step[0] + 1;
 }
 
 // Plain streaming copy (16-byte accesses), the bandwidth and counter
@@ -1031,5 +1010,19 @@ __global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt,
     dst[s + (size_t)ns * (i + (size_t)nr * (j + (size_t)nt * (k + (size_t)ne * l)))] = src[x];
   }
 }
+
+// explicit instantiations used by the host code
+template __global__ void k_r<0>(Launch);
+template __global__ void k_r<1>(Launch);
+template __global__ void k_r<2>(Launch);
+template __global__ void k_p<0>(Launch);
+template __global__ void k_p<1>(Launch);
+template __global__ void k_plane<SEQ_E>(Launch, int);
+template __global__ void k_plane<SEQ_MU>(Launch, int);
+template __global__ void k_plane<SEQ_CEX>(Launch, int);
+template __global__ void k_plane<SEQ_ATM>(Launch, int);
+template __global__ void k_plane<SEQ_SUM>(Launch, int);
+template __global__ void k_plane<SEQ_STEP>(Launch, int);
+template __global__ void k_plane<SEQ_LOSS>(Launch, int);
 
 }  // namespace ramgpu

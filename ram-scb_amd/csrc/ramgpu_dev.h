@@ -1,17 +1,18 @@
 // ramgpu_dev.h -- device-side data of the RAM split step (gfx950).
 //
 // F2 is kept on the device per species in a [I][J][L][K] layout (K fastest):
-// the (K,L) plane of one (I,J) cell is one contiguous run of NE*NPA doubles,
-// so the energy and pitch-angle sweeps (plus the pointwise loss operators
-// that sit between them) work on whole contiguous planes, and the radial and
-// local-time sweeps work on tiles of KW consecutive (L,K) positions across
-// the full (I,J) plane.
+// the (K,L) plane of one (I,J) cell is one contiguous run of NE*NPA doubles.
+// The radial and local-time sweeps run one thread per line with consecutive
+// lanes on consecutive (L,K) positions, so every load and store of a wave is
+// one contiguous run; the energy and pitch-angle sweeps (plus the pointwise
+// loss operators between them) stage whole (K,L) planes in LDS.
 //
 // Coefficient tables hold the reference's intermediate quantities (its
 // GPA, GPR1.., DRD1.., CMUDOT.. terms) computed in the reference's expression
 // order, plus correctly rounded reciprocals of every divisor that is reused
 // across cells, so that each Courant number is rebuilt per cell with exactly
-// the reference's sequence of roundings.
+// the reference's sequence of roundings.  Tables read together are packed
+// together (one base address, consecutive fields).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,29 +22,20 @@
 namespace ramgpu {
 
 constexpr int kMaxSpecies = 8;
-constexpr int kMaxOps = 16;
+constexpr int kLineThreads = 256;      // R/P sweep kernels: one line per thread
+constexpr int kPlaneThreads = 512;     // plane kernels: one (K,L) plane per block
+constexpr int kMaxCellsPerThread = 12; // NE*NPA <= kPlaneThreads*kMaxCellsPerThread
+constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
 
-// sweep-program op codes
-enum Op : int {
-  OP_NONE = 0,
-  OP_R = 1,     // DRIFTR   (IJ kernel); arg=1 records the CFL maxima
-  OP_P = 2,     // DRIFTP   (IJ kernel); arg=1 records the CFL maxima
-  OP_SEAM = 3,  // F2(:,:,NT)=F2(:,:,1) and magnetopause mask (IJ kernel)
-  OP_E = 4,     // DRIFTE   (KL kernel); arg=1 records the CFL maxima
-  OP_MU = 5,    // DRIFTMU  (KL kernel); arg=1 records the CFL maxima
-  OP_CEX = 6,   // CHAREXCHANGE (KL kernel)
-  OP_ATM = 7,   // ATMOL    (KL kernel)
-  OP_SUM = 8,   // SUMRC partial sum into slot `arg` (either kernel)
-  OP_LOSS = 9,  // fused SUMRC,CEX,SUMRC,ATMOL,SUMRC,ATMOL,SUMRC,CEX,SUMRC chain
-};
+// packed per-(I,J) terms of the plane sweeps, kij[(i*NT+j)*8 + ..]
+enum KijField { KIJ_DRD1, KIJ_DPD1, KIJ_BIJ, KIJ_RBIJ, KIJ_DRM1, KIJ_DPM1, KIJ_DB2, KIJ_MG };
+// packed per-(I,J,L) DRIFTE terms, kle[((i*NT+j)*NPA+l)*8 + ..]
+enum KleField { KLE_HS, KLE_RHS, KLE_DRD2, KLE_DPD2, KLE_GR, KLE_GP, KLE_DB1, KLE_DI1 };
+// packed per-(I,J,L) DRIFTMU terms, klm[((i*NT+j)*NPA+l)*8 + ..]
+enum KlmField { KLM_BHS, KLM_RBHS, KLM_DRM2, KLM_DPM2, KLM_CMUDOT, KLM_GMR, KLM_GMP, KLM_DI2 };
 
-struct Program {
-  int n;
-  int op[kMaxOps];
-  int arg[kMaxOps];
-};
-
-// All device pointers and scalars one launch needs (passed by value).
+// All device pointers and scalars of one problem (one ensemble member).
+// Lives in device memory; kernels read it through a pointer.
 struct Tab {
   int ns, nr, nt, ne, npa;
   double dts, betalim, fraccfl, mdr, dphi, conf1, conf2;
@@ -60,24 +52,20 @@ struct Tab {
   const double *bnes, *dbdt, *vt, *eir, *eip;
   const int *mgnp;
   const double *fgeos;   // (NS,NT,NE,NPA)
-  const double *bscale;  // (NS)
+  double *bscale;        // (NS)
+  const double *lz, *phi;  // LZ(NR+1), PHI(NT) for the device-side VT of a Kp schedule
   // reciprocals of grid widths: 1/DE(K), 1/WE(K), 1/DMU(L), 1/WMU(L)
   double *rde, *rwe, *rdmu, *rwmu;
-  // per I
-  double *rrlz;        // 1/RLZ(I)
-  double *rsum, *rrsum;  // RLZ(I)+0.5*MDR and reciprocal (DRIFTR)
-  // per (I,J), index i*NT+j
-  double *cr, *bsr, *rbsr;          // DRIFTR: CR, BNES(I,J)+BNES(I+1,J), 1/..
-  double *pa, *pce, *pbs, *rpbs;    // DRIFTP: VT-term*P1, EIR term, BNES sum, 1/..
-  double *drd1, *dpd1, *bij, *rbij; // DRIFTE DRD1/DPD1, BNES(I,J), 1/BNES(I,J)
-  double *drm1, *dpm1, *db2;        // DRIFTMU DRM1/DPM1/dBdt2
-  // per (L,I,J), index (l*NR+i)*NT+j  -- read by the (I,J)-plane sweeps
-  double *q1;                       // DRIFTR CGR3/(FNHS(I)+FNHS(I+1))
-  double *gpa, *fhs, *rfhs;         // DRIFTP GPA1+GPA2, FNHS(J)+FNHS(J1), 1/..
-  // per (I,J,L), index (i*NT+j)*NPA+l  -- read by the (K,L)-plane sweeps
-  double *gr, *gp, *drd2, *dpd2, *db1, *di1, *hs, *rhs;        // DRIFTE
-  double *cmudot, *gmr, *gmp, *drm2, *dpm2, *di2, *bhs, *rbhs;  // DRIFTMU
-  double *hdns, *invfnhs;                                       // loss
+  double *rrlz;    // [i] 1/RLZ(I)
+  double2 *rsi;    // [i] {RLZ(I)+0.5*MDR, 1/..}                       (DRIFTR)
+  double4 *rij;    // [i*NT+j] {CR, BNES(I,J)+BNES(I+1,J), 1/.., outsideMGNP(I,J)}
+  double *q1;      // [(l*NR+i)*NT+j] CGR3/(FNHS(I)+FNHS(I+1))         (DRIFTR)
+  double4 *pij;    // [i*NT+j] {VT term*P1, EIR term, BNES(I,J)+BNES(I,J1), 1/..}
+  double4 *pl;     // [(l*NR+i)*NT+j] {GPA1+GPA2, FNHS(J)+FNHS(J1), 1/.., outsideMGNP(I,J)}
+  double *kij;     // [(i*NT+j)*8] KijField
+  double *kle;     // [((i*NT+j)*NPA+l)*8] KleField
+  double *klm;     // [((i*NT+j)*NPA+l)*8] KlmField
+  double2 *kll;    // [(i*NT+j)*NPA+l] {HDNS, 1/FNHS}                   (loss)
   // per species
   double *p4;     // [s][k] DRIFTR P4
   double *ekp;    // [s][k] EBND*1e3*(GRBND+1)/2/GRBND (DRIFTE EDT1 prefix)
@@ -95,7 +83,7 @@ struct Tab {
   unsigned long long *cflmax;
   double *dtdrift;      // [s][4]
   // float reciprocals for exact small-integer division (fast_div)
-  float rnt, rne;
+  float rnt, rne, rp;   // 1/NT, 1/NE, 1/(NE*NPA)
   // state
   double *f2;
   double *partials;  // [slot][s][maxblk]
@@ -103,8 +91,40 @@ struct Tab {
   double *energy;    // [s][4]: SETRC, LSDR, LSCHA, LSATM
 };
 
+// One unit of a launch: blocks [b0, b0+nb) of species s of problem tabs[tab].
+struct Job {
+  int tab, s, b0, nb;
+};
+
+// A launch over several units.  With vmap == nullptr the launch covers
+// species s0, s0+1, .. of tabs[0], nb blocks each; otherwise vmap[vb] is the
+// job of virtual block vb.  Blocks loop over virtual blocks with stride
+// gridDim.x, so the grid may be smaller than nv.
+struct Launch {
+  const Tab *tabs;
+  const Job *jobs;
+  const int *vmap;
+  int nv, nb, s0;
+};
+
 inline __host__ __device__ size_t f2_species_stride(const Tab &t) {
   return (size_t)t.nr * t.nt * t.npa * t.ne;
+}
+
+// block counts of the kernels, per (problem, species)
+inline __host__ __device__ int blocks_r(const Tab &t) {
+  return (t.nt * t.ne * t.npa + kLineThreads - 1) / kLineThreads;
+}
+inline __host__ __device__ int blocks_p(const Tab &t) {
+  return ((t.nr - 1) * t.ne * t.npa + kLineThreads - 1) / kLineThreads;
+}
+inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.nt; }
+
+// LDS bytes of the plane kernel
+inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
+  const size_t P = (size_t)ne * npa;
+  return (2 * P + 8 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 64 +
+          (size_t)ne + npa) * sizeof(double);
 }
 
 }  // namespace ramgpu

@@ -1,0 +1,326 @@
+// sweep_common.h -- device helpers shared by the forward kernels
+// (kernels.hip) and their adjoints (adjoint.hip): exact-rounding arithmetic,
+// the limited face value, launch units, block reductions, the DRIFTR
+// Courant number and ghosts, and the LDS layout of the plane kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ramgpu_dev.h"
+
+namespace ramgpu {
+
+static constexpr double kOME = 7.3E-5;      // src/ModRamDrift.f90:224
+static constexpr double kQ = 1.602E-19;     // src/ModRamConst.f90:19
+static constexpr double kCS = 2.998E8;      // src/ModRamConst.f90:20
+static constexpr double kRE = 6.371E6;      // src/ModRamConst.f90:14
+
+// geometry accessors, 1-based as the reference
+#define GEO3(A, I, J, L) (A)[((I)-1) + (size_t)(t.nr + 1) * (((J)-1) + (size_t)t.nt * ((L)-1))]
+#define GEO2(A, I, J) (A)[((I)-1) + (size_t)(t.nr + 1) * ((J)-1)]
+#define MGNP(I, J) t.mgnp[((I)-1) + (size_t)t.nr * ((J)-1)]
+
+// n/d for 0 <= n < 2^22, d >= 1, from rd = RN_float(1/d): the float
+// quotient is within one of the true one, and one remainder test fixes it.
+__device__ __forceinline__ int fast_div(int n, int d, float rd) {
+  int q = __float2int_rz((float)n * rd);
+  const int r = n - q * d;
+  if (r < 0) --q;
+  else if (r >= d) ++q;
+  return q;
+}
+
+// a/d, correctly rounded, from rd = RN(1/d): q = RN(a*rd) lies within one ulp
+// of a/d, the residual a - d*q is exact under FMA, and one correction step
+// rounds to RN(a/d) (Markstein, IBM J. Res. Dev. 34(1), 1990).
+__device__ __forceinline__ double div_rn(double a, double d, double rd) {
+  const double q = a * rd;
+  const double r = fma(-q, d, a);
+  return fma(r, rd, q);
+}
+
+// Limited face value of the reference's upwind scheme (src/ModRamDrift.f90:
+// 169-183) for the face between F(I) = f0 and F(I+1) = fp1.  fn and fnm1 are
+// F(N) and F(N-1) with N = I+1-sgn as the reference forms it (including its
+// periodic wrap in DRIFTP); cn is the Courant number as the limiter's CORR
+// term sees it (C, C/DE(K) or C/DMU(L)).
+__device__ __forceinline__ double face_value(double f0, double fp1, double fn, double fnm1,
+                                             double c, double cn, double beta) {
+  const double sg = (c < 0) ? -1.0 : 1.0;
+  const double x = fp1 - f0;
+  const double fup = 0.5 * (f0 + fp1 - sg * x);
+  if (fabs(x) <= 1.E-27) return fup;
+  const double r = (fn - fnm1) / x;
+  if (r <= 0) return fup;
+  const double lim = fmax(fmin(beta * r, 1.0), fmin(r, beta));
+  const double corr = -0.5 * (cn - sg) * x;
+  return fup + lim * corr;
+}
+
+// The same face value with its partial derivatives with respect to the four
+// stencil values (reverse mode).  fb = FUP + lim(R)*CORR is piecewise linear:
+// on each piece lim = a + b*R, so with kap = -0.5*(cn-sg), X = fp1-f0,
+//   dfb/dfn = kap*b,  dfb/dfnm1 = -kap*b,  dfb/dX = kap*a,
+//   dfb/df0 = (1+sg)/2 - kap*a,  dfb/dfp1 = (1-sg)/2 + kap*a.
+// Subgradient at the kinks of lim = max(min(beta*R,1), min(R,beta)): a tie
+// of min or max takes the derivative of its first argument; R <= 0 and
+// |X| <= 1e-27 take the upwind branch (lim = 0), as the forward does.
+struct FaceD {
+  double fb, d0, d1, dn, dm;
+};
+
+__device__ __forceinline__ FaceD face_d(double f0, double fp1, double fn, double fnm1, double c, double cn,
+                                        double beta) {
+  const double sg = (c < 0) ? -1.0 : 1.0;
+  const double x = fp1 - f0;
+  const double fup = 0.5 * (f0 + fp1 - sg * x);
+  FaceD r;
+  r.fb = fup;
+  r.d0 = 0.5 * (1.0 + sg);
+  r.d1 = 0.5 * (1.0 - sg);
+  r.dn = 0.0;
+  r.dm = 0.0;
+  if (fabs(x) <= 1.E-27) return r;
+  const double rr = (fn - fnm1) / x;
+  if (rr <= 0) return r;
+  const double A = fmin(beta * rr, 1.0), B = fmin(rr, beta);
+  const double lim = fmax(A, B);
+  double a, b;
+  if (A >= B) {
+    if (beta * rr <= 1.0) { a = 0.0; b = beta; }
+    else { a = 1.0; b = 0.0; }
+  } else {
+    if (rr <= beta) { a = 0.0; b = 1.0; }
+    else { a = beta; b = 0.0; }
+  }
+  const double kap = -0.5 * (cn - sg);
+  r.fb = fup + lim * (kap * x);
+  r.d0 -= kap * a;
+  r.d1 += kap * a;
+  r.dn = kap * b;
+  r.dm = -kap * b;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Launch units
+struct Unit {
+  const Tab *t;
+  int s, b;
+};
+
+__device__ __forceinline__ Unit unit_of(const Launch &L, int vb) {
+  if (!L.vmap) {
+    const int n = vb / L.nb;
+    return {L.tabs, L.s0 + n, vb - n * L.nb};
+  }
+  const Job jb = L.jobs[L.vmap[vb]];
+  return {L.tabs + jb.tab, jb.s, vb - jb.b0};
+}
+
+// ---------------------------------------------------------------------------
+// Block reductions (fixed order: deterministic for a given launch shape).
+template <int N>
+__device__ inline void block_sum_n(double (&v)[N], double *red, double (&out)[N]) {
+  for (int m = 0; m < N; ++m)
+    for (int off = 32; off > 0; off >>= 1) v[m] += __shfl_down(v[m], off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0)
+    for (int m = 0; m < N; ++m) red[wid * N + m] = v[m];
+  __syncthreads();
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int m = 0; m < N; ++m) {
+    double tot = 0.0;
+    for (int w = 0; w < nw; ++w) tot += red[w * N + m];
+    out[m] = tot;
+  }
+  __syncthreads();
+}
+
+__device__ inline double block_max(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double m = 0.0;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < nw; ++w) m = fmax(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+__device__ inline double block_min(double v, double *red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double m = v;
+  const int nw = (blockDim.x + 63) >> 6;
+  for (int w = 0; w < nw; ++w) m = fmin(m, red[w]);
+  __syncthreads();
+  return m;
+}
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long *slot, double v) {
+  atomicMax(slot, (unsigned long long)__double_as_longlong(v));  // v >= 0
+}
+
+// Radial Courant number CDriftR(I,J,K,L) = CR(I,J) + CGR with
+// CGR = CGR3/(FNHS+FNHS')*P4/2./(BNES+BNES')/(RLZ+0.5*MDR)
+// (src/ModRamDrift.f90:140-146).  One function for the sweep, the CFL and the
+// ghost carry, so the inflow/outflow decision is the same everywhere.
+__device__ __forceinline__ double c_radial(const Tab &t, double p4k, int i, int j, int l) {
+  const double4 r = t.rij[(size_t)i * t.nt + j];
+  const double2 rs = t.rsi[i];
+  const double x = (t.q1[((size_t)l * t.nr + i) * t.nt + j] * p4k) * 0.5;
+  return r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
+}
+
+__device__ __forceinline__ bool line_inflow(const Tab &t, int s, int j, int k, int l) {
+  return c_radial(t, t.p4[s * t.ne + k], t.nr - 1, j, l) < 0;
+}
+
+// F(NR+1) (conf=CONF1) or F(NR+2) (conf=CONF2) of inflow line (J,K,L):
+// 0 outside the magnetopause, else FGEOS*CONF*FNHS(NR,J,L) (:161-167).
+__device__ __forceinline__ double ghost_value(const Tab &t, int s, int j, int k, int l, double conf) {
+  const int NR = t.nr;
+  if (t.mgnp[(NR - 1) + (size_t)NR * j] == 1) return 0.0;
+  const double fg = t.fgeos[s + (size_t)t.ns * (j + (size_t)t.nt * (k + (size_t)t.ne * l))] * t.bscale[s];
+  return fg * conf * t.fnhs[(NR - 1) + (size_t)(NR + 1) * (j + (size_t)t.nt * l)];
+}
+
+__device__ inline double stale_ghost(const Tab &t, int s, int j, int k, int l) {
+  for (int jj = j - 1; jj >= 0; --jj)
+    if (line_inflow(t, s, jj, k, l)) return ghost_value(t, s, jj, k, l, t.conf1);
+  const int p = k * t.npa + l;
+  const int c = t.carry[(size_t)s * t.ne * t.npa + p];
+  if (c < 0) return 0.0;
+  const int pp = c / t.nt, jj = c % t.nt;
+  return ghost_value(t, s, jj, pp / t.npa, pp % t.npa, t.conf1);
+}
+
+// The line (J',K',L') whose F(NR+1) stale_ghost reads, as its FGEOS index
+// s + NS*(J' + NT*(K' + NE*L')), or -1 when it reads 0.
+__device__ inline long stale_ghost_source(const Tab &t, int s, int j, int k, int l) {
+  for (int jj = j - 1; jj >= 0; --jj)
+    if (line_inflow(t, s, jj, k, l)) return s + (long)t.ns * (jj + (long)t.nt * (k + (long)t.ne * l));
+  const int c = t.carry[(size_t)s * t.ne * t.npa + k * t.npa + l];
+  if (c < 0) return -1;
+  const int pp = c / t.nt, jj = c % t.nt;
+  return s + (long)t.ns * (jj + (long)t.nt * (pp / t.npa + (long)t.ne * (pp % t.npa)));
+}
+
+// d F(NR+1) / d FGEOS(S,J,K,L) of an inflow line (conf = CONF1 or CONF2)
+__device__ __forceinline__ double ghost_dfgeos(const Tab &t, int s, int j, int l, double conf) {
+  const int NR = t.nr;
+  if (t.mgnp[(NR - 1) + (size_t)NR * j] == 1) return 0.0;
+  return t.bscale[s] * conf * t.fnhs[(NR - 1) + (size_t)(NR + 1) * (j + (size_t)t.nt * l)];
+}
+
+enum PlaneSeq : int {
+  SEQ_E = 0,     // DRIFTE
+  SEQ_MU = 1,    // DRIFTMU
+  SEQ_CEX = 2,   // CHAREXCHANGE
+  SEQ_ATM = 3,   // ATMOL
+  SEQ_SUM = 4,   // SUMRC partial sum, slot 0
+  SEQ_STEP = 5,  // the middle of ram_run: DRIFTE, DRIFTMU, loss chain, DRIFTMU, DRIFTE
+  SEQ_LOSS = 6,  // the loss chain alone (CEX, ATMOL, ATMOL, CEX with five SUMRC partials)
+};
+
+// LDS layout of the plane kernel (doubles)
+struct PlaneLds {
+  double *F, *H, *cf, *cl, *kt, *lt, *red;
+  unsigned long long *cmx;
+};
+enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
+enum LtField { LT_DMU, LT_RDMU, LT_WMU, LT_RWMU, LT_N };
+
+__device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
+  const int P = ne * npa;
+  PlaneLds d;
+  d.F = sm;
+  d.H = d.F + P;
+  d.cf = d.H + P;            // 8*NPA: current phase's per-L coefficients
+  d.cl = d.cf + 8 * npa;     // 2*NPA: HDNS, 1/FNHS
+  d.kt = d.cl + 2 * npa;     // KT_N*NE
+  d.lt = d.kt + KT_N * ne;   // LT_N*NPA
+  d.red = d.lt + LT_N * npa; // 64
+  d.cmx = (unsigned long long *)(d.red + 64);  // NE+NPA
+  return d;
+}
+
+struct PlaneCtx {
+  int s, i, j, NE, NPA, P, nmine;
+  size_t ij;
+  double rlz, rrlz, qs, beta;
+  float rne;
+};
+
+// (K,L) of the n-th cell owned by this thread: cell index threadIdx.x + n*512
+#define CELL_KL(n)                                          \
+  const int cc = threadIdx.x + (n) * kPlaneThreads;          \
+  const int l = fast_div(cc, c.NE, c.rne), k = cc - l * c.NE; \
+  (void)k
+
+// Block set-up of the plane kernels: context of plane b of species s (I>=2)
+// and the LDS copies of the plane (from src, device layout of one species),
+// of the per-K / per-L tables and, with need_loss, of HDNS and 1/FNHS.
+__device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, const double *src, bool need_loss,
+                                   PlaneCtx &c, PlaneLds &d) {
+  const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr;
+  c.s = u.s;
+  c.i = 1 + u.b / NT;  // I = i+1 >= 2
+  c.j = u.b - (c.i - 1) * NT;
+  c.NE = NE;
+  c.NPA = NPA;
+  c.P = NE * NPA;
+  c.ij = (size_t)c.i * NT + c.j;
+  c.rlz = t.rlz[c.i];
+  c.rrlz = t.rrlz[c.i];
+  c.qs = (double)t.charge[c.s];  // x/QS == x*QS for QS = +-1
+  c.beta = t.betalim;
+  c.rne = t.rne;
+  c.nmine = (c.P - (int)threadIdx.x + kPlaneThreads - 1) / kPlaneThreads;  // cells tid, tid+512, .. < P
+  d = plane_lds(sm, NE, NPA);
+  const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+  for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = gF[n];
+  const size_t sk = (size_t)c.s * NE, sik = ((size_t)c.s * NR + c.i) * NE;
+  for (int k = threadIdx.x; k < NE; k += blockDim.x) {
+    d.kt[KT_EKP * NE + k] = t.ekp[sk + k];
+    d.kt[KT_MKP * NE + k] = t.mkp[sk + k];
+    d.kt[KT_SIGV * NE + k] = t.sigv[sk + k];
+    d.kt[KT_EDOT * NE + k] = t.edot[sik + k];
+    d.kt[KT_LATL * NE + k] = t.latl[sik + k];
+    d.kt[KT_DE * NE + k] = t.de[k];
+    d.kt[KT_RDE * NE + k] = t.rde[k];
+    d.kt[KT_WE * NE + k] = t.we[k];
+    d.kt[KT_RWE * NE + k] = t.rwe[k];
+    d.kt[KT_EKEV * NE + k] = t.ekev[k];
+  }
+  for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
+    d.lt[LT_DMU * NPA + l] = t.dmu[l];
+    d.lt[LT_RDMU * NPA + l] = t.rdmu[l];
+    d.lt[LT_WMU * NPA + l] = t.wmu[l];
+    d.lt[LT_RWMU * NPA + l] = t.rwmu[l];
+    if (need_loss) {
+      const double2 hv = t.kll[c.ij * NPA + l];
+      d.cl[2 * l] = hv.x;
+      d.cl[2 * l + 1] = hv.y;
+    }
+  }
+  __syncthreads();
+}
+
+// CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49)
+__device__ __forceinline__ double cex_factor(const PlaneLds &d, int NE, int k, int l, double dts) {
+  return exp(-(d.kt[KT_SIGV * NE + k] * d.cl[2 * l] * dts));
+}
+// ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501) as exp(log(ATLOS)*(1/FNHS))
+__device__ __forceinline__ double atm_factor(const PlaneLds &d, int NE, int k, int l) {
+  return exp(d.kt[KT_LATL * NE + k] * d.cl[2 * l + 1]);
+}
+
+}  // namespace ramgpu

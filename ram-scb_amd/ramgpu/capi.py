@@ -28,9 +28,14 @@ EXPORTS = (
     "ramgpu_cepara", "ramgpu_charexchange", "ramgpu_atmol", "ramgpu_sumrc", "ramgpu_ram_run",
     "ramgpu_ram_run_schedule", "ramgpu_last_run_ms", "ramgpu_kernel_timing",
     "ramgpu_kernel_times", "ramgpu_copy_probe",
+    "ramgpu_batch_create", "ramgpu_batch_destroy", "ramgpu_batch_ram_run",
+    "ramgpu_batch_ram_run_schedule", "ramgpu_batch_last_run_ms", "ramgpu_batch_use_graph",
+    "ramgpu_batch_autotune", "ramgpu_batch_get_shapes", "ramgpu_batch_set_shapes",
+    "ramgpu_driftr_bwd", "ramgpu_driftp_bwd", "ramgpu_drifte_bwd", "ramgpu_driftmu_bwd",
+    "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
 )
 
-KERNEL_KINDS = ("prepare", "ij", "kl", "finalize", "vs")
+KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p")
 
 
 class SpeciesOut(C.Structure):
@@ -87,6 +92,21 @@ def load_library(path: Path | None = None):
     lib.ramgpu_kernel_timing.argtypes = [v, C.c_int]
     lib.ramgpu_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
     lib.ramgpu_copy_probe.argtypes = [C.c_int, C.c_size_t, C.c_int, _dp]
+    lib.ramgpu_batch_create.argtypes = [pp, C.POINTER(C.c_void_p), C.c_int]
+    lib.ramgpu_batch_destroy.argtypes = [v]
+    lib.ramgpu_batch_ram_run.argtypes = [v, C.c_int, C.POINTER(SpeciesOut)]
+    lib.ramgpu_batch_ram_run_schedule.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
+    lib.ramgpu_batch_last_run_ms.argtypes = [v, _dp]
+    lib.ramgpu_batch_use_graph.argtypes = [v, C.c_int]
+    lib.ramgpu_batch_autotune.argtypes = [v, C.c_int, _dp]
+    lib.ramgpu_batch_get_shapes.argtypes = [v, _ip]
+    lib.ramgpu_batch_set_shapes.argtypes = [v, _ip]
+    lib.ramgpu_driftr_bwd.argtypes = [v, C.c_int, _dp, _dp]
+    for n in ("driftp", "drifte", "driftmu", "charexchange", "atmol"):
+        getattr(lib, f"ramgpu_{n}_bwd").argtypes = [v, C.c_int, _dp]
+    lib.ramgpu_ram_run_bwd.argtypes = [v, _dp, _dp]
+    lib.ramgpu_adjoint_run.argtypes = [v, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp,
+                                       _dp, _dp]
     for n in EXPORTS:
         if n != "ramgpu_last_error":
             getattr(lib, n).restype = C.c_int
@@ -281,6 +301,57 @@ class RamGpu:
                                                      None if bs is None else _p(bs), out))
         return [o.as_dict() for o in out]
 
+    # -- reverse mode ------------------------------------------------------
+    def op_bwd(self, op, s, adj, fgeos_adj=None):
+        """dL/d(input) of operator `op` (reference name) for species s at the
+        current F2, from adj = dL/d(output) (NS,NR,NT,NE,NPA).  DRIFTR also
+        returns dL/dFGEOS (NS,NT,NE,NPA) accumulated onto fgeos_adj."""
+        a = np.array(adj, dtype=np.float64, order="F", copy=True)
+        if a.shape != self.dims:
+            raise ValueError("adj must have the shape of F2")
+        if op == "driftr":
+            g = np.zeros((self.ns, self.nt, self.ne, self.npa), order="F") if fgeos_adj is None else \
+                np.array(fgeos_adj, dtype=np.float64, order="F", copy=True)
+            self._check(self.lib.ramgpu_driftr_bwd(self.h, s, _p(a), _p(g)))
+            return a, g
+        self._check(getattr(self.lib, f"ramgpu_{op}_bwd")(self.h, s, _p(a)))
+        return a
+
+    def ram_run_bwd(self, adj):
+        """(dL/dF2_in, dL/dFGEOS) of one ram_run step at the current F2."""
+        a = np.array(adj, dtype=np.float64, order="F", copy=True)
+        g = np.zeros((self.ns, self.nt, self.ne, self.npa), order="F")
+        self._check(self.lib.ramgpu_ram_run_bwd(self.h, _p(a), _p(g)))
+        return a, g
+
+    def adjoint_run(self, nsteps, pts, w, target, ckpt_every=10, kp=None, bscale=None):
+        """Loss and gradients of the virtual-satellite misfit over nsteps.
+        pts: (npts,4) 1-based S,I,J,K; w: (npts,NPA); target: (nsteps,npts).
+        Returns (loss, dL/dF2_0 (NS,NR,NT,NE,NPA), dL/dFGEOS (nsteps,NS,NT,NE,NPA))."""
+        pts = np.ascontiguousarray(pts, dtype=np.int32)
+        npts = pts.shape[0]
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        target = np.ascontiguousarray(target, dtype=np.float64)
+        if w.shape != (npts, self.npa) or target.shape != (nsteps, npts):
+            raise ValueError("w must be (npts,NPA), target (nsteps,npts)")
+        kpa = None if kp is None else np.ascontiguousarray(kp, dtype=np.float64)
+        bsa = None if bscale is None else np.ascontiguousarray(bscale, dtype=np.float64)
+        if kpa is not None and kpa.size != nsteps:
+            raise ValueError("kp must have nsteps entries")
+        if bsa is not None and bsa.size != nsteps * self.ns:
+            raise ValueError("bscale must have nsteps*NS entries")
+        loss = C.c_double(0.0)
+        gf = np.zeros(self.dims, order="F")
+        gb = np.zeros((nsteps, self.ns, self.nt, self.ne, self.npa))
+        gbf = np.zeros(nsteps * self.ns * self.nt * self.ne * self.npa)
+        self._check(self.lib.ramgpu_adjoint_run(
+            self.h, nsteps, ckpt_every, None if kpa is None else _p(kpa), None if bsa is None else _p(bsa),
+            npts, pts.ctypes.data_as(_ip), _p(w), _p(target), C.byref(loss), _p(gf), _p(gbf)))
+        per = self.ns * self.nt * self.ne * self.npa
+        for n in range(nsteps):
+            gb[n] = gbf[n * per:(n + 1) * per].reshape((self.ns, self.nt, self.ne, self.npa), order="F")
+        return loss.value, gf, gb
+
     def kernel_timing(self, enable: bool = True):
         self._check(self.lib.ramgpu_kernel_timing(self.h, int(bool(enable))))
 
@@ -295,3 +366,82 @@ class RamGpu:
         ms = C.c_double(0.0)
         self._check(self.lib.ramgpu_last_run_ms(self.h, C.byref(ms)))
         return ms.value
+
+
+class RamBatch:
+    """An ensemble of RamGpu members stepped together (``ramgpu_batch``).
+    The members stay owned by the caller and must outlive the ensemble."""
+
+    def __init__(self, members):
+        self.lib = load_library()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.h.value for m in self.members])
+        h = C.c_void_p()
+        self._check(self.lib.ramgpu_batch_create(C.byref(h), arr, len(self.members)))
+        self.h = h
+        self.sum_ns = sum(m.ns for m in self.members)
+
+    def _check(self, rc):
+        if rc != 0:
+            raise RamGpuError(f"ramgpu status {rc}: {self.lib.ramgpu_last_error().decode()}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ramgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _split(self, out):
+        res, o = [], 0
+        for m in self.members:
+            res.append([out[o + s].as_dict() for s in range(m.ns)])
+            o += m.ns
+        return res
+
+    def use_graph(self, enable=True):
+        self._check(self.lib.ramgpu_batch_use_graph(self.h, int(bool(enable))))
+
+    def ram_run(self, nsteps=1):
+        out = (SpeciesOut * self.sum_ns)()
+        self._check(self.lib.ramgpu_batch_ram_run(self.h, nsteps, out))
+        return self._split(out)
+
+    def ram_run_schedule(self, kp, bscale=None):
+        """kp: (nsteps, M); bscale: (nsteps, sum NS) or None."""
+        kp = np.ascontiguousarray(kp, dtype=np.float64)
+        n = kp.shape[0]
+        if kp.shape != (n, len(self.members)):
+            raise ValueError("kp must be (nsteps, members)")
+        bs = None
+        if bscale is not None:
+            bs = np.ascontiguousarray(bscale, dtype=np.float64)
+            if bs.shape != (n, self.sum_ns):
+                raise ValueError("bscale must be (nsteps, sum of member species)")
+        out = (SpeciesOut * self.sum_ns)()
+        self._check(self.lib.ramgpu_batch_ram_run_schedule(self.h, n, _p(kp),
+                                                           None if bs is None else _p(bs), out))
+        return self._split(out)
+
+    def last_run_ms(self) -> float:
+        ms = C.c_double(0.0)
+        self._check(self.lib.ramgpu_batch_last_run_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def autotune(self, reps=3):
+        ms = (C.c_double * 3)()
+        self._check(self.lib.ramgpu_batch_autotune(self.h, reps, ms))
+        return {"r": ms[0], "p": ms[1], "plane": ms[2]}
+
+    def shapes(self):
+        caps = (C.c_int * 3)()
+        self._check(self.lib.ramgpu_batch_get_shapes(self.h, caps))
+        return {"r": caps[0], "p": caps[1], "plane": caps[2]}
+
+    def set_shapes(self, r=0, p=0, plane=0):
+        caps = (C.c_int * 3)(r, p, plane)
+        self._check(self.lib.ramgpu_batch_set_shapes(self.h, caps))

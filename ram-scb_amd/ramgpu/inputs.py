@@ -212,3 +212,28 @@ def multi_species_schedule(nsteps: int, seed: int = 0):
     kps = [2.0 + 4.0 * n / max(nsteps - 1, 1) for n in range(nsteps)]
     scales = np.exp(0.3 * rng.standard_normal(nsteps))
     return list(zip(kps, scales.tolist()))
+
+
+def satellite_points(problem: Problem, npts: int, seed: int = 0, species: int = 1):
+    """Seeded virtual-satellite points (S,I,J,K), 1-based, away from the
+    boundary rows, and their omnidirectional-flux weights
+    w[p, L] = 4*pi*WMU(L) / (FFACTOR(S,I,K,L) * FNHS(I,J,L)), so that
+    omega_p = sum_L w[p,L] * F2(S,I,J,K,L) is the flux of src/ModRamRun.f90:214
+    (as src/ModRamSats.f90:661 samples it)."""
+    g = problem.grid
+    rng = np.random.default_rng(seed)
+    pts = np.zeros((npts, 4), dtype=np.int32)
+    w = np.zeros((npts, g.npa))
+    for p in range(npts):
+        i = int(rng.integers(2, g.nr))          # I = 3..NR (0-based 2..NR-1)
+        j = int(rng.integers(1, g.nt - 1))      # J = 2..NT-1
+        k = int(rng.integers(1, g.ne - 1))      # K = 2..NE-1
+        pts[p] = (species, i + 1, j + 1, k + 1)
+        ff = g.ffactor[species - 1, i, k, :]
+        w[p] = 4.0 * np.pi * g.wmu / (ff * problem.geom.fnhs[i, j, :])
+    return pts, w
+
+
+def satellite_flux(f2: np.ndarray, pts: np.ndarray, w: np.ndarray) -> np.ndarray:
+    """omega_p of satellite_points for a Fortran-layout F2 (NS,NR,NT,NE,NPA)."""
+    return np.array([np.dot(w[p], f2[s - 1, i - 1, j - 1, k - 1, :]) for p, (s, i, j, k) in enumerate(pts)])

@@ -1,0 +1,183 @@
+"""GPU reverse mode: the backward of every operator, of one ram_run step and
+of the checkpointed multi-step misfit gradient, checked against FP64 central
+finite differences of the oracle (the CPU restatement of the reference).
+
+Each operator is piecewise linear in (F2, FGEOS), so along a direction v the
+central difference (O(x+hv) - O(x-hv))/2h equals J v exactly unless a limiter
+kink or a clamp lies within +-h; with relative perturbations h = 1e-6 that is
+rare, and the check is the dot-product identity <ybar, J v> = <J^T ybar, v>
+to a tolerance of 1e-7 relative (rounding of the difference quotient is
+~1e-10).
+"""
+import numpy as np
+import pytest
+
+from conftest import gpu_available
+from helpers import random_problem
+from oracle.oracle import Oracle
+from ramgpu import inputs, species as sp
+from ramgpu.capi import RamGpu
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="no GPU")]
+
+TOL = 1e-7
+H = 1e-6
+
+
+def _oracle_op(p, f2, op, s, fgeos=None):
+    o = Oracle(p)
+    o.set_f2(f2)
+    if fgeos is not None:
+        o.set_fgeos(fgeos)
+    o.call("cepara", s)
+    o.call("driftpara", s)
+    o.call(op, s)
+    return o.f2
+
+
+def _state(p, seed):
+    """A realistic operator input: the problem's F2 after one oracle step."""
+    o = Oracle(p)
+    o.ram_run(2)
+    return o.f2
+
+
+def _check(what, lhs, rhs, scale):
+    err = abs(lhs - rhs) / scale
+    print(f"{what}: <ybar,Jv> {lhs:.12e}  <J^T ybar,v> {rhs:.12e}  rel {err:.2e}", flush=True)
+    assert err <= TOL, what
+
+
+@pytest.mark.parametrize("op", ["driftr", "driftp", "drifte", "driftmu", "charexchange", "atmol"])
+@pytest.mark.parametrize("seed", [0, 1])
+def test_operator_vjp_matches_oracle_fd(op, seed):
+    p = random_problem(seed=30 + seed, species=(sp.HYDROGEN, sp.OXYGEN), spiky=(seed == 1))
+    x = _state(p, seed)
+    rng = np.random.default_rng(seed)
+    for s in (1, 2):
+        v = np.asfortranarray(x * rng.standard_normal(x.shape))
+        ybar = np.asfortranarray(rng.standard_normal(x.shape))
+        yp = _oracle_op(p, x + H * v, op, s)
+        ym = _oracle_op(p, x - H * v, op, s)
+        jv = (yp - ym) / (2 * H)
+        g = RamGpu.from_problem(p)
+        g.put_f2(x)
+        g.cepara(s)
+        g.driftpara(s)
+        res = g.op_bwd(op, s, ybar)
+        xbar = res[0] if op == "driftr" else res
+        lhs, rhs = float(np.sum(ybar * jv)), float(np.sum(xbar * v))
+        _check(f"{op} S={s} seed={seed} (F2)", lhs, rhs, np.linalg.norm(ybar) * np.linalg.norm(jv) + 1e-300)
+        if op == "driftr":  # boundary direction
+            fg = p.fgeos
+            d = np.asfortranarray(fg * rng.standard_normal(fg.shape))
+            yp = _oracle_op(p, x, op, s, fgeos=fg + H * d)
+            ym = _oracle_op(p, x, op, s, fgeos=fg - H * d)
+            jd = (yp - ym) / (2 * H)
+            lhs, rhs = float(np.sum(ybar * jd)), float(np.sum(res[1] * d))
+            if np.linalg.norm(jd) > 0:
+                _check(f"driftr S={s} seed={seed} (FGEOS)", lhs, rhs, np.linalg.norm(ybar) * np.linalg.norm(jd))
+        g.close()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_ram_run_step_vjp_matches_oracle_fd(seed):
+    p = random_problem(seed=40 + seed, species=(sp.HYDROGEN, sp.ELECTRON), spiky=(seed == 1))
+    x = _state(p, seed)
+    rng = np.random.default_rng(10 + seed)
+    v = np.asfortranarray(x * rng.standard_normal(x.shape))
+    d = np.asfortranarray(p.fgeos * rng.standard_normal(p.fgeos.shape))
+    ybar = np.asfortranarray(rng.standard_normal(x.shape))
+
+    def step(f2, fgeos):
+        o = Oracle(p)
+        o.set_f2(f2)
+        o.set_fgeos(fgeos)
+        o.ram_run(2)
+        return o.f2
+    jv = (step(x + H * v, p.fgeos) - step(x - H * v, p.fgeos)) / (2 * H)
+    jd = (step(x, p.fgeos + H * d) - step(x, p.fgeos - H * d)) / (2 * H)
+    g = RamGpu.from_problem(p)
+    g.put_f2(x)
+    xbar, fbar = g.ram_run_bwd(ybar)
+    assert np.array_equal(g.get_f2(), x), "ram_run_bwd must leave F2 unchanged"
+    _check(f"ram_run step seed={seed} (F2)", float(np.sum(ybar * jv)), float(np.sum(xbar * v)),
+           np.linalg.norm(ybar) * np.linalg.norm(jv))
+    _check(f"ram_run step seed={seed} (FGEOS)", float(np.sum(ybar * jd)), float(np.sum(fbar * d)),
+           np.linalg.norm(ybar) * np.linalg.norm(jd))
+    g.close()
+
+
+def _misfit_setup(nsteps, seed=7):
+    p = inputs.make_problem(nr=8, nt=9, ne=8, npa=10, species=(sp.HYDROGEN,), kp=3.0, seed=seed)
+    pts, w = inputs.satellite_points(p, 8, seed=seed)
+    rng = np.random.default_rng(seed)
+    # truth: a run from f0 perturbed by 20%
+    truth = Oracle(p)
+    truth.set_f2(p.f2 * (1.0 + 0.2 * rng.uniform(-1, 1, p.f2.shape)))
+    kp = np.linspace(2.0, 5.0, nsteps)
+    bs = np.exp(0.3 * rng.standard_normal(nsteps))
+    target = np.zeros((nsteps, len(pts)))
+    for n in range(nsteps):
+        truth.set_vt(inputs.volland_stern(p.grid, kp[n]))
+        truth.set_fgeos(p.fgeos * bs[n])
+        truth.ram_run(1)
+        target[n] = np.log(inputs.satellite_flux(truth.f2, pts, w))
+    return p, pts, w, target, kp, bs
+
+
+def _oracle_loss(p, f0, fgeos_steps, pts, w, target, kp):
+    o = Oracle(p)
+    o.set_f2(f0)
+    tot = 0.0
+    for n in range(len(kp)):
+        o.set_vt(inputs.volland_stern(p.grid, kp[n]))
+        o.set_fgeos(fgeos_steps[n])
+        o.ram_run(1)
+        r = np.log(inputs.satellite_flux(o.f2, pts, w)) - target[n]
+        tot += float(np.sum(r * r))
+    return tot
+
+
+def test_adjoint_run_gradient_matches_oracle_fd_and_checkpoints():
+    nsteps = 6
+    p, pts, w, target, kp, bs = _misfit_setup(nsteps)
+    g = RamGpu.from_problem(p)
+    g.set_kp(kp[0])
+    results = {}
+    for ck in (1, 2, 6):
+        g.put_f2(p.f2)
+        results[ck] = g.adjoint_run(nsteps, pts, w, target, ckpt_every=ck, kp=kp, bscale=bs)
+    loss, gf, gb = results[2]
+    for ck in (1, 6):  # the checkpoint interval does not change the arithmetic
+        assert results[ck][0] == loss
+        assert np.array_equal(results[ck][1], gf)
+        assert np.array_equal(results[ck][2], gb)
+    fg_steps = [p.fgeos * bs[n] for n in range(nsteps)]
+    ref = _oracle_loss(p, p.f2, fg_steps, pts, w, target, kp)
+    print(f"loss gpu {loss:.15e} oracle {ref:.15e}", flush=True)
+    assert abs(loss - ref) <= 1e-10 * abs(ref)
+    rng = np.random.default_rng(3)
+    # 8 random directions in f0
+    for m in range(8):
+        v = p.f2 * rng.standard_normal(p.f2.shape)
+        lp = _oracle_loss(p, p.f2 + H * v, fg_steps, pts, w, target, kp)
+        lm = _oracle_loss(p, p.f2 - H * v, fg_steps, pts, w, target, kp)
+        fd = (lp - lm) / (2 * H)
+        ad = float(np.sum(gf * v))
+        err = abs(fd - ad) / (abs(fd) + 1e-30)
+        print(f"f0 direction {m}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
+        assert err <= 1e-6
+    # per-step boundary: the FGEOS of one step at a time
+    for n in (0, nsteps // 2, nsteps - 1):
+        d = p.fgeos * rng.standard_normal(p.fgeos.shape)
+        fp = [f + (H * d if m == n else 0) for m, f in enumerate(fg_steps)]
+        fm = [f - (H * d if m == n else 0) for m, f in enumerate(fg_steps)]
+        fd = (_oracle_loss(p, p.f2, fp, pts, w, target, kp) - _oracle_loss(p, p.f2, fm, pts, w, target, kp)) / (2 * H)
+        # the step's applied boundary is FGEOS*bscale: dL/dFGEOS_applied = grad / bscale
+        ad = float(np.sum(gb[n] / bs[n] * d))
+        err = abs(fd - ad) / (abs(fd) + 1e-30)
+        print(f"boundary step {n}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
+        assert err <= 1e-6 or abs(fd - ad) <= 1e-9 * abs(loss)
+    g.close()
