@@ -17,7 +17,17 @@ events on the library's stream; the cpu_baseline object is the oracle's
 C restatement (OpenMP over species, as the reference) on a bounded sample of
 the same workload.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1]
+Two more objects in the same line, each on a bounded sample:
+  "batched": an ensemble of uneven members (half 20x25x35x72, half
+    40x48x48x90, 4 species each, per-member Kp ~ U(1,7) and boundary scale
+    ~ LogNormal(0,0.5)) stepped as one launch per kernel under a captured
+    HIP graph with autotuned launch grids, against the same members stepped
+    one after another (one ram_run call each);
+  "adjoint": H+ on 20x25x35x72, 60 steps, virtual-satellite misfit at 8
+    points, gradient w.r.t. the initial F2 and every step's FGEOS with
+    checkpoints every 10 steps, against the 60-step forward alone.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M]
 """
 from __future__ import annotations
 
@@ -35,6 +45,11 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBPS = 8000.0       # MI355X HBM3E, AMD specification
 BYTES_PER_CELL_PASS = 16     # one FP64 read + one FP64 write of F2 per cell
+# algorithmic F2 passes per launch of each sweep kernel: k_r one (DRIFTR),
+# k_p one (DRIFTP, rows I>=2), k_plane one plane pass covering DRIFTE,
+# DRIFTMU, the loss chain, DRIFTMU, DRIFTE (rows I>=2)
+KERNEL_NAMES = {"r": "k_r (radial sweep, DRIFTR)", "p": "k_p (azimuthal sweep, DRIFTP)",
+                "plane": "k_plane (energy + pitch-angle sweeps and loss chain, one (K,L) plane per block)"}
 ALGO_BYTES_PER_CELL_UPDATE = 128  # 8 directional sweeps x (read + write) of F2
 METRIC = "FP64 RAM split-step cell-updates/s + HBM roofline %; batched & adjoint vs plain"
 TRAFFIC_FILE = REPO / "profiles" / "traffic_multi_species.txt"
@@ -100,6 +115,96 @@ def calibrate_copy(nbytes: int = 1 << 29, reps: int = 5):
     print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps + 1, "copy_GBps": gbps}))
 
 
+def bench_batched(members: int, steps: int, seed: int):
+    """Uneven ensemble in one launch per kernel (graph-captured step, tuned
+    grids) vs the same members stepped one after another."""
+    from ramgpu import inputs
+    from ramgpu.capi import RamBatch, RamGpu
+
+    nsmall = members - members // 2
+    shapes = [(20, 25, 35, 72)] * nsmall + [(40, 48, 48, 90)] * (members // 2)
+    protos = {sh: inputs.make_problem(*sh, inputs.SPECIES4, inputs.COMPOSITION4, kp=3.0, seed=seed + 5)
+              for sh in set(shapes)}
+    rng = np.random.default_rng(seed + 9)
+    kpm = rng.uniform(1.0, 7.0, members)
+    scm = np.exp(0.5 * rng.standard_normal(members))
+    log(f"bench: ensemble of {members} members ({nsmall} x 20x25x35x72, {members // 2} x 40x48x48x90, "
+        f"4 species), {steps} steps")
+    mem = []
+    for m, sh in enumerate(shapes):
+        g = RamGpu.from_problem(protos[sh])
+        g.set_kp(kpm[m])
+        g.set_boundary_scale(scm[m])
+        mem.append(g)
+    cells = sum(g.ns * g.nr * g.nt * g.ne * g.npa for g in mem)
+    b = RamBatch(mem)
+    best = b.autotune(reps=2)
+    kp = np.tile(kpm, (steps, 1))
+    bs = np.repeat(np.tile(scm, (steps, 1)), 4, axis=1)
+    b.ram_run_schedule(kp[:1], bs[:1])          # warm: capture and replay once
+    b.ram_run_schedule(kp, bs)
+    ms_batch = b.last_run_ms()
+    # the same work, member after member: one ram_run_schedule call per member
+    t0 = time.perf_counter()
+    dev_ms = 0.0
+    for m, g in enumerate(mem):
+        g.ram_run_schedule(kp[:, m], bs[:, 4 * m:4 * m + 4].ravel())
+        dev_ms += g.last_run_ms()
+    wall_seq = (time.perf_counter() - t0) * 1e3
+    # one plain step of every member, one after another (what a per-member
+    # host loop does each step)
+    t0 = time.perf_counter()
+    for g in mem:
+        g.ram_run(1)
+    wall_one = (time.perf_counter() - t0) * 1e3
+    b.close()
+    for g in mem:
+        g.close()
+    res = {"members": members, "steps": steps, "cells": cells,
+           "value": cells * steps / (ms_batch * 1e-3), "unit": "cell-updates/s",
+           "ms_per_step": ms_batch / steps,
+           "sequential_device_ms_per_step": dev_ms / steps,
+           "sequential_wall_ms_per_step": wall_seq / steps,
+           "one_step_member_loop_wall_ms": wall_one,
+           "speedup_vs_sequential_device": dev_ms / ms_batch,
+           "speedup_vs_member_loop_wall": wall_one / (ms_batch / steps),
+           "autotune_best_ms": best, "graph": True}
+    log(f"bench: ensemble {ms_batch / steps:.3f} ms/step, sequential device {dev_ms / steps:.3f} ms/step, "
+        f"member loop wall {wall_one:.3f} ms/step")
+    return res
+
+
+def bench_adjoint(seed: int, nsteps: int = 60, ckpt: int = 10):
+    """Checkpointed adjoint of the 60-step H+ run vs the forward alone."""
+    from ramgpu import inputs, species as sp
+    from ramgpu.capi import RamGpu
+
+    p = inputs.make_problem(20, 25, 35, 72, (sp.HYDROGEN,), (1.0,), kp=3.0, seed=seed + 11)
+    pts, w = inputs.satellite_points(p, 8, seed=seed + 11)
+    rng = np.random.default_rng(seed + 12)
+    g = RamGpu.from_problem(p)
+    g.put_f2(p.f2 * (1.0 + 0.2 * rng.uniform(-1, 1, p.f2.shape)))  # truth run
+    target = np.zeros((nsteps, len(pts)))
+    for n in range(nsteps):
+        g.ram_run(1)
+        target[n] = np.log(inputs.satellite_flux(g.get_f2(), pts, w))
+    g.put_f2(p.f2)
+    g.ram_run(nsteps)                      # warm
+    g.put_f2(p.f2)
+    g.ram_run(nsteps)
+    fwd_ms = g.last_run_ms()
+    g.put_f2(p.f2)
+    g.adjoint_run(nsteps, pts, w, target, ckpt_every=ckpt)   # warm
+    g.put_f2(p.f2)
+    loss, gf, gb = g.adjoint_run(nsteps, pts, w, target, ckpt_every=ckpt)
+    adj_ms = g.last_run_ms()
+    g.close()
+    log(f"bench: adjoint {adj_ms:.3f} ms (forward {fwd_ms:.3f} ms, ratio {adj_ms / fwd_ms:.2f}), loss {loss:.6e}")
+    return {"config": "H+ 20x25x35x72, 60 steps, 8 satellite points, checkpoint every 10",
+            "forward_ms": fwd_ms, "adjoint_ms": adj_ms, "ratio": adj_ms / fwd_ms, "loss": loss,
+            "grad_f2_norm": float(np.linalg.norm(gf)), "grad_fgeos_norm": float(np.linalg.norm(gb))}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +214,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy (counter calibration)")
+    ap.add_argument("--members", type=int, default=32,
+                    help="ensemble members of the batched measurement (half of each grid)")
+    ap.add_argument("--batched-steps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,17 +269,21 @@ def main():
     ms_per_step = dev_ms / args.steps
     value = cells * args.steps / (dev_ms * 1e-3)
     # dominant kernel by total device time
-    dom = max(("ij", "kl"), key=lambda k: ktimes[k][0])
-    ms_tot, nlaunch = ktimes[dom]
-    avg_s = ms_tot / nlaunch * 1e-3
-    per_launch = BYTES_PER_CELL_PASS * (cells_kl if dom == "kl" else cells)
-    achieved = per_launch / avg_s / 1e9
-    kname = {"kl": "k_kl (energy + pitch-angle sweeps, loss operators)",
-             "ij": "k_ij (radial + azimuthal sweeps)"}[dom]
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBPS, "traffic": measured_traffic(),
-                "kernel": kname, "bytes_per_launch": per_launch,
-                "avg_launch_us": avg_s * 1e6}
+    dom = max(("r", "p", "plane"), key=lambda k: ktimes[k][0])
+    per_kernel = {}
+    for k in ("r", "p", "plane"):
+        ms_tot, nlaunch = ktimes[k]
+        if not nlaunch:
+            continue
+        avg_s = ms_tot / nlaunch * 1e-3
+        per_launch = BYTES_PER_CELL_PASS * (cells if k == "r" else cells_kl)
+        per_kernel[k] = {"avg_launch_us": avg_s * 1e6, "bytes_per_launch": per_launch,
+                         "achieved_GBps": per_launch / avg_s / 1e9, "launches": nlaunch}
+    d = per_kernel[dom]
+    roofline = {"bound": "hbm", "achieved": d["achieved_GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": d["achieved_GBps"] / HBM_PEAK_GBPS, "traffic": measured_traffic(),
+                "kernel": KERNEL_NAMES[dom], "bytes_per_launch": d["bytes_per_launch"],
+                "avg_launch_us": d["avg_launch_us"], "per_kernel": per_kernel}
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
@@ -185,6 +299,10 @@ def main():
         "roofline": roofline,
     }
     dev.close()
+    if not args.no_batched:
+        out["batched"] = bench_batched(args.members, args.batched_steps, args.seed)
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(args.seed)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(problem)

@@ -7,7 +7,9 @@ central difference (O(x+hv) - O(x-hv))/2h equals J v exactly unless a limiter
 kink or a clamp lies within +-h; with relative perturbations h = 1e-6 that is
 rare, and the check is the dot-product identity <ybar, J v> = <J^T ybar, v>
 to a tolerance of 1e-7 relative (rounding of the difference quotient is
-~1e-10).
+~1e-10).  The multi-step misfit gradient (nonlinear through log) is checked
+to 1e-6 relative per direction, or to the rounding floor of the central
+difference where the directional derivative is that small.
 """
 import numpy as np
 import pytest
@@ -109,6 +111,12 @@ def test_ram_run_step_vjp_matches_oracle_fd(seed):
     g.close()
 
 
+def fd_noise(loss):
+    """Rounding floor of a central difference of the loss with step H: each
+    evaluation carries ~eps*|L| per step of accumulated rounding; 100x margin."""
+    return 100 * np.finfo(float).eps * abs(loss) / H
+
+
 def _misfit_setup(nsteps, seed=7):
     p = inputs.make_problem(nr=8, nt=9, ne=8, npa=10, species=(sp.HYDROGEN,), kp=3.0, seed=seed)
     pts, w = inputs.satellite_points(p, 8, seed=seed)
@@ -168,7 +176,7 @@ def test_adjoint_run_gradient_matches_oracle_fd_and_checkpoints():
         ad = float(np.sum(gf * v))
         err = abs(fd - ad) / (abs(fd) + 1e-30)
         print(f"f0 direction {m}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
-        assert err <= 1e-6
+        assert err <= 1e-6 or abs(fd - ad) <= fd_noise(loss)
     # per-step boundary: the FGEOS of one step at a time
     for n in (0, nsteps // 2, nsteps - 1):
         d = p.fgeos * rng.standard_normal(p.fgeos.shape)
@@ -179,5 +187,5 @@ def test_adjoint_run_gradient_matches_oracle_fd_and_checkpoints():
         ad = float(np.sum(gb[n] / bs[n] * d))
         err = abs(fd - ad) / (abs(fd) + 1e-30)
         print(f"boundary step {n}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
-        assert err <= 1e-6 or abs(fd - ad) <= 1e-9 * abs(loss)
+        assert err <= 1e-6 or abs(fd - ad) <= fd_noise(loss)
     g.close()
