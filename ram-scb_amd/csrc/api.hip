@@ -197,19 +197,38 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp) {
   return launch_check("k_p");
 }
 
+int plane_maxc(int ne, int npa) {
+  const int e = (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne);
+  const int m = (ne + kPlaneThreads / npa - 1) / (kPlaneThreads / npa);
+  return (e <= 8 && m <= 8) ? 8 : 16;
+}
+
+template <int MAXC>
+void launch_plane_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
+  const dim3 blk(kPlaneThreads);
+  switch (seq) {
+    case SEQ_E: hipLaunchKernelGGL((k_plane<SEQ_E, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_MU: hipLaunchKernelGGL((k_plane<SEQ_MU, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_CEX: hipLaunchKernelGGL((k_plane<SEQ_CEX, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_ATM: hipLaunchKernelGGL((k_plane<SEQ_ATM, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_SUM: hipLaunchKernelGGL((k_plane<SEQ_SUM, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_LOSS: hipLaunchKernelGGL((k_plane<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, rec); break;
+    default: hipLaunchKernelGGL((k_plane<SEQ_STEP, MAXC>), grid, blk, lds, st, L, rec); break;
+  }
+}
+
+void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
+  if (maxc <= 8)
+    launch_plane_t<8>(seq, grid, lds, st, L, rec);
+  else
+    launch_plane_t<16>(seq, grid, lds, st, L, rec);
+}
+
 int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp) {
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = plane_lds_bytes(c->ne, c->npa);
   tbeg(c, RAMGPU_KERNEL_PLANE);
-  switch (seq) {
-    case SEQ_E: hipLaunchKernelGGL(k_plane<SEQ_E>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    case SEQ_MU: hipLaunchKernelGGL(k_plane<SEQ_MU>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    case SEQ_CEX: hipLaunchKernelGGL(k_plane<SEQ_CEX>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    case SEQ_ATM: hipLaunchKernelGGL(k_plane<SEQ_ATM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    case SEQ_SUM: hipLaunchKernelGGL(k_plane<SEQ_SUM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    case SEQ_LOSS: hipLaunchKernelGGL(k_plane<SEQ_LOSS>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-    default: hipLaunchKernelGGL(k_plane<SEQ_STEP>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, rec); break;
-  }
+  launch_plane(seq, plane_maxc(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, rec);
   tend(c);
   return launch_check("k_plane");
 }
@@ -255,6 +274,12 @@ int check_dims(int nr, int nt, int ne, int npa) {
   const long long P = (long long)ne * npa;
   if (P > (long long)kPlaneThreads * kMaxCellsPerThread)
     return fail(RAMGPU_EINVAL, "NE*NPA=%lld above %d", P, kPlaneThreads * kMaxCellsPerThread);
+  // plane kernels: a thread owns one K of ceil(NPA/(512/NE)) rows, or one L of
+  // ceil(NE/(512/NPA)) columns
+  if (ne > kPlaneThreads || npa > kPlaneThreads ||
+      (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne) > kMaxCellsPerThread ||
+      (ne + kPlaneThreads / npa - 1) / (kPlaneThreads / npa) > kMaxCellsPerThread)
+    return fail(RAMGPU_EINVAL, "plane %dx%d too large for one block of %d threads", ne, npa, kPlaneThreads);
   if ((long long)nt * P >= (1LL << 22) || (long long)nr * nt >= (1LL << 22))
     return fail(RAMGPU_EINVAL, "grid %dx%dx%dx%d: NT*NE*NPA must stay below 2^22", nr, nt, ne, npa);
   return RAMGPU_OK;
@@ -263,10 +288,12 @@ int check_dims(int nr, int nt, int ne, int npa) {
 // Let the plane kernels use the device's whole LDS; refuse planes that do
 // not fit it.
 int set_plane_lds_limit(size_t limit) {
-  const void *fns[] = {(const void *)k_plane<SEQ_E>, (const void *)k_plane<SEQ_MU>,
-                       (const void *)k_plane<SEQ_CEX>, (const void *)k_plane<SEQ_ATM>,
-                       (const void *)k_plane<SEQ_SUM>, (const void *)k_plane<SEQ_STEP>,
-                       (const void *)k_plane<SEQ_LOSS>};
+  const void *fns[] = {
+      (const void *)k_plane<SEQ_E, 8>,    (const void *)k_plane<SEQ_MU, 8>,   (const void *)k_plane<SEQ_CEX, 8>,
+      (const void *)k_plane<SEQ_ATM, 8>,  (const void *)k_plane<SEQ_SUM, 8>,  (const void *)k_plane<SEQ_STEP, 8>,
+      (const void *)k_plane<SEQ_LOSS, 8>, (const void *)k_plane<SEQ_E, 16>,   (const void *)k_plane<SEQ_MU, 16>,
+      (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
+      (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>};
   for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
   return RAMGPU_OK;
 }

@@ -44,6 +44,7 @@ struct ramgpu_batch {
   Tab *d_tabs = nullptr;
   Plan plan[BK_N];
   size_t plane_lds = 0;
+  int plane_maxc = 8;  // the largest any member needs
   int sum_ns = 0;
   std::vector<int> bs_off;  // first species of member m in a schedule row
   int *d_bsoff = nullptr;
@@ -160,7 +161,7 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
                lf = launch_for(b, BK_FIN);
   hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, lr);
   hipLaunchKernelGGL(k_p<1>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, lp);
-  hipLaunchKernelGGL(k_plane<SEQ_STEP>, dim3(grid_for(b, BK_PLANE)), dim3(kPlaneThreads), b->plane_lds, st, lk, 1);
+  launch_plane(SEQ_STEP, b->plane_maxc, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
   hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, lp);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, lr);
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(256), 0, st, lf, 15, 2);
@@ -254,6 +255,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->bs_off.push_back(b->sum_ns);
     b->sum_ns += c->ns;
     b->plane_lds = std::max(b->plane_lds, plane_lds_bytes(c->ne, c->npa));
+    b->plane_maxc = std::max(b->plane_maxc, plane_maxc(c->ne, c->npa));
   }
   if ((rc = batch_alloc(b, (void **)&b->d_bsoff, nmembers * sizeof(int))) ||
       (rc = batch_alloc(b, (void **)&b->d_step, sizeof(int))))
@@ -399,8 +401,7 @@ int ramgpu_batch_autotune(ramgpu_batch *b, int reps, double *ms_best) {
         else if (kind == BK_P)
           hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, kind)), dim3(kLineThreads), 0, b->stream, L);
         else
-          hipLaunchKernelGGL(k_plane<SEQ_STEP>, dim3(grid_for(b, kind)), dim3(kPlaneThreads), b->plane_lds,
-                             b->stream, L, 0);
+          launch_plane(SEQ_STEP, b->plane_maxc, dim3(grid_for(b, kind)), b->plane_lds, b->stream, L, 0);
         (void)hipEventRecord(e1, b->stream);
         if (hipEventSynchronize(e1) != hipSuccess) {
           rc = fail(RAMGPU_EHIP, "autotune launch failed");

@@ -28,7 +28,7 @@ __global__ void k_coef(Launch L);
 __global__ void k_ghost(Launch L);
 template <int MODE> __global__ void k_r(Launch L);
 template <int MODE> __global__ void k_p(Launch L);
-template <int SEQ> __global__ void k_plane(Launch L, int rec);
+template <int SEQ, int MAXC> __global__ void k_plane(Launch L, int rec);
 __global__ void k_finalize(Launch L, int cflmask, int summode);
 __global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
                      double kp_val, int nr1, int nt);
@@ -99,5 +99,9 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp);
 int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp);
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
 int step_once(ramgpu_ctx *c);
+// cells per thread the plane kernels need for an NE x NPA plane (8 or 16)
+int plane_maxc(int ne, int npa);
+// launch k_plane<seq, maxc>
+void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
 
 }  // namespace ramgpu_host

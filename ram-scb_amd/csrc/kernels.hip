@@ -570,72 +570,85 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
 // Sequences (template SEQ):
 // (plane layout helpers: sweep_common.h)
 
+// Row-wise loop of the plane kernels: this thread's cells (K = c.ke, rows
+// L = c.le0 + n*c.lestep), n = 0..kMaxCellsPerThread-1.
+#define ROW_CELLS(n)                                 \
+  const int l = c.le0 + (n) * c.lestep;              \
+  if (!c.eact || l >= NPA) break;                    \
+  const int cc = l * NE + c.ke
+
 // DRIFTE on the plane (src/ModRamDrift.f90:333-369).  F(1),F(0) relativistic
-// ghosts from F(2); F(NE+1)=F(NE+2)=0.
+// ghosts from F(2), once per row; F(NE+1)=F(NE+2)=0.  Each thread owns one K
+// of several rows, so the per-K factors sit in registers and the CFL maximum
+// of its cells is one value.
+template <int MAXC>
 __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   {
     const double *src = t.kle + c.ij * NPA * 8;
     for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
+    const double *rel = t.rel + c.s * 8;
+    for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
+      const double g1 = d.F[l * NE + 1] * rel[0] / rel[1] * rel[2];
+      d.gh[2 * l] = g1;
+      d.gh[2 * l + 1] = g1 * rel[3] / rel[0] * rel[4];
+    }
     if (rec)
       for (int k = threadIdx.x; k < NE; k += blockDim.x) d.cmx[k] = 0ull;
   }
   __syncthreads();
   const double *k8 = t.kij + c.ij * 8;
   const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double *rel = t.rel + c.s * 8;
-  const double r0 = rel[0], r1 = rel[1], r2 = rel[2], r3 = rel[3], r4 = rel[4];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
-  double O[kMaxCellsPerThread];
+  const int k = c.ke;
+  const double ekp = d.kt[KT_EKP * NE + k], edot = d.kt[KT_EDOT * NE + k];
+  const double de = d.kt[KT_DE * NE + k], rde = d.kt[KT_RDE * NE + k];
+  const double we = d.kt[KT_WE * NE + k], rwe = d.kt[KT_RWE * NE + k];
+  const bool top = k + 1 >= NE;
+  const double we1 = top ? 1.0 : d.kt[KT_WE * NE + k + 1], rwe1 = top ? 1.0 : d.kt[KT_RWE * NE + k + 1];
+  double O[MAXC];
+  double cmax = 0.0;
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
-    if (n < c.nmine) {
-      CELL_KL(n);
-      const double *e = d.cf + l * 8;
-      // CDriftE(I,J,K,L), :337-342
-      const double edt1 =
-          div_rn(div_rn(div_rn(d.kt[KT_EKP * NE + k], e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
-      const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
-      const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
-      const double ce =
-          d.kt[KT_EDOT * NE + k] * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
-      if (rec) atomic_max_pos(&d.cmx[k], fmax(fabs(ce), 1E-10));
-      const double *row = d.F + l * NE;
-      const double fk = row[k];
-      const double g1 = row[1] * r0 / r1 * r2;
-      const double g0 = g1 * r3 / r0 * r4;
-      const double f0 = (k == 0) ? g1 : fk;
-      const double fp1 = (k + 1 < NE) ? row[k + 1] : 0.0;
-      double fn, fnm1;
-      if (ce < 0) {
-        fn = (k + 2 < NE) ? row[k + 2] : 0.0;
-        fnm1 = fp1;
-      } else {
-        fn = f0;
-        fnm1 = (k == 0) ? g0 : (k == 1) ? g1 : row[k - 1];
-      }
-      const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, d.kt[KT_DE * NE + k], d.kt[KT_RDE * NE + k]), beta);
-      O[n] = fk - div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * fb;
-      d.H[cc] = (k + 1 < NE) ? div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * fb : 0.0;
+  for (int n = 0; n < MAXC; ++n) {
+    ROW_CELLS(n);
+    const double *e = d.cf + l * 8;
+    // CDriftE(I,J,K,L), :337-342
+    const double edt1 = div_rn(div_rn(div_rn(ekp, e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
+    const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
+    const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
+    const double ce = edot * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
+    cmax = fmax(cmax, fmax(fabs(ce), 1E-10));
+    const double *row = d.F + l * NE;
+    const double fk = row[k];
+    const double f0 = (k == 0) ? d.gh[2 * l] : fk;
+    const double fp1 = top ? 0.0 : row[k + 1];
+    double fn, fnm1;
+    if (ce < 0) {
+      fn = (k + 2 < NE) ? row[k + 2] : 0.0;
+      fnm1 = fp1;
+    } else {
+      fn = f0;
+      fnm1 = (k == 0) ? d.gh[2 * l + 1] : (k == 1) ? d.gh[2 * l] : row[k - 1];
     }
+    const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, de, rde), beta);
+    O[n] = fk - div_rn(ce, we, rwe) * fb;
+    d.H[cc] = top ? 0.0 : div_rn(ce, we1, rwe1) * fb;
   }
+  if (rec && c.eact) atomic_max_pos(&d.cmx[k], cmax);
   __syncthreads();
+  if (k >= 1) {
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
-    if (n < c.nmine) {
-      CELL_KL(n);
-      if (k >= 1) {
-        double v = O[n] + d.H[cc - 1];
-        if (v < 0) v = 1E-15;
-        d.F[cc] = v;
-      }
+    for (int n = 0; n < MAXC; ++n) {
+      ROW_CELLS(n);
+      double v = O[n] + d.H[cc - 1];
+      if (v < 0) v = 1E-15;
+      d.F[cc] = v;
     }
   }
   if (rec) {
     __syncthreads();
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
-    for (int k = threadIdx.x; k < NE; k += blockDim.x)
-      if (d.cmx[k]) atomicMax(&g[2 + k], d.cmx[k]);
+    for (int kk = threadIdx.x; kk < NE; kk += blockDim.x) atomicMax(&g[2 + kk], d.cmx[kk]);
   }
   __syncthreads();
 }
@@ -643,6 +656,9 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
 // DRIFTMU on the plane (src/ModRamDrift.f90:413-466).  F(1)=F(2) for the
 // stencil; limiter faces L=2..NPA-2; CDriftMu(1)=0, FBND(1)=0,
 // FBND(NPA-1)=F(NPA); top cell F2(NPA)=F2(NPA-1)*FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1).
+// Each thread owns one L of several columns K: the per-L terms sit in
+// registers.
+template <int MAXC>
 __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   {
@@ -656,58 +672,72 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
   const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
-  double O[kMaxCellsPerThread];
+  const int l = c.lm, L1 = l + 1;
+  double O[MAXC];
+  // faces L=2..NPA-1 carry a flux; face 1 carries none (CDriftMu(1)=0)
+  const bool face = c.mact && L1 >= 2 && L1 <= NPA - 1;
+  if (c.mact && L1 >= 2) {
+    const double *m = d.cf + l * 8;
+    const double bhs = m[KLM_BHS], rbhs = m[KLM_RBHS], drm2 = m[KLM_DRM2], dpm2 = m[KLM_DPM2];
+    const double cmudot = m[KLM_CMUDOT], gmr = m[KLM_GMR], gmp = m[KLM_GMP], di2 = m[KLM_DI2];
+    const double dmu = d.lt[LT_DMU * NPA + l], rdmu = d.lt[LT_RDMU * NPA + l];
+    const double wmu = d.lt[LT_WMU * NPA + l], rwmu = d.lt[LT_RWMU * NPA + l];
+    const double wmu1 = face ? d.lt[LT_WMU * NPA + l + 1] : 1.0, rwmu1 = face ? d.lt[LT_RWMU * NPA + l + 1] : 1.0;
+    double cmax = 0.0;
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
-    if (n < c.nmine) {
-      CELL_KL(n);
-      const int L1 = l + 1;
+    for (int n = 0; n < MAXC; ++n) {
+      const int k = c.km0 + n * c.kmstep;
+      if (k >= NE) break;
+      const int cc = l * NE + k;
+      // CDriftMu(I,J,K,L), :418-433 (column L=NPA: only its CFL term)
+      const double edt = div_rn(div_rn(div_rn(d.kt[KT_MKP * NE + k], bhs, rbhs), rlz, rrlz), bij, rbij) * qs;
+      const double drdm = drm1 + edt * drm2 * rlz;
+      const double dpdm = dpm1 - edt * dpm2;
+      const double cm = -cmudot * (gmr * drdm + gmp * dpdm + db2 + di2);
+      cmax = fmax(cmax, fmax(fabs(cm), 1E-32));
       double on = 0.0, h = 0.0;
-      if (L1 >= 2) {
-        const double *m = d.cf + l * 8;
-        // CDriftMu(I,J,K,L), :418-433
-        const double edt =
-            div_rn(div_rn(div_rn(d.kt[KT_MKP * NE + k], m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij) * qs;
-        const double drdm = drm1 + edt * m[KLM_DRM2] * rlz;
-        const double dpdm = dpm1 - edt * m[KLM_DPM2];
-        const double cm = -m[KLM_CMUDOT] * (m[KLM_GMR] * drdm + m[KLM_GMP] * dpdm + db2 + m[KLM_DI2]);
-        if (rec) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
-        if (L1 <= NPA - 1) {
-          double fb;
-          const double f0 = d.F[cc];
-          if (L1 == NPA - 1) {
-            fb = d.F[cc + NE];  // FBND(NPA-1) = F(NPA)
+      if (face) {
+        double fb;
+        const double f0 = d.F[cc];
+        if (L1 == NPA - 1) {
+          fb = d.F[cc + NE];  // FBND(NPA-1) = F(NPA)
+        } else {
+          const double fp1 = d.F[cc + NE];
+          double fn, fnm1;
+          if (cm < 0) {
+            fn = d.F[cc + 2 * NE];
+            fnm1 = fp1;
           } else {
-            const double fp1 = d.F[cc + NE];
-            double fn, fnm1;
-            if (cm < 0) {
-              fn = d.F[cc + 2 * NE];
-              fnm1 = fp1;
-            } else {
-              fn = f0;
-              fnm1 = (L1 == 2) ? f0 : d.F[cc - NE];  // F(1) = F(2)
-            }
-            fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
+            fn = f0;
+            fnm1 = (L1 == 2) ? f0 : d.F[cc - NE];  // F(1) = F(2)
           }
-          on = div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb;
-          h = div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * fb;
+          fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, dmu, rdmu), beta);
         }
+        on = div_rn(cm, wmu, rwmu) * fb;
+        h = div_rn(cm, wmu1, rwmu1) * fb;
       }
-      // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
       O[n] = d.F[cc] - on;
       d.H[cc] = h;
     }
+    if (rec) atomic_max_pos(&d.cmx[NE + l], cmax);
+  } else if (c.mact) {  // L = 1: no flux
+#pragma unroll
+    for (int n = 0; n < MAXC; ++n) {
+      const int k = c.km0 + n * c.kmstep;
+      if (k >= NE) break;
+      d.H[l * NE + k] = 0.0;
+    }
   }
   __syncthreads();
+  if (c.mact && l >= 1 && l <= NPA - 2) {
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
-    if (n < c.nmine) {
-      CELL_KL(n);
-      if (l >= 1 && l <= NPA - 2) {
-        double v = O[n] + d.H[cc - NE];
-        if (v < 0) v = 1E-15;
-        d.F[cc] = v;
-      }
+    for (int n = 0; n < MAXC; ++n) {
+      const int k = c.km0 + n * c.kmstep;
+      if (k >= NE) break;
+      const int cc = l * NE + k;
+      double v = O[n] + d.H[cc - NE];
+      if (v < 0) v = 1E-15;
+      d.F[cc] = v;
     }
   }
   __syncthreads();
@@ -719,8 +749,7 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   }
   if (rec) {
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
-    for (int l = threadIdx.x; l < NPA; l += blockDim.x)
-      if (d.cmx[NE + l]) atomicMax(&g[2 + NE + l], d.cmx[NE + l]);
+    for (int ll = 1 + threadIdx.x; ll < NPA; ll += blockDim.x) atomicMax(&g[2 + NE + ll], d.cmx[NE + ll]);
   }
   __syncthreads();
 }
@@ -728,34 +757,40 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
 // The loss chain of ram_run between the two drift halves (src/ModRamRun.f90:
 // 77-141): SUMRC, [CEX, SUMRC], ATMOL, SUMRC, ATMOL, SUMRC, [CEX, SUMRC], with
 // the SUMRC partials of this plane block b in slots 0..4.
+template <int MAXC>
 __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, int b) {
   const int NE = c.NE, NPA = c.NPA, NT = t.nt;
   const bool cex = t.cex[c.s] != 0;
   const int upa = t.upa[c.i];
   const bool sum_row = c.j <= NT - 2;
+  const int k = c.ke;
+  const double sigv = d.kt[KT_SIGV * NE + k], latl = d.kt[KT_LATL * NE + k];
+  const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
+  const double dts = t.dts;
   double a[5] = {0, 0, 0, 0, 0}, tot[5];
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n)
-    if (n < c.nmine) {
-      CELL_KL(n);
-      const bool w_on = sum_row && k >= 1 && l >= 1;
-      const bool cx = cex && k >= 1 && l >= 1;
-      const bool atm = k >= 1 && l + 1 >= upa;
-      double f = d.F[cc];
-      const double cxf = cx ? cex_factor(d, NE, k, l, t.dts) : 1.0;
-      const double af = atm ? atm_factor(d, NE, k, l) : 1.0;
-      const double wk = d.kt[KT_WE * NE + k], wl = d.lt[LT_WMU * NPA + l], ek = d.kt[KT_EKEV * NE + k];
-      if (w_on) a[0] += ek * (f * wk * wl);
-      if (cx) f = f * cxf;
-      if (w_on) a[1] += ek * (f * wk * wl);
-      if (atm) f = f * af;
-      if (w_on) a[2] += ek * (f * wk * wl);
-      if (atm) f = f * af;
-      if (w_on) a[3] += ek * (f * wk * wl);
-      if (cx) f = f * cxf;
-      if (w_on) a[4] += ek * (f * wk * wl);
-      d.F[cc] = f;
-    }
+  for (int n = 0; n < MAXC; ++n) {
+    ROW_CELLS(n);
+    const bool w_on = sum_row && k >= 1 && l >= 1;
+    const bool cx = cex && k >= 1 && l >= 1;
+    const bool atm = k >= 1 && l + 1 >= upa;
+    double f = d.F[cc];
+    // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49);
+    // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS))
+    const double cxf = cx ? exp(-(sigv * d.cl[2 * l] * dts)) : 1.0;
+    const double af = atm ? exp(latl * d.cl[2 * l + 1]) : 1.0;
+    const double wl = d.lt[LT_WMU * NPA + l];
+    if (w_on) a[0] += ek * (f * wk * wl);
+    if (cx) f = f * cxf;
+    if (w_on) a[1] += ek * (f * wk * wl);
+    if (atm) f = f * af;
+    if (w_on) a[2] += ek * (f * wk * wl);
+    if (atm) f = f * af;
+    if (w_on) a[3] += ek * (f * wk * wl);
+    if (cx) f = f * cxf;
+    if (w_on) a[4] += ek * (f * wk * wl);
+    d.F[cc] = f;
+  }
   block_sum_n<5>(a, d.red, tot);
   if (threadIdx.x < 5) {
     double v = tot[0];
@@ -765,7 +800,7 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   }
 }
 
-template <int SEQ>
+template <int SEQ, int MAXC>
 __global__ void __launch_bounds__(kPlaneThreads, 4) k_plane(Launch L, int rec) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
@@ -781,51 +816,51 @@ __global__ void __launch_bounds__(kPlaneThreads, 4) k_plane(Launch L, int rec) {
     const bool rec_here = rec && inside;
 
     if constexpr (SEQ == SEQ_E) {
-      plane_e(t, c, d, rec_here);
+      plane_e<MAXC>(t, c, d, rec_here);
     } else if constexpr (SEQ == SEQ_MU) {
-      plane_mu(t, c, d, rec_here);
+      plane_mu<MAXC>(t, c, d, rec_here);
     } else if constexpr (SEQ == SEQ_CEX) {
-      if (t.cex[c.s] != 0) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
+      if (t.cex[c.s] != 0 && c.ke >= 1) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
 #pragma unroll
-        for (int n = 0; n < kMaxCellsPerThread; ++n)
-          if (n < c.nmine) {
-            CELL_KL(n);
-            if (k >= 1 && l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, k, l, t.dts);
-          }
+        for (int n = 0; n < MAXC; ++n) {
+          ROW_CELLS(n);
+          if (l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, c.ke, l, t.dts);
+        }
       }
       __syncthreads();
     } else if constexpr (SEQ == SEQ_ATM) {
       // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
       const int upa = t.upa[c.i];
+      if (c.ke >= 1) {
 #pragma unroll
-      for (int n = 0; n < kMaxCellsPerThread; ++n)
-        if (n < c.nmine) {
-          CELL_KL(n);
-          if (k >= 1 && l + 1 >= upa) d.F[cc] = d.F[cc] * atm_factor(d, NE, k, l);
+        for (int n = 0; n < MAXC; ++n) {
+          ROW_CELLS(n);
+          if (l + 1 >= upa) d.F[cc] = d.F[cc] * atm_factor(d, NE, c.ke, l);
         }
+      }
       __syncthreads();
     } else if constexpr (SEQ == SEQ_SUM) {
       // SUMRC over I>=2, K>=2, L>=2, J<=NT-1 (src/ModRamRun.f90:246-255), slot 0
       double a[1] = {0.0}, tot[1];
-      if (c.j <= NT - 2) {
+      const int k = c.ke;
+      if (c.j <= NT - 2 && k >= 1) {
 #pragma unroll
-        for (int n = 0; n < kMaxCellsPerThread; ++n)
-          if (n < c.nmine) {
-            CELL_KL(n);
-            if (k >= 1 && l >= 1)
-              a[0] += d.kt[KT_EKEV * NE + k] * (d.F[cc] * d.kt[KT_WE * NE + k] * d.lt[LT_WMU * NPA + l]);
-          }
+        for (int n = 0; n < MAXC; ++n) {
+          ROW_CELLS(n);
+          if (l >= 1)
+            a[0] += d.kt[KT_EKEV * NE + k] * (d.F[cc] * d.kt[KT_WE * NE + k] * d.lt[LT_WMU * NPA + l]);
+        }
       }
       block_sum_n<1>(a, d.red, tot);
       if (threadIdx.x == 0) t.partials[((size_t)0 * t.ns + c.s) * t.maxblk + u.b] = tot[0];
     } else if constexpr (SEQ == SEQ_LOSS) {
-      plane_loss(t, c, d, u.b);
+      plane_loss<MAXC>(t, c, d, u.b);
     } else {  // SEQ_STEP
-      plane_e(t, c, d, rec_here);
-      plane_mu(t, c, d, rec_here);
-      plane_loss(t, c, d, u.b);
-      plane_mu(t, c, d, false);
-      plane_e(t, c, d, false);
+      plane_e<MAXC>(t, c, d, rec_here);
+      plane_mu<MAXC>(t, c, d, rec_here);
+      plane_loss<MAXC>(t, c, d, u.b);
+      plane_mu<MAXC>(t, c, d, false);
+      plane_e<MAXC>(t, c, d, false);
     }
 
     if constexpr (SEQ != SEQ_SUM) {
@@ -1017,12 +1052,19 @@ template __global__ void k_r<1>(Launch);
 template __global__ void k_r<2>(Launch);
 template __global__ void k_p<0>(Launch);
 template __global__ void k_p<1>(Launch);
-template __global__ void k_plane<SEQ_E>(Launch, int);
-template __global__ void k_plane<SEQ_MU>(Launch, int);
-template __global__ void k_plane<SEQ_CEX>(Launch, int);
-template __global__ void k_plane<SEQ_ATM>(Launch, int);
-template __global__ void k_plane<SEQ_SUM>(Launch, int);
-template __global__ void k_plane<SEQ_STEP>(Launch, int);
-template __global__ void k_plane<SEQ_LOSS>(Launch, int);
+template __global__ void k_plane<SEQ_E, 8>(Launch, int);
+template __global__ void k_plane<SEQ_MU, 8>(Launch, int);
+template __global__ void k_plane<SEQ_CEX, 8>(Launch, int);
+template __global__ void k_plane<SEQ_ATM, 8>(Launch, int);
+template __global__ void k_plane<SEQ_SUM, 8>(Launch, int);
+template __global__ void k_plane<SEQ_STEP, 8>(Launch, int);
+template __global__ void k_plane<SEQ_LOSS, 8>(Launch, int);
+template __global__ void k_plane<SEQ_E, 16>(Launch, int);
+template __global__ void k_plane<SEQ_MU, 16>(Launch, int);
+template __global__ void k_plane<SEQ_CEX, 16>(Launch, int);
+template __global__ void k_plane<SEQ_ATM, 16>(Launch, int);
+template __global__ void k_plane<SEQ_SUM, 16>(Launch, int);
+template __global__ void k_plane<SEQ_STEP, 16>(Launch, int);
+template __global__ void k_plane<SEQ_LOSS, 16>(Launch, int);
 
 }  // namespace ramgpu

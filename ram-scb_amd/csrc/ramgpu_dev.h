@@ -24,7 +24,7 @@ namespace ramgpu {
 constexpr int kMaxSpecies = 8;
 constexpr int kLineThreads = 256;      // R/P sweep kernels: one line per thread
 constexpr int kPlaneThreads = 512;     // plane kernels: one (K,L) plane per block
-constexpr int kMaxCellsPerThread = 12; // NE*NPA <= kPlaneThreads*kMaxCellsPerThread
+constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
 
 // packed per-(I,J) terms of the plane sweeps, kij[(i*NT+j)*8 + ..]
@@ -124,7 +124,7 @@ inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.n
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
   return (2 * P + 8 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 64 +
-          (size_t)ne + npa) * sizeof(double);
+          2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
 }
 
 }  // namespace ramgpu

@@ -232,7 +232,7 @@ enum PlaneSeq : int {
 
 // LDS layout of the plane kernel (doubles)
 struct PlaneLds {
-  double *F, *H, *cf, *cl, *kt, *lt, *red;
+  double *F, *H, *cf, *cl, *kt, *lt, *red, *gh;
   unsigned long long *cmx;
 };
 enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
@@ -248,7 +248,8 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.kt = d.cl + 2 * npa;     // KT_N*NE
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
   d.red = d.lt + LT_N * npa; // 64
-  d.cmx = (unsigned long long *)(d.red + 64);  // NE+NPA
+  d.gh = d.red + 64;         // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
+  d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE+NPA
   return d;
 }
 
@@ -257,6 +258,11 @@ struct PlaneCtx {
   size_t ij;
   double rlz, rrlz, qs, beta;
   float rne;
+  // row-wise mapping (DRIFTE, losses, SUMRC): this thread owns K = ke of rows
+  // L = le0, le0+lestep, ..; column-wise mapping (DRIFTMU): L = lm of
+  // columns K = km0, km0+kmstep, ..
+  int ke, le0, lestep, lm, km0, kmstep;
+  bool eact, mact;
 };
 
 // (K,L) of the n-th cell owned by this thread: cell index threadIdx.x + n*512
@@ -284,6 +290,14 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.beta = t.betalim;
   c.rne = t.rne;
   c.nmine = (c.P - (int)threadIdx.x + kPlaneThreads - 1) / kPlaneThreads;  // cells tid, tid+512, .. < P
+  c.lestep = kPlaneThreads / NE;
+  c.le0 = fast_div((int)threadIdx.x, NE, c.rne);
+  c.ke = (int)threadIdx.x - c.le0 * NE;
+  c.eact = (int)threadIdx.x < c.lestep * NE;
+  c.kmstep = kPlaneThreads / NPA;
+  c.lm = (int)threadIdx.x / c.kmstep;
+  c.km0 = (int)threadIdx.x - c.lm * c.kmstep;
+  c.mact = (int)threadIdx.x < c.kmstep * NPA;
   d = plane_lds(sm, NE, NPA);
   const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
   for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = gF[n];
