@@ -161,7 +161,10 @@ int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
 #define RAMGPU_KERNEL_FINALIZE 3 /* DtDrift* and SUMRC bookkeeping            */
 #define RAMGPU_KERNEL_VS 4       /* Volland-Stern VT from Kp                  */
 #define RAMGPU_KERNEL_P 5        /* azimuthal sweeps (DRIFTP)                 */
-#define RAMGPU_KERNEL_KINDS 6
+#define RAMGPU_KERNEL_R_BWD 6    /* adjoint of DRIFTR                         */
+#define RAMGPU_KERNEL_P_BWD 7    /* adjoint of DRIFTP                         */
+#define RAMGPU_KERNEL_PLANE_BWD 8 /* adjoints of DRIFTE, DRIFTMU, losses      */
+#define RAMGPU_KERNEL_KINDS 9
 int ramgpu_kernel_timing(ramgpu_ctx *ctx, int enable);
 /* Measurement support: a plain device copy of `bytes` (read + write) run
  * `reps` times on device `device`; gbps receives the achieved read+write

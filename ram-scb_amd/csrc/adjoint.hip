@@ -340,12 +340,8 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   // forward face of cell (k,l): Courant number, limited value and slots
   auto eface = [&](int k, int l, double &ce, FaceD &D, int (&tg)[4], double (&gf)[4]) {
-    const double *e = d.cf + l * 8;
-    const double edt1 =
-        div_rn(div_rn(div_rn(d.kt[KT_EKP * NE + k], e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
-    const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
-    const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
-    ce = d.kt[KT_EDOT * NE + k] * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
+    ce = c_energy(d.cf + l * 8, d.kt[KT_EKP * NE + k], d.kt[KT_EDOT * NE + k], drd1, dpd1, bij, rbij, rlz, rrlz,
+                  qs);
     const double *row = d.F + l * NE;
     const double g1 = row[1] * r0 / r1 * r2;
     const double g0 = g1 * r3 / r0 * r4;
@@ -461,12 +457,7 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   // face L = l+1 (2..NPA-1) of column k: Courant number, value, slots (rows)
   auto mface = [&](int k, int l, double &cm, FaceD &D, int (&tg)[4]) {
-    const double *m = d.cf + l * 8;
-    const double edt =
-        div_rn(div_rn(div_rn(d.kt[KT_MKP * NE + k], m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij) * qs;
-    const double drdm = drm1 + edt * m[KLM_DRM2] * rlz;
-    const double dpdm = dpm1 - edt * m[KLM_DPM2];
-    cm = -m[KLM_CMUDOT] * (m[KLM_GMR] * drdm + m[KLM_GMP] * dpdm + db2 + m[KLM_DI2]);
+    cm = c_mu(d.cf + l * 8, d.kt[KT_MKP * NE + k], drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
     const int L1 = l + 1, cc = l * NE + k;
     if (L1 == NPA - 1) {  // FBND(NPA-1) = F(NPA)
       D.fb = d.F[cc + NE];

@@ -67,7 +67,7 @@ size_t bwd_lds(const ramgpu_ctx *c) {
 // backward kernels over species s0..s0+nsp-1
 int bwd_r(ramgpu_ctx *c, int final_sweep, const double *x, double *adj, double *bbar, int s0, int nsp) {
   const Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
-  tbeg(c, RAMGPU_KERNEL_R);
+  tbeg(c, RAMGPU_KERNEL_R_BWD);
   if (final_sweep)
     hipLaunchKernelGGL(k_r_bwd<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj, bbar);
   else
@@ -78,7 +78,7 @@ int bwd_r(ramgpu_ctx *c, int final_sweep, const double *x, double *adj, double *
 
 int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
   const Launch L = launch_of(c, s0, nsp, blocks_p(c->t));
-  tbeg(c, RAMGPU_KERNEL_P);
+  tbeg(c, RAMGPU_KERNEL_P_BWD);
   hipLaunchKernelGGL(k_p_bwd, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj);
   tend(c);
   return launch_check("k_p_bwd");
@@ -89,7 +89,7 @@ int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int 
   set_bwd_lds_limit(c->lds_limit);
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = bwd_lds(c);
-  tbeg(c, RAMGPU_KERNEL_PLANE);
+  tbeg(c, RAMGPU_KERNEL_PLANE_BWD);
   switch (seq) {
     case SEQ_E: hipLaunchKernelGGL(k_plane_bwd<SEQ_E>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
     case SEQ_MU: hipLaunchKernelGGL(k_plane_bwd<SEQ_MU>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
@@ -101,23 +101,25 @@ int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int 
   return launch_check("k_plane_bwd");
 }
 
-// One ram_run step (all species) with the input of every operator saved to
-// tape[slot*nf2]: the fused step's kernels run one operator at a time, with
-// the same arithmetic, so the state, the SUMRC bookkeeping and the CFL
-// limits are those of step_once.
-int taped_step(ramgpu_ctx *c, double *tape) {
+// One ram_run step (all species) from tape slot T_R1 (the step's input,
+// filled by the caller) to y, every operator writing its output into the
+// next slot, so that the tape holds the input of each operator.  The
+// kernels are those of the fused step, one operator at a time, with the
+// same arithmetic: the state, the SUMRC bookkeeping and the CFL limits are
+// those of step_once.
+int taped_step(ramgpu_ctx *c, double *tape, double *y) {
   int rc;
   if ((rc = prepare(c))) return rc;
   const int rec = c->cfl_valid ? 0 : 1, ns = c->ns;
   const size_t n = nf2(c);
-  double *f = c->t.f2;
-  auto save = [&](int slot) { return copy_d2d(c, tape + (size_t)slot * n, f, n); };
-  if ((rc = save(T_R1)) || (rc = run_r(c, rec, 0, ns)) || (rc = save(T_P1)) || (rc = run_p(c, rec, 0, ns)) ||
-      (rc = save(T_E1)) || (rc = run_plane(c, SEQ_E, rec, 0, ns)) || (rc = save(T_MU1)) ||
-      (rc = run_plane(c, SEQ_MU, rec, 0, ns)) || (rc = save(T_LOSS)) || (rc = run_plane(c, SEQ_LOSS, 0, 0, ns)) ||
-      (rc = save(T_MU2)) || (rc = run_plane(c, SEQ_MU, 0, 0, ns)) || (rc = save(T_E2)) ||
-      (rc = run_plane(c, SEQ_E, 0, 0, ns)) || (rc = save(T_P2)) || (rc = run_p(c, 0, 0, ns)) ||
-      (rc = save(T_R2)) || (rc = run_r(c, 2, 0, ns)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+  auto x = [&](int slot) { return tape + (size_t)slot * n; };
+  if ((rc = run_r(c, rec, 0, ns, x(T_R1), x(T_P1))) || (rc = run_p(c, rec, 0, ns, x(T_P1), x(T_E1))) ||
+      (rc = run_plane(c, SEQ_E, rec, 0, ns, x(T_E1), x(T_MU1))) ||
+      (rc = run_plane(c, SEQ_MU, rec, 0, ns, x(T_MU1), x(T_LOSS))) ||
+      (rc = run_plane(c, SEQ_LOSS, 0, 0, ns, x(T_LOSS), x(T_MU2))) ||
+      (rc = run_plane(c, SEQ_MU, 0, 0, ns, x(T_MU2), x(T_E2))) ||
+      (rc = run_plane(c, SEQ_E, 0, 0, ns, x(T_E2), x(T_P2))) || (rc = run_p(c, 0, 0, ns, x(T_P2), x(T_R2))) ||
+      (rc = run_r(c, 2, 0, ns, x(T_R2), y)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
     return rc;
   c->cfl_valid = true;
   return RAMGPU_OK;
@@ -233,11 +235,10 @@ int ramgpu_ram_run_bwd(ramgpu_ctx *c, double *adj, double *fgeos_adj) {
     if ((rc = b.alloc(&db, nfg(c)))) return rc;
     HIP_TRY(hipMemcpyAsync(db, fgeos_adj, nfg(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
-  // the step runs on the state and is undone afterwards
-  if ((rc = copy_d2d(c, save_f2, c->t.f2, n)) || (rc = copy_d2d(c, save_e, c->t.energy, (size_t)c->ns * 4)) ||
-      (rc = taped_step(c, tape)) || (rc = copy_d2d(c, c->t.f2, save_f2, n)) ||
-      (rc = copy_d2d(c, c->t.energy, save_e, (size_t)c->ns * 4)) || (rc = upload_f2(c, da, adj)) ||
-      (rc = step_bwd(c, tape, da, db)) || (rc = download_f2(c, adj, da)))
+  // the step runs from a copy of the state; the SUMRC state is restored
+  if ((rc = copy_d2d(c, tape, c->t.f2, n)) || (rc = copy_d2d(c, save_e, c->t.energy, (size_t)c->ns * 4)) ||
+      (rc = taped_step(c, tape, save_f2)) || (rc = copy_d2d(c, c->t.energy, save_e, (size_t)c->ns * 4)) ||
+      (rc = upload_f2(c, da, adj)) || (rc = step_bwd(c, tape, da, db)) || (rc = download_f2(c, adj, da)))
     return rc;
   if (db) {
     HIP_TRY(hipMemcpyAsync(fgeos_adj, db, nfg(c) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -308,10 +309,12 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   if (bbar && (rc = fill(c, bbar, 0.0, (size_t)nsteps * ng))) return rc;
   for (int g = nseg - 1; g >= 0; --g) {
     const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
-    if ((rc = copy_d2d(c, c->t.f2, ckpt + (size_t)g * n, n))) return rc;
-    for (int s = n0; s < n1; ++s)
-      if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n)))
+    if ((rc = copy_d2d(c, tape, ckpt + (size_t)g * n, n))) return rc;  // slot T_R1 of the first step
+    for (int s = n0; s < n1; ++s) {
+      double *y = (s == n1 - 1) ? c->t.f2 : tape + (size_t)(s + 1 - n0) * T_N * n + (size_t)T_R1 * n;
+      if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, y)))
         return rc;
+    }
     for (int s = n1 - 1; s >= n0; --s) {
       const double *y = (s == n1 - 1) ? c->t.f2 : tape + (size_t)(s + 1 - n0) * T_N * n + (size_t)T_R1 * n;
       if ((rc = misfit(y, s, scratch_loss, lam)) || (rc = set_step_inputs(c, d_kp, d_bs, s)) ||

@@ -173,8 +173,10 @@ int prepare(ramgpu_ctx *c) {
 
 // mode: 0 sweep, 1 sweep + CFL record, 2 last sweep of ram_run (SUMRC slot 5,
 // seam, mask)
-int run_r(ramgpu_ctx *c, int mode, int s0, int nsp) {
-  const Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
+int run_r(ramgpu_ctx *c, int mode, int s0, int nsp, const double *src, double *dst) {
+  Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
+  L.src = src;
+  L.dst = dst;
   tbeg(c, RAMGPU_KERNEL_R);
   if (mode == 0)
     hipLaunchKernelGGL(k_r<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
@@ -186,8 +188,14 @@ int run_r(ramgpu_ctx *c, int mode, int s0, int nsp) {
   return launch_check("k_r");
 }
 
-int run_p(ramgpu_ctx *c, int rec, int s0, int nsp) {
-  const Launch L = launch_of(c, s0, nsp, blocks_p(c->t));
+int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src, double *dst) {
+  // out of place, extra lines copy the unswept I=1 rows
+  const bool copy = dst && dst != src;
+  const int P = c->ne * c->npa;
+  const int nb = copy ? ((c->nr - 1) * P + c->nt * P + kLineThreads - 1) / kLineThreads : blocks_p(c->t);
+  Launch L = launch_of(c, s0, nsp, nb);
+  L.src = src;
+  L.dst = dst;
   tbeg(c, RAMGPU_KERNEL_P);
   if (rec)
     hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
@@ -199,8 +207,7 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp) {
 
 int plane_maxc(int ne, int npa) {
   const int e = (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne);
-  const int m = (ne + kPlaneThreads / npa - 1) / (kPlaneThreads / npa);
-  return (e <= 8 && m <= 8) ? 8 : 16;
+  return (e <= 8) ? 8 : 16;
 }
 
 template <int MAXC>
@@ -224,8 +231,12 @@ void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, cons
     launch_plane_t<16>(seq, grid, lds, st, L, rec);
 }
 
-int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp) {
-  const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
+int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *src, double *dst) {
+  // out of place (not SUMRC), extra blocks copy the unswept I=1 planes
+  const bool copy = dst && dst != src && seq != SEQ_SUM;
+  Launch L = launch_of(c, s0, nsp, blocks_kl(c->t) + (copy ? c->nt : 0));
+  L.src = src;
+  L.dst = dst;
   const size_t lds = plane_lds_bytes(c->ne, c->npa);
   tbeg(c, RAMGPU_KERNEL_PLANE);
   launch_plane(seq, plane_maxc(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, rec);
@@ -274,11 +285,9 @@ int check_dims(int nr, int nt, int ne, int npa) {
   const long long P = (long long)ne * npa;
   if (P > (long long)kPlaneThreads * kMaxCellsPerThread)
     return fail(RAMGPU_EINVAL, "NE*NPA=%lld above %d", P, kPlaneThreads * kMaxCellsPerThread);
-  // plane kernels: a thread owns one K of ceil(NPA/(512/NE)) rows, or one L of
-  // ceil(NE/(512/NPA)) columns
+  // plane kernels: a thread owns one K of ceil(NPA/(512/NE)) rows
   if (ne > kPlaneThreads || npa > kPlaneThreads ||
-      (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne) > kMaxCellsPerThread ||
-      (ne + kPlaneThreads / npa - 1) / (kPlaneThreads / npa) > kMaxCellsPerThread)
+      (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne) > kMaxCellsPerThread)
     return fail(RAMGPU_EINVAL, "plane %dx%d too large for one block of %d threads", ne, npa, kPlaneThreads);
   if ((long long)nt * P >= (1LL << 22) || (long long)nr * nt >= (1LL << 22))
     return fail(RAMGPU_EINVAL, "grid %dx%dx%dx%d: NT*NE*NPA must stay below 2^22", nr, nt, ne, npa);

@@ -94,9 +94,11 @@ ramgpu::Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb);
 int prepare(ramgpu_ctx *c);
 int fill_out(ramgpu_ctx *c, ramgpu_species_out *out);
 int check_species(ramgpu_ctx *c, int s);
-int run_r(ramgpu_ctx *c, int mode, int s0, int nsp);
-int run_p(ramgpu_ctx *c, int rec, int s0, int nsp);
-int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp);
+// sweeps of species s0..s0+nsp-1, in place on F2 or from src into dst
+int run_r(ramgpu_ctx *c, int mode, int s0, int nsp, const double *src = nullptr, double *dst = nullptr);
+int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src = nullptr, double *dst = nullptr);
+int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *src = nullptr,
+              double *dst = nullptr);
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
 int step_once(ramgpu_ctx *c);
 // cells per thread the plane kernels need for an NE x NPA plane (8 or 16)

@@ -387,8 +387,10 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
       const double p4k = t.p4[s * NE + k];
       const double beta = t.betalim;
       const size_t di = (size_t)NT * P;  // stride between I and I+1
-      double *fl = t.f2 + (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
-      double *fs = fl + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
+      const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
+      const double *fl = (L.src ? L.src : t.f2) + off;
+      double *fo = (L.dst ? L.dst : t.f2) + off;
+      double *fs = fo + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
       const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
       double gh1 = 0.0, gh2 = 0.0;
       if (in) {
@@ -410,8 +412,11 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
       double gprev = c1 * (in ? f2v : 0.0);
       if (MODE == 2) {
         const double mg1 = t.rij[j].w, mgs = t.rij[NT - 1].w;
-        if (mg1 == 1.0) fl[0] = 1e-31;
+        if (mg1 == 1.0) fo[0] = 1e-31;
+        else if (fo != fl) fo[0] = f1;
         if (j == 0) fs[0] = (mgs == 1.0) ? 1e-31 : f1;
+      } else if (fo != fl) {
+        fo[0] = f1;  // cell 1 is not updated
       }
       double wm1 = f1, w0 = f2v, w1 = f3, w2 = fnext;  // F(I-1), F(I), F(I+1), F(I+2) at I=2
       for (int I = 2; I <= NR; ++I) {
@@ -448,10 +453,10 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
         gprev = g;
         if (MODE == 2) {
           if (sum_line) acc += ekk * (v * wek * wml);
-          fl[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;
+          fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;
           if (j == 0) fs[(size_t)i * di] = (t.rij[(size_t)i * NT + NT - 1].w == 1.0) ? 1e-31 : v;
         } else {
-          fl[(size_t)i * di] = v;
+          fo[(size_t)i * di] = v;
         }
         wm1 = w0;
         w0 = w1;
@@ -488,14 +493,23 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
     const int P = NE * t.npa;
     const int line = u.b * kLineThreads + threadIdx.x;
     double cmax = 0.0;
-    if (line < (NR - 1) * P) {
+    if (line >= (NR - 1) * P && L.dst && L.dst != L.src) {
+      // out of place: the I=1 rows are not swept; extra lines copy them
+      const int x = line - (NR - 1) * P;
+      if (x < NT * P) {
+        const size_t off = (size_t)s * f2_species_stride(t) + x;
+        L.dst[off] = (L.src ? L.src : t.f2)[off];
+      }
+    } else if (line < (NR - 1) * P) {
       const int i1 = fast_div(line, P, t.rp);
       const int q = line - i1 * P;
       const int i = i1 + 1;
       const int l = fast_div(q, NE, t.rne), k = q - l * NE;
       const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
       const double beta = t.betalim, omet = t.omet;
-      double *fl = t.f2 + (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;  // F(J) at (J-1)*P
+      const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;  // F(J) at off+(J-1)*P
+      const double *fl = (L.src ? L.src : t.f2) + off;
+      double *fo = (L.dst ? L.dst : t.f2) + off;
       const double4 *pij = t.pij + (size_t)i * NT;
       const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
       // Courant number of face J (src/ModRamDrift.f90:236-239)
@@ -538,7 +552,7 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
         const double g = cp * face_value(w0, w1, fn, fnm1, cp, cp, beta);
         double v = w0 - g + gprev;
         if (v < 0) v = 1E-15;
-        fl[(size_t)(J - 1) * P] = v;
+        fo[(size_t)(J - 1) * P] = v;
         gprev = g;
         wm1 = w0;
         w0 = w1;
@@ -548,8 +562,8 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
 #undef CAZIM
       double v = FN - gN + gprev;
       if (v < 0) v = 1E-15;
-      fl[(size_t)(NT - 1) * P] = v;
-      fl[0] = v;
+      fo[(size_t)(NT - 1) * P] = v;
+      fo[0] = v;
     }
     if (MODE == 1) {
       const double m = block_max(cmax, red);
@@ -580,13 +594,11 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
 // DRIFTE on the plane (src/ModRamDrift.f90:333-369).  F(1),F(0) relativistic
 // ghosts from F(2), once per row; F(NE+1)=F(NE+2)=0.  Each thread owns one K
 // of several rows, so the per-K factors sit in registers and the CFL maximum
-// of its cells is one value.
+// of its cells is one value.  The per-L terms are in d.cf.
 template <int MAXC>
 __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   {
-    const double *src = t.kle + c.ij * NPA * 8;
-    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
     const double *rel = t.rel + c.s * 8;
     for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
       const double g1 = d.F[l * NE + 1] * rel[0] / rel[1] * rel[2];
@@ -611,12 +623,7 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
-    const double *e = d.cf + l * 8;
-    // CDriftE(I,J,K,L), :337-342
-    const double edt1 = div_rn(div_rn(div_rn(ekp, e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
-    const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
-    const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
-    const double ce = edot * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
+    const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz, qs);
     cmax = fmax(cmax, fmax(fabs(ce), 1E-10));
     const double *row = d.F + l * NE;
     const double fk = row[k];
@@ -656,47 +663,31 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
 // DRIFTMU on the plane (src/ModRamDrift.f90:413-466).  F(1)=F(2) for the
 // stencil; limiter faces L=2..NPA-2; CDriftMu(1)=0, FBND(1)=0,
 // FBND(NPA-1)=F(NPA); top cell F2(NPA)=F2(NPA-1)*FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1).
-// Each thread owns one L of several columns K: the per-L terms sit in
-// registers.
+// Same ownership as plane_e (consecutive lanes on consecutive K of a row);
+// the per-L terms are read from d.cm and d.lt (one address per row).
 template <int MAXC>
 __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
-  {
-    const double *src = t.klm + c.ij * NPA * 8;
-    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
-    if (rec)
-      for (int l = threadIdx.x; l < NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
+  if (rec) {
+    for (int l = threadIdx.x; l < NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
+    __syncthreads();
   }
-  __syncthreads();
   const double *k8 = t.kij + c.ij * 8;
   const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
   const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
-  const int l = c.lm, L1 = l + 1;
+  const int k = c.ke;
+  const double mkp = d.kt[KT_MKP * NE + k];
   double O[MAXC];
-  // faces L=2..NPA-1 carry a flux; face 1 carries none (CDriftMu(1)=0)
-  const bool face = c.mact && L1 >= 2 && L1 <= NPA - 1;
-  if (c.mact && L1 >= 2) {
-    const double *m = d.cf + l * 8;
-    const double bhs = m[KLM_BHS], rbhs = m[KLM_RBHS], drm2 = m[KLM_DRM2], dpm2 = m[KLM_DPM2];
-    const double cmudot = m[KLM_CMUDOT], gmr = m[KLM_GMR], gmp = m[KLM_GMP], di2 = m[KLM_DI2];
-    const double dmu = d.lt[LT_DMU * NPA + l], rdmu = d.lt[LT_RDMU * NPA + l];
-    const double wmu = d.lt[LT_WMU * NPA + l], rwmu = d.lt[LT_RWMU * NPA + l];
-    const double wmu1 = face ? d.lt[LT_WMU * NPA + l + 1] : 1.0, rwmu1 = face ? d.lt[LT_RWMU * NPA + l + 1] : 1.0;
-    double cmax = 0.0;
 #pragma unroll
-    for (int n = 0; n < MAXC; ++n) {
-      const int k = c.km0 + n * c.kmstep;
-      if (k >= NE) break;
-      const int cc = l * NE + k;
-      // CDriftMu(I,J,K,L), :418-433 (column L=NPA: only its CFL term)
-      const double edt = div_rn(div_rn(div_rn(d.kt[KT_MKP * NE + k], bhs, rbhs), rlz, rrlz), bij, rbij) * qs;
-      const double drdm = drm1 + edt * drm2 * rlz;
-      const double dpdm = dpm1 - edt * dpm2;
-      const double cm = -cmudot * (gmr * drdm + gmp * dpdm + db2 + di2);
-      cmax = fmax(cmax, fmax(fabs(cm), 1E-32));
-      double on = 0.0, h = 0.0;
-      if (face) {
+  for (int n = 0; n < MAXC; ++n) {
+    ROW_CELLS(n);
+    const int L1 = l + 1;
+    double on = 0.0, h = 0.0;
+    if (L1 >= 2) {
+      const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
+      if (rec) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
+      if (L1 <= NPA - 1) {
         double fb;
         const double f0 = d.F[cc];
         if (L1 == NPA - 1) {
@@ -711,30 +702,21 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
             fn = f0;
             fnm1 = (L1 == 2) ? f0 : d.F[cc - NE];  // F(1) = F(2)
           }
-          fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, dmu, rdmu), beta);
+          fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
         }
-        on = div_rn(cm, wmu, rwmu) * fb;
-        h = div_rn(cm, wmu1, rwmu1) * fb;
+        on = div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb;
+        h = div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * fb;
       }
-      O[n] = d.F[cc] - on;
-      d.H[cc] = h;
     }
-    if (rec) atomic_max_pos(&d.cmx[NE + l], cmax);
-  } else if (c.mact) {  // L = 1: no flux
-#pragma unroll
-    for (int n = 0; n < MAXC; ++n) {
-      const int k = c.km0 + n * c.kmstep;
-      if (k >= NE) break;
-      d.H[l * NE + k] = 0.0;
-    }
+    // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
+    O[n] = d.F[cc] - on;
+    d.H[cc] = h;
   }
   __syncthreads();
-  if (c.mact && l >= 1 && l <= NPA - 2) {
 #pragma unroll
-    for (int n = 0; n < MAXC; ++n) {
-      const int k = c.km0 + n * c.kmstep;
-      if (k >= NE) break;
-      const int cc = l * NE + k;
+  for (int n = 0; n < MAXC; ++n) {
+    ROW_CELLS(n);
+    if (l >= 1 && l <= NPA - 2) {
       double v = O[n] + d.H[cc - NE];
       if (v < 0) v = 1E-15;
       d.F[cc] = v;
@@ -744,8 +726,8 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   {
     const double hN = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA), hN1 = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
     const double muN = t.mu[NPA - 1], muN1 = t.mu[NPA - 2];
-    for (int k = threadIdx.x; k < NE; k += blockDim.x)
-      d.F[(NPA - 1) * NE + k] = d.F[(NPA - 2) * NE + k] * hN * muN / hN1 / muN1;
+    for (int kk = threadIdx.x; kk < NE; kk += blockDim.x)
+      d.F[(NPA - 1) * NE + kk] = d.F[(NPA - 2) * NE + kk] * hN * muN / hN1 / muN1;
   }
   if (rec) {
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
@@ -801,19 +783,33 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
 }
 
 template <int SEQ, int MAXC>
-__global__ void __launch_bounds__(kPlaneThreads, 4) k_plane(Launch L, int rec) {
+__global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(Launch L, int rec) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int NE = t.ne, NPA = t.npa, NT = t.nt;
+    if (u.b >= blocks_kl(t)) {  // out of place: copy plane (I=1, J) of the unswept row
+      const size_t off = (size_t)u.s * f2_species_stride(t) + (size_t)(u.b - blocks_kl(t)) * NE * NPA;
+      const double *src = (L.src ? L.src : t.f2) + off;
+      for (int n = threadIdx.x; n < NE * NPA; n += blockDim.x) L.dst[off + n] = src[n];
+      continue;
+    }
     constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS;
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, t.f2, need_loss, c, d);
-    double *gF = t.f2 + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+    plane_setup(t, u, sm, L.src ? L.src : t.f2, need_loss, c, d);
+    double *gF = (L.dst ? L.dst : t.f2) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
     const bool inside = t.kij[c.ij * 8 + KIJ_MG] == 0.0;
     const bool rec_here = rec && inside;
+    if constexpr (SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP) {  // per-L terms of both sweeps
+      const double *se = t.kle + c.ij * NPA * 8, *sm8 = t.klm + c.ij * NPA * 8;
+      for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) {
+        d.cf[n] = se[n];
+        d.cm[n] = sm8[n];
+      }
+      __syncthreads();
+    }
 
     if constexpr (SEQ == SEQ_E) {
       plane_e<MAXC>(t, c, d, rec_here);
@@ -881,7 +877,7 @@ __global__ void __launch_bounds__(kPlaneThreads, 4) k_plane(Launch L, int rec) {
 // slots 0..5 (slots 0..4 from the plane blocks, slot 5 from the R blocks).
 // One block per (problem, species).
 __global__ void k_finalize(Launch L, int cflmask, int summode) {
-  __shared__ double red[16];
+  __shared__ double red[4 * kSumSlots];
   __shared__ double S[kSumSlots];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
@@ -920,14 +916,19 @@ __global__ void k_finalize(Launch L, int cflmask, int summode) {
       for (int e = threadIdx.x; e < 2 + NE + NPA; e += blockDim.x) m[e] = 0ull;
     }
     if (summode != 0) {
+      // all slots in one pass: slots 0..4 from the plane blocks, slot 5 from the R blocks
       const int nslot = (summode == 1) ? 1 : kSumSlots;
-      for (int sl = 0; sl < nslot; ++sl) {
-        const int nb = (summode == 2 && sl == 5) ? blocks_r(t) : blocks_kl(t);
-        double acc[1] = {0.0}, tot[1];
-        for (int b = threadIdx.x; b < nb; b += blockDim.x) acc[0] += t.partials[((size_t)sl * t.ns + s) * t.maxblk + b];
-        block_sum_n<1>(acc, red, tot);
-        if (threadIdx.x == 0) S[sl] = tot[0];
-      }
+      const int nbk = blocks_kl(t), nbr = blocks_r(t);
+      const double *pp = t.partials + (size_t)s * t.maxblk;
+      const size_t slot = (size_t)t.ns * t.maxblk;
+      double acc[kSumSlots] = {0, 0, 0, 0, 0, 0}, tot[kSumSlots];
+      for (int b = threadIdx.x; b < nbk; b += blockDim.x)
+        for (int sl = 0; sl < nslot && sl < 5; ++sl) acc[sl] += pp[sl * slot + b];
+      if (nslot == kSumSlots)
+        for (int b = threadIdx.x; b < nbr; b += blockDim.x) acc[5] += pp[5 * slot + b];
+      block_sum_n<kSumSlots>(acc, red, tot);
+      if (threadIdx.x == 0)
+        for (int sl = 0; sl < kSumSlots; ++sl) S[sl] = tot[sl];
       __syncthreads();
       if (threadIdx.x == 0) {
         double *e = t.energy + s * 4;

@@ -100,11 +100,16 @@ struct Job {
 // species s0, s0+1, .. of tabs[0], nb blocks each; otherwise vmap[vb] is the
 // job of virtual block vb.  Blocks loop over virtual blocks with stride
 // gridDim.x, so the grid may be smaller than nv.
+// src/dst (sweep kernels): read F2 from src and write to dst (device layout
+// of the whole problem) instead of updating tabs[..].f2 in place; null for in
+// place.
 struct Launch {
   const Tab *tabs;
   const Job *jobs;
   const int *vmap;
   int nv, nb, s0;
+  const double *src;
+  double *dst;
 };
 
 inline __host__ __device__ size_t f2_species_stride(const Tab &t) {
@@ -123,7 +128,7 @@ inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.n
 // LDS bytes of the plane kernel
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
-  return (2 * P + 8 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 64 +
+  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 64 +
           2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
 }
 

@@ -179,6 +179,28 @@ __device__ __forceinline__ double c_radial(const Tab &t, double p4k, int i, int 
   return r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
 }
 
+// CDriftE(I,J,K,L) (src/ModRamDrift.f90:337-342) from the per-(I,J,L)
+// terms e[KleField], the per-(I,J) DRD1, DPD1, BNES and the per-(S,K)
+// prefix ekp and EDOT.
+__device__ __forceinline__ double c_energy(const double *e, double ekp, double edot, double drd1, double dpd1,
+                                           double bij, double rbij, double rlz, double rrlz, double qs) {
+  const double edt1 = div_rn(div_rn(div_rn(ekp, e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
+  const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
+  const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
+  return edot * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
+}
+
+// CDriftMu(I,J,K,L) (src/ModRamDrift.f90:418-433) from the per-(I,J,L)
+// terms m[KlmField], the per-(I,J) DRM1, DPM1, dBdt2, BNES and the per-(S,K)
+// prefix mkp.
+__device__ __forceinline__ double c_mu(const double *m, double mkp, double drm1, double dpm1, double db2,
+                                       double bij, double rbij, double rlz, double rrlz, double qs) {
+  const double edt = div_rn(div_rn(div_rn(mkp, m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij) * qs;
+  const double drdm = drm1 + edt * m[KLM_DRM2] * rlz;
+  const double dpdm = dpm1 - edt * m[KLM_DPM2];
+  return -m[KLM_CMUDOT] * (m[KLM_GMR] * drdm + m[KLM_GMP] * dpdm + db2 + m[KLM_DI2]);
+}
+
 __device__ __forceinline__ bool line_inflow(const Tab &t, int s, int j, int k, int l) {
   return c_radial(t, t.p4[s * t.ne + k], t.nr - 1, j, l) < 0;
 }
@@ -232,7 +254,7 @@ enum PlaneSeq : int {
 
 // LDS layout of the plane kernel (doubles)
 struct PlaneLds {
-  double *F, *H, *cf, *cl, *kt, *lt, *red, *gh;
+  double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *gh;
   unsigned long long *cmx;
 };
 enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
@@ -243,8 +265,9 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   PlaneLds d;
   d.F = sm;
   d.H = d.F + P;
-  d.cf = d.H + P;            // 8*NPA: current phase's per-L coefficients
-  d.cl = d.cf + 8 * npa;     // 2*NPA: HDNS, 1/FNHS
+  d.cf = d.H + P;            // 8*NPA: per-L DRIFTE terms (KleField)
+  d.cm = d.cf + 8 * npa;     // 8*NPA: per-L DRIFTMU terms (KlmField)
+  d.cl = d.cm + 8 * npa;     // 2*NPA: HDNS, 1/FNHS
   d.kt = d.cl + 2 * npa;     // KT_N*NE
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
   d.red = d.lt + LT_N * npa; // 64
@@ -258,11 +281,10 @@ struct PlaneCtx {
   size_t ij;
   double rlz, rrlz, qs, beta;
   float rne;
-  // row-wise mapping (DRIFTE, losses, SUMRC): this thread owns K = ke of rows
-  // L = le0, le0+lestep, ..; column-wise mapping (DRIFTMU): L = lm of
-  // columns K = km0, km0+kmstep, ..
-  int ke, le0, lestep, lm, km0, kmstep;
-  bool eact, mact;
+  // row-wise ownership of the forward plane kernels: this thread owns K = ke
+  // of rows L = le0, le0+lestep, ..
+  int ke, le0, lestep;
+  bool eact;
 };
 
 // (K,L) of the n-th cell owned by this thread: cell index threadIdx.x + n*512
@@ -294,10 +316,6 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.le0 = fast_div((int)threadIdx.x, NE, c.rne);
   c.ke = (int)threadIdx.x - c.le0 * NE;
   c.eact = (int)threadIdx.x < c.lestep * NE;
-  c.kmstep = kPlaneThreads / NPA;
-  c.lm = (int)threadIdx.x / c.kmstep;
-  c.km0 = (int)threadIdx.x - c.lm * c.kmstep;
-  c.mact = (int)threadIdx.x < c.kmstep * NPA;
   d = plane_lds(sm, NE, NPA);
   const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
   for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = gF[n];

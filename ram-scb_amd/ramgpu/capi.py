@@ -35,7 +35,7 @@ EXPORTS = (
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
 )
 
-KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p")
+KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd")
 
 
 class SpeciesOut(C.Structure):
