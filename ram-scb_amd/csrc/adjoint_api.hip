@@ -86,7 +86,10 @@ int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
 
 int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int nsp) {
   if (bwd_lds(c) > c->lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
-  set_bwd_lds_limit(c->lds_limit);
+  if (!c->bwd_lds_set) {
+    set_bwd_lds_limit(c->lds_limit);
+    c->bwd_lds_set = true;
+  }
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = bwd_lds(c);
   tbeg(c, RAMGPU_KERNEL_PLANE_BWD);

@@ -71,6 +71,7 @@ struct ramgpu_ctx {
   std::vector<hipEvent_t> tev;  // 2 per recorded launch
   std::vector<int> tkind;
   size_t tused = 0;
+  bool bwd_lds_set = false;  // LDS attribute of the plane adjoint kernels set
 };
 
 namespace ramgpu_host {
