@@ -2,10 +2,11 @@
 
 Tolerance (the stated bar for this FP64 path): the device result must match
 the oracle's to <= 1e-12 relative, norm-wise and cell-wise over cells above
-1e-12 of the maximum.  The device tabulates the drift coefficients in
-factored form and uses fused multiply-adds, so agreement is to rounding, not
-bit-exact.  SUMRC sums are compared to 1e-12 relative of SETRC (the oracle
-sums sequentially like the reference; the device sums in a tree).
+1e-12 of the maximum.  The drift operators are in fact bit-identical
+(checked exactly in test_drift_operators_bit_exact); the loss operators use
+the device's exp/log10/pow, which may differ from the host's in the last
+ulp.  SUMRC sums are compared to 1e-12 relative of SETRC (the oracle sums
+sequentially like the reference; the device sums in a tree).
 """
 import numpy as np
 import pytest
