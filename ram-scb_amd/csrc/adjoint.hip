@@ -22,21 +22,49 @@
 namespace ramgpu {
 
 // ---------------------------------------------------------------------------
+// Line adjoints.  Face I of a line carries G_I = C_I*FBND_I; cell I is
+// updated as F(I) - G_I + G_(I-1), so the adjoint of G_I is -w_I + w_(I+1),
+// with w the output adjoint of each unclamped updated cell.  FBND_I depends
+// on the stencil F(I), F(I+1) and, by the upwind side, on F(I+2), F(I+1)
+// (C<0) or F(I), F(I-1) (C>=0): fixed offsets from I, so a backward walk
+// keeps the adjoint of cells I-1..I+2 in four registers (acc[0..3]) and
+// writes cell I+2 back when the window moves on.  The stencil values are a
+// register window of the input, one load per cell, issued one step ahead.
+struct LFace {
+  double G, p0, p1, pn, pm;  // G and C * dFBND / d(F(I), F(I+1), F(N), F(N-1))
+  bool neg;                  // C < 0: N = I+2, else N = I
+};
+
+__device__ __forceinline__ LFace line_face(double c, double xm1, double x0, double x1, double x2, double beta) {
+  const bool neg = c < 0;
+  const FaceD D = face_d(x0, x1, neg ? x2 : x0, neg ? x1 : xm1, c, c, beta);
+  return {c * D.fb, c * D.d0, c * D.d1, c * D.dn, c * D.dm, neg};
+}
+
+// adjoint gb of face f's value into the window of cells I-1..I+2
+__device__ __forceinline__ void line_route(const LFace &f, double gb, double (&acc)[4]) {
+  const double a = f.pn * gb, b = f.pm * gb;
+  acc[1] += f.p0 * gb;
+  acc[2] += f.p1 * gb;
+  if (f.neg) {
+    acc[3] += a;
+    acc[2] += b;
+  } else {
+    acc[1] += a;
+    acc[0] += b;
+  }
+}
+
 // DRIFTR adjoint: one thread per radial line (J,K,L), walking I downward.
-// Face I (value G_I = C_I*FBND_I) feeds cells I (-G_I) and I+1 (+G_I); the
-// adjoint of G_I is -w_I + w_(I+1) with w the output adjoint of each
-// unclamped updated cell.  The partials of FBND_I go to stencil cells
-// I-1..I+2 (a 4-register window, written back once complete) or to FGEOS
-// (ghosts F(NR+1), F(NR+2), and the stale ghost of another line).
+// The outer ghosts are cells NR+1 and NR+2 of the window: F(NR+1) =
+// FGEOS*CONF1*FNHS and F(NR+2) = FGEOS*CONF2*FNHS on inflow lines; on an
+// outflow line F(NR+1) is the stale ghost that face NR-1 reads when its
+// Courant number is negative (another line's F(NR+1)), and face NR is
+// FBND(NR) = F(NR).  Face 1 is FBND(1) = F(2) on inflow lines, else 0.
+// The adjoints of the ghosts go to FGEOS.
 // FINAL: the last sweep of ram_run, whose output then went through
 // F2(:,:,NT)=F2(:,:,1) and the magnetopause mask; its J=NT line output was
 // discarded (zero adjoint), and line J=1 also receives line NT's adjoint.
-struct RFace {
-  double G;
-  double p[4];  // C * dFBND/d(slot)
-  int tgt[4];   // slot cell I' (1..NR), 0 none, -1 F(NR+1), -2 F(NR+2), -3 stale ghost
-};
-
 template <int FINAL>
 __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *xin, double *adj, double *bbar) {
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
@@ -58,16 +86,11 @@ __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *
     double *al = adj + off;
     double *as = al + (size_t)(NT - 1 - j) * P;  // line J=NT
     const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
-    double gh1 = 0.0, gh2 = 0.0, dg1 = 0.0, dg2 = 0.0;
+    double gh1 = 0.0, gh2 = 0.0;
     if (in) {
       gh1 = ghost_value(t, s, j, k, l, t.conf1);
       gh2 = ghost_value(t, s, j, k, l, t.conf2);
-      dg1 = ghost_dfgeos(t, s, j, l, t.conf1);
-      dg2 = ghost_dfgeos(t, s, j, l, t.conf2);
     }
-    const long own = s + (long)t.ns * (j + (long)NT * (k + (long)NE * l));
-    double gacc1 = 0.0, gacc2 = 0.0;
-
     auto X = [&](int I) -> double { return xl[(size_t)(I - 1) * di]; };
     auto ybar = [&](int I) -> double {
       double y = al[(size_t)(I - 1) * di];
@@ -77,131 +100,83 @@ __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *
       }
       return y;
     };
-    auto face = [&](int I) -> RFace {
-      RFace f;
-      const int i = I - 1;
-      const double ci = c_radial(t, p4k, i, j, l);
-      f.tgt[0] = f.tgt[1] = f.tgt[2] = f.tgt[3] = 0;
-      f.p[0] = f.p[1] = f.p[2] = f.p[3] = 0.0;
-      if (I == 1) {  // FBND(1) = F(2) on inflow lines, else 0
-        f.G = ci * (in ? X(2) : 0.0);
-        if (in) {
-          f.tgt[0] = 2;
-          f.p[0] = ci;
-        }
-        return f;
-      }
-      if (I == NR && !in) {  // FBND(NR) = F(NR)
-        f.G = ci * X(NR);
-        f.tgt[0] = NR;
-        f.p[0] = ci;
-        return f;
-      }
-      const double f0 = X(I);
-      const double fp1 = (I == NR) ? gh1 : X(I + 1);
-      const int tp1 = (I == NR) ? -1 : I + 1;
-      double fn, fnm1;
-      int tn, tm;
-      if (ci < 0) {
-        fnm1 = fp1;
-        tm = tp1;
-        if (I == NR) {
-          fn = gh2;
-          tn = -2;
-        } else if (I + 2 <= NR) {
-          fn = X(I + 2);
-          tn = I + 2;
-        } else if (in) {
-          fn = gh1;
-          tn = -1;
-        } else {
-          fn = stale_ghost(t, s, j, k, l);
-          tn = -3;
-        }
-      } else {
-        fn = f0;
-        tn = I;
-        fnm1 = X(I - 1);
-        tm = I - 1;
-      }
-      const FaceD D = face_d(f0, fp1, fn, fnm1, ci, ci, beta);
-      f.G = ci * D.fb;
-      f.tgt[0] = I;
-      f.p[0] = ci * D.d0;
-      f.tgt[1] = tp1;
-      f.p[1] = ci * D.d1;
-      f.tgt[2] = tn;
-      f.p[2] = ci * D.dn;
-      f.tgt[3] = tm;
-      f.p[3] = ci * D.dm;
-      return f;
-    };
-    // window acc[0..3] holds the adjoint of cells I-1..I+2 for face I
+    // face NR, then the window X(NR-2..NR+1) for the first step
+    double w0 = X(NR - 2), w1 = X(NR - 1), w2 = X(NR), w3 = gh1;
+    LFace fc;
+    {
+      const double c = c_radial(t, p4k, NR - 1, j, l);
+      if (in) fc = line_face(c, w1, w2, gh1, gh2, beta);
+      else fc = {c * w2, c, 0.0, 0.0, 0.0, false};  // FBND(NR) = F(NR)
+    }
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    auto route = [&](int I, int tgt, double v) {
-      if (tgt > 0) {
-        const int m = tgt - (I - 1);
-        if (m == 0) acc[0] += v;
-        else if (m == 1) acc[1] += v;
-        else if (m == 2) acc[2] += v;
-        else acc[3] += v;
-      } else if (tgt == -1) {
-        gacc1 += v;
-      } else if (tgt == -2) {
-        gacc2 += v;
-      } else if (tgt == -3 && bbar) {
-        const long src = stale_ghost_source(t, s, j, k, l);
-        if (src >= 0) {  // src = s + NS*(J' + NT*(K' + NE*L'))
-          const long r = src / t.ns;
-          const int jj = (int)(r % NT), ll = (int)(r / ((long)NT * NE));
-          atomicAdd(&bbar[src], v * ghost_dfgeos(t, s, jj, ll, t.conf1));
-        }
-      }
-    };
-    RFace fc = face(NR);
-    double wnext = 0.0;
+    double wnext = 0.0, g1 = 0.0, g2 = 0.0;
+    double xpf = X(NR - 3), ypf = ybar(NR);
     for (int I = NR; I >= 2; --I) {
-      const RFace fp = face(I - 1);
-      double v = X(I) - fc.G + fp.G;
-      const double w = (v < 0) ? 0.0 : ybar(I);
+      const double yI = ypf, xn = xpf;
+      if (I >= 5) xpf = X(I - 4);
+      if (I >= 3) ypf = ybar(I - 1);
+      // face I-1 from the window X(I-2..I+1)
+      LFace fp;
+      const double c = c_radial(t, p4k, I - 2, j, l);
+      if (I >= 3) {
+        double x2 = w3;
+        if (I == NR && !in && c < 0) x2 = stale_ghost(t, s, j, k, l);  // F(NR+1) read by face NR-1
+        fp = line_face(c, w0, w1, w2, x2, beta);
+      } else {
+        fp = {c * (in ? w2 : 0.0), 0.0, in ? c : 0.0, 0.0, 0.0, false};  // FBND(1)
+      }
+      const double v = w2 - fc.G + fp.G;
+      const double w = (v < 0) ? 0.0 : yI;
       acc[1] += w;  // cell I's own term
-      const double gb = -w + wnext;
-      for (int m = 0; m < 4; ++m)
-        if (fc.tgt[m] != 0) route(I, fc.tgt[m], fc.p[m] * gb);
+      line_route(fc, -w + wnext, acc);
       if (I + 2 <= NR) al[(size_t)(I + 1) * di] = acc[3];
+      else if (I + 2 == NR + 1) g1 = acc[3];
+      else g2 = acc[3];
       acc[3] = acc[2];
       acc[2] = acc[1];
       acc[1] = acc[0];
       acc[0] = 0.0;
       fc = fp;
       wnext = w;
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+      w0 = xn;
     }
     // face 1 (window: cells 0..3); cell 1 is not updated: its output adjoint passes
-    for (int m = 0; m < 4; ++m)
-      if (fc.tgt[m] != 0) route(1, fc.tgt[m], fc.p[m] * wnext);
+    line_route(fc, wnext, acc);
     acc[1] += ybar(1);
-    if (NR >= 3) al[2 * di] = acc[3];
+    al[2 * di] = acc[3];
     al[di] = acc[2];
     al[0] = acc[1];
     if (FINAL && j == 0)  // line NT's adjoint has been folded into line 1
       for (int I = 1; I <= NR; ++I) as[(size_t)(I - 1) * di] = 0.0;
-    if (bbar && (gacc1 != 0.0 || gacc2 != 0.0)) atomicAdd(&bbar[own], gacc1 * dg1 + gacc2 * dg2);
+    if (bbar) {
+      if (in) {
+        if (g1 != 0.0 || g2 != 0.0) {
+          const long own = s + (long)t.ns * (j + (long)NT * (k + (long)NE * l));
+          atomicAdd(&bbar[own], g1 * ghost_dfgeos(t, s, j, l, t.conf1) + g2 * ghost_dfgeos(t, s, j, l, t.conf2));
+        }
+      } else if (g1 != 0.0) {  // the stale ghost: F(NR+1) of an earlier inflow line
+        const long src = stale_ghost_source(t, s, j, k, l);
+        if (src >= 0) {  // src = s + NS*(J' + NT*(K' + NE*L'))
+          const long r = src / t.ns;
+          const int jj = (int)(r % NT), ll = (int)(r / ((long)NT * NE));
+          atomicAdd(&bbar[src], g1 * ghost_dfgeos(t, s, jj, ll, t.conf1));
+        }
+      }
+    }
   }
 }
 
-// ---------------------------------------------------------------------------
-// DRIFTP adjoint: one thread per ring (I>=2,K,L), walking J downward.  Cell
-// J>=2 is updated from faces J and J-1 (face 1 is face NT), and the output
-// F2(I,1) is a copy of F2(I,NT), so its adjoint joins cell NT's.  The input
-// F(1) only feeds stencils.  Stencil partials go to a 4-register window for
-// cells >= 4 and to three accumulators for cells 1..3, which the periodic
-// wrap reaches from faces NT-1 and NT.
-struct PFace {
-  double G;
-  double p[4];
-  int tgt[4];
-};
-
+// DRIFTP adjoint: one thread per ring (I>=2,K,L), walking J downward.  The
+// ring is periodic through cells NT+1 = 2 and NT+2 = 3 (face NT reads
+// F(2), F(3) as its F(NT+1), F(NT+2)); face NT-1 with C<0 reads F(2), F(1)
+// as its F(N), F(N-1) (the reference wraps N, not N-1).  Cell J>=2 is
+// updated from faces J and J-1, face 1 being face NT; the output F2(I,1) is
+// a copy of F2(I,NT), so its adjoint joins cell NT's, and the input F(1)
+// only feeds stencils.  The adjoints of cells 1..3, which the wrapped faces
+// reach first, collect in wr[] and join the window's at the end.
 __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *xin, double *adj) {
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
@@ -223,96 +198,78 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
     const double4 *pij = t.pij + (size_t)i * NT;
     const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
     auto X = [&](int J) -> double { return xl[(size_t)(J - 1) * P]; };
-    auto face = [&](int J) -> PFace {
+    auto cface = [&](int J) -> double {
       const double4 a = pij[J - 1];
       const double4 g = pl[J - 1];
       const double uu = div_rn(p2k * g.x, g.y, g.z);
       const double num = a.x - uu - a.y;
-      const double cp = div_rn(num, a.z, a.w) + omet;
-      int t0 = J, t1, tn, tm;
-      if (J == NT) {
-        t1 = 2;
-        if (cp < 0) {
-          tn = 3;
-          tm = 2;
-        } else {
-          tn = NT;
-          tm = NT - 1;
-        }
-      } else {
-        t1 = J + 1;
-        if (cp < 0) {
-          if (J + 2 <= NT) {
-            tn = J + 2;
-            tm = J + 1;
-          } else {
-            tn = 2;
-            tm = 1;
-          }
-        } else {
-          tn = J;
-          tm = J - 1;
-        }
-      }
-      const FaceD D = face_d(X(t0), X(t1), X(tn), X(tm), cp, cp, beta);
-      PFace f;
-      f.G = cp * D.fb;
-      f.tgt[0] = t0;
-      f.p[0] = cp * D.d0;
-      f.tgt[1] = t1;
-      f.p[1] = cp * D.d1;
-      f.tgt[2] = tn;
-      f.p[2] = cp * D.dn;
-      f.tgt[3] = tm;
-      f.p[3] = cp * D.dm;
-      return f;
+      return div_rn(num, a.z, a.w) + omet;
     };
-    const double yb1 = al[0], ybN = al[(size_t)(NT - 1) * P];
-    const PFace fN = face(NT), f2 = face(2);
-    double v2 = X(2) - f2.G + fN.G;
-    const double w2 = (v2 < 0) ? 0.0 : al[P];
+    const double x1 = X(1), x2 = X(2), x3 = X(3);
+    const double yb1 = al[0];
+    // face NT (stencil NT-1, NT, 2, 3) and face 2
+    double w0 = X(NT - 2), w1 = X(NT - 1), w2 = X(NT);
+    const LFace fN = line_face(cface(NT), w1, w2, x2, x3, beta);
+    const LFace f2 = line_face(cface(2), x1, x2, x3, X(4), beta);
+    const double v2 = x2 - f2.G + fN.G;
+    const double w2c = (v2 < 0) ? 0.0 : al[P];
     double wr[4] = {0.0, 0.0, 0.0, 0.0};  // cells 1..3 (index 0 unused)
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};  // cells J-1..J+2 (>= 4) for face J
-    auto route = [&](int J, int tgt, double v) {
-      if (tgt <= 3) {
-        if (tgt == 1) wr[1] += v;
-        else if (tgt == 2) wr[2] += v;
-        else wr[3] += v;
-      } else {
-        const int m = tgt - (J - 1);
-        if (m == 0) acc[0] += v;
-        else if (m == 1) acc[1] += v;
-        else if (m == 2) acc[2] += v;
-        else acc[3] += v;
-      }
-    };
-    PFace fc = fN;
-    double wnext = w2;  // face NT feeds cell 2 as face 1
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    LFace fc = fN;
+    double wnext = w2c;  // face NT feeds cell 2 as face 1
+    double w3 = x2;      // window X(J-2..J+1), X(NT+1) = X(2)
+    double xpf = X(NT - 3), ypf = al[(size_t)(NT - 1) * P] + yb1;
     for (int J = NT; J >= 3; --J) {
-      const PFace fp = (J - 1 == 2) ? f2 : face(J - 1);
-      double v = X(J) - fc.G + fp.G;
-      const double yb = (J == NT) ? ybN + yb1 : al[(size_t)(J - 1) * P];
-      const double w = (v < 0) ? 0.0 : yb;
-      route(J, J, w);
+      const double yJ = ypf, xn = xpf;
+      if (J >= 5) xpf = X(J - 4);
+      if (J >= 4) ypf = al[(size_t)(J - 2) * P];
+      LFace fp;
+      if (J - 1 == 2) {
+        fp = f2;
+      } else {
+        const double c = cface(J - 1);
+        // face NT-1 with C<0 reads F(N-1) = F(1), not F(NT)
+        if (J == NT && c < 0) {
+          const FaceD D = face_d(w1, w2, x2, x1, c, c, beta);
+          fp = {c * D.fb, c * D.d0, c * D.d1, c * D.dn, c * D.dm, true};
+        } else {
+          fp = line_face(c, w0, w1, w2, w3, beta);
+        }
+      }
+      const double v = w2 - fc.G + fp.G;
+      const double w = (v < 0) ? 0.0 : yJ;
+      acc[1] += w;
       const double gb = -w + wnext;
-      for (int m = 0; m < 4; ++m) route(J, fc.tgt[m], fc.p[m] * gb);
-      if (J + 2 <= NT && J + 2 >= 4) al[(size_t)(J + 1) * P] = acc[3];
+      if (J == NT - 1 && fc.neg) {  // face NT-1 with C<0: F(N) -> cell 2 (as NT+1), F(N-1) -> cell 1
+        acc[1] += fc.p0 * gb;
+        acc[2] += fc.p1 * gb;
+        acc[3] += fc.pn * gb;
+        wr[1] += fc.pm * gb;
+      } else {
+        line_route(fc, gb, acc);
+      }
+      if (J + 2 <= NT) al[(size_t)(J + 1) * P] = acc[3];
+      else if (J + 2 == NT + 1) wr[2] += acc[3];
+      else wr[3] += acc[3];
       acc[3] = acc[2];
       acc[2] = acc[1];
       acc[1] = acc[0];
       acc[0] = 0.0;
       fc = fp;
       wnext = w;
+      w3 = w2;
+      w2 = w1;
+      w1 = w0;
+      w0 = xn;
     }
     {  // J = 2
-      route(2, 2, w2);
-      const double gb = -w2 + wnext;
-      for (int m = 0; m < 4; ++m) route(2, f2.tgt[m], f2.p[m] * gb);
+      acc[1] += w2c;
+      line_route(f2, -w2c + wnext, acc);
       al[3 * (size_t)P] = acc[3];  // cell 4 (NT >= 4)
     }
-    al[0] = wr[1];
-    al[P] = wr[2];
-    al[2 * (size_t)P] = wr[3];
+    al[0] = wr[1] + acc[0];
+    al[P] = wr[2] + acc[1];
+    al[2 * (size_t)P] = wr[3] + acc[2];
   }
 }
 
@@ -325,6 +282,7 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)
 // or f0, fnm1 = fp1 (C<0) or F(K-1) / ghosts F(0), F(1).
+template <int MAXC>
 __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
   {
@@ -380,9 +338,9 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
     }
     D = face_d(f0, fp1, fn, fnm1, ce, div_rn(ce, d.kt[KT_DE * NE + k], d.kt[KT_RDE * NE + k]), beta);
   };
-  double O[kMaxCellsPerThread];
+  double O[MAXC];
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       double ce;
@@ -397,7 +355,7 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   __syncthreads();
   // flux weights w (unclamped updated cells) into Y; pass-through terms in O
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       const double yb = Y[cc];
@@ -414,11 +372,11 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n)
+  for (int n = 0; n < MAXC; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * kPlaneThreads] = O[n];
   __syncthreads();
-#pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+#pragma unroll 1
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       double ce;
@@ -442,6 +400,7 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
 // L and L-1 (face 1 carries nothing); the output F2(NPA) is F2(NPA-1) times
 // FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1), so its adjoint joins cell NPA-1's;
 // the input F(NPA) only feeds FBND(NPA-1) = F(NPA) and stencils.
+template <int MAXC>
 __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
   {
@@ -484,9 +443,9 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
     }
     D = face_d(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
   };
-  double O[kMaxCellsPerThread];
+  double O[MAXC];
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       double on = 0.0, h = 0.0;
@@ -505,7 +464,7 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       const double yb = Y[cc];
@@ -522,11 +481,11 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n)
+  for (int n = 0; n < MAXC; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * kPlaneThreads] = O[n];
   __syncthreads();
-#pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n) {
+#pragma unroll 1
+  for (int n = 0; n < MAXC; ++n) {
     if (n < c.nmine) {
       CELL_KL(n);
       if (l >= 1 && l <= NPA - 2) {
@@ -550,12 +509,13 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
 // Loss operators: F2 *= factor on their cell ranges, so the adjoint is
 // multiplied by the same factors.  which: 0 CHAREXCHANGE, 1 ATMOL, 2 the
 // chain CEX, ATMOL, ATMOL, CEX of ram_run.
+template <int MAXC>
 __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y, int which) {
   const int NE = c.NE;
   const bool cex = t.cex[c.s] != 0;
   const int upa = t.upa[c.i];
 #pragma unroll
-  for (int n = 0; n < kMaxCellsPerThread; ++n)
+  for (int n = 0; n < MAXC; ++n)
     if (n < c.nmine) {
       CELL_KL(n);
       const bool cx = cex && k >= 1 && l >= 1 && which != 1;
@@ -577,25 +537,51 @@ __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &
   __syncthreads();
 }
 
-template <int SEQ>
-__global__ void __launch_bounds__(kPlaneThreads, 4) k_plane_bwd(Launch L, const double *xin, double *adj) {
+// SEQ_STEP: the adjoint of the whole middle of ram_run (DRIFTE, DRIFTMU, the
+// loss chain, DRIFTMU, DRIFTE) in one pass over the plane.  xin is tape slot
+// 0; slots 0..4 hold the inputs of the five phases (as k_plane<SEQ_TAPE>
+// writes them).  The adjoint stays in LDS between the phases: each phase
+// leaves the adjoint of its input in H, which becomes the next phase's Y.
+template <int SEQ, int MAXC>
+__global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane_bwd(Launch L, const double *xin,
+                                                                                 double *adj) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_LOSS;
+    constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_LOSS || SEQ == SEQ_STEP;
+    const size_t po = (size_t)u.s * f2_species_stride(t), ns = f2_elems(t);
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, xin, need_loss, c, d);
+    plane_setup(t, u, sm, (SEQ == SEQ_STEP) ? xin + 4 * ns : xin, need_loss, c, d);
     double *Y = (double *)(d.cmx + c.NE + c.NPA);
-    double *ga = adj + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+    double *ga = adj + po + c.ij * c.P;
     for (int n = threadIdx.x; n < c.P; n += blockDim.x) Y[n] = ga[n];
     __syncthreads();
-    if constexpr (SEQ == SEQ_E) plane_e_bwd(t, c, d, Y);
-    else if constexpr (SEQ == SEQ_MU) plane_mu_bwd(t, c, d, Y);
-    else if constexpr (SEQ == SEQ_CEX) plane_loss_bwd(t, c, d, Y, 0);
-    else if constexpr (SEQ == SEQ_ATM) plane_loss_bwd(t, c, d, Y, 1);
-    else plane_loss_bwd(t, c, d, Y, 2);
+    if constexpr (SEQ == SEQ_E) plane_e_bwd<MAXC>(t, c, d, Y);
+    else if constexpr (SEQ == SEQ_MU) plane_mu_bwd<MAXC>(t, c, d, Y);
+    else if constexpr (SEQ == SEQ_CEX) plane_loss_bwd<MAXC>(t, c, d, Y, 0);
+    else if constexpr (SEQ == SEQ_ATM) plane_loss_bwd<MAXC>(t, c, d, Y, 1);
+    else if constexpr (SEQ == SEQ_LOSS) plane_loss_bwd<MAXC>(t, c, d, Y, 2);
+    else {
+      // phases alternate the two adjoint planes: A holds the output adjoint
+      // of a phase, B receives the adjoint of its input
+      const double *x = xin + po + c.ij * c.P;
+      double *const A = Y, *const B = d.H;
+      PlaneLds d2 = d;
+      d2.H = A;
+      plane_e_bwd<MAXC>(t, c, d, A);  // E2: A -> B
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[3 * ns + n];
+      __syncthreads();
+      plane_mu_bwd<MAXC>(t, c, d2, B);  // MU2: B -> A
+      plane_loss_bwd<MAXC>(t, c, d, A, 2);  // loss chain: A -> B (needs only its factors)
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[ns + n];
+      __syncthreads();
+      plane_mu_bwd<MAXC>(t, c, d2, B);  // MU1: B -> A
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[n];
+      __syncthreads();
+      plane_e_bwd<MAXC>(t, c, d, A);  // E1: A -> B
+    }
     for (int n = threadIdx.x; n < c.P; n += blockDim.x) ga[n] = d.H[n];
     __syncthreads();
   }
@@ -639,10 +625,18 @@ __global__ void k_fill(double *y, double v, size_t n) {
 
 template __global__ void k_r_bwd<0>(Launch, const double *, double *, double *);
 template __global__ void k_r_bwd<1>(Launch, const double *, double *, double *);
-template __global__ void k_plane_bwd<SEQ_E>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_MU>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_CEX>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_ATM>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_LOSS>(Launch, const double *, double *);
+
+template __global__ void k_plane_bwd<SEQ_E, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_MU, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_CEX, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_ATM, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_LOSS, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_STEP, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_E, 16>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_MU, 16>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_CEX, 16>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_ATM, 16>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_LOSS, 16>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_STEP, 16>(Launch, const double *, double *);
 
 }  // namespace ramgpu

@@ -21,6 +21,7 @@ using namespace ramgpu_host;
 namespace {
 
 // tape slots: the inputs of the operators of one ram_run step, in order
+// (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
 enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
 
 size_t nf2(const ramgpu_ctx *c) { return (size_t)c->ns * c->nr * c->nt * c->ne * c->npa; }
@@ -52,12 +53,28 @@ int copy_d2d(ramgpu_ctx *c, double *dst, const double *src, size_t n) {
   return RAMGPU_OK;
 }
 
-int set_bwd_lds_limit(size_t limit) {
-  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E>, (const void *)k_plane_bwd<SEQ_MU>,
-                       (const void *)k_plane_bwd<SEQ_CEX>, (const void *)k_plane_bwd<SEQ_ATM>,
-                       (const void *)k_plane_bwd<SEQ_LOSS>};
+template <int MAXC>
+void set_bwd_lds_limit_t(size_t limit) {
+  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E, MAXC>,   (const void *)k_plane_bwd<SEQ_MU, MAXC>,
+                       (const void *)k_plane_bwd<SEQ_CEX, MAXC>, (const void *)k_plane_bwd<SEQ_ATM, MAXC>,
+                       (const void *)k_plane_bwd<SEQ_LOSS, MAXC>, (const void *)k_plane_bwd<SEQ_STEP, MAXC>};
   for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
-  return RAMGPU_OK;
+}
+
+// cells per thread of the plane adjoints (cell n of a thread: tid + n*512)
+int bwd_maxc(const ramgpu_ctx *c) { return (c->ne * c->npa <= 8 * kPlaneThreads) ? 8 : 16; }
+
+template <int MAXC>
+void launch_plane_bwd(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x, double *adj) {
+  const dim3 blk(kPlaneThreads);
+  switch (seq) {
+    case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_CEX: hipLaunchKernelGGL((k_plane_bwd<SEQ_CEX, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_ATM: hipLaunchKernelGGL((k_plane_bwd<SEQ_ATM, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_LOSS: hipLaunchKernelGGL((k_plane_bwd<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    default: hipLaunchKernelGGL((k_plane_bwd<SEQ_STEP, MAXC>), grid, blk, lds, st, L, x, adj); break;
+  }
 }
 
 size_t bwd_lds(const ramgpu_ctx *c) {
@@ -87,19 +104,17 @@ int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
 int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int nsp) {
   if (bwd_lds(c) > c->lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
   if (!c->bwd_lds_set) {
-    set_bwd_lds_limit(c->lds_limit);
+    set_bwd_lds_limit_t<8>(c->lds_limit);
+    set_bwd_lds_limit_t<16>(c->lds_limit);
     c->bwd_lds_set = true;
   }
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = bwd_lds(c);
   tbeg(c, RAMGPU_KERNEL_PLANE_BWD);
-  switch (seq) {
-    case SEQ_E: hipLaunchKernelGGL(k_plane_bwd<SEQ_E>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
-    case SEQ_MU: hipLaunchKernelGGL(k_plane_bwd<SEQ_MU>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
-    case SEQ_CEX: hipLaunchKernelGGL(k_plane_bwd<SEQ_CEX>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
-    case SEQ_ATM: hipLaunchKernelGGL(k_plane_bwd<SEQ_ATM>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
-    default: hipLaunchKernelGGL(k_plane_bwd<SEQ_LOSS>, dim3(L.nv), dim3(kPlaneThreads), lds, c->stream, L, x, adj); break;
-  }
+  if (bwd_maxc(c) == 8)
+    launch_plane_bwd<8>(seq, dim3(L.nv), lds, c->stream, L, x, adj);
+  else
+    launch_plane_bwd<16>(seq, dim3(L.nv), lds, c->stream, L, x, adj);
   tend(c);
   return launch_check("k_plane_bwd");
 }
@@ -107,9 +122,9 @@ int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int 
 // One ram_run step (all species) from tape slot T_R1 (the step's input,
 // filled by the caller) to y, every operator writing its output into the
 // next slot, so that the tape holds the input of each operator.  The
-// kernels are those of the fused step, one operator at a time, with the
-// same arithmetic: the state, the SUMRC bookkeeping and the CFL limits are
-// those of step_once.
+// kernels are those of the fused step with the same arithmetic (the plane
+// kernel also storing its phase inputs): the state, the SUMRC bookkeeping
+// and the CFL limits are those of step_once.
 int taped_step(ramgpu_ctx *c, double *tape, double *y) {
   int rc;
   if ((rc = prepare(c))) return rc;
@@ -117,12 +132,9 @@ int taped_step(ramgpu_ctx *c, double *tape, double *y) {
   const size_t n = nf2(c);
   auto x = [&](int slot) { return tape + (size_t)slot * n; };
   if ((rc = run_r(c, rec, 0, ns, x(T_R1), x(T_P1))) || (rc = run_p(c, rec, 0, ns, x(T_P1), x(T_E1))) ||
-      (rc = run_plane(c, SEQ_E, rec, 0, ns, x(T_E1), x(T_MU1))) ||
-      (rc = run_plane(c, SEQ_MU, rec, 0, ns, x(T_MU1), x(T_LOSS))) ||
-      (rc = run_plane(c, SEQ_LOSS, 0, 0, ns, x(T_LOSS), x(T_MU2))) ||
-      (rc = run_plane(c, SEQ_MU, 0, 0, ns, x(T_MU2), x(T_E2))) ||
-      (rc = run_plane(c, SEQ_E, 0, 0, ns, x(T_E2), x(T_P2))) || (rc = run_p(c, 0, 0, ns, x(T_P2), x(T_R2))) ||
-      (rc = run_r(c, 2, 0, ns, x(T_R2), y)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+      (rc = run_plane(c, SEQ_TAPE, rec, 0, ns, x(T_E1), x(T_P2))) ||
+      (rc = run_p(c, 0, 0, ns, x(T_P2), x(T_R2))) || (rc = run_r(c, 2, 0, ns, x(T_R2), y)) ||
+      (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
     return rc;
   c->cfl_valid = true;
   return RAMGPU_OK;
@@ -137,9 +149,7 @@ int step_bwd(ramgpu_ctx *c, const double *tape, double *adj, double *bbar) {
   auto x = [&](int slot) { return tape + (size_t)slot * n; };
   int rc;
   if ((rc = bwd_r(c, 1, x(T_R2), adj, bbar, 0, ns)) || (rc = bwd_p(c, x(T_P2), adj, 0, ns)) ||
-      (rc = bwd_plane(c, SEQ_E, x(T_E2), adj, 0, ns)) || (rc = bwd_plane(c, SEQ_MU, x(T_MU2), adj, 0, ns)) ||
-      (rc = bwd_plane(c, SEQ_LOSS, x(T_LOSS), adj, 0, ns)) || (rc = bwd_plane(c, SEQ_MU, x(T_MU1), adj, 0, ns)) ||
-      (rc = bwd_plane(c, SEQ_E, x(T_E1), adj, 0, ns)) || (rc = bwd_p(c, x(T_P1), adj, 0, ns)) ||
+      (rc = bwd_plane(c, SEQ_STEP, x(T_E1), adj, 0, ns)) || (rc = bwd_p(c, x(T_P1), adj, 0, ns)) ||
       (rc = bwd_r(c, 0, x(T_R1), adj, bbar, 0, ns)))
     return rc;
   return RAMGPU_OK;
@@ -296,13 +306,26 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
                        lm);
     return launch_check("k_misfit");
   };
+  // the input of step s of segment g's tape (s in the segment) and the output of step s
+  auto tape_in = [&](int s, int n0) { return tape + (size_t)(s - n0) * T_N * n + (size_t)T_R1 * n; };
+  auto step_out = [&](int s, int n0, int n1) { return (s == n1 - 1) ? c->t.f2 : tape_in(s + 1, n0); };
+  const int nlast0 = (nseg - 1) * ckpt_every;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  // forward, keeping a checkpoint at the start of every segment
+  // forward, keeping a checkpoint at the start of every segment; the last
+  // segment is run taped at once, so the backward need not recompute it
   for (int s = 0; s < nsteps; ++s) {
-    if (s % ckpt_every == 0 && (rc = copy_d2d(c, ckpt + (size_t)(s / ckpt_every) * n, c->t.f2, n))) return rc;
-    if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = step_once(c)) ||
-        (rc = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr)))
+    if (s % ckpt_every == 0 && s < nlast0 && (rc = copy_d2d(c, ckpt + (size_t)(s / ckpt_every) * n, c->t.f2, n)))
       return rc;
+    if ((rc = set_step_inputs(c, d_kp, d_bs, s))) return rc;
+    if (s < nlast0) {
+      if ((rc = step_once(c)) || (rc = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr))) return rc;
+    } else {
+      if (s == nlast0 && (rc = copy_d2d(c, tape_in(s, nlast0), c->t.f2, n))) return rc;
+      double *y = step_out(s, nlast0, nsteps);
+      if ((rc = taped_step(c, tape + (size_t)(s - nlast0) * T_N * n, y)) ||
+          (rc = misfit(y, s, lossv + (size_t)s * npts, nullptr)))
+        return rc;
+    }
   }
   if ((rc = copy_d2d(c, final_f2, c->t.f2, n)) || (rc = copy_d2d(c, final_e, c->t.energy, (size_t)c->ns * 4)))
     return rc;
@@ -312,14 +335,15 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   if (bbar && (rc = fill(c, bbar, 0.0, (size_t)nsteps * ng))) return rc;
   for (int g = nseg - 1; g >= 0; --g) {
     const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
-    if ((rc = copy_d2d(c, tape, ckpt + (size_t)g * n, n))) return rc;  // slot T_R1 of the first step
-    for (int s = n0; s < n1; ++s) {
-      double *y = (s == n1 - 1) ? c->t.f2 : tape + (size_t)(s + 1 - n0) * T_N * n + (size_t)T_R1 * n;
-      if ((rc = set_step_inputs(c, d_kp, d_bs, s)) || (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, y)))
-        return rc;
+    if (g < nseg - 1) {  // recompute the segment with its tape
+      if ((rc = copy_d2d(c, tape, ckpt + (size_t)g * n, n))) return rc;  // slot T_R1 of the first step
+      for (int s = n0; s < n1; ++s)
+        if ((rc = set_step_inputs(c, d_kp, d_bs, s)) ||
+            (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, step_out(s, n0, n1))))
+          return rc;
     }
     for (int s = n1 - 1; s >= n0; --s) {
-      const double *y = (s == n1 - 1) ? c->t.f2 : tape + (size_t)(s + 1 - n0) * T_N * n + (size_t)T_R1 * n;
+      const double *y = step_out(s, n0, n1);
       if ((rc = misfit(y, s, scratch_loss, lam)) || (rc = set_step_inputs(c, d_kp, d_bs, s)) ||
           (rc = prepare(c)) ||
           (rc = step_bwd(c, tape + (size_t)(s - n0) * T_N * n, lam, bbar ? bbar + (size_t)s * ng : nullptr)))

@@ -220,6 +220,7 @@ void launch_plane_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch
     case SEQ_ATM: hipLaunchKernelGGL((k_plane<SEQ_ATM, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_SUM: hipLaunchKernelGGL((k_plane<SEQ_SUM, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_LOSS: hipLaunchKernelGGL((k_plane<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_TAPE: hipLaunchKernelGGL((k_plane<SEQ_TAPE, MAXC>), grid, blk, lds, st, L, rec); break;
     default: hipLaunchKernelGGL((k_plane<SEQ_STEP, MAXC>), grid, blk, lds, st, L, rec); break;
   }
 }
@@ -302,7 +303,8 @@ int set_plane_lds_limit(size_t limit) {
       (const void *)k_plane<SEQ_ATM, 8>,  (const void *)k_plane<SEQ_SUM, 8>,  (const void *)k_plane<SEQ_STEP, 8>,
       (const void *)k_plane<SEQ_LOSS, 8>, (const void *)k_plane<SEQ_E, 16>,   (const void *)k_plane<SEQ_MU, 16>,
       (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
-      (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>};
+      (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>, (const void *)k_plane<SEQ_TAPE, 8>,
+      (const void *)k_plane<SEQ_TAPE, 16>};
   for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
   return RAMGPU_OK;
 }

@@ -17,7 +17,7 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
 __global__ void k_tick(int *step);
 template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
 __global__ void k_p_bwd(Launch L, const double *xin, double *adj);
-template <int SEQ> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
+template <int SEQ, int MAXC> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
 __global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
                          const double *w, const double *target, double *loss_out, double *lam);
 __global__ void k_axpy(double *y, const double *x, double a, size_t n);

@@ -795,14 +795,15 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(La
       for (int n = threadIdx.x; n < NE * NPA; n += blockDim.x) L.dst[off + n] = src[n];
       continue;
     }
-    constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS;
+    constexpr bool need_loss =
+        SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS || SEQ == SEQ_TAPE;
     PlaneCtx c;
     PlaneLds d;
     plane_setup(t, u, sm, L.src ? L.src : t.f2, need_loss, c, d);
     double *gF = (L.dst ? L.dst : t.f2) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
     const bool inside = t.kij[c.ij * 8 + KIJ_MG] == 0.0;
     const bool rec_here = rec && inside;
-    if constexpr (SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP) {  // per-L terms of both sweeps
+    if constexpr (SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE) {  // per-L terms
       const double *se = t.kle + c.ij * NPA * 8, *sm8 = t.klm + c.ij * NPA * 8;
       for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) {
         d.cf[n] = se[n];
@@ -851,6 +852,23 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(La
       if (threadIdx.x == 0) t.partials[((size_t)0 * t.ns + c.s) * t.maxblk + u.b] = tot[0];
     } else if constexpr (SEQ == SEQ_LOSS) {
       plane_loss<MAXC>(t, c, d, u.b);
+    } else if constexpr (SEQ == SEQ_TAPE) {
+      // src is tape slot 0 (this phase's input); phase m's output, the input
+      // of phase m+1, goes to slot m+1, the last to dst.  The sweeps rewrite
+      // F only after a barrier; the loss chain rewrites its cells at once, so
+      // a barrier separates it from the store before it.
+      const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * c.P, ns = f2_elems(t);
+      double *slot = const_cast<double *>(L.src) + po;
+      plane_e<MAXC>(t, c, d, rec_here);
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[ns + n] = d.F[n];
+      plane_mu<MAXC>(t, c, d, rec_here);
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[2 * ns + n] = d.F[n];
+      __syncthreads();
+      plane_loss<MAXC>(t, c, d, u.b);
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[3 * ns + n] = d.F[n];
+      plane_mu<MAXC>(t, c, d, false);
+      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[4 * ns + n] = d.F[n];
+      plane_e<MAXC>(t, c, d, false);
     } else {  // SEQ_STEP
       plane_e<MAXC>(t, c, d, rec_here);
       plane_mu<MAXC>(t, c, d, rec_here);
@@ -1060,6 +1078,7 @@ template __global__ void k_plane<SEQ_ATM, 8>(Launch, int);
 template __global__ void k_plane<SEQ_SUM, 8>(Launch, int);
 template __global__ void k_plane<SEQ_STEP, 8>(Launch, int);
 template __global__ void k_plane<SEQ_LOSS, 8>(Launch, int);
+template __global__ void k_plane<SEQ_TAPE, 8>(Launch, int);
 template __global__ void k_plane<SEQ_E, 16>(Launch, int);
 template __global__ void k_plane<SEQ_MU, 16>(Launch, int);
 template __global__ void k_plane<SEQ_CEX, 16>(Launch, int);
@@ -1067,5 +1086,6 @@ template __global__ void k_plane<SEQ_ATM, 16>(Launch, int);
 template __global__ void k_plane<SEQ_SUM, 16>(Launch, int);
 template __global__ void k_plane<SEQ_STEP, 16>(Launch, int);
 template __global__ void k_plane<SEQ_LOSS, 16>(Launch, int);
+template __global__ void k_plane<SEQ_TAPE, 16>(Launch, int);
 
 }  // namespace ramgpu

@@ -250,7 +250,11 @@ enum PlaneSeq : int {
   SEQ_SUM = 4,   // SUMRC partial sum, slot 0
   SEQ_STEP = 5,  // the middle of ram_run: DRIFTE, DRIFTMU, loss chain, DRIFTMU, DRIFTE
   SEQ_LOSS = 6,  // the loss chain alone (CEX, ATMOL, ATMOL, CEX with five SUMRC partials)
+  SEQ_TAPE = 7,  // SEQ_STEP out of place, also writing the input of each later phase to a tape
 };
+
+// Elements of one F2-sized buffer (all species) of the problem t.
+__device__ __forceinline__ size_t f2_elems(const Tab &t) { return (size_t)t.ns * f2_species_stride(t); }
 
 // LDS layout of the plane kernel (doubles)
 struct PlaneLds {
