@@ -251,6 +251,22 @@ int ramgpu_batch_autotune(ramgpu_batch *batch, int reps, double *ms_best /* [3] 
 /* Launch grid caps of R, P, plane (0 = one block per work unit). */
 int ramgpu_batch_get_shapes(ramgpu_batch *batch, int *caps /* [3] */);
 int ramgpu_batch_set_shapes(ramgpu_batch *batch, const int *caps /* [3] */);
+/* Reverse mode of an ensemble: ramgpu_adjoint_run for every member at once
+ * (each kernel one launch over all members), from the members' current F2,
+ * with the per-member schedule of ramgpu_batch_ram_run_schedule (kp
+ * required, bscale may be NULL).  Member m has npts[m] misfit points
+ * pts[m][4p..4p+3], weights w[m][p*NPA_m + L-1] and targets
+ * target[m][n*npts[m]+p], as ramgpu_adjoint_run; loss[m] receives its loss,
+ * grad_f2[m] (may be NULL, or hold NULLs) its dL/dF2 at the start, and
+ * grad_fgeos[m] (likewise) its [nsteps](NS,NT,NE,NPA) dL/dFGEOS.  A member's
+ * results equal those of ramgpu_adjoint_run on that member alone.  When the
+ * taped segments of all members do not fit the device, each segment's states
+ * are kept and every step re-taped just before its adjoint (one more forward
+ * step per step, the same results). */
+int ramgpu_batch_adjoint_run(ramgpu_batch *batch, int nsteps, int ckpt_every, const double *kp,
+                             const double *bscale, const int *npts, const int *const *pts,
+                             const double *const *w, const double *const *target, double *loss,
+                             double *const *grad_f2, double *const *grad_fgeos);
 
 #ifdef __cplusplus
 }

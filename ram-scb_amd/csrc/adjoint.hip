@@ -82,9 +82,10 @@ __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *
     const double beta = t.betalim;
     const size_t di = (size_t)NT * P;
     const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
-    const double *xl = xin + off;
-    double *al = adj + off;
+    const double *xl = (xin ? xin : launch_src(L, t)) + off;
+    double *al = (adj ? adj : t.lam) + off;
     double *as = al + (size_t)(NT - 1 - j) * P;  // line J=NT
+    double *bb = bbar ? bbar : (t.bbar ? t.bbar + (size_t)L.bstep * t.ns * NT * NE * t.npa : nullptr);
     const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
     double gh1 = 0.0, gh2 = 0.0;
     if (in) {
@@ -151,18 +152,18 @@ __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *
     al[0] = acc[1];
     if (FINAL && j == 0)  // line NT's adjoint has been folded into line 1
       for (int I = 1; I <= NR; ++I) as[(size_t)(I - 1) * di] = 0.0;
-    if (bbar) {
+    if (bb) {
       if (in) {
         if (g1 != 0.0 || g2 != 0.0) {
           const long own = s + (long)t.ns * (j + (long)NT * (k + (long)NE * l));
-          atomicAdd(&bbar[own], g1 * ghost_dfgeos(t, s, j, l, t.conf1) + g2 * ghost_dfgeos(t, s, j, l, t.conf2));
+          atomicAdd(&bb[own], g1 * ghost_dfgeos(t, s, j, l, t.conf1) + g2 * ghost_dfgeos(t, s, j, l, t.conf2));
         }
       } else if (g1 != 0.0) {  // the stale ghost: F(NR+1) of an earlier inflow line
         const long src = stale_ghost_source(t, s, j, k, l);
         if (src >= 0) {  // src = s + NS*(J' + NT*(K' + NE*L'))
           const long r = src / t.ns;
           const int jj = (int)(r % NT), ll = (int)(r / ((long)NT * NE));
-          atomicAdd(&bbar[src], g1 * ghost_dfgeos(t, s, jj, ll, t.conf1));
+          atomicAdd(&bb[src], g1 * ghost_dfgeos(t, s, jj, ll, t.conf1));
         }
       }
     }
@@ -193,8 +194,8 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
     const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
     const double beta = t.betalim, omet = t.omet;
     const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;
-    const double *xl = xin + off;
-    double *al = adj + off;
+    const double *xl = (xin ? xin : launch_src(L, t)) + off;
+    double *al = (adj ? adj : t.lam) + off;
     const double4 *pij = t.pij + (size_t)i * NT;
     const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
     auto X = [&](int J) -> double { return xl[(size_t)(J - 1) * P]; };
@@ -551,11 +552,12 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane_bw
     const Tab &t = *u.t;
     constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_LOSS || SEQ == SEQ_STEP;
     const size_t po = (size_t)u.s * f2_species_stride(t), ns = f2_elems(t);
+    const double *x0 = xin ? xin : launch_src(L, t);
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, (SEQ == SEQ_STEP) ? xin + 4 * ns : xin, need_loss, c, d);
+    plane_setup(t, u, sm, (SEQ == SEQ_STEP) ? x0 + 4 * ns : x0, need_loss, c, d);
     double *Y = (double *)(d.cmx + c.NE + c.NPA);
-    double *ga = adj + po + c.ij * c.P;
+    double *ga = (adj ? adj : t.lam) + po + c.ij * c.P;
     for (int n = threadIdx.x; n < c.P; n += blockDim.x) Y[n] = ga[n];
     __syncthreads();
     if constexpr (SEQ == SEQ_E) plane_e_bwd<MAXC>(t, c, d, Y);
@@ -566,7 +568,7 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane_bw
     else {
       // phases alternate the two adjoint planes: A holds the output adjoint
       // of a phase, B receives the adjoint of its input
-      const double *x = xin + po + c.ij * c.P;
+      const double *x = x0 + po + c.ij * c.P;
       double *const A = Y, *const B = d.H;
       PlaneLds d2 = d;
       d2.H = A;
@@ -594,20 +596,38 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane_bw
 // one thread per point; with lam != NULL also lam += d loss / d F2
 // (= 2*(log(omega)-target)/omega * w, atomically: points may share cells).
 // pts[p] = {S,I,J,K} 1-based.
-__global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
-                         const double *w, const double *target, double *loss_out, double *lam) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= npts) return;
+__device__ inline void misfit_point(int p, const double *f2, int nr, int nt, int ne, int npa, const int *pts,
+                                    const double *w, const double *target, double *loss_out, double *lam) {
   const int s = pts[4 * p] - 1, i = pts[4 * p + 1] - 1, j = pts[4 * p + 2] - 1, k = pts[4 * p + 3] - 1;
   const size_t base = ((((size_t)s * nr + i) * nt + j) * npa) * ne + k;  // + l*NE
   const double *wp = w + (size_t)p * npa;
   double om = 0.0;
   for (int l = 0; l < npa; ++l) om += wp[l] * f2[base + (size_t)l * ne];
   const double r = log(om) - target[p];
-  loss_out[p] = r * r;
+  if (loss_out) loss_out[p] = r * r;
   if (lam) {
     const double g = 2.0 * r / om;
     for (int l = 0; l < npa; ++l) atomicAdd(&lam[base + (size_t)l * ne], g * wp[l]);
+  }
+}
+
+__global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
+                         const double *w, const double *target, double *loss_out, double *lam) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < npts) misfit_point(p, f2, nr, nt, ne, npa, pts, w, target, loss_out, lam);
+}
+
+// The same for every problem of an ensemble launch (one unit per problem):
+// the state after step `step` is the launch's source (a tape slot or F2);
+// losses go to Tab::mloss[step], and with adjoint != 0 dL/dF2 to Tab::lam.
+__global__ void k_misfit_ens(Launch L, int step, int adjoint) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const size_t np = (size_t)t.mnpts;
+    for (int p = threadIdx.x; p < t.mnpts; p += blockDim.x)
+      misfit_point(p, launch_src(L, t), t.nr, t.nt, t.ne, t.npa, t.mpts, t.mw, t.mtarget + (size_t)step * np,
+                   adjoint ? nullptr : t.mloss + (size_t)step * np, adjoint ? t.lam : nullptr);
   }
 }
 

@@ -18,11 +18,68 @@
 using namespace ramgpu;
 using namespace ramgpu_host;
 
-namespace {
+namespace ramgpu_host {
 
-// tape slots: the inputs of the operators of one ram_run step, in order
-// (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
-enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
+template <int MAXC>
+void set_bwd_lds_limit_t(size_t limit) {
+  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E, MAXC>,   (const void *)k_plane_bwd<SEQ_MU, MAXC>,
+                       (const void *)k_plane_bwd<SEQ_CEX, MAXC>, (const void *)k_plane_bwd<SEQ_ATM, MAXC>,
+                       (const void *)k_plane_bwd<SEQ_LOSS, MAXC>, (const void *)k_plane_bwd<SEQ_STEP, MAXC>};
+  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
+}
+
+// cells per thread of the plane adjoints (cell n of a thread: tid + n*512)
+int plane_bwd_maxc(int ne, int npa) { return (ne * npa <= 8 * kPlaneThreads) ? 8 : 16; }
+
+template <int MAXC>
+void launch_plane_bwd_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x, double *adj) {
+  const dim3 blk(kPlaneThreads);
+  switch (seq) {
+    case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_CEX: hipLaunchKernelGGL((k_plane_bwd<SEQ_CEX, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_ATM: hipLaunchKernelGGL((k_plane_bwd<SEQ_ATM, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_LOSS: hipLaunchKernelGGL((k_plane_bwd<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    default: hipLaunchKernelGGL((k_plane_bwd<SEQ_STEP, MAXC>), grid, blk, lds, st, L, x, adj); break;
+  }
+}
+
+void set_plane_bwd_lds_limit(size_t limit) {
+  set_bwd_lds_limit_t<8>(limit);
+  set_bwd_lds_limit_t<16>(limit);
+}
+
+void launch_plane_bwd(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
+                      double *adj) {
+  if (maxc <= 8)
+    launch_plane_bwd_t<8>(seq, grid, lds, st, L, x, adj);
+  else
+    launch_plane_bwd_t<16>(seq, grid, lds, st, L, x, adj);
+}
+
+// host Fortran-layout F2 <-> device-layout buffer (through the context's stage)
+int upload_f2(ramgpu_ctx *c, double *dst, const double *host) {
+  const size_t n = (size_t)c->ns * c->nr * c->nt * c->ne * c->npa;
+  HIP_TRY(hipMemcpyAsync(c->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_f2_in, dim3(1024), dim3(256), 0, c->stream, dst, (const double *)c->d_stage, c->ns, c->nr,
+                     c->nt, c->ne, c->npa);
+  return launch_check("k_f2_in");
+}
+
+int download_f2(ramgpu_ctx *c, double *host, const double *src) {
+  const size_t n = (size_t)c->ns * c->nr * c->nt * c->ne * c->npa;
+  hipLaunchKernelGGL(k_f2_out, dim3(1024), dim3(256), 0, c->stream, c->d_stage, src, c->ns, c->nr, c->nt, c->ne,
+                     c->npa);
+  int rc = launch_check("k_f2_out");
+  if (rc) return rc;
+  HIP_TRY(hipMemcpyAsync(host, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+}  // namespace ramgpu_host
+
+namespace {
 
 size_t nf2(const ramgpu_ctx *c) { return (size_t)c->ns * c->nr * c->nt * c->ne * c->npa; }
 size_t nfg(const ramgpu_ctx *c) { return (size_t)c->ns * c->nt * c->ne * c->npa; }
@@ -53,30 +110,6 @@ int copy_d2d(ramgpu_ctx *c, double *dst, const double *src, size_t n) {
   return RAMGPU_OK;
 }
 
-template <int MAXC>
-void set_bwd_lds_limit_t(size_t limit) {
-  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E, MAXC>,   (const void *)k_plane_bwd<SEQ_MU, MAXC>,
-                       (const void *)k_plane_bwd<SEQ_CEX, MAXC>, (const void *)k_plane_bwd<SEQ_ATM, MAXC>,
-                       (const void *)k_plane_bwd<SEQ_LOSS, MAXC>, (const void *)k_plane_bwd<SEQ_STEP, MAXC>};
-  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
-}
-
-// cells per thread of the plane adjoints (cell n of a thread: tid + n*512)
-int bwd_maxc(const ramgpu_ctx *c) { return (c->ne * c->npa <= 8 * kPlaneThreads) ? 8 : 16; }
-
-template <int MAXC>
-void launch_plane_bwd(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x, double *adj) {
-  const dim3 blk(kPlaneThreads);
-  switch (seq) {
-    case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_CEX: hipLaunchKernelGGL((k_plane_bwd<SEQ_CEX, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_ATM: hipLaunchKernelGGL((k_plane_bwd<SEQ_ATM, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_LOSS: hipLaunchKernelGGL((k_plane_bwd<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    default: hipLaunchKernelGGL((k_plane_bwd<SEQ_STEP, MAXC>), grid, blk, lds, st, L, x, adj); break;
-  }
-}
-
 size_t bwd_lds(const ramgpu_ctx *c) {
   return plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double);
 }
@@ -104,17 +137,13 @@ int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
 int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int nsp) {
   if (bwd_lds(c) > c->lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
   if (!c->bwd_lds_set) {
-    set_bwd_lds_limit_t<8>(c->lds_limit);
-    set_bwd_lds_limit_t<16>(c->lds_limit);
+    set_plane_bwd_lds_limit(c->lds_limit);
     c->bwd_lds_set = true;
   }
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = bwd_lds(c);
   tbeg(c, RAMGPU_KERNEL_PLANE_BWD);
-  if (bwd_maxc(c) == 8)
-    launch_plane_bwd<8>(seq, dim3(L.nv), lds, c->stream, L, x, adj);
-  else
-    launch_plane_bwd<16>(seq, dim3(L.nv), lds, c->stream, L, x, adj);
+  launch_plane_bwd(seq, plane_bwd_maxc(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, x, adj);
   tend(c);
   return launch_check("k_plane_bwd");
 }
@@ -152,26 +181,6 @@ int step_bwd(ramgpu_ctx *c, const double *tape, double *adj, double *bbar) {
       (rc = bwd_plane(c, SEQ_STEP, x(T_E1), adj, 0, ns)) || (rc = bwd_p(c, x(T_P1), adj, 0, ns)) ||
       (rc = bwd_r(c, 0, x(T_R1), adj, bbar, 0, ns)))
     return rc;
-  return RAMGPU_OK;
-}
-
-// host Fortran-layout F2 <-> device-layout buffer (through the context's stage)
-int upload_f2(ramgpu_ctx *c, double *dst, const double *host) {
-  const size_t n = nf2(c);
-  HIP_TRY(hipMemcpyAsync(c->d_stage, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_f2_in, dim3(1024), dim3(256), 0, c->stream, dst, (const double *)c->d_stage, c->ns, c->nr,
-                     c->nt, c->ne, c->npa);
-  return launch_check("k_f2_in");
-}
-
-int download_f2(ramgpu_ctx *c, double *host, const double *src) {
-  const size_t n = nf2(c);
-  hipLaunchKernelGGL(k_f2_out, dim3(1024), dim3(256), 0, c->stream, c->d_stage, src, c->ns, c->nr, c->nt, c->ne,
-                     c->npa);
-  int rc = launch_check("k_f2_out");
-  if (rc) return rc;
-  HIP_TRY(hipMemcpyAsync(host, c->d_stage, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
   return RAMGPU_OK;
 }
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <vector>
 
 #include "ctx.h"
@@ -25,7 +26,10 @@ using namespace ramgpu_host;
 
 namespace {
 
-enum BatchKind { BK_SCHED, BK_COEFIJ, BK_GHOST, BK_R, BK_P, BK_PLANE, BK_FIN, BK_N };
+// kernel kinds of an ensemble launch; the last four serve reverse mode: P and
+// plane out of place (with the copy lines / planes of the unswept I=1 rows),
+// one unit per member (misfit) and a whole-F2 copy per member
+enum BatchKind { BK_SCHED, BK_COEFIJ, BK_GHOST, BK_R, BK_P, BK_PLANE, BK_FIN, BK_POOP, BK_PLOOP, BK_MEM, BK_COPY, BK_N };
 
 struct Plan {
   Job *jobs = nullptr;
@@ -67,16 +71,20 @@ int batch_alloc(ramgpu_batch *b, void **p, size_t bytes) {
 }
 
 int blocks_of(int kind, const ramgpu_ctx *c) {
+  const int P = c->ne * c->npa;
   switch (kind) {
     case BK_COEFIJ: return (c->nr * c->nt + 255) / 256;
     case BK_R: return blocks_r(c->t);
     case BK_P: return blocks_p(c->t);
     case BK_PLANE: return blocks_kl(c->t);
+    case BK_POOP: return ((c->nr - 1) * P + c->nt * P + kLineThreads - 1) / kLineThreads;
+    case BK_PLOOP: return blocks_kl(c->t) + c->nt;
+    case BK_COPY: return 256;
     default: return 1;
   }
 }
 
-bool per_member(int kind) { return kind == BK_SCHED || kind == BK_COEFIJ; }
+bool per_member(int kind) { return kind == BK_SCHED || kind == BK_COEFIJ || kind == BK_MEM || kind == BK_COPY; }
 
 // Job table and block map of one kernel kind, members in order.
 int build_plan(ramgpu_batch *b, int kind) {
@@ -169,6 +177,78 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
   return launch_check("ensemble step");
 }
 
+
+// ---- reverse mode ----------------------------------------------------------
+// The launches of the ensemble adjoint name each member's buffers by tape
+// slot (Launch::ssl/dsl, 1-based; 0 = the member's F2 as source, -1 as an
+// out-of-place destination), so one launch serves members of any size.
+
+Launch slot_launch(const ramgpu_batch *b, int kind, int ssl, int dsl, int bstep = 0) {
+  Launch L = launch_for(b, kind);
+  L.ssl = ssl;
+  L.dsl = dsl;
+  L.bstep = bstep;
+  return L;
+}
+
+// VT / boundary scale of step `step` from the device schedule, then the
+// VT-dependent tables and the ghost carry
+void enq_sched(ramgpu_batch *b, int step) {
+  hipStream_t st = b->stream;
+  hipLaunchKernelGGL(k_setstep, dim3(1), dim3(64), 0, st, b->d_step, step);
+  const Launch ls = launch_for(b, BK_SCHED);
+  hipLaunchKernelGGL(k_sched, dim3(grid_for(b, BK_SCHED)), dim3(256), 0, st, ls, (const double *)b->d_kp,
+                     (const double *)b->d_bs, (const int *)b->d_bsoff, (const int *)b->d_step, (int)b->m.size(),
+                     b->sum_ns);
+  const Launch lc = launch_for(b, BK_COEFIJ);
+  hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc);
+  const Launch lg = launch_for(b, BK_GHOST);
+  hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), 0, st, lg);
+}
+
+void enq_copy(ramgpu_batch *b, int ssl, int dsl) {
+  const Launch L = slot_launch(b, BK_COPY, ssl, dsl);
+  hipLaunchKernelGGL(k_copy_ens, dim3(L.nv), dim3(256), 0, b->stream, L);
+}
+
+void enq_misfit(ramgpu_batch *b, int ssl, int step, int adjoint) {
+  const Launch L = slot_launch(b, BK_MEM, ssl, 0);
+  hipLaunchKernelGGL(k_misfit_ens, dim3(L.nv), dim3(64), 0, b->stream, L, step, adjoint);
+}
+
+// One step with every operator's input on the tape group starting at slot g
+// (0-based), the step's output to dsl_out; the arithmetic of enqueue_step.
+void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
+  hipStream_t st = b->stream;
+  Launch L = slot_launch(b, BK_R, g + T_R1 + 1, g + T_P1 + 1);
+  hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L);
+  L = slot_launch(b, BK_POOP, g + T_P1 + 1, g + T_E1 + 1);
+  hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), 0, st, L);
+  L = slot_launch(b, BK_PLOOP, g + T_E1 + 1, g + T_P2 + 1);
+  launch_plane(SEQ_TAPE, b->plane_maxc, dim3(L.nv), b->plane_lds, st, L, 1);
+  L = slot_launch(b, BK_POOP, g + T_P2 + 1, g + T_R2 + 1);
+  hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), 0, st, L);
+  L = slot_launch(b, BK_R, g + T_R2 + 1, dsl_out);
+  hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L);
+  const Launch lf = launch_for(b, BK_FIN);
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(256), 0, st, lf, 15, 2);
+}
+
+// adjoint of the step taped at group g: Tab::lam in place, Tab::bbar[step]
+void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_maxc) {
+  hipStream_t st = b->stream;
+  Launch L = slot_launch(b, BK_R, g + T_R2 + 1, 0, step);
+  hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L, nullptr, nullptr, nullptr);
+  L = slot_launch(b, BK_P, g + T_P2 + 1, 0);
+  hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
+  L = slot_launch(b, BK_PLANE, g + T_E1 + 1, 0);
+  launch_plane_bwd(SEQ_STEP, bwd_maxc, dim3(grid_for(b, BK_PLANE)), bwd_lds, st, L, nullptr, nullptr);
+  L = slot_launch(b, BK_P, g + T_P1 + 1, 0);
+  hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
+  L = slot_launch(b, BK_R, g + T_R1 + 1, 0, step);
+  hipLaunchKernelGGL(k_r_bwd<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L, nullptr, nullptr, nullptr);
+}
+
 int ensure_graph(ramgpu_batch *b, bool sched) {
   hipGraphExec_t &e = b->exec[sched ? 1 : 0];
   if (e) return RAMGPU_OK;
@@ -212,6 +292,40 @@ int finish(ramgpu_batch *b, bool sched, ramgpu_species_out *out) {
     if (rc) return rc;
     o += c->ns;
   }
+  return RAMGPU_OK;
+}
+
+// Upload a per-member Kp / boundary-scale schedule of nsteps steps (bscale
+// may be NULL: the members' current scales) and reset the step counter.
+int load_schedule(ramgpu_batch *b, int nsteps, const double *kp, const double *bscale) {
+  const int M = (int)b->m.size();
+  int rc;
+  if (nsteps > b->sched_cap) {
+    // the graph reads the schedule buffers by address: rebuild it with them
+    destroy_graphs(b);
+    int cap = std::max(nsteps, 1);
+    if ((rc = batch_alloc(b, (void **)&b->d_kp, (size_t)cap * M * sizeof(double))) ||
+        (rc = batch_alloc(b, (void **)&b->d_bs, (size_t)cap * b->sum_ns * sizeof(double))))
+      return rc;
+    b->sched_cap = cap;
+  }
+  HIP_TRY(hipMemcpyAsync(b->d_kp, kp, (size_t)nsteps * M * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  if (bscale) {
+    HIP_TRY(hipMemcpyAsync(b->d_bs, bscale, (size_t)nsteps * b->sum_ns * sizeof(double), hipMemcpyHostToDevice,
+                           b->stream));
+  } else {
+    std::vector<double> cur((size_t)nsteps * b->sum_ns);
+    for (int mi = 0; mi < M; ++mi) {
+      std::vector<double> sc(b->m[mi]->ns);
+      HIP_TRY(hipMemcpy(sc.data(), b->m[mi]->d_bscale, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
+      for (int n = 0; n < nsteps; ++n)
+        for (int s = 0; s < b->m[mi]->ns; ++s) cur[(size_t)n * b->sum_ns + b->bs_off[mi] + s] = sc[s];
+    }
+    HIP_TRY(hipMemcpy(b->d_bs, cur.data(), cur.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  const int zero = 0;
+  HIP_TRY(hipMemcpyAsync(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
   return RAMGPU_OK;
 }
 
@@ -308,33 +422,7 @@ int ramgpu_batch_ram_run_schedule(ramgpu_batch *b, int nsteps, const double *kp,
   for (size_t mi = 0; mi < b->m.size(); ++mi)
     if (!b->m[mi]->have_lzphi)
       return fail(RAMGPU_ESTATE, "member %zu: ramgpu_set_kp(lz, phi) must precede a Kp schedule", mi);
-  const int M = (int)b->m.size();
-  if (nsteps > b->sched_cap) {
-    // the graph reads the schedule buffers by address: rebuild it with them
-    destroy_graphs(b);
-    int cap = std::max(nsteps, 1);
-    if ((rc = batch_alloc(b, (void **)&b->d_kp, (size_t)cap * M * sizeof(double))) ||
-        (rc = batch_alloc(b, (void **)&b->d_bs, (size_t)cap * b->sum_ns * sizeof(double))))
-      return rc;
-    b->sched_cap = cap;
-  }
-  HIP_TRY(hipMemcpyAsync(b->d_kp, kp, (size_t)nsteps * M * sizeof(double), hipMemcpyHostToDevice, b->stream));
-  if (bscale) {
-    HIP_TRY(hipMemcpyAsync(b->d_bs, bscale, (size_t)nsteps * b->sum_ns * sizeof(double), hipMemcpyHostToDevice,
-                           b->stream));
-  } else {
-    std::vector<double> cur((size_t)nsteps * b->sum_ns);
-    for (int mi = 0; mi < M; ++mi) {
-      std::vector<double> sc(b->m[mi]->ns);
-      HIP_TRY(hipMemcpy(sc.data(), b->m[mi]->d_bscale, sc.size() * sizeof(double), hipMemcpyDeviceToHost));
-      for (int n = 0; n < nsteps; ++n)
-        for (int s = 0; s < b->m[mi]->ns; ++s) cur[(size_t)n * b->sum_ns + b->bs_off[mi] + s] = sc[s];
-    }
-    HIP_TRY(hipMemcpy(b->d_bs, cur.data(), cur.size() * sizeof(double), hipMemcpyHostToDevice));
-  }
-  const int zero = 0;
-  HIP_TRY(hipMemcpyAsync(b->d_step, &zero, sizeof(int), hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
+  if ((rc = load_schedule(b, nsteps, kp, bscale))) return rc;
   if ((rc = sync_members(b))) return rc;
   HIP_TRY(hipEventRecord(b->ev0, b->stream));
   if ((rc = run_steps(b, nsteps, true))) return rc;
@@ -351,6 +439,213 @@ int ramgpu_batch_last_run_ms(ramgpu_batch *b, double *ms) {
   HIP_TRY(hipEventElapsedTime(&f, b->ev0, b->ev1));
   *ms = f;
   return RAMGPU_OK;
+}
+
+// Reverse mode of an ensemble: the flux-misfit gradient of every member
+// (ramgpu_adjoint_run for each member, all members in the same launches).
+// Tape slots of member m, each one F2 of that member: nseg checkpoints,
+// then either one tape group per step of a segment (the segment is taped in
+// one recomputation) or, when that does not fit the device, seglen states
+// plus one group (the segment's states are recomputed, then each step is
+// re-taped just before its adjoint).  The arithmetic is the same either way.
+int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const double *kp, const double *bscale,
+                             const int *npts, const int *const *pts, const double *const *w,
+                             const double *const *target, double *loss, double *const *grad_f2,
+                             double *const *grad_fgeos) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  const int M = (int)b->m.size();
+  if (nsteps < 1 || ckpt_every < 1) return fail(RAMGPU_EINVAL, "nsteps and ckpt_every must be >= 1");
+  if (!kp) return fail(RAMGPU_EINVAL, "kp schedule is null");
+  if (!npts || !pts || !w || !target) return fail(RAMGPU_EINVAL, "misfit points, weights and targets required");
+  for (int mi = 0; mi < M; ++mi) {
+    const ramgpu_ctx *c = b->m[mi];
+    if (!c->have_lzphi) return fail(RAMGPU_ESTATE, "member %d: ramgpu_set_kp(lz, phi) must precede a Kp schedule", mi);
+    if (npts[mi] < 1 || !pts[mi] || !w[mi] || !target[mi])
+      return fail(RAMGPU_EINVAL, "member %d: misfit points, weights and targets required", mi);
+    for (int p = 0; p < npts[mi]; ++p) {
+      const int *q = pts[mi] + 4 * p;
+      if (q[0] < 1 || q[0] > c->ns || q[1] < 1 || q[1] > c->nr || q[2] < 1 || q[2] > c->nt || q[3] < 1 ||
+          q[3] > c->ne)
+        return fail(RAMGPU_EINVAL, "member %d: misfit point %d outside the grid", mi, p);
+    }
+  }
+  const int nseg = (nsteps + ckpt_every - 1) / ckpt_every, seglen = std::min(ckpt_every, nsteps);
+  // plane adjoint LDS
+  size_t bwd_lds = 0, lds_limit = b->m[0]->lds_limit;
+  int bwd_maxc = 8;
+  for (ramgpu_ctx *c : b->m) {
+    bwd_lds = std::max(bwd_lds, plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double));
+    bwd_maxc = std::max(bwd_maxc, plane_bwd_maxc(c->ne, c->npa));
+  }
+  if (bwd_lds > lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
+  set_plane_bwd_lds_limit(lds_limit);
+  // memory plan
+  size_t nall = 0, ngall = 0;
+  for (ramgpu_ctx *c : b->m) {
+    nall += (size_t)c->ns * c->nr * c->nt * c->ne * c->npa;
+    ngall += (size_t)c->ns * c->nt * c->ne * c->npa;
+  }
+  size_t freeb = 0, totb = 0;
+  HIP_TRY(hipMemGetInfo(&freeb, &totb));
+  const size_t fixed = 8 * (2 * nall + (grad_fgeos ? (size_t)nsteps * ngall : 0)) + ((size_t)64 << 20);
+  const size_t one = (size_t)(nseg + seglen * T_N) * nall * 8, two = (size_t)(nseg + seglen + T_N) * nall * 8;
+  // two-level: states, then one re-taped step at a time (forced with
+  // RAMGPU_ADJOINT_RETAPE=1, which tests the scheme on small ensembles)
+  const char *force = getenv("RAMGPU_ADJOINT_RETAPE");
+  bool split;
+  if (force && force[0] == '1' && fixed + two <= freeb * 0.92) split = true;
+  else if (fixed + one <= freeb * 0.92) split = false;
+  else if (fixed + two <= freeb * 0.92) split = true;
+  else return fail(RAMGPU_ENOMEM, "ensemble adjoint needs %.1f GB, %.1f GB free", (fixed + two) / 1e9, freeb / 1e9);
+  const int nslots = split ? nseg + seglen + T_N : nseg + seglen * T_N;
+  const int CK = 0, ST = nseg, TG = split ? nseg + seglen : nseg;
+  auto group = [&](int s, int n0) { return split ? TG : TG + (s - n0) * T_N; };
+  // buffers
+  std::vector<void *> bufs;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(b->stream);
+    for (void *q : bufs) (void)hipFree(q);
+  };
+  auto alloc = [&](void **q, size_t bytes) {
+    if (hipMalloc(q, std::max<size_t>(bytes, 16)) != hipSuccess) return false;
+    bufs.push_back(*q);
+    return true;
+  };
+  std::vector<double *> fin(M), fin_e(M);
+  for (int mi = 0; mi < M; ++mi) {
+    ramgpu_ctx *c = b->m[mi];
+    const size_t n = (size_t)c->ns * c->nr * c->nt * c->ne * c->npa, ng = (size_t)c->ns * c->nt * c->ne * c->npa;
+    const size_t np = npts[mi];
+    void *tape, *lam, *bb = nullptr, *mp, *mw, *mt, *ml, *f, *fe;
+    if (!alloc(&tape, (size_t)nslots * n * 8) || !alloc(&lam, n * 8) ||
+        (grad_fgeos && grad_fgeos[mi] && !alloc(&bb, (size_t)nsteps * ng * 8)) || !alloc(&mp, np * 4 * sizeof(int)) ||
+        !alloc(&mw, np * c->npa * 8) || !alloc(&mt, (size_t)nsteps * np * 8) || !alloc(&ml, (size_t)nsteps * np * 8) ||
+        !alloc(&f, n * 8) || !alloc(&fe, (size_t)c->ns * 4 * 8)) {
+      release();
+      return fail(RAMGPU_ENOMEM, "ensemble adjoint: hipMalloc failed (member %d)", mi);
+    }
+    HIP_TRY(hipMemcpy(mp, pts[mi], np * 4 * sizeof(int), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(mw, w[mi], np * c->npa * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(mt, target[mi], (size_t)nsteps * np * 8, hipMemcpyHostToDevice));
+    c->t.tape = (double *)tape;
+    c->t.lam = (double *)lam;
+    c->t.bbar = (double *)bb;
+    c->t.mpts = (const int *)mp;
+    c->t.mw = (const double *)mw;
+    c->t.mtarget = (const double *)mt;
+    c->t.mloss = (double *)ml;
+    c->t.mnpts = (int)np;
+    fin[mi] = (double *)f;
+    fin_e[mi] = (double *)fe;
+  }
+  auto detach = [&]() {  // the members' descriptors no longer name the buffers
+    for (ramgpu_ctx *c : b->m) {
+      c->t.tape = c->t.lam = c->t.bbar = c->t.mloss = nullptr;
+      c->t.mpts = nullptr;
+      c->t.mw = c->t.mtarget = nullptr;
+      c->t.mnpts = 0;
+      c->tab_dirty = true;
+      c->efield_dirty = true;  // VT and scales are those of the schedule
+    }
+    release();
+    return sync_members(b);
+  };
+  if ((rc = load_schedule(b, nsteps, kp, bscale)) || (rc = sync_members(b))) {
+    detach();
+    return rc;
+  }
+  hipStream_t st = b->stream;
+  const int nlast0 = (nseg - 1) * ckpt_every;
+  HIP_TRY(hipEventRecord(b->ev0, st));
+  // forward, keeping a checkpoint at the start of every segment; the last
+  // segment is taped at once when whole segments are taped
+  for (int s = 0; s < nsteps; ++s) {
+    enq_sched(b, s);
+    if (!split && s >= nlast0) {
+      if (s == nlast0) enq_copy(b, 0, group(s, nlast0) + T_R1 + 1);
+      const int out = (s == nsteps - 1) ? -1 : group(s + 1, nlast0) + T_R1 + 1;
+      enq_taped_step(b, group(s, nlast0), out);
+      enq_misfit(b, out < 0 ? 0 : out, s, 0);
+    } else {
+      if (s % ckpt_every == 0) enq_copy(b, 0, CK + s / ckpt_every + 1);
+      enqueue_step(b, false);
+      enq_misfit(b, 0, s, 0);
+    }
+  }
+  for (int mi = 0; mi < M; ++mi) {
+    ramgpu_ctx *c = b->m[mi];
+    const size_t n = (size_t)c->ns * c->nr * c->nt * c->ne * c->npa;
+    HIP_TRY(hipMemcpyAsync(fin[mi], c->t.f2, n * 8, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(fin_e[mi], c->t.energy, (size_t)c->ns * 32, hipMemcpyDeviceToDevice, st));
+    const size_t ng = (size_t)c->ns * c->nt * c->ne * c->npa;
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->t.lam, 0.0, n);
+    if (c->t.bbar) hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, st, c->t.bbar, 0.0, (size_t)nsteps * ng);
+  }
+  // backward, segment by segment
+  for (int g = nseg - 1; g >= 0; --g) {
+    const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
+    auto out_of = [&](int s) { return (s == n1 - 1) ? -1 : (split ? ST + (s + 1 - n0) + 1 : group(s + 1, n0) + T_R1 + 1); };
+    if (split) {  // the segment's states
+      enq_copy(b, CK + g + 1, -1);
+      for (int s = n0; s < n1; ++s) {
+        enq_copy(b, 0, ST + (s - n0) + 1);
+        enq_sched(b, s);
+        enqueue_step(b, false);
+      }
+    } else if (g < nseg - 1) {  // the segment taped
+      enq_copy(b, CK + g + 1, group(n0, n0) + T_R1 + 1);
+      for (int s = n0; s < n1; ++s) {
+        enq_sched(b, s);
+        enq_taped_step(b, group(s, n0), out_of(s));
+      }
+    }
+    for (int s = n1 - 1; s >= n0; --s) {
+      const int out = out_of(s);
+      if (split) {
+        enq_copy(b, ST + (s - n0) + 1, TG + T_R1 + 1);
+        enq_sched(b, s);
+        enq_taped_step(b, TG, out);  // rewrites the step's output with the same values
+      } else {
+        enq_sched(b, s);
+      }
+      enq_misfit(b, out < 0 ? 0 : out, s, 1);
+      enq_step_bwd(b, group(s, n0), s, bwd_lds, bwd_maxc);
+    }
+    if ((rc = launch_check("ensemble adjoint"))) {
+      detach();
+      return rc;
+    }
+  }
+  HIP_TRY(hipEventRecord(b->ev1, st));
+  enq_sched(b, nsteps - 1);  // the members keep the inputs of the last step
+  // results; the members are left in the state after the run
+  for (int mi = 0; mi < M; ++mi) {
+    ramgpu_ctx *c = b->m[mi];
+    const size_t n = (size_t)c->ns * c->nr * c->nt * c->ne * c->npa, ng = (size_t)c->ns * c->nt * c->ne * c->npa;
+    const size_t np = npts[mi];
+    HIP_TRY(hipMemcpyAsync(c->t.f2, fin[mi], n * 8, hipMemcpyDeviceToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->t.energy, fin_e[mi], (size_t)c->ns * 32, hipMemcpyDeviceToDevice, st));
+    std::vector<double> lv((size_t)nsteps * np);
+    HIP_TRY(hipMemcpyAsync(lv.data(), c->t.mloss, lv.size() * 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (loss) {
+      double tot = 0.0;
+      for (double v : lv) tot += v;
+      loss[mi] = tot;
+    }
+    if (grad_f2 && grad_f2[mi]) {
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      if ((rc = download_f2(c, grad_f2[mi], c->t.lam))) {
+        detach();
+        return rc;
+      }
+    }
+    if (c->t.bbar) {
+      HIP_TRY(hipMemcpy(grad_fgeos[mi], c->t.bbar, (size_t)nsteps * ng * 8, hipMemcpyDeviceToHost));
+    }
+  }
+  return detach();
 }
 
 // Autotuner: for each sweep kernel (R, P, plane) time one launch of every

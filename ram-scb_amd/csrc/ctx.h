@@ -15,11 +15,14 @@ namespace ramgpu {
 __global__ void k_sched(Launch L, const double *kp, const double *bscale, const int *bs_off, const int *step,
                         int nmembers, int stride_bs);
 __global__ void k_tick(int *step);
+__global__ void k_setstep(int *step, int v);
+__global__ void k_copy_ens(Launch L);
 template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
 __global__ void k_p_bwd(Launch L, const double *xin, double *adj);
 template <int SEQ, int MAXC> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
 __global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
                          const double *w, const double *target, double *loss_out, double *lam);
+__global__ void k_misfit_ens(Launch L, int step, int adjoint);
 __global__ void k_axpy(double *y, const double *x, double a, size_t n);
 __global__ void k_fill(double *y, double v, size_t n);
 __global__ void k_species(Launch L);
@@ -106,5 +109,16 @@ int step_once(ramgpu_ctx *c);
 int plane_maxc(int ne, int npa);
 // launch k_plane<seq, maxc>
 void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
+// tape slots: the inputs of the operators of one ram_run step, in order
+// (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
+enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
+// plane adjoint kernels
+int plane_bwd_maxc(int ne, int npa);
+void set_plane_bwd_lds_limit(size_t limit);
+void launch_plane_bwd(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L,
+                      const double *x, double *adj);
+// host Fortran-layout F2 <-> device-layout buffer, through the context's stage
+int upload_f2(ramgpu_ctx *c, double *dst, const double *host);
+int download_f2(ramgpu_ctx *c, double *host, const double *src);
 
 }  // namespace ramgpu_host

@@ -388,8 +388,8 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
       const double beta = t.betalim;
       const size_t di = (size_t)NT * P;  // stride between I and I+1
       const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
-      const double *fl = (L.src ? L.src : t.f2) + off;
-      double *fo = (L.dst ? L.dst : t.f2) + off;
+      const double *fl = launch_src(L, t) + off;
+      double *fo = launch_dst(L, t) + off;
       double *fs = fo + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
       const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
       double gh1 = 0.0, gh2 = 0.0;
@@ -493,12 +493,12 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
     const int P = NE * t.npa;
     const int line = u.b * kLineThreads + threadIdx.x;
     double cmax = 0.0;
-    if (line >= (NR - 1) * P && L.dst && L.dst != L.src) {
+    if (line >= (NR - 1) * P && out_of_place(L)) {
       // out of place: the I=1 rows are not swept; extra lines copy them
       const int x = line - (NR - 1) * P;
       if (x < NT * P) {
         const size_t off = (size_t)s * f2_species_stride(t) + x;
-        L.dst[off] = (L.src ? L.src : t.f2)[off];
+        launch_dst(L, t)[off] = launch_src(L, t)[off];
       }
     } else if (line < (NR - 1) * P) {
       const int i1 = fast_div(line, P, t.rp);
@@ -508,8 +508,8 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
       const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
       const double beta = t.betalim, omet = t.omet;
       const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;  // F(J) at off+(J-1)*P
-      const double *fl = (L.src ? L.src : t.f2) + off;
-      double *fo = (L.dst ? L.dst : t.f2) + off;
+      const double *fl = launch_src(L, t) + off;
+      double *fo = launch_dst(L, t) + off;
       const double4 *pij = t.pij + (size_t)i * NT;
       const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
       // Courant number of face J (src/ModRamDrift.f90:236-239)
@@ -791,16 +791,17 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(La
     const int NE = t.ne, NPA = t.npa, NT = t.nt;
     if (u.b >= blocks_kl(t)) {  // out of place: copy plane (I=1, J) of the unswept row
       const size_t off = (size_t)u.s * f2_species_stride(t) + (size_t)(u.b - blocks_kl(t)) * NE * NPA;
-      const double *src = (L.src ? L.src : t.f2) + off;
-      for (int n = threadIdx.x; n < NE * NPA; n += blockDim.x) L.dst[off + n] = src[n];
+      const double *src = launch_src(L, t) + off;
+      double *dst = launch_dst(L, t) + off;
+      for (int n = threadIdx.x; n < NE * NPA; n += blockDim.x) dst[n] = src[n];
       continue;
     }
     constexpr bool need_loss =
         SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS || SEQ == SEQ_TAPE;
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, L.src ? L.src : t.f2, need_loss, c, d);
-    double *gF = (L.dst ? L.dst : t.f2) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+    plane_setup(t, u, sm, launch_src(L, t), need_loss, c, d);
+    double *gF = launch_dst(L, t) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
     const bool inside = t.kij[c.ij * 8 + KIJ_MG] == 0.0;
     const bool rec_here = rec && inside;
     if constexpr (SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE) {  // per-L terms
@@ -858,7 +859,7 @@ __global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(La
       // F only after a barrier; the loss chain rewrites its cells at once, so
       // a barrier separates it from the store before it.
       const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * c.P, ns = f2_elems(t);
-      double *slot = const_cast<double *>(L.src) + po;
+      double *slot = const_cast<double *>(launch_src(L, t)) + po;
       plane_e<MAXC>(t, c, d, rec_here);
       for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[ns + n] = d.F[n];
       plane_mu<MAXC>(t, c, d, rec_here);
@@ -1021,6 +1022,25 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
 
 __global__ void k_tick(int *step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = step[0] + 1;
+}
+
+__global__ void k_setstep(int *step, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) step[0] = v;
+}
+
+// Whole-F2 copy of every problem of an ensemble launch, source to
+// destination as the launch names them (tape slots or F2); each unit is one
+// problem, its nb blocks striding over it.
+__global__ void k_copy_ens(Launch L) {
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+    const Unit u = unit_of(L, vb);
+    const Tab &t = *u.t;
+    const int nb = L.vmap ? L.jobs[L.vmap[vb]].nb : L.nb;
+    const size_t n = (size_t)t.ns * f2_species_stride(t);
+    const double *src = launch_src(L, t);
+    double *dst = launch_dst(L, t);
+    for (size_t m = (size_t)u.b * blockDim.x + threadIdx.x; m < n; m += (size_t)nb * blockDim.x) dst[m] = src[m];
+  }
 }
 
 // Plain streaming copy (16-byte accesses), the bandwidth and counter

@@ -89,6 +89,15 @@ struct Tab {
   double *partials;  // [slot][s][maxblk]
   int maxblk;
   double *energy;    // [s][4]: SETRC, LSDR, LSCHA, LSATM
+  // reverse mode of an ensemble (null otherwise): the problem's tape of
+  // F2-sized slots, its adjoint lam, the adjoint of each step's FGEOS
+  // ([step][NS,NT,NE,NPA]), and its flux-misfit points, weights
+  // ([p][NPA]), targets and per-step losses ([step][p])
+  double *tape, *lam, *bbar;
+  const int *mpts;
+  const double *mw, *mtarget;
+  double *mloss;
+  int mnpts;
 };
 
 // One unit of a launch: blocks [b0, b0+nb) of species s of problem tabs[tab].
@@ -102,7 +111,9 @@ struct Job {
 // gridDim.x, so the grid may be smaller than nv.
 // src/dst (sweep kernels): read F2 from src and write to dst (device layout
 // of the whole problem) instead of updating tabs[..].f2 in place; null for in
-// place.
+// place.  ssl/dsl name them per problem instead (ensembles): n > 0 is slot
+// n-1 of the problem's tape, dsl < 0 its own F2 written out of place, 0 not
+// used.
 struct Launch {
   const Tab *tabs;
   const Job *jobs;
@@ -110,11 +121,25 @@ struct Launch {
   int nv, nb, s0;
   const double *src;
   double *dst;
+  int ssl, dsl;
+  int bstep;  // reverse mode of an ensemble: the step whose FGEOS adjoint (Tab::bbar) DRIFTR's adjoint feeds
 };
 
 inline __host__ __device__ size_t f2_species_stride(const Tab &t) {
   return (size_t)t.nr * t.nt * t.npa * t.ne;
 }
+
+// the input and output F2 of a launch for problem t, and whether they differ
+inline __host__ __device__ const double *launch_src(const Launch &L, const Tab &t) {
+  if (L.ssl > 0) return t.tape + (size_t)(L.ssl - 1) * t.ns * f2_species_stride(t);
+  return L.src ? L.src : t.f2;
+}
+inline __host__ __device__ double *launch_dst(const Launch &L, const Tab &t) {
+  if (L.dsl > 0) return t.tape + (size_t)(L.dsl - 1) * t.ns * f2_species_stride(t);
+  if (L.dsl < 0) return t.f2;
+  return L.dst ? L.dst : t.f2;
+}
+inline __host__ __device__ bool out_of_place(const Launch &L) { return L.dsl != 0 || (L.dst && L.dst != L.src); }
 
 // block counts of the kernels, per (problem, species)
 inline __host__ __device__ int blocks_r(const Tab &t) {

@@ -33,6 +33,7 @@ EXPORTS = (
     "ramgpu_batch_autotune", "ramgpu_batch_get_shapes", "ramgpu_batch_set_shapes",
     "ramgpu_driftr_bwd", "ramgpu_driftp_bwd", "ramgpu_drifte_bwd", "ramgpu_driftmu_bwd",
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
+    "ramgpu_batch_adjoint_run",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd")
@@ -107,6 +108,8 @@ def load_library(path: Path | None = None):
     lib.ramgpu_ram_run_bwd.argtypes = [v, _dp, _dp]
     lib.ramgpu_adjoint_run.argtypes = [v, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp,
                                        _dp, _dp]
+    _pp = C.POINTER(C.c_void_p)
+    lib.ramgpu_batch_adjoint_run.argtypes = [v, C.c_int, C.c_int, _dp, _dp, _ip, _pp, _pp, _pp, _dp, _pp, _pp]
     for n in EXPORTS:
         if n != "ramgpu_last_error":
             getattr(lib, n).restype = C.c_int
@@ -445,3 +448,50 @@ class RamBatch:
     def set_shapes(self, r=0, p=0, plane=0):
         caps = (C.c_int * 3)(r, p, plane)
         self._check(self.lib.ramgpu_batch_set_shapes(self.h, caps))
+
+    def adjoint_run(self, nsteps, pts, w, target, kp, bscale=None, ckpt_every=10, grad_fgeos=True):
+        """Misfit losses and gradients of every member (ramgpu_batch_adjoint_run).
+        pts[m]: (npts_m,4) 1-based S,I,J,K; w[m]: (npts_m,NPA_m); target[m]:
+        (nsteps,npts_m); kp: (nsteps, M); bscale: (nsteps, sum NS) or None.
+        Returns (loss[M], [dL/dF2_0 per member], [dL/dFGEOS (nsteps,NS,NT,NE,NPA) per member or None])."""
+        M = len(self.members)
+        kp = np.ascontiguousarray(kp, dtype=np.float64)
+        if kp.shape != (nsteps, M):
+            raise ValueError("kp must be (nsteps, members)")
+        bs = None
+        if bscale is not None:
+            bs = np.ascontiguousarray(bscale, dtype=np.float64)
+            if bs.shape != (nsteps, self.sum_ns):
+                raise ValueError("bscale must be (nsteps, sum of member species)")
+        keep, npts = [], (C.c_int * M)()
+        pp, wp, tp, gfp, gbp = ((C.c_void_p * M)() for _ in range(5))
+        gfs, gbs = [], []
+        for i, m in enumerate(self.members):
+            pa = np.ascontiguousarray(pts[i], dtype=np.int32)
+            wa = np.ascontiguousarray(w[i], dtype=np.float64)
+            ta = np.ascontiguousarray(target[i], dtype=np.float64)
+            if pa.ndim != 2 or pa.shape[1] != 4 or wa.shape != (pa.shape[0], m.npa) or \
+                    ta.shape != (nsteps, pa.shape[0]):
+                raise ValueError(f"member {i}: pts (n,4), w (n,NPA), target (nsteps,n) expected")
+            gf = np.zeros(m.dims, order="F")
+            gb = np.zeros(nsteps * m.ns * m.nt * m.ne * m.npa) if grad_fgeos else None
+            keep += [pa, wa, ta]
+            npts[i] = pa.shape[0]
+            pp[i], wp[i], tp[i] = pa.ctypes.data, wa.ctypes.data, ta.ctypes.data
+            gfp[i] = gf.ctypes.data
+            gbp[i] = gb.ctypes.data if gb is not None else None
+            gfs.append(gf)
+            gbs.append(gb)
+        loss = np.zeros(M)
+        self._check(self.lib.ramgpu_batch_adjoint_run(
+            self.h, nsteps, ckpt_every, _p(kp), None if bs is None else _p(bs), npts, pp, wp, tp, _p(loss),
+            gfp, gbp if grad_fgeos else None))
+        out_gb = []
+        for m, gb in zip(self.members, gbs):
+            if gb is None:
+                out_gb.append(None)
+                continue
+            per = m.ns * m.nt * m.ne * m.npa
+            out_gb.append(np.stack([gb[n * per:(n + 1) * per].reshape((m.ns, m.nt, m.ne, m.npa), order="F")
+                                    for n in range(nsteps)]))
+        return loss, gfs, out_gb
