@@ -25,7 +25,10 @@ Two more objects in the same line, each on a bounded sample:
     one after another (one ram_run call each);
   "adjoint": H+ on 20x25x35x72, 60 steps, virtual-satellite misfit at 8
     points, gradient w.r.t. the initial F2 and every step's FGEOS with
-    checkpoints every 10 steps, against the 60-step forward alone.
+    checkpoints every 10 steps, against the 60-step forward alone;
+  "batched_adjoint": the same misfit gradient for an ensemble of 4-species
+    40x96x64x90 members (Kp ~ U(1,8) per member), 120 steps with checkpoints
+    every 20, in one launch per kernel, against the ensemble's forward.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M]
 """
@@ -205,6 +208,55 @@ def bench_adjoint(seed: int, nsteps: int = 60, ckpt: int = 10):
             "grad_f2_norm": float(np.linalg.norm(gf)), "grad_fgeos_norm": float(np.linalg.norm(gb))}
 
 
+def bench_batched_adjoint(members: int, nsteps: int, ckpt: int, seed: int):
+    """Ensemble adjoint at high resolution: members of 4 species on
+    40x96x64x90 with per-member Kp ~ U(1,8), 8 satellite points each,
+    gradient w.r.t. every member's initial F2 and per-step FGEOS with
+    checkpoints every `ckpt` steps, against the same ensemble's forward."""
+    from ramgpu import inputs
+    from ramgpu.capi import RamBatch, RamGpu
+
+    shape = (40, 96, 64, 90)
+    log(f"bench: ensemble adjoint, {members} members of 4 species on {'x'.join(map(str, shape))}, "
+        f"{nsteps} steps, checkpoint every {ckpt}")
+    proto = inputs.make_problem(*shape, inputs.SPECIES4, inputs.COMPOSITION4, kp=3.0, seed=seed + 31)
+    rng = np.random.default_rng(seed + 32)
+    kp = rng.uniform(1.0, 8.0, (nsteps, members))
+    mem = []
+    for m in range(members):
+        g = RamGpu.from_problem(proto)
+        g.set_kp(kp[0, m])
+        mem.append(g)
+    pts, w, tgt = [], [], []
+    for m in range(members):
+        pt, wt = inputs.satellite_points(proto, 8, seed=seed + 40 + m, species=2)
+        om = np.log(inputs.satellite_flux(proto.f2, pt, wt))
+        pts.append(pt)
+        w.append(wt)
+        tgt.append(om[None, :] + 0.2 * rng.standard_normal((nsteps, len(pt))))
+    cells = members * proto.grid.ns * int(np.prod(shape))
+    b = RamBatch(mem)
+    b.ram_run_schedule(kp[:1])                 # warm
+    for g in mem:
+        g.put_f2(proto.f2)
+    b.ram_run_schedule(kp)
+    fwd_ms = b.last_run_ms()
+    for g in mem:
+        g.put_f2(proto.f2)
+    loss, gf, gb = b.adjoint_run(nsteps, pts, w, tgt, kp, ckpt_every=ckpt)
+    adj_ms = b.last_run_ms()
+    b.close()
+    for g in mem:
+        g.close()
+    log(f"bench: ensemble adjoint {adj_ms:.1f} ms (forward {fwd_ms:.1f} ms, ratio {adj_ms / fwd_ms:.2f})")
+    return {"config": f"{members} members x 4 species on {'x'.join(map(str, shape))}, {nsteps} steps, "
+                      f"checkpoint every {ckpt}, 8 satellite points per member, Kp ~ U(1,8)",
+            "members": members, "cells": cells, "forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "ratio": adj_ms / fwd_ms, "forward_cell_updates_per_s": cells * nsteps / (fwd_ms * 1e-3),
+            "loss": [float(x) for x in loss],
+            "grad_f2_norm": [float(np.linalg.norm(x)) for x in gf]}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +271,9 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--ens-adjoint-members", type=int, default=2,
+                    help="members of the high-resolution ensemble adjoint (0: skip)")
+    ap.add_argument("--ens-adjoint-steps", type=int, default=120)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -303,6 +358,9 @@ def main():
         out["batched"] = bench_batched(args.members, args.batched_steps, args.seed)
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args.seed)
+        if args.ens_adjoint_members > 0:
+            out["batched_adjoint"] = bench_batched_adjoint(args.ens_adjoint_members, args.ens_adjoint_steps, 20,
+                                                           args.seed)
     if not args.no_cpu_baseline:
         log("bench: timing the CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(problem)
