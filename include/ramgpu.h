@@ -68,6 +68,15 @@ int ramgpu_set_species(ramgpu_ctx *ctx, int s, int charge, int cex_kind, double 
 
 /* ModRamTiming::DTs, ModRamParams::BetaLim, ModRamVariables::FracCFL */
 int ramgpu_set_params(ramgpu_ctx *ctx, double dts, double betalim, double fraccfl);
+/* Launch shape of the energy / pitch-angle plane kernels (no counterpart in
+ * the reference; a tuning control): blocks of `threads` (a multiple of 64,
+ * at most 1024) in which each thread owns one K of ceil(NPA/floor(threads/NE))
+ * rows, at most `maxc` (8 or 16; 16 with at most 512 threads).  0, 0
+ * restores the shape the library picks from NE, NPA and the device's LDS.
+ * The SUMRC sums add the plane's cells in an order that follows the shape, so
+ * ensembles keep each member's shape. */
+int ramgpu_set_plane_shape(ramgpu_ctx *ctx, int threads, int maxc);
+int ramgpu_get_plane_shape(ramgpu_ctx *ctx, int *threads, int *maxc);
 
 /* Grid of ARRAYS, src/ModRamInit.f90:364-587 */
 int ramgpu_set_grid(ramgpu_ctx *ctx, double mdr, double dphi, double conf1, double conf2,

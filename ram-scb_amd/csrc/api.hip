@@ -205,14 +205,40 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src, double *ds
   return launch_check("k_p");
 }
 
-int plane_maxc(int ne, int npa) {
-  const int e = (npa + kPlaneThreads / ne - 1) / (kPlaneThreads / ne);
-  return (e <= 8) ? 8 : 16;
+// Shape of the forward plane kernels for an NE x NPA plane: a block of T
+// threads gives each thread one K of R = ceil(NPA / floor(T/NE)) rows, and
+// the row bound MAXC (8 or 16) sets the register budget (4 or 2 waves per
+// SIMD; MAXC 16 with blocks of at most 512 threads).  The phases of a plane
+// are chains of dependent LDS reads, limiter divisions and barriers, so a
+// block's time follows its R rows divided among the blocks resident on a
+// CU (limited by the waves of the register budget and by the LDS, lds_cu per
+// CU); blocks of 4, 8, 12 or 16 waves, which spread evenly over the four
+// SIMDs, measured fastest on MI355X (35x72: 512 threads; 48x90: 768; 64x90:
+// 1024).  Ties go to the smaller block.  A function of (NE, NPA, LDS) only,
+// so a problem keeps its mapping in any launch.
+PlaneShape plane_shape(int ne, int npa, size_t lds, size_t lds_cu) {
+  PlaneShape best{0, 0};
+  double bcost = 1e300;
+  for (int T = 256; T <= kPlaneMaxThreads; T += 256) {
+    const int lestep = T / ne;
+    if (lestep < 1) continue;
+    const int rows = (npa + lestep - 1) / lestep;
+    const int maxc = rows <= 8 ? 8 : 16;
+    if (rows > kMaxCellsPerThread || (maxc == 16 && T > kPlaneThreads)) continue;
+    const int res = std::min(4 * (maxc == 8 ? 4 : 2) / (T / 64), (int)(lds_cu / lds));
+    if (res < 1) continue;
+    const double cost = (double)rows / res;
+    if (cost < bcost - 1e-12) {
+      bcost = cost;
+      best = {T, maxc};
+    }
+  }
+  return best;
 }
 
 template <int MAXC>
-void launch_plane_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
-  const dim3 blk(kPlaneThreads);
+void launch_plane_t(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
+  const dim3 blk(threads);
   switch (seq) {
     case SEQ_E: hipLaunchKernelGGL((k_plane<SEQ_E, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_MU: hipLaunchKernelGGL((k_plane<SEQ_MU, MAXC>), grid, blk, lds, st, L, rec); break;
@@ -225,11 +251,11 @@ void launch_plane_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch
   }
 }
 
-void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
-  if (maxc <= 8)
-    launch_plane_t<8>(seq, grid, lds, st, L, rec);
+void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st, const Launch &L, int rec) {
+  if (sh.maxc <= 8)
+    launch_plane_t<8>(seq, sh.threads, grid, lds, st, L, rec);
   else
-    launch_plane_t<16>(seq, grid, lds, st, L, rec);
+    launch_plane_t<16>(seq, sh.threads, grid, lds, st, L, rec);
 }
 
 int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *src, double *dst) {
@@ -240,7 +266,7 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
   L.dst = dst;
   const size_t lds = plane_lds_bytes(c->ne, c->npa);
   tbeg(c, RAMGPU_KERNEL_PLANE);
-  launch_plane(seq, plane_maxc(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, rec);
+  launch_plane(seq, c->plane, dim3(L.nv), lds, c->stream, L, rec);
   tend(c);
   return launch_check("k_plane");
 }
@@ -393,6 +419,10 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   if (plane_lds_bytes(ne, npa) > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS (%zu bytes)", ne * npa,
                         c->lds_limit));
+  // gfx950: the LDS of a CU is the largest a block may use
+  c->plane = plane_shape(ne, npa, plane_lds_bytes(ne, npa), c->lds_limit);
+  if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));
+  t.plane_threads = c->plane.threads;
 #define A(p, n) \
   if ((rc = dev_alloc(c, &p, n))) return cleanup(rc)
   A(c->d_rlz, nr1); A(c->d_ekev, ne); A(c->d_ebnd, ne); A(c->d_we, ne); A(c->d_de, ne);
@@ -476,6 +506,34 @@ int ramgpu_set_params(ramgpu_ctx *c, double dts, double betalim, double fraccfl)
   c->t.fraccfl = fraccfl;
   c->tables_dirty = true;
   c->tab_dirty = true;
+  return RAMGPU_OK;
+}
+
+int ramgpu_set_plane_shape(ramgpu_ctx *c, int threads, int maxc) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (threads == 0 && maxc == 0) {  // back to the model's choice
+    c->plane = plane_shape(c->ne, c->npa, plane_lds_bytes(c->ne, c->npa), c->lds_limit);
+  } else {
+    if (threads < 64 || threads % 64 || threads > kPlaneMaxThreads || (maxc != 8 && maxc != 16) ||
+        (maxc == 16 && threads > kPlaneThreads))
+      return fail(RAMGPU_EINVAL, "plane shape %d threads, %d rows: not a built kernel shape", threads, maxc);
+    const int lestep = threads / c->ne;
+    if (lestep < 1 || (c->npa + lestep - 1) / lestep > maxc)
+      return fail(RAMGPU_EINVAL, "plane shape %d threads holds %d rows per thread, above %d", threads,
+                  lestep < 1 ? 0 : (c->npa + lestep - 1) / lestep, maxc);
+    c->plane = {threads, maxc};
+  }
+  c->t.plane_threads = c->plane.threads;
+  c->tab_dirty = true;
+  return RAMGPU_OK;
+}
+
+int ramgpu_get_plane_shape(ramgpu_ctx *c, int *threads, int *maxc) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (threads) *threads = c->plane.threads;
+  if (maxc) *maxc = c->plane.maxc;
   return RAMGPU_OK;
 }
 

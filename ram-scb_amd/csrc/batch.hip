@@ -48,7 +48,7 @@ struct ramgpu_batch {
   Tab *d_tabs = nullptr;
   Plan plan[BK_N];
   size_t plane_lds = 0;
-  int plane_maxc = 8;  // the largest any member needs
+  PlaneShape plane{64, 8};  // the largest block and row bound any member needs
   int sum_ns = 0;
   std::vector<int> bs_off;  // first species of member m in a schedule row
   int *d_bsoff = nullptr;
@@ -169,7 +169,7 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
                lf = launch_for(b, BK_FIN);
   hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, lr);
   hipLaunchKernelGGL(k_p<1>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, lp);
-  launch_plane(SEQ_STEP, b->plane_maxc, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
+  launch_plane(SEQ_STEP, b->plane, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
   hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, lp);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, lr);
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(256), 0, st, lf, 15, 2);
@@ -225,7 +225,7 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
   L = slot_launch(b, BK_POOP, g + T_P1 + 1, g + T_E1 + 1);
   hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), 0, st, L);
   L = slot_launch(b, BK_PLOOP, g + T_E1 + 1, g + T_P2 + 1);
-  launch_plane(SEQ_TAPE, b->plane_maxc, dim3(L.nv), b->plane_lds, st, L, 1);
+  launch_plane(SEQ_TAPE, b->plane, dim3(L.nv), b->plane_lds, st, L, 1);
   L = slot_launch(b, BK_POOP, g + T_P2 + 1, g + T_R2 + 1);
   hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), 0, st, L);
   L = slot_launch(b, BK_R, g + T_R2 + 1, dsl_out);
@@ -369,7 +369,8 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->bs_off.push_back(b->sum_ns);
     b->sum_ns += c->ns;
     b->plane_lds = std::max(b->plane_lds, plane_lds_bytes(c->ne, c->npa));
-    b->plane_maxc = std::max(b->plane_maxc, plane_maxc(c->ne, c->npa));
+    b->plane.threads = std::max(b->plane.threads, c->plane.threads);
+    b->plane.maxc = std::max(b->plane.maxc, c->plane.maxc);
   }
   if ((rc = batch_alloc(b, (void **)&b->d_bsoff, nmembers * sizeof(int))) ||
       (rc = batch_alloc(b, (void **)&b->d_step, sizeof(int))))
@@ -696,7 +697,7 @@ int ramgpu_batch_autotune(ramgpu_batch *b, int reps, double *ms_best) {
         else if (kind == BK_P)
           hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, kind)), dim3(kLineThreads), 0, b->stream, L);
         else
-          launch_plane(SEQ_STEP, b->plane_maxc, dim3(grid_for(b, kind)), b->plane_lds, b->stream, L, 0);
+          launch_plane(SEQ_STEP, b->plane, dim3(grid_for(b, kind)), b->plane_lds, b->stream, L, 0);
         (void)hipEventRecord(e1, b->stream);
         if (hipEventSynchronize(e1) != hipSuccess) {
           rc = fail(RAMGPU_EHIP, "autotune launch failed");

@@ -44,6 +44,10 @@ __global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt,
 using ramgpu::Tab;
 using ramgpu::kMaxSpecies;
 
+struct PlaneShape {
+  int threads, maxc;  // block size and row bound (template) of the forward plane kernels
+};
+
 struct ramgpu_ctx {
   int device = 0;
   int ns = 0, nr = 0, nt = 0, ne = 0, npa = 0;
@@ -51,6 +55,7 @@ struct ramgpu_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   size_t lds_limit = 65536;
+  PlaneShape plane{ramgpu::kPlaneThreads, 16};
   Tab t{};            // host image of the problem descriptor
   Tab *d_tab = nullptr;  // its device copy, read by every kernel
   bool tab_dirty = true;
@@ -106,9 +111,9 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
 int step_once(ramgpu_ctx *c);
 // cells per thread the plane kernels need for an NE x NPA plane (8 or 16)
-int plane_maxc(int ne, int npa);
+PlaneShape plane_shape(int ne, int npa, size_t lds, size_t lds_cu);
 // launch k_plane<seq, maxc>
-void launch_plane(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
+void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
 // tape slots: the inputs of the operators of one ram_run step, in order
 // (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
 enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };

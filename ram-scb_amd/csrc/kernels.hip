@@ -750,7 +750,7 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
   const double dts = t.dts;
   double a[5] = {0, 0, 0, 0, 0}, tot[5];
-#pragma unroll
+#pragma unroll 1  // no per-cell registers: a rolled loop keeps the exp temporaries of one cell live
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
     const bool w_on = sum_row && k >= 1 && l >= 1;
@@ -782,8 +782,12 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   }
 }
 
+// MAXC bounds the rows a thread owns; the register budget of each bound
+// (4 or 2 waves per SIMD; blocks up to 1024 or 512 threads) fits the block
+// sizes the host pairs it with.
 template <int SEQ, int MAXC>
-__global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane(Launch L, int rec) {
+__global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads, (MAXC <= 8) ? 4 : 2)
+    k_plane(Launch L, int rec) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);

@@ -23,7 +23,8 @@ namespace ramgpu {
 
 constexpr int kMaxSpecies = 8;
 constexpr int kLineThreads = 256;      // R/P sweep kernels: one line per thread
-constexpr int kPlaneThreads = 512;     // plane kernels: one (K,L) plane per block
+constexpr int kPlaneThreads = 512;     // plane adjoint kernels: one (K,L) plane per block
+constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chosen per problem
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
 
@@ -84,6 +85,10 @@ struct Tab {
   double *dtdrift;      // [s][4]
   // float reciprocals for exact small-integer division (fast_div)
   float rnt, rne, rp;   // 1/NT, 1/NE, 1/(NE*NPA)
+  // threads of the forward plane kernels' row mapping (a multiple of 64):
+  // thread n < (T/NE)*NE owns K = n%NE of rows L = n/NE + m*(T/NE); a launch
+  // with a larger block leaves the threads above idle
+  int plane_threads;
   // state
   double *f2;
   double *partials;  // [slot][s][maxblk]
@@ -153,7 +158,7 @@ inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.n
 // LDS bytes of the plane kernel
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
-  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 64 +
+  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 +
           2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
 }
 

@@ -274,8 +274,8 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.cl = d.cm + 8 * npa;     // 2*NPA: HDNS, 1/FNHS
   d.kt = d.cl + 2 * npa;     // KT_N*NE
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
-  d.red = d.lt + LT_N * npa; // 64
-  d.gh = d.red + 64;         // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
+  d.red = d.lt + LT_N * npa; // 128: block reductions (up to 16 waves x 5 sums)
+  d.gh = d.red + 128;        // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
   d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE+NPA
   return d;
 }
@@ -286,7 +286,7 @@ struct PlaneCtx {
   double rlz, rrlz, qs, beta;
   float rne;
   // row-wise ownership of the forward plane kernels: this thread owns K = ke
-  // of rows L = le0, le0+lestep, ..
+  // of rows L = le0, le0+lestep, .. with lestep = blockDim/NE
   int ke, le0, lestep;
   bool eact;
 };
@@ -316,7 +316,7 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.beta = t.betalim;
   c.rne = t.rne;
   c.nmine = (c.P - (int)threadIdx.x + kPlaneThreads - 1) / kPlaneThreads;  // cells tid, tid+512, .. < P
-  c.lestep = kPlaneThreads / NE;
+  c.lestep = t.plane_threads / NE;  // the problem's block size sets the rows per thread
   c.le0 = fast_div((int)threadIdx.x, NE, c.rne);
   c.ke = (int)threadIdx.x - c.le0 * NE;
   c.eact = (int)threadIdx.x < c.lestep * NE;

@@ -33,7 +33,7 @@ EXPORTS = (
     "ramgpu_batch_autotune", "ramgpu_batch_get_shapes", "ramgpu_batch_set_shapes",
     "ramgpu_driftr_bwd", "ramgpu_driftp_bwd", "ramgpu_drifte_bwd", "ramgpu_driftmu_bwd",
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
-    "ramgpu_batch_adjoint_run",
+    "ramgpu_batch_adjoint_run", "ramgpu_set_plane_shape", "ramgpu_get_plane_shape",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd")
@@ -108,6 +108,8 @@ def load_library(path: Path | None = None):
     lib.ramgpu_ram_run_bwd.argtypes = [v, _dp, _dp]
     lib.ramgpu_adjoint_run.argtypes = [v, C.c_int, C.c_int, _dp, _dp, C.c_int, _ip, _dp, _dp, _dp,
                                        _dp, _dp]
+    lib.ramgpu_set_plane_shape.argtypes = [v, C.c_int, C.c_int]
+    lib.ramgpu_get_plane_shape.argtypes = [v, _ip, _ip]
     _pp = C.POINTER(C.c_void_p)
     lib.ramgpu_batch_adjoint_run.argtypes = [v, C.c_int, C.c_int, _dp, _dp, _ip, _pp, _pp, _pp, _dp, _pp, _pp]
     for n in EXPORTS:
@@ -354,6 +356,15 @@ class RamGpu:
         for n in range(nsteps):
             gb[n] = gbf[n * per:(n + 1) * per].reshape((self.ns, self.nt, self.ne, self.npa), order="F")
         return loss.value, gf, gb
+
+    def set_plane_shape(self, threads=0, maxc=0):
+        """Block size and row bound of the plane kernels (0, 0: the library's choice)."""
+        self._check(self.lib.ramgpu_set_plane_shape(self.h, threads, maxc))
+
+    def plane_shape(self):
+        t, m = C.c_int(), C.c_int()
+        self._check(self.lib.ramgpu_get_plane_shape(self.h, C.byref(t), C.byref(m)))
+        return t.value, m.value
 
     def kernel_timing(self, enable: bool = True):
         self._check(self.lib.ramgpu_kernel_timing(self.h, int(bool(enable))))
