@@ -18,6 +18,24 @@
 
 using namespace ramgpu;
 
+namespace {
+// 2^(j/64), j = 0..63, correctly rounded (the table of exp_tab)
+const double kExp2Tab[64] = {
+    1.0, 1.0108892860517005, 1.0218971486541166, 1.0330248790212284, 1.0442737824274138, 1.0556451783605572,
+    1.0671404006768237, 1.0787607977571199, 1.0905077326652577, 1.102382583307841, 1.1143867425958924,
+    1.1265216186082418, 1.1387886347566916, 1.1511892299529827, 1.1637248587775775, 1.1763969916502812,
+    1.189207115002721, 1.202156731452703, 1.215247359980469, 1.22848053610687, 1.241857812073484,
+    1.255380757024691, 1.2690509571917332, 1.2828700160787783, 1.2968395546510096, 1.3109612115247644,
+    1.3252366431597413, 1.339667524053303, 1.3542555469368927, 1.3690024229745905, 1.383909881963832,
+    1.3989796725383112, 1.4142135623730951, 1.42961333839197, 1.4451808069770467, 1.460917794180647,
+    1.4768261459394993, 1.4929077282912648, 1.5091644275934228, 1.5255981507445384, 1.5422108254079407,
+    1.559004400237837, 1.5759808451078865, 1.593142151342267, 1.6104903319492543, 1.6280274218573478,
+    1.645755478153965, 1.6636765803267364, 1.681792830507429, 1.7001063537185235, 1.718619298122478,
+    1.7373338352737062, 1.7562521603732995, 1.7753764925265212, 1.7947090750031072, 1.8142521755003989,
+    1.8340080864093424, 1.8539791250833855, 1.8741676341103, 1.8945759815869656, 1.9152065613971474,
+    1.9360617934922943, 1.9571441241754002, 1.978456026387951};
+}  // namespace
+
 namespace ramgpu_host {
 
 thread_local std::string g_err;
@@ -441,6 +459,7 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   A(t.atlos, (size_t)ns * nr * ne); A(t.rel, (size_t)ns * 8);
   A(t.carry, (size_t)ns * ne * npa);
   A(t.latl, (size_t)ns * nr * ne);
+  A(t.ex64, 64);
   A(c->d_tab, 1);
   A(t.cflmax, (size_t)ns * (2 + ne + npa));
   A(t.dtdrift, (size_t)ns * 4);
@@ -448,6 +467,8 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   A(c->d_stage, nf2);
   A(t.partials, (size_t)(kSumSlots + 1) * ns * t.maxblk);  // + ELORC slot of ramgpu_sumrc
   A(t.energy, (size_t)ns * 4);
+  if (hipMemcpy(t.ex64, kExp2Tab, sizeof(kExp2Tab), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(RAMGPU_EHIP, "hipMemcpy failed"));
 #undef A
   t.rlz = c->d_rlz; t.ekev = c->d_ekev; t.ebnd = c->d_ebnd; t.we = c->d_we; t.de = c->d_de;
   t.grel = c->d_grel; t.grbnd = c->d_grbnd; t.v = c->d_v; t.mu = c->d_mu; t.wmu = c->d_wmu;

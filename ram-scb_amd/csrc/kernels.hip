@@ -749,8 +749,9 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   const double sigv = d.kt[KT_SIGV * NE + k], latl = d.kt[KT_LATL * NE + k];
   const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
   const double dts = t.dts;
+  const double ekw = ek * wk;
   double a[5] = {0, 0, 0, 0, 0}, tot[5];
-#pragma unroll 1  // no per-cell registers: a rolled loop keeps the exp temporaries of one cell live
+#pragma unroll
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
     const bool w_on = sum_row && k >= 1 && l >= 1;
@@ -758,19 +759,21 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
     const bool atm = k >= 1 && l + 1 >= upa;
     double f = d.F[cc];
     // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49);
-    // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS))
-    const double cxf = cx ? exp(-(sigv * d.cl[2 * l] * dts)) : 1.0;
-    const double af = atm ? exp(latl * d.cl[2 * l + 1]) : 1.0;
-    const double wl = d.lt[LT_WMU * NPA + l];
-    if (w_on) a[0] += ek * (f * wk * wl);
-    if (cx) f = f * cxf;
-    if (w_on) a[1] += ek * (f * wk * wl);
-    if (atm) f = f * af;
-    if (w_on) a[2] += ek * (f * wk * wl);
-    if (atm) f = f * af;
-    if (w_on) a[3] += ek * (f * wk * wl);
-    if (cx) f = f * cxf;
-    if (w_on) a[4] += ek * (f * wk * wl);
+    // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS)); a factor of 1
+    // leaves a cell outside the operator's range unchanged, and a weight of 0
+    // leaves it out of the sums
+    const double cxf = cx ? exp_tab(-(sigv * d.cl[2 * l] * dts), d.ex) : 1.0;
+    const double af = atm ? exp_tab(latl * d.cl[2 * l + 1], d.ex) : 1.0;
+    const double q = w_on ? ekw * d.lt[LT_WMU * NPA + l] : 0.0;  // EKEV*WE*WMU
+    a[0] = fma(f, q, a[0]);
+    f = f * cxf;
+    a[1] = fma(f, q, a[1]);
+    f = f * af;
+    a[2] = fma(f, q, a[2]);
+    f = f * af;
+    a[3] = fma(f, q, a[3]);
+    f = f * cxf;
+    a[4] = fma(f, q, a[4]);
     d.F[cc] = f;
   }
   block_sum_n<5>(a, d.red, tot);

@@ -78,6 +78,7 @@ struct Tab {
   double *rel;    // [s][8]: GREL1, GREL2, sq1, GRZERO, sq0
   int *carry;     // [s][NE*NPA] DRIFTR ghost carry
   double *latl;   // [s][i][k] log(ATLOS): ATLOS**(1/FNHS) = exp(log(ATLOS)/FNHS)
+  double *ex64;   // [j] 2^(j/64), the table of the loss factors' exponential
   // CFL maxima of max(|C|,floor) over cells inside the magnetopause, as the
   // bit patterns of non-negative doubles (atomicMax order = double order):
   // [s][0] R, [s][1] P, [s][2+K] E per K, [s][2+NE+L] Mu per L
@@ -158,7 +159,7 @@ inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.n
 // LDS bytes of the plane kernel
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
-  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 +
+  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 +
           2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
 }
 

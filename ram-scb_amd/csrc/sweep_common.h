@@ -39,6 +39,24 @@ __device__ __forceinline__ double div_rn(double a, double d, double rd) {
   return fma(r, rd, q);
 }
 
+// exp(x) by Tang's table-driven method (ACM TOMS 15(2):144-157, 1989):
+// x = (64m + j) ln2/64 + r with |r| <= ln2/128, exp(x) = 2^m 2^(j/64) (1 + p(r)),
+// p the degree-5 Taylor polynomial of e^r - 1 (truncation below 4e-17), tab[j]
+// = 2^(j/64) correctly rounded.  Within about 1 ulp of exp; it serves the loss
+// factors, whose bar is 1e-12 relative.  Any NaN or out-of-range argument
+// still indexes tab within 0..63.
+__host__ __device__ inline double exp_tab(double x, const double *tab) {
+  const double inv = 92.33248261689366;                           // 64/ln2
+  const double l1 = 0.010830424696249145, l2 = 3.623510646634843e-19;  // ln2/64 = l1 + l2
+  double n = rint(x * inv);
+  n = fmin(fmax(n, -131072.0), 131072.0);
+  const double r = fma(-n, l2, fma(-n, l1, x));
+  const int ni = (int)n;
+  const int j = ni & 63, m = (ni - j) / 64;
+  const double p = r * (1.0 + r * (0.5 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0)))));
+  return ldexp(fma(tab[j], p, tab[j]), m);
+}
+
 // Limited face value of the reference's upwind scheme (src/ModRamDrift.f90:
 // 169-183) for the face between F(I) = f0 and F(I+1) = fp1.  fn and fnm1 are
 // F(N) and F(N-1) with N = I+1-sgn as the reference forms it (including its
@@ -258,7 +276,7 @@ __device__ __forceinline__ size_t f2_elems(const Tab &t) { return (size_t)t.ns *
 
 // LDS layout of the plane kernel (doubles)
 struct PlaneLds {
-  double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *gh;
+  double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *ex, *gh;
   unsigned long long *cmx;
 };
 enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
@@ -275,7 +293,8 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.kt = d.cl + 2 * npa;     // KT_N*NE
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
   d.red = d.lt + LT_N * npa; // 128: block reductions (up to 16 waves x 5 sums)
-  d.gh = d.red + 128;        // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
+  d.ex = d.red + 128;        // 64: 2^(j/64) of exp_tab
+  d.gh = d.ex + 64;          // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
   d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE+NPA
   return d;
 }
@@ -347,16 +366,18 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
       d.cl[2 * l + 1] = hv.y;
     }
   }
+  if (need_loss)
+    for (int j = threadIdx.x; j < 64; j += blockDim.x) d.ex[j] = t.ex64[j];
   __syncthreads();
 }
 
 // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49)
 __device__ __forceinline__ double cex_factor(const PlaneLds &d, int NE, int k, int l, double dts) {
-  return exp(-(d.kt[KT_SIGV * NE + k] * d.cl[2 * l] * dts));
+  return exp_tab(-(d.kt[KT_SIGV * NE + k] * d.cl[2 * l] * dts), d.ex);
 }
 // ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501) as exp(log(ATLOS)*(1/FNHS))
 __device__ __forceinline__ double atm_factor(const PlaneLds &d, int NE, int k, int l) {
-  return exp(d.kt[KT_LATL * NE + k] * d.cl[2 * l + 1]);
+  return exp_tab(d.kt[KT_LATL * NE + k] * d.cl[2 * l + 1], d.ex);
 }
 
 }  // namespace ramgpu
