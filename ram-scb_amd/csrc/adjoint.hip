@@ -374,7 +374,7 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   __syncthreads();
 #pragma unroll
   for (int n = 0; n < MAXC; ++n)
-    if (n < c.nmine) d.H[threadIdx.x + n * kPlaneThreads] = O[n];
+    if (n < c.nmine) d.H[threadIdx.x + n * blockDim.x] = O[n];
   __syncthreads();
 #pragma unroll 1
   for (int n = 0; n < MAXC; ++n) {
@@ -483,7 +483,7 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   __syncthreads();
 #pragma unroll
   for (int n = 0; n < MAXC; ++n)
-    if (n < c.nmine) d.H[threadIdx.x + n * kPlaneThreads] = O[n];
+    if (n < c.nmine) d.H[threadIdx.x + n * blockDim.x] = O[n];
   __syncthreads();
 #pragma unroll 1
   for (int n = 0; n < MAXC; ++n) {
@@ -544,7 +544,7 @@ __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &
 // writes them).  The adjoint stays in LDS between the phases: each phase
 // leaves the adjoint of its input in H, which becomes the next phase's Y.
 template <int SEQ, int MAXC>
-__global__ void __launch_bounds__(kPlaneThreads, (MAXC <= 8) ? 4 : 2) k_plane_bwd(Launch L, const double *xin,
+__global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane_bwd(Launch L, const double *xin,
                                                                                  double *adj) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
@@ -652,11 +652,5 @@ template __global__ void k_plane_bwd<SEQ_CEX, 8>(Launch, const double *, double 
 template __global__ void k_plane_bwd<SEQ_ATM, 8>(Launch, const double *, double *);
 template __global__ void k_plane_bwd<SEQ_LOSS, 8>(Launch, const double *, double *);
 template __global__ void k_plane_bwd<SEQ_STEP, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_E, 16>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_MU, 16>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_CEX, 16>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_ATM, 16>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_LOSS, 16>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_STEP, 16>(Launch, const double *, double *);
 
 }  // namespace ramgpu

@@ -29,11 +29,14 @@ void set_bwd_lds_limit_t(size_t limit) {
 }
 
 // cells per thread of the plane adjoints (cell n of a thread: tid + n*512)
-int plane_bwd_maxc(int ne, int npa) { return (ne * npa <= 8 * kPlaneThreads) ? 8 : 16; }
+// threads of the plane adjoints: cell n of a thread is tid + n*threads, at
+// most 8 cells per thread (NE*NPA <= 8192)
+int plane_bwd_threads(int ne, int npa) { return (ne * npa <= 8 * kPlaneThreads) ? kPlaneThreads : kPlaneMaxThreads; }
 
 template <int MAXC>
-void launch_plane_bwd_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x, double *adj) {
-  const dim3 blk(kPlaneThreads);
+void launch_plane_bwd_t(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
+                        double *adj) {
+  const dim3 blk(threads);
   switch (seq) {
     case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, MAXC>), grid, blk, lds, st, L, x, adj); break;
     case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, MAXC>), grid, blk, lds, st, L, x, adj); break;
@@ -44,17 +47,11 @@ void launch_plane_bwd_t(int seq, dim3 grid, size_t lds, hipStream_t st, const La
   }
 }
 
-void set_plane_bwd_lds_limit(size_t limit) {
-  set_bwd_lds_limit_t<8>(limit);
-  set_bwd_lds_limit_t<16>(limit);
-}
+void set_plane_bwd_lds_limit(size_t limit) { set_bwd_lds_limit_t<8>(limit); }
 
-void launch_plane_bwd(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
+void launch_plane_bwd(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
                       double *adj) {
-  if (maxc <= 8)
-    launch_plane_bwd_t<8>(seq, grid, lds, st, L, x, adj);
-  else
-    launch_plane_bwd_t<16>(seq, grid, lds, st, L, x, adj);
+  launch_plane_bwd_t<8>(seq, threads, grid, lds, st, L, x, adj);
 }
 
 // host Fortran-layout F2 <-> device-layout buffer (through the context's stage)
@@ -143,7 +140,7 @@ int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int 
   const Launch L = launch_of(c, s0, nsp, blocks_kl(c->t));
   const size_t lds = bwd_lds(c);
   tbeg(c, RAMGPU_KERNEL_PLANE_BWD);
-  launch_plane_bwd(seq, plane_bwd_maxc(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, x, adj);
+  launch_plane_bwd(seq, plane_bwd_threads(c->ne, c->npa), dim3(L.nv), lds, c->stream, L, x, adj);
   tend(c);
   return launch_check("k_plane_bwd");
 }

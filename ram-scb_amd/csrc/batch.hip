@@ -235,14 +235,14 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
 }
 
 // adjoint of the step taped at group g: Tab::lam in place, Tab::bbar[step]
-void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_maxc) {
+void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_threads) {
   hipStream_t st = b->stream;
   Launch L = slot_launch(b, BK_R, g + T_R2 + 1, 0, step);
   hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L, nullptr, nullptr, nullptr);
   L = slot_launch(b, BK_P, g + T_P2 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_PLANE, g + T_E1 + 1, 0);
-  launch_plane_bwd(SEQ_STEP, bwd_maxc, dim3(grid_for(b, BK_PLANE)), bwd_lds, st, L, nullptr, nullptr);
+  launch_plane_bwd(SEQ_STEP, bwd_threads, dim3(grid_for(b, BK_PLANE)), bwd_lds, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_P, g + T_P1 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_R, g + T_R1 + 1, 0, step);
@@ -474,10 +474,10 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every, seglen = std::min(ckpt_every, nsteps);
   // plane adjoint LDS
   size_t bwd_lds = 0, lds_limit = b->m[0]->lds_limit;
-  int bwd_maxc = 8;
+  int bwd_threads = kPlaneThreads;
   for (ramgpu_ctx *c : b->m) {
     bwd_lds = std::max(bwd_lds, plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double));
-    bwd_maxc = std::max(bwd_maxc, plane_bwd_maxc(c->ne, c->npa));
+    bwd_threads = std::max(bwd_threads, plane_bwd_threads(c->ne, c->npa));
   }
   if (bwd_lds > lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");
   set_plane_bwd_lds_limit(lds_limit);
@@ -611,7 +611,7 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
         enq_sched(b, s);
       }
       enq_misfit(b, out < 0 ? 0 : out, s, 1);
-      enq_step_bwd(b, group(s, n0), s, bwd_lds, bwd_maxc);
+      enq_step_bwd(b, group(s, n0), s, bwd_lds, bwd_threads);
     }
     if ((rc = launch_check("ensemble adjoint"))) {
       detach();

@@ -118,9 +118,9 @@ void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st,
 // (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
 enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
 // plane adjoint kernels
-int plane_bwd_maxc(int ne, int npa);
+int plane_bwd_threads(int ne, int npa);
 void set_plane_bwd_lds_limit(size_t limit);
-void launch_plane_bwd(int seq, int maxc, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L,
+void launch_plane_bwd(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L,
                       const double *x, double *adj);
 // host Fortran-layout F2 <-> device-layout buffer, through the context's stage
 int upload_f2(ramgpu_ctx *c, double *dst, const double *host);

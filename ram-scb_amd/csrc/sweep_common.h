@@ -310,9 +310,9 @@ struct PlaneCtx {
   bool eact;
 };
 
-// (K,L) of the n-th cell owned by this thread: cell index threadIdx.x + n*512
+// (K,L) of the n-th cell owned by this thread (plane adjoints): cell index threadIdx.x + n*blockDim
 #define CELL_KL(n)                                          \
-  const int cc = threadIdx.x + (n) * kPlaneThreads;          \
+  const int cc = threadIdx.x + (n) * (int)blockDim.x;        \
   const int l = fast_div(cc, c.NE, c.rne), k = cc - l * c.NE; \
   (void)k
 
@@ -334,7 +334,7 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.qs = (double)t.charge[c.s];  // x/QS == x*QS for QS = +-1
   c.beta = t.betalim;
   c.rne = t.rne;
-  c.nmine = (c.P - (int)threadIdx.x + kPlaneThreads - 1) / kPlaneThreads;  // cells tid, tid+512, .. < P
+  c.nmine = (c.P - (int)threadIdx.x + (int)blockDim.x - 1) / (int)blockDim.x;  // cells tid, tid+blockDim, .. < P
   c.lestep = t.plane_threads / NE;  // the problem's block size sets the rows per thread
   c.le0 = fast_div((int)threadIdx.x, NE, c.rne);
   c.ke = (int)threadIdx.x - c.le0 * NE;

@@ -9,7 +9,11 @@ rare, and the check is the dot-product identity <ybar, J v> = <J^T ybar, v>
 to a tolerance of 1e-7 relative (rounding of the difference quotient is
 ~1e-10).  The multi-step misfit gradient (nonlinear through log) is checked
 to 1e-6 relative per direction, or to the rounding floor of the central
-difference where the directional derivative is that small.
+difference where the directional derivative is that small, along 64 random
+directions of the initial F2 and the FGEOS of three single steps.  A limiter
+kink of some cell within +-h of the base point biases the difference quotient
+(the map is only piecewise linear); such a direction is checked again at h/10,
+where the kink is ten times less likely to lie inside the interval.
 """
 import numpy as np
 import pytest
@@ -111,10 +115,10 @@ def test_ram_run_step_vjp_matches_oracle_fd(seed):
     g.close()
 
 
-def fd_noise(loss):
-    """Rounding floor of a central difference of the loss with step H: each
+def fd_noise(loss, h=H):
+    """Rounding floor of a central difference of the loss with step h: each
     evaluation carries ~eps*|L| per step of accumulated rounding; 100x margin."""
-    return 100 * np.finfo(float).eps * abs(loss) / H
+    return 100 * np.finfo(float).eps * abs(loss) / h
 
 
 def _misfit_setup(nsteps, seed=7):
@@ -167,16 +171,24 @@ def test_adjoint_run_gradient_matches_oracle_fd_and_checkpoints():
     print(f"loss gpu {loss:.15e} oracle {ref:.15e}", flush=True)
     assert abs(loss - ref) <= 1e-10 * abs(ref)
     rng = np.random.default_rng(3)
-    # 8 random directions in f0
-    for m in range(8):
+    # 64 random directions in f0 (central differences of the oracle, h = 1e-6 relative)
+    worst = 0.0
+    for m in range(64):
         v = p.f2 * rng.standard_normal(p.f2.shape)
-        lp = _oracle_loss(p, p.f2 + H * v, fg_steps, pts, w, target, kp)
-        lm = _oracle_loss(p, p.f2 - H * v, fg_steps, pts, w, target, kp)
-        fd = (lp - lm) / (2 * H)
         ad = float(np.sum(gf * v))
-        err = abs(fd - ad) / (abs(fd) + 1e-30)
-        print(f"f0 direction {m}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
-        assert err <= 1e-6 or abs(fd - ad) <= fd_noise(loss)
+        for h in (H, H / 10):  # a limiter kink within +-h biases the quotient: retry once at h/10
+            lp = _oracle_loss(p, p.f2 + h * v, fg_steps, pts, w, target, kp)
+            lm = _oracle_loss(p, p.f2 - h * v, fg_steps, pts, w, target, kp)
+            fd = (lp - lm) / (2 * h)
+            err = abs(fd - ad) / (abs(fd) + 1e-30)
+            ok = err <= 1e-6 or abs(fd - ad) <= fd_noise(loss, h)
+            if m < 8 or not ok or h != H:
+                print(f"f0 direction {m} h={h:.0e}: fd {fd:.10e} adjoint {ad:.10e} rel {err:.2e}", flush=True)
+            if ok:
+                break
+        worst = max(worst, min(err, abs(fd - ad) / fd_noise(loss, h)))
+        assert ok
+    print(f"64 f0 directions: worst rel error {worst:.2e} (or fraction of the rounding floor)", flush=True)
     # per-step boundary: the FGEOS of one step at a time
     for n in (0, nsteps // 2, nsteps - 1):
         d = p.fgeos * rng.standard_normal(p.fgeos.shape)
