@@ -620,24 +620,25 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
   const double we1 = top ? 1.0 : d.kt[KT_WE * NE + k + 1], rwe1 = top ? 1.0 : d.kt[KT_RWE * NE + k + 1];
   double O[MAXC];
   double cmax = 0.0;
+  // stencil slots as LDS offsets from F: the cell itself, its ghost F(1)
+  // for K=1, and the one cell the upwind side adds (F(K+2), or F(K-1) /
+  // the ghosts F(0), F(1)); reads past a row land in the plane or in H and
+  // are masked
+  const int gho = (int)(d.gh - d.F);
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
     const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz, qs);
     cmax = fmax(cmax, fmax(fabs(ce), 1E-10));
-    const double *row = d.F + l * NE;
-    const double fk = row[k];
-    const double f0 = (k == 0) ? d.gh[2 * l] : fk;
-    const double fp1 = top ? 0.0 : row[k + 1];
-    double fn, fnm1;
-    if (ce < 0) {
-      fn = (k + 2 < NE) ? row[k + 2] : 0.0;
-      fnm1 = fp1;
-    } else {
-      fn = f0;
-      fnm1 = (k == 0) ? d.gh[2 * l + 1] : (k == 1) ? d.gh[2 * l] : row[k - 1];
-    }
-    const double fb = face_value(f0, fp1, fn, fnm1, ce, div_rn(ce, de, rde), beta);
+    const bool neg = ce < 0;
+    const double fk = d.F[cc];
+    const double f0 = (k == 0) ? d.F[gho + 2 * l] : fk;
+    const double fp1 = top ? 0.0 : d.F[cc + 1];
+    const int up = (k >= 2) ? cc - 1 : gho + 2 * l + (k == 0 ? 1 : 0);
+    const double a = d.F[neg ? cc + 2 : up];
+    const double fn = neg ? ((k + 2 < NE) ? a : 0.0) : f0;
+    const double fnm1 = neg ? fp1 : a;
+    const double fb = face_value_sel(f0, fp1, fn, fnm1, ce, div_rn(ce, de, rde), beta);
     O[n] = fk - div_rn(ce, we, rwe) * fb;
     d.H[cc] = top ? 0.0 : div_rn(ce, we1, rwe1) * fb;
   }
@@ -683,33 +684,23 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
     const int L1 = l + 1;
-    double on = 0.0, h = 0.0;
-    if (L1 >= 2) {
-      const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
-      if (rec) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
-      if (L1 <= NPA - 1) {
-        double fb;
-        const double f0 = d.F[cc];
-        if (L1 == NPA - 1) {
-          fb = d.F[cc + NE];  // FBND(NPA-1) = F(NPA)
-        } else {
-          const double fp1 = d.F[cc + NE];
-          double fn, fnm1;
-          if (cm < 0) {
-            fn = d.F[cc + 2 * NE];
-            fnm1 = fp1;
-          } else {
-            fn = f0;
-            fnm1 = (L1 == 2) ? f0 : d.F[cc - NE];  // F(1) = F(2)
-          }
-          fb = face_value(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
-        }
-        on = div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb;
-        h = div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * fb;
-      }
-    }
-    // face 1: CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0
-    O[n] = d.F[cc] - on;
+    // every row forms the face value; rows 1 and NPA keep 0 (face 1:
+    // CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0; row NPA is
+    // the top cell).  Reads past the plane land in H and are masked.
+    const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
+    if (rec && L1 >= 2) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
+    const bool neg = cm < 0;
+    const double f0 = d.F[cc];
+    const double fp1 = d.F[cc + NE];
+    const double a = d.F[neg ? cc + 2 * NE : (L1 <= 2 ? cc : cc - NE)];  // F(1) = F(2)
+    const double fn = neg ? a : f0, fnm1 = neg ? fp1 : a;
+    const int lw = (L1 <= NPA - 1) ? l + 1 : l;  // WMU(L+1) of the face's upper cell (in range)
+    const double fl = face_value_sel(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
+    const double fb = (L1 == NPA - 1) ? fp1 : fl;  // FBND(NPA-1) = F(NPA)
+    const bool face = L1 >= 2 && L1 <= NPA - 1;
+    const double on = face ? div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb : 0.0;
+    const double h = face ? div_rn(cm, d.lt[LT_WMU * NPA + lw], d.lt[LT_RWMU * NPA + lw]) * fb : 0.0;
+    O[n] = f0 - on;
     d.H[cc] = h;
   }
   __syncthreads();

@@ -75,6 +75,21 @@ __device__ __forceinline__ double face_value(double f0, double fp1, double fn, d
   return fup + lim * corr;
 }
 
+// face_value without branches: every lane forms the ratio and the limited
+// value, then keeps FUP where the reference returns early (|X| <= 1e-27 or
+// R <= 0); the arithmetic of the value kept is face_value's.
+__device__ __forceinline__ double face_value_sel(double f0, double fp1, double fn, double fnm1,
+                                                 double c, double cn, double beta) {
+  const double sg = (c < 0) ? -1.0 : 1.0;
+  const double x = fp1 - f0;
+  const double fup = 0.5 * (f0 + fp1 - sg * x);
+  const double r = (fn - fnm1) / x;
+  const double lim = fmax(fmin(beta * r, 1.0), fmin(r, beta));
+  const double corr = -0.5 * (cn - sg) * x;
+  const double v = fup + lim * corr;
+  return (fabs(x) <= 1.E-27 || r <= 0) ? fup : v;
+}
+
 // The same face value with its partial derivatives with respect to the four
 // stencil values (reverse mode).  fb = FUP + lim(R)*CORR is piecewise linear:
 // on each piece lim = a + b*R, so with kap = -0.5*(cn-sg), X = fp1-f0,
