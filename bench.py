@@ -142,19 +142,24 @@ def bench_batched(members: int, steps: int, seed: int):
         g.set_boundary_scale(scm[m])
         mem.append(g)
     cells = sum(g.ns * g.nr * g.nt * g.ne * g.npa for g in mem)
+    log(f"bench: {members} members on the device ({cells * 8 / 1e9:.1f} GB of F2)")
     b = RamBatch(mem)
     best = b.autotune(reps=2)
+    log(f"bench: autotuned grids {b.shapes()}")
     kp = np.tile(kpm, (steps, 1))
     bs = np.repeat(np.tile(scm, (steps, 1)), 4, axis=1)
     b.ram_run_schedule(kp[:1], bs[:1])          # warm: capture and replay once
     b.ram_run_schedule(kp, bs)
     ms_batch = b.last_run_ms()
+    log(f"bench: ensemble run {ms_batch:.1f} ms; now the members one after another")
     # the same work, member after member: one ram_run_schedule call per member
     t0 = time.perf_counter()
     dev_ms = 0.0
     for m, g in enumerate(mem):
         g.ram_run_schedule(kp[:, m], bs[:, 4 * m:4 * m + 4].ravel())
         dev_ms += g.last_run_ms()
+        if (m + 1) % 64 == 0 and m + 1 < members:
+            log(f"bench:   {m + 1} members done")
     wall_seq = (time.perf_counter() - t0) * 1e3
     # one plain step of every member, one after another (what a per-member
     # host loop does each step)
