@@ -3,9 +3,9 @@
 Tolerance (the stated bar for this FP64 path): the device result must match
 the oracle's to <= 1e-12 relative, norm-wise and cell-wise over cells above
 1e-12 of the maximum.  The drift operators are in fact bit-identical
-(checked exactly in test_drift_operators_bit_exact); the loss operators use
-the device's exp/log10/pow, which may differ from the host's in the last
-ulp.  SUMRC sums are compared to 1e-12 relative of SETRC (the oracle sums
+(checked exactly in test_drift_operators_bit_exact); the loss factors use
+a table-driven exponential and the device's log/log10/pow, which may differ
+from the host's in the last ulp or two.  SUMRC sums are compared to 1e-12 relative of SETRC (the oracle sums
 sequentially like the reference; the device sums in a tree).
 """
 import numpy as np
@@ -102,6 +102,37 @@ def test_ram_run_uneven_large_grid():
     p = inputs.make_problem(nr=40, nt=48, ne=48, npa=90, species=inputs.SPECIES4,
                             composition=inputs.COMPOSITION4, kp=4.0, seed=9)
     _run_compare(p, 1, "4 species 40x48x48x90, 1 step")
+
+
+def test_ram_run_high_resolution_planes():
+    # 64x90 planes (the high-resolution ensemble's): 1024-thread plane blocks
+    p = inputs.make_problem(nr=8, nt=10, ne=64, npa=90, species=(inputs.SPECIES4[1], inputs.SPECIES4[3]),
+                            composition=(0.7, 0.25), kp=5.0, seed=12)
+    _run_compare(p, 2, "H+,O+ 8x10x64x90, 2 steps")
+
+
+@pytest.mark.parametrize("shape", [(256, 16), (768, 8), (1024, 8)])
+def test_plane_shape_changes_no_drift_result(shape):
+    # every launch shape of the plane kernels gives the same F2, bit for bit
+    # (the per-cell arithmetic does not depend on the thread mapping); the
+    # SUMRC sums add in the mapping's order and stay within the bar
+    p = inputs.multi_species(seed=8)
+    g0 = RamGpu.from_problem(p)
+    out0 = g0.ram_run(2)
+    g1 = RamGpu.from_problem(p)
+    g1.set_plane_shape(*shape)
+    assert g1.plane_shape() == shape
+    out1 = g1.ram_run(2)
+    assert np.array_equal(g0.get_f2(), g1.get_f2()), f"plane shape {shape} changed F2"
+    for a, b in zip(out0, out1):
+        for k in ("setrc", "lsdr", "lscha", "lsatm"):
+            assert abs(a[k] - b[k]) <= RTOL * abs(a["setrc"]), (shape, k, a[k], b[k])
+        for k in ("dtdrift_r", "dtdrift_p", "dtdrift_e", "dtdrift_mu"):
+            assert a[k] == b[k], (shape, k)
+    g1.set_plane_shape(0, 0)
+    assert g1.plane_shape() == g0.plane_shape()
+    g0.close()
+    g1.close()
 
 
 def test_ram_run_schedule_kp_ramp():
