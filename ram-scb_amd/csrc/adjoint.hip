@@ -112,13 +112,20 @@ __global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double wnext = 0.0, g1 = 0.0, g2 = 0.0;
     double xpf = X(NR - 3), ypf = ybar(NR);
+    // the descriptor's table pointers, read once (the adjoint's stores could
+    // alias the descriptor); CDriftR as c_radial forms it
+    const double4 *const rijp = t.rij + j;
+    const double2 *const rsip = t.rsi;
+    const double *const q1p = t.q1 + (size_t)l * NR * NT + j;
     for (int I = NR; I >= 2; --I) {
       const double yI = ypf, xn = xpf;
       if (I >= 5) xpf = X(I - 4);
       if (I >= 3) ypf = ybar(I - 1);
       // face I-1 from the window X(I-2..I+1)
       LFace fp;
-      const double c = c_radial(t, p4k, I - 2, j, l);
+      const double4 rr = rijp[(size_t)(I - 2) * NT];
+      const double2 rs = rsip[I - 2];
+      const double c = rr.x + div_rn(div_rn((q1p[(size_t)(I - 2) * NT] * p4k) * 0.5, rr.y, rr.z), rs.x, rs.y);
       if (I >= 3) {
         double x2 = w3;
         if (I == NR && !in && c < 0) x2 = stale_ghost(t, s, j, k, l);  // F(NR+1) read by face NR-1
@@ -283,7 +290,7 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)
 // or f0, fnm1 = fp1 (C<0) or F(K-1) / ghosts F(0), F(1).
-template <int MAXC>
+template <int BT>
 __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
   {
@@ -339,11 +346,11 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
     }
     D = face_d(f0, fp1, fn, fnm1, ce, div_rn(ce, d.kt[KT_DE * NE + k], d.kt[KT_RDE * NE + k]), beta);
   };
-  double O[MAXC];
+  double O[8];
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       double ce;
       FaceD D;
       int tg[4];
@@ -356,9 +363,9 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   __syncthreads();
   // flux weights w (unclamped updated cells) into Y; pass-through terms in O
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       const double yb = Y[cc];
       if (k >= 1) {
         const double v = O[n] + d.H[cc - 1];
@@ -373,13 +380,13 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n)
-    if (n < c.nmine) d.H[threadIdx.x + n * blockDim.x] = O[n];
+  for (int n = 0; n < 8; ++n)
+    if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
 #pragma unroll 1
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       double ce;
       FaceD D;
       int tg[4];
@@ -401,7 +408,7 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
 // L and L-1 (face 1 carries nothing); the output F2(NPA) is F2(NPA-1) times
 // FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1), so its adjoint joins cell NPA-1's;
 // the input F(NPA) only feeds FBND(NPA-1) = F(NPA) and stencils.
-template <int MAXC>
+template <int BT>
 __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
   {
@@ -444,11 +451,11 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
     }
     D = face_d(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
   };
-  double O[MAXC];
+  double O[8];
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       double on = 0.0, h = 0.0;
       if (l >= 1 && l <= NPA - 2) {
         double cm;
@@ -465,9 +472,9 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       const double yb = Y[cc];
       if (l >= 1 && l <= NPA - 2) {
         const double v = O[n] + d.H[cc - NE];
@@ -482,13 +489,13 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   }
   __syncthreads();
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n)
-    if (n < c.nmine) d.H[threadIdx.x + n * blockDim.x] = O[n];
+  for (int n = 0; n < 8; ++n)
+    if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
 #pragma unroll 1
-  for (int n = 0; n < MAXC; ++n) {
+  for (int n = 0; n < 8; ++n) {
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       if (l >= 1 && l <= NPA - 2) {
         double cm;
         FaceD D;
@@ -510,15 +517,15 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
 // Loss operators: F2 *= factor on their cell ranges, so the adjoint is
 // multiplied by the same factors.  which: 0 CHAREXCHANGE, 1 ATMOL, 2 the
 // chain CEX, ATMOL, ATMOL, CEX of ram_run.
-template <int MAXC>
+template <int BT>
 __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y, int which) {
   const int NE = c.NE;
   const bool cex = t.cex[c.s] != 0;
   const int upa = t.upa[c.i];
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n)
+  for (int n = 0; n < 8; ++n)
     if (n < c.nmine) {
-      CELL_KL(n);
+      CELL_KL_T(n, BT);
       const bool cx = cex && k >= 1 && l >= 1 && which != 1;
       const bool atm = k >= 1 && l + 1 >= upa && which != 0;
       double y = Y[cc];
@@ -543,8 +550,8 @@ __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &
 // 0; slots 0..4 hold the inputs of the five phases (as k_plane<SEQ_TAPE>
 // writes them).  The adjoint stays in LDS between the phases: each phase
 // leaves the adjoint of its input in H, which becomes the next phase's Y.
-template <int SEQ, int MAXC>
-__global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane_bwd(Launch L, const double *xin,
+template <int SEQ, int BT>
+__global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin,
                                                                                  double *adj) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
@@ -560,11 +567,11 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane_bwd(Launch L, con
     double *ga = (adj ? adj : t.lam) + po + c.ij * c.P;
     for (int n = threadIdx.x; n < c.P; n += blockDim.x) Y[n] = ga[n];
     __syncthreads();
-    if constexpr (SEQ == SEQ_E) plane_e_bwd<MAXC>(t, c, d, Y);
-    else if constexpr (SEQ == SEQ_MU) plane_mu_bwd<MAXC>(t, c, d, Y);
-    else if constexpr (SEQ == SEQ_CEX) plane_loss_bwd<MAXC>(t, c, d, Y, 0);
-    else if constexpr (SEQ == SEQ_ATM) plane_loss_bwd<MAXC>(t, c, d, Y, 1);
-    else if constexpr (SEQ == SEQ_LOSS) plane_loss_bwd<MAXC>(t, c, d, Y, 2);
+    if constexpr (SEQ == SEQ_E) plane_e_bwd<BT>(t, c, d, Y);
+    else if constexpr (SEQ == SEQ_MU) plane_mu_bwd<BT>(t, c, d, Y);
+    else if constexpr (SEQ == SEQ_CEX) plane_loss_bwd<BT>(t, c, d, Y, 0);
+    else if constexpr (SEQ == SEQ_ATM) plane_loss_bwd<BT>(t, c, d, Y, 1);
+    else if constexpr (SEQ == SEQ_LOSS) plane_loss_bwd<BT>(t, c, d, Y, 2);
     else {
       // phases alternate the two adjoint planes: A holds the output adjoint
       // of a phase, B receives the adjoint of its input
@@ -572,17 +579,17 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane_bwd(Launch L, con
       double *const A = Y, *const B = d.H;
       PlaneLds d2 = d;
       d2.H = A;
-      plane_e_bwd<MAXC>(t, c, d, A);  // E2: A -> B
+      plane_e_bwd<BT>(t, c, d, A);  // E2: A -> B
       for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[3 * ns + n];
       __syncthreads();
-      plane_mu_bwd<MAXC>(t, c, d2, B);  // MU2: B -> A
-      plane_loss_bwd<MAXC>(t, c, d, A, 2);  // loss chain: A -> B (needs only its factors)
+      plane_mu_bwd<BT>(t, c, d2, B);  // MU2: B -> A
+      plane_loss_bwd<BT>(t, c, d, A, 2);  // loss chain: A -> B (needs only its factors)
       for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[ns + n];
       __syncthreads();
-      plane_mu_bwd<MAXC>(t, c, d2, B);  // MU1: B -> A
+      plane_mu_bwd<BT>(t, c, d2, B);  // MU1: B -> A
       for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[n];
       __syncthreads();
-      plane_e_bwd<MAXC>(t, c, d, A);  // E1: A -> B
+      plane_e_bwd<BT>(t, c, d, A);  // E1: A -> B
     }
     for (int n = threadIdx.x; n < c.P; n += blockDim.x) ga[n] = d.H[n];
     __syncthreads();
@@ -646,11 +653,17 @@ __global__ void k_fill(double *y, double v, size_t n) {
 template __global__ void k_r_bwd<0>(Launch, const double *, double *, double *);
 template __global__ void k_r_bwd<1>(Launch, const double *, double *, double *);
 
-template __global__ void k_plane_bwd<SEQ_E, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_MU, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_CEX, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_ATM, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_LOSS, 8>(Launch, const double *, double *);
-template __global__ void k_plane_bwd<SEQ_STEP, 8>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_E, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_E, 1024>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_MU, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_MU, 1024>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_CEX, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_CEX, 1024>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_ATM, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_ATM, 1024>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_LOSS, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_LOSS, 1024>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_STEP, 512>(Launch, const double *, double *);
+template __global__ void k_plane_bwd<SEQ_STEP, 1024>(Launch, const double *, double *);
 
 }  // namespace ramgpu

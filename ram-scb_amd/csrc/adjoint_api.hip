@@ -20,11 +20,11 @@ using namespace ramgpu_host;
 
 namespace ramgpu_host {
 
-template <int MAXC>
+template <int BT>
 void set_bwd_lds_limit_t(size_t limit) {
-  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E, MAXC>,   (const void *)k_plane_bwd<SEQ_MU, MAXC>,
-                       (const void *)k_plane_bwd<SEQ_CEX, MAXC>, (const void *)k_plane_bwd<SEQ_ATM, MAXC>,
-                       (const void *)k_plane_bwd<SEQ_LOSS, MAXC>, (const void *)k_plane_bwd<SEQ_STEP, MAXC>};
+  const void *fns[] = {(const void *)k_plane_bwd<SEQ_E, BT>,   (const void *)k_plane_bwd<SEQ_MU, BT>,
+                       (const void *)k_plane_bwd<SEQ_CEX, BT>, (const void *)k_plane_bwd<SEQ_ATM, BT>,
+                       (const void *)k_plane_bwd<SEQ_LOSS, BT>, (const void *)k_plane_bwd<SEQ_STEP, BT>};
   for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
 }
 
@@ -33,25 +33,31 @@ void set_bwd_lds_limit_t(size_t limit) {
 // most 8 cells per thread (NE*NPA <= 8192)
 int plane_bwd_threads(int ne, int npa) { return (ne * npa <= 8 * kPlaneThreads) ? kPlaneThreads : kPlaneMaxThreads; }
 
-template <int MAXC>
-void launch_plane_bwd_t(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
+template <int BT>
+void launch_plane_bwd_t(int seq, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
                         double *adj) {
-  const dim3 blk(threads);
+  const dim3 blk(BT);
   switch (seq) {
-    case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_CEX: hipLaunchKernelGGL((k_plane_bwd<SEQ_CEX, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_ATM: hipLaunchKernelGGL((k_plane_bwd<SEQ_ATM, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    case SEQ_LOSS: hipLaunchKernelGGL((k_plane_bwd<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, x, adj); break;
-    default: hipLaunchKernelGGL((k_plane_bwd<SEQ_STEP, MAXC>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_E: hipLaunchKernelGGL((k_plane_bwd<SEQ_E, BT>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_MU: hipLaunchKernelGGL((k_plane_bwd<SEQ_MU, BT>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_CEX: hipLaunchKernelGGL((k_plane_bwd<SEQ_CEX, BT>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_ATM: hipLaunchKernelGGL((k_plane_bwd<SEQ_ATM, BT>), grid, blk, lds, st, L, x, adj); break;
+    case SEQ_LOSS: hipLaunchKernelGGL((k_plane_bwd<SEQ_LOSS, BT>), grid, blk, lds, st, L, x, adj); break;
+    default: hipLaunchKernelGGL((k_plane_bwd<SEQ_STEP, BT>), grid, blk, lds, st, L, x, adj); break;
   }
 }
 
-void set_plane_bwd_lds_limit(size_t limit) { set_bwd_lds_limit_t<8>(limit); }
+void set_plane_bwd_lds_limit(size_t limit) {
+  set_bwd_lds_limit_t<kPlaneThreads>(limit);
+  set_bwd_lds_limit_t<kPlaneMaxThreads>(limit);
+}
 
 void launch_plane_bwd(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const Launch &L, const double *x,
                       double *adj) {
-  launch_plane_bwd_t<8>(seq, threads, grid, lds, st, L, x, adj);
+  if (threads <= kPlaneThreads)
+    launch_plane_bwd_t<kPlaneThreads>(seq, grid, lds, st, L, x, adj);
+  else
+    launch_plane_bwd_t<kPlaneMaxThreads>(seq, grid, lds, st, L, x, adj);
 }
 
 // host Fortran-layout F2 <-> device-layout buffer (through the context's stage)

@@ -19,7 +19,7 @@ __global__ void k_setstep(int *step, int v);
 __global__ void k_copy_ens(Launch L);
 template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
 __global__ void k_p_bwd(Launch L, const double *xin, double *adj);
-template <int SEQ, int MAXC> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
+template <int SEQ, int BT> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
 __global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
                          const double *w, const double *target, double *loss_out, double *lam);
 __global__ void k_misfit_ens(Launch L, int step, int adjoint);

@@ -419,12 +419,17 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
         fo[0] = f1;  // cell 1 is not updated
       }
       double wm1 = f1, w0 = f2v, w1 = f3, w2 = fnext;  // F(I-1), F(I), F(I+1), F(I+2) at I=2
+      // the descriptor's table pointers, read once: the stores to F2 below
+      // could alias the descriptor, which would reload them every step
+      const double4 *const rijp = t.rij + j;
+      const double2 *const rsip = t.rsi;
+      const double *const q1p = t.q1 + (size_t)l * NR * NT + j;
       for (int I = 2; I <= NR; ++I) {
         const int i = I - 1;
         fnext = (I + 3 <= NR) ? fl[(size_t)(I + 2) * di] : 0.0;
-        const double4 r = t.rij[(size_t)i * NT + j];
-        const double2 rs = t.rsi[i];
-        const double x = (t.q1[((size_t)l * NR + i) * NT + j] * p4k) * 0.5;
+        const double4 r = rijp[(size_t)i * NT];
+        const double2 rs = rsip[i];
+        const double x = (q1p[(size_t)i * NT] * p4k) * 0.5;
         const double ci = r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
         if (MODE == 1 && r.w == 0.0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));
         double fb;
@@ -454,7 +459,7 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
         if (MODE == 2) {
           if (sum_line) acc += ekk * (v * wek * wml);
           fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;
-          if (j == 0) fs[(size_t)i * di] = (t.rij[(size_t)i * NT + NT - 1].w == 1.0) ? 1e-31 : v;
+          if (j == 0) fs[(size_t)i * di] = (rijp[(size_t)i * NT + NT - 1].w == 1.0) ? 1e-31 : v;
         } else {
           fo[(size_t)i * di] = v;
         }

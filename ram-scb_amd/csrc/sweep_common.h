@@ -325,9 +325,9 @@ struct PlaneCtx {
   bool eact;
 };
 
-// (K,L) of the n-th cell owned by this thread (plane adjoints): cell index threadIdx.x + n*blockDim
-#define CELL_KL(n)                                          \
-  const int cc = threadIdx.x + (n) * (int)blockDim.x;        \
+// (K,L) of the n-th cell owned by this thread of a BT-thread block (plane adjoints): cell threadIdx.x + n*BT
+#define CELL_KL_T(n, BT)                                    \
+  const int cc = threadIdx.x + (n) * (BT);                   \
   const int l = fast_div(cc, c.NE, c.rne), k = cc - l * c.NE; \
   (void)k
 
