@@ -293,14 +293,10 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 template <int BT>
 __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
-  {
-    const double *src = t.kle + c.ij * NPA * 8;
-    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
-  }
   __syncthreads();
-  const double *k8 = t.kij + c.ij * 8;
+  const double *k8 = d.pc + PC_KIJ;  // the per-L terms are in d.cf (plane_setup)
   const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double *rel = t.rel + c.s * 8;
+  const double *rel = d.pc + PC_REL;
   const double r0 = rel[0], r1 = rel[1], r2 = rel[2], r3 = rel[3], r4 = rel[4];
   const double dg1 = r0 / r1 * r2, dg0 = dg1 * r3 / r0 * r4;  // dF(1)/dF(2), dF(0)/dF(2)
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
@@ -411,20 +407,16 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
 template <int BT>
 __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
   const int NE = c.NE, NPA = c.NPA;
-  {
-    const double *src = t.klm + c.ij * NPA * 8;
-    for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) d.cf[n] = src[n];
-  }
-  const double hN = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA), hN1 = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
-  const double ratio = hN * t.mu[NPA - 1] / hN1 / t.mu[NPA - 2];
+  const double hN = d.pc[PC_HN], hN1 = d.pc[PC_HN1];
+  const double ratio = hN * d.pc[PC_MUN] / hN1 / d.pc[PC_MUN1];
   __syncthreads();
-  const double *k8 = t.kij + c.ij * 8;
+  const double *k8 = d.pc + PC_KIJ;  // the per-L terms are in d.cm (plane_setup)
   const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
   const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   // face L = l+1 (2..NPA-1) of column k: Courant number, value, slots (rows)
   auto mface = [&](int k, int l, double &cm, FaceD &D, int (&tg)[4]) {
-    cm = c_mu(d.cf + l * 8, d.kt[KT_MKP * NE + k], drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
+    cm = c_mu(d.cm + l * 8, d.kt[KT_MKP * NE + k], drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
     const int L1 = l + 1, cc = l * NE + k;
     if (L1 == NPA - 1) {  // FBND(NPA-1) = F(NPA)
       D.fb = d.F[cc + NE];
@@ -520,8 +512,8 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
 template <int BT>
 __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y, int which) {
   const int NE = c.NE;
-  const bool cex = t.cex[c.s] != 0;
-  const int upa = t.upa[c.i];
+  const bool cex = c.cex != 0;
+  const int upa = c.upa;
 #pragma unroll
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) {
@@ -529,7 +521,7 @@ __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &
       const bool cx = cex && k >= 1 && l >= 1 && which != 1;
       const bool atm = k >= 1 && l + 1 >= upa && which != 0;
       double y = Y[cc];
-      const double cxf = cx ? cex_factor(d, NE, k, l, t.dts) : 1.0;
+      const double cxf = cx ? cex_factor(d, NE, k, l, d.pc[PC_DTS]) : 1.0;
       const double af = atm ? atm_factor(d, NE, k, l) : 1.0;
       if (which == 2) {
         if (cx) y = y * cxf;
@@ -562,10 +554,11 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
     const double *x0 = xin ? xin : launch_src(L, t);
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, (SEQ == SEQ_STEP) ? x0 + 4 * ns : x0, need_loss, c, d);
+    constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP;
+    plane_setup<4>(t, u, sm, (SEQ == SEQ_STEP) ? x0 + 4 * ns : x0, need_loss, need_kl, c, d);
     double *Y = (double *)(d.cmx + c.NE + c.NPA);
     double *ga = (adj ? adj : t.lam) + po + c.ij * c.P;
-    for (int n = threadIdx.x; n < c.P; n += blockDim.x) Y[n] = ga[n];
+    block_copy<4>(Y, ga, c.P);
     __syncthreads();
     if constexpr (SEQ == SEQ_E) plane_e_bwd<BT>(t, c, d, Y);
     else if constexpr (SEQ == SEQ_MU) plane_mu_bwd<BT>(t, c, d, Y);
@@ -580,18 +573,18 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
       PlaneLds d2 = d;
       d2.H = A;
       plane_e_bwd<BT>(t, c, d, A);  // E2: A -> B
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[3 * ns + n];
+      block_copy<4>(d.F, x + 3 * ns, c.P);
       __syncthreads();
       plane_mu_bwd<BT>(t, c, d2, B);  // MU2: B -> A
       plane_loss_bwd<BT>(t, c, d, A, 2);  // loss chain: A -> B (needs only its factors)
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[ns + n];
+      block_copy<4>(d.F, x + ns, c.P);
       __syncthreads();
       plane_mu_bwd<BT>(t, c, d2, B);  // MU1: B -> A
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = x[n];
+      block_copy<4>(d.F, x, c.P);
       __syncthreads();
       plane_e_bwd<BT>(t, c, d, A);  // E1: A -> B
     }
-    for (int n = threadIdx.x; n < c.P; n += blockDim.x) ga[n] = d.H[n];
+    block_copy<4>(ga, d.H, c.P);
     __syncthreads();
   }
 }

@@ -604,7 +604,7 @@ template <int MAXC>
 __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   {
-    const double *rel = t.rel + c.s * 8;
+    const double *rel = d.pc + PC_REL;
     for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
       const double g1 = d.F[l * NE + 1] * rel[0] / rel[1] * rel[2];
       d.gh[2 * l] = g1;
@@ -614,7 +614,7 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
       for (int k = threadIdx.x; k < NE; k += blockDim.x) d.cmx[k] = 0ull;
   }
   __syncthreads();
-  const double *k8 = t.kij + c.ij * 8;
+  const double *k8 = d.pc + PC_KIJ;
   const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   const int k = c.ke;
@@ -678,7 +678,7 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
     for (int l = threadIdx.x; l < NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
     __syncthreads();
   }
-  const double *k8 = t.kij + c.ij * 8;
+  const double *k8 = d.pc + PC_KIJ;
   const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
   const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
@@ -720,8 +720,8 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   }
   __syncthreads();
   {
-    const double hN = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA), hN1 = GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
-    const double muN = t.mu[NPA - 1], muN1 = t.mu[NPA - 2];
+    const double hN = d.pc[PC_HN], hN1 = d.pc[PC_HN1];
+    const double muN = d.pc[PC_MUN], muN1 = d.pc[PC_MUN1];
     for (int kk = threadIdx.x; kk < NE; kk += blockDim.x)
       d.F[(NPA - 1) * NE + kk] = d.F[(NPA - 2) * NE + kk] * hN * muN / hN1 / muN1;
   }
@@ -737,14 +737,14 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
 // the SUMRC partials of this plane block b in slots 0..4.
 template <int MAXC>
 __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, int b) {
-  const int NE = c.NE, NPA = c.NPA, NT = t.nt;
-  const bool cex = t.cex[c.s] != 0;
-  const int upa = t.upa[c.i];
+  const int NE = c.NE, NPA = c.NPA, NT = c.NT;
+  const bool cex = c.cex != 0;
+  const int upa = c.upa;
   const bool sum_row = c.j <= NT - 2;
   const int k = c.ke;
   const double sigv = d.kt[KT_SIGV * NE + k], latl = d.kt[KT_LATL * NE + k];
   const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
-  const double dts = t.dts;
+  const double dts = d.pc[PC_DTS];
   const double ekw = ek * wk;
   double a[5] = {0, 0, 0, 0, 0}, tot[5];
 #pragma unroll
@@ -796,42 +796,35 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       const size_t off = (size_t)u.s * f2_species_stride(t) + (size_t)(u.b - blocks_kl(t)) * NE * NPA;
       const double *src = launch_src(L, t) + off;
       double *dst = launch_dst(L, t) + off;
-      for (int n = threadIdx.x; n < NE * NPA; n += blockDim.x) dst[n] = src[n];
+      block_copy<8>(dst, src, NE * NPA);
       continue;
     }
     constexpr bool need_loss =
         SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS || SEQ == SEQ_TAPE;
     PlaneCtx c;
     PlaneLds d;
-    plane_setup(t, u, sm, launch_src(L, t), need_loss, c, d);
+    constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE;
+    plane_setup<8>(t, u, sm, launch_src(L, t), need_loss, need_kl, c, d);
     double *gF = launch_dst(L, t) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
-    const bool inside = t.kij[c.ij * 8 + KIJ_MG] == 0.0;
+    const bool inside = d.pc[PC_KIJ + KIJ_MG] == 0.0;
     const bool rec_here = rec && inside;
-    if constexpr (SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE) {  // per-L terms
-      const double *se = t.kle + c.ij * NPA * 8, *sm8 = t.klm + c.ij * NPA * 8;
-      for (int n = threadIdx.x; n < NPA * 8; n += blockDim.x) {
-        d.cf[n] = se[n];
-        d.cm[n] = sm8[n];
-      }
-      __syncthreads();
-    }
 
     if constexpr (SEQ == SEQ_E) {
       plane_e<MAXC>(t, c, d, rec_here);
     } else if constexpr (SEQ == SEQ_MU) {
       plane_mu<MAXC>(t, c, d, rec_here);
     } else if constexpr (SEQ == SEQ_CEX) {
-      if (t.cex[c.s] != 0 && c.ke >= 1) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
+      if (c.cex != 0 && c.ke >= 1) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
 #pragma unroll
         for (int n = 0; n < MAXC; ++n) {
           ROW_CELLS(n);
-          if (l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, c.ke, l, t.dts);
+          if (l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, c.ke, l, d.pc[PC_DTS]);
         }
       }
       __syncthreads();
     } else if constexpr (SEQ == SEQ_ATM) {
       // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
-      const int upa = t.upa[c.i];
+      const int upa = c.upa;
       if (c.ke >= 1) {
 #pragma unroll
         for (int n = 0; n < MAXC; ++n) {
@@ -864,14 +857,14 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * c.P, ns = f2_elems(t);
       double *slot = const_cast<double *>(launch_src(L, t)) + po;
       plane_e<MAXC>(t, c, d, rec_here);
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[ns + n] = d.F[n];
+      block_copy<4>(slot + ns, d.F, c.P);
       plane_mu<MAXC>(t, c, d, rec_here);
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[2 * ns + n] = d.F[n];
+      block_copy<4>(slot + 2 * ns, d.F, c.P);
       __syncthreads();
       plane_loss<MAXC>(t, c, d, u.b);
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[3 * ns + n] = d.F[n];
+      block_copy<4>(slot + 3 * ns, d.F, c.P);
       plane_mu<MAXC>(t, c, d, false);
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) slot[4 * ns + n] = d.F[n];
+      block_copy<4>(slot + 4 * ns, d.F, c.P);
       plane_e<MAXC>(t, c, d, false);
     } else {  // SEQ_STEP
       plane_e<MAXC>(t, c, d, rec_here);
@@ -881,9 +874,7 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       plane_e<MAXC>(t, c, d, false);
     }
 
-    if constexpr (SEQ != SEQ_SUM) {
-      for (int n = threadIdx.x; n < c.P; n += blockDim.x) gF[n] = d.F[n];
-    }
+    if constexpr (SEQ != SEQ_SUM) block_copy<8>(gF, d.F, c.P);
     __syncthreads();
   }
 }

@@ -159,7 +159,7 @@ inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.n
 // LDS bytes of the plane kernel
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
-  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 +
+  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
           2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
 }
 

@@ -291,11 +291,16 @@ __device__ __forceinline__ size_t f2_elems(const Tab &t) { return (size_t)t.ns *
 
 // LDS layout of the plane kernel (doubles)
 struct PlaneLds {
-  double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *ex, *gh;
+  double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *ex, *pc, *gh;
   unsigned long long *cmx;
 };
 enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
 enum LtField { LT_DMU, LT_RDMU, LT_WMU, LT_RWMU, LT_N };
+// Scalars of one plane, staged in LDS with the plane so that the phases read
+// them as broadcasts instead of making dependent global loads after each
+// barrier: the (I,J) row of kij, the species' relativistic factors, the
+// top-cell FNHS(NPA), FNHS(NPA-1), MU(NPA), MU(NPA-1) of DRIFTMU, and DTs.
+enum PcField { PC_KIJ = 0, PC_REL = 8, PC_HN = 16, PC_HN1, PC_MUN, PC_MUN1, PC_DTS, PC_N = 24 };
 
 __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   const int P = ne * npa;
@@ -309,13 +314,15 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
   d.red = d.lt + LT_N * npa; // 128: block reductions (up to 16 waves x 5 sums)
   d.ex = d.red + 128;        // 64: 2^(j/64) of exp_tab
-  d.gh = d.ex + 64;          // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
+  d.pc = d.ex + 64;          // PC_N: the plane's scalars (PcField)
+  d.gh = d.pc + PC_N;        // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
   d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE+NPA
   return d;
 }
 
 struct PlaneCtx {
-  int s, i, j, NE, NPA, P, nmine;
+  int s, i, j, NE, NPA, NT, P, nmine;
+  int cex, upa;  // the species' charge-exchange kind, UPA(I)
   size_t ij;
   double rlz, rrlz, qs, beta;
   float rne;
@@ -331,34 +338,147 @@ struct PlaneCtx {
   const int l = fast_div(cc, c.NE, c.rne), k = cc - l * c.NE; \
   (void)k
 
+// Block copy of n doubles between global memory and LDS (either way).  A
+// strided loop of plain assignments waits for each load before its store
+// (flat loads are not moved past stores that may alias them), so each pass
+// costs a memory latency; here every thread has up to U loads in flight
+// before its first store.
+template <int U>
+__device__ __forceinline__ void block_copy(double *dst, const double *src, int n) {
+  const int T = blockDim.x;
+  for (int i0 = threadIdx.x; i0 < n; i0 += U * T) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = (i0 + u * T < n) ? src[i0 + u * T] : 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * T < n) dst[i0 + u * T] = v[u];
+  }
+}
+
 // Block set-up of the plane kernels: context of plane b of species s (I>=2)
 // and the LDS copies of the plane (from src, device layout of one species),
-// of the per-K / per-L tables and, with need_loss, of HDNS and 1/FNHS.
+// of the plane's scalars (PcField), of the per-K / per-L tables, with
+// need_loss of HDNS, 1/FNHS and the exponential's table, and with need_kl of
+// the per-L DRIFTE and DRIFTMU terms.  The loads go in two batches, each
+// issued whole before its LDS stores (two memory latencies per plane; one
+// batch would need more registers than the phases); plane cells beyond
+// KS per thread follow in a further pass (KS sets the registers the set-up
+// takes beside what the kernel keeps live across it).
+template <int KS>
 __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, const double *src, bool need_loss,
-                                   PlaneCtx &c, PlaneLds &d) {
+                                   bool need_kl, PlaneCtx &c, PlaneLds &d) {
   const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr;
+  const int tid = threadIdx.x, T = blockDim.x;
   c.s = u.s;
   c.i = 1 + u.b / NT;  // I = i+1 >= 2
   c.j = u.b - (c.i - 1) * NT;
   c.NE = NE;
   c.NPA = NPA;
+  c.NT = NT;
   c.P = NE * NPA;
   c.ij = (size_t)c.i * NT + c.j;
   c.rlz = t.rlz[c.i];
   c.rrlz = t.rrlz[c.i];
   c.qs = (double)t.charge[c.s];  // x/QS == x*QS for QS = +-1
+  c.cex = t.cex[c.s];
+  c.upa = t.upa[c.i];
   c.beta = t.betalim;
   c.rne = t.rne;
-  c.nmine = (c.P - (int)threadIdx.x + (int)blockDim.x - 1) / (int)blockDim.x;  // cells tid, tid+blockDim, .. < P
+  c.nmine = (c.P - tid + T - 1) / T;  // cells tid, tid+blockDim, .. < P
   c.lestep = t.plane_threads / NE;  // the problem's block size sets the rows per thread
-  c.le0 = fast_div((int)threadIdx.x, NE, c.rne);
-  c.ke = (int)threadIdx.x - c.le0 * NE;
-  c.eact = (int)threadIdx.x < c.lestep * NE;
+  c.le0 = fast_div(tid, NE, c.rne);
+  c.ke = tid - c.le0 * NE;
+  c.eact = tid < c.lestep * NE;
   d = plane_lds(sm, NE, NPA);
-  const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
-  for (int n = threadIdx.x; n < c.P; n += blockDim.x) d.F[n] = gF[n];
+  const int P = c.P, nkl = 8 * NPA;
+  const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * P;
+  const double *se = t.kle + c.ij * nkl, *sm8 = t.klm + c.ij * nkl;
+
+  // batch A: the plane, its per-L terms and scalars
+  double vf[KS];
+#pragma unroll
+  for (int m = 0; m < KS; ++m) vf[m] = (tid + m * T < P) ? gF[tid + m * T] : 0.0;
+  double vc[2] = {0.0, 0.0}, vm[2] = {0.0, 0.0};
+  if (need_kl) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (tid + m * T < nkl) {
+        vc[m] = se[tid + m * T];
+        vm[m] = sm8[tid + m * T];
+      }
+  }
+  double vp = 0.0;
+  if (tid < PC_N) {
+    const double *a;
+    if (tid < PC_REL) a = t.kij + c.ij * 8 + tid;
+    else if (tid < PC_HN) a = t.rel + c.s * 8 + (tid - PC_REL);
+    else if (tid == PC_HN) a = &GEO3(t.fnhs, c.i + 1, c.j + 1, NPA);
+    else if (tid == PC_HN1) a = &GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
+    else if (tid == PC_MUN) a = t.mu + NPA - 1;
+    else if (tid == PC_MUN1) a = t.mu + NPA - 2;
+    else a = &t.dts;
+    vp = *a;
+  }
+#pragma unroll
+  for (int m = 0; m < KS; ++m)
+    if (tid + m * T < P) d.F[tid + m * T] = vf[m];
+  if (need_kl) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (tid + m * T < nkl) {
+        d.cf[tid + m * T] = vc[m];
+        d.cm[tid + m * T] = vm[m];
+      }
+  }
+  if (tid < PC_N) d.pc[tid] = vp;
+
+  // batch B: per-K and per-L tables
   const size_t sk = (size_t)c.s * NE, sik = ((size_t)c.s * NR + c.i) * NE;
-  for (int k = threadIdx.x; k < NE; k += blockDim.x) {
+  double vk[KT_N];
+  if (tid < NE) {
+    vk[KT_EKP] = t.ekp[sk + tid];
+    vk[KT_MKP] = t.mkp[sk + tid];
+    vk[KT_SIGV] = t.sigv[sk + tid];
+    vk[KT_EDOT] = t.edot[sik + tid];
+    vk[KT_LATL] = t.latl[sik + tid];
+    vk[KT_DE] = t.de[tid];
+    vk[KT_RDE] = t.rde[tid];
+    vk[KT_WE] = t.we[tid];
+    vk[KT_RWE] = t.rwe[tid];
+    vk[KT_EKEV] = t.ekev[tid];
+  }
+  double vl[LT_N];
+  double2 hv = make_double2(0.0, 0.0);
+  if (tid < NPA) {
+    vl[LT_DMU] = t.dmu[tid];
+    vl[LT_RDMU] = t.rdmu[tid];
+    vl[LT_WMU] = t.wmu[tid];
+    vl[LT_RWMU] = t.rwmu[tid];
+    if (need_loss) hv = t.kll[c.ij * NPA + tid];
+  }
+  const double vx = (need_loss && tid < 64) ? t.ex64[tid] : 0.0;
+
+  if (tid < NE)
+#pragma unroll
+    for (int f = 0; f < KT_N; ++f) d.kt[f * NE + tid] = vk[f];
+  if (tid < NPA) {
+#pragma unroll
+    for (int f = 0; f < LT_N; ++f) d.lt[f * NPA + tid] = vl[f];
+    if (need_loss) {
+      d.cl[2 * tid] = hv.x;
+      d.cl[2 * tid + 1] = hv.y;
+    }
+  }
+  if (need_loss && tid < 64) d.ex[tid] = vx;
+
+  // what one pass did not cover (large planes, small blocks)
+  if (P > KS * T) block_copy<4>(d.F + KS * T, gF + KS * T, P - KS * T);
+  if (need_kl && nkl > 2 * T) {
+    block_copy<2>(d.cf + 2 * T, se + 2 * T, nkl - 2 * T);
+    block_copy<2>(d.cm + 2 * T, sm8 + 2 * T, nkl - 2 * T);
+  }
+  for (int k = tid + T; k < NE; k += T) {
     d.kt[KT_EKP * NE + k] = t.ekp[sk + k];
     d.kt[KT_MKP * NE + k] = t.mkp[sk + k];
     d.kt[KT_SIGV * NE + k] = t.sigv[sk + k];
@@ -370,19 +490,17 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
     d.kt[KT_RWE * NE + k] = t.rwe[k];
     d.kt[KT_EKEV * NE + k] = t.ekev[k];
   }
-  for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
+  for (int l = tid + T; l < NPA; l += T) {
     d.lt[LT_DMU * NPA + l] = t.dmu[l];
     d.lt[LT_RDMU * NPA + l] = t.rdmu[l];
     d.lt[LT_WMU * NPA + l] = t.wmu[l];
     d.lt[LT_RWMU * NPA + l] = t.rwmu[l];
     if (need_loss) {
-      const double2 hv = t.kll[c.ij * NPA + l];
-      d.cl[2 * l] = hv.x;
-      d.cl[2 * l + 1] = hv.y;
+      const double2 h = t.kll[c.ij * NPA + l];
+      d.cl[2 * l] = h.x;
+      d.cl[2 * l + 1] = h.y;
     }
   }
-  if (need_loss)
-    for (int j = threadIdx.x; j < 64; j += blockDim.x) d.ex[j] = t.ex64[j];
   __syncthreads();
 }
 
