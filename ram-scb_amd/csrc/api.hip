@@ -292,7 +292,7 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp) {
   const Launch L = launch_of(c, s0, nsp, 1);
   tbeg(c, RAMGPU_KERNEL_FINALIZE);
-  hipLaunchKernelGGL(k_finalize, dim3(L.nv), dim3(256), 0, c->stream, L, cflmask, summode);
+  hipLaunchKernelGGL(k_finalize, dim3(L.nv), dim3(kFinalizeThreads), 0, c->stream, L, cflmask, summode);
   tend(c);
   return launch_check("k_finalize");
 }
@@ -631,7 +631,8 @@ int ramgpu_set_kp(ramgpu_ctx *c, double kp, const double *lz, const double *phi)
   }
   if (!c->have_lzphi) return fail(RAMGPU_EINVAL, "ramgpu_set_kp needs lz and phi on its first call");
   hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
-                     (const double *)c->d_phi, (const double *)nullptr, kp, c->nr + 1, c->nt);
+                     (const double *)c->d_phi, (const double *)nullptr, kp, c->nr + 1, c->nt,
+                     (const double *)nullptr, (double *)nullptr, 0);
   if ((rc = launch_check("k_vs"))) return rc;
   c->have_vt = true;
   c->efield_dirty = true;
@@ -796,18 +797,12 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
   for (int n = 0; n < nsteps && rc == RAMGPU_OK; ++n) {
     tbeg(c, RAMGPU_KERNEL_VS);
     hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
-                       (const double *)c->d_phi, (const double *)(d_kp + n), 0.0, c->nr + 1, c->nt);
+                       (const double *)c->d_phi, (const double *)(d_kp + n), 0.0, c->nr + 1, c->nt,
+                       (const double *)(d_bs ? d_bs + (size_t)n * c->ns : nullptr), c->d_bscale, c->ns);
     tend(c);
     if ((rc = launch_check("k_vs"))) break;
     c->efield_dirty = true;
-    if (d_bs) {
-      if (hipMemcpyAsync(c->d_bscale, d_bs + (size_t)n * c->ns, c->ns * sizeof(double),
-                         hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
-        rc = fail(RAMGPU_EHIP, "boundary scale copy failed");
-        break;
-      }
-      c->ghost_dirty = true;
-    }
+    if (d_bs) c->ghost_dirty = true;
     rc = step_once(c);
   }
   (void)hipEventRecord(c->ev1, c->stream);

@@ -172,7 +172,7 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
   launch_plane(SEQ_STEP, b->plane, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
   hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, lp);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, lr);
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(256), 0, st, lf, 15, 2);
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
   if (sched) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, b->d_step);
   return launch_check("ensemble step");
 }
@@ -231,7 +231,7 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
   L = slot_launch(b, BK_R, g + T_R2 + 1, dsl_out);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L);
   const Launch lf = launch_for(b, BK_FIN);
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(256), 0, st, lf, 15, 2);
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
 }
 
 // adjoint of the step taped at group g: Tab::lam in place, Tab::bbar[step]

@@ -34,7 +34,7 @@ template <int MODE> __global__ void k_p(Launch L);
 template <int SEQ, int MAXC> __global__ void k_plane(Launch L, int rec);
 __global__ void k_finalize(Launch L, int cflmask, int summode);
 __global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
-                     double kp_val, int nr1, int nt);
+                     double kp_val, int nr1, int nt, const double *bs_src, double *bs_dst, int ns);
 __global__ void k_copy(double2 *dst, const double2 *src, size_t n2);
 __global__ void k_f2_in(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);
 __global__ void k_f2_out(double *dst, const double *src, int ns, int nr, int nt, int ne, int npa);

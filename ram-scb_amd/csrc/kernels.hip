@@ -315,44 +315,51 @@ __global__ void k_coef(Launch L) {
 // order, or 0 when no inflow line came before.  Whether a line is inflow
 // depends on geometry, fields and grid only, so the carry is precomputed:
 // carry[s][p] is the index p'*NT+(J'-1) of the last inflow line before plane
-// p=(K-1)*NPA+(L-1), or -1.  One block per (problem, species); block-wide
-// max-scan.
-__global__ void k_ghost(Launch L) {
-  __shared__ int s_scan[1024];
+// p=(K-1)*NPA+(L-1), or -1.  One block per (problem, species): each thread
+// finds the top inflow J of its planes (four J per memory round trip) into
+// LDS, then a block-wide exclusive max-scan over the planes in order (wave
+// shuffles, then the waves' totals).
+__global__ void __launch_bounds__(1024) k_ghost(Launch L) {
+  __shared__ int s_last[kMaxPlaneCells];
+  __shared__ int s_wave[16];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    const int s = u.s;
-    const int P = t.ne * t.npa;
-    const int nthr = blockDim.x;
-    const int chunk = (P + nthr - 1) / nthr;
-    const int p0 = threadIdx.x * chunk, p1 = min(P, p0 + chunk);
-    int last = -1;
-    for (int p = p0; p < p1; ++p) {
-      const int k = p / t.npa, l = p % t.npa;
-      for (int j = t.nt - 1; j >= 0; --j)
-        if (line_inflow(t, s, j, k, l)) {
-          last = p * t.nt + j;
-          break;
-        }
+    const int s = u.s, NT = t.nt, NPA = t.npa;
+    const int P = t.ne * NPA, T = blockDim.x, tid = threadIdx.x;
+    for (int p = tid; p < P; p += T) {
+      const int k = p / NPA, l = p - k * NPA;
+      int last = -1;
+      for (int j0 = NT - 1; j0 >= 0 && last < 0; j0 -= 4) {
+        bool in[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) in[m] = line_inflow(t, s, max(j0 - m, 0), k, l) && j0 - m >= 0;
+#pragma unroll
+        for (int m = 3; m >= 0; --m)
+          if (in[m]) last = p * NT + j0 - m;  // the highest inflow J of the group
+      }
+      s_last[p] = last;
     }
-    s_scan[threadIdx.x] = last;
     __syncthreads();
-    for (int off = 1; off < nthr; off <<= 1) {  // inclusive max-scan
-      const int v = (threadIdx.x >= off) ? s_scan[threadIdx.x - off] : -1;
-      __syncthreads();
-      s_scan[threadIdx.x] = max(s_scan[threadIdx.x], v);
-      __syncthreads();
+    // planes [p0, p1) of this thread, in order
+    const int chunk = (P + T - 1) / T;
+    const int p0 = min(P, tid * chunk), p1 = min(P, p0 + chunk);
+    int v = -1;
+    for (int p = p0; p < p1; ++p) v = max(v, s_last[p]);
+    const int lane = tid & 63, w = tid >> 6;
+    for (int off = 1; off < 64; off <<= 1) {  // inclusive max-scan in the wave
+      const int o = __shfl_up(v, off, 64);
+      if (lane >= off) v = max(v, o);
     }
-    int run = (threadIdx.x > 0) ? s_scan[threadIdx.x - 1] : -1;
+    if (lane == 63) s_wave[w] = v;
+    __syncthreads();
+    int run = __shfl_up(v, 1, 64);
+    if (lane == 0) run = -1;
+    for (int ww = 0; ww < w; ++ww) run = max(run, s_wave[ww]);
+    // the index of an inflow line grows with p, so the last one is the maximum
     for (int p = p0; p < p1; ++p) {
       t.carry[(size_t)s * P + p] = run;
-      const int k = p / t.npa, l = p % t.npa;
-      for (int j = t.nt - 1; j >= 0; --j)
-        if (line_inflow(t, s, j, k, l)) {
-          run = p * t.nt + j;
-          break;
-        }
+      run = max(run, s_last[p]);
     }
     __syncthreads();
   }
@@ -889,95 +896,132 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
 // from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
 // slots 0..5 (slots 0..4 from the plane blocks, slot 5 from the R blocks).
 // One block per (problem, species).
-__global__ void k_finalize(Launch L, int cflmask, int summode) {
-  __shared__ double red[4 * kSumSlots];
-  __shared__ double S[kSumSlots];
+__global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cflmask, int summode) {
+  constexpr int NV = 2 + kSumSlots;  // the DtDriftE and DtDriftMu minima, then the SUMRC slot sums
+  __shared__ double red[(kFinalizeThreads / 64) * NV];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    const int s = u.s;
+    const int s = u.s, tid = threadIdx.x, T = blockDim.x;
     const int NE = t.ne, NPA = t.npa;
     const double a = t.fraccfl * t.dts;
     unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
-    if (cflmask) {
-      double de = 10000.0, dm = 10000.0;
-      if (cflmask & 4)
-        for (int k = threadIdx.x; k < NE; k += blockDim.x) {
-          const double v = __longlong_as_double((long long)m[2 + k]);
-          if (v > 0) de = fmin(de, a * t.de[k] / v);
-        }
-      if (cflmask & 8)
-        for (int l = 1 + threadIdx.x; l < NPA; l += blockDim.x) {
-          const double v = __longlong_as_double((long long)m[2 + NE + l]);
-          if (v > 0) dm = fmin(dm, a * t.dmu[l] / v);
-        }
-      de = block_min(de, red);
-      dm = block_min(dm, red);
-      if (threadIdx.x == 0) {
-        if (cflmask & 1) {
-          const double v = __longlong_as_double((long long)m[0]);
-          t.dtdrift[s * 4 + 0] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
-        }
-        if (cflmask & 2) {
-          const double v = __longlong_as_double((long long)m[1]);
-          t.dtdrift[s * 4 + 1] = (v > 0) ? fmin(100000.0, a / v) : 100000.0;
-        }
-        if (cflmask & 4) t.dtdrift[s * 4 + 2] = de;
-        if (cflmask & 8) t.dtdrift[s * 4 + 3] = dm;
-      }
-      __syncthreads();
-      for (int e = threadIdx.x; e < 2 + NE + NPA; e += blockDim.x) m[e] = 0ull;
+    // every load of the block first: thread 0's scalars, the CFL maxima, the
+    // plane and radial blocks' partial sums
+    double e0[4] = {0.0, 0.0, 0.0, 0.0};
+    unsigned long long m01[2] = {0ull, 0ull};
+    int cex = 0;
+    if (tid == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) e0[q] = t.energy[s * 4 + q];
+      m01[0] = m[0];
+      m01[1] = m[1];
+      cex = t.cex[s];
     }
+    double v[NV];
+    v[0] = 10000.0;
+    v[1] = 10000.0;
+#pragma unroll
+    for (int q = 2; q < NV; ++q) v[q] = 0.0;
+    if (cflmask & 4)
+      for (int k = tid; k < NE; k += T) {
+        const double c = __longlong_as_double((long long)m[2 + k]);
+        if (c > 0) v[0] = fmin(v[0], a * t.de[k] / c);
+      }
+    if (cflmask & 8)
+      for (int l = 1 + tid; l < NPA; l += T) {
+        const double c = __longlong_as_double((long long)m[2 + NE + l]);
+        if (c > 0) v[1] = fmin(v[1], a * t.dmu[l] / c);
+      }
     if (summode != 0) {
-      // all slots in one pass: slots 0..4 from the plane blocks, slot 5 from the R blocks
+      // slots 0..4 from the plane blocks, slot 5 from the R blocks
       const int nslot = (summode == 1) ? 1 : kSumSlots;
       const int nbk = blocks_kl(t), nbr = blocks_r(t);
       const double *pp = t.partials + (size_t)s * t.maxblk;
       const size_t slot = (size_t)t.ns * t.maxblk;
-      double acc[kSumSlots] = {0, 0, 0, 0, 0, 0}, tot[kSumSlots];
-      for (int b = threadIdx.x; b < nbk; b += blockDim.x)
-        for (int sl = 0; sl < nslot && sl < 5; ++sl) acc[sl] += pp[sl * slot + b];
+      for (int b = tid; b < nbk; b += T)
+        for (int sl = 0; sl < nslot && sl < 5; ++sl) v[2 + sl] += pp[sl * slot + b];
       if (nslot == kSumSlots)
-        for (int b = threadIdx.x; b < nbr; b += blockDim.x) acc[5] += pp[5 * slot + b];
-      block_sum_n<kSumSlots>(acc, red, tot);
-      if (threadIdx.x == 0)
-        for (int sl = 0; sl < kSumSlots; ++sl) S[sl] = tot[sl];
-      __syncthreads();
-      if (threadIdx.x == 0) {
+        for (int b = tid; b < nbr; b += T) v[2 + 5] += pp[5 * slot + b];
+    }
+    // one block reduction (fixed order): minima of v[0..1], sums of the rest
+    for (int off = 32; off > 0; off >>= 1) {
+      v[0] = fmin(v[0], __shfl_down(v[0], off, 64));
+      v[1] = fmin(v[1], __shfl_down(v[1], off, 64));
+#pragma unroll
+      for (int q = 2; q < NV; ++q) v[q] += __shfl_down(v[q], off, 64);
+    }
+    const int lane = tid & 63, w = tid >> 6, nw = (T + 63) >> 6;
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < NV; ++q) red[w * NV + q] = v[q];
+    __syncthreads();
+    if (tid == 0) {
+      double r[NV];
+#pragma unroll
+      for (int q = 0; q < NV; ++q) r[q] = red[q];
+      for (int ww = 1; ww < nw; ++ww) {
+        r[0] = fmin(r[0], red[ww * NV]);
+        r[1] = fmin(r[1], red[ww * NV + 1]);
+#pragma unroll
+        for (int q = 2; q < NV; ++q) r[q] += red[ww * NV + q];
+      }
+      if (cflmask & 1) {
+        const double c = __longlong_as_double((long long)m01[0]);
+        t.dtdrift[s * 4 + 0] = (c > 0) ? fmin(100000.0, a / c) : 100000.0;
+      }
+      if (cflmask & 2) {
+        const double c = __longlong_as_double((long long)m01[1]);
+        t.dtdrift[s * 4 + 1] = (c > 0) ? fmin(100000.0, a / c) : 100000.0;
+      }
+      if (cflmask & 4) t.dtdrift[s * 4 + 2] = r[0];
+      if (cflmask & 8) t.dtdrift[s * 4 + 3] = r[1];
+      if (summode != 0) {
+        const double *S = r + 2;
         double *e = t.energy + s * 4;
-        const double old = e[0];
+        const double old = e0[0];
         if (summode == 1) {
           e[0] = S[0];
           t.partials[((size_t)kSumSlots * t.ns + s) * t.maxblk] = old - S[0];  // ELORC
         } else {
-          const bool cex = t.cex[s] != 0;
+          double lsdr = e0[1], lscha = e0[2], lsatm = e0[3];
           double prev = old;
-          e[1] += prev - S[0];  // LSDR
+          lsdr += prev - S[0];
           prev = S[0];
           if (cex) {
-            e[2] += prev - S[1];  // LSCHA
+            lscha += prev - S[1];
             prev = S[1];
           }
-          e[3] += prev - S[2];  // LSATM
+          lsatm += prev - S[2];
           prev = S[2];
-          e[3] += prev - S[3];  // LSATM
+          lsatm += prev - S[3];
           prev = S[3];
           if (cex) {
-            e[2] += prev - S[4];  // LSCHA
+            lscha += prev - S[4];
             prev = S[4];
           }
-          e[1] += prev - S[5];  // LSDR
+          lsdr += prev - S[5];
           e[0] = S[5];
+          e[1] = lsdr;
+          e[2] = lscha;
+          e[3] = lsatm;
         }
       }
-      __syncthreads();
     }
+    __syncthreads();  // every read of the maxima is done
+    if (cflmask)
+      for (int e = tid; e < 2 + NE + NPA; e += T) m[e] = 0ull;
   }
 }
 
 // VT(I,J)=AVS*(LZ(I)*RE)**2*SIN(PHI(J)-PHIOFS), src/ModRamRun.f90:45-54
+// Volland-Stern VT(I,J) from Kp (*kp_ptr, or kp_val); with bs_src, also the
+// step's boundary-flux scales bs_src[0..ns) into bs_dst (one launch per step
+// of a schedule instead of a kernel and a copy).
 __global__ void k_vs(double *vt, const double *lz, const double *phi, const double *kp_ptr,
-                     double kp_val, int nr1, int nt) {
+                     double kp_val, int nr1, int nt, const double *bs_src, double *bs_dst, int ns) {
+  if (bs_src && blockIdx.x == 0)
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) bs_dst[s] = bs_src[s];
   const double kp = kp_ptr ? *kp_ptr : kp_val;
   const double d = 1.0 - 0.159 * kp + 0.0093 * (kp * kp);
   const double avs = 7.05E-6 / (d * d * d) / kRE;
