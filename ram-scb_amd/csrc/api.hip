@@ -195,13 +195,14 @@ int run_r(ramgpu_ctx *c, int mode, int s0, int nsp, const double *src, double *d
   Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
   L.src = src;
   L.dst = dst;
+  const size_t lds = radial_lds_bytes(c->nr, c->ne, c->npa);
   tbeg(c, RAMGPU_KERNEL_R);
   if (mode == 0)
-    hipLaunchKernelGGL(k_r<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+    hipLaunchKernelGGL(k_r<0>, dim3(L.nv), dim3(kLineThreads), lds, c->stream, L);
   else if (mode == 1)
-    hipLaunchKernelGGL(k_r<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+    hipLaunchKernelGGL(k_r<1>, dim3(L.nv), dim3(kLineThreads), lds, c->stream, L);
   else
-    hipLaunchKernelGGL(k_r<2>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+    hipLaunchKernelGGL(k_r<2>, dim3(L.nv), dim3(kLineThreads), lds, c->stream, L);
   tend(c);
   return launch_check("k_r");
 }
@@ -339,8 +340,8 @@ int check_dims(int nr, int nt, int ne, int npa) {
   return RAMGPU_OK;
 }
 
-// Let the plane kernels use the device's whole LDS; refuse planes that do
-// not fit it.
+// Let the plane kernels and the radial sweep use the device's whole LDS
+// (less each kernel's static LDS); refuse planes that do not fit it.
 int set_plane_lds_limit(size_t limit) {
   const void *fns[] = {
       (const void *)k_plane<SEQ_E, 8>,    (const void *)k_plane<SEQ_MU, 8>,   (const void *)k_plane<SEQ_CEX, 8>,
@@ -348,8 +349,16 @@ int set_plane_lds_limit(size_t limit) {
       (const void *)k_plane<SEQ_LOSS, 8>, (const void *)k_plane<SEQ_E, 16>,   (const void *)k_plane<SEQ_MU, 16>,
       (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
       (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>, (const void *)k_plane<SEQ_TAPE, 8>,
-      (const void *)k_plane<SEQ_TAPE, 16>};
-  for (const void *f : fns) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)limit);
+      (const void *)k_plane<SEQ_TAPE, 16>, (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>};
+  for (const void *f : fns) {
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, f) != hipSuccess ||
+        hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(limit - fa.sharedSizeBytes)) !=
+            hipSuccess) {
+      (void)hipGetLastError();
+      return fail(RAMGPU_EHIP, "cannot raise the LDS limit of a kernel to %zu bytes", limit);
+    }
+  }
   return RAMGPU_OK;
 }
 
@@ -437,6 +446,8 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   if (plane_lds_bytes(ne, npa) > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS (%zu bytes)", ne * npa,
                         c->lds_limit));
+  if (radial_lds_bytes(nr, ne, npa) + 1024 > c->lds_limit)
+    return cleanup(fail(RAMGPU_EINVAL, "NR=%d: the radial sweep's face terms do not fit the device LDS", nr));
   // gfx950: the LDS of a CU is the largest a block may use
   c->plane = plane_shape(ne, npa, plane_lds_bytes(ne, npa), c->lds_limit);
   if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));

@@ -368,37 +368,100 @@ __global__ void __launch_bounds__(1024) k_ghost(Launch L) {
 // ---------------------------------------------------------------------------
 // DRIFTR, src/ModRamDrift.f90:139-191: one thread per radial line (J,K,L),
 // consecutive lanes on consecutive (L,K) positions.  The thread walks I with
-// a register window F(I-1..I+2) of old values (F(I+3) loaded one step
-// ahead), forms face I, and updates cell I from faces I and I-1.
+// a register window F(I-1..I+2) of old values, forms face I, and updates
+// cell I from faces I and I-1.  F(I+3) comes from a ring of registers loaded
+// four to eight steps ahead, and the face terms (q1, rij, rsi) of the
+// block's lines are staged in LDS first, so a step waits on no memory
+// latency.
 // MODE 0: sweep; 1: sweep and record the CFL maximum; 2: the last sweep of
 // ram_run: also the SUMRC partial sum of src/ModRamRun.f90:246-255 (slot 5),
 // then F2(:,:,NT)=F2(:,:,1) and the magnetopause mask (:186,197-201).  In
 // mode 2 the J=NT line is not swept: the seam overwrites it with line J=1,
 // whose thread stores both.
+
+// Source of element e of the block's face-term tables (layout of k_r's LDS:
+// q1 of npair (J,L) pairs [pair][I], rij of nj J values [j][I][4], rsi
+// [I][2], the magnetopause flag rij(I,NT).w [I]).
+__device__ __forceinline__ gptr<const double> radial_term_src(const Tab &t, int e, int npair, int nj, int pair0,
+                                                              int j0) {
+  const int NR = t.nr, NT = t.nt;
+  const int nq = npair * NR, nr4 = nj * NR * 4;
+  if (e < nq) {
+    const int pp = e / NR, i = e - pp * NR, pair = pair0 + pp;
+    const int jj = pair / t.npa, l = pair - jj * t.npa;
+    return as_global(t.q1) + ((size_t)l * NR + i) * NT + jj;
+  }
+  const gptr<const double> rij = as_global((const double *)t.rij);
+  e -= nq;
+  if (e < nr4) {
+    const int jl = e / (4 * NR), rem = e - jl * 4 * NR, i = rem >> 2;
+    return rij + ((size_t)i * NT + j0 + jl) * 4 + (rem & 3);
+  }
+  e -= nr4;
+  if (e < 2 * NR) return as_global((const double *)t.rsi) + e;
+  e -= 2 * NR;
+  return rij + ((size_t)e * NT + NT - 1) * 4 + 3;
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
   __shared__ double red[kLineThreads / 64];
+  extern __shared__ double sterm[];  // the block's face terms (radial_lds_bytes)
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int s = u.s;
     const int NR = t.nr, NT = t.nt, NE = t.ne;
     const int P = NE * t.npa;
-    const int line = u.b * kLineThreads + threadIdx.x;
+    const int line0 = u.b * kLineThreads;
+    const int line = line0 + threadIdx.x;
     const int j = fast_div(line, P, t.rp);
     const bool act = line < NT * P && !(MODE == 2 && j == NT - 1);
+    // face terms of the block's lines: (J,L) pairs pair0.., J values j0..
+    const int lastl = min(line0 + kLineThreads, NT * P) - 1;
+    const int pair0 = line0 / NE, npair = lastl / NE - pair0 + 1;
+    const int j0 = line0 / P, nj = lastl / P - j0 + 1;
+    double *const qt = sterm;                      // [npair][NR]
+    double *const rt = qt + (size_t)npair * NR;    // [nj][NR][4]
+    double *const rst = rt + (size_t)nj * NR * 4;  // [NR][2]
+    double *const mgt = rst + 2 * NR;              // [NR]
+    {
+      const int ntot = (npair + 4 * nj + 3) * NR;
+      for (int e0 = threadIdx.x; e0 < ntot; e0 += 4 * kLineThreads) {
+        double v[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int e = e0 + m * kLineThreads;
+          v[m] = (e < ntot) ? *radial_term_src(t, e, npair, nj, pair0, j0) : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (e0 + m * kLineThreads < ntot) sterm[e0 + m * kLineThreads] = v[m];
+      }
+    }
+    __syncthreads();
     double cmax = 0.0, acc = 0.0;
     if (act) {
       const int q = line - j * P;
       const int l = fast_div(q, NE, t.rne), k = q - l * NE;
-      const double p4k = t.p4[s * NE + k];
       const double beta = t.betalim;
       const size_t di = (size_t)NT * P;  // stride between I and I+1
       const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
-      const double *fl = launch_src(L, t) + off;
-      double *fo = launch_dst(L, t) + off;
-      double *fs = fo + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
-      const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
+      const gptr<const double> fl = as_global(launch_src(L, t) + off);
+      const gptr<double> fo = as_global(launch_dst(L, t) + off);
+      const gptr<double> fs = fo + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
+      const double *const qrow = qt + (size_t)(j * t.npa + l - pair0) * NR;
+      const double *const rrow = rt + (size_t)(j - j0) * NR * 4;
+#define RTERM(II) make_double4(rrow[4 * (II)], rrow[4 * (II) + 1], rrow[4 * (II) + 2], rrow[4 * (II) + 3])
+#define RSTERM(II) make_double2(rst[2 * (II)], rst[2 * (II) + 1])
+      // F(1..4) and the ring's first F(I+3), I = 2..5, in one batch
+      double fw[4], pa[4], pb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) fw[m] = fl[(size_t)m * di];  // NR >= 4
+#pragma unroll
+      for (int m = 0; m < 4; ++m) pa[m] = (5 + m <= NR) ? fl[(size_t)(4 + m) * di] : 0.0;
+      const double p4k = t.p4[s * NE + k];
+      const bool in = c_radial_of(RTERM(NR - 1), RSTERM(NR - 1), qrow[NR - 1], p4k) < 0;
       double gh1 = 0.0, gh2 = 0.0;
       if (in) {
         gh1 = ghost_value(t, s, j, k, l, t.conf1);
@@ -412,13 +475,14 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
       }
       const bool sum_line = MODE == 2 && k >= 1 && l >= 1 && j <= NT - 2;
       // face 1: FBND(1) = F(2) on inflow lines, 0 otherwise (:156,164)
-      const double f1 = fl[0], f2v = fl[di], f3 = fl[2 * di];
-      double fnext = fl[3 * di];  // F(4); NR >= 4
-      const double c1 = c_radial(t, p4k, 0, j, l);
-      if (MODE == 1 && t.rij[j].w == 0.0) cmax = fmax(cmax, fmax(fabs(c1), 1E-10));
+      const double f1 = fw[0], f2v = fw[1], f3 = fw[2];
+      double fnext = fw[3];  // F(4)
+      const double4 r1 = RTERM(0);
+      const double c1 = c_radial_of(r1, RSTERM(0), qrow[0], p4k);
+      if (MODE == 1 && r1.w == 0.0) cmax = fmax(cmax, fmax(fabs(c1), 1E-10));
       double gprev = c1 * (in ? f2v : 0.0);
       if (MODE == 2) {
-        const double mg1 = t.rij[j].w, mgs = t.rij[NT - 1].w;
+        const double mg1 = r1.w, mgs = mgt[0];
         if (mg1 == 1.0) fo[0] = 1e-31;
         else if (fo != fl) fo[0] = f1;
         if (j == 0) fs[0] = (mgs == 1.0) ? 1e-31 : f1;
@@ -426,55 +490,67 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
         fo[0] = f1;  // cell 1 is not updated
       }
       double wm1 = f1, w0 = f2v, w1 = f3, w2 = fnext;  // F(I-1), F(I), F(I+1), F(I+2) at I=2
-      // the descriptor's table pointers, read once: the stores to F2 below
-      // could alias the descriptor, which would reload them every step
-      const double4 *const rijp = t.rij + j;
-      const double2 *const rsip = t.rsi;
-      const double *const q1p = t.q1 + (size_t)l * NR * NT + j;
-      for (int I = 2; I <= NR; ++I) {
-        const int i = I - 1;
-        fnext = (I + 3 <= NR) ? fl[(size_t)(I + 2) * di] : 0.0;
-        const double4 r = rijp[(size_t)i * NT];
-        const double2 rs = rsip[i];
-        const double x = (q1p[(size_t)i * NT] * p4k) * 0.5;
-        const double ci = r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
-        if (MODE == 1 && r.w == 0.0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));
-        double fb;
-        if (I == NR && !in) {
-          fb = w0;  // FBND(NR) = F(NR) on outflow lines
-        } else {
-          const double fp1 = (I == NR) ? gh1 : w1;
-          double fn, fnm1;
-          if (ci < 0) {
-            fnm1 = fp1;
-            if (I == NR)
-              fn = gh2;
-            else if (I + 2 <= NR)
-              fn = w2;
-            else  // I == NR-1: the ghost F(NR+1)
-              fn = in ? gh1 : stale_ghost(t, s, j, k, l);
-          } else {
-            fn = w0;
-            fnm1 = wm1;
-          }
-          fb = face_value(w0, fp1, fn, fnm1, ci, ci, beta);
-        }
-        const double g = ci * fb;
-        double v = w0 - g + gprev;
-        if (v < 0) v = 1E-15;
-        gprev = g;
-        if (MODE == 2) {
-          if (sum_line) acc += ekk * (v * wek * wml);
-          fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;
-          if (j == 0) fs[(size_t)i * di] = (rijp[(size_t)i * NT + NT - 1].w == 1.0) ? 1e-31 : v;
-        } else {
-          fo[(size_t)i * di] = v;
-        }
-        wm1 = w0;
-        w0 = w1;
-        w1 = w2;
-        w2 = fnext;
+      // step I with F(I+3) = FN (0 beyond NR)
+#define RSTEP(IV, FN)                                                             \
+  {                                                                               \
+    const int I = (IV), i = I - 1;                                                \
+    fnext = (FN);                                                                 \
+    const double4 r = RTERM(i);                                                   \
+    const double ci = c_radial_of(r, RSTERM(i), qrow[i], p4k);                    \
+    if (MODE == 1 && r.w == 0.0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));        \
+    double fb;                                                                    \
+    if (I == NR && !in) {                                                         \
+      fb = w0; /* FBND(NR) = F(NR) on outflow lines */                            \
+    } else {                                                                      \
+      const double fp1 = (I == NR) ? gh1 : w1;                                    \
+      double fn, fnm1;                                                            \
+      if (ci < 0) {                                                               \
+        fnm1 = fp1;                                                               \
+        if (I == NR)                                                              \
+          fn = gh2;                                                               \
+        else if (I + 2 <= NR)                                                     \
+          fn = w2;                                                                \
+        else /* I == NR-1: the ghost F(NR+1) */                                   \
+          fn = in ? gh1 : stale_ghost(t, s, j, k, l);                             \
+      } else {                                                                    \
+        fn = w0;                                                                  \
+        fnm1 = wm1;                                                               \
+      }                                                                           \
+      fb = face_value(w0, fp1, fn, fnm1, ci, ci, beta);                           \
+    }                                                                             \
+    const double g = ci * fb;                                                     \
+    double v = w0 - g + gprev;                                                    \
+    if (v < 0) v = 1E-15;                                                         \
+    gprev = g;                                                                    \
+    if (MODE == 2) {                                                              \
+      if (sum_line) acc += ekk * (v * wek * wml);                                 \
+      fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;                              \
+      if (j == 0) fs[(size_t)i * di] = (mgt[i] == 1.0) ? 1e-31 : v;               \
+    } else {                                                                      \
+      fo[(size_t)i * di] = v;                                                     \
+    }                                                                             \
+    wm1 = w0;                                                                     \
+    w0 = w1;                                                                      \
+    w1 = w2;                                                                      \
+    w2 = fnext;                                                                   \
+  }
+      // steps I0..I0+3 read ring pa, I0+4..I0+7 ring pb; each ring is
+      // reloaded four steps before its next use
+      for (int I0 = 2; I0 <= NR; I0 += 8) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pb[m] = (I0 + 7 + m <= NR) ? fl[(size_t)(I0 + 6 + m) * di] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (I0 + m <= NR) RSTEP(I0 + m, pa[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pa[m] = (I0 + 11 + m <= NR) ? fl[(size_t)(I0 + 10 + m) * di] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (I0 + 4 + m <= NR) RSTEP(I0 + 4 + m, pb[m]);
       }
+#undef RSTEP
+#undef RTERM
+#undef RSTERM
     }
     if (MODE == 1) {
       const double m = block_max(cmax, red);
@@ -485,6 +561,7 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
       block_sum_n<1>(a, red, tot);
       if (threadIdx.x == 0) t.partials[((size_t)5 * t.ns + s) * t.maxblk + u.b] = tot[0];
     }
+    __syncthreads();  // the face terms of this block are read
   }
 }
 

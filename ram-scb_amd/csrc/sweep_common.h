@@ -135,6 +135,25 @@ __device__ __forceinline__ FaceD face_d(double f0, double fp1, double fn, double
   return r;
 }
 
+// Global-memory views of pointers: loads and stores through them are
+// global (not flat) instructions, so a wait for one load does not also wait
+// for LDS traffic or for every earlier flat access.
+template <class T>
+using gptr = __attribute__((address_space(1))) T *;
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T *p) {
+  return (gptr<T>)p;
+}
+// element n of a double4 / double2 table read through a global view
+__device__ __forceinline__ double4 ld_d4(gptr<const double> p, size_t n) {
+  const gptr<const double> q = p + 4 * n;
+  return make_double4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ double2 ld_d2(gptr<const double> p, size_t n) {
+  const gptr<const double> q = p + 2 * n;
+  return make_double2(q[0], q[1]);
+}
+
 // ---------------------------------------------------------------------------
 // Launch units
 struct Unit {
@@ -205,11 +224,13 @@ __device__ __forceinline__ void atomic_max_pos(unsigned long long *slot, double 
 // CGR = CGR3/(FNHS+FNHS')*P4/2./(BNES+BNES')/(RLZ+0.5*MDR)
 // (src/ModRamDrift.f90:140-146).  One function for the sweep, the CFL and the
 // ghost carry, so the inflow/outflow decision is the same everywhere.
-__device__ __forceinline__ double c_radial(const Tab &t, double p4k, int i, int j, int l) {
-  const double4 r = t.rij[(size_t)i * t.nt + j];
-  const double2 rs = t.rsi[i];
-  const double x = (t.q1[((size_t)l * t.nr + i) * t.nt + j] * p4k) * 0.5;
+// from the face's terms: r = rij(I,J), rs = rsi(I), q1 = q1(L,I,J)
+__device__ __forceinline__ double c_radial_of(double4 r, double2 rs, double q1, double p4k) {
+  const double x = (q1 * p4k) * 0.5;
   return r.x + div_rn(div_rn(x, r.y, r.z), rs.x, rs.y);
+}
+__device__ __forceinline__ double c_radial(const Tab &t, double p4k, int i, int j, int l) {
+  return c_radial_of(t.rij[(size_t)i * t.nt + j], t.rsi[i], t.q1[((size_t)l * t.nr + i) * t.nt + j], p4k);
 }
 
 // CDriftE(I,J,K,L) (src/ModRamDrift.f90:337-342) from the per-(I,J,L)
