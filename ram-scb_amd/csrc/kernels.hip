@@ -404,7 +404,7 @@ __device__ __forceinline__ gptr<const double> radial_term_src(const Tab &t, int 
 }
 
 template <int MODE>
-__global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
+__global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
   __shared__ double red[kLineThreads / 64];
   extern __shared__ double sterm[];  // the block's face terms (radial_lds_bytes)
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
@@ -490,49 +490,42 @@ __global__ void __launch_bounds__(kLineThreads) k_r(Launch L) {
         fo[0] = f1;  // cell 1 is not updated
       }
       double wm1 = f1, w0 = f2v, w1 = f3, w2 = fnext;  // F(I-1), F(I), F(I+1), F(I+2) at I=2
-      // step I with F(I+3) = FN (0 beyond NR)
-#define RSTEP(IV, FN)                                                             \
-  {                                                                               \
-    const int I = (IV), i = I - 1;                                                \
-    fnext = (FN);                                                                 \
-    const double4 r = RTERM(i);                                                   \
-    const double ci = c_radial_of(r, RSTERM(i), qrow[i], p4k);                    \
-    if (MODE == 1 && r.w == 0.0) cmax = fmax(cmax, fmax(fabs(ci), 1E-10));        \
-    double fb;                                                                    \
-    if (I == NR && !in) {                                                         \
-      fb = w0; /* FBND(NR) = F(NR) on outflow lines */                            \
-    } else {                                                                      \
-      const double fp1 = (I == NR) ? gh1 : w1;                                    \
-      double fn, fnm1;                                                            \
-      if (ci < 0) {                                                               \
-        fnm1 = fp1;                                                               \
-        if (I == NR)                                                              \
-          fn = gh2;                                                               \
-        else if (I + 2 <= NR)                                                     \
-          fn = w2;                                                                \
-        else /* I == NR-1: the ghost F(NR+1) */                                   \
-          fn = in ? gh1 : stale_ghost(t, s, j, k, l);                             \
-      } else {                                                                    \
-        fn = w0;                                                                  \
-        fnm1 = wm1;                                                               \
-      }                                                                           \
-      fb = face_value(w0, fp1, fn, fnm1, ci, ci, beta);                           \
-    }                                                                             \
-    const double g = ci * fb;                                                     \
-    double v = w0 - g + gprev;                                                    \
-    if (v < 0) v = 1E-15;                                                         \
-    gprev = g;                                                                    \
-    if (MODE == 2) {                                                              \
-      if (sum_line) acc += ekk * (v * wek * wml);                                 \
-      fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;                              \
-      if (j == 0) fs[(size_t)i * di] = (mgt[i] == 1.0) ? 1e-31 : v;               \
-    } else {                                                                      \
-      fo[(size_t)i * di] = v;                                                     \
-    }                                                                             \
-    wm1 = w0;                                                                     \
-    w0 = w1;                                                                      \
-    w1 = w2;                                                                      \
-    w2 = fnext;                                                                   \
+      // F(NR+1) of face NR-1 when it is upwinded from outside: the line's own
+      // ghost on inflow lines, else the one a previous inflow line left
+      // (needed only when C(NR-1) < 0; formed here, once, so the steps have
+      // no branches)
+      double fgh = gh1;
+      if (!in && c_radial_of(RTERM(NR - 2), RSTERM(NR - 2), qrow[NR - 2], p4k) < 0) fgh = stale_ghost(t, s, j, k, l);
+      // step I with F(I+3) = FN (0 beyond NR); selects only, so the compiler
+      // can overlap a step's Courant number with the previous step's limiter
+#define RSTEP(IV, FN)                                                              \
+  {                                                                                \
+    const int I = (IV), i = I - 1;                                                 \
+    fnext = (FN);                                                                  \
+    const double4 r = RTERM(i);                                                    \
+    const double ci = c_radial_of(r, RSTERM(i), qrow[i], p4k);                     \
+    if (MODE == 1) cmax = (r.w == 0.0) ? fmax(cmax, fmax(fabs(ci), 1E-10)) : cmax; \
+    const bool last = I == NR, neg = ci < 0;                                       \
+    const double fp1 = last ? gh1 : w1;                                            \
+    /* upwind side for C<0: ghost F(NR+2) at face NR, F(NR+1) at face NR-1 */      \
+    const double fneg = last ? gh2 : ((I + 2 <= NR) ? w2 : fgh);                   \
+    const double fv = face_value_sel(w0, fp1, neg ? fneg : w0, neg ? fp1 : wm1, ci, ci, beta); \
+    const double fb = (last && !in) ? w0 : fv; /* FBND(NR) = F(NR) on outflow lines */ \
+    const double g = ci * fb;                                                      \
+    double v = w0 - g + gprev;                                                     \
+    if (v < 0) v = 1E-15;                                                          \
+    gprev = g;                                                                     \
+    if (MODE == 2) {                                                               \
+      acc += sum_line ? ekk * (v * wek * wml) : 0.0;                               \
+      fo[(size_t)i * di] = (r.w == 1.0) ? 1e-31 : v;                               \
+      if (j == 0) fs[(size_t)i * di] = (mgt[i] == 1.0) ? 1e-31 : v;                \
+    } else {                                                                       \
+      fo[(size_t)i * di] = v;                                                      \
+    }                                                                              \
+    wm1 = w0;                                                                      \
+    w0 = w1;                                                                       \
+    w1 = w2;                                                                       \
+    w2 = fnext;                                                                    \
   }
       // steps I0..I0+3 read ring pa, I0+4..I0+7 ring pb; each ring is
       // reloaded four steps before its next use
