@@ -593,41 +593,55 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
 // Flux misfit at virtual-satellite points (loss of the adjoint driver):
 //   omega_p = sum_L w[p][L] * F2(S_p,I_p,J_p,K_p,L),
 //   loss_p  = (log(omega_p) - target_p)^2,
-// one thread per point; with lam != NULL also lam += d loss / d F2
+// one 64-thread block per point; with lam != NULL also lam += d loss / d F2
 // (= 2*(log(omega)-target)/omega * w, atomically: points may share cells).
-// pts[p] = {S,I,J,K} 1-based.
+// pts[p] = {S,I,J,K} 1-based.  The products w*F2 are formed by all lanes at
+// once and summed by lane 0 in the order of L (so omega is the sequential
+// sum, whatever the launch shape).
+constexpr int kMisfitLds = 1025;  // NPA <= 1024 products and the adjoint factor
 __device__ inline void misfit_point(int p, const double *f2, int nr, int nt, int ne, int npa, const int *pts,
-                                    const double *w, const double *target, double *loss_out, double *lam) {
+                                    const double *w, const double *target, double *loss_out, double *lam,
+                                    double *sh) {
   const int s = pts[4 * p] - 1, i = pts[4 * p + 1] - 1, j = pts[4 * p + 2] - 1, k = pts[4 * p + 3] - 1;
   const size_t base = ((((size_t)s * nr + i) * nt + j) * npa) * ne + k;  // + l*NE
   const double *wp = w + (size_t)p * npa;
-  double om = 0.0;
-  for (int l = 0; l < npa; ++l) om += wp[l] * f2[base + (size_t)l * ne];
-  const double r = log(om) - target[p];
-  if (loss_out) loss_out[p] = r * r;
-  if (lam) {
-    const double g = 2.0 * r / om;
-    for (int l = 0; l < npa; ++l) atomicAdd(&lam[base + (size_t)l * ne], g * wp[l]);
+  for (int l = threadIdx.x; l < npa; l += blockDim.x) sh[l] = wp[l] * f2[base + (size_t)l * ne];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double om = 0.0;
+    for (int l = 0; l < npa; ++l) om += sh[l];
+    const double r = log(om) - target[p];
+    if (loss_out) loss_out[p] = r * r;
+    sh[npa] = 2.0 * r / om;
   }
+  __syncthreads();
+  if (lam) {
+    const double g = sh[npa];
+    for (int l = threadIdx.x; l < npa; l += blockDim.x) atomicAdd(&lam[base + (size_t)l * ne], g * wp[l]);
+  }
+  __syncthreads();  // sh is reused by the block's next point
 }
 
 __global__ void k_misfit(const double *f2, int nr, int nt, int ne, int npa, int npts, const int *pts,
                          const double *w, const double *target, double *loss_out, double *lam) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < npts) misfit_point(p, f2, nr, nt, ne, npa, pts, w, target, loss_out, lam);
+  __shared__ double sh[kMisfitLds];
+  for (int p = blockIdx.x; p < npts; p += gridDim.x)
+    misfit_point(p, f2, nr, nt, ne, npa, pts, w, target, loss_out, lam, sh);
 }
 
-// The same for every problem of an ensemble launch (one unit per problem):
-// the state after step `step` is the launch's source (a tape slot or F2);
-// losses go to Tab::mloss[step], and with adjoint != 0 dL/dF2 to Tab::lam.
+// The same for every problem of an ensemble launch (one unit per problem
+// along x, points along y): the state after step `step` is the launch's
+// source (a tape slot or F2); losses go to Tab::mloss[step], and with
+// adjoint != 0 dL/dF2 to Tab::lam.
 __global__ void k_misfit_ens(Launch L, int step, int adjoint) {
+  __shared__ double sh[kMisfitLds];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const size_t np = (size_t)t.mnpts;
-    for (int p = threadIdx.x; p < t.mnpts; p += blockDim.x)
+    for (int p = blockIdx.y; p < t.mnpts; p += gridDim.y)
       misfit_point(p, launch_src(L, t), t.nr, t.nt, t.ne, t.npa, t.mpts, t.mw, t.mtarget + (size_t)step * np,
-                   adjoint ? nullptr : t.mloss + (size_t)step * np, adjoint ? t.lam : nullptr);
+                   adjoint ? nullptr : t.mloss + (size_t)step * np, adjoint ? t.lam : nullptr, sh);
   }
 }
 

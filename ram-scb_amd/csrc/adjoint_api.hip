@@ -312,7 +312,7 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   HIP_TRY(hipMemcpyAsync(dpts, pts, (size_t)4 * npts * sizeof(int), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(dw, w, (size_t)npts * c->npa * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(dtarget, target, (size_t)nsteps * npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  const int mb = (npts + 63) / 64;
+  const int mb = std::max(npts, 1);  // one block per point
   auto misfit = [&](const double *state, int step, double *lout, double *lm) {
     hipLaunchKernelGGL(k_misfit, dim3(mb), dim3(64), 0, c->stream, state, c->nr, c->nt, c->ne, c->npa, npts,
                        (const int *)dpts, (const double *)dw, (const double *)(dtarget + (size_t)step * npts), lout,

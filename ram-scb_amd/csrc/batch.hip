@@ -48,6 +48,7 @@ struct ramgpu_batch {
   Tab *d_tabs = nullptr;
   Plan plan[BK_N];
   size_t plane_lds = 0, line_lds = 0;
+  int mpts_max = 0;  // most satellite points of a member (ensemble adjoint)
   PlaneShape plane{64, 8};  // the largest block and row bound any member needs
   int sum_ns = 0;
   std::vector<int> bs_off;  // first species of member m in a schedule row
@@ -213,7 +214,7 @@ void enq_copy(ramgpu_batch *b, int ssl, int dsl) {
 
 void enq_misfit(ramgpu_batch *b, int ssl, int step, int adjoint) {
   const Launch L = slot_launch(b, BK_MEM, ssl, 0);
-  hipLaunchKernelGGL(k_misfit_ens, dim3(L.nv), dim3(64), 0, b->stream, L, step, adjoint);
+  hipLaunchKernelGGL(k_misfit_ens, dim3(L.nv, std::max(b->mpts_max, 1)), dim3(64), 0, b->stream, L, step, adjoint);
 }
 
 // One step with every operator's input on the tape group starting at slot g
@@ -538,6 +539,7 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
     c->t.mtarget = (const double *)mt;
     c->t.mloss = (double *)ml;
     c->t.mnpts = (int)np;
+    b->mpts_max = std::max(b->mpts_max, (int)np);
     fin[mi] = (double *)f;
     fin_e[mi] = (double *)fe;
   }
