@@ -65,115 +65,150 @@ __device__ __forceinline__ void line_route(const LFace &f, double gb, double (&a
 // FINAL: the last sweep of ram_run, whose output then went through
 // F2(:,:,NT)=F2(:,:,1) and the magnetopause mask; its J=NT line output was
 // discarded (zero adjoint), and line J=1 also receives line NT's adjoint.
+// The output adjoint of cell I in the final sweep: 0 where the mask set the
+// cell, plus line J=NT's adjoint folded into line J=1 (the seam copy).
+__device__ __forceinline__ double ybar_of(double a, double b, bool masked, bool fold) {
+  double y = masked ? 0.0 : a;
+  if (fold) y += b;
+  return y;
+}
+
 template <int FINAL>
-__global__ void __launch_bounds__(kLineThreads) k_r_bwd(Launch L, const double *xin, double *adj, double *bbar) {
+__global__ void __launch_bounds__(kLineThreads, FINAL ? 3 : 4) k_r_bwd(Launch L, const double *xin, double *adj, double *bbar) {
+  extern __shared__ double sterm[];  // the block's face terms (radial_lds_bytes)
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int s = u.s;
     const int NR = t.nr, NT = t.nt, NE = t.ne;
     const int P = NE * t.npa;
-    const int line = u.b * kLineThreads + threadIdx.x;
+    const int line0 = u.b * kLineThreads;
+    const int line = line0 + threadIdx.x;
     const int j = fast_div(line, P, t.rp);
-    if (line >= NT * P || (FINAL && j == NT - 1)) continue;
-    const int q = line - j * P;
-    const int l = fast_div(q, NE, t.rne), k = q - l * NE;
-    const double p4k = t.p4[s * NE + k];
-    const double beta = t.betalim;
-    const size_t di = (size_t)NT * P;
-    const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
-    const double *xl = (xin ? xin : launch_src(L, t)) + off;
-    double *al = (adj ? adj : t.lam) + off;
-    double *as = al + (size_t)(NT - 1 - j) * P;  // line J=NT
-    double *bb = bbar ? bbar : (t.bbar ? t.bbar + (size_t)L.bstep * t.ns * NT * NE * t.npa : nullptr);
-    const bool in = c_radial(t, p4k, NR - 1, j, l) < 0;
-    double gh1 = 0.0, gh2 = 0.0;
-    if (in) {
-      gh1 = ghost_value(t, s, j, k, l, t.conf1);
-      gh2 = ghost_value(t, s, j, k, l, t.conf2);
-    }
-    auto X = [&](int I) -> double { return xl[(size_t)(I - 1) * di]; };
-    auto ybar = [&](int I) -> double {
-      double y = al[(size_t)(I - 1) * di];
-      if (FINAL) {
-        if (t.rij[(size_t)(I - 1) * NT + j].w == 1.0) y = 0.0;
-        if (j == 0 && t.rij[(size_t)(I - 1) * NT + NT - 1].w != 1.0) y += as[(size_t)(I - 1) * di];
-      }
-      return y;
-    };
-    // face NR, then the window X(NR-2..NR+1) for the first step
-    double w0 = X(NR - 2), w1 = X(NR - 1), w2 = X(NR), w3 = gh1;
-    LFace fc;
-    {
-      const double c = c_radial(t, p4k, NR - 1, j, l);
-      if (in) fc = line_face(c, w1, w2, gh1, gh2, beta);
-      else fc = {c * w2, c, 0.0, 0.0, 0.0, false};  // FBND(NR) = F(NR)
-    }
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    double wnext = 0.0, g1 = 0.0, g2 = 0.0;
-    double xpf = X(NR - 3), ypf = ybar(NR);
-    // the descriptor's table pointers, read once (the adjoint's stores could
-    // alias the descriptor); CDriftR as c_radial forms it
-    const double4 *const rijp = t.rij + j;
-    const double2 *const rsip = t.rsi;
-    const double *const q1p = t.q1 + (size_t)l * NR * NT + j;
-    for (int I = NR; I >= 2; --I) {
-      const double yI = ypf, xn = xpf;
-      if (I >= 5) xpf = X(I - 4);
-      if (I >= 3) ypf = ybar(I - 1);
-      // face I-1 from the window X(I-2..I+1)
-      LFace fp;
-      const double4 rr = rijp[(size_t)(I - 2) * NT];
-      const double2 rs = rsip[I - 2];
-      const double c = rr.x + div_rn(div_rn((q1p[(size_t)(I - 2) * NT] * p4k) * 0.5, rr.y, rr.z), rs.x, rs.y);
-      if (I >= 3) {
-        double x2 = w3;
-        if (I == NR && !in && c < 0) x2 = stale_ghost(t, s, j, k, l);  // F(NR+1) read by face NR-1
-        fp = line_face(c, w0, w1, w2, x2, beta);
-      } else {
-        fp = {c * (in ? w2 : 0.0), 0.0, in ? c : 0.0, 0.0, 0.0, false};  // FBND(1)
-      }
-      const double v = w2 - fc.G + fp.G;
-      const double w = (v < 0) ? 0.0 : yI;
-      acc[1] += w;  // cell I's own term
-      line_route(fc, -w + wnext, acc);
-      if (I + 2 <= NR) al[(size_t)(I + 1) * di] = acc[3];
-      else if (I + 2 == NR + 1) g1 = acc[3];
-      else g2 = acc[3];
-      acc[3] = acc[2];
-      acc[2] = acc[1];
-      acc[1] = acc[0];
-      acc[0] = 0.0;
-      fc = fp;
-      wnext = w;
-      w3 = w2;
-      w2 = w1;
-      w1 = w0;
-      w0 = xn;
-    }
-    // face 1 (window: cells 0..3); cell 1 is not updated: its output adjoint passes
-    line_route(fc, wnext, acc);
-    acc[1] += ybar(1);
-    al[2 * di] = acc[3];
-    al[di] = acc[2];
-    al[0] = acc[1];
-    if (FINAL && j == 0)  // line NT's adjoint has been folded into line 1
-      for (int I = 1; I <= NR; ++I) as[(size_t)(I - 1) * di] = 0.0;
-    if (bb) {
+    const RadialTerms rt = stage_radial_terms(t, line0, sterm);
+    if (line < NT * P && !(FINAL && j == NT - 1)) {
+      const int q = line - j * P;
+      const int l = fast_div(q, NE, t.rne), k = q - l * NE;
+      const double p4k = t.p4[s * NE + k];
+      const double beta = t.betalim;
+      const size_t di = (size_t)NT * P;
+      const size_t off = (size_t)s * f2_species_stride(t) + (size_t)j * P + q;
+      const gptr<const double> xl = as_global((xin ? xin : launch_src(L, t)) + off);
+      const gptr<double> al = as_global((adj ? adj : t.lam) + off);
+      const gptr<double> as = al + (size_t)(NT - 1 - j) * P;  // line J=NT
+      double *bb = bbar ? bbar : (t.bbar ? t.bbar + (size_t)L.bstep * t.ns * NT * NE * t.npa : nullptr);
+      const double *const qrow = rt.q_row(j, l, t.npa), *const rrow = rt.r_row(j), *const rst = rt.rs, *const mgt = rt.mg;
+#define RTERM(II) make_double4(rrow[4 * (II)], rrow[4 * (II) + 1], rrow[4 * (II) + 2], rrow[4 * (II) + 3])
+#define RSTERM(II) make_double2(rst[2 * (II)], rst[2 * (II) + 1])
+#define CFACE(II) c_radial_of(RTERM(II), RSTERM(II), qrow[II], p4k)  // CDriftR at face II+1
+#define MASKED(II) (FINAL && rrow[4 * ((II)-1) + 3] == 1.0)
+#define FOLD(II) (FINAL && j == 0 && mgt[(II)-1] != 1.0)
+      const double cnr = CFACE(NR - 1);
+      const bool in = cnr < 0;
+      double gh1 = 0.0, gh2 = 0.0;
       if (in) {
-        if (g1 != 0.0 || g2 != 0.0) {
-          const long own = s + (long)t.ns * (j + (long)NT * (k + (long)NE * l));
-          atomicAdd(&bb[own], g1 * ghost_dfgeos(t, s, j, l, t.conf1) + g2 * ghost_dfgeos(t, s, j, l, t.conf2));
-        }
-      } else if (g1 != 0.0) {  // the stale ghost: F(NR+1) of an earlier inflow line
-        const long src = stale_ghost_source(t, s, j, k, l);
-        if (src >= 0) {  // src = s + NS*(J' + NT*(K' + NE*L'))
-          const long r = src / t.ns;
-          const int jj = (int)(r % NT), ll = (int)(r / ((long)NT * NE));
-          atomicAdd(&bb[src], g1 * ghost_dfgeos(t, s, jj, ll, t.conf1));
+        gh1 = ghost_value(t, s, j, k, l, t.conf1);
+        gh2 = ghost_value(t, s, j, k, l, t.conf2);
+      }
+      // face NR, then the window X(NR-2..NR+1) for the first step
+      double w0 = xl[(size_t)(NR - 3) * di], w1 = xl[(size_t)(NR - 2) * di], w2 = xl[(size_t)(NR - 1) * di],
+             w3 = gh1;
+      LFace fc;
+      if (in) fc = line_face(cnr, w1, w2, gh1, gh2, beta);
+      else fc = {cnr * w2, cnr, 0.0, 0.0, 0.0, false};  // FBND(NR) = F(NR)
+      // F(NR+1) read by face NR-1 in the first step: the stale ghost when it
+      // is upwinded from outside an outflow line (formed once, here)
+      const double x2nr = (!in && CFACE(NR - 2) < 0) ? stale_ghost(t, s, j, k, l) : w3;
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      double wnext = 0.0, g1 = 0.0, g2 = 0.0;
+      // step I reads X(max(I-3,1)) (the cell entering the window), the
+      // output adjoint of cell I and, for the seam, line J=NT's; they come
+      // from two rings of registers loaded four to eight steps ahead
+#define BLOAD(IV, XR, YR, SR)                                             \
+  {                                                                       \
+    const int I_ = (IV);                                                  \
+    if (I_ >= 2) {                                                        \
+      XR = xl[(size_t)(max(I_ - 3, 1) - 1) * di];                         \
+      YR = al[(size_t)(I_ - 1) * di];                                     \
+      SR = (FINAL && j == 0) ? as[(size_t)(I_ - 1) * di] : 0.0;           \
+    }                                                                     \
+  }
+#define BSTEP(IV, XN, YA, SB)                                             \
+  {                                                                       \
+    const int I = (IV);                                                   \
+    const double yI = ybar_of(YA, SB, MASKED(I), FOLD(I)), xn = (XN);     \
+    LFace fp;                                                             \
+    const double c = CFACE(I - 2); /* face I-1 from the window X(I-2..I+1) */ \
+    if (I >= 3) {                                                         \
+      fp = line_face(c, w0, w1, w2, (I == NR) ? x2nr : w3, beta);         \
+    } else {                                                              \
+      fp = {c * (in ? w2 : 0.0), 0.0, in ? c : 0.0, 0.0, 0.0, false}; /* FBND(1) */ \
+    }                                                                     \
+    const double v = w2 - fc.G + fp.G;                                    \
+    const double w = (v < 0) ? 0.0 : yI;                                  \
+    acc[1] += w; /* cell I's own term */                                  \
+    line_route(fc, -w + wnext, acc);                                      \
+    if (I + 2 <= NR) al[(size_t)(I + 1) * di] = acc[3];                   \
+    else if (I + 2 == NR + 1) g1 = acc[3];                                \
+    else g2 = acc[3];                                                     \
+    acc[3] = acc[2];                                                      \
+    acc[2] = acc[1];                                                      \
+    acc[1] = acc[0];                                                      \
+    acc[0] = 0.0;                                                         \
+    fc = fp;                                                              \
+    wnext = w;                                                            \
+    w3 = w2;                                                              \
+    w2 = w1;                                                              \
+    w1 = w0;                                                              \
+    w0 = xn;                                                              \
+  }
+      double xa[4], ya[4], sa[4], xb[4], yb[4], sb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) BLOAD(NR - m, xa[m], ya[m], sa[m]);
+      for (int I0 = NR; I0 >= 2; I0 -= 8) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) BLOAD(I0 - 4 - m, xb[m], yb[m], sb[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (I0 - m >= 2) BSTEP(I0 - m, xa[m], ya[m], sa[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) BLOAD(I0 - 8 - m, xa[m], ya[m], sa[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (I0 - 4 - m >= 2) BSTEP(I0 - 4 - m, xb[m], yb[m], sb[m]);
+      }
+      // face 1 (window: cells 0..3); cell 1 is not updated: its output adjoint passes
+      line_route(fc, wnext, acc);
+      acc[1] += ybar_of(al[0], (FINAL && j == 0) ? as[0] : 0.0, MASKED(1), FOLD(1));
+      al[2 * di] = acc[3];
+      al[di] = acc[2];
+      al[0] = acc[1];
+#undef BSTEP
+#undef BLOAD
+#undef FOLD
+#undef MASKED
+#undef CFACE
+#undef RSTERM
+#undef RTERM
+      if (FINAL && j == 0)  // line NT's adjoint has been folded into line 1
+        for (int I = 1; I <= NR; ++I) as[(size_t)(I - 1) * di] = 0.0;
+      if (bb) {
+        if (in) {
+          if (g1 != 0.0 || g2 != 0.0) {
+            const long own = s + (long)t.ns * (j + (long)NT * (k + (long)NE * l));
+            atomicAdd(&bb[own], g1 * ghost_dfgeos(t, s, j, l, t.conf1) + g2 * ghost_dfgeos(t, s, j, l, t.conf2));
+          }
+        } else if (g1 != 0.0) {  // the stale ghost: F(NR+1) of an earlier inflow line
+          const long src = stale_ghost_source(t, s, j, k, l);
+          if (src >= 0) {  // src = s + NS*(J' + NT*(K' + NE*L'))
+            const long r = src / t.ns;
+            const int jj = (int)(r % NT), ll = (int)(r / ((long)NT * NE));
+            atomicAdd(&bb[src], g1 * ghost_dfgeos(t, s, jj, ll, t.conf1));
+          }
         }
       }
     }
+    __syncthreads();  // the block's face terms are read
   }
 }
 

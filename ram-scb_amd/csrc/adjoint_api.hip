@@ -122,9 +122,11 @@ int bwd_r(ramgpu_ctx *c, int final_sweep, const double *x, double *adj, double *
   const Launch L = launch_of(c, s0, nsp, blocks_r(c->t));
   tbeg(c, RAMGPU_KERNEL_R_BWD);
   if (final_sweep)
-    hipLaunchKernelGGL(k_r_bwd<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj, bbar);
+    hipLaunchKernelGGL(k_r_bwd<1>, dim3(L.nv), dim3(kLineThreads), radial_lds_bytes(c->nr, c->ne, c->npa), c->stream,
+                       L, x, adj, bbar);
   else
-    hipLaunchKernelGGL(k_r_bwd<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj, bbar);
+    hipLaunchKernelGGL(k_r_bwd<0>, dim3(L.nv), dim3(kLineThreads), radial_lds_bytes(c->nr, c->ne, c->npa), c->stream,
+                       L, x, adj, bbar);
   tend(c);
   return launch_check("k_r_bwd");
 }

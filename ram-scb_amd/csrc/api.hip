@@ -349,7 +349,8 @@ int set_plane_lds_limit(size_t limit) {
       (const void *)k_plane<SEQ_LOSS, 8>, (const void *)k_plane<SEQ_E, 16>,   (const void *)k_plane<SEQ_MU, 16>,
       (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
       (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>, (const void *)k_plane<SEQ_TAPE, 8>,
-      (const void *)k_plane<SEQ_TAPE, 16>, (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>};
+      (const void *)k_plane<SEQ_TAPE, 16>, (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
+      (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>};
   for (const void *f : fns) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(&fa, f) != hipSuccess ||

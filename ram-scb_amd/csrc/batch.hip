@@ -239,7 +239,8 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
 void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_threads) {
   hipStream_t st = b->stream;
   Launch L = slot_launch(b, BK_R, g + T_R2 + 1, 0, step);
-  hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L, nullptr, nullptr,
+                     nullptr);
   L = slot_launch(b, BK_P, g + T_P2 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_PLANE, g + T_E1 + 1, 0);
@@ -247,7 +248,8 @@ void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_thre
   L = slot_launch(b, BK_P, g + T_P1 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), 0, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_R, g + T_R1 + 1, 0, step);
-  hipLaunchKernelGGL(k_r_bwd<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), 0, st, L, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_r_bwd<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L, nullptr, nullptr,
+                     nullptr);
 }
 
 int ensure_graph(ramgpu_batch *b, bool sched) {

@@ -379,30 +379,6 @@ __global__ void __launch_bounds__(1024) k_ghost(Launch L) {
 // mode 2 the J=NT line is not swept: the seam overwrites it with line J=1,
 // whose thread stores both.
 
-// Source of element e of the block's face-term tables (layout of k_r's LDS:
-// q1 of npair (J,L) pairs [pair][I], rij of nj J values [j][I][4], rsi
-// [I][2], the magnetopause flag rij(I,NT).w [I]).
-__device__ __forceinline__ gptr<const double> radial_term_src(const Tab &t, int e, int npair, int nj, int pair0,
-                                                              int j0) {
-  const int NR = t.nr, NT = t.nt;
-  const int nq = npair * NR, nr4 = nj * NR * 4;
-  if (e < nq) {
-    const int pp = e / NR, i = e - pp * NR, pair = pair0 + pp;
-    const int jj = pair / t.npa, l = pair - jj * t.npa;
-    return as_global(t.q1) + ((size_t)l * NR + i) * NT + jj;
-  }
-  const gptr<const double> rij = as_global((const double *)t.rij);
-  e -= nq;
-  if (e < nr4) {
-    const int jl = e / (4 * NR), rem = e - jl * 4 * NR, i = rem >> 2;
-    return rij + ((size_t)i * NT + j0 + jl) * 4 + (rem & 3);
-  }
-  e -= nr4;
-  if (e < 2 * NR) return as_global((const double *)t.rsi) + e;
-  e -= 2 * NR;
-  return rij + ((size_t)e * NT + NT - 1) * 4 + 3;
-}
-
 template <int MODE>
 __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
   __shared__ double red[kLineThreads / 64];
@@ -417,29 +393,7 @@ __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
     const int line = line0 + threadIdx.x;
     const int j = fast_div(line, P, t.rp);
     const bool act = line < NT * P && !(MODE == 2 && j == NT - 1);
-    // face terms of the block's lines: (J,L) pairs pair0.., J values j0..
-    const int lastl = min(line0 + kLineThreads, NT * P) - 1;
-    const int pair0 = line0 / NE, npair = lastl / NE - pair0 + 1;
-    const int j0 = line0 / P, nj = lastl / P - j0 + 1;
-    double *const qt = sterm;                      // [npair][NR]
-    double *const rt = qt + (size_t)npair * NR;    // [nj][NR][4]
-    double *const rst = rt + (size_t)nj * NR * 4;  // [NR][2]
-    double *const mgt = rst + 2 * NR;              // [NR]
-    {
-      const int ntot = (npair + 4 * nj + 3) * NR;
-      for (int e0 = threadIdx.x; e0 < ntot; e0 += 4 * kLineThreads) {
-        double v[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          const int e = e0 + m * kLineThreads;
-          v[m] = (e < ntot) ? *radial_term_src(t, e, npair, nj, pair0, j0) : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          if (e0 + m * kLineThreads < ntot) sterm[e0 + m * kLineThreads] = v[m];
-      }
-    }
-    __syncthreads();
+    const RadialTerms rt = stage_radial_terms(t, line0, sterm);
     double cmax = 0.0, acc = 0.0;
     if (act) {
       const int q = line - j * P;
@@ -450,8 +404,7 @@ __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
       const gptr<const double> fl = as_global(launch_src(L, t) + off);
       const gptr<double> fo = as_global(launch_dst(L, t) + off);
       const gptr<double> fs = fo + (size_t)(NT - 1 - j) * P;  // line J=NT (seam target in mode 2)
-      const double *const qrow = qt + (size_t)(j * t.npa + l - pair0) * NR;
-      const double *const rrow = rt + (size_t)(j - j0) * NR * 4;
+      const double *const qrow = rt.q_row(j, l, t.npa), *const rrow = rt.r_row(j), *const rst = rt.rs, *const mgt = rt.mg;
 #define RTERM(II) make_double4(rrow[4 * (II)], rrow[4 * (II) + 1], rrow[4 * (II) + 2], rrow[4 * (II) + 3])
 #define RSTERM(II) make_double2(rst[2 * (II)], rst[2 * (II) + 1])
       // F(1..4) and the ring's first F(I+3), I = 2..5, in one batch

@@ -171,6 +171,69 @@ __device__ __forceinline__ Unit unit_of(const Launch &L, int vb) {
 }
 
 // ---------------------------------------------------------------------------
+// Face terms of the radial lines of one block (kLineThreads consecutive
+// lines from line0), staged in LDS by the whole block: q1 of the block's
+// (J,L) pairs [pair][I], rij of its J values [j][I][4], rsi [I][2] and the
+// flags rij(I,NT).w [I] (radial_lds_bytes).  The sweeps then read their face
+// terms without memory latency.
+// source of element e of the tables
+__device__ __forceinline__ gptr<const double> radial_term_src(const Tab &t, int e, int npair, int nj, int pair0,
+                                                              int j0) {
+  const int NR = t.nr, NT = t.nt;
+  const int nq = npair * NR, nr4 = nj * NR * 4;
+  if (e < nq) {
+    const int pp = e / NR, i = e - pp * NR, pair = pair0 + pp;
+    const int jj = pair / t.npa, l = pair - jj * t.npa;
+    return as_global(t.q1) + ((size_t)l * NR + i) * NT + jj;
+  }
+  const gptr<const double> rij = as_global((const double *)t.rij);
+  e -= nq;
+  if (e < nr4) {
+    const int jl = e / (4 * NR), rem = e - jl * 4 * NR, i = rem >> 2;
+    return rij + ((size_t)i * NT + j0 + jl) * 4 + (rem & 3);
+  }
+  e -= nr4;
+  if (e < 2 * NR) return as_global((const double *)t.rsi) + e;
+  e -= 2 * NR;
+  return rij + ((size_t)e * NT + NT - 1) * 4 + 3;
+}
+
+struct RadialTerms {
+  const double *q, *r, *rs, *mg;
+  int pair0, j0, nr;
+  __device__ const double *q_row(int j, int l, int npa) const { return q + (size_t)(j * npa + l - pair0) * nr; }
+  __device__ const double *r_row(int j) const { return r + (size_t)(j - j0) * nr * 4; }
+};
+__device__ inline RadialTerms stage_radial_terms(const Tab &t, int line0, double *sterm) {
+  const int NR = t.nr, NE = t.ne, P = NE * t.npa;
+  const int lastl = min(line0 + kLineThreads, t.nt * P) - 1;
+  const int pair0 = line0 / NE, npair = lastl / NE - pair0 + 1;
+  const int j0 = line0 / P, nj = lastl / P - j0 + 1;
+  const int ntot = (npair + 4 * nj + 3) * NR;
+  for (int e0 = threadIdx.x; e0 < ntot; e0 += 4 * kLineThreads) {
+    double v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = e0 + m * kLineThreads;
+      v[m] = (e < ntot) ? *radial_term_src(t, e, npair, nj, pair0, j0) : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      if (e0 + m * kLineThreads < ntot) sterm[e0 + m * kLineThreads] = v[m];
+  }
+  __syncthreads();
+  RadialTerms rt;
+  rt.q = sterm;
+  rt.r = rt.q + (size_t)npair * NR;
+  rt.rs = rt.r + (size_t)nj * NR * 4;
+  rt.mg = rt.rs + 2 * NR;
+  rt.pair0 = pair0;
+  rt.j0 = j0;
+  rt.nr = NR;
+  return rt;
+}
+
+// ---------------------------------------------------------------------------
 // Block reductions (fixed order: deterministic for a given launch shape).
 template <int N>
 __device__ inline void block_sum_n(double (&v)[N], double *red, double (&out)[N]) {
