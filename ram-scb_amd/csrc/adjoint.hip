@@ -221,98 +221,131 @@ __global__ void __launch_bounds__(kLineThreads, FINAL ? 3 : 4) k_r_bwd(Launch L,
 // only feeds stencils.  The adjoints of cells 1..3, which the wrapped faces
 // reach first, collect in wr[] and join the window's at the end.
 __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *xin, double *adj) {
+  extern __shared__ double sterm[];  // the block's face terms (azim_lds_bytes)
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int s = u.s;
     const int NR = t.nr, NT = t.nt, NE = t.ne;
     const int P = NE * t.npa;
-    const int line = u.b * kLineThreads + threadIdx.x;
-    if (line >= (NR - 1) * P) continue;
-    const int i1 = fast_div(line, P, t.rp);
-    const int q = line - i1 * P;
-    const int i = i1 + 1;
-    const int l = fast_div(q, NE, t.rne), k = q - l * NE;
-    const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
-    const double beta = t.betalim, omet = t.omet;
-    const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;
-    const double *xl = (xin ? xin : launch_src(L, t)) + off;
-    double *al = (adj ? adj : t.lam) + off;
-    const double4 *pij = t.pij + (size_t)i * NT;
-    const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
-    auto X = [&](int J) -> double { return xl[(size_t)(J - 1) * P]; };
-    auto cface = [&](int J) -> double {
-      const double4 a = pij[J - 1];
-      const double4 g = pl[J - 1];
-      const double uu = div_rn(p2k * g.x, g.y, g.z);
-      const double num = a.x - uu - a.y;
-      return div_rn(num, a.z, a.w) + omet;
-    };
-    const double x1 = X(1), x2 = X(2), x3 = X(3);
-    const double yb1 = al[0];
-    // face NT (stencil NT-1, NT, 2, 3) and face 2
-    double w0 = X(NT - 2), w1 = X(NT - 1), w2 = X(NT);
-    const LFace fN = line_face(cface(NT), w1, w2, x2, x3, beta);
-    const LFace f2 = line_face(cface(2), x1, x2, x3, X(4), beta);
-    const double v2 = x2 - f2.G + fN.G;
-    const double w2c = (v2 < 0) ? 0.0 : al[P];
-    double wr[4] = {0.0, 0.0, 0.0, 0.0};  // cells 1..3 (index 0 unused)
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    LFace fc = fN;
-    double wnext = w2c;  // face NT feeds cell 2 as face 1
-    double w3 = x2;      // window X(J-2..J+1), X(NT+1) = X(2)
-    double xpf = X(NT - 3), ypf = al[(size_t)(NT - 1) * P] + yb1;
-    for (int J = NT; J >= 3; --J) {
-      const double yJ = ypf, xn = xpf;
-      if (J >= 5) xpf = X(J - 4);
-      if (J >= 4) ypf = al[(size_t)(J - 2) * P];
-      LFace fp;
-      if (J - 1 == 2) {
-        fp = f2;
-      } else {
-        const double c = cface(J - 1);
-        // face NT-1 with C<0 reads F(N-1) = F(1), not F(NT)
-        if (J == NT && c < 0) {
-          const FaceD D = face_d(w1, w2, x2, x1, c, c, beta);
-          fp = {c * D.fb, c * D.d0, c * D.d1, c * D.dn, c * D.dm, true};
-        } else {
-          fp = line_face(c, w0, w1, w2, w3, beta);
-        }
+    const int line0 = u.b * kLineThreads;
+    const int line = line0 + threadIdx.x;
+    const int nring = (NR - 1) * P;
+    if (line0 >= nring) continue;  // (uniform over the block)
+    const AzimTerms at = stage_azim_terms(t, line0, min(line0 + kLineThreads, nring) - 1, sterm);
+    if (line < nring) {
+      const int i1 = fast_div(line, P, t.rp);
+      const int q = line - i1 * P;
+      const int i = i1 + 1;
+      const int l = fast_div(q, NE, t.rne), k = q - l * NE;
+      const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
+      const double beta = t.betalim, omet = t.omet;
+      const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;
+      const gptr<const double> xl = as_global((xin ? xin : launch_src(L, t)) + off);
+      const gptr<double> al = as_global((adj ? adj : t.lam) + off);
+      const double *const ap = at.pij_row(i1), *const gp = at.pl_row(i1, l, t.npa);
+#define CFACE(JJ) (div_rn(ap[4 * ((JJ)-1)] - div_rn(p2k * gp[4 * ((JJ)-1)], gp[4 * ((JJ)-1) + 1], gp[4 * ((JJ)-1) + 2]) - \
+                          ap[4 * ((JJ)-1) + 1],                                                                  \
+                          ap[4 * ((JJ)-1) + 2], ap[4 * ((JJ)-1) + 3]) +                                          \
+                   omet)
+#define XJ(JJ) xl[(size_t)((JJ)-1) * P]
+      const double x1 = XJ(1), x2 = XJ(2), x3 = XJ(3), x4 = XJ(4);
+      const double yb1 = al[0], yb2 = al[P];
+      // face NT (stencil NT-1, NT, 2, 3) and face 2
+      double w0 = XJ(NT - 2), w1 = XJ(NT - 1), w2 = XJ(NT);
+      const LFace fN = line_face(CFACE(NT), w1, w2, x2, x3, beta);
+      const LFace f2 = line_face(CFACE(2), x1, x2, x3, x4, beta);
+      const double v2 = x2 - f2.G + fN.G;
+      const double w2c = (v2 < 0) ? 0.0 : yb2;
+      double wr[4] = {0.0, 0.0, 0.0, 0.0};  // cells 1..3 (index 0 unused)
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      LFace fc = fN;
+      double wnext = w2c;  // face NT feeds cell 2 as face 1
+      double w3 = x2;      // window X(J-2..J+1), X(NT+1) = X(2)
+      // step J reads X(max(J-3,1)) (the cell entering the window) and the
+      // output adjoint of cell J (plus cell 1's at J = NT: F2(I,1) =
+      // F2(I,NT)), from two rings of registers loaded four to eight steps
+      // ahead
+#define PLOAD(JV, XR, YR)                        \
+  {                                              \
+    const int J_ = (JV);                         \
+    if (J_ >= 3) {                               \
+      XR = xl[(size_t)(max(J_ - 3, 1) - 1) * P]; \
+      YR = al[(size_t)(J_ - 1) * P];             \
+    }                                            \
+  }
+#define PSTEPB(JV, XN, YR)                                                          \
+  {                                                                                 \
+    const int J = (JV);                                                             \
+    const double yJ = (J == NT) ? (YR) + yb1 : (YR), xn = (XN);                     \
+    LFace fp;                                                                       \
+    if (J - 1 == 2) {                                                               \
+      fp = f2;                                                                      \
+    } else {                                                                        \
+      const double c = CFACE(J - 1);                                                \
+      /* face NT-1 with C<0 reads F(N-1) = F(1), not F(NT) */                       \
+      if (J == NT && c < 0) {                                                       \
+        const FaceD D = face_d(w1, w2, x2, x1, c, c, beta);                         \
+        fp = {c * D.fb, c * D.d0, c * D.d1, c * D.dn, c * D.dm, true};             \
+      } else {                                                                      \
+        fp = line_face(c, w0, w1, w2, w3, beta);                                    \
+      }                                                                             \
+    }                                                                               \
+    const double v = w2 - fc.G + fp.G;                                              \
+    const double w = (v < 0) ? 0.0 : yJ;                                            \
+    acc[1] += w;                                                                    \
+    const double gb = -w + wnext;                                                   \
+    if (J == NT - 1 && fc.neg) { /* face NT-1, C<0: F(N) -> cell 2 (as NT+1), F(N-1) -> cell 1 */ \
+      acc[1] += fc.p0 * gb;                                                         \
+      acc[2] += fc.p1 * gb;                                                         \
+      acc[3] += fc.pn * gb;                                                         \
+      wr[1] += fc.pm * gb;                                                          \
+    } else {                                                                        \
+      line_route(fc, gb, acc);                                                      \
+    }                                                                               \
+    if (J + 2 <= NT) al[(size_t)(J + 1) * P] = acc[3];                              \
+    else if (J + 2 == NT + 1) wr[2] += acc[3];                                      \
+    else wr[3] += acc[3];                                                           \
+    acc[3] = acc[2];                                                                \
+    acc[2] = acc[1];                                                                \
+    acc[1] = acc[0];                                                                \
+    acc[0] = 0.0;                                                                   \
+    fc = fp;                                                                        \
+    wnext = w;                                                                      \
+    w3 = w2;                                                                        \
+    w2 = w1;                                                                        \
+    w1 = w0;                                                                        \
+    w0 = xn;                                                                        \
+  }
+      double xa[4], ya[4], xb[4], yb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) PLOAD(NT - m, xa[m], ya[m]);
+      for (int J0 = NT; J0 >= 3; J0 -= 8) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) PLOAD(J0 - 4 - m, xb[m], yb[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (J0 - m >= 3) PSTEPB(J0 - m, xa[m], ya[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) PLOAD(J0 - 8 - m, xa[m], ya[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (J0 - 4 - m >= 3) PSTEPB(J0 - 4 - m, xb[m], yb[m]);
       }
-      const double v = w2 - fc.G + fp.G;
-      const double w = (v < 0) ? 0.0 : yJ;
-      acc[1] += w;
-      const double gb = -w + wnext;
-      if (J == NT - 1 && fc.neg) {  // face NT-1 with C<0: F(N) -> cell 2 (as NT+1), F(N-1) -> cell 1
-        acc[1] += fc.p0 * gb;
-        acc[2] += fc.p1 * gb;
-        acc[3] += fc.pn * gb;
-        wr[1] += fc.pm * gb;
-      } else {
-        line_route(fc, gb, acc);
+#undef PSTEPB
+#undef PLOAD
+#undef XJ
+#undef CFACE
+      {  // J = 2
+        acc[1] += w2c;
+        line_route(f2, -w2c + wnext, acc);
+        al[3 * (size_t)P] = acc[3];  // cell 4 (NT >= 4)
       }
-      if (J + 2 <= NT) al[(size_t)(J + 1) * P] = acc[3];
-      else if (J + 2 == NT + 1) wr[2] += acc[3];
-      else wr[3] += acc[3];
-      acc[3] = acc[2];
-      acc[2] = acc[1];
-      acc[1] = acc[0];
-      acc[0] = 0.0;
-      fc = fp;
-      wnext = w;
-      w3 = w2;
-      w2 = w1;
-      w1 = w0;
-      w0 = xn;
+      al[0] = wr[1] + acc[0];
+      al[P] = wr[2] + acc[1];
+      al[2 * (size_t)P] = wr[3] + acc[2];
     }
-    {  // J = 2
-      acc[1] += w2c;
-      line_route(f2, -w2c + wnext, acc);
-      al[3 * (size_t)P] = acc[3];  // cell 4 (NT >= 4)
-    }
-    al[0] = wr[1] + acc[0];
-    al[P] = wr[2] + acc[1];
-    al[2 * (size_t)P] = wr[3] + acc[2];
+    __syncthreads();  // the block's face terms are read
   }
 }
 

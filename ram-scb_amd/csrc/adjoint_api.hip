@@ -134,7 +134,8 @@ int bwd_r(ramgpu_ctx *c, int final_sweep, const double *x, double *adj, double *
 int bwd_p(ramgpu_ctx *c, const double *x, double *adj, int s0, int nsp) {
   const Launch L = launch_of(c, s0, nsp, blocks_p(c->t));
   tbeg(c, RAMGPU_KERNEL_P_BWD);
-  hipLaunchKernelGGL(k_p_bwd, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L, x, adj);
+  hipLaunchKernelGGL(k_p_bwd, dim3(L.nv), dim3(kLineThreads), azim_lds_bytes(c->nt, c->ne, c->npa), c->stream, L, x,
+                     adj);
   tend(c);
   return launch_check("k_p_bwd");
 }

@@ -217,9 +217,9 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src, double *ds
   L.dst = dst;
   tbeg(c, RAMGPU_KERNEL_P);
   if (rec)
-    hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+    hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), azim_lds_bytes(c->nt, c->ne, c->npa), c->stream, L);
   else
-    hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), 0, c->stream, L);
+    hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), azim_lds_bytes(c->nt, c->ne, c->npa), c->stream, L);
   tend(c);
   return launch_check("k_p");
 }
@@ -350,7 +350,8 @@ int set_plane_lds_limit(size_t limit) {
       (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
       (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>, (const void *)k_plane<SEQ_TAPE, 8>,
       (const void *)k_plane<SEQ_TAPE, 16>, (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
-      (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>};
+      (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>, (const void *)k_p<0>, (const void *)k_p<1>,
+      (const void *)k_p_bwd};
   for (const void *f : fns) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(&fa, f) != hipSuccess ||
@@ -449,6 +450,8 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
                         c->lds_limit));
   if (radial_lds_bytes(nr, ne, npa) + 1024 > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NR=%d: the radial sweep's face terms do not fit the device LDS", nr));
+  if (azim_lds_bytes(nt, ne, npa) + 1024 > c->lds_limit)
+    return cleanup(fail(RAMGPU_EINVAL, "NT=%d: the azimuthal sweep's face terms do not fit the device LDS", nt));
   // gfx950: the LDS of a CU is the largest a block may use
   c->plane = plane_shape(ne, npa, plane_lds_bytes(ne, npa), c->lds_limit);
   if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));

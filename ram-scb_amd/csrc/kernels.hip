@@ -520,22 +520,27 @@ __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
 template <int MODE>
 __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
   __shared__ double red[kLineThreads / 64];
+  extern __shared__ double sterm[];  // the block's face terms (azim_lds_bytes)
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int s = u.s;
     const int NR = t.nr, NT = t.nt, NE = t.ne;
     const int P = NE * t.npa;
-    const int line = u.b * kLineThreads + threadIdx.x;
+    const int line0 = u.b * kLineThreads;
+    const int line = line0 + threadIdx.x;
+    const int nring = (NR - 1) * P;
+    AzimTerms at = {};
+    if (line0 < nring) at = stage_azim_terms(t, line0, min(line0 + kLineThreads, nring) - 1, sterm);
     double cmax = 0.0;
-    if (line >= (NR - 1) * P && out_of_place(L)) {
+    if (line >= nring && out_of_place(L)) {
       // out of place: the I=1 rows are not swept; extra lines copy them
-      const int x = line - (NR - 1) * P;
+      const int x = line - nring;
       if (x < NT * P) {
         const size_t off = (size_t)s * f2_species_stride(t) + x;
         launch_dst(L, t)[off] = launch_src(L, t)[off];
       }
-    } else if (line < (NR - 1) * P) {
+    } else if (line < nring) {
       const int i1 = fast_div(line, P, t.rp);
       const int q = line - i1 * P;
       const int i = i1 + 1;
@@ -543,57 +548,68 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
       const double p2k = t.p2[((size_t)s * NR + i) * NE + k];
       const double beta = t.betalim, omet = t.omet;
       const size_t off = (size_t)s * f2_species_stride(t) + (size_t)i * NT * P + q;  // F(J) at off+(J-1)*P
-      const double *fl = launch_src(L, t) + off;
-      double *fo = launch_dst(L, t) + off;
-      const double4 *pij = t.pij + (size_t)i * NT;
-      const double4 *pl = t.pl + ((size_t)l * NR + i) * NT;
+      const gptr<const double> fl = as_global(launch_src(L, t) + off);
+      const gptr<double> fo = as_global(launch_dst(L, t) + off);
+      const double *const ap = at.pij_row(i1), *const gp = at.pl_row(i1, l, t.npa);
       // Courant number of face J (src/ModRamDrift.f90:236-239)
 #define CAZIM(JJ, OUT, MG)                                                 \
   {                                                                        \
-    const double4 a_ = pij[(JJ)-1];                                        \
-    const double4 g_ = pl[(JJ)-1];                                         \
-    const double u_ = div_rn(p2k * g_.x, g_.y, g_.z);                      \
-    const double num_ = a_.x - u_ - a_.y;                                  \
-    OUT = div_rn(num_, a_.z, a_.w) + omet;                                 \
-    MG = g_.w;                                                             \
+    const double *a_ = ap + 4 * ((JJ)-1), *g_ = gp + 4 * ((JJ)-1);         \
+    const double u_ = div_rn(p2k * g_[0], g_[1], g_[2]);                   \
+    const double num_ = a_[0] - u_ - a_[1];                                \
+    OUT = div_rn(num_, a_[2], a_[3]) + omet;                               \
+    MG = g_[3];                                                            \
   }
+      // F(1..4), F(NT-1), F(NT) and the ring's first F(J+3), J = 2..5
       const double F1 = fl[0], F2o = fl[P], F3 = fl[2 * (size_t)P];
       const double FNm1 = fl[(size_t)(NT - 2) * P], FN = fl[(size_t)(NT - 1) * P];
+      double wm1 = F1, w0 = F2o, w1 = F3, w2 = fl[3 * (size_t)P];  // F(J-1..J+2) at J=2; NT >= 4
+      double pa[4], pb[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) pa[m] = (5 + m <= NT) ? fl[(size_t)(4 + m) * P] : 0.0;
       double cN, mgN;
       CAZIM(NT, cN, mgN);
       if (MODE == 1 && mgN == 0.0) cmax = fmax(cmax, fmax(fabs(cN), 1E-10));
       // face NT: J1 = 2; upwind N = NT+2 -> 3 (:250-251)
       const double gN = cN * face_value(FN, F2o, (cN < 0) ? F3 : FN, (cN < 0) ? F2o : FNm1, cN, cN, beta);
       double gprev = gN;
-      double wm1 = F1, w0 = F2o, w1 = F3, w2 = fl[3 * (size_t)P];  // F(J-1..J+2) at J=2; NT >= 4
-      for (int J = 2; J <= NT - 1; ++J) {
-        const double fnext = (J + 3 <= NT) ? fl[(size_t)(J + 2) * P] : 0.0;
-        double cp, mg;
-        CAZIM(J, cp, mg);
-        if (MODE == 1 && mg == 0.0) cmax = fmax(cmax, fmax(fabs(cp), 1E-10));
-        double fn, fnm1;
-        if (cp < 0) {
-          if (J + 2 <= NT) {
-            fn = w2;
-            fnm1 = w1;
-          } else {  // J = NT-1: N = NT+1 -> 2, N-1 = 1
-            fn = F2o;
-            fnm1 = F1;
-          }
-        } else {
-          fn = w0;
-          fnm1 = wm1;
-        }
-        const double g = cp * face_value(w0, w1, fn, fnm1, cp, cp, beta);
-        double v = w0 - g + gprev;
-        if (v < 0) v = 1E-15;
-        fo[(size_t)(J - 1) * P] = v;
-        gprev = g;
-        wm1 = w0;
-        w0 = w1;
-        w1 = w2;
-        w2 = fnext;
+      // step J with F(J+3) = FN_ (0 beyond NT); selects only
+#define PSTEP(JV, FN_)                                                          \
+  {                                                                             \
+    const int J = (JV);                                                         \
+    const double fnext = (FN_);                                                 \
+    double cp, mg;                                                              \
+    CAZIM(J, cp, mg);                                                           \
+    if (MODE == 1) cmax = (mg == 0.0) ? fmax(cmax, fmax(fabs(cp), 1E-10)) : cmax; \
+    /* C<0: N = J+2, or at J = NT-1 N = NT+1 -> 2, N-1 = 1 */                   \
+    const bool neg = cp < 0, inr = J + 2 <= NT;                                 \
+    const double fn = neg ? (inr ? w2 : F2o) : w0;                              \
+    const double fnm1 = neg ? (inr ? w1 : F1) : wm1;                            \
+    const double g = cp * face_value_sel(w0, w1, fn, fnm1, cp, cp, beta);      \
+    double v = w0 - g + gprev;                                                  \
+    if (v < 0) v = 1E-15;                                                       \
+    fo[(size_t)(J - 1) * P] = v;                                                \
+    gprev = g;                                                                  \
+    wm1 = w0;                                                                   \
+    w0 = w1;                                                                    \
+    w1 = w2;                                                                    \
+    w2 = fnext;                                                                 \
+  }
+      // steps J0..J0+3 read ring pa, J0+4..J0+7 ring pb; each ring is
+      // reloaded four steps before its next use
+      for (int J0 = 2; J0 <= NT - 1; J0 += 8) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pb[m] = (J0 + 7 + m <= NT) ? fl[(size_t)(J0 + 6 + m) * P] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (J0 + m <= NT - 1) PSTEP(J0 + m, pa[m]);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) pa[m] = (J0 + 11 + m <= NT) ? fl[(size_t)(J0 + 10 + m) * P] : 0.0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+          if (J0 + 4 + m <= NT - 1) PSTEP(J0 + 4 + m, pb[m]);
       }
+#undef PSTEP
 #undef CAZIM
       double v = FN - gN + gprev;
       if (v < 0) v = 1E-15;
@@ -604,6 +620,7 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
       const double m = block_max(cmax, red);
       if (threadIdx.x == 0 && m > 0) atomic_max_pos(&t.cflmax[(size_t)s * (2 + NE + t.npa) + 1], m);
     }
+    __syncthreads();  // the block's face terms are read
   }
 }
 

@@ -158,6 +158,12 @@ inline __host__ __device__ int blocks_p(const Tab &t) {
 }
 inline __host__ __device__ int blocks_kl(const Tab &t) { return (t.nr - 1) * t.nt; }
 
+// LDS bytes of the azimuthal sweep: pl of a block's (I,L) pairs and pij of
+// its I values
+inline __host__ __device__ size_t azim_lds_bytes(int nt, int ne, int npa) {
+  const int npair = (kLineThreads - 1) / ne + 2, ni = (kLineThreads - 1) / (ne * npa) + 2;
+  return (size_t)(npair + ni) * nt * 4 * sizeof(double);
+}
 // LDS bytes of the radial sweep: the face terms of a block's lines, q1 of
 // its (J,L) pairs, rij of its J values, rsi and the flags rij(I,NT).w
 inline __host__ __device__ size_t radial_lds_bytes(int nr, int ne, int npa) {

@@ -198,6 +198,51 @@ __device__ __forceinline__ gptr<const double> radial_term_src(const Tab &t, int 
   return rij + ((size_t)e * NT + NT - 1) * 4 + 3;
 }
 
+// Face terms of the azimuthal rings of one block (lines line0..lastl, line
+// = (I-2)*NE*NPA + (L-1)*NE + K-1): pl of the block's (I,L) pairs [pair][J][4]
+// and pij of its I values [i][J][4] (azim_lds_bytes), staged by the whole
+// block.
+struct AzimTerms {
+  const double *pl, *pij;
+  int pair0, i0, nt;
+  __device__ const double *pl_row(int i1, int l, int npa) const { return pl + (size_t)(i1 * npa + l - pair0) * nt * 4; }
+  __device__ const double *pij_row(int i1) const { return pij + (size_t)(i1 - i0) * nt * 4; }
+};
+__device__ inline AzimTerms stage_azim_terms(const Tab &t, int line0, int lastl, double *sterm) {
+  const int NR = t.nr, NT = t.nt, NE = t.ne, NPA = t.npa, P = NE * NPA;
+  const int pair0 = line0 / NE, npair = lastl / NE - pair0 + 1;
+  const int i0 = line0 / P, ni = lastl / P - i0 + 1;
+  const int n4 = NT * 4, ntot = (npair + ni) * n4;
+  const gptr<const double> plg = as_global((const double *)t.pl), pijg = as_global((const double *)t.pij);
+  for (int e0 = threadIdx.x; e0 < ntot; e0 += 4 * blockDim.x) {
+    double v[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const int e = e0 + m * blockDim.x;
+      v[m] = 0.0;
+      if (e < npair * n4) {
+        const int pp = e / n4, rem = e - pp * n4, pair = pair0 + pp;
+        const int i1 = pair / NPA, l = pair - i1 * NPA;
+        v[m] = plg[((size_t)l * NR + i1 + 1) * n4 + rem];
+      } else if (e < ntot) {
+        const int f = e - npair * n4, ii = f / n4, rem = f - ii * n4;
+        v[m] = pijg[(size_t)(i0 + ii + 1) * n4 + rem];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+      if (e0 + m * blockDim.x < ntot) sterm[e0 + m * blockDim.x] = v[m];
+  }
+  __syncthreads();
+  AzimTerms at;
+  at.pl = sterm;
+  at.pij = sterm + (size_t)npair * n4;
+  at.pair0 = pair0;
+  at.i0 = i0;
+  at.nt = NT;
+  return at;
+}
+
 struct RadialTerms {
   const double *q, *r, *rs, *mg;
   int pair0, j0, nr;
