@@ -159,19 +159,21 @@ int bwd_plane(ramgpu_ctx *c, int seq, const double *x, double *adj, int s0, int 
 // next slot, so that the tape holds the input of each operator.  The
 // kernels are those of the fused step with the same arithmetic (the plane
 // kernel also storing its phase inputs): the state, the SUMRC bookkeeping
-// and the CFL limits are those of step_once.
-int taped_step(ramgpu_ctx *c, double *tape, double *y) {
+// and the CFL limits are those of step_once.  With bookkeeping false (a
+// segment recomputed for its tape, whose SUMRC state and CFL limits are
+// discarded) only F2 and the tape are formed.
+int taped_step(ramgpu_ctx *c, double *tape, double *y, bool bookkeeping = true) {
   int rc;
   if ((rc = prepare(c))) return rc;
-  const int rec = c->cfl_valid ? 0 : 1, ns = c->ns;
+  const int rec = (c->cfl_valid || !bookkeeping) ? 0 : 1, ns = c->ns;
   const size_t n = nf2(c);
   auto x = [&](int slot) { return tape + (size_t)slot * n; };
   if ((rc = run_r(c, rec, 0, ns, x(T_R1), x(T_P1))) || (rc = run_p(c, rec, 0, ns, x(T_P1), x(T_E1))) ||
       (rc = run_plane(c, SEQ_TAPE, rec, 0, ns, x(T_E1), x(T_P2))) ||
       (rc = run_p(c, 0, 0, ns, x(T_P2), x(T_R2))) || (rc = run_r(c, 2, 0, ns, x(T_R2), y)) ||
-      (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+      (bookkeeping && (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns))))
     return rc;
-  c->cfl_valid = true;
+  if (bookkeeping) c->cfl_valid = true;
   return RAMGPU_OK;
 }
 
@@ -355,7 +357,7 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
       if ((rc = copy_d2d(c, tape, ckpt + (size_t)g * n, n))) return rc;  // slot T_R1 of the first step
       for (int s = n0; s < n1; ++s)
         if ((rc = set_step_inputs(c, d_kp, d_bs, s)) ||
-            (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, step_out(s, n0, n1))))
+            (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, step_out(s, n0, n1), false)))
           return rc;
     }
     for (int s = n1 - 1; s >= n0; --s) {

@@ -219,20 +219,30 @@ void enq_misfit(ramgpu_batch *b, int ssl, int step, int adjoint) {
 
 // One step with every operator's input on the tape group starting at slot g
 // (0-based), the step's output to dsl_out; the arithmetic of enqueue_step.
-void enq_taped_step(ramgpu_batch *b, int g, int dsl_out) {
+// With bookkeeping false (a segment recomputed for its tape) the CFL limits
+// and the SUMRC chain are not formed: only the states are needed.
+void enq_taped_step(ramgpu_batch *b, int g, int dsl_out, bool bookkeeping = true) {
   hipStream_t st = b->stream;
   Launch L = slot_launch(b, BK_R, g + T_R1 + 1, g + T_P1 + 1);
-  hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
+  if (bookkeeping)
+    hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
+  else
+    hipLaunchKernelGGL(k_r<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
   L = slot_launch(b, BK_POOP, g + T_P1 + 1, g + T_E1 + 1);
-  hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
+  if (bookkeeping)
+    hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
+  else
+    hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
   L = slot_launch(b, BK_PLOOP, g + T_E1 + 1, g + T_P2 + 1);
-  launch_plane(SEQ_TAPE, b->plane, dim3(L.nv), b->plane_lds, st, L, 1);
+  launch_plane(SEQ_TAPE, b->plane, dim3(L.nv), b->plane_lds, st, L, bookkeeping ? 1 : 0);
   L = slot_launch(b, BK_POOP, g + T_P2 + 1, g + T_R2 + 1);
   hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
   L = slot_launch(b, BK_R, g + T_R2 + 1, dsl_out);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
-  const Launch lf = launch_for(b, BK_FIN);
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
+  if (bookkeeping) {
+    const Launch lf = launch_for(b, BK_FIN);
+    hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
+  }
 }
 
 // adjoint of the step taped at group g: Tab::lam in place, Tab::bbar[step]
@@ -604,7 +614,7 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
       enq_copy(b, CK + g + 1, group(n0, n0) + T_R1 + 1);
       for (int s = n0; s < n1; ++s) {
         enq_sched(b, s);
-        enq_taped_step(b, group(s, n0), out_of(s));
+        enq_taped_step(b, group(s, n0), out_of(s), false);
       }
     }
     for (int s = n1 - 1; s >= n0; --s) {
@@ -612,7 +622,7 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
       if (split) {
         enq_copy(b, ST + (s - n0) + 1, TG + T_R1 + 1);
         enq_sched(b, s);
-        enq_taped_step(b, TG, out);  // rewrites the step's output with the same values
+        enq_taped_step(b, TG, out, false);  // rewrites the step's output with the same values
       } else {
         enq_sched(b, s);
       }
