@@ -58,7 +58,12 @@ typedef struct {
 const char *ramgpu_last_error(void);
 
 /* Context holding one (NS,NR,NT,NE,NPA) problem on device `device`.
- * Replaces the allocation in ram_allocate, src/ModRamInit.f90:67-138. */
+ * Replaces the allocation in ram_allocate, src/ModRamInit.f90:67-138.
+ * Sizes: every axis >= 4; NE*NPA <= 8192, one (K,L) plane in one block's
+ * LDS; NT*NE*NPA and NR*NT below 2^22; and NR, NT small enough that the
+ * face terms a sweep block stages in LDS fit the device (160 KB on MI355X:
+ * NR <= 149 and NT <= 62 at NE = NPA = 4, NR <= 1017 and NT <= 462 at
+ * 35 x 72).  Returns RAMGPU_EINVAL otherwise. */
 int ramgpu_create(ramgpu_ctx **ctx, int ns, int nr, int nt, int ne, int npa, int device);
 int ramgpu_destroy(ramgpu_ctx *ctx);
 

@@ -47,7 +47,7 @@ struct ramgpu_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Tab *d_tabs = nullptr;
   Plan plan[BK_N];
-  size_t plane_lds = 0, line_lds = 0, azim_lds = 0;
+  size_t plane_lds = 0, radial_lds = 0, azim_lds = 0;
   int mpts_max = 0;  // most satellite points of a member (ensemble adjoint)
   PlaneShape plane{64, 8};  // the largest block and row bound any member needs
   int sum_ns = 0;
@@ -168,11 +168,11 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
   }
   const Launch lr = launch_for(b, BK_R), lp = launch_for(b, BK_P), lk = launch_for(b, BK_PLANE),
                lf = launch_for(b, BK_FIN);
-  hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, lr);
+  hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, lr);
   hipLaunchKernelGGL(k_p<1>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, lp);
   launch_plane(SEQ_STEP, b->plane, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
   hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, lp);
-  hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, lr);
+  hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, lr);
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
   if (sched) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, b->d_step);
   return launch_check("ensemble step");
@@ -225,9 +225,9 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out, bool bookkeeping = true
   hipStream_t st = b->stream;
   Launch L = slot_launch(b, BK_R, g + T_R1 + 1, g + T_P1 + 1);
   if (bookkeeping)
-    hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
+    hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, L);
   else
-    hipLaunchKernelGGL(k_r<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
+    hipLaunchKernelGGL(k_r<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, L);
   L = slot_launch(b, BK_POOP, g + T_P1 + 1, g + T_E1 + 1);
   if (bookkeeping)
     hipLaunchKernelGGL(k_p<1>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
@@ -238,7 +238,7 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out, bool bookkeeping = true
   L = slot_launch(b, BK_POOP, g + T_P2 + 1, g + T_R2 + 1);
   hipLaunchKernelGGL(k_p<0>, dim3(L.nv), dim3(kLineThreads), b->azim_lds, st, L);
   L = slot_launch(b, BK_R, g + T_R2 + 1, dsl_out);
-  hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L);
+  hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, L);
   if (bookkeeping) {
     const Launch lf = launch_for(b, BK_FIN);
     hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
@@ -249,7 +249,7 @@ void enq_taped_step(ramgpu_batch *b, int g, int dsl_out, bool bookkeeping = true
 void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_threads) {
   hipStream_t st = b->stream;
   Launch L = slot_launch(b, BK_R, g + T_R2 + 1, 0, step);
-  hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L, nullptr, nullptr,
+  hipLaunchKernelGGL(k_r_bwd<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, L, nullptr, nullptr,
                      nullptr);
   L = slot_launch(b, BK_P, g + T_P2 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, L, nullptr, nullptr);
@@ -258,7 +258,7 @@ void enq_step_bwd(ramgpu_batch *b, int g, int step, size_t bwd_lds, int bwd_thre
   L = slot_launch(b, BK_P, g + T_P1 + 1, 0);
   hipLaunchKernelGGL(k_p_bwd, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, L, nullptr, nullptr);
   L = slot_launch(b, BK_R, g + T_R1 + 1, 0, step);
-  hipLaunchKernelGGL(k_r_bwd<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->line_lds, st, L, nullptr, nullptr,
+  hipLaunchKernelGGL(k_r_bwd<0>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, L, nullptr, nullptr,
                      nullptr);
 }
 
@@ -382,7 +382,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->bs_off.push_back(b->sum_ns);
     b->sum_ns += c->ns;
     b->plane_lds = std::max(b->plane_lds, plane_lds_bytes(c->ne, c->npa));
-    b->line_lds = std::max(b->line_lds, radial_lds_bytes(c->nr, c->ne, c->npa));
+    b->radial_lds = std::max(b->radial_lds, radial_lds_bytes(c->nr, c->ne, c->npa));
     b->azim_lds = std::max(b->azim_lds, azim_lds_bytes(c->nt, c->ne, c->npa));
     b->plane.threads = std::max(b->plane.threads, c->plane.threads);
     b->plane.maxc = std::max(b->plane.maxc, c->plane.maxc);
@@ -709,7 +709,7 @@ int ramgpu_batch_autotune(ramgpu_batch *b, int reps, double *ms_best) {
       for (int r = 0; r <= reps; ++r) {  // first launch warms up
         (void)hipEventRecord(e0, b->stream);
         if (kind == BK_R)
-          hipLaunchKernelGGL(k_r<0>, dim3(grid_for(b, kind)), dim3(kLineThreads), b->line_lds, b->stream, L);
+          hipLaunchKernelGGL(k_r<0>, dim3(grid_for(b, kind)), dim3(kLineThreads), b->radial_lds, b->stream, L);
         else if (kind == BK_P)
           hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, kind)), dim3(kLineThreads), b->azim_lds, b->stream, L);
         else
