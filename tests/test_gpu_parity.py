@@ -173,3 +173,20 @@ def test_drift_operators_bit_exact(seed):
             ref = o.dtdrift()[s - 1, ["driftr", "driftp", "drifte", "driftmu"].index(op)]
             assert dt == ref, (op, dt, ref)
     g.close()
+
+
+def test_sweep_lds_limits_at_the_edge():
+    # 4x4 planes: a 256-line block spans 65 (J,L) or (I,L) pairs and 17 J or
+    # I values, so the face terms the sweeps stage in LDS grow fastest with
+    # NR (radial) and NT (azimuthal).  One past each limit is refused; at the
+    # limit (about 159 KB of face terms) the step matches the oracle.
+    import ctypes as C
+    from ramgpu import capi
+    lib = capi.load_library()
+    h = C.c_void_p()
+    assert lib.ramgpu_create(C.byref(h), 1, 8, 63, 4, 4, 0) == 1
+    assert b"NT=63" in lib.ramgpu_last_error()
+    assert lib.ramgpu_create(C.byref(h), 1, 150, 9, 4, 4, 0) == 1
+    assert b"NR=150" in lib.ramgpu_last_error()
+    _run_compare(random_problem(nr=8, nt=62, ne=4, npa=4, seed=21, spiky=True), 2, "8x62x4x4, 2 steps")
+    _run_compare(random_problem(nr=149, nt=9, ne=4, npa=4, seed=22, spiky=True), 2, "149x9x4x4, 2 steps")
