@@ -173,7 +173,9 @@ int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
 #define RAMGPU_KERNEL_R 1        /* radial sweeps (DRIFTR)                    */
 #define RAMGPU_KERNEL_PLANE 2    /* energy / pitch-angle sweeps, losses, SUMRC */
 #define RAMGPU_KERNEL_FINALIZE 3 /* DtDrift* and SUMRC bookkeeping            */
-#define RAMGPU_KERNEL_VS 4       /* Volland-Stern VT from Kp                  */
+#define RAMGPU_KERNEL_VS 4       /* Volland-Stern VT from Kp; a schedule forms
+                                     it with the Courant-number terms, timed
+                                     under RAMGPU_KERNEL_PREPARE            */
 #define RAMGPU_KERNEL_P 5        /* azimuthal sweeps (DRIFTP)                 */
 #define RAMGPU_KERNEL_R_BWD 6    /* adjoint of DRIFTR                         */
 #define RAMGPU_KERNEL_P_BWD 7    /* adjoint of DRIFTP                         */

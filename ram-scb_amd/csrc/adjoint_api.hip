@@ -226,13 +226,7 @@ int op_bwd(ramgpu_ctx *c, int s, int which, double *adj, double *fgeos_adj) {
 // The context's per-step inputs of a schedule (as ramgpu_ram_run_schedule).
 int set_step_inputs(ramgpu_ctx *c, const double *d_kp, const double *d_bs, int n) {
   if (d_kp) {
-    hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
-                       (const double *)c->d_phi, d_kp + n, 0.0, c->nr + 1, c->nt,
-                       (const double *)(d_bs ? d_bs + (size_t)n * c->ns : nullptr), c->d_bscale, c->ns);
-    int rc = launch_check("k_vs");
-    if (rc) return rc;
-    c->efield_dirty = true;
-    if (d_bs) c->ghost_dirty = true;
+    return apply_step_inputs(c, d_kp + n, d_bs ? d_bs + (size_t)n * c->ns : nullptr);
   } else if (d_bs) {
     HIP_TRY(hipMemcpyAsync(c->d_bscale, d_bs + (size_t)n * c->ns, c->ns * sizeof(double), hipMemcpyDeviceToDevice,
                            c->stream));

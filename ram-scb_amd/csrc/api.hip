@@ -171,7 +171,9 @@ int prepare(ramgpu_ctx *c) {
   if (c->efield_dirty) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
     const Launch l = launch_of(c, 0, 1, (c->nr * c->nt + 255) / 256);
-    hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), 0, c->stream, l);
+    const size_t lds = c->step_kp ? (size_t)(c->nr + 1 + c->nt) * sizeof(double) : 0;
+    hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), lds, c->stream, l, c->step_kp, c->step_bs);
+    c->step_kp = c->step_bs = nullptr;
     tend(c);
     if ((rc = launch_check("k_coef_ij"))) return rc;
     c->efield_dirty = false;
@@ -187,6 +189,19 @@ int prepare(ramgpu_ctx *c) {
     c->ghost_dirty = false;
   }
   return RAMGPU_OK;
+}
+
+// One step of a Kp schedule (device pointers to its Kp and, if given, its
+// boundary scales): VT and the boundary scales are formed with the step's
+// Courant-number terms in k_coef_ij, at once.
+int apply_step_inputs(ramgpu_ctx *c, const double *d_kp_n, const double *d_bs_n) {
+  c->step_kp = d_kp_n;
+  c->step_bs = d_bs_n;
+  c->efield_dirty = true;
+  if (d_bs_n) c->ghost_dirty = true;
+  const int rc = prepare(c);
+  c->step_kp = c->step_bs = nullptr;  // never left pointing into the caller's buffers
+  return rc;
 }
 
 // mode: 0 sweep, 1 sweep + CFL record, 2 last sweep of ram_run (SUMRC slot 5,
@@ -810,14 +825,7 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   for (int n = 0; n < nsteps && rc == RAMGPU_OK; ++n) {
-    tbeg(c, RAMGPU_KERNEL_VS);
-    hipLaunchKernelGGL(k_vs, dim3(1), dim3(256), 0, c->stream, c->d_vt, (const double *)c->d_lz,
-                       (const double *)c->d_phi, (const double *)(d_kp + n), 0.0, c->nr + 1, c->nt,
-                       (const double *)(d_bs ? d_bs + (size_t)n * c->ns : nullptr), c->d_bscale, c->ns);
-    tend(c);
-    if ((rc = launch_check("k_vs"))) break;
-    c->efield_dirty = true;
-    if (d_bs) c->ghost_dirty = true;
+    if ((rc = apply_step_inputs(c, d_kp + n, d_bs ? d_bs + (size_t)n * c->ns : nullptr))) break;
     rc = step_once(c);
   }
   (void)hipEventRecord(c->ev1, c->stream);

@@ -162,7 +162,8 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
     hipLaunchKernelGGL(k_sched, dim3(grid_for(b, BK_SCHED)), dim3(256), 0, st, ls, (const double *)b->d_kp,
                        (const double *)b->d_bs, (const int *)b->d_bsoff, (const int *)b->d_step, M, b->sum_ns);
     const Launch lc = launch_for(b, BK_COEFIJ);
-    hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc);
+    hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc, (const double *)nullptr,
+                     (const double *)nullptr);
     const Launch lg = launch_for(b, BK_GHOST);
     hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), 0, st, lg);
   }
@@ -202,7 +203,8 @@ void enq_sched(ramgpu_batch *b, int step) {
                      (const double *)b->d_bs, (const int *)b->d_bsoff, (const int *)b->d_step, (int)b->m.size(),
                      b->sum_ns);
   const Launch lc = launch_for(b, BK_COEFIJ);
-  hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc);
+  hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc, (const double *)nullptr,
+                     (const double *)nullptr);
   const Launch lg = launch_for(b, BK_GHOST);
   hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), 0, st, lg);
 }

@@ -26,7 +26,7 @@ __global__ void k_misfit_ens(Launch L, int step, int adjoint);
 __global__ void k_axpy(double *y, const double *x, double a, size_t n);
 __global__ void k_fill(double *y, double v, size_t n);
 __global__ void k_species(Launch L);
-__global__ void k_coef_ij(Launch L);
+__global__ void k_coef_ij(Launch L, const double *kp, const double *bs);
 __global__ void k_coef(Launch L);
 __global__ void k_ghost(Launch L);
 template <int MODE> __global__ void k_r(Launch L);
@@ -73,6 +73,9 @@ struct ramgpu_ctx {
   bool have_lzphi = false;
   bool tables_dirty = true, ghost_dirty = true;
   bool efield_dirty = true;  // only the VT-dependent per-(I,J) terms are stale
+  // one step of a Kp schedule (device pointers to its Kp and boundary
+  // scales), applied by the next prepare (apply_step_inputs)
+  const double *step_kp = nullptr, *step_bs = nullptr;
   bool cfl_valid = false;  // DtDrift* current for the present tables
   // opt-in per-kernel timing: event pairs around every launch, by kind
   bool timing = false;
@@ -101,6 +104,7 @@ void tend(ramgpu_ctx *c);
 int sync_tab(ramgpu_ctx *c);
 ramgpu::Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb);
 int prepare(ramgpu_ctx *c);
+int apply_step_inputs(ramgpu_ctx *c, const double *d_kp_n, const double *d_bs_n);
 int fill_out(ramgpu_ctx *c, ramgpu_species_out *out);
 int check_species(ramgpu_ctx *c, int s);
 // sweeps of species s0..s0+nsp-1, in place on F2 or from src into dst

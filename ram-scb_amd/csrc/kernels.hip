@@ -123,64 +123,106 @@ __global__ void k_species(Launch L) {
 //   ATMOL   src/ModRamLoss.f90:501         1/FNHS
 // One unit per problem; k_coef_ij: one thread per (I,J), k_coef: one block per
 // (I,J).
-__global__ void k_coef_ij(Launch L) {
+// Volland-Stern potential VT = AVS*(LZ*RE)^2*sin(PHI-PHIOFS) with AVS =
+// 7.05e-6/(1-0.159Kp+0.0093Kp^2)^3/RE (src/ModRamRun.f90:45-54): one
+// expression for every kernel that forms it, so they agree bit for bit.
+__device__ __forceinline__ double vs_avs(double kp) {
+  const double d = 1.0 - 0.159 * kp + 0.0093 * (kp * kp);
+  return 7.05E-6 / (d * d * d) / kRE;
+}
+__device__ __forceinline__ double vs_vt(double avs, double lz, double phi) {
+  const double lre = lz * kRE;
+  return avs * (lre * lre) * sin(phi - 0.0);
+}
+
+// With kp (one step of a Kp schedule) VT is formed here from Kp, where it
+// is read, and stored for later launches: the block stages (LZ*RE)^2 and
+// sin(PHI), which do not depend on Kp, in LDS (coef_ij_lds_bytes), and VT =
+// (AVS*(LZ*RE)^2)*sin(PHI) is the product vs_vt forms.  bs, when given, is
+// the step's boundary-flux scales, copied to the descriptor's.
+__global__ void k_coef_ij(Launch L, const double *kp, const double *bs) {
+  extern __shared__ double svt[];  // with kp: (LZ*RE)^2 [NR+1], then sin(PHI) [NT]
+  const double avs = kp ? vs_avs(*kp) : 0.0;
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    const int n = u.b * blockDim.x + threadIdx.x;
-    if (n >= t.nr * t.nt) continue;
-    const int i = n / t.nt, j = n % t.nt;
-    const int I = i + 1, J = j + 1;
-    const int NT = t.nt;
-    const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
-    const double rlz = t.rlz[i];
-    const size_t ij = (size_t)i * NT + j;
-    const double mg = (double)MGNP(I, J);
-    if (j == 0) {
-      t.rrlz[i] = 1.0 / rlz;
-      const double rs = rlz + 0.5 * mdr;
-      t.rsi[i] = make_double2(rs, 1.0 / rs);
-    }
-    {
-      const int J0 = (J == 1) ? NT - 1 : J - 1;
-      const int J1 = (J == NT) ? 2 : J + 1;
-      const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
-      const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
-      const double dv = GEO2(t.vt, I, J0) + GEO2(t.vt, I + 1, J0) - GEO2(t.vt, I, J1) - GEO2(t.vt, I + 1, J1);
-      const double cr = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
-      t.rij[ij] = make_double4(cr, bsum, 1.0 / bsum, mg);
-    }
-    double pa = 0.0, pce = 0.0, pbs = 1.0;
-    double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
-    const double bij = GEO2(t.bnes, I, J);
-    if (I >= 2) {
-      if (J >= 2) {
-        const int J1 = (J == NT) ? 2 : J + 1;
-        const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
-        pa = (GEO2(t.vt, I + 1, J) + GEO2(t.vt, I + 1, J1) - GEO2(t.vt, I - 1, J) - GEO2(t.vt, I - 1, J1)) * p1;
-        pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
-        pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
+    const int NR1 = t.nr + 1;
+    if (bs && u.b == 0)
+      for (int s = threadIdx.x; s < t.ns; s += blockDim.x) t.bscale[s] = bs[s];
+    if (kp) {
+      for (int x = threadIdx.x; x < NR1 + t.nt; x += blockDim.x) {
+        if (x < NR1) {
+          const double lre = t.lz[x] * kRE;
+          svt[x] = lre * lre;
+        } else {
+          svt[x] = sin(t.phi[x - NR1] - 0.0);
+        }
       }
-      const int J0 = (J == 1) ? NT - 1 : J - 1;
-      const int J2 = (J == NT) ? 2 : J + 1;
-      drd1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-      dpd1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-      // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
-      drm1 = (GEO2(t.eip, I, J) * rlz - (GEO2(t.vt, I, J2) - GEO2(t.vt, I, J0)) / 2.0 / dphi) / bij;
-      dpm1 = kOME * rlz + ((GEO2(t.vt, I + 1, J) - GEO2(t.vt, I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
-      db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
+      __syncthreads();
     }
-    t.pij[ij] = make_double4(pa, pce, pbs, 1.0 / pbs);
-    double *k8 = t.kij + ij * 8;
-    k8[KIJ_DRD1] = drd1;
-    k8[KIJ_DPD1] = dpd1;
-    k8[KIJ_BIJ] = bij;
-    k8[KIJ_RBIJ] = 1.0 / bij;
-    k8[KIJ_DRM1] = drm1;
-    k8[KIJ_DPM1] = dpm1;
-    k8[KIJ_DB2] = db2;
-    k8[KIJ_MG] = mg;
+    const int n = u.b * blockDim.x + threadIdx.x;
+    if (n < t.nr * t.nt) {
+#define VTV(II, JJ) (kp ? (avs * svt[(II)-1]) * svt[NR1 + (JJ)-1] : GEO2(t.vt, II, JJ))
+      const int i = n / t.nt, j = n % t.nt;
+      const int I = i + 1, J = j + 1;
+      const int NT = t.nt;
+      const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
+      const double rlz = t.rlz[i];
+      const size_t ij = (size_t)i * NT + j;
+      const double mg = (double)MGNP(I, J);
+      if (j == 0) {
+        t.rrlz[i] = 1.0 / rlz;
+        const double rs = rlz + 0.5 * mdr;
+        t.rsi[i] = make_double2(rs, 1.0 / rs);
+      }
+      {
+        const int J0 = (J == 1) ? NT - 1 : J - 1;
+        const int J1 = (J == NT) ? 2 : J + 1;
+        const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
+        const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
+        const double dv = VTV(I, J0) + VTV(I + 1, J0) - VTV(I, J1) - VTV(I + 1, J1);
+        const double cr = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
+        t.rij[ij] = make_double4(cr, bsum, 1.0 / bsum, mg);
+      }
+      double pa = 0.0, pce = 0.0, pbs = 1.0;
+      double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
+      const double bij = GEO2(t.bnes, I, J);
+      if (I >= 2) {
+        if (J >= 2) {
+          const int J1 = (J == NT) ? 2 : J + 1;
+          const double p1 = dts / dphi / 2.0 / mdr / rlz;  // :68
+          pa = (VTV(I + 1, J) + VTV(I + 1, J1) - VTV(I - 1, J) - VTV(I - 1, J1)) * p1;
+          pce = (GEO2(t.eir, I, J1) + GEO2(t.eir, I, J)) / rlz * dts / dphi;
+          pbs = GEO2(t.bnes, I, J) + GEO2(t.bnes, I, J1);
+        }
+        const int J0 = (J == 1) ? NT - 1 : J - 1;
+        const int J2 = (J == NT) ? 2 : J + 1;
+        drd1 = (GEO2(t.eip, I, J) * rlz - (VTV(I, J2) - VTV(I, J0)) / 2.0 / dphi) / bij;
+        dpd1 = kOME * rlz + ((VTV(I + 1, J) - VTV(I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+        // DRIFTMU's DRM1/DPM1 (:415-416) are the same expressions as DRD1/DPD1
+        drm1 = (GEO2(t.eip, I, J) * rlz - (VTV(I, J2) - VTV(I, J0)) / 2.0 / dphi) / bij;
+        dpm1 = kOME * rlz + ((VTV(I + 1, J) - VTV(I - 1, J)) / 2.0 / mdr - GEO2(t.eir, I, J)) / bij;
+        db2 = GEO2(t.dbdt, I, J) / 2.0 / bij * rlz;  // :431
+      }
+      t.pij[ij] = make_double4(pa, pce, pbs, 1.0 / pbs);
+      double *k8 = t.kij + ij * 8;
+      k8[KIJ_DRD1] = drd1;
+      k8[KIJ_DPD1] = dpd1;
+      k8[KIJ_BIJ] = bij;
+      k8[KIJ_RBIJ] = 1.0 / bij;
+      k8[KIJ_DRM1] = drm1;
+      k8[KIJ_DPM1] = dpm1;
+      k8[KIJ_DB2] = db2;
+      k8[KIJ_MG] = mg;
+      if (kp) {  // VT of this (I,J), and of I=NR+1 from the last row
+        double *vt = const_cast<double *>(t.vt);
+        GEO2(vt, I, J) = VTV(I, J);
+        if (I == t.nr) GEO2(vt, I + 1, J) = VTV(I + 1, J);
+      }
+    }
+    if (kp) __syncthreads();  // svt is rewritten for the next unit
   }
+#undef VTV
 }
 
 __global__ void k_coef(Launch L) {
@@ -1062,13 +1104,10 @@ __global__ void k_vs(double *vt, const double *lz, const double *phi, const doub
                      double kp_val, int nr1, int nt, const double *bs_src, double *bs_dst, int ns) {
   if (bs_src && blockIdx.x == 0)
     for (int s = threadIdx.x; s < ns; s += blockDim.x) bs_dst[s] = bs_src[s];
-  const double kp = kp_ptr ? *kp_ptr : kp_val;
-  const double d = 1.0 - 0.159 * kp + 0.0093 * (kp * kp);
-  const double avs = 7.05E-6 / (d * d * d) / kRE;
+  const double avs = vs_avs(kp_ptr ? *kp_ptr : kp_val);
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < nr1 * nt; n += gridDim.x * blockDim.x) {
     const int i = n % nr1, j = n / nr1;
-    const double lre = lz[i] * kRE;
-    vt[n] = avs * (lre * lre) * sin(phi[j] - 0.0);
+    vt[n] = vs_vt(avs, lz[i], phi[j]);
   }
 }
 
@@ -1082,15 +1121,12 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
     const Tab &t = *u.t;
     const int m = (int)(u.t - L.tabs);
     const int n = *step;
-    const double kpv = kp[(size_t)n * nmembers + m];
-    const double d = 1.0 - 0.159 * kpv + 0.0093 * (kpv * kpv);
-    const double avs = 7.05E-6 / (d * d * d) / kRE;
+    const double avs = vs_avs(kp[(size_t)n * nmembers + m]);
     double *vt = const_cast<double *>(t.vt);
     const int nr1 = t.nr + 1;
     for (int x = threadIdx.x; x < nr1 * t.nt; x += blockDim.x) {
       const int i = x % nr1, j = x / nr1;
-      const double lre = t.lz[i] * kRE;
-      vt[x] = avs * (lre * lre) * sin(t.phi[j] - 0.0);
+      vt[x] = vs_vt(avs, t.lz[i], t.phi[j]);
     }
     if (bscale)
       for (int s = threadIdx.x; s < t.ns; s += blockDim.x)
