@@ -102,36 +102,31 @@ struct FaceD {
   double fb, d0, d1, dn, dm;
 };
 
+// Every case is formed and the one that applies is selected (no divergent
+// branches in the adjoint sweeps), with the values of the case analysis:
+// the upwind face when |X| <= 1e-27 or R <= 0, else the limited face, its
+// derivatives from the active branch of the limiter (a, b).
 __device__ __forceinline__ FaceD face_d(double f0, double fp1, double fn, double fnm1, double c, double cn,
                                         double beta) {
   const double sg = (c < 0) ? -1.0 : 1.0;
   const double x = fp1 - f0;
   const double fup = 0.5 * (f0 + fp1 - sg * x);
-  FaceD r;
-  r.fb = fup;
-  r.d0 = 0.5 * (1.0 + sg);
-  r.d1 = 0.5 * (1.0 - sg);
-  r.dn = 0.0;
-  r.dm = 0.0;
-  if (fabs(x) <= 1.E-27) return r;
+  const double u0 = 0.5 * (1.0 + sg), u1 = 0.5 * (1.0 - sg);
   const double rr = (fn - fnm1) / x;
-  if (rr <= 0) return r;
+  const bool lin = !(fabs(x) <= 1.E-27) && !(rr <= 0);
   const double A = fmin(beta * rr, 1.0), B = fmin(rr, beta);
   const double lim = fmax(A, B);
-  double a, b;
-  if (A >= B) {
-    if (beta * rr <= 1.0) { a = 0.0; b = beta; }
-    else { a = 1.0; b = 0.0; }
-  } else {
-    if (rr <= beta) { a = 0.0; b = 1.0; }
-    else { a = beta; b = 0.0; }
-  }
+  const bool ab = A >= B;
+  const bool lo = ab ? (beta * rr <= 1.0) : (rr <= beta);
+  const double a = lo ? 0.0 : (ab ? 1.0 : beta);
+  const double b = lo ? (ab ? beta : 1.0) : 0.0;
   const double kap = -0.5 * (cn - sg);
-  r.fb = fup + lim * (kap * x);
-  r.d0 -= kap * a;
-  r.d1 += kap * a;
-  r.dn = kap * b;
-  r.dm = -kap * b;
+  FaceD r;
+  r.fb = lin ? fup + lim * (kap * x) : fup;
+  r.d0 = lin ? u0 - kap * a : u0;
+  r.d1 = lin ? u1 + kap * a : u1;
+  r.dn = lin ? kap * b : 0.0;
+  r.dm = lin ? -kap * b : 0.0;
   return r;
 }
 
