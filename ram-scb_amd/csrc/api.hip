@@ -150,6 +150,13 @@ int prepare(ramgpu_ctx *c) {
   int rc = ready(c);
   if (rc) return rc;
   Tab &t = c->t;
+  // a pending finalize goes with the step's k_coef_ij, or alone before any
+  // table changes
+  if (c->fin_mask >= 0 && (c->tables_dirty || !c->efield_dirty)) {
+    const int m = c->fin_mask;
+    c->fin_mask = -1;
+    if ((rc = run_finalize(c, m, c->fin_mode, 0, c->ns))) return rc;
+  }
   if (c->tables_dirty) {
     t.omet = 7.3E-5 * t.dts / t.dphi;  // OME*DTs/DPHI, src/ModRamDrift.f90:239
     c->tab_dirty = true;
@@ -172,7 +179,15 @@ int prepare(ramgpu_ctx *c) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
     const Launch l = launch_of(c, 0, 1, (c->nr * c->nt + 255) / 256);
     const size_t lds = c->step_kp ? (size_t)(c->nr + 1 + c->nt) * sizeof(double) : 0;
-    hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), lds, c->stream, l, c->step_kp, c->step_bs);
+    if (c->fin_mask >= 0) {  // with the previous step's finalize
+      const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
+      const Launch lf = launch_of(c, 0, c->ns, 1);
+      hipLaunchKernelGGL(k_coef_fin, dim3(lc.nv + lf.nv), dim3(kFinalizeThreads), lds, c->stream, lc, c->step_kp,
+                         c->step_bs, lf, c->fin_mask, c->fin_mode);
+      c->fin_mask = -1;
+    } else {
+      hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), lds, c->stream, l, c->step_kp, c->step_bs);
+    }
     c->step_kp = c->step_bs = nullptr;
     tend(c);
     if ((rc = launch_check("k_coef_ij"))) return rc;
@@ -405,14 +420,22 @@ int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
 // only, so they are recorded in the first sweep of each kind after a table
 // change and reused until the next one.
 int ramgpu_host::step_once(ramgpu_ctx *c) {
+  int cflmask, rc;
+  if ((rc = step_body(c, &cflmask)) || (rc = run_finalize(c, cflmask, 2, 0, c->ns))) return rc;
+  return RAMGPU_OK;
+}
+
+// The sweeps of one step; its finalize (cflmask, SUMRC mode 2) is the
+// caller's.
+int ramgpu_host::step_body(ramgpu_ctx *c, int *cflmask) {
   int rc;
   if ((rc = prepare(c))) return rc;
   const int rec = c->cfl_valid ? 0 : 1;
   const int ns = c->ns;
   if ((rc = run_r(c, rec, 0, ns)) || (rc = run_p(c, rec, 0, ns)) ||
-      (rc = run_plane(c, SEQ_STEP, rec, 0, ns)) || (rc = run_p(c, 0, 0, ns)) ||
-      (rc = run_r(c, 2, 0, ns)) || (rc = run_finalize(c, rec ? 15 : 0, 2, 0, ns)))
+      (rc = run_plane(c, SEQ_STEP, rec, 0, ns)) || (rc = run_p(c, 0, 0, ns)) || (rc = run_r(c, 2, 0, ns)))
     return rc;
+  *cflmask = rec ? 15 : 0;
   c->cfl_valid = true;
   return RAMGPU_OK;
 }
@@ -824,9 +847,19 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  // each step's finalize runs with the next step's first launch (prepare),
+  // the last one before the end of the timed run
   for (int n = 0; n < nsteps && rc == RAMGPU_OK; ++n) {
     if ((rc = apply_step_inputs(c, d_kp + n, d_bs ? d_bs + (size_t)n * c->ns : nullptr))) break;
-    rc = step_once(c);
+    int cflmask = 0;
+    if ((rc = step_body(c, &cflmask))) break;
+    c->fin_mask = cflmask;
+    c->fin_mode = 2;
+  }
+  if (c->fin_mask >= 0) {
+    const int m = c->fin_mask;
+    c->fin_mask = -1;
+    if (rc == RAMGPU_OK) rc = run_finalize(c, m, c->fin_mode, 0, c->ns);
   }
   (void)hipEventRecord(c->ev1, c->stream);
   (void)hipStreamSynchronize(c->stream);

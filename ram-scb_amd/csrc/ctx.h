@@ -27,6 +27,7 @@ __global__ void k_axpy(double *y, const double *x, double a, size_t n);
 __global__ void k_fill(double *y, double v, size_t n);
 __global__ void k_species(Launch L);
 __global__ void k_coef_ij(Launch L, const double *kp, const double *bs);
+__global__ void k_coef_fin(Launch Lc, const double *kp, const double *bs, Launch Lf, int cflmask, int summode);
 __global__ void k_coef(Launch L);
 __global__ void k_ghost(Launch L);
 template <int MODE> __global__ void k_r(Launch L);
@@ -76,6 +77,10 @@ struct ramgpu_ctx {
   // one step of a Kp schedule (device pointers to its Kp and boundary
   // scales), applied by the next prepare (apply_step_inputs)
   const double *step_kp = nullptr, *step_bs = nullptr;
+  // the finalize (CFL mask, SUMRC mode) of the previous step of a Kp
+  // schedule, run with the next step's k_coef_ij; -1 when none (never left
+  // pending outside ramgpu_ram_run_schedule)
+  int fin_mask = -1, fin_mode = 0;
   bool cfl_valid = false;  // DtDrift* current for the present tables
   // opt-in per-kernel timing: event pairs around every launch, by kind
   bool timing = false;
@@ -114,6 +119,7 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
               double *dst = nullptr);
 int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
 int step_once(ramgpu_ctx *c);
+int step_body(ramgpu_ctx *c, int *cflmask);
 // cells per thread the plane kernels need for an NE x NPA plane (8 or 16)
 PlaneShape plane_shape(int ne, int npa, size_t lds, size_t lds_cu);
 // launch k_plane<seq, maxc>

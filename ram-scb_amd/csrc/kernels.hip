@@ -140,10 +140,9 @@ __device__ __forceinline__ double vs_vt(double avs, double lz, double phi) {
 // sin(PHI), which do not depend on Kp, in LDS (coef_ij_lds_bytes), and VT =
 // (AVS*(LZ*RE)^2)*sin(PHI) is the product vs_vt forms.  bs, when given, is
 // the step's boundary-flux scales, copied to the descriptor's.
-__global__ void k_coef_ij(Launch L, const double *kp, const double *bs) {
-  extern __shared__ double svt[];  // with kp: (LZ*RE)^2 [NR+1], then sin(PHI) [NT]
-  const double avs = kp ? vs_avs(*kp) : 0.0;
-  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+__device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, const double *bs, double avs,
+                                    double *svt) {
+  {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int NR1 = t.nr + 1;
@@ -220,9 +219,15 @@ __global__ void k_coef_ij(Launch L, const double *kp, const double *bs) {
         if (I == t.nr) GEO2(vt, I + 1, J) = VTV(I + 1, J);
       }
     }
-    if (kp) __syncthreads();  // svt is rewritten for the next unit
+    if (kp) __syncthreads();  // svt is rewritten for the block's next unit
   }
 #undef VTV
+}
+
+__global__ void k_coef_ij(Launch L, const double *kp, const double *bs) {
+  extern __shared__ double svt[];  // with kp: (LZ*RE)^2 [NR+1], then sin(PHI) [NT]
+  const double avs = kp ? vs_avs(*kp) : 0.0;
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) coef_ij_unit(L, vb, kp, bs, avs, svt);
 }
 
 __global__ void k_coef(Launch L) {
@@ -978,10 +983,10 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
 // from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
 // slots 0..5 (slots 0..4 from the plane blocks, slot 5 from the R blocks).
 // One block per (problem, species).
-__global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cflmask, int summode) {
-  constexpr int NV = 2 + kSumSlots;  // the DtDriftE and DtDriftMu minima, then the SUMRC slot sums
-  __shared__ double red[(kFinalizeThreads / 64) * NV];
-  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
+constexpr int kFinNV = 2 + kSumSlots;  // the DtDriftE and DtDriftMu minima, then the SUMRC slot sums
+__device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int summode, double *red) {
+  constexpr int NV = kFinNV;
+  {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
     const int s = u.s, tid = threadIdx.x, T = blockDim.x;
@@ -1093,7 +1098,28 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cfl
     __syncthreads();  // every read of the maxima is done
     if (cflmask)
       for (int e = tid; e < 2 + NE + NPA; e += T) m[e] = 0ull;
+    __syncthreads();  // red is reused by the block's next unit
   }
+}
+
+__global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cflmask, int summode) {
+  __shared__ double red[(kFinalizeThreads / 64) * kFinNV];
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) finalize_unit(L, vb, cflmask, summode, red);
+}
+
+// One schedule step's start and the previous step's end in one launch (the
+// two are independent; apart, each is a latency-bound launch of its own):
+// blocks below Lc.nv form the step's Courant-number terms as k_coef_ij, the
+// others finish the previous step as k_finalize, one block per species.
+// Blocks of kFinalizeThreads, as k_finalize's, so the sums are its own.
+__global__ void __launch_bounds__(kFinalizeThreads) k_coef_fin(Launch Lc, const double *kp, const double *bs, Launch Lf,
+                                                               int cflmask, int summode) {
+  extern __shared__ double svt[];
+  __shared__ double red[(kFinalizeThreads / 64) * kFinNV];
+  if ((int)blockIdx.x < Lc.nv)
+    coef_ij_unit(Lc, blockIdx.x, kp, bs, kp ? vs_avs(*kp) : 0.0, svt);
+  else
+    finalize_unit(Lf, blockIdx.x - Lc.nv, cflmask, summode, red);
 }
 
 // VT(I,J)=AVS*(LZ(I)*RE)**2*SIN(PHI(J)-PHIOFS), src/ModRamRun.f90:45-54
