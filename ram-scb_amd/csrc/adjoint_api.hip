@@ -324,22 +324,35 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   const int nlast0 = (nseg - 1) * ckpt_every;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   // forward, keeping a checkpoint at the start of every segment; the last
-  // segment is run taped at once, so the backward need not recompute it
-  for (int s = 0; s < nsteps; ++s) {
-    if (s % ckpt_every == 0 && s < nlast0 && (rc = copy_d2d(c, ckpt + (size_t)(s / ckpt_every) * n, c->t.f2, n)))
-      return rc;
-    if ((rc = set_step_inputs(c, d_kp, d_bs, s))) return rc;
-    if (s < nlast0) {
-      if ((rc = step_once(c)) || (rc = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr))) return rc;
-    } else {
-      if (s == nlast0 && (rc = copy_d2d(c, tape_in(s, nlast0), c->t.f2, n))) return rc;
-      double *y = step_out(s, nlast0, nsteps);
-      if ((rc = taped_step(c, tape + (size_t)(s - nlast0) * T_N * n, y)) ||
-          (rc = misfit(y, s, lossv + (size_t)s * npts, nullptr)))
-        return rc;
+  // segment is run taped at once, so the backward need not recompute it.
+  // A plain step's finalize runs with the next step's first launch; none is
+  // left pending when the pass ends, on error too.
+  auto forward = [&]() -> int {
+    int r;
+    for (int s = 0; s < nsteps; ++s) {
+      if (s % ckpt_every == 0 && s < nlast0 && (r = copy_d2d(c, ckpt + (size_t)(s / ckpt_every) * n, c->t.f2, n)))
+        return r;
+      if ((r = set_step_inputs(c, d_kp, d_bs, s))) return r;
+      if (s < nlast0) {
+        int cflmask = 0;
+        if ((r = step_body(c, &cflmask))) return r;
+        c->fin_mask = cflmask;
+        c->fin_mode = 2;
+        if ((r = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr))) return r;
+      } else {
+        if (s == nlast0 && (r = copy_d2d(c, tape_in(s, nlast0), c->t.f2, n))) return r;
+        double *y = step_out(s, nlast0, nsteps);
+        if ((r = taped_step(c, tape + (size_t)(s - nlast0) * T_N * n, y)) ||
+            (r = misfit(y, s, lossv + (size_t)s * npts, nullptr)))
+          return r;
+      }
     }
-  }
-  if ((rc = copy_d2d(c, final_f2, c->t.f2, n)) || (rc = copy_d2d(c, final_e, c->t.energy, (size_t)c->ns * 4)))
+    return RAMGPU_OK;
+  };
+  rc = forward();
+  const int rf = flush_finalize(c);
+  if (rc == RAMGPU_OK) rc = rf;
+  if (rc || (rc = copy_d2d(c, final_f2, c->t.f2, n)) || (rc = copy_d2d(c, final_e, c->t.energy, (size_t)c->ns * 4)))
     return rc;
   // backward, segment by segment: recompute the segment with its tape, then
   // walk its steps in reverse

@@ -206,6 +206,14 @@ int prepare(ramgpu_ctx *c) {
   return RAMGPU_OK;
 }
 
+// The pending finalize of a schedule's last step, alone.
+int flush_finalize(ramgpu_ctx *c) {
+  if (c->fin_mask < 0) return RAMGPU_OK;
+  const int m = c->fin_mask;
+  c->fin_mask = -1;
+  return run_finalize(c, m, c->fin_mode, 0, c->ns);
+}
+
 // One step of a Kp schedule (device pointers to its Kp and, if given, its
 // boundary scales): VT and the boundary scales are formed with the step's
 // Courant-number terms in k_coef_ij, at once.
@@ -856,11 +864,8 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
     c->fin_mask = cflmask;
     c->fin_mode = 2;
   }
-  if (c->fin_mask >= 0) {
-    const int m = c->fin_mask;
-    c->fin_mask = -1;
-    if (rc == RAMGPU_OK) rc = run_finalize(c, m, c->fin_mode, 0, c->ns);
-  }
+  const int rf = flush_finalize(c);
+  if (rc == RAMGPU_OK) rc = rf;
   (void)hipEventRecord(c->ev1, c->stream);
   (void)hipStreamSynchronize(c->stream);
   (void)hipFree(d_kp);

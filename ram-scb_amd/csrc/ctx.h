@@ -78,8 +78,9 @@ struct ramgpu_ctx {
   // scales), applied by the next prepare (apply_step_inputs)
   const double *step_kp = nullptr, *step_bs = nullptr;
   // the finalize (CFL mask, SUMRC mode) of the previous step of a Kp
-  // schedule, run with the next step's k_coef_ij; -1 when none (never left
-  // pending outside ramgpu_ram_run_schedule)
+  // schedule, run with the next step's k_coef_ij (prepare); -1 when none.
+  // Only the schedule loops of ramgpu_ram_run_schedule and
+  // ramgpu_adjoint_run set it, and they flush it before they return.
   int fin_mask = -1, fin_mode = 0;
   bool cfl_valid = false;  // DtDrift* current for the present tables
   // opt-in per-kernel timing: event pairs around every launch, by kind
@@ -110,6 +111,7 @@ int sync_tab(ramgpu_ctx *c);
 ramgpu::Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb);
 int prepare(ramgpu_ctx *c);
 int apply_step_inputs(ramgpu_ctx *c, const double *d_kp_n, const double *d_bs_n);
+int flush_finalize(ramgpu_ctx *c);
 int fill_out(ramgpu_ctx *c, ramgpu_species_out *out);
 int check_species(ramgpu_ctx *c, int s);
 // sweeps of species s0..s0+nsp-1, in place on F2 or from src into dst
