@@ -176,29 +176,28 @@ int prepare(ramgpu_ctx *c) {
     c->efield_dirty = true;
   }
   if (c->efield_dirty) {
+    // the step's Courant-number terms, the previous schedule step's finalize
+    // if pending, and the ghost carry, in one launch
     tbeg(c, RAMGPU_KERNEL_PREPARE);
-    const Launch l = launch_of(c, 0, 1, (c->nr * c->nt + 255) / 256);
-    const size_t lds = c->step_kp ? (size_t)(c->nr + 1 + c->nt) * sizeof(double) : 0;
-    if (c->fin_mask >= 0) {  // with the previous step's finalize
-      const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
-      const Launch lf = launch_of(c, 0, c->ns, 1);
-      hipLaunchKernelGGL(k_coef_fin, dim3(lc.nv + lf.nv), dim3(kFinalizeThreads), lds, c->stream, lc, c->step_kp,
-                         c->step_bs, lf, c->fin_mask, c->fin_mode);
-      c->fin_mask = -1;
-    } else {
-      hipLaunchKernelGGL(k_coef_ij, dim3(l.nv), dim3(256), lds, c->stream, l, c->step_kp, c->step_bs);
-    }
+    const int nvt = c->step_kp ? c->nr + 1 + c->nt : 0;
+    const size_t lds = (size_t)(nvt + 4 * c->nt) * sizeof(double);
+    const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
+    const Launch lf = launch_of(c, 0, c->fin_mask >= 0 ? c->ns : 0, 1);
+    const Launch lg = launch_of(c, 0, c->ns, 1);
+    hipLaunchKernelGGL(k_step_head, dim3(lc.nv + lf.nv + lg.nv), dim3(kFinalizeThreads), lds, c->stream, lc,
+                       c->step_kp, c->step_bs, lf, c->fin_mask, c->fin_mode, lg, nvt);
+    c->fin_mask = -1;
     c->step_kp = c->step_bs = nullptr;
     tend(c);
-    if ((rc = launch_check("k_coef_ij"))) return rc;
+    if ((rc = launch_check("k_step_head"))) return rc;
     c->efield_dirty = false;
-    c->ghost_dirty = true;
+    c->ghost_dirty = false;
     c->cfl_valid = false;
   }
   if (c->ghost_dirty) {
     tbeg(c, RAMGPU_KERNEL_PREPARE);
     const Launch l = launch_of(c, 0, c->ns, 1);
-    hipLaunchKernelGGL(k_ghost, dim3(l.nv), dim3(1024), 0, c->stream, l);
+    hipLaunchKernelGGL(k_ghost, dim3(l.nv), dim3(1024), (size_t)4 * c->nt * sizeof(double), c->stream, l);
     tend(c);
     if ((rc = launch_check("k_ghost"))) return rc;
     c->ghost_dirty = false;

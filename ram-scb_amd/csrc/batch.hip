@@ -47,7 +47,7 @@ struct ramgpu_batch {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   Tab *d_tabs = nullptr;
   Plan plan[BK_N];
-  size_t plane_lds = 0, radial_lds = 0, azim_lds = 0;
+  size_t plane_lds = 0, radial_lds = 0, azim_lds = 0, ghost_lds = 0;
   int mpts_max = 0;  // most satellite points of a member (ensemble adjoint)
   PlaneShape plane{64, 8};  // the largest block and row bound any member needs
   int sum_ns = 0;
@@ -165,7 +165,7 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
     hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc, (const double *)nullptr,
                      (const double *)nullptr);
     const Launch lg = launch_for(b, BK_GHOST);
-    hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), 0, st, lg);
+    hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), b->ghost_lds, st, lg);
   }
   const Launch lr = launch_for(b, BK_R), lp = launch_for(b, BK_P), lk = launch_for(b, BK_PLANE),
                lf = launch_for(b, BK_FIN);
@@ -206,7 +206,7 @@ void enq_sched(ramgpu_batch *b, int step) {
   hipLaunchKernelGGL(k_coef_ij, dim3(grid_for(b, BK_COEFIJ)), dim3(256), 0, st, lc, (const double *)nullptr,
                      (const double *)nullptr);
   const Launch lg = launch_for(b, BK_GHOST);
-  hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), 0, st, lg);
+  hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), b->ghost_lds, st, lg);
 }
 
 void enq_copy(ramgpu_batch *b, int ssl, int dsl) {
@@ -386,6 +386,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->plane_lds = std::max(b->plane_lds, plane_lds_bytes(c->ne, c->npa));
     b->radial_lds = std::max(b->radial_lds, radial_lds_bytes(c->nr, c->ne, c->npa));
     b->azim_lds = std::max(b->azim_lds, azim_lds_bytes(c->nt, c->ne, c->npa));
+    b->ghost_lds = std::max(b->ghost_lds, (size_t)4 * c->nt * sizeof(double));
     b->plane.threads = std::max(b->plane.threads, c->plane.threads);
     b->plane.maxc = std::max(b->plane.maxc, c->plane.maxc);
   }

@@ -27,7 +27,8 @@ __global__ void k_axpy(double *y, const double *x, double a, size_t n);
 __global__ void k_fill(double *y, double v, size_t n);
 __global__ void k_species(Launch L);
 __global__ void k_coef_ij(Launch L, const double *kp, const double *bs);
-__global__ void k_coef_fin(Launch Lc, const double *kp, const double *bs, Launch Lf, int cflmask, int summode);
+__global__ void k_step_head(Launch Lc, const double *kp, const double *bs, Launch Lf, int cflmask, int summode,
+                            Launch Lg, int nvt);
 __global__ void k_coef(Launch L);
 __global__ void k_ghost(Launch L);
 template <int MODE> __global__ void k_r(Launch L);

@@ -140,28 +140,55 @@ __device__ __forceinline__ double vs_vt(double avs, double lz, double phi) {
 // sin(PHI), which do not depend on Kp, in LDS (coef_ij_lds_bytes), and VT =
 // (AVS*(LZ*RE)^2)*sin(PHI) is the product vs_vt forms.  bs, when given, is
 // the step's boundary-flux scales, copied to the descriptor's.
+// (LZ*RE)^2 [NR+1] and sin(PHI) [NT] into svt, by the block
+__device__ inline void stage_vt_factors(const Tab &t, double *svt) {
+  const int NR1 = t.nr + 1;
+  for (int x = threadIdx.x; x < NR1 + t.nt; x += blockDim.x) {
+    if (x < NR1) {
+      const double lre = t.lz[x] * kRE;
+      svt[x] = lre * lre;
+    } else {
+      svt[x] = sin(t.phi[x - NR1] - 0.0);
+    }
+  }
+  __syncthreads();
+}
+
+// VT(I,J): the array, or with kp (avs) the factors staged in svt
+#define VTV(II, JJ) (kp ? (avs * svt[(II)-1]) * svt[t.nr + 1 + (JJ)-1] : GEO2(t.vt, II, JJ))
+
+// DRIFTR's terms of face I (0-based i) on line J: rij = {CR, BNES(I,J) +
+// BNES(I+1,J), its reciprocal, outsideMGNP(I,J)} (src/ModRamDrift.f90:
+// 118-127) and rsi = {RLZ+MDR/2, its reciprocal}; one definition for the
+// table and for the ghost carry, which forms the face-NR row itself
+__device__ __forceinline__ double4 radial_rij(const Tab &t, int i, int j, const double *kp, double avs,
+                                              const double *svt) {
+  const int I = i + 1, J = j + 1, NT = t.nt;
+  const double dts = t.dts, mdr = t.mdr, dphi = t.dphi;
+  const double rlz = t.rlz[i];
+  const int J0 = (J == 1) ? NT - 1 : J - 1;
+  const int J1 = (J == NT) ? 2 : J + 1;
+  const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
+  const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
+  const double dv = VTV(I, J0) + VTV(I + 1, J0) - VTV(I, J1) - VTV(I + 1, J1);
+  const double cr = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
+  return make_double4(cr, bsum, 1.0 / bsum, (double)MGNP(I, J));
+}
+__device__ __forceinline__ double2 radial_rsi(const Tab &t, int i) {
+  const double rs = t.rlz[i] + 0.5 * t.mdr;
+  return make_double2(rs, 1.0 / rs);
+}
+
 __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, const double *bs, double avs,
                                     double *svt) {
   {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    const int NR1 = t.nr + 1;
     if (bs && u.b == 0)
       for (int s = threadIdx.x; s < t.ns; s += blockDim.x) t.bscale[s] = bs[s];
-    if (kp) {
-      for (int x = threadIdx.x; x < NR1 + t.nt; x += blockDim.x) {
-        if (x < NR1) {
-          const double lre = t.lz[x] * kRE;
-          svt[x] = lre * lre;
-        } else {
-          svt[x] = sin(t.phi[x - NR1] - 0.0);
-        }
-      }
-      __syncthreads();
-    }
+    if (kp) stage_vt_factors(t, svt);
     const int n = u.b * blockDim.x + threadIdx.x;
     if (n < t.nr * t.nt) {
-#define VTV(II, JJ) (kp ? (avs * svt[(II)-1]) * svt[NR1 + (JJ)-1] : GEO2(t.vt, II, JJ))
       const int i = n / t.nt, j = n % t.nt;
       const int I = i + 1, J = j + 1;
       const int NT = t.nt;
@@ -171,18 +198,9 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
       const double mg = (double)MGNP(I, J);
       if (j == 0) {
         t.rrlz[i] = 1.0 / rlz;
-        const double rs = rlz + 0.5 * mdr;
-        t.rsi[i] = make_double2(rs, 1.0 / rs);
+        t.rsi[i] = radial_rsi(t, i);
       }
-      {
-        const int J0 = (J == 1) ? NT - 1 : J - 1;
-        const int J1 = (J == NT) ? 2 : J + 1;
-        const double vr = dts / mdr / (rlz + 0.5 * mdr) / 2.0 / dphi;  // :67
-        const double bsum = GEO2(t.bnes, I, J) + GEO2(t.bnes, I + 1, J);
-        const double dv = VTV(I, J0) + VTV(I + 1, J0) - VTV(I, J1) - VTV(I + 1, J1);
-        const double cr = vr * dv / bsum + (GEO2(t.eip, I, J) + GEO2(t.eip, I + 1, J)) / bsum * dts / mdr;
-        t.rij[ij] = make_double4(cr, bsum, 1.0 / bsum, mg);
-      }
+      t.rij[ij] = radial_rij(t, i, j, kp, avs, svt);
       double pa = 0.0, pce = 0.0, pbs = 1.0;
       double drd1 = 0.0, dpd1 = 0.0, drm1 = 0.0, dpm1 = 0.0, db2 = 0.0;
       const double bij = GEO2(t.bnes, I, J);
@@ -221,7 +239,6 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
     }
     if (kp) __syncthreads();  // svt is rewritten for the block's next unit
   }
-#undef VTV
 }
 
 __global__ void k_coef_ij(Launch L, const double *kp, const double *bs) {
@@ -366,50 +383,75 @@ __global__ void k_coef(Launch L) {
 // finds the top inflow J of its planes (four J per memory round trip) into
 // LDS, then a block-wide exclusive max-scan over the planes in order (wave
 // shuffles, then the waves' totals).
+// One (problem, species) of the ghost carry.  The face-NR terms it tests
+// (rij(NR-1,.), rsi(NR-1)) are formed here with the table's own code, so the
+// carry does not wait for k_coef_ij; rrow holds them (4*NT doubles), svt the
+// VT factors when kp is given.
+__device__ inline void ghost_unit(const Launch &L, int vb, const double *kp, double avs, double *svt, double *rrow,
+                                  int *s_last, int *s_wave) {
+  const Unit u = unit_of(L, vb);
+  const Tab &t = *u.t;
+  const int s = u.s, NR = t.nr, NT = t.nt, NPA = t.npa;
+  const int P = t.ne * NPA, T = blockDim.x, tid = threadIdx.x;
+  if (kp) stage_vt_factors(t, svt);
+  for (int j = tid; j < NT; j += T) {
+    const double4 r = radial_rij(t, NR - 1, j, kp, avs, svt);
+    rrow[4 * j] = r.x;
+    rrow[4 * j + 1] = r.y;
+    rrow[4 * j + 2] = r.z;
+    rrow[4 * j + 3] = r.w;
+  }
+  __syncthreads();
+  const double2 rs = radial_rsi(t, NR - 1);
+  for (int p = tid; p < P; p += T) {
+    const int k = p / NPA, l = p - k * NPA;
+    const double p4k = t.p4[s * t.ne + k];
+    const gptr<const double> q1 = as_global(t.q1 + ((size_t)l * NR + NR - 1) * NT);
+    int last = -1;
+    for (int j0 = NT - 1; j0 >= 0 && last < 0; j0 -= 4) {
+      double c[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int j = max(j0 - m, 0);
+        c[m] = c_radial_of(make_double4(rrow[4 * j], rrow[4 * j + 1], rrow[4 * j + 2], rrow[4 * j + 3]), rs, q1[j],
+                           p4k);
+      }
+#pragma unroll
+      for (int m = 3; m >= 0; --m)
+        if (j0 - m >= 0 && c[m] < 0) last = p * NT + j0 - m;  // the highest inflow J of the group
+    }
+    s_last[p] = last;
+  }
+  __syncthreads();
+  // planes [p0, p1) of this thread, in order
+  const int chunk = (P + T - 1) / T;
+  const int p0 = min(P, tid * chunk), p1 = min(P, p0 + chunk);
+  int v = -1;
+  for (int p = p0; p < p1; ++p) v = max(v, s_last[p]);
+  const int lane = tid & 63, w = tid >> 6;
+  for (int off = 1; off < 64; off <<= 1) {  // inclusive max-scan in the wave
+    const int o = __shfl_up(v, off, 64);
+    if (lane >= off) v = max(v, o);
+  }
+  if (lane == 63) s_wave[w] = v;
+  __syncthreads();
+  int run = __shfl_up(v, 1, 64);
+  if (lane == 0) run = -1;
+  for (int ww = 0; ww < w; ++ww) run = max(run, s_wave[ww]);
+  // the index of an inflow line grows with p, so the last one is the maximum
+  for (int p = p0; p < p1; ++p) {
+    t.carry[(size_t)s * P + p] = run;
+    run = max(run, s_last[p]);
+  }
+  __syncthreads();
+}
+#undef VTV
+
 __global__ void __launch_bounds__(1024) k_ghost(Launch L) {
+  extern __shared__ double rrow[];  // 4*NT: ghost_rows_lds_bytes
   __shared__ int s_last[kMaxPlaneCells];
   __shared__ int s_wave[16];
-  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
-    const Unit u = unit_of(L, vb);
-    const Tab &t = *u.t;
-    const int s = u.s, NT = t.nt, NPA = t.npa;
-    const int P = t.ne * NPA, T = blockDim.x, tid = threadIdx.x;
-    for (int p = tid; p < P; p += T) {
-      const int k = p / NPA, l = p - k * NPA;
-      int last = -1;
-      for (int j0 = NT - 1; j0 >= 0 && last < 0; j0 -= 4) {
-        bool in[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) in[m] = line_inflow(t, s, max(j0 - m, 0), k, l) && j0 - m >= 0;
-#pragma unroll
-        for (int m = 3; m >= 0; --m)
-          if (in[m]) last = p * NT + j0 - m;  // the highest inflow J of the group
-      }
-      s_last[p] = last;
-    }
-    __syncthreads();
-    // planes [p0, p1) of this thread, in order
-    const int chunk = (P + T - 1) / T;
-    const int p0 = min(P, tid * chunk), p1 = min(P, p0 + chunk);
-    int v = -1;
-    for (int p = p0; p < p1; ++p) v = max(v, s_last[p]);
-    const int lane = tid & 63, w = tid >> 6;
-    for (int off = 1; off < 64; off <<= 1) {  // inclusive max-scan in the wave
-      const int o = __shfl_up(v, off, 64);
-      if (lane >= off) v = max(v, o);
-    }
-    if (lane == 63) s_wave[w] = v;
-    __syncthreads();
-    int run = __shfl_up(v, 1, 64);
-    if (lane == 0) run = -1;
-    for (int ww = 0; ww < w; ++ww) run = max(run, s_wave[ww]);
-    // the index of an inflow line grows with p, so the last one is the maximum
-    for (int p = p0; p < p1; ++p) {
-      t.carry[(size_t)s * P + p] = run;
-      run = max(run, s_last[p]);
-    }
-    __syncthreads();
-  }
+  for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) ghost_unit(L, vb, nullptr, 0.0, nullptr, rrow, s_last, s_wave);
 }
 
 // ---------------------------------------------------------------------------
@@ -1107,19 +1149,30 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cfl
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) finalize_unit(L, vb, cflmask, summode, red);
 }
 
-// One schedule step's start and the previous step's end in one launch (the
-// two are independent; apart, each is a latency-bound launch of its own):
-// blocks below Lc.nv form the step's Courant-number terms as k_coef_ij, the
-// others finish the previous step as k_finalize, one block per species.
-// Blocks of kFinalizeThreads, as k_finalize's, so the sums are its own.
-__global__ void __launch_bounds__(kFinalizeThreads) k_coef_fin(Launch Lc, const double *kp, const double *bs, Launch Lf,
-                                                               int cflmask, int summode) {
-  extern __shared__ double svt[];
+// The head of a step in one launch (each part on its own is a latency-bound
+// launch of a few microseconds): blocks below Lc.nv form the step's
+// Courant-number terms (k_coef_ij), the next Lf.nv finish the previous step
+// of a schedule (k_finalize, one block per species; none when Lf.nv = 0),
+// the last Lg.nv form the step's ghost carry (k_ghost).  The parts touch
+// disjoint data.  Blocks of kFinalizeThreads, as k_finalize's and
+// k_ghost's, so the sums and scans are theirs.  Dynamic LDS: the VT factors
+// (NR+1+NT) then the ghost's face row (4*NT).
+__global__ void __launch_bounds__(kFinalizeThreads) k_step_head(Launch Lc, const double *kp, const double *bs,
+                                                                Launch Lf, int cflmask, int summode, Launch Lg,
+                                                                int nvt) {
+  extern __shared__ double dyn[];
   __shared__ double red[(kFinalizeThreads / 64) * kFinNV];
-  if ((int)blockIdx.x < Lc.nv)
-    coef_ij_unit(Lc, blockIdx.x, kp, bs, kp ? vs_avs(*kp) : 0.0, svt);
-  else
-    finalize_unit(Lf, blockIdx.x - Lc.nv, cflmask, summode, red);
+  __shared__ int s_last[kMaxPlaneCells];
+  __shared__ int s_wave[kFinalizeThreads / 64];
+  const double avs = kp ? vs_avs(*kp) : 0.0;
+  int b = blockIdx.x;
+  if (b < Lc.nv) {
+    coef_ij_unit(Lc, b, kp, bs, avs, dyn);
+  } else if ((b -= Lc.nv) < Lf.nv) {
+    finalize_unit(Lf, b, cflmask, summode, red);
+  } else {
+    ghost_unit(Lg, b - Lf.nv, kp, avs, dyn, dyn + nvt, s_last, s_wave);
+  }
 }
 
 // VT(I,J)=AVS*(LZ(I)*RE)**2*SIN(PHI(J)-PHIOFS), src/ModRamRun.f90:45-54

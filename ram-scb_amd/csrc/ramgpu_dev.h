@@ -28,7 +28,7 @@ constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chos
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bound, checked at creation
 constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
-constexpr int kFinalizeThreads = 512;  // one block per (problem, species) of k_finalize
+constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 
 // packed per-(I,J) terms of the plane sweeps, kij[(i*NT+j)*8 + ..]
 enum KijField { KIJ_DRD1, KIJ_DPD1, KIJ_BIJ, KIJ_RBIJ, KIJ_DRM1, KIJ_DPM1, KIJ_DB2, KIJ_MG };
