@@ -19,7 +19,7 @@ so the throughput is timed without them); the cpu_baseline object is the oracle'
 C restatement (OpenMP over species, as the reference) on a bounded sample of
 the same workload.
 
-Two more objects in the same line, each on a bounded sample:
+More objects in the same line, each on a bounded sample:
   "batched": an ensemble of uneven members (half 20x25x35x72, half
     40x48x48x90, 4 species each, per-member Kp ~ U(1,7) and boundary scale
     ~ LogNormal(0,0.5)) stepped as one launch per kernel under a captured
@@ -30,7 +30,10 @@ Two more objects in the same line, each on a bounded sample:
     checkpoints every 10 steps, against the 60-step forward alone;
   "batched_adjoint": the same misfit gradient for an ensemble of 4-species
     40x96x64x90 members (Kp ~ U(1,8) per member), 120 steps with checkpoints
-    every 20, in one launch per kernel, against the ensemble's forward.
+    every 20, in one launch per kernel, against the ensemble's forward;
+  "moments": the block after ram_run's species loop on the multi-species
+    F2 -- ANISCH of every species (PPERT, PPART and the densities), alone and
+    with FLUX -- per launch.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M]
 """
@@ -264,6 +267,25 @@ def bench_batched_adjoint(members: int, nsteps: int, ckpt: int, seed: int):
             "grad_f2_norm": [float(np.linalg.norm(x)) for x in gf]}
 
 
+def bench_moments(dev, cells_kl: int, reps: int = 50):
+    """ANISCH of every species (and with FLUX) on the resident F2: the block
+    after ram_run's species loop.  Kernel time from HIP events around each
+    launch; algorithmic bytes 8 B per cell read (+ 8 B written for FLUX)."""
+    dev.moments(flux=False)
+    res = {}
+    for tag, flux, n, nbytes in (("anisch", False, reps, 8), ("anisch_flux", True, reps // 5, 16)):
+        dev.kernel_timing(True)
+        for _ in range(n):
+            dev.moments(flux=flux)
+        ms, nl = dev.kernel_times()["moments"]
+        dev.kernel_timing(False)
+        avg = ms / nl * 1e-3
+        res[tag] = {"avg_launch_us": avg * 1e6, "bytes_per_launch": nbytes * cells_kl,
+                    "achieved_GBps": nbytes * cells_kl / avg / 1e9, "launches": nl}
+        log(f"bench: moments {tag:12s} {avg * 1e6:8.2f} us  {nbytes * cells_kl / avg / 1e9:7.0f} GB/s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -371,6 +393,7 @@ def main():
         "copy_peak_GBps": copy_peak,
         "roofline": roofline,
     }
+    out["moments"] = bench_moments(dev, cells_kl)
     dev.close()
     if not args.no_batched:
         out["batched"] = bench_batched(args.members, args.batched_steps, args.seed)

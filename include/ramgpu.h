@@ -146,6 +146,30 @@ int ramgpu_atmol(ramgpu_ctx *ctx, int s);
  * elorc = old - new.  Either may be NULL. */
 int ramgpu_sumrc(ramgpu_ctx *ctx, int s, double *setrc, double *elorc);
 
+/* ---- moments of the device-resident F2 ---------------------------------- */
+
+/* The moment factors of ARRAYS (src/ModRamInit.f90:560-576): FFACTOR
+ * (NS,NR,NE,NPA), ERNH and EPP (NS,NE).  Needed before the calls below. */
+int ramgpu_set_moment_factors(ramgpu_ctx *ctx, const double *ffactor, const double *ernh,
+                              const double *epp);
+/* ANISCH(S), src/ModRamRun.f90:265-415, without its wave-particle branch
+ * (:422-, DoUseWPI) and its file output.  Sets F2(S,2:NR,:,2:NE,1) =
+ * F2(S,2:NR,:,2:NE,2) on the device as the reference does, and writes the
+ * (S, 2..NR, 1..NT) entries of PPERT and PPART, and of EDENT and RNHTT (the
+ * energy and number densities of its wans file), all (NS,NR,NT) arrays;
+ * other entries are left alone, any pointer may be NULL.  The energy
+ * windows end at min(khi, NE) (the reference reads past NE when NE < 30). */
+int ramgpu_anisch(ramgpu_ctx *ctx, int s, double *ppert, double *ppart, double *edent, double *rnhtt);
+/* FLUX(NS,NR,NT,NE,NPA) = F2/FFACTOR/FNHS, src/ModRamRun.f90:208-222, of
+ * the current F2; 0 outside I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1, as the
+ * reference's FLUX (zeroed at allocation, src/ModRamInit.f90:78). */
+int ramgpu_flux(ramgpu_ctx *ctx, double *flux);
+/* The block of ram_run after its species loop (src/ModRamRun.f90:207-222):
+ * ANISCH of every species, then FLUX, in one launch.  flux may be NULL: then
+ * only the NS*NR*NT moments leave the device. */
+int ramgpu_moments(ramgpu_ctx *ctx, double *ppert, double *ppart, double *edent, double *rnhtt,
+                   double *flux);
+
 /* ---- the fused split step ----------------------------------------------- */
 
 /* `nsteps` consecutive ram_run calls (src/ModRamRun.f90:64-205, optional
@@ -180,7 +204,8 @@ int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
 #define RAMGPU_KERNEL_R_BWD 6    /* adjoint of DRIFTR                         */
 #define RAMGPU_KERNEL_P_BWD 7    /* adjoint of DRIFTP                         */
 #define RAMGPU_KERNEL_PLANE_BWD 8 /* adjoints of DRIFTE, DRIFTMU, losses      */
-#define RAMGPU_KERNEL_KINDS 9
+#define RAMGPU_KERNEL_MOMENTS 9  /* ANISCH pressure moments and FLUX          */
+#define RAMGPU_KERNEL_KINDS 10
 int ramgpu_kernel_timing(ramgpu_ctx *ctx, int enable);
 /* Measurement support: a plain device copy of `bytes` (read + write) run
  * `reps` times on device `device`; gbps receives the achieved read+write

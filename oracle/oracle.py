@@ -86,6 +86,10 @@ class Oracle:
             lib.oracle_loss_alloc.argtypes = [sp, C.POINTER(_Loss)]
             lib.oracle_loss_alloc.restype = C.c_int
             lib.oracle_loss_free.argtypes = [C.POINTER(_Loss)]
+            lib.oracle_anisch.argtypes = [sp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+            lib.oracle_anisch.restype = None
+            lib.oracle_flux.argtypes = [sp, _dp, _dp]
+            lib.oracle_flux.restype = None
             cls._lib = lib
         return cls._lib
 
@@ -123,6 +127,7 @@ class Oracle:
                   "lsdr", "lscha", "lsatm"):
             setattr(st, n, d(n, np.zeros(g.ns)))
         self._keep = keep
+        self._grid = g
         self.st = st
         self._drift = {}
         self._loss = {}
@@ -191,6 +196,34 @@ class Oracle:
         shp = (self.nr, self.nt, self.ne, self.npa) if name == "charge" else (self.nr, self.ne)
         n = int(np.prod(shp))
         return np.ctypeslib.as_array(getattr(ls, name), shape=(n,)).reshape(shp, order="F").copy()
+
+    def _moment_factors(self):
+        if "ffactor" not in self._keep:
+            g = self._grid
+            self._keep["ffactor"] = _fd(g.ffactor)
+            self._keep["ernh"] = _fd(g.ernh)
+            self._keep["epp"] = _fd(g.epp)
+        return self._keep["ffactor"], self._keep["ernh"], self._keep["epp"]
+
+    def anisch(self, s: int, out=None):
+        """ANISCH(S): updates F2's L=1 row and the (S, 2..NR, :) entries of
+        out = {ppert, ppart, edent, rnhtt} (NS,NR,NT) arrays, zeros when new."""
+        ff, ernh, epp = self._moment_factors()
+        shp = (self.ns, self.nr, self.nt)
+        if out is None:
+            out = {n: np.zeros(shp, order="F") for n in ("ppert", "ppart", "edent", "rnhtt")}
+        self.lib().oracle_anisch(C.byref(self.st), s, _ptr(ff), _ptr(ernh), _ptr(epp),
+                                 *(_ptr(out[n]) for n in ("ppert", "ppart", "edent", "rnhtt")))
+        return out
+
+    def flux(self, out=None) -> np.ndarray:
+        """FLUX (NS,NR,NT,NE,NPA) of the current F2; entries outside the
+        reference's ranges keep out's values (zeros when new)."""
+        ff, _, _ = self._moment_factors()
+        if out is None:
+            out = np.zeros((self.ns, self.nr, self.nt, self.ne, self.npa), order="F")
+        self.lib().oracle_flux(C.byref(self.st), _ptr(ff), _ptr(out))
+        return out
 
     def ram_run(self, nthreads: int = 1) -> float:
         out = C.c_double(0.0)

@@ -540,6 +540,84 @@ void oracle_sumrc(oracle_state *st, int s) {
   st->elorc[s - 1] = enold - sum;
 }
 
+/* ANISCH(S), src/ModRamRun.f90:265-415, without the wave-particle branch
+ * after it (:422-, DoUseWPI only) and without its file output.  For I=2..NR
+ * and every J: F2(S,I,J,K,1)=F2(S,I,J,K,2) for K=2..NE, then the pitch-angle
+ * sums over L=1..int(UPA(I)-1) of each K and the energy windows
+ * K=2..6, 7..10, 11..25, 26..30, 31..NE (:297-298, khi(5)=NE), each window's
+ * sums scaled by RFAC=4*PI/(CS*100) before they are added to the totals.
+ * The reference indexes F2 past NE when NE < 30; here each window ends at
+ * min(khi, NE), which is the reference's for NE >= 30.
+ * ffactor (NS,NR,NE,NPA), ernh and epp (NS,NE); the outputs (NS,NR,NT) get
+ * their (S, 2..NR, 1..NT) entries; edent and rnhtt (the EDENT and RNHTT of
+ * the reference's wans file, :405) may be NULL. */
+#define FF(A, S, I, K, L) \
+  (A)[((S)-1) + (size_t)st->ns * (((I)-1) + (size_t)st->nr * (((K)-1) + (size_t)st->ne * ((L)-1)))]
+#define SIJ(A, S, I, J) (A)[((S)-1) + (size_t)st->ns * (((I)-1) + (size_t)st->nr * ((J)-1))]
+void oracle_anisch(oracle_state *st, int s, const double *ffactor, const double *ernh, const double *epp,
+                   double *ppert, double *ppart, double *edent, double *rnhtt) {
+  const double cv = 2.998E8 * 100;  /* CS*100 */
+  const double rfac = 4 * 3.1415926535897932384626433832795 / cv;
+  int khi[5] = {6, 10, 25, 30, st->ne};
+  for (int i = 2; i <= st->nr; ++i)
+    for (int j = 1; j <= st->nt; ++j) {
+      int klo = 2;
+      double ppt = 0.0, ppa = 0.0, rnhtt_ = 0.0, edent_ = 0.0;
+      for (int iwa = 0; iwa < 5; ++iwa) {
+        double pper = 0.0, ppar = 0.0, rnht = 0.0, eden = 0.0;
+        const int kh = khi[iwa] < st->ne ? khi[iwa] : st->ne;
+        for (int k = klo; k <= kh; ++k) {
+          F2(s, i, j, k, 1) = F2(s, i, j, k, 2);
+          double sume = 0.0, suma = 0.0, sumn = 0.0;
+          const int u = (int)(V1(st->upa, i) - 1);
+          for (int l = 1; l <= u; ++l) {
+            const double ernm = V1(st->wmu, l) / FF(ffactor, s, i, k, l) / G3(st->fnhs, i, j, l);
+            const double epma = ernm * V1(st->mu, l) * V1(st->mu, l);
+            const double epme = ernm - epma;
+            sume = sume + F2(s, i, j, k, l) * epme;
+            suma = suma + F2(s, i, j, k, l) * epma;
+            sumn = sumn + F2(s, i, j, k, l) * ernm;
+          }
+          pper = pper + SK(epp, s, k) * sume;
+          ppar = ppar + SK(epp, s, k) * suma;
+          rnht = rnht + SK(ernh, s, k) * sumn;
+          eden = eden + SK(ernh, s, k) * V1(st->ekev, k) * sumn;
+        }
+        rnht = rnht * rfac;
+        eden = eden * rfac;
+        ppar = 2 * rfac * ppar;
+        pper = rfac * pper;
+        klo = khi[iwa] + 1;
+        edent_ = edent_ + eden;
+        ppt = ppt + pper;
+        ppa = ppa + ppar;
+        rnhtt_ = rnhtt_ + rnht;
+      }
+      SIJ(ppert, s, i, j) = ppt;
+      SIJ(ppart, s, i, j) = ppa;
+      if (edent) SIJ(edent, s, i, j) = edent_;
+      if (rnhtt) SIJ(rnhtt, s, i, j) = rnhtt_;
+    }
+}
+
+/* FLUX(S,I,J,K,L) = F2/FFACTOR/FNHS for I=2..NR, K=2..NE, L=2..NPA,
+ * J=1..NT-1 (src/ModRamRun.f90:208-222); other entries are left as they are
+ * (the reference's FLUX is zero there: it is zeroed at allocation,
+ * src/ModRamInit.f90:78, and written only by this loop). */
+void oracle_flux(const oracle_state *st, const double *ffactor, double *flux) {
+  for (int s = 1; s <= st->ns; ++s)
+    for (int i = 2; i <= st->nr; ++i)
+      for (int k = 2; k <= st->ne; ++k)
+        for (int l = 2; l <= st->npa; ++l)
+          for (int j = 1; j <= st->nt - 1; ++j) {
+            const size_t x = (size_t)(s - 1) + (size_t)st->ns * ((i - 1) + (size_t)st->nr *
+                             ((j - 1) + (size_t)st->nt * ((k - 1) + (size_t)st->ne * (l - 1))));
+            flux[x] = F2(s, i, j, k, l) / FF(ffactor, s, i, k, l) / G3(st->fnhs, i, j, l);
+          }
+}
+#undef FF
+#undef SIJ
+
 /* The per-species body of ram_run, src/ModRamRun.f90:66-184, optional
  * processes off. */
 static int ram_run_species(oracle_state *st, int s) {

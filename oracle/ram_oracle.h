@@ -90,6 +90,16 @@ void oracle_sumrc(oracle_state *st, int s);
  * DtsNext before the DtsMin floor in *dts_next (may be NULL). */
 int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next);
 
+/* ANISCH(S) without its wave-particle branch (src/ModRamRun.f90:265-415):
+ * PPERT, PPART (and EDENT, RNHTT of its wans file, may be NULL), all
+ * (NS,NR,NT), from FFACTOR (NS,NR,NE,NPA), ERNH and EPP (NS,NE); sets
+ * F2(S,2:NR,:,2:NE,1) = F2(S,2:NR,:,2:NE,2) as the reference does. */
+void oracle_anisch(oracle_state *st, int s, const double *ffactor, const double *ernh, const double *epp,
+                   double *ppert, double *ppart, double *edent, double *rnhtt);
+/* FLUX = F2/FFACTOR/FNHS over I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1
+ * (src/ModRamRun.f90:208-222); FLUX (NS,NR,NT,NE,NPA). */
+void oracle_flux(const oracle_state *st, const double *ffactor, double *flux);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,6 +70,10 @@ struct ramgpu_ctx {
   int *d_mgnp;
   double *d_lz, *d_phi, *d_kp;
   double *d_stage;  // Fortran-layout staging buffer for F2 transfers
+  // moments (ANISCH, FLUX): FFACTOR as [s][i][l][k], ERNH and EPP (NS,NE),
+  // PPERT/PPART/EDENT/RNHTT (4 x (NS,NR,NT)), FLUX in F2's layout; allocated
+  // by their first use
+  double *d_ff = nullptr, *d_ernh = nullptr, *d_epp = nullptr, *d_mom = nullptr, *d_flux = nullptr;
   bool have_grid = false, have_geom = false, have_vt = false, have_bnd = false, have_f2 = false;
   bool have_species[kMaxSpecies] = {};
   bool have_lzphi = false;

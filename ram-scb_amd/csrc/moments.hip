@@ -1,0 +1,223 @@
+// moments.hip -- the pressure moments (ANISCH) and the flux conversion (FLUX)
+// on the device-resident F2: the block of ram_run after its species loop
+// (src/ModRamRun.f90:207-222) and ANISCH (:265-415) without its
+// wave-particle branch (:422-, taken only with DoUseWPI) and file output.
+// A host that needs PPERT/PPART every step downloads NS*NR*NT values of each
+// instead of F2.
+#include <algorithm>
+#include <vector>
+
+#include "ctx.h"
+
+namespace ramgpu {
+
+constexpr int kMomWaves = 4;  // (S,I,J) planes per block, one wave each
+
+// One wave per (S,I,J) plane of F2 ([S][I][J][L][K], NE*NPA contiguous).
+// Lane k holds energy K = k+1 (NE > 64: further passes of 64) and walks L in
+// the reference's order, so SUME, SUMA, SUMN of each K are the reference's
+// sequential sums; lane 0 then adds the per-K terms in K order into the
+// energy windows K=2..6, 7..10, 11..25, 26..30, 31..NE (:297-298; each
+// window ends at min(khi, NE): the reference reads past NE when NE < 30).
+// FLUX = F2/FFACTOR/FNHS of the plane is written in the same pass, 0 outside
+// I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1, where the reference's FLUX keeps
+// the zero of its allocation (src/ModRamInit.f90:78).
+// ff: FFACTOR as [s][i][l][k]; mom: PPERT, PPART, EDENT, RNHTT, each
+// (NS,NR,NT) in the reference's layout.
+__global__ void __launch_bounds__(64 * kMomWaves)
+    k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int nsp, int anisch,
+              const double *__restrict__ ff, const double *__restrict__ ernh, const double *__restrict__ epp,
+              const double *__restrict__ ekev, const double *__restrict__ wmu, const double *__restrict__ mu,
+              const double *__restrict__ fnhs, const int *__restrict__ upa, double *__restrict__ flux,
+              double *__restrict__ mom) {
+  __shared__ double red[kMomWaves][4][64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nplane = nsp * (nr - 1) * nt;
+  const int p = blockIdx.x * kMomWaves + wv;
+  const bool live = p < nplane;
+  const int pp = live ? p : 0;
+  const int j = pp % nt;
+  const int i = 1 + (pp / nt) % (nr - 1);  // I = 2..NR
+  const int s = s0 + pp / (nt * (nr - 1));
+  const size_t P = (size_t)ne * npa;
+  const size_t plane = (((size_t)s * nr + i) * nt + j) * P;
+  double *F = f2 + plane;
+  double *FX = flux ? flux + plane : nullptr;
+  const double *ffp = ff + ((size_t)s * nr + i) * P;
+  const double *fn = fnhs + i + (size_t)(nr + 1) * j;  // FNHS(I,J,L) = fn[fst*(L-1)]
+  const size_t fst = (size_t)(nr + 1) * nt;
+  const int u = upa[i] - 1;  // int(UPA(I)-1): UPA holds whole numbers
+  const bool jflux = j <= nt - 2;
+  const int lend = FX ? npa : (anisch ? std::min(u, npa) : 0);
+  constexpr double kRfac = 4 * 3.1415926535897932384626433832795 / (2.998E8 * 100);  // 4*PI/(CS*100)
+  // last K (1-based) of window w
+  auto khi = [ne](int w) { return std::min(w == 0 ? 6 : w == 1 ? 10 : w == 2 ? 25 : w == 3 ? 30 : ne, ne); };
+  int w = 0;
+  double win[4] = {0.0, 0.0, 0.0, 0.0}, tot[4] = {0.0, 0.0, 0.0, 0.0};
+  auto close_window = [&]() {
+    // RNHT*RFAC, EDEN*RFAC, 2*RFAC*PPAR, RFAC*PPER, then the totals (:375-397)
+    tot[0] = tot[0] + kRfac * win[0];
+    tot[1] = tot[1] + (2 * kRfac) * win[1];
+    tot[2] = tot[2] + win[3] * kRfac;
+    tot[3] = tot[3] + win[2] * kRfac;
+    win[0] = win[1] = win[2] = win[3] = 0.0;
+  };
+  for (int k0 = 0; k0 < ne; k0 += 64) {
+    const int k = k0 + lane;
+    const bool on = live && k < ne;
+    const bool kin = on && k >= 1;  // K = 2..NE
+    double se = 0.0, sa = 0.0, sn = 0.0;
+    if (on) {
+      const double f2v = F[ne + k];  // F2(S,I,J,K,2)
+      if (anisch && kin) F[k] = f2v;  // F2(S,I,J,K,1) = F2(S,I,J,K,2)
+      for (int l = 0; l < lend; ++l) {
+        const double f = (l == 0 && anisch && kin) ? f2v : F[(size_t)l * ne + k];
+        const double ffv = ffp[(size_t)l * ne + k], fnv = fn[fst * l];
+        if (anisch && kin && l < u) {
+          const double ernm = wmu[l] / ffv / fnv;
+          const double epma = ernm * mu[l] * mu[l];
+          const double epme = ernm - epma;
+          se = se + f * epme;
+          sa = sa + f * epma;
+          sn = sn + f * ernm;
+        }
+        if (FX) FX[(size_t)l * ne + k] = (kin && l >= 1 && jflux) ? f / ffv / fnv : 0.0;
+      }
+    }
+    if (anisch) {
+      const double e = kin ? epp[s + (size_t)ns * k] : 0.0, h = kin ? ernh[s + (size_t)ns * k] : 0.0;
+      const double ek = kin ? ekev[k] : 0.0;
+      red[wv][0][lane] = e * se;       // EPP(S,K)*SUME
+      red[wv][1][lane] = e * sa;       // EPP(S,K)*SUMA
+      red[wv][2][lane] = h * sn;       // ERNH(S,K)*SUMN
+      red[wv][3][lane] = h * ek * sn;  // ERNH(S,K)*EKEV(K)*SUMN
+    }
+    __syncthreads();
+    if (anisch && live && lane == 0) {
+      const int kend = std::min(k0 + 64, ne);
+      for (int kk = std::max(k0, 1); kk < kend; ++kk) {
+        while (kk + 1 > khi(w)) {  // K = kk+1 is past window w
+          close_window();
+          ++w;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) win[q] = win[q] + red[wv][q][kk - k0];
+      }
+    }
+    __syncthreads();
+  }
+  if (anisch && live && lane == 0) {
+    for (; w < 5; ++w) close_window();
+    const size_t n3 = (size_t)ns * nr * nt, x = s + (size_t)ns * (i + (size_t)nr * j);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mom[q * n3 + x] = tot[q];
+  }
+}
+
+}  // namespace ramgpu
+
+using namespace ramgpu_host;
+using ramgpu::k_moments;
+using ramgpu::kMomWaves;
+
+namespace {
+
+int alloc_zero(ramgpu_ctx *c, double **p, size_t n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess)
+    return fail(RAMGPU_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(double));
+  c->allocs.push_back(q);
+  HIP_TRY(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(double), c->stream));
+  *p = (double *)q;
+  return RAMGPU_OK;
+}
+
+// ANISCH of species s0..s0+nsp-1 (anisch != 0) and/or FLUX of them into
+// d_flux (flux), one launch.
+int run_moments(ramgpu_ctx *c, int s0, int nsp, int anisch, bool flux) {
+  if (!c->have_grid) return fail(RAMGPU_ESTATE, "ramgpu_set_grid has not been called");
+  if (!c->have_geom) return fail(RAMGPU_ESTATE, "ramgpu_set_geometry has not been called");
+  if (!c->have_f2) return fail(RAMGPU_ESTATE, "ramgpu_put_f2 has not been called");
+  if (!c->d_ff) return fail(RAMGPU_ESTATE, "ramgpu_set_moment_factors has not been called");
+  int rc;
+  if (flux && !c->d_flux && (rc = alloc_zero(c, &c->d_flux, (size_t)c->ns * c->nr * c->nt * c->ne * c->npa)))
+    return rc;
+  if (c->nr < 2) return RAMGPU_OK;
+  const int nplane = nsp * (c->nr - 1) * c->nt;
+  tbeg(c, RAMGPU_KERNEL_MOMENTS);
+  hipLaunchKernelGGL(k_moments, dim3((nplane + kMomWaves - 1) / kMomWaves), dim3(64 * kMomWaves), 0, c->stream,
+                     c->t.f2, c->ns, c->nr, c->nt, c->ne, c->npa, s0, nsp, anisch, (const double *)c->d_ff,
+                     (const double *)c->d_ernh, (const double *)c->d_epp, (const double *)c->d_ekev,
+                     (const double *)c->d_wmu, (const double *)c->d_mu, (const double *)c->d_fnhs,
+                     (const int *)c->d_upa, flux ? c->d_flux : nullptr, c->d_mom);
+  tend(c);
+  return launch_check("k_moments");
+}
+
+// the (S, 2..NR, 1..NT) entries of species s0..s0+nsp-1 into the caller's
+// (NS,NR,NT) arrays (any may be NULL)
+int download_moments(ramgpu_ctx *c, int s0, int nsp, double *ppert, double *ppart, double *edent,
+                     double *rnhtt) {
+  const size_t n3 = (size_t)c->ns * c->nr * c->nt;
+  std::vector<double> m(4 * n3);
+  HIP_TRY(hipMemcpyAsync(m.data(), c->d_mom, m.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  double *out[4] = {ppert, ppart, edent, rnhtt};
+  for (int q = 0; q < 4; ++q) {
+    if (!out[q]) continue;
+    for (int j = 0; j < c->nt; ++j)
+      for (int i = 1; i < c->nr; ++i)
+        for (int s = s0; s < s0 + nsp; ++s) {
+          const size_t x = s + (size_t)c->ns * (i + (size_t)c->nr * j);
+          out[q][x] = m[q * n3 + x];
+        }
+  }
+  return RAMGPU_OK;
+}
+
+}  // namespace
+
+int ramgpu_set_moment_factors(ramgpu_ctx *c, const double *ffactor, const double *ernh, const double *epp) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!ffactor || !ernh || !epp) return fail(RAMGPU_EINVAL, "ffactor, ernh and epp must not be null");
+  const size_t ns = c->ns, nr = c->nr, ne = c->ne, npa = c->npa;
+  if (!c->d_ff) {
+    if ((rc = alloc_zero(c, &c->d_ff, ns * nr * ne * npa)) || (rc = alloc_zero(c, &c->d_ernh, ns * ne)) ||
+        (rc = alloc_zero(c, &c->d_epp, ns * ne)) || (rc = alloc_zero(c, &c->d_mom, 4 * ns * nr * c->nt)))
+      return rc;
+  }
+  // FFACTOR(NS,NR,NE,NPA) -> [s][i][l][k], so that a plane's K run is contiguous
+  std::vector<double> t(ns * nr * ne * npa);
+  for (size_t l = 0; l < npa; ++l)
+    for (size_t k = 0; k < ne; ++k)
+      for (size_t i = 0; i < nr; ++i)
+        for (size_t s = 0; s < ns; ++s)
+          t[((s * nr + i) * npa + l) * ne + k] = ffactor[s + ns * (i + nr * (k + ne * l))];
+  HIP_TRY(hipMemcpyAsync(c->d_ff, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_ernh, ernh, ns * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->d_epp, epp, ns * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+int ramgpu_anisch(ramgpu_ctx *c, int s, double *ppert, double *ppart, double *edent, double *rnhtt) {
+  int rc = check_ctx(c);
+  if (rc || (rc = check_species(c, s)) || (rc = run_moments(c, s - 1, 1, 1, false))) return rc;
+  return download_moments(c, s - 1, 1, ppert, ppart, edent, rnhtt);
+}
+
+int ramgpu_flux(ramgpu_ctx *c, double *flux) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!flux) return fail(RAMGPU_EINVAL, "flux is null");
+  if ((rc = run_moments(c, 0, c->ns, 0, true))) return rc;
+  return download_f2(c, flux, c->d_flux);
+}
+
+int ramgpu_moments(ramgpu_ctx *c, double *ppert, double *ppart, double *edent, double *rnhtt, double *flux) {
+  int rc = check_ctx(c);
+  if (rc || (rc = run_moments(c, 0, c->ns, 1, flux != nullptr))) return rc;
+  if ((rc = download_moments(c, 0, c->ns, ppert, ppart, edent, rnhtt))) return rc;
+  return flux ? download_f2(c, flux, c->d_flux) : RAMGPU_OK;
+}

@@ -34,9 +34,11 @@ EXPORTS = (
     "ramgpu_driftr_bwd", "ramgpu_driftp_bwd", "ramgpu_drifte_bwd", "ramgpu_driftmu_bwd",
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
     "ramgpu_batch_adjoint_run", "ramgpu_set_plane_shape", "ramgpu_get_plane_shape",
+    "ramgpu_set_moment_factors", "ramgpu_anisch", "ramgpu_flux", "ramgpu_moments",
 )
 
-KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd")
+KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
+                "moments")
 
 
 class SpeciesOut(C.Structure):
@@ -87,6 +89,10 @@ def load_library(path: Path | None = None):
         getattr(lib, "ramgpu_" + n).argtypes = [v, C.c_int, _dp]
     lib.ramgpu_driftend.argtypes = [v]
     lib.ramgpu_sumrc.argtypes = [v, C.c_int, _dp, _dp]
+    lib.ramgpu_set_moment_factors.argtypes = [v, _dp, _dp, _dp]
+    lib.ramgpu_anisch.argtypes = [v, C.c_int, _dp, _dp, _dp, _dp]
+    lib.ramgpu_flux.argtypes = [v, _dp]
+    lib.ramgpu_moments.argtypes = [v, _dp, _dp, _dp, _dp, _dp]
     lib.ramgpu_ram_run.argtypes = [v, C.c_int, C.POINTER(SpeciesOut)]
     lib.ramgpu_ram_run_schedule.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
@@ -181,6 +187,8 @@ class RamGpu:
         self.set_efield(problem.vt)
         self.set_boundary(problem.fgeos)
         self.put_f2(problem.f2)
+        if getattr(g, "ernh", None) is not None:
+            self.set_moment_factors(g)
         return self
 
     def set_species(self, s, charge, cex_kind, rmas):
@@ -236,6 +244,37 @@ class RamGpu:
         out = np.zeros(self.dims, order="F")
         self._check(self.lib.ramgpu_get_f2(self.h, _p(out)))
         return out
+
+    # -- moments (ANISCH, FLUX) ------------------------------------------
+    MOMENTS = ("ppert", "ppart", "edent", "rnhtt")
+
+    def set_moment_factors(self, g):
+        ns, nr, nt, ne, npa = self.dims
+        arrs = [_fd(g.ffactor, ns * nr * ne * npa), _fd(g.ernh, ns * ne), _fd(g.epp, ns * ne)]
+        self._check(self.lib.ramgpu_set_moment_factors(self.h, *[_p(a) for a in arrs]))
+
+    def anisch(self, s: int, out=None) -> dict:
+        """ANISCH(S): the (S, 2..NR, :) entries of out's (NS,NR,NT) arrays."""
+        ns, nr, nt, _, _ = self.dims
+        if out is None:
+            out = {n: np.zeros((ns, nr, nt), order="F") for n in self.MOMENTS}
+        self._check(self.lib.ramgpu_anisch(self.h, s, *[_p(out[n]) for n in self.MOMENTS]))
+        return out
+
+    def flux(self) -> np.ndarray:
+        out = np.zeros(self.dims, order="F")
+        self._check(self.lib.ramgpu_flux(self.h, _p(out)))
+        return out
+
+    def moments(self, flux: bool = True, out=None):
+        """ANISCH of every species and (flux=True) FLUX, in one launch."""
+        ns, nr, nt, _, _ = self.dims
+        if out is None:
+            out = {n: np.zeros((ns, nr, nt), order="F") for n in self.MOMENTS}
+        fx = np.zeros(self.dims, order="F") if flux else None
+        self._check(self.lib.ramgpu_moments(self.h, *[_p(out[n]) for n in self.MOMENTS],
+                                            _p(fx) if flux else None))
+        return out, fx
 
     def energy_state(self):
         arrs = [np.zeros(self.ns) for _ in range(4)]

@@ -108,6 +108,8 @@ class RamGrid:
     cone: np.ndarray = field(default=None)     # (NR+4)
     upa: np.ndarray = field(default=None)      # (NR) real-valued index
     ffactor: np.ndarray = field(default=None)  # (NS,NR,NE,NPA)
+    ernh: np.ndarray = field(default=None)     # (NS,NE)
+    epp: np.ndarray = field(default=None)      # (NS,NE)
 
     @property
     def ns(self) -> int:
@@ -257,6 +259,16 @@ def build_grid(nr: int, nt: int, ne: int, npa: int, species, elb: float = 0.1) -
                     ff[s, i, k, l] = base * mu[l]
                 ff[s, i, k, 0] = ff[s, i, k, 1]
     g.ffactor = ff
+
+    # energy factors of the moments, :573-575
+    ernh = np.zeros((ns, ne))
+    epp = np.zeros((ns, ne))
+    for s in range(ns):
+        for k in range(ne):
+            gk = grel[s, k]
+            ernh[s, k] = we[k] * gk / math.sqrt((gk - 1.0) * (gk + 1.0))
+            epp[s, k] = ernh[s, k] * ekev[k]
+    g.ernh, g.epp = ernh, epp
 
     # ghost-cell factors, :580-581
     g.conf1 = ((lz[nr - 1] + dl1) / lz[nr - 1]) ** 2
