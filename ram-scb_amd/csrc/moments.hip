@@ -11,34 +11,41 @@
 
 namespace ramgpu {
 
-constexpr int kMomWaves = 4;  // (S,I,J) planes per block, one wave each
+constexpr int kMomThreads = 256;  // one (S,I,J) plane per block
+constexpr int kMomChunk = 512;    // cells of a plane staged per pass (whole rows of NE)
+constexpr int kMomMaxNE = 4 * kMomThreads;
 
-// One wave per (S,I,J) plane of F2 ([S][I][J][L][K], NE*NPA contiguous).
-// Lane k holds energy K = k+1 (NE > 64: further passes of 64) and walks L in
-// the reference's order, so SUME, SUMA, SUMN of each K are the reference's
-// sequential sums; lane 0 then adds the per-K terms in K order into the
-// energy windows K=2..6, 7..10, 11..25, 26..30, 31..NE (:297-298; each
+// rows of a staged pass and the dynamic LDS of k_moments (doubles)
+__host__ __device__ inline int mom_rows(int ne) { return ne < kMomChunk ? kMomChunk / ne : 1; }
+inline size_t mom_lds_doubles(int ne) { return 3 * (size_t)mom_rows(ne) * ne + 4 * (size_t)ne; }
+
+// One block per (S,I,J) plane of F2 ([S][I][J][L][K], NE*NPA contiguous),
+// in passes of mom_rows(NE) pitch-angle rows: every thread forms the terms
+// F2*EPME, F2*EPMA, F2*ERNM of its cells (and FLUX = F2/FFACTOR/FNHS) into
+// LDS; then the thread of each energy K adds its column's terms row by row,
+// so SUME, SUMA and SUMN are the reference's sequential sums over
+// L=1..int(UPA(I)-1).  Thread 0 finally adds the per-K terms in K order into
+// the energy windows K=2..6, 7..10, 11..25, 26..30, 31..NE (:297-298; each
 // window ends at min(khi, NE): the reference reads past NE when NE < 30).
-// FLUX = F2/FFACTOR/FNHS of the plane is written in the same pass, 0 outside
-// I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1, where the reference's FLUX keeps
-// the zero of its allocation (src/ModRamInit.f90:78).
+// FLUX is 0 outside I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1, where the
+// reference's FLUX keeps the zero of its allocation (src/ModRamInit.f90:78).
 // ff: FFACTOR as [s][i][l][k]; mom: PPERT, PPART, EDENT, RNHTT, each
-// (NS,NR,NT) in the reference's layout.
-__global__ void __launch_bounds__(64 * kMomWaves)
-    k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int nsp, int anisch,
+// (NS,NR,NT) in the reference's layout.  MAXK: energies per thread (NE <=
+// MAXK*kMomThreads).
+template <int MAXK>
+__global__ void __launch_bounds__(kMomThreads)
+    k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int anisch,
               const double *__restrict__ ff, const double *__restrict__ ernh, const double *__restrict__ epp,
               const double *__restrict__ ekev, const double *__restrict__ wmu, const double *__restrict__ mu,
               const double *__restrict__ fnhs, const int *__restrict__ upa, double *__restrict__ flux,
               double *__restrict__ mom) {
-  __shared__ double red[kMomWaves][4][64];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nplane = nsp * (nr - 1) * nt;
-  const int p = blockIdx.x * kMomWaves + wv;
-  const bool live = p < nplane;
-  const int pp = live ? p : 0;
-  const int j = pp % nt;
-  const int i = 1 + (pp / nt) % (nr - 1);  // I = 2..NR
-  const int s = s0 + pp / (nt * (nr - 1));
+  extern __shared__ double sm[];
+  const int p = blockIdx.x;
+  const int j = p % nt;
+  const int i = 1 + (p / nt) % (nr - 1);  // I = 2..NR
+  const int s = s0 + p / (nt * (nr - 1));
+  const int R = mom_rows(ne), C = R * ne;
+  double *pe = sm, *pa = sm + C, *pn = sm + 2 * C, *term = sm + 3 * C;
   const size_t P = (size_t)ne * npa;
   const size_t plane = (((size_t)s * nr + i) * nt + j) * P;
   double *F = f2 + plane;
@@ -49,76 +56,93 @@ __global__ void __launch_bounds__(64 * kMomWaves)
   const int u = upa[i] - 1;  // int(UPA(I)-1): UPA holds whole numbers
   const bool jflux = j <= nt - 2;
   const int lend = FX ? npa : (anisch ? std::min(u, npa) : 0);
+  double se[MAXK], sa[MAXK], sn[MAXK];
+#pragma unroll
+  for (int q = 0; q < MAXK; ++q) se[q] = sa[q] = sn[q] = 0.0;
+  for (int l0 = 0; l0 < lend; l0 += R) {
+    const int rows = std::min(R, lend - l0);
+    for (int c = threadIdx.x; c < rows * ne; c += kMomThreads) {
+      const int r = c / ne, k = c - r * ne, l = l0 + r;
+      const bool kin = k >= 1;  // K = 2..NE
+      double f;
+      if (l == 0 && anisch && kin) {
+        f = F[ne + k];  // F2(S,I,J,K,1) = F2(S,I,J,K,2)
+        F[k] = f;
+      } else {
+        f = F[(size_t)l * ne + k];
+      }
+      const double ffv = ffp[(size_t)l * ne + k], fnv = fn[fst * l];
+      double te = 0.0, ta = 0.0, tn = 0.0;
+      if (anisch && kin && l < u) {
+        const double ernm = wmu[l] / ffv / fnv;
+        const double epma = ernm * mu[l] * mu[l];
+        const double epme = ernm - epma;
+        te = f * epme;
+        ta = f * epma;
+        tn = f * ernm;
+      }
+      pe[c] = te;
+      pa[c] = ta;
+      pn[c] = tn;
+      if (FX) FX[(size_t)l * ne + k] = (kin && l >= 1 && jflux) ? f / ffv / fnv : 0.0;
+    }
+    __syncthreads();
+    if (anisch) {
+#pragma unroll
+      for (int q = 0; q < MAXK; ++q) {
+        const int k = threadIdx.x + q * kMomThreads;
+        if (k >= 1 && k < ne)
+          for (int r = 0; r < rows && l0 + r < u; ++r) {
+            se[q] = se[q] + pe[r * ne + k];
+            sa[q] = sa[q] + pa[r * ne + k];
+            sn[q] = sn[q] + pn[r * ne + k];
+          }
+      }
+    }
+    __syncthreads();
+  }
+  if (!anisch) return;
+#pragma unroll
+  for (int q = 0; q < MAXK; ++q) {
+    const int k = threadIdx.x + q * kMomThreads;
+    if (k >= 1 && k < ne) {
+      const double e = epp[s + (size_t)ns * k], h = ernh[s + (size_t)ns * k];
+      term[k] = e * se[q];                    // EPP(S,K)*SUME
+      term[ne + k] = e * sa[q];               // EPP(S,K)*SUMA
+      term[2 * ne + k] = h * sn[q];           // ERNH(S,K)*SUMN
+      term[3 * ne + k] = h * ekev[k] * sn[q]; // ERNH(S,K)*EKEV(K)*SUMN
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   constexpr double kRfac = 4 * 3.1415926535897932384626433832795 / (2.998E8 * 100);  // 4*PI/(CS*100)
-  // last K (1-based) of window w
-  auto khi = [ne](int w) { return std::min(w == 0 ? 6 : w == 1 ? 10 : w == 2 ? 25 : w == 3 ? 30 : ne, ne); };
-  int w = 0;
-  double win[4] = {0.0, 0.0, 0.0, 0.0}, tot[4] = {0.0, 0.0, 0.0, 0.0};
-  auto close_window = [&]() {
+  double tot[4] = {0.0, 0.0, 0.0, 0.0};
+  int k = 1;  // 0-based energy, K = k+1 from 2
+  for (int w = 0; w < 5; ++w) {
+    const int khi = std::min(w == 0 ? 6 : w == 1 ? 10 : w == 2 ? 25 : w == 3 ? 30 : ne, ne);
+    double win[4] = {0.0, 0.0, 0.0, 0.0};
+    for (; k + 1 <= khi; ++k) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) win[q] = win[q] + term[q * ne + k];
+    }
     // RNHT*RFAC, EDEN*RFAC, 2*RFAC*PPAR, RFAC*PPER, then the totals (:375-397)
     tot[0] = tot[0] + kRfac * win[0];
     tot[1] = tot[1] + (2 * kRfac) * win[1];
     tot[2] = tot[2] + win[3] * kRfac;
     tot[3] = tot[3] + win[2] * kRfac;
-    win[0] = win[1] = win[2] = win[3] = 0.0;
-  };
-  for (int k0 = 0; k0 < ne; k0 += 64) {
-    const int k = k0 + lane;
-    const bool on = live && k < ne;
-    const bool kin = on && k >= 1;  // K = 2..NE
-    double se = 0.0, sa = 0.0, sn = 0.0;
-    if (on) {
-      const double f2v = F[ne + k];  // F2(S,I,J,K,2)
-      if (anisch && kin) F[k] = f2v;  // F2(S,I,J,K,1) = F2(S,I,J,K,2)
-      for (int l = 0; l < lend; ++l) {
-        const double f = (l == 0 && anisch && kin) ? f2v : F[(size_t)l * ne + k];
-        const double ffv = ffp[(size_t)l * ne + k], fnv = fn[fst * l];
-        if (anisch && kin && l < u) {
-          const double ernm = wmu[l] / ffv / fnv;
-          const double epma = ernm * mu[l] * mu[l];
-          const double epme = ernm - epma;
-          se = se + f * epme;
-          sa = sa + f * epma;
-          sn = sn + f * ernm;
-        }
-        if (FX) FX[(size_t)l * ne + k] = (kin && l >= 1 && jflux) ? f / ffv / fnv : 0.0;
-      }
-    }
-    if (anisch) {
-      const double e = kin ? epp[s + (size_t)ns * k] : 0.0, h = kin ? ernh[s + (size_t)ns * k] : 0.0;
-      const double ek = kin ? ekev[k] : 0.0;
-      red[wv][0][lane] = e * se;       // EPP(S,K)*SUME
-      red[wv][1][lane] = e * sa;       // EPP(S,K)*SUMA
-      red[wv][2][lane] = h * sn;       // ERNH(S,K)*SUMN
-      red[wv][3][lane] = h * ek * sn;  // ERNH(S,K)*EKEV(K)*SUMN
-    }
-    __syncthreads();
-    if (anisch && live && lane == 0) {
-      const int kend = std::min(k0 + 64, ne);
-      for (int kk = std::max(k0, 1); kk < kend; ++kk) {
-        while (kk + 1 > khi(w)) {  // K = kk+1 is past window w
-          close_window();
-          ++w;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) win[q] = win[q] + red[wv][q][kk - k0];
-      }
-    }
-    __syncthreads();
   }
-  if (anisch && live && lane == 0) {
-    for (; w < 5; ++w) close_window();
-    const size_t n3 = (size_t)ns * nr * nt, x = s + (size_t)ns * (i + (size_t)nr * j);
+  const size_t n3 = (size_t)ns * nr * nt, x = s + (size_t)ns * (i + (size_t)nr * j);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) mom[q * n3 + x] = tot[q];
-  }
+  for (int q = 0; q < 4; ++q) mom[q * n3 + x] = tot[q];
 }
 
 }  // namespace ramgpu
 
 using namespace ramgpu_host;
 using ramgpu::k_moments;
-using ramgpu::kMomWaves;
+using ramgpu::kMomMaxNE;
+using ramgpu::kMomThreads;
+using ramgpu::mom_lds_doubles;
 
 namespace {
 
@@ -139,17 +163,26 @@ int run_moments(ramgpu_ctx *c, int s0, int nsp, int anisch, bool flux) {
   if (!c->have_geom) return fail(RAMGPU_ESTATE, "ramgpu_set_geometry has not been called");
   if (!c->have_f2) return fail(RAMGPU_ESTATE, "ramgpu_put_f2 has not been called");
   if (!c->d_ff) return fail(RAMGPU_ESTATE, "ramgpu_set_moment_factors has not been called");
+  if (c->ne > kMomMaxNE) return fail(RAMGPU_EINVAL, "moments: NE=%d above %d", c->ne, kMomMaxNE);
   int rc;
   if (flux && !c->d_flux && (rc = alloc_zero(c, &c->d_flux, (size_t)c->ns * c->nr * c->nt * c->ne * c->npa)))
     return rc;
   if (c->nr < 2) return RAMGPU_OK;
   const int nplane = nsp * (c->nr - 1) * c->nt;
+  const size_t lds = mom_lds_doubles(c->ne) * sizeof(double);
   tbeg(c, RAMGPU_KERNEL_MOMENTS);
-  hipLaunchKernelGGL(k_moments, dim3((nplane + kMomWaves - 1) / kMomWaves), dim3(64 * kMomWaves), 0, c->stream,
-                     c->t.f2, c->ns, c->nr, c->nt, c->ne, c->npa, s0, nsp, anisch, (const double *)c->d_ff,
-                     (const double *)c->d_ernh, (const double *)c->d_epp, (const double *)c->d_ekev,
-                     (const double *)c->d_wmu, (const double *)c->d_mu, (const double *)c->d_fnhs,
-                     (const int *)c->d_upa, flux ? c->d_flux : nullptr, c->d_mom);
+  if (c->ne <= kMomThreads)
+    hipLaunchKernelGGL(k_moments<1>, dim3(nplane), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
+                       c->nt, c->ne, c->npa, s0, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
+                       (const double *)c->d_epp, (const double *)c->d_ekev, (const double *)c->d_wmu,
+                       (const double *)c->d_mu, (const double *)c->d_fnhs, (const int *)c->d_upa,
+                       flux ? c->d_flux : nullptr, c->d_mom);
+  else
+    hipLaunchKernelGGL(k_moments<4>, dim3(nplane), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
+                       c->nt, c->ne, c->npa, s0, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
+                       (const double *)c->d_epp, (const double *)c->d_ekev, (const double *)c->d_wmu,
+                       (const double *)c->d_mu, (const double *)c->d_fnhs, (const int *)c->d_upa,
+                       flux ? c->d_flux : nullptr, c->d_mom);
   tend(c);
   return launch_check("k_moments");
 }

@@ -1,8 +1,8 @@
 """GPU parity of the moments (ANISCH, FLUX) through the C ABI.
 
-Each device moment is the reference's sum in the reference's order (one lane
-per energy walks the pitch angles in order; one lane adds the energies in
-order, window by window), so PPERT, PPART, EDENT, RNHTT, FLUX and the F2 row
+Each device moment is the reference's sum in the reference's order (the
+thread of each energy adds its pitch-angle terms row by row; one thread adds
+the energies in order, window by window), so PPERT, PPART, EDENT, RNHTT, FLUX and the F2 row
 ANISCH rewrites are compared with the oracle bit for bit on the same F2.
 """
 import numpy as np
@@ -48,11 +48,15 @@ def test_anisch_ne_below_30_and_random_geometry():
     _check_anisch(random_problem(seed=4, spiky=True), "random 8x9x7x10")
 
 
-def test_anisch_two_lane_passes():
-    # NE = 72 > 64: each wave takes the energies in two passes
+def test_anisch_wide_energy_grids():
+    # NE = 72: five staged passes of 7 rows; NE = 300: one row per pass and
+    # more energies than threads in a block
     p = inputs.make_problem(nr=6, nt=5, ne=72, npa=40, species=inputs.SPECIES4[1:3],
                             composition=(0.7, 0.05), seed=7)
     _check_anisch(p, "H+,He+ 6x5x72x40")
+    p = inputs.make_problem(nr=5, nt=4, ne=300, npa=8, species=inputs.SPECIES4[1:2], composition=(0.7,),
+                            seed=8)
+    _check_anisch(p, "H+ 5x4x300x8")
 
 
 def test_flux_matches_oracle():
