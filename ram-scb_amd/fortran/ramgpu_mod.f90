@@ -187,6 +187,33 @@ module ramgpu_mod
        real(c_double), intent(out) :: setrc, elorc
      end function ramgpu_sumrc
 
+     integer(c_int) function ramgpu_set_moment_factors(ctx, ffactor, ernh, epp) &
+          bind(C, name="ramgpu_set_moment_factors")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: ffactor(*), ernh(*), epp(*)
+     end function ramgpu_set_moment_factors
+
+     integer(c_int) function ramgpu_anisch(ctx, s, ppert, ppart, edent, rnhtt) bind(C, name="ramgpu_anisch")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(inout) :: ppert(*), ppart(*), edent(*), rnhtt(*)
+     end function ramgpu_anisch
+
+     integer(c_int) function ramgpu_flux(ctx, flux) bind(C, name="ramgpu_flux")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: flux(*)
+     end function ramgpu_flux
+
+     integer(c_int) function ramgpu_moments(ctx, ppert, ppart, edent, rnhtt, flux) bind(C, name="ramgpu_moments")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(inout) :: ppert(*), ppart(*), edent(*), rnhtt(*)
+       real(c_double), intent(out), optional :: flux(*)  ! absent: NULL, no FLUX
+     end function ramgpu_moments
+
      integer(c_int) function ramgpu_ram_run(ctx, nsteps, out) bind(C, name="ramgpu_ram_run")
        import :: c_int, c_ptr, ramgpu_species_out
        type(c_ptr), value :: ctx
