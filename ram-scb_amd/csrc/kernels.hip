@@ -811,7 +811,7 @@ template <int MAXC>
 __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   if (rec) {
-    for (int l = threadIdx.x; l < NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
+    for (int l = threadIdx.x; l < kMuSpread * NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
     __syncthreads();
   }
   const double *k8 = d.pc + PC_KIJ;
@@ -829,7 +829,9 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
     // CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0; row NPA is
     // the top cell).  Reads past the plane land in H and are masked.
     const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
-    if (rec && L1 >= 2) atomic_max_pos(&d.cmx[NE + l], fmax(fabs(cm), 1E-32));
+    // the row's maximum in kMuSpread slots (by K), so that fewer lanes of a
+    // wave update one LDS address
+    if (rec && L1 >= 2) atomic_max_pos(&d.cmx[NE + kMuSpread * l + (k & (kMuSpread - 1))], fmax(fabs(cm), 1E-32));
     const bool neg = cm < 0;
     const double f0 = d.F[cc];
     const double fp1 = d.F[cc + NE];
@@ -863,7 +865,11 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   }
   if (rec) {
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
-    for (int ll = 1 + threadIdx.x; ll < NPA; ll += blockDim.x) atomicMax(&g[2 + NE + ll], d.cmx[NE + ll]);
+    for (int ll = 1 + threadIdx.x; ll < NPA; ll += blockDim.x) {
+      unsigned long long m = d.cmx[NE + kMuSpread * ll];
+      for (int q = 1; q < kMuSpread; ++q) m = max(m, d.cmx[NE + kMuSpread * ll + q]);
+      atomicMax(&g[2 + NE + ll], m);
+    }
   }
   __syncthreads();
 }

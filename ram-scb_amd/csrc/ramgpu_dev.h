@@ -28,6 +28,7 @@ constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chos
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bound, checked at creation
 constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
+constexpr int kMuSpread = 8;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 8)
 constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 
 // packed per-(I,J) terms of the plane sweeps, kij[(i*NT+j)*8 + ..]
@@ -175,7 +176,7 @@ inline __host__ __device__ size_t radial_lds_bytes(int nr, int ne, int npa) {
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
   const size_t P = (size_t)ne * npa;
   return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
-          2 * (size_t)npa + (size_t)ne + npa) * sizeof(double);
+          2 * (size_t)npa + (size_t)ne + (size_t)kMuSpread * npa) * sizeof(double);
 }
 
 }  // namespace ramgpu

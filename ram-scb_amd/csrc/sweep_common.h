@@ -440,7 +440,7 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.ex = d.red + 128;        // 64: 2^(j/64) of exp_tab
   d.pc = d.ex + 64;          // PC_N: the plane's scalars (PcField)
   d.gh = d.pc + PC_N;        // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
-  d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE+NPA
+  d.cmx = (unsigned long long *)(d.gh + 2 * npa);  // NE + kMuSpread*NPA
   return d;
 }
 
