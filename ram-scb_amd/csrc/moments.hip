@@ -33,7 +33,7 @@ inline size_t mom_lds_doubles(int ne) { return 3 * (size_t)mom_rows(ne) * ne + 4
 // (NS,NR,NT) in the reference's layout.  MAXK: energies per thread (NE <=
 // MAXK*kMomThreads).
 template <int MAXK>
-__global__ void __launch_bounds__(kMomThreads)
+__global__ void __launch_bounds__(kMomThreads, MAXK == 1 ? 8 : 4)
     k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int anisch,
               const double *__restrict__ ff, const double *__restrict__ ernh, const double *__restrict__ epp,
               const double *__restrict__ ekev, const double *__restrict__ wmu, const double *__restrict__ mu,
@@ -59,10 +59,22 @@ __global__ void __launch_bounds__(kMomThreads)
   double se[MAXK], sa[MAXK], sn[MAXK];
 #pragma unroll
   for (int q = 0; q < MAXK; ++q) se[q] = sa[q] = sn[q] = 0.0;
+  // a thread's cells of a pass (R*NE <= CPT*kMomThreads)
+  constexpr int CPT = MAXK == 1 ? 2 : 4;
+  int cr[CPT], ck[CPT];  // row in the pass and energy of each cell (the same in every pass)
+#pragma unroll
+  for (int q = 0; q < CPT; ++q) {
+    const int c = threadIdx.x + q * kMomThreads;
+    cr[q] = c / ne;
+    ck[q] = c - cr[q] * ne;
+  }
   for (int l0 = 0; l0 < lend; l0 += R) {
     const int rows = std::min(R, lend - l0);
-    for (int c = threadIdx.x; c < rows * ne; c += kMomThreads) {
-      const int r = c / ne, k = c - r * ne, l = l0 + r;
+#pragma unroll
+    for (int q = 0; q < CPT; ++q) {
+      const int c = threadIdx.x + q * kMomThreads;
+      if (c >= rows * ne) continue;
+      const int k = ck[q], l = l0 + cr[q];
       const bool kin = k >= 1;  // K = 2..NE
       double f;
       if (l == 0 && anisch && kin) {
@@ -91,8 +103,11 @@ __global__ void __launch_bounds__(kMomThreads)
 #pragma unroll
       for (int q = 0; q < MAXK; ++q) {
         const int k = threadIdx.x + q * kMomThreads;
+        // rows at L > int(UPA(I)-1) hold +0 terms: adding +0 leaves a sum
+        // that starts at +0 unchanged, so the whole pass is added
         if (k >= 1 && k < ne)
-          for (int r = 0; r < rows && l0 + r < u; ++r) {
+#pragma unroll 4
+          for (int r = 0; r < rows; ++r) {
             se[q] = se[q] + pe[r * ne + k];
             sa[q] = sa[q] + pa[r * ne + k];
             sn[q] = sn[q] + pn[r * ne + k];
@@ -114,22 +129,29 @@ __global__ void __launch_bounds__(kMomThreads)
     }
   }
   __syncthreads();
+  // thread 4w+q adds quantity q over energy window w in K order (a window's
+  // own sum, from 0, as the reference's PPER, PPAR, RNHT, EDEN); thread 0
+  // then scales the windows and adds them in order (:375-397)
+  double *wsum = pe;  // the passes' terms are no longer needed
+  if (threadIdx.x < 20) {
+    const int q = threadIdx.x & 3, w = threadIdx.x >> 2;
+    auto khi = [ne](int m) { return std::min(m == 0 ? 6 : m == 1 ? 10 : m == 2 ? 25 : m == 3 ? 30 : ne, ne); };
+    const int klo = w == 0 ? 2 : khi(w - 1) + 1;  // 1-based
+    double v = 0.0;
+    for (int K = klo; K <= khi(w); ++K) v = v + term[q * ne + K - 1];
+    wsum[threadIdx.x] = v;
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
   constexpr double kRfac = 4 * 3.1415926535897932384626433832795 / (2.998E8 * 100);  // 4*PI/(CS*100)
   double tot[4] = {0.0, 0.0, 0.0, 0.0};
-  int k = 1;  // 0-based energy, K = k+1 from 2
-  for (int w = 0; w < 5; ++w) {
-    const int khi = std::min(w == 0 ? 6 : w == 1 ? 10 : w == 2 ? 25 : w == 3 ? 30 : ne, ne);
-    double win[4] = {0.0, 0.0, 0.0, 0.0};
-    for (; k + 1 <= khi; ++k) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) win[q] = win[q] + term[q * ne + k];
-    }
-    // RNHT*RFAC, EDEN*RFAC, 2*RFAC*PPAR, RFAC*PPER, then the totals (:375-397)
-    tot[0] = tot[0] + kRfac * win[0];
-    tot[1] = tot[1] + (2 * kRfac) * win[1];
-    tot[2] = tot[2] + win[3] * kRfac;
-    tot[3] = tot[3] + win[2] * kRfac;
+  for (int w = 0; w < 5; ++w) {
+    // RNHT*RFAC, EDEN*RFAC, 2*RFAC*PPAR, RFAC*PPER, then the totals
+    tot[0] = tot[0] + kRfac * wsum[4 * w];
+    tot[1] = tot[1] + (2 * kRfac) * wsum[4 * w + 1];
+    tot[2] = tot[2] + wsum[4 * w + 3] * kRfac;
+    tot[3] = tot[3] + wsum[4 * w + 2] * kRfac;
   }
   const size_t n3 = (size_t)ns * nr * nt, x = s + (size_t)ns * (i + (size_t)nr * j);
 #pragma unroll
