@@ -624,7 +624,7 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
     PlaneLds d;
     constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP;
     plane_setup<4>(t, u, sm, (SEQ == SEQ_STEP) ? x0 + 4 * ns : x0, need_loss, need_kl, c, d);
-    double *Y = (double *)(d.cmx + c.NE + kMuSpread * c.NPA);
+    double *Y = (double *)(d.cmx + c.NE + c.NPA);  // one CFL slot per row (plane_bwd_lds_bytes)
     double *ga = (adj ? adj : t.lam) + po + c.ij * c.P;
     block_copy<4>(Y, ga, c.P);
     __syncthreads();

@@ -114,7 +114,7 @@ int copy_d2d(ramgpu_ctx *c, double *dst, const double *src, size_t n) {
 }
 
 size_t bwd_lds(const ramgpu_ctx *c) {
-  return plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double);
+  return plane_bwd_lds_bytes(c->ne, c->npa);
 }
 
 // backward kernels over species s0..s0+nsp-1

@@ -494,7 +494,7 @@ int ramgpu_batch_adjoint_run(ramgpu_batch *b, int nsteps, int ckpt_every, const 
   size_t bwd_lds = 0, lds_limit = b->m[0]->lds_limit;
   int bwd_threads = kPlaneThreads;
   for (ramgpu_ctx *c : b->m) {
-    bwd_lds = std::max(bwd_lds, plane_lds_bytes(c->ne, c->npa) + (size_t)c->ne * c->npa * sizeof(double));
+    bwd_lds = std::max(bwd_lds, plane_bwd_lds_bytes(c->ne, c->npa));
     bwd_threads = std::max(bwd_threads, plane_bwd_threads(c->ne, c->npa));
   }
   if (bwd_lds > lds_limit) return fail(RAMGPU_EINVAL, "plane adjoint does not fit the device LDS");

@@ -173,10 +173,16 @@ inline __host__ __device__ size_t radial_lds_bytes(int nr, int ne, int npa) {
 }
 
 // LDS bytes of the plane kernel
-inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa) {
+// LDS of the forward plane kernels; mu_slots: LDS slots per row of DRIFTMU's
+// CFL maximum (the adjoint kernels record none and lay out one, followed by
+// their adjoint plane: plane_bwd_lds_bytes)
+inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa, int mu_slots = kMuSpread) {
   const size_t P = (size_t)ne * npa;
   return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
-          2 * (size_t)npa + (size_t)ne + (size_t)kMuSpread * npa) * sizeof(double);
+          2 * (size_t)npa + (size_t)ne + (size_t)mu_slots * npa) * sizeof(double);
+}
+inline __host__ __device__ size_t plane_bwd_lds_bytes(int ne, int npa) {
+  return plane_lds_bytes(ne, npa, 1) + (size_t)ne * npa * sizeof(double);
 }
 
 }  // namespace ramgpu
