@@ -89,7 +89,16 @@ def test_operator_vjp_matches_oracle_fd(op, seed):
 
 @pytest.mark.parametrize("seed", [0, 1])
 def test_ram_run_step_vjp_matches_oracle_fd(seed):
-    p = random_problem(seed=40 + seed, species=(sp.HYDROGEN, sp.ELECTRON), spiky=(seed == 1))
+    _step_vjp(random_problem(seed=40 + seed, species=(sp.HYDROGEN, sp.ELECTRON), spiky=(seed == 1)), seed)
+
+
+def test_ram_run_step_vjp_on_64x90_planes():
+    # the ensemble-adjoint members' planes: the plane adjoint kernels hold the
+    # forward's plane layout and the adjoint plane in LDS, their largest use
+    _step_vjp(random_problem(nr=4, nt=4, ne=64, npa=90, seed=47, species=(sp.HYDROGEN,)), 0)
+
+
+def _step_vjp(p, seed):
     x = _state(p, seed)
     rng = np.random.default_rng(10 + seed)
     v = np.asfortranarray(x * rng.standard_normal(x.shape))
@@ -102,17 +111,24 @@ def test_ram_run_step_vjp_matches_oracle_fd(seed):
         o.set_fgeos(fgeos)
         o.ram_run(2)
         return o.f2
-    jv = (step(x + H * v, p.fgeos) - step(x - H * v, p.fgeos)) / (2 * H)
-    jd = (step(x, p.fgeos + H * d) - step(x, p.fgeos - H * d)) / (2 * H)
     g = RamGpu.from_problem(p)
     g.put_f2(x)
     xbar, fbar = g.ram_run_bwd(ybar)
     assert np.array_equal(g.get_f2(), x), "ram_run_bwd must leave F2 unchanged"
-    _check(f"ram_run step seed={seed} (F2)", float(np.sum(ybar * jv)), float(np.sum(xbar * v)),
-           np.linalg.norm(ybar) * np.linalg.norm(jv))
-    _check(f"ram_run step seed={seed} (FGEOS)", float(np.sum(ybar * jd)), float(np.sum(fbar * d)),
-           np.linalg.norm(ybar) * np.linalg.norm(jd))
     g.close()
+    # the step is piecewise linear: a central difference whose interval
+    # holds a limiter kink of some cell is off by that cell's jump, so a
+    # direction that misses at h is checked again at h/10
+    for what, dx, df, bar in (("F2", v, 0 * d, xbar), ("FGEOS", 0 * v, d, fbar)):
+        dirn = v if what == "F2" else d
+        for h in (H, H / 10):
+            jv = (step(x + h * dx, p.fgeos + h * df) - step(x - h * dx, p.fgeos - h * df)) / (2 * h)
+            lhs, rhs = float(np.sum(ybar * jv)), float(np.sum(bar * dirn))
+            err = abs(lhs - rhs) / (np.linalg.norm(ybar) * np.linalg.norm(jv))
+            print(f"ram_run step seed={seed} ({what}) h={h:.0e}: rel {err:.2e}", flush=True)
+            if err <= TOL:
+                break
+        assert err <= TOL, (what, err)
 
 
 def fd_noise(loss, h=H):
