@@ -61,6 +61,10 @@ KERNEL_NAMES = {"r": "k_r (radial sweep, DRIFTR)", "p": "k_p (azimuthal sweep, D
 ALGO_BYTES_PER_CELL_UPDATE = 128  # 8 directional sweeps x (read + write) of F2
 METRIC = "FP64 RAM split-step cell-updates/s + HBM roofline %; batched & adjoint vs plain"
 TRAFFIC_FILE = REPO / "profiles" / "traffic_multi_species.txt"
+SQ_FILE = REPO / "profiles" / "sq_counters_multi_species.txt"
+# MI355X vector issue: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one
+# wave64 VALU instruction (FP64 included) per 4 cycles per SIMD
+VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 def log(*a):
@@ -82,6 +86,20 @@ def measured_traffic():
     for line in TRAFFIC_FILE.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch="):
             return float(line.split("=", 1)[1])
+    return None
+
+
+def measured_valu(kernel: str = "ramgpu::k_plane<5, 8>"):
+    """SQ_INSTS_VALU (wave instructions per launch) of a kernel, from the
+    committed rocprofv3 counter summary."""
+    if not SQ_FILE.exists():
+        return None
+    cur = None
+    for line in SQ_FILE.read_text().splitlines():
+        if line and not line.startswith((" ", "#")):
+            cur = line.strip()
+        elif cur == kernel and line.strip().startswith("SQ_INSTS_VALU "):
+            return float(line.split()[1])
     return None
 
 
@@ -378,6 +396,11 @@ def main():
                 "frac": d["achieved_GBps"] / HBM_PEAK_GBPS, "traffic": measured_traffic(),
                 "kernel": KERNEL_NAMES[dom], "bytes_per_launch": d["bytes_per_launch"],
                 "avg_launch_us": d["avg_launch_us"], "per_kernel": per_kernel}
+    valu = measured_valu() if dom == "plane" else None
+    if valu:
+        # the plane kernel is bound by FP64 vector issue, not HBM: its VALU
+        # wave instructions per launch over the issue rate, against the launch
+        roofline["valu_issue_frac"] = valu / VALU_ISSUE_PER_S / (d["avg_launch_us"] * 1e-6)
     out = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
