@@ -180,12 +180,23 @@ int prepare(ramgpu_ctx *c) {
     // if pending, and the ghost carry, in one launch
     tbeg(c, RAMGPU_KERNEL_PREPARE);
     const int nvt = c->step_kp ? c->nr + 1 + c->nt : 0;
-    const size_t lds = (size_t)(nvt + 4 * c->nt) * sizeof(double);
+    size_t lds = (size_t)(nvt + 4 * c->nt) * sizeof(double);
+    // the ghost's q1 row in LDS where it fits the default 64 KB with the
+    // kernel's static LDS
+    static size_t head_static = 0;
+    if (!head_static) {
+      hipFuncAttributes fa{};
+      if (hipFuncGetAttributes(&fa, (const void *)k_step_head) == hipSuccess) head_static = fa.sharedSizeBytes;
+      (void)hipGetLastError();
+    }
+    const size_t q1b = (size_t)c->npa * c->nt * sizeof(double);
+    const int stage_q1 = head_static && head_static + lds + q1b <= 65536;
+    if (stage_q1) lds += q1b;
     const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
     const Launch lf = launch_of(c, 0, c->fin_mask >= 0 ? c->ns : 0, 1);
     const Launch lg = launch_of(c, 0, c->ns, 1);
     hipLaunchKernelGGL(k_step_head, dim3(lc.nv + lf.nv + lg.nv), dim3(kFinalizeThreads), lds, c->stream, lc,
-                       c->step_kp, c->step_bs, lf, c->fin_mask, c->fin_mode, lg, nvt);
+                       c->step_kp, c->step_bs, lf, c->fin_mask, c->fin_mode, lg, nvt, stage_q1);
     c->fin_mask = -1;
     c->step_kp = c->step_bs = nullptr;
     tend(c);

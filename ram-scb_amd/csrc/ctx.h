@@ -28,7 +28,7 @@ __global__ void k_fill(double *y, double v, size_t n);
 __global__ void k_species(Launch L);
 __global__ void k_coef_ij(Launch L, const double *kp, const double *bs);
 __global__ void k_step_head(Launch Lc, const double *kp, const double *bs, Launch Lf, int cflmask, int summode,
-                            Launch Lg, int nvt);
+                            Launch Lg, int nvt, int stage_q1);
 __global__ void k_coef(Launch L);
 __global__ void k_ghost(Launch L);
 template <int MODE> __global__ void k_r(Launch L);

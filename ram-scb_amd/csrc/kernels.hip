@@ -386,14 +386,21 @@ __global__ void k_coef(Launch L) {
 // One (problem, species) of the ghost carry.  The face-NR terms it tests
 // (rij(NR-1,.), rsi(NR-1)) are formed here with the table's own code, so the
 // carry does not wait for k_coef_ij; rrow holds them (4*NT doubles), svt the
-// VT factors when kp is given.
+// VT factors when kp is given.  q1s (NPA*NT doubles, or null): LDS for the
+// face-NR row of q1, loaded in one pass so that the search for each line's
+// last inflow J reads no global memory.
 __device__ inline void ghost_unit(const Launch &L, int vb, const double *kp, double avs, double *svt, double *rrow,
-                                  int *s_last, int *s_wave) {
+                                  int *s_last, int *s_wave, double *q1s = nullptr) {
   const Unit u = unit_of(L, vb);
   const Tab &t = *u.t;
   const int s = u.s, NR = t.nr, NT = t.nt, NPA = t.npa;
   const int P = t.ne * NPA, T = blockDim.x, tid = threadIdx.x;
   if (kp) stage_vt_factors(t, svt);
+  if (q1s)
+    for (int x = tid; x < NPA * NT; x += T) {
+      const int l = x / NT, j = x - l * NT;
+      q1s[x] = t.q1[((size_t)l * NR + NR - 1) * NT + j];
+    }
   for (int j = tid; j < NT; j += T) {
     const double4 r = radial_rij(t, NR - 1, j, kp, avs, svt);
     rrow[4 * j] = r.x;
@@ -406,21 +413,24 @@ __device__ inline void ghost_unit(const Launch &L, int vb, const double *kp, dou
   for (int p = tid; p < P; p += T) {
     const int k = p / NPA, l = p - k * NPA;
     const double p4k = t.p4[s * t.ne + k];
-    const gptr<const double> q1 = as_global(t.q1 + ((size_t)l * NR + NR - 1) * NT);
-    int last = -1;
-    for (int j0 = NT - 1; j0 >= 0 && last < 0; j0 -= 4) {
-      double c[4];
+    // the highest inflow J of the line, searched down from NT in groups of 4
+    auto search = [&](auto q1) {
+      int last = -1;
+      for (int j0 = NT - 1; j0 >= 0 && last < 0; j0 -= 4) {
+        double c[4];
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        const int j = max(j0 - m, 0);
-        c[m] = c_radial_of(make_double4(rrow[4 * j], rrow[4 * j + 1], rrow[4 * j + 2], rrow[4 * j + 3]), rs, q1[j],
-                           p4k);
+        for (int m = 0; m < 4; ++m) {
+          const int j = max(j0 - m, 0);
+          c[m] = c_radial_of(make_double4(rrow[4 * j], rrow[4 * j + 1], rrow[4 * j + 2], rrow[4 * j + 3]), rs, q1[j],
+                             p4k);
+        }
+#pragma unroll
+        for (int m = 3; m >= 0; --m)
+          if (j0 - m >= 0 && c[m] < 0) last = p * NT + j0 - m;
       }
-#pragma unroll
-      for (int m = 3; m >= 0; --m)
-        if (j0 - m >= 0 && c[m] < 0) last = p * NT + j0 - m;  // the highest inflow J of the group
-    }
-    s_last[p] = last;
+      return last;
+    };
+    s_last[p] = q1s ? search(q1s + (size_t)l * NT) : search(as_global(t.q1 + ((size_t)l * NR + NR - 1) * NT));
   }
   __syncthreads();
   // planes [p0, p1) of this thread, in order
@@ -1162,10 +1172,11 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_finalize(Launch L, int cfl
 // the last Lg.nv form the step's ghost carry (k_ghost).  The parts touch
 // disjoint data.  Blocks of kFinalizeThreads, as k_finalize's and
 // k_ghost's, so the sums and scans are theirs.  Dynamic LDS: the VT factors
-// (NR+1+NT) then the ghost's face row (4*NT).
+// (NR+1+NT), the ghost's face row (4*NT), then (stage_q1) the face-NR row of
+// q1 (NPA*NT).
 __global__ void __launch_bounds__(kFinalizeThreads) k_step_head(Launch Lc, const double *kp, const double *bs,
                                                                 Launch Lf, int cflmask, int summode, Launch Lg,
-                                                                int nvt) {
+                                                                int nvt, int stage_q1) {
   extern __shared__ double dyn[];
   __shared__ double red[(kFinalizeThreads / 64) * kFinNV];
   __shared__ int s_last[kMaxPlaneCells];
@@ -1177,7 +1188,8 @@ __global__ void __launch_bounds__(kFinalizeThreads) k_step_head(Launch Lc, const
   } else if ((b -= Lc.nv) < Lf.nv) {
     finalize_unit(Lf, b, cflmask, summode, red);
   } else {
-    ghost_unit(Lg, b - Lf.nv, kp, avs, dyn, dyn + nvt, s_last, s_wave);
+    const int nt = Lg.tabs[0].nt;
+    ghost_unit(Lg, b - Lf.nv, kp, avs, dyn, dyn + nvt, s_last, s_wave, stage_q1 ? dyn + nvt + 4 * nt : nullptr);
   }
 }
 
