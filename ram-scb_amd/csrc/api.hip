@@ -652,6 +652,7 @@ int ramgpu_set_grid(ramgpu_ctx *c, double mdr, double dphi, double conf1, double
   c->t.dphi = dphi;
   c->t.conf1 = conf1;
   c->t.conf2 = conf2;
+  c->mom_dirty = true;  // the moments' WMU/FFACTOR table
   c->have_grid = true;
   c->tables_dirty = true;
   c->tab_dirty = true;

@@ -74,6 +74,7 @@ struct ramgpu_ctx {
   // PPERT/PPART/EDENT/RNHTT (4 x (NS,NR,NT)), FLUX in F2's layout; allocated
   // by their first use
   double *d_ff = nullptr, *d_ernh = nullptr, *d_epp = nullptr, *d_mom = nullptr, *d_flux = nullptr;
+  bool mom_dirty = true;  // d_ff's quotient tables (RN(1/FFACTOR), RN(WMU/FFACTOR)) to be formed
   bool have_grid = false, have_geom = false, have_vt = false, have_bnd = false, have_f2 = false;
   bool have_species[kMaxSpecies] = {};
   bool have_lzphi = false;

@@ -15,9 +15,24 @@ constexpr int kMomThreads = 256;  // one (S,I,J) plane per block
 constexpr int kMomChunk = 512;    // cells of a plane staged per pass (whole rows of NE)
 constexpr int kMomMaxNE = 4 * kMomThreads;
 
-// rows of a staged pass and the dynamic LDS of k_moments (doubles)
+// rows of a staged pass and the dynamic LDS of k_moments (doubles): the
+// pass's terms, the per-K terms, then FNHS(I,J,L) and its reciprocal
 __host__ __device__ inline int mom_rows(int ne) { return ne < kMomChunk ? kMomChunk / ne : 1; }
-inline size_t mom_lds_doubles(int ne) { return 3 * (size_t)mom_rows(ne) * ne + 4 * (size_t)ne; }
+inline size_t mom_lds_doubles(int ne, int npa) {
+  return 3 * (size_t)mom_rows(ne) * ne + 4 * (size_t)ne + 2 * (size_t)npa;
+}
+
+// The quotient tables beside FFACTOR (each n doubles, [s][i][l][k]):
+// RN(1/FFACTOR) and RN(WMU(L)/FFACTOR), so that the kernel's divisions are
+// div_rn's exact reciprocal form: ERNM = RN(RN(WMU/FFACTOR)/FNHS) and
+// FLUX = RN(RN(F2/FFACTOR)/FNHS) as the reference's two divisions.
+__global__ void k_mom_tables(double *ff, const double *__restrict__ wmu, int ne, int npa, size_t n) {
+  for (size_t x = blockIdx.x * (size_t)blockDim.x + threadIdx.x; x < n; x += (size_t)gridDim.x * blockDim.x) {
+    const int l = (int)((x / ne) % npa);
+    ff[n + x] = 1.0 / ff[x];
+    ff[2 * n + x] = wmu[l] / ff[x];
+  }
+}
 
 // One block per (S,I,J) plane of F2 ([S][I][J][L][K], NE*NPA contiguous),
 // in passes of mom_rows(NE) pitch-angle rows: every thread forms the terms
@@ -29,7 +44,7 @@ inline size_t mom_lds_doubles(int ne) { return 3 * (size_t)mom_rows(ne) * ne + 4
 // window ends at min(khi, NE): the reference reads past NE when NE < 30).
 // FLUX is 0 outside I=2..NR, K=2..NE, L=2..NPA, J=1..NT-1, where the
 // reference's FLUX keeps the zero of its allocation (src/ModRamInit.f90:78).
-// ff: FFACTOR as [s][i][l][k]; mom: PPERT, PPART, EDENT, RNHTT, each
+// ff: FFACTOR as [s][i][l][k], then its k_mom_tables quotients; mom: PPERT, PPART, EDENT, RNHTT, each
 // (NS,NR,NT) in the reference's layout.  MAXK: energies per thread (NE <=
 // MAXK*kMomThreads).
 template <int MAXK>
@@ -46,13 +61,20 @@ __global__ void __launch_bounds__(kMomThreads, MAXK == 1 ? 8 : 4)
   const int s = s0 + p / (nt * (nr - 1));
   const int R = mom_rows(ne), C = R * ne;
   double *pe = sm, *pa = sm + C, *pn = sm + 2 * C, *term = sm + 3 * C;
-  const size_t P = (size_t)ne * npa;
+  double *fnl = term + 4 * ne, *rfl = fnl + npa;  // FNHS(I,J,L), RN(1/FNHS(I,J,L))
+  const size_t P = (size_t)ne * npa, nff = (size_t)ns * nr * P;
   const size_t plane = (((size_t)s * nr + i) * nt + j) * P;
   double *F = f2 + plane;
   double *FX = flux ? flux + plane : nullptr;
-  const double *ffp = ff + ((size_t)s * nr + i) * P;
+  const double *ffp = ff + ((size_t)s * nr + i) * P, *rffp = ffp + nff, *wfp = ffp + 2 * nff;
   const double *fn = fnhs + i + (size_t)(nr + 1) * j;  // FNHS(I,J,L) = fn[fst*(L-1)]
   const size_t fst = (size_t)(nr + 1) * nt;
+  for (int l = threadIdx.x; l < npa; l += kMomThreads) {
+    const double v = fn[fst * l];
+    fnl[l] = v;
+    rfl[l] = 1.0 / v;
+  }
+  __syncthreads();
   const int u = upa[i] - 1;  // int(UPA(I)-1): UPA holds whole numbers
   const bool jflux = j <= nt - 2;
   const int lend = FX ? npa : (anisch ? std::min(u, npa) : 0);
@@ -83,10 +105,10 @@ __global__ void __launch_bounds__(kMomThreads, MAXK == 1 ? 8 : 4)
       } else {
         f = F[(size_t)l * ne + k];
       }
-      const double ffv = ffp[(size_t)l * ne + k], fnv = fn[fst * l];
+      const double fnv = fnl[l], rfnv = rfl[l];
       double te = 0.0, ta = 0.0, tn = 0.0;
       if (anisch && kin && l < u) {
-        const double ernm = wmu[l] / ffv / fnv;
+        const double ernm = div_rn(wfp[(size_t)l * ne + k], fnv, rfnv);  // WMU/FFACTOR/FNHS
         const double epma = ernm * mu[l] * mu[l];
         const double epme = ernm - epma;
         te = f * epme;
@@ -96,7 +118,10 @@ __global__ void __launch_bounds__(kMomThreads, MAXK == 1 ? 8 : 4)
       pe[c] = te;
       pa[c] = ta;
       pn[c] = tn;
-      if (FX) FX[(size_t)l * ne + k] = (kin && l >= 1 && jflux) ? f / ffv / fnv : 0.0;
+      if (FX) {
+        const size_t x = (size_t)l * ne + k;
+        FX[x] = (kin && l >= 1 && jflux) ? div_rn(div_rn(f, ffp[x], rffp[x]), fnv, rfnv) : 0.0;
+      }
     }
     __syncthreads();
     if (anisch) {
@@ -189,9 +214,16 @@ int run_moments(ramgpu_ctx *c, int s0, int nsp, int anisch, bool flux) {
   int rc;
   if (flux && !c->d_flux && (rc = alloc_zero(c, &c->d_flux, (size_t)c->ns * c->nr * c->nt * c->ne * c->npa)))
     return rc;
+  if (c->mom_dirty) {  // the quotient tables, after new factors or a new grid (WMU)
+    const size_t n = (size_t)c->ns * c->nr * c->ne * c->npa;
+    hipLaunchKernelGGL(ramgpu::k_mom_tables, dim3(256), dim3(256), 0, c->stream, c->d_ff,
+                       (const double *)c->d_wmu, c->ne, c->npa, n);
+    if ((rc = launch_check("k_mom_tables"))) return rc;
+    c->mom_dirty = false;
+  }
   if (c->nr < 2) return RAMGPU_OK;
   const int nplane = nsp * (c->nr - 1) * c->nt;
-  const size_t lds = mom_lds_doubles(c->ne) * sizeof(double);
+  const size_t lds = mom_lds_doubles(c->ne, c->npa) * sizeof(double);
   tbeg(c, RAMGPU_KERNEL_MOMENTS);
   if (c->ne <= kMomThreads)
     hipLaunchKernelGGL(k_moments<1>, dim3(nplane), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
@@ -238,7 +270,7 @@ int ramgpu_set_moment_factors(ramgpu_ctx *c, const double *ffactor, const double
   if (!ffactor || !ernh || !epp) return fail(RAMGPU_EINVAL, "ffactor, ernh and epp must not be null");
   const size_t ns = c->ns, nr = c->nr, ne = c->ne, npa = c->npa;
   if (!c->d_ff) {
-    if ((rc = alloc_zero(c, &c->d_ff, ns * nr * ne * npa)) || (rc = alloc_zero(c, &c->d_ernh, ns * ne)) ||
+    if ((rc = alloc_zero(c, &c->d_ff, 3 * ns * nr * ne * npa)) || (rc = alloc_zero(c, &c->d_ernh, ns * ne)) ||
         (rc = alloc_zero(c, &c->d_epp, ns * ne)) || (rc = alloc_zero(c, &c->d_mom, 4 * ns * nr * c->nt)))
       return rc;
   }
@@ -253,6 +285,7 @@ int ramgpu_set_moment_factors(ramgpu_ctx *c, const double *ffactor, const double
   HIP_TRY(hipMemcpyAsync(c->d_ernh, ernh, ns * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->d_epp, epp, ns * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  c->mom_dirty = true;
   return RAMGPU_OK;
 }
 
