@@ -14,6 +14,7 @@ namespace ramgpu {
 constexpr int kMomThreads = 256;  // one (S,I,J) plane per block
 constexpr int kMomChunk = 512;    // cells of a plane staged per pass (whole rows of NE)
 constexpr int kMomMaxNE = 4 * kMomThreads;
+constexpr int kXcds = 8;          // MI355X: blocks go round-robin to 8 XCDs, each with its own L2
 
 // rows of a staged pass and the dynamic LDS of k_moments (doubles): the
 // pass's terms, the per-K terms, then FNHS(I,J,L) and its reciprocal
@@ -49,13 +50,19 @@ __global__ void k_mom_tables(double *ff, const double *__restrict__ wmu, int ne,
 // MAXK*kMomThreads).
 template <int MAXK>
 __global__ void __launch_bounds__(kMomThreads, MAXK == 1 ? 8 : 4)
-    k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int anisch,
+    k_moments(double *__restrict__ f2, int ns, int nr, int nt, int ne, int npa, int s0, int nsp, int anisch,
               const double *__restrict__ ff, const double *__restrict__ ernh, const double *__restrict__ epp,
               const double *__restrict__ ekev, const double *__restrict__ wmu, const double *__restrict__ mu,
               const double *__restrict__ fnhs, const int *__restrict__ upa, double *__restrict__ flux,
               double *__restrict__ mom) {
   extern __shared__ double sm[];
-  const int p = blockIdx.x;
+  // XCD-aware: the blocks an XCD receives (blockIdx mod 8) take one
+  // contiguous range of planes, so the FFACTOR tables of an (S,I), shared by
+  // its NT planes, are fetched into about one L2; a block past the last
+  // plane exits whole
+  const int nplane = nsp * (nr - 1) * nt, per = (nplane + kXcds - 1) / kXcds;
+  const int p = (blockIdx.x % kXcds) * per + blockIdx.x / kXcds;
+  if (p >= nplane) return;
   const int j = p % nt;
   const int i = 1 + (p / nt) % (nr - 1);  // I = 2..NR
   const int s = s0 + p / (nt * (nr - 1));
@@ -189,6 +196,7 @@ using namespace ramgpu_host;
 using ramgpu::k_moments;
 using ramgpu::kMomMaxNE;
 using ramgpu::kMomThreads;
+using ramgpu::kXcds;
 using ramgpu::mom_lds_doubles;
 
 namespace {
@@ -223,17 +231,18 @@ int run_moments(ramgpu_ctx *c, int s0, int nsp, int anisch, bool flux) {
   }
   if (c->nr < 2) return RAMGPU_OK;
   const int nplane = nsp * (c->nr - 1) * c->nt;
+  const int nblk = (nplane + kXcds - 1) / kXcds * kXcds;  // every XCD gets ceil(nplane/8) planes' blocks
   const size_t lds = mom_lds_doubles(c->ne, c->npa) * sizeof(double);
   tbeg(c, RAMGPU_KERNEL_MOMENTS);
   if (c->ne <= kMomThreads)
-    hipLaunchKernelGGL(k_moments<1>, dim3(nplane), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
-                       c->nt, c->ne, c->npa, s0, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
+    hipLaunchKernelGGL(k_moments<1>, dim3(nblk), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
+                       c->nt, c->ne, c->npa, s0, nsp, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
                        (const double *)c->d_epp, (const double *)c->d_ekev, (const double *)c->d_wmu,
                        (const double *)c->d_mu, (const double *)c->d_fnhs, (const int *)c->d_upa,
                        flux ? c->d_flux : nullptr, c->d_mom);
   else
-    hipLaunchKernelGGL(k_moments<4>, dim3(nplane), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
-                       c->nt, c->ne, c->npa, s0, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
+    hipLaunchKernelGGL(k_moments<4>, dim3(nblk), dim3(kMomThreads), lds, c->stream, c->t.f2, c->ns, c->nr,
+                       c->nt, c->ne, c->npa, s0, nsp, anisch, (const double *)c->d_ff, (const double *)c->d_ernh,
                        (const double *)c->d_epp, (const double *)c->d_ekev, (const double *)c->d_wmu,
                        (const double *)c->d_mu, (const double *)c->d_fnhs, (const int *)c->d_upa,
                        flux ? c->d_flux : nullptr, c->d_mom);
