@@ -1,6 +1,7 @@
 !  ramgpu_driver.f90 -- a Fortran host driving the MI355X split step through
 !  ramgpu_mod, the way the reference's ram_run would: module-style arrays in
-!  Fortran order, one context, per-species set-up, then nsteps ram_run calls.
+!  Fortran order, one context, per-species set-up, then nsteps ram_run calls
+!  and the block after ram_run's species loop (ANISCH of every species, FLUX).
 !
 !  Usage: ramgpu_driver <input.bin> <output.bin>
 !  input.bin (stream, native endian): int32 ns,nr,nt,ne,npa,nsteps; then the
@@ -8,8 +9,9 @@
 !  cex(ns); real64 rmas(ns), rlz(nr+1), ekev,ebnd,we,de(ne), grel,grbnd,v(ns,ne),
 !  mu,wmu,dmu(npa), upa(nr), fnhs,fnis,bounhs,bounis,hdns,didt,dibndt(nr+1,nt,npa),
 !  bnes,dbdt,vt,eir,eip(nr+1,nt); int32 mgnp(nr,nt); real64 fgeos(ns,nt,ne,npa),
-!  f2(ns,nr,nt,ne,npa).
-!  output.bin: f2 after the steps, then setrc,lsdr,lscha,lsatm(ns).
+!  f2(ns,nr,nt,ne,npa), ffactor(ns,nr,ne,npa), ernh,epp(ns,ne).
+!  output.bin: f2 after the steps and ANISCH, setrc,lsdr,lscha,lsatm(ns),
+!  ppert,ppart,edent,rnhtt(ns,nr,nt), flux(ns,nr,nt,ne,npa).
 program ramgpu_driver
   use iso_c_binding
   use ramgpu_mod
@@ -25,6 +27,8 @@ program ramgpu_driver
   real(c_double), allocatable :: bnes(:,:), dbdt(:,:), vt(:,:), eir(:,:), eip(:,:)
   real(c_double), allocatable :: fgeos(:,:,:,:), f2(:,:,:,:,:)
   real(c_double), allocatable :: setrc(:), lsdr(:), lscha(:), lsatm(:)
+  real(c_double), allocatable :: ffactor(:,:,:,:), ernh(:,:), epp(:,:), flux(:,:,:,:,:)
+  real(c_double), allocatable :: ppert(:,:,:), ppart(:,:,:), edent(:,:,:), rnhtt(:,:,:)
   type(ramgpu_species_out), allocatable :: out(:)
   character(len=512) :: fin, fout
   integer :: u
@@ -41,9 +45,13 @@ program ramgpu_driver
   allocate(bnes(nr+1,nt), dbdt(nr+1,nt), vt(nr+1,nt), eir(nr+1,nt), eip(nr+1,nt), mgnp(nr,nt))
   allocate(fgeos(ns,nt,ne,npa), f2(ns,nr,nt,ne,npa), setrc(ns), lsdr(ns), lscha(ns), lsatm(ns))
   allocate(out(ns))
+  allocate(ffactor(ns,nr,ne,npa), ernh(ns,ne), epp(ns,ne), flux(ns,nr,nt,ne,npa))
+  allocate(ppert(ns,nr,nt), ppart(ns,nr,nt), edent(ns,nr,nt), rnhtt(ns,nr,nt))
+  ppert = 0; ppart = 0; edent = 0; rnhtt = 0
   read(u) charge, cex, rmas, rlz, ekev, ebnd, we, de, grel, grbnd, v, mu, wmu, dmu, upa
   read(u) fnhs, fnis, bounhs, bounis, hdns, didt, dibndt, bnes, dbdt, vt, eir, eip, mgnp
   read(u) fgeos, f2
+  read(u) ffactor, ernh, epp
   close(u)
 
   rc = ramgpu_create(ctx, ns, nr, nt, ne, npa, 0_c_int)
@@ -60,12 +68,16 @@ program ramgpu_driver
   call check(ramgpu_set_boundary(ctx, fgeos), "set_boundary")
   call check(ramgpu_put_f2(ctx, f2), "put_f2")
   call check(ramgpu_ram_run(ctx, nsteps, out), "ram_run")
+  ! src/ModRamRun.f90:207-222: ANISCH of every species (FLUX left out), then FLUX
+  call check(ramgpu_set_moment_factors(ctx, ffactor, ernh, epp), "set_moment_factors")
+  call check(ramgpu_moments(ctx, ppert, ppart, edent, rnhtt), "moments")
+  call check(ramgpu_flux(ctx, flux), "flux")
   call check(ramgpu_get_f2(ctx, f2), "get_f2")
   call check(ramgpu_get_energy_state(ctx, setrc, lsdr, lscha, lsatm), "get_energy_state")
   call check(ramgpu_destroy(ctx), "destroy")
 
   open(newunit=u, file=trim(fout), access="stream", form="unformatted", status="replace")
-  write(u) f2, setrc, lsdr, lscha, lsatm
+  write(u) f2, setrc, lsdr, lscha, lsatm, ppert, ppart, edent, rnhtt, flux
   close(u)
 
 contains

@@ -1,6 +1,7 @@
 """The Fortran host path: ram-scb_amd/fortran/ramgpu_driver (built from
 ramgpu_mod.f90, the ISO_C_BINDING interface) runs split steps on the GPU from
-Fortran-ordered module arrays; the result must match the oracle."""
+Fortran-ordered module arrays, then the moments (ANISCH, FLUX); the results
+must match the oracle."""
 import subprocess
 
 import numpy as np
@@ -26,7 +27,7 @@ def write_bundle(path, p, nsteps):
                   geo.didt, geo.dibndt, geo.bnes, geo.dbdt, p.vt, geo.eir, geo.eip):
             np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
         np.asarray(geo.outside_mgnp, dtype=np.int32).ravel(order="F").tofile(f)
-        for a in (p.fgeos, p.f2):
+        for a in (p.fgeos, p.f2, g.ffactor, g.ernh, g.epp):
             np.asarray(a, dtype=np.float64).ravel(order="F").tofile(f)
 
 
@@ -48,12 +49,23 @@ def test_fortran_host_matches_oracle(tmp_path, which):
     g = p.grid
     n = g.ns * g.nr * g.nt * g.ne * g.npa
     raw = np.fromfile(fout, dtype=np.float64)
+    n3 = g.ns * g.nr * g.nt
     f2 = raw[:n].reshape((g.ns, g.nr, g.nt, g.ne, g.npa), order="F")
-    energy = raw[n:].reshape(4, g.ns)
+    energy = raw[n:n + 4 * g.ns].reshape(4, g.ns)
+    mom = raw[n + 4 * g.ns:n + 4 * g.ns + 4 * n3].reshape((4, n3))
+    flux = raw[n + 4 * g.ns + 4 * n3:].reshape((g.ns, g.nr, g.nt, g.ne, g.npa), order="F")
     o = Oracle(p)
     for _ in range(nsteps):
         o.ram_run(4)
+    out = None
+    for s in range(1, g.ns + 1):
+        out = o.anisch(s, out)
+    oflux = o.flux()
     assert rel_err(f2, o.f2) <= 1e-12 and max_rel_err(f2, o.f2) <= 1e-12
+    for row, name in enumerate(("ppert", "ppart", "edent", "rnhtt")):
+        a, b = mom[row].reshape((g.ns, g.nr, g.nt), order="F"), out[name]
+        assert rel_err(a, b) <= 1e-12 and max_rel_err(a, b) <= 1e-12, name
+    assert rel_err(flux, oflux) <= 1e-12 and max_rel_err(flux, oflux) <= 1e-12
     for row, name in enumerate(("setrc", "lsdr", "lscha", "lsatm")):
         ref = o.arr(name)
         assert np.all(np.abs(energy[row] - ref) <= 1e-12 * np.abs(o.arr("setrc"))), name
