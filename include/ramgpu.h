@@ -185,6 +185,22 @@ int ramgpu_ram_run(ramgpu_ctx *ctx, int nsteps, ramgpu_species_out *out);
 int ramgpu_ram_run_schedule(ramgpu_ctx *ctx, int nsteps, const double *kp,
                             const double *bscale, ramgpu_species_out *out);
 
+/* ramgpu_ram_run_schedule without any allocation (no counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * step, such as tight forecast loops and harnesses that forbid hipMalloc).
+ * The schedule is staged in a fixed array of 65536 doubles that the library
+ * keeps on every device it runs on until the process ends, instead of a
+ * per-call allocation: nsteps * (1 + NS) doubles with bscale, nsteps without,
+ * must fit it (RAMGPU_EINVAL otherwise; e.g. 13107 steps of 4 species).  The
+ * calls of all contexts on a device share that array, which keeps the last
+ * call's schedule: the library serialises these calls within a process and
+ * returns only after its stream has finished reading the array, so calls from
+ * several host threads wait for each other; concurrent calls from several
+ * processes on one device are not coordinated.  Results are bit-identical to
+ * ramgpu_ram_run_schedule's. */
+int ramgpu_ram_run_schedule_noalloc(ramgpu_ctx *ctx, int nsteps, const double *kp,
+                                    const double *bscale, ramgpu_species_out *out);
+
 /* Synchronise and return the device time (ms) of the last ram_run call,
  * measured with HIP events on the library's stream. */
 int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,37 @@ int fail(int code, const char *fmt, ...) {
 }  // namespace ramgpu_host
 
 using namespace ramgpu_host;
+
+// Staging of ramgpu_ram_run_schedule_noalloc: the Kp schedule and boundary
+// scales of one call (nsteps * (1 + NS) doubles at most), in a fixed array of
+// every device's memory instead of a per-call allocation.  Host code never
+// names the array itself (its name is a host-side stand-in, not a device
+// address): it uses the address hipGetSymbolAddress returns on each device.
+namespace {
+constexpr size_t kNoAllocDoubles = size_t(1) << 16;  // 512 KiB per device
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+std::mutex g_scratch_mu;                  // guards g_scratch_ptr, serialises the calls
+double *g_scratch_ptr[kMaxDevices] = {};  // the array's address on each device
+
+// The array's address on `device`, which must be current; looked up once per
+// device (in ramgpu_create, or the entry point's first call there).
+int scratch_address(int device, double **out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(RAMGPU_EINVAL, "device %d outside 0..%d of the staging table", device, kMaxDevices - 1);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(RAMGPU_EHIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    }
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return RAMGPU_OK;
+}
+}  // namespace
 
 
 namespace ramgpu_host {
@@ -183,14 +215,8 @@ int prepare(ramgpu_ctx *c) {
     size_t lds = (size_t)(nvt + 4 * c->nt) * sizeof(double);
     // the ghost's q1 row in LDS where it fits the default 64 KB with the
     // kernel's static LDS
-    static size_t head_static = 0;
-    if (!head_static) {
-      hipFuncAttributes fa{};
-      if (hipFuncGetAttributes(&fa, (const void *)k_step_head) == hipSuccess) head_static = fa.sharedSizeBytes;
-      (void)hipGetLastError();
-    }
     const size_t q1b = (size_t)c->npa * c->nt * sizeof(double);
-    const int stage_q1 = head_static && head_static + lds + q1b <= 65536;
+    const int stage_q1 = c->head_static && c->head_static + lds + q1b <= 65536;
     if (stage_q1) lds += q1b;
     const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
     const Launch lf = launch_of(c, 0, c->fin_mask >= 0 ? c->ns : 0, 1);
@@ -416,9 +442,9 @@ int set_plane_lds_limit(size_t limit) {
 
 int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
   if (!out) return RAMGPU_OK;
-  std::vector<double> e(c->ns * 4), d(c->ns * 4);
-  HIP_TRY(hipMemcpyAsync(e.data(), c->t.energy, e.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipMemcpyAsync(d.data(), c->t.dtdrift, d.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double e[4 * kMaxSpecies], d[4 * kMaxSpecies];
+  HIP_TRY(hipMemcpyAsync(e, c->t.energy, (size_t)c->ns * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(d, c->t.dtdrift, (size_t)c->ns * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int s = 0; s < c->ns; ++s) {
     out[s].setrc = e[s * 4 + 0];
@@ -556,6 +582,16 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   if (hipMemcpy(c->d_bscale, ones.data(), ns * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(RAMGPU_EHIP, "hipMemcpy failed"));
   if ((rc = set_plane_lds_limit(c->lds_limit))) return cleanup(rc);
+  {
+    hipFuncAttributes fa{};
+    if (hipFuncGetAttributes(&fa, (const void *)k_step_head) == hipSuccess) c->head_static = fa.sharedSizeBytes;
+    (void)hipGetLastError();
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    double *sp = nullptr;
+    if ((rc = scratch_address(device, &sp))) return cleanup(rc);
+  }
   *out = c;
   return RAMGPU_OK;
 }
@@ -848,26 +884,26 @@ int ramgpu_ram_run(ramgpu_ctx *c, int nsteps, ramgpu_species_out *out) {
   return fill_out(c, out);
 }
 
-int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const double *bscale,
-                            ramgpu_species_out *out) {
+}  // extern "C"
+
+namespace {
+// Checks shared by the two schedule entry points.
+int schedule_args(ramgpu_ctx *c, int nsteps, const double *kp) {
   int rc = check_ctx(c);
   if (rc || (rc = ready(c))) return rc;
   if (nsteps < 0) return fail(RAMGPU_EINVAL, "nsteps < 0");
   if (!kp) return fail(RAMGPU_EINVAL, "kp schedule is null");
   if (!c->have_lzphi) return fail(RAMGPU_ESTATE, "ramgpu_set_kp(lz, phi) must precede a Kp schedule");
-  // the whole schedule is staged on the device before the timed steps
-  double *d_kp = nullptr, *d_bs = nullptr;
-  HIP_TRY(hipMalloc(&d_kp, std::max(nsteps, 1) * sizeof(double)));
-  HIP_TRY(hipMemcpyAsync(d_kp, kp, nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  if (bscale) {
-    HIP_TRY(hipMalloc(&d_bs, std::max(nsteps, 1) * c->ns * sizeof(double)));
-    HIP_TRY(hipMemcpyAsync(d_bs, bscale, (size_t)nsteps * c->ns * sizeof(double), hipMemcpyHostToDevice,
-                           c->stream));
-  }
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+// The steps of a Kp schedule already staged on the device (d_kp[nsteps],
+// d_bs[nsteps*NS] or null).  Each step's finalize runs with the next step's
+// first launch (prepare), the last one before the end of the timed run.
+// Returns after the stream has finished, so the staging may be reused.
+int run_schedule_staged(ramgpu_ctx *c, int nsteps, const double *d_kp, const double *d_bs) {
+  int rc = RAMGPU_OK;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  // each step's finalize runs with the next step's first launch (prepare),
-  // the last one before the end of the timed run
   for (int n = 0; n < nsteps && rc == RAMGPU_OK; ++n) {
     if ((rc = apply_step_inputs(c, d_kp + n, d_bs ? d_bs + (size_t)n * c->ns : nullptr))) break;
     int cflmask = 0;
@@ -878,10 +914,56 @@ int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const d
   const int rf = flush_finalize(c);
   if (rc == RAMGPU_OK) rc = rf;
   (void)hipEventRecord(c->ev1, c->stream);
-  (void)hipStreamSynchronize(c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == RAMGPU_OK)
+    rc = fail(RAMGPU_EHIP, "hipStreamSynchronize failed after a Kp schedule");
+  return rc;
+}
+}  // namespace
+
+extern "C" {
+
+int ramgpu_ram_run_schedule(ramgpu_ctx *c, int nsteps, const double *kp, const double *bscale,
+                            ramgpu_species_out *out) {
+  int rc = schedule_args(c, nsteps, kp);
+  if (rc) return rc;
+  // the whole schedule is staged on the device before the timed steps
+  double *d_kp = nullptr, *d_bs = nullptr;
+  HIP_TRY(hipMalloc(&d_kp, std::max(nsteps, 1) * sizeof(double)));
+  if (bscale && hipMalloc(&d_bs, std::max(nsteps, 1) * c->ns * sizeof(double)) != hipSuccess) {
+    (void)hipFree(d_kp);
+    return fail(RAMGPU_ENOMEM, "hipMalloc of the boundary-scale schedule failed");
+  }
+  if (hipMemcpyAsync(d_kp, kp, nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      (bscale && hipMemcpyAsync(d_bs, bscale, (size_t)nsteps * c->ns * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream) != hipSuccess) ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(RAMGPU_EHIP, "staging the Kp schedule failed");
+  if (rc == RAMGPU_OK) rc = run_schedule_staged(c, nsteps, d_kp, d_bs);
   (void)hipFree(d_kp);
   if (d_bs) (void)hipFree(d_bs);
   if (rc) return rc;
+  return fill_out(c, out);
+}
+
+int ramgpu_ram_run_schedule_noalloc(ramgpu_ctx *c, int nsteps, const double *kp, const double *bscale,
+                                    ramgpu_species_out *out) {
+  int rc = schedule_args(c, nsteps, kp);
+  if (rc) return rc;
+  const size_t need = (size_t)nsteps * (1 + (bscale ? (size_t)c->ns : 0));
+  if (need > kNoAllocDoubles)
+    return fail(RAMGPU_EINVAL, "schedule of %d steps needs %zu staged doubles, above the %zu of the fixed array",
+                nsteps, need, kNoAllocDoubles);
+  // one call at a time per process: the calls share every device's array
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  double *sp = nullptr;
+  if ((rc = scratch_address(c->device, &sp))) return rc;
+  double *d_kp = sp, *d_bs = bscale ? sp + nsteps : nullptr;
+  if (nsteps > 0 &&
+      (hipMemcpyAsync(d_kp, kp, nsteps * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+       (d_bs && hipMemcpyAsync(d_bs, bscale, (size_t)nsteps * c->ns * sizeof(double), hipMemcpyHostToDevice,
+                               c->stream) != hipSuccess)))
+    return fail(RAMGPU_EHIP, "staging the Kp schedule failed");
+  if ((rc = run_schedule_staged(c, nsteps, d_kp, d_bs))) return rc;
   return fill_out(c, out);
 }
 

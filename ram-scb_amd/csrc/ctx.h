@@ -95,6 +95,7 @@ struct ramgpu_ctx {
   std::vector<int> tkind;
   size_t tused = 0;
   bool bwd_lds_set = false;  // LDS attribute of the plane adjoint kernels set
+  size_t head_static = 0;    // static LDS of k_step_head (0: unknown, q1 row not staged)
 };
 
 namespace ramgpu_host {

@@ -35,6 +35,7 @@ EXPORTS = (
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
     "ramgpu_batch_adjoint_run", "ramgpu_set_plane_shape", "ramgpu_get_plane_shape",
     "ramgpu_set_moment_factors", "ramgpu_anisch", "ramgpu_flux", "ramgpu_moments",
+    "ramgpu_ram_run_schedule_noalloc",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
@@ -95,6 +96,7 @@ def load_library(path: Path | None = None):
     lib.ramgpu_moments.argtypes = [v, _dp, _dp, _dp, _dp, _dp]
     lib.ramgpu_ram_run.argtypes = [v, C.c_int, C.POINTER(SpeciesOut)]
     lib.ramgpu_ram_run_schedule.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
+    lib.ramgpu_ram_run_schedule_noalloc.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
     lib.ramgpu_kernel_timing.argtypes = [v, C.c_int]
     lib.ramgpu_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
@@ -336,13 +338,16 @@ class RamGpu:
         self._check(self.lib.ramgpu_ram_run(self.h, nsteps, out))
         return [o.as_dict() for o in out]
 
-    def ram_run_schedule(self, kp, bscale=None):
+    def ram_run_schedule(self, kp, bscale=None, noalloc=False):
+        """nsteps = len(kp) steps with a per-step Kp and boundary scales
+        bscale (nsteps*NS); noalloc: the entry point that stages the schedule
+        in the library's fixed device array instead of allocating."""
         kp = _fd(kp)
         n = kp.size
         bs = None if bscale is None else _fd(bscale, n * self.ns)
         out = (SpeciesOut * self.ns)()
-        self._check(self.lib.ramgpu_ram_run_schedule(self.h, n, _p(kp),
-                                                     None if bs is None else _p(bs), out))
+        fn = self.lib.ramgpu_ram_run_schedule_noalloc if noalloc else self.lib.ramgpu_ram_run_schedule
+        self._check(fn(self.h, n, _p(kp), None if bs is None else _p(bs), out))
         return [o.as_dict() for o in out]
 
     # -- reverse mode ------------------------------------------------------
