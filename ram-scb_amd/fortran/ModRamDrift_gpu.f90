@@ -12,6 +12,14 @@
 !  from CEPARA, the first operator of a species in ram_run) and back in
 !  DRIFTEND (the last); geometry, field, boundary and DTs are uploaded with
 !  it, since the reference may change them between steps.
+!
+!  ram_run also runs host operators on F2 between these calls when they are
+!  switched on (src/ModRamRun.f90:81-168): COULEN/COULMU (DoUseCoulomb),
+!  WAVELO/WPADIF (species%WPI, on for electrons by default), WPADIF for EMIC
+!  (species%EMIC and DoUseEMIC) and FLCScatter (species%FLC and DoUseFLC).
+!  For such a species the host copy stays authoritative: every operator here
+!  puts the species' F2 on the device before it runs and takes it back after
+!  (host_sync), so those host operators see and keep every change.
 module ModRamDrift
 
   use iso_c_binding
@@ -24,7 +32,8 @@ module ModRamDrift
 
   type(c_ptr), save, allocatable :: gpu_ctx(:)
   integer, save :: current_s = 0
-  !$OMP THREADPRIVATE(current_s)
+  logical, save :: host_sync = .false.
+  !$OMP THREADPRIVATE(current_s, host_sync)
 
   contains
 
@@ -37,12 +46,31 @@ module ModRamDrift
     end if
   end subroutine gpu_check
 
+  ! Host F2 of species S to the device before an operator, when host
+  ! operators also update it this step.
+  subroutine sync_in(S)
+    use ModRamVariables, ONLY: F2
+    integer, intent(in) :: S
+    if (host_sync) call gpu_check(ramgpu_put_f2(gpu_ctx(S), F2(S:S,:,:,:,:)), 'put_f2')
+  end subroutine sync_in
+
+  ! ... and back after it.
+  subroutine sync_out(S)
+    use ModRamVariables, ONLY: F2
+    integer, intent(in) :: S
+    real(c_double), allocatable :: f(:,:,:,:,:)
+    if (.not. host_sync) return
+    allocate(f(1, nR, nT, nE, nPA))
+    call gpu_check(ramgpu_get_f2(gpu_ctx(S), f), 'get_f2')
+    F2(S:S,:,:,:,:) = f
+  end subroutine sync_out
+
   ! Upload the state species S needs and its F2 slice (first call per species).
   subroutine ramgpu_begin_species(S)
     use ModRamConst,     ONLY: PI
     use ModRamGrids,     ONLY: nS
     use ModRamTiming,    ONLY: DTs
-    use ModRamParams,    ONLY: BetaLim
+    use ModRamParams,    ONLY: BetaLim, DoUseCoulomb, DoUseEMIC, DoUseFLC
     use ModRamVariables, ONLY: F2, FNHS, FNIS, BOUNHS, BOUNIS, HDNS, dIdt, dIbndt, BNES, &
                                dBdt, VT, EIR, EIP, outsideMGNP, FGEOS, RLZ, EKEV, EBND, &
                                WE, DE, GREL, GRBND, V, RMAS, MU, WMU, DMU, UPA, MDR, &
@@ -62,6 +90,8 @@ module ModRamDrift
     !$OMP END CRITICAL (ramgpu_ctx_alloc)
     ctx = gpu_ctx(S)
     current_s = S
+    host_sync = species(S)%WPI .or. DoUseCoulomb .or. (species(S)%EMIC .and. DoUseEMIC) .or. &
+                (species(S)%FLC .and. DoUseFLC)
     cex = RAMGPU_CEX_NONE                     ! CEPARA's species selection, src/ModRamLoss.f90:37-83
     if (species(S)%CEX) then
        select case (species(S)%s_name)
@@ -91,7 +121,9 @@ module ModRamDrift
     use ModRamVariables, ONLY: F2
     real(c_double), allocatable :: f(:,:,:,:,:)
     allocate(f(1, nR, nT, nE, nPA))
-    call gpu_check(ramgpu_get_f2(gpu_ctx(current_s), f), 'get_f2')
+    if (.not. host_sync) &  ! with host_sync the host copy is already current
+         call gpu_check(ramgpu_get_f2(gpu_ctx(current_s), f), 'get_f2')
+    if (host_sync) f = F2(current_s:current_s,:,:,:,:)
     F2(current_s:current_s,:,:,:,:) = f
     call gpu_check(ramgpu_driftend(gpu_ctx(current_s)), 'driftend')
   END SUBROUTINE DRIFTEND
@@ -105,28 +137,36 @@ module ModRamDrift
   SUBROUTINE DRIFTR(S)
     integer, intent(in) :: S
     real(c_double) :: dt
+    call sync_in(S)
     call gpu_check(ramgpu_driftr(gpu_ctx(S), 1_c_int, dt), 'driftr')
+    call sync_out(S)
     DtDriftR(S) = dt
   END SUBROUTINE DRIFTR
 
   SUBROUTINE DRIFTP(S)
     integer, intent(in) :: S
     real(c_double) :: dt
+    call sync_in(S)
     call gpu_check(ramgpu_driftp(gpu_ctx(S), 1_c_int, dt), 'driftp')
+    call sync_out(S)
     DtDriftP(S) = dt
   END SUBROUTINE DRIFTP
 
   SUBROUTINE DRIFTE(S)
     integer, intent(in) :: S
     real(c_double) :: dt
+    call sync_in(S)
     call gpu_check(ramgpu_drifte(gpu_ctx(S), 1_c_int, dt), 'drifte')
+    call sync_out(S)
     DtDriftE(S) = dt
   END SUBROUTINE DRIFTE
 
   SUBROUTINE DRIFTMU(S)
     integer, intent(in) :: S
     real(c_double) :: dt
+    call sync_in(S)
     call gpu_check(ramgpu_driftmu(gpu_ctx(S), 1_c_int, dt), 'driftmu')
+    call sync_out(S)
     DtDriftMu(S) = dt
   END SUBROUTINE DRIFTMU
 
@@ -141,12 +181,16 @@ module ModRamDrift
 
   subroutine gpu_charexchange(S)
     integer, intent(in) :: S
+    call sync_in(S)
     call gpu_check(ramgpu_charexchange(gpu_ctx(S), 1_c_int), 'charexchange')
+    call sync_out(S)
   end subroutine gpu_charexchange
 
   subroutine gpu_atmol(S)
     integer, intent(in) :: S
+    call sync_in(S)
     call gpu_check(ramgpu_atmol(gpu_ctx(S), 1_c_int), 'atmol')
+    call sync_out(S)
   end subroutine gpu_atmol
 
   subroutine gpu_sumrc(S)
@@ -155,6 +199,7 @@ module ModRamDrift
     real(c_double) :: setrc1(1), zero(1), setrc_new, elorc_new
     setrc1(1) = SETRC(S)
     zero(1) = 0.0_c_double
+    call sync_in(S)
     call gpu_check(ramgpu_set_energy_state(gpu_ctx(S), setrc1, zero, zero, zero), 'energy')
     call gpu_check(ramgpu_sumrc(gpu_ctx(S), 1_c_int, setrc_new, elorc_new), 'sumrc')
     SETRC(S) = setrc_new

@@ -71,6 +71,30 @@ def build_fortran(verbose: bool = False) -> Path:
     return FDRIVER
 
 
+LEVEL1_SRC = REPO / "tests" / "fortran"
+LEVEL1_DRIVER = LEVEL1_SRC / "_build" / "level1_driver"
+
+
+def build_level1(verbose: bool = False) -> Path:
+    """The Level-1 drop-in (fortran/ModRamDrift_gpu.f90) compiled against the
+    test-side module stand-ins of tests/fortran/ and linked into the test's
+    host program, with OpenMP and the reference's precision flags."""
+    fc = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
+    obj = LEVEL1_DRIVER.parent
+    obj.mkdir(exist_ok=True)
+    srcs = [FORTRAN / "ramgpu_mod.f90", LEVEL1_SRC / "level1_modules.f90",
+            FORTRAN / "ModRamDrift_gpu.f90", LEVEL1_SRC / "level1_driver.f90"]
+    cmd = [fc, "-O2", "-fopenmp", "-fdefault-real-8", "-fdefault-double-8", "-module-dir", str(obj),
+           *map(str, srcs), "-o", str(LEVEL1_DRIVER), f"-L{PKG}", "-lramgpu", f"-Wl,-rpath,{PKG}"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=str(obj))
+    if r.returncode != 0:
+        raise RuntimeError(f"amdflang failed ({r.returncode}):\n{r.stderr}")
+    return LEVEL1_DRIVER
+
+
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
     print(build_fortran(verbose="-v" in sys.argv))
+    print(build_level1(verbose="-v" in sys.argv))
