@@ -53,6 +53,8 @@ typedef struct {
   double setrc;                       /* SETRC(S) after the step          */
   double lsdr, lscha, lsatm;          /* LSDR/LSCHA/LSATM(S), cumulative  */
   double dtdrift_r, dtdrift_p, dtdrift_e, dtdrift_mu;
+  double lswae, lscoe, lscsc;         /* LSWAE (WAVELO), LSCOE/LSCSC (Coulomb
+                                         energy loss / scattering), cumulative */
 } ramgpu_species_out;
 
 const char *ramgpu_last_error(void);
@@ -124,6 +126,26 @@ int ramgpu_set_energy_state(ramgpu_ctx *ctx, const double *setrc, const double *
 int ramgpu_get_energy_state(ramgpu_ctx *ctx, double *setrc, double *lsdr, double *lscha,
                             double *lsatm);
 
+/* LSWAE/LSCOE/LSCSC(S) (src/ModRamRun.f90:84-98,151-168; zero after
+ * create).  Any pointer may be NULL. */
+int ramgpu_set_loss_state(ramgpu_ctx *ctx, const double *lswae, const double *lscoe, const double *lscsc);
+int ramgpu_get_loss_state(ramgpu_ctx *ctx, double *lswae, double *lscoe, double *lscsc);
+
+/* Electron lifetimes (WAVELO, src/ModRamWPI.f90:580-636), taken in ram_run by
+ * the species with species%WPI set while DoUseWPI is off (the reference's
+ * default for electrons, src/ModRamSpecies.f90:49; src/ModRamRun.f90:91-99,
+ * 151-159).  ramgpu_set_wavelo gives species s the lifetime tables WALOS1,
+ * WALOS2, WALOS3 (NR,NE) of WAVEPARA1/2 (src/ModRamWPI.f90:18-182) and
+ * switches WAVELO on for it (all three NULL: off).  ramgpu_set_wave_params
+ * sets KP and the plasmapause, 5.39-0.382*Kpmax12 or, with a plasmasphere
+ * model (use_plasmasphere, DoUsePlasmasphere), the largest LZ(I), I=2..NR,
+ * with NECR(I,J) (NR,NT) above 50; lz(NR+1) is LZ.  KP follows every later
+ * ramgpu_set_kp and each step of a Kp schedule. */
+int ramgpu_set_wavelo(ramgpu_ctx *ctx, int s, const double *walos1, const double *walos2,
+                      const double *walos3);
+int ramgpu_set_wave_params(ramgpu_ctx *ctx, double kp, double kpmax12, const double *lz,
+                           const double *necr, int use_plasmasphere);
+
 /* ---- operators, one per reference subroutine (S is 1-based) ------------- */
 
 /* DRIFTPARA(S), src/ModRamDrift.f90:36-88: builds the drift coefficient
@@ -142,6 +164,8 @@ int ramgpu_driftend(ramgpu_ctx *ctx);
 int ramgpu_cepara(ramgpu_ctx *ctx, int s);
 int ramgpu_charexchange(ramgpu_ctx *ctx, int s);
 int ramgpu_atmol(ramgpu_ctx *ctx, int s);
+/* WAVELO(S), src/ModRamWPI.f90:580-636 (the species must have its tables) */
+int ramgpu_wavelo(ramgpu_ctx *ctx, int s);
 /* SUMRC(S), src/ModRamRun.f90:231-259: setrc receives the new SETRC(S),
  * elorc = old - new.  Either may be NULL. */
 int ramgpu_sumrc(ramgpu_ctx *ctx, int s, double *setrc, double *elorc);
@@ -172,11 +196,16 @@ int ramgpu_moments(ramgpu_ctx *ctx, double *ppert, double *ppart, double *edent,
 
 /* ---- the fused split step ----------------------------------------------- */
 
-/* `nsteps` consecutive ram_run calls (src/ModRamRun.f90:64-205, optional
- * processes off): per species R,P,E,Mu drifts, charge exchange, atmospheric
- * loss twice, charge exchange, Mu,E,P,R drifts with the SUMRC bookkeeping,
- * then the periodic local-time copy and the magnetopause mask.  out (may be
- * NULL) receives NS records for the last step. */
+/* `nsteps` consecutive ram_run calls (src/ModRamRun.f90:64-205): per species
+ * R,P,E,Mu drifts, [WAVELO], charge exchange, atmospheric loss twice, charge
+ * exchange, [WAVELO], Mu,E,P,R drifts with the SUMRC bookkeeping, then the
+ * periodic local-time copy and the magnetopause mask.  WAVELO only for the
+ * species given its tables (ramgpu_set_wavelo).  The other optional
+ * processes of ram_run -- Coulomb collisions (DoUseCoulomb), WPADIF
+ * (DoUseWPI, and EMIC with DoUseEMIC) and FLC scattering (DoUseFLC) -- are not
+ * part of this step: a host that switches any of them on must not replace
+ * ram_run's species loop with this call.  out (may be NULL) receives NS
+ * records for the last step. */
 int ramgpu_ram_run(ramgpu_ctx *ctx, int nsteps, ramgpu_species_out *out);
 
 /* The same with a per-step Kp and boundary scale schedule held on the host:
@@ -297,6 +326,12 @@ int ramgpu_batch_ram_run_schedule(ramgpu_batch *batch, int nsteps, const double 
                                   const double *bscale, ramgpu_species_out *out);
 /* Device time (ms) of the last ensemble run, HIP events on its stream. */
 int ramgpu_batch_last_run_ms(ramgpu_batch *batch, double *ms);
+/* Opt-in per-launch timing of the ensemble's steps, as ramgpu_kernel_timing
+ * (measurement support; steps are launched eagerly, not from the graph, while
+ * it is on) and the summed times and launch counts per kernel kind since the
+ * last query (RAMGPU_KERNEL_* indices). */
+int ramgpu_batch_kernel_timing(ramgpu_batch *batch, int enable);
+int ramgpu_batch_kernel_times(ramgpu_batch *batch, double *total_ms, int *launches);
 /* 1 (default): replay a captured graph per step; 0: launch eagerly. */
 int ramgpu_batch_use_graph(ramgpu_batch *batch, int enable);
 /* Autotuner: times every candidate launch grid of the R, P and plane

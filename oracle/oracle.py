@@ -33,7 +33,8 @@ class _State(C.Structure):
          ("vt", _dp), ("eir", _dp), ("eip", _dp), ("outside_mgnp", _ip), ("fgeos", _dp),
          ("f2", _dp), ("dtdrift_r", _dp), ("dtdrift_p", _dp), ("dtdrift_e", _dp),
          ("dtdrift_mu", _dp), ("setrc", _dp), ("elorc", _dp), ("lsdr", _dp), ("lscha", _dp),
-         ("lsatm", _dp)]
+         ("lsatm", _dp), ("wpi", _ip), ("walos1", _dp), ("walos2", _dp), ("walos3", _dp),
+         ("lz", _dp), ("rlpp", _dp), ("kp", C.c_double), ("lswae", _dp)]
 
 
 class _Drift(C.Structure):
@@ -78,6 +79,7 @@ class Oracle:
             lib.oracle_charexchange.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
             lib.oracle_atmol.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
             lib.oracle_sumrc.argtypes = [sp, C.c_int]
+            lib.oracle_wavelo.argtypes = [sp, C.c_int]
             lib.oracle_ram_run.argtypes = [sp, C.c_int, _dp]
             lib.oracle_ram_run.restype = C.c_int
             lib.oracle_drift_alloc.argtypes = [sp, C.POINTER(_Drift)]
@@ -124,13 +126,31 @@ class Oracle:
         st.fgeos = d("fgeos", problem.fgeos)
         st.f2 = d("f2", problem.f2.copy(order="F"))
         for n in ("dtdrift_r", "dtdrift_p", "dtdrift_e", "dtdrift_mu", "setrc", "elorc",
-                  "lsdr", "lscha", "lsatm"):
+                  "lsdr", "lscha", "lsatm", "lswae"):
             setattr(st, n, d(n, np.zeros(g.ns)))
+        st.lz = d("lz", g.lz)
+        wave = getattr(problem, "wave", None)
+        if wave is not None:
+            self._set_wave(st, d, i, wave)
         self._keep = keep
         self._grid = g
         self.st = st
         self._drift = {}
         self._loss = {}
+
+    @staticmethod
+    def _set_wave(st, d, i, wave):
+        """WAVELO inputs: wave.wpi (NS) flags, wave.walos1/2/3 (NS,NR,NE),
+        wave.rlpp (NT), wave.kp."""
+        st.wpi = i("wpi", wave.wpi)
+        st.walos1 = d("walos1", wave.walos1)
+        st.walos2 = d("walos2", wave.walos2)
+        st.walos3 = d("walos3", wave.walos3)
+        st.rlpp = d("rlpp", wave.rlpp)
+        st.kp = float(wave.kp)
+
+    def set_wave_kp(self, kp):
+        self.st.kp = float(kp)
 
     # -- state access -------------------------------------------------------
     @property
@@ -179,6 +199,8 @@ class Oracle:
             getattr(lib, "oracle_" + op)(C.byref(self.st), s, C.byref(self._l(s)))
         elif op == "sumrc":
             lib.oracle_sumrc(C.byref(self.st), s)
+        elif op == "wavelo":
+            lib.oracle_wavelo(C.byref(self.st), s)
         else:
             raise ValueError(op)
 

@@ -524,6 +524,40 @@ void oracle_atmol(oracle_state *st, int s, const oracle_loss_scratch *ls) {
       }
 }
 
+/* WAVELO, src/ModRamWPI.f90:580-636: electron lifetimes inside (WALOS1
+ * scaled by the wave amplitude Bw) and outside the plasmapause (WALOS2 with
+ * the strong-diffusion WALOS3, energy corrections below 5 keV; 5 days / KP
+ * above 1 MeV); F2 *= exp(-DTs/TAU) on K,I,L >= 2, all J. */
+void oracle_wavelo(oracle_state *st, int s) {
+  double bw = 30.;
+  if (st->kp >= 4.0) bw = 100.;
+  const int ns = st->ns, nr = st->nr;
+#define WL(a, i, k) (a)[(s - 1) + (size_t)ns * (((i)-1) + (size_t)nr * ((k)-1))]
+  for (int k = 2; k <= st->ne; ++k)
+    for (int i = 2; i <= st->nr; ++i)
+      for (int j = 1; j <= st->nt; ++j)
+        for (int l = 2; l <= st->npa; ++l) {
+          const double lz = V1(st->lz, i), ek = V1(st->ekev, k);
+          double tau = 0.0;
+          if (lz <= V1(st->rlpp, j)) {
+            const double b = 10. / bw;
+            tau = WL(st->walos1, i, k) * (b * b);
+          } else if (lz > V1(st->rlpp, j)) {
+            if (ek <= 1000.) {
+              tau = WL(st->walos2, i, k) * (1 + WL(st->walos3, i, k) / WL(st->walos2, i, k));
+              if (ek <= 1.1)
+                tau = tau * 37.5813 * exp(-1.81255 * ek);
+              else if (ek > 1.1 && ek <= 5.)
+                tau = tau * (7.5 - 1.15 * ek);
+            } else if (ek > 1000.) {
+              tau = 5. * 3600 * 24 / st->kp;
+            }
+          }
+          F2(s, i, j, k, l) = F2(s, i, j, k, l) * exp(-st->dts / tau);
+        }
+#undef WL
+}
+
 /* SUMRC, src/ModRamRun.f90:231-259 (sequential sum in the I,K,L,J order). */
 void oracle_sumrc(oracle_state *st, int s) {
   st->elorc[s - 1] = 0.0;
@@ -629,6 +663,7 @@ static int ram_run_species(oracle_state *st, int s) {
     return -1;
   }
   const int cex = st->cex_kind[s - 1] != ORACLE_CEX_NONE;
+  const int wpi = st->wpi && st->wpi[s - 1];
   oracle_cepara(st, s, &ls);
   oracle_driftpara(st, s, &d);
   oracle_driftr(st, s, &d);
@@ -637,6 +672,11 @@ static int ram_run_species(oracle_state *st, int s) {
   oracle_driftmu(st, s, &d);
   oracle_sumrc(st, s);
   st->lsdr[s - 1] += st->elorc[s - 1];
+  if (wpi) {
+    oracle_wavelo(st, s);
+    oracle_sumrc(st, s);
+    st->lswae[s - 1] += st->elorc[s - 1];
+  }
   if (cex) {
     oracle_charexchange(st, s, &ls);
     oracle_sumrc(st, s);
@@ -652,6 +692,11 @@ static int ram_run_species(oracle_state *st, int s) {
     oracle_charexchange(st, s, &ls);
     oracle_sumrc(st, s);
     st->lscha[s - 1] += st->elorc[s - 1];
+  }
+  if (wpi) {
+    oracle_wavelo(st, s);
+    oracle_sumrc(st, s);
+    st->lswae[s - 1] += st->elorc[s - 1];
   }
   oracle_driftmu(st, s, &d);
   oracle_drifte(st, s, &d);

@@ -51,6 +51,14 @@ typedef struct {
   double *f2;
   double *dtdrift_r, *dtdrift_p, *dtdrift_e, *dtdrift_mu; /* (NS) */
   double *setrc, *elorc, *lsdr, *lscha, *lsatm;          /* (NS) */
+  /* electron lifetimes, WAVELO (src/ModRamWPI.f90:580-636), in ram_run for
+   * species with species%WPI and DoUseWPI off; wpi NULL: off for all */
+  const int *wpi;                          /* (NS) species%WPI              */
+  const double *walos1, *walos2, *walos3;  /* (NS,NR,NE) WAVEPARA1/2 tables */
+  const double *lz;                        /* (NR+1) LZ                     */
+  const double *rlpp;                      /* (NT) plasmapause L of WAVELO  */
+  double kp;                               /* KP                            */
+  double *lswae;                           /* (NS) LSWAE                    */
 } oracle_state;
 
 /* Per-species drift tables built by DRIFTPARA (src/ModRamDrift.f90:15-19). */
@@ -83,9 +91,12 @@ void oracle_cepara(const oracle_state *st, int s, oracle_loss_scratch *l);
 void oracle_charexchange(oracle_state *st, int s, const oracle_loss_scratch *l);
 void oracle_atmol(oracle_state *st, int s, const oracle_loss_scratch *l);
 void oracle_sumrc(oracle_state *st, int s);
+/* WAVELO(S), src/ModRamWPI.f90:580-636, with the plasmapause RLpp(J) of
+ * :596-603 given in st->rlpp. */
+void oracle_wavelo(oracle_state *st, int s);
 
 /* One ram_run call (src/ModRamRun.f90:64-205) with the optional processes
- * (Coulomb, WPI, EMIC, FLC, plasmasphere) off.  nthreads>1 runs the species
+ * Coulomb, EMIC, FLC and WPADIF off; WAVELO for species with wpi set.  nthreads>1 runs the species
  * loop with OpenMP as the reference does (src/ModRamRun.f90:64).  Returns
  * DtsNext before the DtsMin floor in *dts_next (may be NULL). */
 int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next);

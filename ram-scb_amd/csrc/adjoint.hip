@@ -370,8 +370,8 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   // forward face of cell (k,l): Courant number, limited value and slots
   auto eface = [&](int k, int l, double &ce, FaceD &D, int (&tg)[4], double (&gf)[4]) {
-    ce = c_energy(d.cf + l * 8, d.kt[KT_EKP * NE + k], d.kt[KT_EDOT * NE + k], drd1, dpd1, bij, rbij, rlz, rrlz,
-                  qs);
+    ce = c_energy(d.cf + l * 8, d.kt[KT_EKP * NE + k] * qs, d.kt[KT_EDOT * NE + k], drd1, dpd1, bij, rbij, rlz,
+                  rrlz);
     const double *row = d.F + l * NE;
     const double g1 = row[1] * r0 / r1 * r2;
     const double g0 = g1 * r3 / r0 * r4;
@@ -484,7 +484,7 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
   // face L = l+1 (2..NPA-1) of column k: Courant number, value, slots (rows)
   auto mface = [&](int k, int l, double &cm, FaceD &D, int (&tg)[4]) {
-    cm = c_mu(d.cm + l * 8, d.kt[KT_MKP * NE + k], drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
+    cm = c_mu(d.cm + l * 8, d.kt[KT_MKP * NE + k] * qs, drm1, dpm1, db2, bij, rbij, rlz, rrlz);
     const int L1 = l + 1, cc = l * NE + k;
     if (L1 == NPA - 1) {  // FBND(NPA-1) = F(NPA)
       D.fb = d.F[cc + NE];
@@ -591,11 +591,15 @@ __device__ void plane_loss_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &
       double y = Y[cc];
       const double cxf = cx ? cex_factor(d, NE, k, l, d.pc[PC_DTS]) : 1.0;
       const double af = atm ? atm_factor(d, NE, k, l) : 1.0;
-      if (which == 2) {
+      // WAVELO's factor (1 for K=1 and species without it), on L>=2
+      const double wf = (l >= 1) ? d.kt[KT_WAVE * NE + k] : 1.0;
+      if (which == 2) {  // the chain WAVELO, CEX, ATMOL, ATMOL, CEX, WAVELO, reversed
+        y = y * wf;
         if (cx) y = y * cxf;
         if (atm) y = y * af;
         if (atm) y = y * af;
         if (cx) y = y * cxf;
+        y = y * wf;
       } else {
         if (cx) y = y * cxf;
         if (atm) y = y * af;
@@ -624,7 +628,7 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
     PlaneLds d;
     constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP;
     plane_setup<4>(t, u, sm, (SEQ == SEQ_STEP) ? x0 + 4 * ns : x0, need_loss, need_kl, c, d);
-    double *Y = (double *)(d.cmx + c.NE + c.NPA);  // one CFL slot per row (plane_bwd_lds_bytes)
+    double *Y = (double *)d.cmx;  // in place of the CFL slots (plane_bwd_lds_bytes)
     double *ga = (adj ? adj : t.lam) + po + c.ij * c.P;
     block_copy<4>(Y, ga, c.P);
     __syncthreads();

@@ -254,15 +254,17 @@ int ramgpu_ram_run_bwd(ramgpu_ctx *c, double *adj, double *fgeos_adj) {
   DevBuf b;
   double *tape = nullptr, *da = nullptr, *db = nullptr, *save_f2 = nullptr, *save_e = nullptr;
   if ((rc = b.alloc(&tape, T_N * n)) || (rc = b.alloc(&da, n)) || (rc = b.alloc(&save_f2, n)) ||
-      (rc = b.alloc(&save_e, (size_t)c->ns * 4)))
+      (rc = b.alloc(&save_e, (size_t)c->ns * 7)))
     return rc;
   if (fgeos_adj) {
     if ((rc = b.alloc(&db, nfg(c)))) return rc;
     HIP_TRY(hipMemcpyAsync(db, fgeos_adj, nfg(c) * sizeof(double), hipMemcpyHostToDevice, c->stream));
   }
   // the step runs from a copy of the state; the SUMRC state is restored
-  if ((rc = copy_d2d(c, tape, c->t.f2, n)) || (rc = copy_d2d(c, save_e, c->t.energy, (size_t)c->ns * 4)) ||
-      (rc = taped_step(c, tape, save_f2)) || (rc = copy_d2d(c, c->t.energy, save_e, (size_t)c->ns * 4)) ||
+  const size_t ne4 = (size_t)c->ns * 4, nx3 = (size_t)c->ns * 3;
+  if ((rc = copy_d2d(c, tape, c->t.f2, n)) || (rc = copy_d2d(c, save_e, c->t.energy, ne4)) ||
+      (rc = copy_d2d(c, save_e + ne4, c->t.lsx, nx3)) || (rc = taped_step(c, tape, save_f2)) ||
+      (rc = copy_d2d(c, c->t.energy, save_e, ne4)) || (rc = copy_d2d(c, c->t.lsx, save_e + ne4, nx3)) ||
       (rc = upload_f2(c, da, adj)) || (rc = step_bwd(c, tape, da, db)) || (rc = download_f2(c, adj, da)))
     return rc;
   if (db) {

@@ -119,8 +119,11 @@ int ready(ramgpu_ctx *c) {
   if (!c->have_vt) return fail(RAMGPU_ESTATE, "no electric field (ramgpu_set_efield/ramgpu_set_kp)");
   if (!c->have_bnd) return fail(RAMGPU_ESTATE, "ramgpu_set_boundary has not been called");
   if (!c->have_f2) return fail(RAMGPU_ESTATE, "ramgpu_put_f2 has not been called");
-  for (int s = 0; s < c->ns; ++s)
+  for (int s = 0; s < c->ns; ++s) {
     if (!c->have_species[s]) return fail(RAMGPU_ESTATE, "species %d not set", s + 1);
+    if (c->t.wpi[s] && !c->have_wave)
+      return fail(RAMGPU_ESTATE, "species %d takes WAVELO: ramgpu_set_wave_params has not been called", s + 1);
+  }
   return RAMGPU_OK;
 }
 
@@ -340,6 +343,7 @@ void launch_plane_t(int seq, int threads, dim3 grid, size_t lds, hipStream_t st,
     case SEQ_SUM: hipLaunchKernelGGL((k_plane<SEQ_SUM, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_LOSS: hipLaunchKernelGGL((k_plane<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_TAPE: hipLaunchKernelGGL((k_plane<SEQ_TAPE, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_WAVE: hipLaunchKernelGGL((k_plane<SEQ_WAVE, MAXC>), grid, blk, lds, st, L, rec); break;
     default: hipLaunchKernelGGL((k_plane<SEQ_STEP, MAXC>), grid, blk, lds, st, L, rec); break;
   }
 }
@@ -417,15 +421,14 @@ int check_dims(int nr, int nt, int ne, int npa) {
 // Let the plane kernels and the radial sweep use the device's whole LDS
 // (less each kernel's static LDS); refuse planes that do not fit it.
 int set_plane_lds_limit(size_t limit) {
+#define PLANE_K(S) (const void *)k_plane<S, 8>, (const void *)k_plane<S, 16>
   const void *fns[] = {
-      (const void *)k_plane<SEQ_E, 8>,    (const void *)k_plane<SEQ_MU, 8>,   (const void *)k_plane<SEQ_CEX, 8>,
-      (const void *)k_plane<SEQ_ATM, 8>,  (const void *)k_plane<SEQ_SUM, 8>,  (const void *)k_plane<SEQ_STEP, 8>,
-      (const void *)k_plane<SEQ_LOSS, 8>, (const void *)k_plane<SEQ_E, 16>,   (const void *)k_plane<SEQ_MU, 16>,
-      (const void *)k_plane<SEQ_CEX, 16>, (const void *)k_plane<SEQ_ATM, 16>, (const void *)k_plane<SEQ_SUM, 16>,
-      (const void *)k_plane<SEQ_STEP, 16>, (const void *)k_plane<SEQ_LOSS, 16>, (const void *)k_plane<SEQ_TAPE, 8>,
-      (const void *)k_plane<SEQ_TAPE, 16>, (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
+      PLANE_K(SEQ_E), PLANE_K(SEQ_MU), PLANE_K(SEQ_CEX), PLANE_K(SEQ_ATM), PLANE_K(SEQ_SUM),
+      PLANE_K(SEQ_STEP), PLANE_K(SEQ_LOSS), PLANE_K(SEQ_TAPE), PLANE_K(SEQ_WAVE),
+      (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
       (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>, (const void *)k_p<0>, (const void *)k_p<1>,
       (const void *)k_p_bwd};
+#undef PLANE_K
   for (const void *f : fns) {
     hipFuncAttributes fa;
     if (hipFuncGetAttributes(&fa, f) != hipSuccess ||
@@ -442,9 +445,10 @@ int set_plane_lds_limit(size_t limit) {
 
 int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
   if (!out) return RAMGPU_OK;
-  double e[4 * kMaxSpecies], d[4 * kMaxSpecies];
+  double e[4 * kMaxSpecies], d[4 * kMaxSpecies], x[3 * kMaxSpecies];
   HIP_TRY(hipMemcpyAsync(e, c->t.energy, (size_t)c->ns * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipMemcpyAsync(d, c->t.dtdrift, (size_t)c->ns * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipMemcpyAsync(x, c->t.lsx, (size_t)c->ns * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int s = 0; s < c->ns; ++s) {
     out[s].setrc = e[s * 4 + 0];
@@ -455,6 +459,9 @@ int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
     out[s].dtdrift_p = d[s * 4 + 1];
     out[s].dtdrift_e = d[s * 4 + 2];
     out[s].dtdrift_mu = d[s * 4 + 3];
+    out[s].lswae = x[s * 3 + 0];
+    out[s].lscoe = x[s * 3 + 1];
+    out[s].lscsc = x[s * 3 + 2];
   }
   return RAMGPU_OK;
 }
@@ -564,6 +571,8 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   A(c->d_stage, nf2);
   A(t.partials, (size_t)(kSumSlots + 1) * ns * t.maxblk);  // + ELORC slot of ramgpu_sumrc
   A(t.energy, (size_t)ns * 4);
+  A(t.wl, (size_t)3 * ns * nr * ne); A(t.rlpp, nt); A(t.lsx, (size_t)ns * 3);
+  A(t.wavef, (size_t)ns * nr * nt * ne);
   if (hipMemcpy(t.ex64, kExp2Tab, sizeof(kExp2Tab), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(fail(RAMGPU_EHIP, "hipMemcpy failed"));
 #undef A
@@ -573,7 +582,7 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   t.fnhs = c->d_fnhs; t.fnis = c->d_fnis; t.bounhs = c->d_bounhs; t.bounis = c->d_bounis;
   t.hdns_f = c->d_hdnsf; t.didt = c->d_didt; t.dibndt = c->d_dibndt; t.bnes = c->d_bnes;
   t.dbdt = c->d_dbdt; t.vt = c->d_vt; t.eir = c->d_eir; t.eip = c->d_eip; t.mgnp = c->d_mgnp;
-  t.fgeos = c->d_fgeos; t.bscale = c->d_bscale; t.lz = c->d_lz; t.phi = c->d_phi;
+  t.fgeos = c->d_fgeos; t.bscale = c->d_bscale; t.lz = c->d_lz; t.phi = c->d_phi; t.kpnow = c->d_kp;
   t.dts = 5.0; t.betalim = 1.5; t.fraccfl = 0.8;
   t.rnt = 1.0f / (float)nt;
   t.rne = 1.0f / (float)ne;
@@ -742,6 +751,9 @@ int ramgpu_set_kp(ramgpu_ctx *c, double kp, const double *lz, const double *phi)
                      (const double *)c->d_phi, (const double *)nullptr, kp, c->nr + 1, c->nt,
                      (const double *)nullptr, (double *)nullptr, 0);
   if ((rc = launch_check("k_vs"))) return rc;
+  c->kp_host = kp;  // WAVELO's KP follows the convection's
+  HIP_TRY(hipMemcpyAsync(c->d_kp, &c->kp_host, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   c->have_vt = true;
   c->efield_dirty = true;
   return RAMGPU_OK;
@@ -852,6 +864,92 @@ int ramgpu_atmol(ramgpu_ctx *c, int s) {
   if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
   if ((rc = run_plane(c, SEQ_ATM, 0, s - 1, 1))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+int ramgpu_set_wavelo(ramgpu_ctx *c, int s, const double *walos1, const double *walos2, const double *walos3) {
+  int rc = check_ctx(c);
+  if (rc || (rc = check_species(c, s))) return rc;
+  if (!walos1 && !walos2 && !walos3) {
+    c->t.wpi[s - 1] = 0;
+    c->tab_dirty = true;
+    return RAMGPU_OK;
+  }
+  if (!walos1 || !walos2 || !walos3) return fail(RAMGPU_EINVAL, "WALOS1, WALOS2 and WALOS3 must all be given");
+  const size_t nr = c->nr, ne = c->ne, nik = nr * ne;
+  std::vector<double> w(3 * nik);
+  const double *src[3] = {walos1, walos2, walos3};
+  for (int q = 0; q < 3; ++q)
+    for (size_t k = 0; k < ne; ++k)
+      for (size_t i = 0; i < nr; ++i) w[q * nik + i * ne + k] = src[q][i + nr * k];  // (NR,NE) -> [i][k]
+  for (int q = 0; q < 3; ++q)
+    HIP_TRY(hipMemcpy(c->t.wl + ((size_t)q * c->ns + (s - 1)) * nik, w.data() + q * nik, nik * sizeof(double),
+                      hipMemcpyHostToDevice));
+  c->t.wpi[s - 1] = 1;
+  c->tab_dirty = true;
+  c->efield_dirty = true;  // the step's factors are formed with the Courant-number terms
+  return RAMGPU_OK;
+}
+
+int ramgpu_set_wave_params(ramgpu_ctx *c, double kp, double kpmax12, const double *lz, const double *necr,
+                           int use_plasmasphere) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!lz) return fail(RAMGPU_EINVAL, "lz is null");
+  if (use_plasmasphere && !necr) return fail(RAMGPU_EINVAL, "a plasmasphere model needs NECR");
+  // the plasmapause of WAVELO, src/ModRamWPI.f90:598-603
+  std::vector<double> rl(c->nt);
+  for (int j = 0; j < c->nt; ++j) {
+    rl[j] = 5.39 - 0.382 * kpmax12;
+    if (use_plasmasphere)
+      for (int i = 1; i < c->nr; ++i)
+        if (necr[i + (size_t)c->nr * j] > 50.) rl[j] = lz[i];
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(c->t.rlpp, rl.data(), rl.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_lz, lz, (size_t)(c->nr + 1) * sizeof(double), hipMemcpyHostToDevice));  // WAVELO's LZ(I)
+  c->kp_host = kp;
+  HIP_TRY(hipMemcpy(c->d_kp, &c->kp_host, sizeof(double), hipMemcpyHostToDevice));
+  c->have_wave = true;
+  c->efield_dirty = true;
+  return RAMGPU_OK;
+}
+
+int ramgpu_wavelo(ramgpu_ctx *c, int s) {
+  int rc = check_ctx(c);
+  if (rc || (rc = check_species(c, s)) || (rc = prepare(c))) return rc;
+  if (!c->t.wpi[s - 1]) return fail(RAMGPU_ESTATE, "species %d has no WAVELO tables (ramgpu_set_wavelo)", s);
+  if ((rc = run_plane(c, SEQ_WAVE, 0, s - 1, 1))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+int ramgpu_set_loss_state(ramgpu_ctx *c, const double *lswae, const double *lscoe, const double *lscsc) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  std::vector<double> e(c->ns * 3);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(e.data(), c->t.lsx, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->ns; ++s) {
+    if (lswae) e[s * 3 + 0] = lswae[s];
+    if (lscoe) e[s * 3 + 1] = lscoe[s];
+    if (lscsc) e[s * 3 + 2] = lscsc[s];
+  }
+  HIP_TRY(hipMemcpy(c->t.lsx, e.data(), e.size() * sizeof(double), hipMemcpyHostToDevice));
+  return RAMGPU_OK;
+}
+
+int ramgpu_get_loss_state(ramgpu_ctx *c, double *lswae, double *lscoe, double *lscsc) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  std::vector<double> e(c->ns * 3);
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(e.data(), c->t.lsx, e.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int s = 0; s < c->ns; ++s) {
+    if (lswae) lswae[s] = e[s * 3 + 0];
+    if (lscoe) lscoe[s] = e[s * 3 + 1];
+    if (lscsc) lscsc[s] = e[s * 3 + 2];
+  }
   return RAMGPU_OK;
 }
 

@@ -60,6 +60,11 @@ struct ramgpu_batch {
   hipGraphExec_t exec[2] = {nullptr, nullptr};  // [0] plain step, [1] scheduled step
   std::vector<void *> allocs;
   int ncu = 256;
+  // opt-in per-launch timing (eager steps only): event pairs by kernel kind
+  bool timing = false;
+  std::vector<hipEvent_t> tev;
+  std::vector<int> tkind;
+  size_t tused = 0;
 };
 
 namespace {
@@ -150,6 +155,30 @@ int sync_members(ramgpu_batch *b) {
   return RAMGPU_OK;
 }
 
+// Event pair around one launch of kind `kind` while timing is on.
+void btbeg(ramgpu_batch *b, int kind) {
+  if (!b->timing) return;
+  if (2 * b->tused + 2 > b->tev.size()) {
+    for (int n = 0; n < 2; ++n) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        b->timing = false;
+        return;
+      }
+      b->tev.push_back(e);
+    }
+  }
+  if (b->tkind.size() < b->tused + 1) b->tkind.resize(b->tused + 1);
+  b->tkind[b->tused] = kind;
+  (void)hipEventRecord(b->tev[2 * b->tused], b->stream);
+}
+
+void btend(ramgpu_batch *b) {
+  if (!b->timing) return;
+  (void)hipEventRecord(b->tev[2 * b->tused + 1], b->stream);
+  ++b->tused;
+}
+
 // The launches of one ensemble step.  sched: VT / boundary scale of the step
 // from the device schedule, then the VT-dependent tables and the ghost carry.
 // CFL limits are recorded every step (the same values a member's own
@@ -158,6 +187,7 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
   hipStream_t st = b->stream;
   const int M = (int)b->m.size();
   if (sched) {
+    btbeg(b, RAMGPU_KERNEL_PREPARE);
     const Launch ls = launch_for(b, BK_SCHED);
     hipLaunchKernelGGL(k_sched, dim3(grid_for(b, BK_SCHED)), dim3(256), 0, st, ls, (const double *)b->d_kp,
                        (const double *)b->d_bs, (const int *)b->d_bsoff, (const int *)b->d_step, M, b->sum_ns);
@@ -166,16 +196,29 @@ int enqueue_step(ramgpu_batch *b, bool sched) {
                      (const double *)nullptr);
     const Launch lg = launch_for(b, BK_GHOST);
     hipLaunchKernelGGL(k_ghost, dim3(grid_for(b, BK_GHOST)), dim3(1024), b->ghost_lds, st, lg);
+    btend(b);
   }
   const Launch lr = launch_for(b, BK_R), lp = launch_for(b, BK_P), lk = launch_for(b, BK_PLANE),
                lf = launch_for(b, BK_FIN);
+  btbeg(b, RAMGPU_KERNEL_R);
   hipLaunchKernelGGL(k_r<1>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, lr);
+  btend(b);
+  btbeg(b, RAMGPU_KERNEL_P);
   hipLaunchKernelGGL(k_p<1>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, lp);
+  btend(b);
+  btbeg(b, RAMGPU_KERNEL_PLANE);
   launch_plane(SEQ_STEP, b->plane, dim3(grid_for(b, BK_PLANE)), b->plane_lds, st, lk, 1);
+  btend(b);
+  btbeg(b, RAMGPU_KERNEL_P);
   hipLaunchKernelGGL(k_p<0>, dim3(grid_for(b, BK_P)), dim3(kLineThreads), b->azim_lds, st, lp);
+  btend(b);
+  btbeg(b, RAMGPU_KERNEL_R);
   hipLaunchKernelGGL(k_r<2>, dim3(grid_for(b, BK_R)), dim3(kLineThreads), b->radial_lds, st, lr);
+  btend(b);
+  btbeg(b, RAMGPU_KERNEL_FINALIZE);
   hipLaunchKernelGGL(k_finalize, dim3(grid_for(b, BK_FIN)), dim3(kFinalizeThreads), 0, st, lf, 15, 2);
   if (sched) hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, st, b->d_step);
+  btend(b);
   return launch_check("ensemble step");
 }
 
@@ -287,9 +330,10 @@ int ensure_graph(ramgpu_batch *b, bool sched) {
 
 int run_steps(ramgpu_batch *b, int nsteps, bool sched) {
   int rc;
-  if (b->use_graph && (rc = ensure_graph(b, sched))) return rc;
+  const bool graph = b->use_graph && !b->timing;  // per-launch events need eager launches
+  if (graph && (rc = ensure_graph(b, sched))) return rc;
   for (int n = 0; n < nsteps; ++n) {
-    if (b->use_graph) {
+    if (graph) {
       HIP_TRY(hipGraphLaunch(b->exec[sched ? 1 : 0], b->stream));
     } else if ((rc = enqueue_step(b, sched))) {
       return rc;
@@ -390,6 +434,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->plane.threads = std::max(b->plane.threads, c->plane.threads);
     b->plane.maxc = std::max(b->plane.maxc, c->plane.maxc);
   }
+
   if ((rc = batch_alloc(b, (void **)&b->d_bsoff, nmembers * sizeof(int))) ||
       (rc = batch_alloc(b, (void **)&b->d_step, sizeof(int))))
     return cleanup(rc);
@@ -407,6 +452,7 @@ int ramgpu_batch_destroy(ramgpu_batch *b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   destroy_graphs(b);
   for (void *p : b->allocs) (void)hipFree(p);
+  for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -447,6 +493,34 @@ int ramgpu_batch_ram_run_schedule(ramgpu_batch *b, int nsteps, const double *kp,
   if ((rc = run_steps(b, nsteps, true))) return rc;
   HIP_TRY(hipEventRecord(b->ev1, b->stream));
   return finish(b, true, out);
+}
+
+int ramgpu_batch_kernel_timing(ramgpu_batch *b, int enable) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->timing = enable != 0;
+  b->tused = 0;
+  return RAMGPU_OK;
+}
+
+int ramgpu_batch_kernel_times(ramgpu_batch *b, double *total_ms, int *launches) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!total_ms || !launches) return fail(RAMGPU_EINVAL, "null output array");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  for (int k = 0; k < RAMGPU_KERNEL_KINDS; ++k) {
+    total_ms[k] = 0.0;
+    launches[k] = 0;
+  }
+  for (size_t n = 0; n < b->tused; ++n) {
+    float f = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, b->tev[2 * n], b->tev[2 * n + 1]));
+    total_ms[b->tkind[n]] += f;
+    launches[b->tkind[n]] += 1;
+  }
+  b->tused = 0;
+  return RAMGPU_OK;
 }
 
 int ramgpu_batch_last_run_ms(ramgpu_batch *b, double *ms) {

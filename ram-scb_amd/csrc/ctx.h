@@ -78,6 +78,8 @@ struct ramgpu_ctx {
   bool have_grid = false, have_geom = false, have_vt = false, have_bnd = false, have_f2 = false;
   bool have_species[kMaxSpecies] = {};
   bool have_lzphi = false;
+  bool have_wave = false;  // ramgpu_set_wave_params called (WAVELO's KP and plasmapause)
+  double kp_host = 0.0;    // staging of the Kp copied to Tab::kpnow
   bool tables_dirty = true, ghost_dirty = true;
   bool efield_dirty = true;  // only the VT-dependent per-(I,J) terms are stale
   // one step of a Kp schedule (device pointers to its Kp and boundary

@@ -186,6 +186,7 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
     const Tab &t = *u.t;
     if (bs && u.b == 0)
       for (int s = threadIdx.x; s < t.ns; s += blockDim.x) t.bscale[s] = bs[s];
+    if (kp && u.b == 0 && threadIdx.x == 0) *t.kpnow = *kp;  // WAVELO's KP of the step
     if (kp) stage_vt_factors(t, svt);
     const int n = u.b * blockDim.x + threadIdx.x;
     if (n < t.nr * t.nt) {
@@ -235,6 +236,15 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
         double *vt = const_cast<double *>(t.vt);
         GEO2(vt, I, J) = VTV(I, J);
         if (I == t.nr) GEO2(vt, I + 1, J) = VTV(I + 1, J);
+      }
+      // WAVELO's factors of this (I,J) at the step's Kp, K >= 2, for the
+      // species that take it (src/ModRamWPI.f90:596-630)
+      if (I >= 2) {
+        const double kpn = kp ? *kp : *t.kpnow;
+        for (int s = 0; s < t.ns; ++s)
+          if (t.wpi[s])
+            for (int k = 1; k < t.ne; ++k)
+              t.wavef[(((size_t)s * t.nr + i) * NT + j) * t.ne + k] = wave_factor(t, s, i, j, k, kpn);
       }
     }
     if (kp) __syncthreads();  // svt is rewritten for the block's next unit
@@ -604,7 +614,7 @@ __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
     if (MODE == 2) {
       double a[1] = {acc}, tot[1];
       block_sum_n<1>(a, red, tot);
-      if (threadIdx.x == 0) t.partials[((size_t)5 * t.ns + s) * t.maxblk + u.b] = tot[0];
+      if (threadIdx.x == 0) t.partials[((size_t)(kSumSlots - 1) * t.ns + s) * t.maxblk + u.b] = tot[0];
     }
     __syncthreads();  // the face terms of this block are read
   }
@@ -762,9 +772,11 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
   __syncthreads();
   const double *k8 = d.pc + PC_KIJ;
   const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
+  const double rlz = c.rlz, rrlz = c.rrlz, beta = c.beta;
   const int k = c.ke;
-  const double ekp = d.kt[KT_EKP * NE + k], edot = d.kt[KT_EDOT * NE + k];
+  // QS = +-1 folded into the prefix: the chain of correctly rounded
+  // quotients is odd, so the sign comes out exactly as the reference's /QS
+  const double ekp = d.kt[KT_EKP * NE + k] * c.qs, edot = d.kt[KT_EDOT * NE + k];
   const double de = d.kt[KT_DE * NE + k], rde = d.kt[KT_RDE * NE + k];
   const double we = d.kt[KT_WE * NE + k], rwe = d.kt[KT_RWE * NE + k];
   const bool top = k + 1 >= NE;
@@ -779,7 +791,7 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
-    const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz, qs);
+    const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz);
     cmax = fmax(cmax, fmax(fabs(ce), 1E-10));
     const bool neg = ce < 0;
     const double fk = d.F[cc];
@@ -827,9 +839,9 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
   const double *k8 = d.pc + PC_KIJ;
   const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
   const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double rlz = c.rlz, rrlz = c.rrlz, qs = c.qs, beta = c.beta;
+  const double rlz = c.rlz, rrlz = c.rrlz, beta = c.beta;
   const int k = c.ke;
-  const double mkp = d.kt[KT_MKP * NE + k];
+  const double mkp = d.kt[KT_MKP * NE + k] * c.qs;  // QS folded in, as in plane_e
   double O[MAXC];
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
@@ -838,7 +850,7 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
     // every row forms the face value; rows 1 and NPA keep 0 (face 1:
     // CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0; row NPA is
     // the top cell).  Reads past the plane land in H and are masked.
-    const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz, qs);
+    const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz);
     // the row's maximum in kMuSpread slots (by K), so that fewer lanes of a
     // wave update one LDS address
     if (rec && L1 >= 2) atomic_max_pos(&d.cmx[NE + kMuSpread * l + (k & (kMuSpread - 1))], fmax(fabs(cm), 1E-32));
@@ -885,8 +897,13 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
 }
 
 // The loss chain of ram_run between the two drift halves (src/ModRamRun.f90:
-// 77-141): SUMRC, [CEX, SUMRC], ATMOL, SUMRC, ATMOL, SUMRC, [CEX, SUMRC], with
-// the SUMRC partials of this plane block b in slots 0..4.
+// 77-159): SUMRC, [WAVELO, SUMRC], [CEX, SUMRC], ATMOL, SUMRC, ATMOL, SUMRC,
+// [CEX, SUMRC], [WAVELO, SUMRC], with the SUMRC partials of this plane block b
+// in slots 0..6.  An operator a species does not take multiplies by 1, which
+// leaves its cells and so its slot's sum bit for bit those of the slot before.
+// WAVELO's factor is one per thread (it depends on I, J, K only), so a
+// thread's partial sums after it are its sums before it times the factor
+// (to rounding: exactly when the factor is 1).
 template <int MAXC>
 __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, int b) {
   const int NE = c.NE, NPA = c.NPA, NT = c.NT;
@@ -896,9 +913,11 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   const int k = c.ke;
   const double sigv = d.kt[KT_SIGV * NE + k], latl = d.kt[KT_LATL * NE + k];
   const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
+  const double wave = d.kt[KT_WAVE * NE + k];  // 1 for K=1 and without WAVELO
+  const double wave0 = (c.le0 == 0) ? 1.0 : wave;  // the first cell may be row L=1, which WAVELO skips
   const double dts = d.pc[PC_DTS];
   const double ekw = ek * wk;
-  double a[5] = {0, 0, 0, 0, 0}, tot[5];
+  double a[5] = {0, 0, 0, 0, 0}, tot[7];
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
     ROW_CELLS(n);
@@ -907,13 +926,16 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
     const bool atm = k >= 1 && l + 1 >= upa;
     double f = d.F[cc];
     // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49);
-    // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS)); a factor of 1
-    // leaves a cell outside the operator's range unchanged, and a weight of 0
-    // leaves it out of the sums
+    // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS)); WAVELO's
+    // exp(-DTs/TAU) on L>=2 (src/ModRamWPI.f90:630); a factor of 1 leaves a
+    // cell outside the operator's range unchanged, and a weight of 0 leaves
+    // it out of the sums
     const double cxf = cx ? exp_tab(-(sigv * d.cl[2 * l] * dts), d.ex) : 1.0;
     const double af = atm ? exp_tab(latl * d.cl[2 * l + 1], d.ex) : 1.0;
+    const double wf = (n == 0) ? wave0 : wave;
     const double q = w_on ? ekw * d.lt[LT_WMU * NPA + l] : 0.0;  // EKEV*WE*WMU
     a[0] = fma(f, q, a[0]);
+    f = f * wf;
     f = f * cxf;
     a[1] = fma(f, q, a[1]);
     f = f * af;
@@ -922,12 +944,14 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
     a[3] = fma(f, q, a[3]);
     f = f * cxf;
     a[4] = fma(f, q, a[4]);
-    d.F[cc] = f;
+    d.F[cc] = f * wf;
   }
-  block_sum_n<5>(a, d.red, tot);
-  if (threadIdx.x < 5) {
+  // slots: before WAVELO, after it, after CEX, ATMOL, ATMOL, CEX, after WAVELO
+  double s7[7] = {a[0], a[0] * wave, a[1], a[2], a[3], a[4], a[4] * wave};
+  block_sum_n<7>(s7, d.red, tot);
+  if (threadIdx.x < 7) {
     double v = tot[0];
-    for (int m = 1; m < 5; ++m)
+    for (int m = 1; m < 7; ++m)
       if (threadIdx.x == m) v = tot[m];
     t.partials[((size_t)threadIdx.x * t.ns + c.s) * t.maxblk + b] = v;
   }
@@ -951,8 +975,8 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       block_copy<8>(dst, src, NE * NPA);
       continue;
     }
-    constexpr bool need_loss =
-        SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS || SEQ == SEQ_TAPE;
+    constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS ||
+                               SEQ == SEQ_TAPE || SEQ == SEQ_WAVE;
     PlaneCtx c;
     PlaneLds d;
     constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE;
@@ -983,6 +1007,15 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
           ROW_CELLS(n);
           if (l + 1 >= upa) d.F[cc] = d.F[cc] * atm_factor(d, NE, c.ke, l);
         }
+      }
+      __syncthreads();
+    } else if constexpr (SEQ == SEQ_WAVE) {
+      // WAVELO, src/ModRamWPI.f90:612-633: K>=2, L>=2 (the factor is 1 for K=1)
+      const double wf = d.kt[KT_WAVE * NE + c.ke];
+#pragma unroll
+      for (int n = 0; n < MAXC; ++n) {
+        ROW_CELLS(n);
+        if (l >= 1) d.F[cc] = d.F[cc] * wf;
       }
       __syncthreads();
     } else if constexpr (SEQ == SEQ_SUM) {
@@ -1053,15 +1086,17 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
     unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
     // every load of the block first: thread 0's scalars, the CFL maxima, the
     // plane and radial blocks' partial sums
-    double e0[4] = {0.0, 0.0, 0.0, 0.0};
+    double e0[4] = {0.0, 0.0, 0.0, 0.0}, lswae = 0.0;
     unsigned long long m01[2] = {0ull, 0ull};
-    int cex = 0;
+    int cex = 0, wpi = 0;
     if (tid == 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) e0[q] = t.energy[s * 4 + q];
+      lswae = t.lsx[s * 3];
       m01[0] = m[0];
       m01[1] = m[1];
       cex = t.cex[s];
+      wpi = t.wpi[s];
     }
     double v[NV];
     v[0] = 10000.0;
@@ -1079,15 +1114,15 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
         if (c > 0) v[1] = fmin(v[1], a * t.dmu[l] / c);
       }
     if (summode != 0) {
-      // slots 0..4 from the plane blocks, slot 5 from the R blocks
+      // slots 0..6 from the plane blocks, the last from the R blocks
       const int nslot = (summode == 1) ? 1 : kSumSlots;
       const int nbk = blocks_kl(t), nbr = blocks_r(t);
       const double *pp = t.partials + (size_t)s * t.maxblk;
       const size_t slot = (size_t)t.ns * t.maxblk;
       for (int b = tid; b < nbk; b += T)
-        for (int sl = 0; sl < nslot && sl < 5; ++sl) v[2 + sl] += pp[sl * slot + b];
+        for (int sl = 0; sl < nslot && sl < kSumSlots - 1; ++sl) v[2 + sl] += pp[sl * slot + b];
       if (nslot == kSumSlots)
-        for (int b = tid; b < nbr; b += T) v[2 + 5] += pp[5 * slot + b];
+        for (int b = tid; b < nbr; b += T) v[2 + kSumSlots - 1] += pp[(kSumSlots - 1) * slot + b];
     }
     // one block reduction (fixed order): minima of v[0..1], sums of the rest
     for (int off = 32; off > 0; off >>= 1) {
@@ -1129,27 +1164,38 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
           e[0] = S[0];
           t.partials[((size_t)kSumSlots * t.ns + s) * t.maxblk] = old - S[0];  // ELORC
         } else {
+          // the accumulators of src/ModRamRun.f90:77-175 from consecutive
+          // slot sums, each operator's only where the reference runs it
           double lsdr = e0[1], lscha = e0[2], lsatm = e0[3];
           double prev = old;
           lsdr += prev - S[0];
           prev = S[0];
-          if (cex) {
-            lscha += prev - S[1];
+          if (wpi) {
+            lswae += prev - S[1];
             prev = S[1];
           }
-          lsatm += prev - S[2];
-          prev = S[2];
+          if (cex) {
+            lscha += prev - S[2];
+            prev = S[2];
+          }
           lsatm += prev - S[3];
           prev = S[3];
+          lsatm += prev - S[4];
+          prev = S[4];
           if (cex) {
-            lscha += prev - S[4];
-            prev = S[4];
+            lscha += prev - S[5];
+            prev = S[5];
           }
-          lsdr += prev - S[5];
-          e[0] = S[5];
+          if (wpi) {
+            lswae += prev - S[6];
+            prev = S[6];
+          }
+          lsdr += prev - S[7];
+          e[0] = S[7];
           e[1] = lsdr;
           e[2] = lscha;
           e[3] = lsatm;
+          t.lsx[s * 3] = lswae;
         }
       }
     }
@@ -1219,6 +1265,7 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
     const int m = (int)(u.t - L.tabs);
     const int n = *step;
     const double avs = vs_avs(kp[(size_t)n * nmembers + m]);
+    if (threadIdx.x == 0) *t.kpnow = kp[(size_t)n * nmembers + m];  // WAVELO's KP of the step
     double *vt = const_cast<double *>(t.vt);
     const int nr1 = t.nr + 1;
     for (int x = threadIdx.x; x < nr1 * t.nt; x += blockDim.x) {
@@ -1302,6 +1349,8 @@ template __global__ void k_r<1>(Launch);
 template __global__ void k_r<2>(Launch);
 template __global__ void k_p<0>(Launch);
 template __global__ void k_p<1>(Launch);
+template __global__ void k_plane<SEQ_WAVE, 8>(Launch, int);
+template __global__ void k_plane<SEQ_WAVE, 16>(Launch, int);
 template __global__ void k_plane<SEQ_E, 8>(Launch, int);
 template __global__ void k_plane<SEQ_MU, 8>(Launch, int);
 template __global__ void k_plane<SEQ_CEX, 8>(Launch, int);

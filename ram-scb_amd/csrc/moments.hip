@@ -206,8 +206,8 @@ int alloc_zero(ramgpu_ctx *c, double **p, size_t n) {
   if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(double)) != hipSuccess)
     return fail(RAMGPU_ENOMEM, "hipMalloc of %zu bytes failed", n * sizeof(double));
   c->allocs.push_back(q);
-  HIP_TRY(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(double), c->stream));
   *p = (double *)q;
+  HIP_TRY(hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(double), c->stream));
   return RAMGPU_OK;
 }
 
@@ -217,7 +217,8 @@ int run_moments(ramgpu_ctx *c, int s0, int nsp, int anisch, bool flux) {
   if (!c->have_grid) return fail(RAMGPU_ESTATE, "ramgpu_set_grid has not been called");
   if (!c->have_geom) return fail(RAMGPU_ESTATE, "ramgpu_set_geometry has not been called");
   if (!c->have_f2) return fail(RAMGPU_ESTATE, "ramgpu_put_f2 has not been called");
-  if (!c->d_ff) return fail(RAMGPU_ESTATE, "ramgpu_set_moment_factors has not been called");
+  if (!c->d_ff || !c->d_ernh || !c->d_epp || !c->d_mom)
+    return fail(RAMGPU_ESTATE, "ramgpu_set_moment_factors has not been called (or its allocation failed)");
   if (c->ne > kMomMaxNE) return fail(RAMGPU_EINVAL, "moments: NE=%d above %d", c->ne, kMomMaxNE);
   int rc;
   if (flux && !c->d_flux && (rc = alloc_zero(c, &c->d_flux, (size_t)c->ns * c->nr * c->nt * c->ne * c->npa)))
@@ -278,11 +279,12 @@ int ramgpu_set_moment_factors(ramgpu_ctx *c, const double *ffactor, const double
   if (rc) return rc;
   if (!ffactor || !ernh || !epp) return fail(RAMGPU_EINVAL, "ffactor, ernh and epp must not be null");
   const size_t ns = c->ns, nr = c->nr, ne = c->ne, npa = c->npa;
-  if (!c->d_ff) {
-    if ((rc = alloc_zero(c, &c->d_ff, 3 * ns * nr * ne * npa)) || (rc = alloc_zero(c, &c->d_ernh, ns * ne)) ||
-        (rc = alloc_zero(c, &c->d_epp, ns * ne)) || (rc = alloc_zero(c, &c->d_mom, 4 * ns * nr * c->nt)))
-      return rc;
-  }
+  // each buffer on its own test: a retry after a failed allocation completes the set
+  if ((!c->d_ff && (rc = alloc_zero(c, &c->d_ff, 3 * ns * nr * ne * npa))) ||
+      (!c->d_ernh && (rc = alloc_zero(c, &c->d_ernh, ns * ne))) ||
+      (!c->d_epp && (rc = alloc_zero(c, &c->d_epp, ns * ne))) ||
+      (!c->d_mom && (rc = alloc_zero(c, &c->d_mom, 4 * ns * nr * c->nt))))
+    return rc;
   // FFACTOR(NS,NR,NE,NPA) -> [s][i][l][k], so that a plane's K run is contiguous
   std::vector<double> t(ns * nr * ne * npa);
   for (size_t l = 0; l < npa; ++l)

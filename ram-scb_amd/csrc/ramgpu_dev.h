@@ -27,7 +27,7 @@ constexpr int kPlaneThreads = 512;     // plane adjoint kernels: one (K,L) plane
 constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chosen per problem
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bound, checked at creation
-constexpr int kSumSlots = 6;           // SUMRC partial-sum slots of one ram_run
+constexpr int kSumSlots = 8;           // SUMRC partial-sum slots of one ram_run (7 plane, 1 radial)
 constexpr int kMuSpread = 8;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 8)
 constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 
@@ -81,6 +81,16 @@ struct Tab {
   double *rel;    // [s][8]: GREL1, GREL2, sq1, GRZERO, sq0
   int *carry;     // [s][NE*NPA] DRIFTR ghost carry
   double *latl;   // [s][i][k] log(ATLOS): ATLOS**(1/FNHS) = exp(log(ATLOS)/FNHS)
+  // WAVELO (electron lifetimes) of the species with wpi[s] set: WALOS1/2/3
+  // as wl[(q*NS + s)*NR*NE + i*NE + k], q = 0,1,2; the plasmapause L of each
+  // local time; KP of the current step (written by ramgpu_set_kp, the Kp
+  // schedules and ramgpu_set_wave_params)
+  int wpi[kMaxSpecies];
+  double *wl;
+  double *wavef;  // [((s*NR + i)*NT + j)*NE + k] exp(-DTs/TAU) of the step (k_coef_ij)
+  double *rlpp;   // [j]
+  double *kpnow;  // 1 value
+  double *lsx;    // [s][3]: LSWAE, LSCOE, LSCSC
   double *ex64;   // [j] 2^(j/64), the table of the loss factors' exponential
   // CFL maxima of max(|C|,floor) over cells inside the magnetopause, as the
   // bit patterns of non-negative doubles (atomicMax order = double order):
@@ -174,15 +184,16 @@ inline __host__ __device__ size_t radial_lds_bytes(int nr, int ne, int npa) {
 
 // LDS bytes of the plane kernel
 // LDS of the forward plane kernels; mu_slots: LDS slots per row of DRIFTMU's
-// CFL maximum (the adjoint kernels record none and lay out one, followed by
-// their adjoint plane: plane_bwd_lds_bytes)
+// CFL maximum
 inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa, int mu_slots = kMuSpread) {
   const size_t P = (size_t)ne * npa;
-  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 10 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
+  return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 11 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
           2 * (size_t)npa + (size_t)ne + (size_t)mu_slots * npa) * sizeof(double);
 }
+// the adjoint kernels record no CFL maxima: their adjoint plane takes the
+// place of the CFL slots (NE + mu_slots*NPA doubles)
 inline __host__ __device__ size_t plane_bwd_lds_bytes(int ne, int npa) {
-  return plane_lds_bytes(ne, npa, 1) + (size_t)ne * npa * sizeof(double);
+  return plane_lds_bytes(ne, npa, 0) - (size_t)ne * sizeof(double) + (size_t)ne * npa * sizeof(double);
 }
 
 }  // namespace ramgpu

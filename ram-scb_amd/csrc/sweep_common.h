@@ -41,7 +41,8 @@ __device__ __forceinline__ double div_rn(double a, double d, double rd) {
 
 // exp(x) by Tang's table-driven method (ACM TOMS 15(2):144-157, 1989):
 // x = (64m + j) ln2/64 + r with |r| <= ln2/128, exp(x) = 2^m 2^(j/64) (1 + p(r)),
-// p the degree-5 Taylor polynomial of e^r - 1 (truncation below 4e-17), tab[j]
+// p the degree-5 Taylor polynomial of e^r - 1 (truncation below 4e-17; Horner
+// form with fused multiply-adds), tab[j]
 // = 2^(j/64) correctly rounded.  Within about 1 ulp of exp; it serves the loss
 // factors, whose bar is 1e-12 relative.  Any NaN or out-of-range argument
 // still indexes tab within 0..63.
@@ -53,7 +54,7 @@ __host__ __device__ inline double exp_tab(double x, const double *tab) {
   const double r = fma(-n, l2, fma(-n, l1, x));
   const int ni = (int)n;
   const int j = ni & 63, m = (ni - j) / 64;
-  const double p = r * (1.0 + r * (0.5 + r * (1.0 / 6.0 + r * (1.0 / 24.0 + r * (1.0 / 120.0)))));
+  const double p = r * fma(r, fma(r, fma(r, fma(r, 1.0 / 120.0, 1.0 / 24.0), 1.0 / 6.0), 0.5), 1.0);
   return ldexp(fma(tab[j], p, tab[j]), m);
 }
 
@@ -338,10 +339,13 @@ __device__ __forceinline__ double c_radial(const Tab &t, double p4k, int i, int 
 
 // CDriftE(I,J,K,L) (src/ModRamDrift.f90:337-342) from the per-(I,J,L)
 // terms e[KleField], the per-(I,J) DRD1, DPD1, BNES and the per-(S,K)
-// prefix ekp and EDOT.
-__device__ __forceinline__ double c_energy(const double *e, double ekp, double edot, double drd1, double dpd1,
-                                           double bij, double rbij, double rlz, double rrlz, double qs) {
-  const double edt1 = div_rn(div_rn(div_rn(ekp, e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij) * qs;
+// prefix ekpq = ekp*QS and EDOT.  The reference's EDT1 = ekp/FNHS/RLZ/BNES/QS
+// with QS = +-1 equals the same chain of correctly rounded quotients started
+// from ekp*QS: every step is odd in its dividend, so the sign passes through
+// exactly.
+__device__ __forceinline__ double c_energy(const double *e, double ekpq, double edot, double drd1, double dpd1,
+                                           double bij, double rbij, double rlz, double rrlz) {
+  const double edt1 = div_rn(div_rn(div_rn(ekpq, e[KLE_HS], e[KLE_RHS]), rlz, rrlz), bij, rbij);
   const double drdt = drd1 + edt1 * e[KLE_DRD2] * rlz;
   const double dpdt = dpd1 - edt1 * e[KLE_DPD2];
   return edot * (e[KLE_GR] * drdt + e[KLE_GP] * dpdt + e[KLE_DB1] + e[KLE_DI1]);
@@ -349,10 +353,10 @@ __device__ __forceinline__ double c_energy(const double *e, double ekp, double e
 
 // CDriftMu(I,J,K,L) (src/ModRamDrift.f90:418-433) from the per-(I,J,L)
 // terms m[KlmField], the per-(I,J) DRM1, DPM1, dBdt2, BNES and the per-(S,K)
-// prefix mkp.
-__device__ __forceinline__ double c_mu(const double *m, double mkp, double drm1, double dpm1, double db2,
-                                       double bij, double rbij, double rlz, double rrlz, double qs) {
-  const double edt = div_rn(div_rn(div_rn(mkp, m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij) * qs;
+// prefix mkpq = mkp*QS (QS folded in as in c_energy).
+__device__ __forceinline__ double c_mu(const double *m, double mkpq, double drm1, double dpm1, double db2,
+                                       double bij, double rbij, double rlz, double rrlz) {
+  const double edt = div_rn(div_rn(div_rn(mkpq, m[KLM_BHS], m[KLM_RBHS]), rlz, rrlz), bij, rbij);
   const double drdm = drm1 + edt * m[KLM_DRM2] * rlz;
   const double dpdm = dpm1 - edt * m[KLM_DPM2];
   return -m[KLM_CMUDOT] * (m[KLM_GMR] * drdm + m[KLM_GMP] * dpdm + db2 + m[KLM_DI2]);
@@ -406,8 +410,9 @@ enum PlaneSeq : int {
   SEQ_ATM = 3,   // ATMOL
   SEQ_SUM = 4,   // SUMRC partial sum, slot 0
   SEQ_STEP = 5,  // the middle of ram_run: DRIFTE, DRIFTMU, loss chain, DRIFTMU, DRIFTE
-  SEQ_LOSS = 6,  // the loss chain alone (CEX, ATMOL, ATMOL, CEX with five SUMRC partials)
+  SEQ_LOSS = 6,  // the loss chain alone (WAVELO, CEX, ATMOL, ATMOL, CEX, WAVELO with seven SUMRC partials)
   SEQ_TAPE = 7,  // SEQ_STEP out of place, also writing the input of each later phase to a tape
+  SEQ_WAVE = 8,  // WAVELO
 };
 
 // Elements of one F2-sized buffer (all species) of the problem t.
@@ -418,7 +423,9 @@ struct PlaneLds {
   double *F, *H, *cf, *cm, *cl, *kt, *lt, *red, *ex, *pc, *gh;
   unsigned long long *cmx;
 };
-enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_N };
+// per-K terms of a plane; KT_WAVE is WAVELO's decay factor exp(-DTs/TAU) of
+// the plane's (I,J) (1 for K=1 and for species without WAVELO)
+enum KtField { KT_EKP, KT_MKP, KT_SIGV, KT_EDOT, KT_LATL, KT_DE, KT_RDE, KT_WE, KT_RWE, KT_EKEV, KT_WAVE, KT_N };
 enum LtField { LT_DMU, LT_RDMU, LT_WMU, LT_RWMU, LT_N };
 // Scalars of one plane, staged in LDS with the plane so that the phases read
 // them as broadcasts instead of making dependent global loads after each
@@ -436,7 +443,7 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
   d.cl = d.cm + 8 * npa;     // 2*NPA: HDNS, 1/FNHS
   d.kt = d.cl + 2 * npa;     // KT_N*NE
   d.lt = d.kt + KT_N * ne;   // LT_N*NPA
-  d.red = d.lt + LT_N * npa; // 128: block reductions (up to 16 waves x 5 sums)
+  d.red = d.lt + LT_N * npa; // 128: block reductions (up to 16 waves x 7 sums: 112)
   d.ex = d.red + 128;        // 64: 2^(j/64) of exp_tab
   d.pc = d.ex + 64;          // PC_N: the plane's scalars (PcField)
   d.gh = d.pc + PC_N;        // 2*NPA: DRIFTE's relativistic ghosts F(1), F(0) of each row
@@ -478,6 +485,35 @@ __device__ __forceinline__ void block_copy(double *dst, const double *src, int n
     for (int u = 0; u < U; ++u)
       if (i0 + u * T < n) dst[i0 + u * T] = v[u];
   }
+}
+
+// WAVELO's decay factor exp(-DTs/TAU_LIF) of cell row (I,J) at energy K of
+// species s (src/ModRamWPI.f90:596-630) at Kp kp, 0-based i, j, k with k >= 1
+// (tabulated per step by k_coef_ij, read by the plane kernels): the
+// lifetime WALOS1 scaled by the wave amplitude Bw (30 pT, 100 above Kp 4)
+// inside the plasmapause; outside, WALOS2 with the strong-diffusion WALOS3
+// and the reference's corrections below 5 keV, or 5 days / KP above 1 MeV.
+__device__ inline double wave_factor(const Tab &t, int s, int i, int j, int k, double kp) {
+  const size_t nik = (size_t)t.nr * t.ne, ik = (size_t)i * t.ne + k;
+  const double lz = t.lz[i], rl = t.rlpp[j], ek = t.ekev[k];
+  const double bw = (kp >= 4.0) ? 100. : 30.;
+  double tau = 0.0;
+  if (lz <= rl) {
+    const double b = 10. / bw;
+    tau = t.wl[(size_t)s * nik + ik] * (b * b);
+  } else if (lz > rl) {
+    if (ek <= 1000.) {
+      const double w2 = t.wl[((size_t)t.ns + s) * nik + ik], w3 = t.wl[((size_t)2 * t.ns + s) * nik + ik];
+      tau = w2 * (1 + w3 / w2);
+      if (ek <= 1.1)
+        tau = tau * 37.5813 * exp(-1.81255 * ek);
+      else if (ek > 1.1 && ek <= 5.)
+        tau = tau * (7.5 - 1.15 * ek);
+    } else if (ek > 1000.) {
+      tau = 5. * 3600 * 24 / kp;
+    }
+  }
+  return exp(-t.dts / tau);
 }
 
 // Block set-up of the plane kernels: context of plane b of species s (I>=2)
@@ -571,6 +607,10 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
     vk[KT_WE] = t.we[tid];
     vk[KT_RWE] = t.rwe[tid];
     vk[KT_EKEV] = t.ekev[tid];
+    // WAVELO's factor of the step (1 for K=1 and without WAVELO)
+    vk[KT_WAVE] = (need_loss && t.wpi[c.s] && tid >= 1)
+                      ? t.wavef[((((size_t)c.s * t.nr + c.i) * NT + c.j) * NE) + tid]
+                      : 1.0;
   }
   double vl[LT_N];
   double2 hv = make_double2(0.0, 0.0);
@@ -613,6 +653,8 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
     d.kt[KT_WE * NE + k] = t.we[k];
     d.kt[KT_RWE * NE + k] = t.rwe[k];
     d.kt[KT_EKEV * NE + k] = t.ekev[k];
+    d.kt[KT_WAVE * NE + k] =
+        (need_loss && t.wpi[c.s]) ? t.wavef[((((size_t)c.s * t.nr + c.i) * NT + c.j) * NE) + k] : 1.0;
   }
   for (int l = tid + T; l < NPA; l += T) {
     d.lt[LT_DMU * NPA + l] = t.dmu[l];

@@ -18,6 +18,7 @@ module ramgpu_mod
   type, bind(C) :: ramgpu_species_out
      real(c_double) :: setrc, lsdr, lscha, lsatm
      real(c_double) :: dtdrift_r, dtdrift_p, dtdrift_e, dtdrift_mu
+     real(c_double) :: lswae, lscoe, lscsc
   end type ramgpu_species_out
 
   interface
@@ -179,6 +180,38 @@ module ramgpu_mod
        type(c_ptr), value :: ctx
        integer(c_int), value :: s
      end function ramgpu_atmol
+
+     integer(c_int) function ramgpu_wavelo(ctx, s) bind(C, name="ramgpu_wavelo")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_wavelo
+
+     ! WAVELO's tables of species s: WALOS1, WALOS2, WALOS3 (NR,NE)
+     integer(c_int) function ramgpu_set_wavelo(ctx, s, walos1, walos2, walos3) &
+          bind(C, name="ramgpu_set_wavelo")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(in) :: walos1(*), walos2(*), walos3(*)
+     end function ramgpu_set_wavelo
+
+     ! KP, Kpmax12, LZ(NR+1), NECR(NR,NT) (with use_plasmasphere /= 0)
+     integer(c_int) function ramgpu_set_wave_params(ctx, kp, kpmax12, lz, necr, use_pp) &
+          bind(C, name="ramgpu_set_wave_params")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: kp, kpmax12
+       real(c_double), intent(in) :: lz(*), necr(*)
+       integer(c_int), value :: use_pp
+     end function ramgpu_set_wave_params
+
+     integer(c_int) function ramgpu_get_loss_state(ctx, lswae, lscoe, lscsc) &
+          bind(C, name="ramgpu_get_loss_state")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: lswae(*), lscoe(*), lscsc(*)
+     end function ramgpu_get_loss_state
 
      integer(c_int) function ramgpu_sumrc(ctx, s, setrc, elorc) bind(C, name="ramgpu_sumrc")
        import :: c_int, c_ptr, c_double

@@ -35,7 +35,9 @@ EXPORTS = (
     "ramgpu_charexchange_bwd", "ramgpu_atmol_bwd", "ramgpu_ram_run_bwd", "ramgpu_adjoint_run",
     "ramgpu_batch_adjoint_run", "ramgpu_set_plane_shape", "ramgpu_get_plane_shape",
     "ramgpu_set_moment_factors", "ramgpu_anisch", "ramgpu_flux", "ramgpu_moments",
-    "ramgpu_ram_run_schedule_noalloc",
+    "ramgpu_ram_run_schedule_noalloc", "ramgpu_set_loss_state", "ramgpu_get_loss_state",
+    "ramgpu_set_wavelo", "ramgpu_set_wave_params", "ramgpu_wavelo",
+    "ramgpu_batch_kernel_timing", "ramgpu_batch_kernel_times",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
@@ -44,7 +46,8 @@ KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd"
 
 class SpeciesOut(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("setrc", "lsdr", "lscha", "lsatm", "dtdrift_r",
-                                          "dtdrift_p", "dtdrift_e", "dtdrift_mu")]
+                                          "dtdrift_p", "dtdrift_e", "dtdrift_mu", "lswae",
+                                          "lscoe", "lscsc")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -97,6 +100,13 @@ def load_library(path: Path | None = None):
     lib.ramgpu_ram_run.argtypes = [v, C.c_int, C.POINTER(SpeciesOut)]
     lib.ramgpu_ram_run_schedule.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
     lib.ramgpu_ram_run_schedule_noalloc.argtypes = [v, C.c_int, _dp, _dp, C.POINTER(SpeciesOut)]
+    lib.ramgpu_set_loss_state.argtypes = [v, _dp, _dp, _dp]
+    lib.ramgpu_get_loss_state.argtypes = [v, _dp, _dp, _dp]
+    lib.ramgpu_set_wavelo.argtypes = [v, C.c_int, _dp, _dp, _dp]
+    lib.ramgpu_set_wave_params.argtypes = [v, C.c_double, C.c_double, _dp, _dp, C.c_int]
+    lib.ramgpu_wavelo.argtypes = [v, C.c_int]
+    lib.ramgpu_batch_kernel_timing.argtypes = [v, C.c_int]
+    lib.ramgpu_batch_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
     lib.ramgpu_kernel_timing.argtypes = [v, C.c_int]
     lib.ramgpu_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
@@ -191,7 +201,28 @@ class RamGpu:
         self.put_f2(problem.f2)
         if getattr(g, "ernh", None) is not None:
             self.set_moment_factors(g)
+        if getattr(problem, "wave", None) is not None:
+            self.set_wave(problem.wave)
         return self
+
+    def set_wave(self, wave):
+        """WAVELO for the species flagged in wave.wpi (inputs.WaveInputs)."""
+        self._check(self.lib.ramgpu_set_wave_params(self.h, float(wave.kp), float(wave.kpmax12),
+                                                    _p(self._lz), None, 0))
+        for s in range(1, self.ns + 1):
+            if wave.wpi[s - 1]:
+                w = [_fd(a[s - 1], self.nr * self.ne) for a in (wave.walos1, wave.walos2, wave.walos3)]
+                self._check(self.lib.ramgpu_set_wavelo(self.h, s, *[_p(a) for a in w]))
+            else:
+                self._check(self.lib.ramgpu_set_wavelo(self.h, s, None, None, None))
+
+    def wavelo(self, s):
+        self._check(self.lib.ramgpu_wavelo(self.h, s))
+
+    def loss_state(self):
+        arrs = [np.zeros(self.ns) for _ in range(3)]
+        self._check(self.lib.ramgpu_get_loss_state(self.h, *[_p(a) for a in arrs]))
+        return dict(zip(("lswae", "lscoe", "lscsc"), arrs))
 
     def set_species(self, s, charge, cex_kind, rmas):
         self._check(self.lib.ramgpu_set_species(self.h, s, int(charge), int(cex_kind), float(rmas)))
@@ -460,6 +491,18 @@ class RamBatch:
             res.append([out[o + s].as_dict() for s in range(m.ns)])
             o += m.ns
         return res
+
+    def kernel_timing(self, enable: bool = True):
+        """Per-launch HIP events around every kernel of the following
+        (eagerly launched) ensemble steps."""
+        self._check(self.lib.ramgpu_batch_kernel_timing(self.h, int(bool(enable))))
+
+    def kernel_times(self) -> dict:
+        """{kind: (total_ms, launches)} since the last call (then reset)."""
+        ms = (C.c_double * len(KERNEL_KINDS))()
+        n = (C.c_int * len(KERNEL_KINDS))()
+        self._check(self.lib.ramgpu_batch_kernel_times(self.h, ms, n))
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNEL_KINDS)}
 
     def use_graph(self, enable=True):
         self._check(self.lib.ramgpu_batch_use_graph(self.h, int(bool(enable))))

@@ -167,6 +167,43 @@ def boundary_fgeos(grid: RamGrid, composition, scale: float = 1.0,
 
 
 @dataclass
+class WaveInputs:
+    """Inputs of WAVELO (src/ModRamWPI.f90:580-636) for the species whose
+    species%WPI is set (electrons by default, src/ModRamSpecies.f90:49):
+    per-species flags, the WAVEPARA1/2 lifetime tables, the plasmapause
+    RLpp(J) and Kp."""
+    wpi: np.ndarray       # (NS) int32
+    walos1: np.ndarray    # (NS,NR,NE) [s]
+    walos2: np.ndarray
+    walos3: np.ndarray
+    rlpp: np.ndarray      # (NT)
+    kp: float
+    kpmax12: float
+
+
+def wave_inputs(grid: RamGrid, kp: float, kpmax12: float | None = None, wpi=None) -> WaveInputs:
+    """WAVELO's inputs on this grid: species%WPI as the reference sets it
+    (the electron species) unless wpi (NS flags) is given, WALOS1/2/3 of
+    WAVEPARA1/2 for those species, the plasmapause 5.39 - 0.382 Kpmax12 at
+    every local time (no plasmasphere model), Kpmax12 = Kp unless given."""
+    from . import wpi as W
+    ns, nr, ne = grid.ns, grid.nr, grid.ne
+    if wpi is None:
+        wpi = [1 if spc.wpi else 0 for spc in grid.species]
+    wpi = np.asarray(wpi, dtype=np.int32)
+    w1 = np.zeros((ns, nr, ne), order="F")
+    w2 = np.zeros((ns, nr, ne), order="F")
+    w3 = np.zeros((ns, nr, ne), order="F")
+    t1 = W.wavepara1(grid)
+    for s in range(ns):
+        if wpi[s]:
+            w1[s] = t1
+            w2[s], w3[s] = W.wavepara2(grid, s + 1)
+    km = kp if kpmax12 is None else kpmax12
+    return WaveInputs(wpi, w1, w2, w3, W.plasmapause(grid, km), float(kp), float(km))
+
+
+@dataclass
 class Problem:
     """One member's full input set for the split step."""
     grid: RamGrid
@@ -179,6 +216,7 @@ class Problem:
     dts: float = 5.0
     betalim: float = 1.5
     fraccfl: float = 0.8
+    wave: WaveInputs | None = None   # WAVELO inputs; None: no species takes it
 
 
 SPECIES4 = (sp.ELECTRON, sp.HYDROGEN, sp.HELIUM, sp.OXYGEN)
@@ -186,13 +224,16 @@ COMPOSITION4 = (1.0, 0.7, 0.05, 0.25)
 
 
 def make_problem(nr=20, nt=25, ne=35, npa=72, species=(sp.HYDROGEN,), composition=(1.0,),
-                 kp=3.0, seed=0, boundary_scale=1.0, dts=5.0) -> Problem:
+                 kp=3.0, seed=0, boundary_scale=1.0, dts=5.0, wave=False) -> Problem:
+    """wave: WAVELO for the species whose species%WPI is set (the electron
+    species, as the reference does by default)."""
     grid = build_grid(nr, nt, ne, npa, species)
     geom = dipole_geometry(grid)
     rng = np.random.default_rng(seed)
     f2 = initial_f2(grid, geom, composition, rng)
     fgeos = boundary_fgeos(grid, composition, boundary_scale)
-    return Problem(grid, geom, volland_stern(grid, kp), fgeos, f2, kp, tuple(composition), dts)
+    w = wave_inputs(grid, kp) if wave else None
+    return Problem(grid, geom, volland_stern(grid, kp), fgeos, f2, kp, tuple(composition), dts, wave=w)
 
 
 def plain_h(seed=0) -> Problem:

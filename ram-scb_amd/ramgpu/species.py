@@ -1,11 +1,12 @@
 """RAM species table (mirrors ``DefineSpecies``, src/ModRamSpecies.f90:37-138).
 
 Only the fields the advection step reads are kept: mass (in proton masses),
-charge state, whether the species charge-exchanges, and which analytic
+charge state, whether the species charge-exchanges, which analytic
 cross-section fit ``CEPARA`` selects for it (src/ModRamLoss.f90:37-83, chosen
-by ``species%s_name``).  ``WPI`` is recorded for completeness; electron
-wave-particle lifetimes (``WAVELO``) are not part of this step and are
-switched off for every species here.
+by ``species%s_name``), ``WPI`` (electron wave-particle lifetimes, WAVELO,
+on for electrons as in the reference) and the plasmasphere composition ratio
+the Coulomb parameters use.  Whether a run applies WAVELO is the problem's
+choice (``inputs.make_problem(wave=True)``).
 """
 from __future__ import annotations
 
@@ -24,13 +25,14 @@ class Species:
     cex: bool        # species%CEX
     cex_kind: int    # CEX_* fit used by CEPARA when cex is true
     wpi: bool = False
+    pp_ratio: float = 0.0  # species%plasmasphereRatio
 
 
-# src/ModRamSpecies.f90:43-95 (electron WPI forced off, see module docstring)
-ELECTRON = Species("Electron", "_e", 5.4462E-4, -1, False, CEX_NONE)
-HYDROGEN = Species("Hydrogen", "_H", 1.0, 1, True, CEX_H)
-HELIUM = Species("HeliumP1", "He", 4.0, 1, True, CEX_HE)
-OXYGEN = Species("OxygenP1", "_O", 16.0, 1, True, CEX_O)
+# src/ModRamSpecies.f90:43-104
+ELECTRON = Species("Electron", "_e", 5.4462E-4, -1, False, CEX_NONE, wpi=True, pp_ratio=1.0)
+HYDROGEN = Species("Hydrogen", "_H", 1.0, 1, True, CEX_H, pp_ratio=0.77)
+HELIUM = Species("HeliumP1", "He", 4.0, 1, True, CEX_HE, pp_ratio=0.2)
+OXYGEN = Species("OxygenP1", "_O", 16.0, 1, True, CEX_O, pp_ratio=0.03)
 
 BY_CODE = {sp.code: sp for sp in (ELECTRON, HYDROGEN, HELIUM, OXYGEN)}
 BY_NAME = {sp.name: sp for sp in (ELECTRON, HYDROGEN, HELIUM, OXYGEN)}

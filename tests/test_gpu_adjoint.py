@@ -92,6 +92,13 @@ def test_ram_run_step_vjp_matches_oracle_fd(seed):
     _step_vjp(random_problem(seed=40 + seed, species=(sp.HYDROGEN, sp.ELECTRON), spiky=(seed == 1)), seed)
 
 
+def test_ram_run_step_vjp_with_wavelo():
+    # the electrons' WAVELO factors in the loss chain's adjoint
+    p = random_problem(seed=48, species=(sp.HYDROGEN, sp.ELECTRON))
+    p.wave = inputs.wave_inputs(p.grid, p.kp)
+    _step_vjp(p, 0)
+
+
 def test_ram_run_step_vjp_on_64x90_planes():
     # the ensemble-adjoint members' planes: the plane adjoint kernels hold the
     # forward's plane layout and the adjoint plane in LDS, their largest use

@@ -16,11 +16,13 @@ pytestmark = [pytest.mark.gpu,
 
 
 def _members():
-    """Uneven members: different grids, species sets and fields."""
+    """Uneven members: different grids, species sets and fields; the
+    electrons of the second take WAVELO."""
+    m2 = random_problem(nr=10, nt=12, ne=9, npa=14, species=(sp.ELECTRON, sp.HYDROGEN, sp.HELIUM), seed=22)
+    m2.wave = inputs.wave_inputs(m2.grid, m2.kp, kpmax12=4.0)
     return [
         random_problem(nr=8, nt=9, ne=7, npa=10, species=(sp.HYDROGEN, sp.OXYGEN), seed=21),
-        random_problem(nr=10, nt=12, ne=9, npa=14,
-                       species=(sp.ELECTRON, sp.HYDROGEN, sp.HELIUM), seed=22),
+        m2,
         inputs.make_problem(nr=6, nt=7, ne=6, npa=8, species=(sp.OXYGEN,), kp=2.0, seed=23),
     ]
 

@@ -75,7 +75,7 @@ def _run_compare(p, nsteps, what):
     dt = o.dtdrift()
     for s in range(p.grid.ns):
         ref = dict(setrc=o.arr("setrc")[s], lsdr=o.arr("lsdr")[s], lscha=o.arr("lscha")[s],
-                   lsatm=o.arr("lsatm")[s])
+                   lsatm=o.arr("lsatm")[s], lswae=o.arr("lswae")[s])
         scale = abs(ref["setrc"])
         for k, v in ref.items():
             assert abs(out[s][k] - v) <= RTOL * scale, (what, s, k, out[s][k], v)
@@ -96,6 +96,38 @@ def test_ram_run_random_geometry(seed):
 def test_ram_run_multi_species():
     p = inputs.multi_species(seed=5)
     _run_compare(p, 2, "e-,H+,He+,O+ 20x25x35x72, 2 steps")
+
+
+def test_ram_run_with_wavelo():
+    # the reference's default electron physics: WAVELO twice in the loss chain
+    # (src/ModRamRun.f90:91-99,151-159), LSWAE its energy
+    p = inputs.make_problem(20, 25, 35, 72, inputs.SPECIES4, inputs.COMPOSITION4, kp=3.0, seed=13, wave=True)
+    assert p.wave.wpi.tolist() == [1, 0, 0, 0]
+    _run_compare(p, 2, "e-(WAVELO),H+,He+,O+ 20x25x35x72, 2 steps")
+
+
+def test_wavelo_operator_and_kp_switch():
+    # WAVELO alone, then a Kp ramp across the wave-amplitude switch at Kp 4
+    p = random_problem(seed=14, species=(inputs.SPECIES4[0], inputs.SPECIES4[1]))
+    p.wave = inputs.wave_inputs(p.grid, 3.0, kpmax12=4.5)
+    o = Oracle(p)
+    g = RamGpu.from_problem(p)
+    o.call("wavelo", 1)
+    g.wavelo(1)
+    _assert_close(g.get_f2(), o.f2, "WAVELO(S=1)")
+    g.set_kp(3.0)
+    kps = np.array([3.0, 3.9, 4.0, 5.5])
+    for kp in kps:
+        o.set_vt(inputs.volland_stern(p.grid, kp))
+        o.set_wave_kp(kp)
+        o.ram_run(2)
+    out = g.ram_run_schedule(kps)
+    _assert_close(g.get_f2(), o.f2, "Kp 3 -> 5.5 schedule with WAVELO")
+    for s in range(p.grid.ns):
+        for k in ("setrc", "lsdr", "lscha", "lsatm", "lswae"):
+            ref = o.arr(k)[s]
+            assert abs(out[s][k] - ref) <= RTOL * abs(o.arr("setrc")[s]), (s, k, out[s][k], ref)
+    g.close()
 
 
 def test_ram_run_uneven_large_grid():
