@@ -216,6 +216,12 @@ class RamGpu:
             else:
                 self._check(self.lib.ramgpu_set_wavelo(self.h, s, None, None, None))
 
+    def set_wave_params(self, kp, kpmax12=None):
+        """WAVELO's KP and plasmapause (5.39 - 0.382 Kpmax12; Kpmax12 = kp
+        unless given), no plasmasphere model."""
+        km = kp if kpmax12 is None else kpmax12
+        self._check(self.lib.ramgpu_set_wave_params(self.h, float(kp), float(km), _p(self._lz), None, 0))
+
     def wavelo(self, s):
         self._check(self.lib.ramgpu_wavelo(self.h, s))
 

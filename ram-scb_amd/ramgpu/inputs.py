@@ -241,9 +241,10 @@ def plain_h(seed=0) -> Problem:
     return make_problem(seed=seed)
 
 
-def multi_species(seed=0, nr=20, nt=25, ne=35, npa=72) -> Problem:
-    """e-, H+, He+, O+ (1:0.7:0.05:0.25) on 20x25x35x72, Kp=2 at step 0."""
-    return make_problem(nr, nt, ne, npa, SPECIES4, COMPOSITION4, kp=2.0, seed=seed)
+def multi_species(seed=0, nr=20, nt=25, ne=35, npa=72, wave=True) -> Problem:
+    """e-, H+, He+, O+ (1:0.7:0.05:0.25) on 20x25x35x72, Kp=2 at step 0, with
+    the reference's default electron lifetimes (WAVELO) unless wave=False."""
+    return make_problem(nr, nt, ne, npa, SPECIES4, COMPOSITION4, kp=2.0, seed=seed, wave=wave)
 
 
 def multi_species_schedule(nsteps: int, seed: int = 0):

@@ -177,6 +177,7 @@ def test_ram_run_schedule_kp_ramp():
     bs = np.repeat(np.array([b for _, b in sched]), p.grid.ns)
     for kp, b in sched:
         o.set_vt(inputs.volland_stern(p.grid, kp))
+        o.set_wave_kp(kp)  # the electrons' WAVELO follows the step's Kp
         o.set_fgeos(p.fgeos * b)
         o.ram_run(4)
     g.ram_run_schedule(kps, bs)
