@@ -350,10 +350,11 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 }
 
 // ---------------------------------------------------------------------------
-// Plane adjoints.  LDS: the forward layout (F = the operator's input x, H)
-// plus one more plane Y (the output adjoint, then the flux weights w).
-// Cells of the plane are owned as in k_plane.  The adjoint of the input is
-// assembled in H and written back over adj.
+// Plane adjoints.  LDS: the linear plane layout of PlaneLds (F = the
+// operator's input x, H) plus one more plane Y (the output adjoint, then the
+// flux weights w).  Thread n of a BT-thread block owns cells n, n+BT, ..
+// (CELL_KL_T).  The adjoint of the input is assembled in H and written back
+// over adj.
 
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)

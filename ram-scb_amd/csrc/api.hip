@@ -301,35 +301,54 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src, double *ds
   return launch_check("k_p");
 }
 
-// Shape of the forward plane kernels for an NE x NPA plane: a block of T
-// threads gives each thread one K of R = ceil(NPA / floor(T/NE)) rows, and
-// the row bound MAXC (8 or 16) sets the register budget (4 or 2 waves per
-// SIMD; MAXC 16 with blocks of at most 512 threads).  The phases of a plane
-// are chains of dependent LDS reads, limiter divisions and barriers, so a
-// block's time follows its R rows divided among the blocks resident on a
-// CU (limited by the waves of the register budget and by the LDS, lds_cu per
-// CU); blocks of 4, 8, 12 or 16 waves, which spread evenly over the four
-// SIMDs, measured fastest on MI355X (35x72: 512 threads; 48x90: 768; 64x90:
-// 1024).  Ties go to the smaller block.  A function of (NE, NPA, LDS) only,
-// so a problem keeps its mapping in any launch.
-PlaneShape plane_shape(int ne, int npa, size_t lds, size_t lds_cu) {
-  PlaneShape best{0, 0};
-  double bcost = 1e300;
+// Shape of the forward plane kernels for an NE x NPA plane: blocks of T =
+// 256, 512, 768 or 1024 threads (whole waves on each of the 4 SIMDs), each
+// thread holding one K of the rows of its chunk (plane_chunks_for), the
+// row bound MAXC 8 or 16.  A plane's phases are VALU-bound chains between
+// barriers, so a block's time follows the rows of its busiest SIMD (the
+// longest chunk a wave holds, summed over the waves on that SIMD), over an
+// efficiency that measured (MI355X, 35x72, 48x90 and 64x90 planes) near 1
+// with two or more blocks resident on a CU filling its 16 wave slots (the
+// register budget of 4 waves per SIMD), 0.8 with two or more that do not
+// fill them, and 0.65 with one block, whose barriers stall the whole CU.
+// Residency is limited by the wave slots and by the LDS (lds_cu per CU).
+// Ties go to the smaller block.  A function of (NE, NPA, LDS) only, so a
+// problem keeps its mapping in any launch.
+PlaneShape plane_shape(int ne, int npa, size_t lds_cu) {
+  PlaneShape best{0, 0, 0};
+  double bcost = 0;
   for (int T = 256; T <= kPlaneMaxThreads; T += 256) {
-    const int lestep = T / ne;
-    if (lestep < 1) continue;
-    const int rows = (npa + lestep - 1) / lestep;
-    const int maxc = rows <= 8 ? 8 : 16;
-    if (rows > kMaxCellsPerThread || (maxc == 16 && T > kPlaneThreads)) continue;
-    const int res = std::min(4 * (maxc == 8 ? 4 : 2) / (T / 64), (int)(lds_cu / lds));
+    const PlaneShape sh = plane_shape_for(ne, npa, T);
+    if (sh.threads == 0) continue;
+    const size_t lds = plane_fwd_lds_bytes(ne, npa, T);
+    const int res = std::min(16 / (T / 64), (int)(lds_cu / lds));
     if (res < 1) continue;
-    const double cost = (double)rows / res;
-    if (cost < bcost - 1e-12) {
+    const int q = plane_chunks_for(ne, npa, T), base = npa / q, extra = npa % q;
+    int simd[4] = {0, 0, 0, 0};
+    for (int w = 0; w < T / 64; ++w) {  // rows of wave w: its first (longest) chunk's
+      const int c0 = (64 * w) / ne;
+      if (c0 < q) simd[w & 3] += base + (c0 < extra ? 1 : 0);
+    }
+    const int busiest = std::max(std::max(simd[0], simd[1]), std::max(simd[2], simd[3]));
+    const double eff = res >= 2 ? (res * (T / 64) >= 16 ? 1.0 : 0.8) : 0.65;
+    // planes per CU at a time: res; cost per plane
+    const double cost = busiest / eff / res;
+    if (best.threads == 0 || cost < bcost - 1e-9) {
+      best = sh;
       bcost = cost;
-      best = {T, maxc};
     }
   }
   return best;
+}
+
+// The forward plane shape with blocks of T threads ({0,0,0} if none fits
+// the built kernels): rows is the longest chunk's.
+PlaneShape plane_shape_for(int ne, int npa, int T) {
+  const int q = plane_chunks_for(ne, npa, T);
+  if (q < 1 || T % 64 || T > kPlaneMaxThreads) return {0, 0, 0};
+  const int rows = plane_max_rows(npa, q);
+  if (rows > kMaxCellsPerThread) return {0, 0, 0};
+  return {T, rows <= 8 ? 8 : 16, rows};
 }
 
 template <int MAXC>
@@ -361,7 +380,7 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
   Launch L = launch_of(c, s0, nsp, blocks_kl(c->t) + (copy ? c->nt : 0));
   L.src = src;
   L.dst = dst;
-  const size_t lds = plane_lds_bytes(c->ne, c->npa);
+  const size_t lds = plane_fwd_lds_bytes(c->ne, c->npa, c->plane.threads);
   tbeg(c, RAMGPU_KERNEL_PLANE);
   launch_plane(seq, c->plane, dim3(L.nv), lds, c->stream, L, rec);
   tend(c);
@@ -534,7 +553,7 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   Tab &t = c->t;
   t.ns = ns; t.nr = nr; t.nt = nt; t.ne = ne; t.npa = npa;
   t.maxblk = std::max(blocks_kl(t), blocks_r(t));  // partial sums per (slot, species)
-  if (plane_lds_bytes(ne, npa) > c->lds_limit)
+  if (plane_bwd_lds_bytes(ne, npa) > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NE*NPA=%d plane does not fit the device LDS (%zu bytes)", ne * npa,
                         c->lds_limit));
   if (radial_lds_bytes(nr, ne, npa) + 1024 > c->lds_limit)
@@ -542,9 +561,10 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   if (azim_lds_bytes(nt, ne, npa) + 1024 > c->lds_limit)
     return cleanup(fail(RAMGPU_EINVAL, "NT=%d: the azimuthal sweep's face terms do not fit the device LDS", nt));
   // gfx950: the LDS of a CU is the largest a block may use
-  c->plane = plane_shape(ne, npa, plane_lds_bytes(ne, npa), c->lds_limit);
+  c->plane = plane_shape(ne, npa, c->lds_limit);
   if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));
   t.plane_threads = c->plane.threads;
+  t.plane_chunks = plane_chunks_for(ne, npa, c->plane.threads);
 #define A(p, n) \
   if ((rc = dev_alloc(c, &p, n))) return cleanup(rc)
   A(c->d_rlz, nr1); A(c->d_ekev, ne); A(c->d_ebnd, ne); A(c->d_we, ne); A(c->d_de, ne);
@@ -650,18 +670,20 @@ int ramgpu_set_plane_shape(ramgpu_ctx *c, int threads, int maxc) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (threads == 0 && maxc == 0) {  // back to the model's choice
-    c->plane = plane_shape(c->ne, c->npa, plane_lds_bytes(c->ne, c->npa), c->lds_limit);
+    c->plane = plane_shape(c->ne, c->npa, c->lds_limit);
   } else {
-    if (threads < 64 || threads % 64 || threads > kPlaneMaxThreads || (maxc != 8 && maxc != 16) ||
-        (maxc == 16 && threads > kPlaneThreads))
+    if (threads < 64 || threads % 64 || threads > kPlaneMaxThreads || (maxc != 8 && maxc != 16))
       return fail(RAMGPU_EINVAL, "plane shape %d threads, %d rows: not a built kernel shape", threads, maxc);
-    const int lestep = threads / c->ne;
-    if (lestep < 1 || (c->npa + lestep - 1) / lestep > maxc)
-      return fail(RAMGPU_EINVAL, "plane shape %d threads holds %d rows per thread, above %d", threads,
-                  lestep < 1 ? 0 : (c->npa + lestep - 1) / lestep, maxc);
-    c->plane = {threads, maxc};
+    const PlaneShape sh = plane_shape_for(c->ne, c->npa, threads);
+    if (sh.threads == 0 || sh.rows > maxc)
+      return fail(RAMGPU_EINVAL, "plane shape %d threads holds %d rows per thread, above %d", threads, sh.rows, maxc);
+    const int rows = sh.rows;
+    if (plane_fwd_lds_bytes(c->ne, c->npa, threads) > c->lds_limit)
+      return fail(RAMGPU_EINVAL, "plane shape %d threads: the plane's LDS exceeds the device's", threads);
+    c->plane = {threads, maxc, rows};
   }
   c->t.plane_threads = c->plane.threads;
+  c->t.plane_chunks = plane_chunks_for(c->ne, c->npa, c->plane.threads);
   c->tab_dirty = true;
   return RAMGPU_OK;
 }

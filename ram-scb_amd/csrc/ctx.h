@@ -47,7 +47,7 @@ using ramgpu::Tab;
 using ramgpu::kMaxSpecies;
 
 struct PlaneShape {
-  int threads, maxc;  // block size and row bound (template) of the forward plane kernels
+  int threads, maxc, rows;  // block size, row bound (template) and rows per thread of the forward plane kernels
 };
 
 struct ramgpu_ctx {
@@ -57,7 +57,7 @@ struct ramgpu_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   size_t lds_limit = 65536;
-  PlaneShape plane{ramgpu::kPlaneThreads, 16};
+  PlaneShape plane{ramgpu::kPlaneThreads, 16, 0};
   Tab t{};            // host image of the problem descriptor
   Tab *d_tab = nullptr;  // its device copy, read by every kernel
   bool tab_dirty = true;
@@ -132,7 +132,8 @@ int run_finalize(ramgpu_ctx *c, int cflmask, int summode, int s0, int nsp);
 int step_once(ramgpu_ctx *c);
 int step_body(ramgpu_ctx *c, int *cflmask);
 // cells per thread the plane kernels need for an NE x NPA plane (8 or 16)
-PlaneShape plane_shape(int ne, int npa, size_t lds, size_t lds_cu);
+PlaneShape plane_shape(int ne, int npa, size_t lds_cu);
+PlaneShape plane_shape_for(int ne, int npa, int threads);
 // launch k_plane<seq, maxc>
 void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
 // tape slots: the inputs of the operators of one ram_run step, in order

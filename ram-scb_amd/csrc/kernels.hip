@@ -734,89 +734,237 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
 }
 
 // ---------------------------------------------------------------------------
-// Plane kernel: one block per (species, I>=2, J), holding the (K,L) plane
-// F[l*NE+k] in LDS; thread n owns cells n, n+512, .. and keeps their
-// partial updates in registers.  The reference divides both face fluxes of
-// cell K by that cell's width, CDriftE(K)/WE(K)*FBND(K) and
-// CDriftE(K-1)/WE(K)*FBND(K-1) (src/ModRamDrift.f90:364; :460 for Mu), so
-// the owner of face K keeps O = F(K) - C(K)/WE(K)*FBND(K) and writes to H
-// the term C(K)/WE(K+1)*FBND(K) its upper neighbour adds.
+// Plane kernel: one block per (species, I>=2, J), holding the (K,L) plane in
+// LDS.  Thread n < Q*NE (Q = plane_chunks) owns column K = n%NE of the
+// consecutive rows of chunk q = n/NE (NPA/Q rows, the first NPA%Q chunks
+// one more); only the owner of a cell ever writes it, so a thread may store
+// its own cells without a barrier.  The reference divides both face fluxes of cell K by
+// that cell's width, CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*
+// FBND(K-1) (src/ModRamDrift.f90:364; :460 for Mu): the owner of face K
+// forms F(K) - C(K)/WE(K)*FBND(K) and the term C(K)/WE(K+1)*FBND(K) its
+// upper neighbour adds.  Along L the neighbour is the same thread (the
+// chunk's last row hands its term to the next chunk through LDS); along K
+// it is the next lane, reached by a wavefront shift (the last lane of a
+// wave hands its terms to the next wave through LDS).
 //
-// Sequences (template SEQ):
-// (plane layout helpers: sweep_common.h)
+// LDS rows hold the energy stencil's positions 0..NE+2 of the row:
+// [F(0), F(1), F(2) .. F(NE), F(NE+1)=0, F(NE+2)=0] with DRIFTE's
+// relativistic ghosts in positions 0 and 1 (refreshed before each DRIFTE);
+// the cells K=1 have a column of their own.  So every lane reads its
+// energy stencil at fixed offsets.
+struct PlaneFwd {
+  double *F;   // NPA rows of NE+3 positions
+  double *F1;  // NPA: the cells K=1
+  double *X;   // T: terms handed between chunks (DRIFTMU) and waves (DRIFTE)
+  double *cf, *cm, *cl, *kt, *lt, *red, *ex, *pc;
+  unsigned long long *cmx;
+};
 
-// Row-wise loop of the plane kernels: this thread's cells (K = c.ke, rows
-// L = c.le0 + n*c.lestep), n = 0..kMaxCellsPerThread-1.
-#define ROW_CELLS(n)                                 \
-  const int l = c.le0 + (n) * c.lestep;              \
-  if (!c.eact || l >= NPA) break;                    \
-  const int cc = l * NE + c.ke
+__device__ __forceinline__ PlaneFwd plane_fwd_lds(double *sm, int ne, int npa, int threads) {
+  PlaneFwd d;
+  d.F = sm;
+  d.F1 = d.F + (size_t)npa * (ne + 3);
+  d.X = d.F1 + npa;
+  d.cf = d.X + threads;       // 8*NPA: per-L DRIFTE terms (KleField)
+  d.cm = d.cf + 8 * npa;      // 8*NPA: per-L DRIFTMU terms (KlmField)
+  d.cl = d.cm + 8 * npa;      // 2*NPA: HDNS, 1/FNHS
+  d.kt = d.cl + 2 * npa;      // KT_N*NE
+  d.lt = d.kt + KT_N * ne;    // LT_N*NPA
+  d.red = d.lt + LT_N * npa;  // 128: block reductions (up to 16 waves x 7 sums: 112)
+  d.ex = d.red + 128;         // 64: 2^(j/64) of exp_tab
+  d.pc = d.ex + 64;           // PC_N: the plane's scalars (PcField)
+  d.cmx = (unsigned long long *)(d.pc + PC_N);  // NE + kMuSpread*NPA
+  return d;
+}
+static_assert(KT_N == 11 && LT_N == 4 && PC_N == 24, "plane_fwd_lds_bytes counts these tables");
+
+struct FwdCtx {
+  int s, i, j, NE, NPA, NT, S;  // S = NE+3 positions per LDS row
+  int cex, upa;                 // the species' charge-exchange kind, UPA(I)
+  size_t ij;
+  double rlz, rrlz, qs, beta;
+  int ke, q, l0, R, nr;  // column, chunk, its first row, the longest chunk's rows, this thread's rows (0: idle)
+  double *col;           // cell (ke, l) is col[l*cst]
+  int cst;
+};
+
+// x, hidden from the optimiser: row indices derived from it are formed
+// afresh in each phase instead of being kept in registers across phases.
+__device__ __forceinline__ int fresh(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
+// The value of x in lane `lane` of the wave (a wave-uniform value).
+__device__ __forceinline__ double lane_value(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// A block-uniform x (read from LDS) held in scalar registers.
+__device__ __forceinline__ double uniform(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// x, formed here: keeps the compiler from sinking the arithmetic that
+// makes a row's result down to the barrier, where it would hold every
+// operand of every row instead of one result per row.
+__device__ __forceinline__ double settle(double x) {
+  asm("" : "+v"(x));
+  return x;
+}
+
+// The value of v in lane-1 (lane 0: -0.0, which adds as an exact zero).
+__device__ __forceinline__ double from_lane_below(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138 /* wave_shr:1 */, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)0x80000000, (int)(b >> 32), 0x138, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Block set-up: the context of plane b of species s, this thread's cells
+// (from src, device layout of one species) and the plane's tables.
+template <int MAXC>
+__device__ inline void fwd_setup(const Tab &t, const Unit &u, double *sm, const double *src, bool need_loss,
+                                 bool need_kl, FwdCtx &c, PlaneFwd &d) {
+  const int NE = t.ne, NPA = t.npa, NT = t.nt;
+  const int tid = threadIdx.x;
+  c.s = u.s;
+  c.i = 1 + u.b / NT;  // I = i+1 >= 2
+  c.j = u.b - (c.i - 1) * NT;
+  c.NE = NE;
+  c.NPA = NPA;
+  c.NT = NT;
+  c.S = NE + 3;
+  c.ij = (size_t)c.i * NT + c.j;
+  c.rlz = t.rlz[c.i];
+  c.rrlz = t.rrlz[c.i];
+  c.qs = (double)t.charge[c.s];  // x/QS == x*QS for QS = +-1
+  c.cex = t.cex[c.s];
+  c.upa = t.upa[c.i];
+  c.beta = t.betalim;
+  const int Q = t.plane_chunks, base = NPA / Q, extra = NPA - base * Q;
+  c.R = base + (extra > 0 ? 1 : 0);
+  c.q = fast_div(tid, NE, t.rne);
+  c.ke = tid - c.q * NE;
+  c.l0 = c.q * base + min(c.q, extra);
+  c.nr = (c.q < Q) ? base + (c.q < extra ? 1 : 0) : 0;
+  d = plane_fwd_lds(sm, NE, NPA, blockDim.x);
+  c.col = (c.ke == 0) ? d.F1 : d.F + c.ke + 1;
+  c.cst = (c.ke == 0) ? 1 : c.S;
+  // the plane's base is uniform; cells are 32-bit offsets from it
+  const gptr<const double> gF = as_global(src + (size_t)c.s * f2_species_stride(t) + c.ij * ((size_t)NE * NPA));
+  double vf[MAXC];
+#pragma unroll
+  for (int n = 0; n < MAXC; ++n) vf[n] = (n < c.nr) ? gF[(unsigned)((c.l0 + n) * NE + c.ke)] : 0.0;
+  plane_tables(t, c.s, c.i, c.j, d, need_loss, need_kl);
+  for (int l = tid; l < NPA; l += blockDim.x) {  // F(NE+1) = F(NE+2) = 0
+    d.F[l * c.S + NE + 1] = 0.0;
+    d.F[l * c.S + NE + 2] = 0.0;
+  }
+#pragma unroll
+  for (int n = 0; n < MAXC; ++n)
+    if (n < c.nr) c.col[(c.l0 + n) * c.cst] = vf[n];
+  __syncthreads();
+}
+
+// This thread's cells to dst (device layout of one species' plane).
+template <int MAXC>
+__device__ __forceinline__ void fwd_store(const FwdCtx &c, double *dst) {
+  const gptr<double> g = as_global(dst);
+  const int l0 = fresh(c.l0), NE = c.NE, k = c.ke;
+#pragma unroll
+  for (int n = 0; n < MAXC; ++n)
+    if (n < c.nr) g[(unsigned)((l0 + n) * NE + k)] = c.col[(l0 + n) * c.cst];
+}
+
+// Rows of one pass of the sweeps: a thread's rows go in passes of at most
+// kPassRows, each ending in a barrier before its rows are rewritten, so the
+// registers that hold a pass's results do not grow with R.
+constexpr int kPassRows = 8;
 
 // DRIFTE on the plane (src/ModRamDrift.f90:333-369).  F(1),F(0) relativistic
-// ghosts from F(2), once per row; F(NE+1)=F(NE+2)=0.  Each thread owns one K
-// of several rows, so the per-K factors sit in registers and the CFL maximum
-// of its cells is one value.  The per-L terms are in d.cf.
+// ghosts from F(2), once per row; F(NE+1)=F(NE+2)=0; K=1 is not updated.
+// Each thread owns one K, so the per-K factors sit in registers and the CFL
+// maximum of its cells is one value.  The per-L terms are in d.cf.  Rows
+// are independent, so a pass rewrites its rows while the next pass reads
+// others.
 template <int MAXC>
-__device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
-  const int NE = c.NE, NPA = c.NPA;
+__device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec) {
+  const int NE = c.NE, NPA = c.NPA, S = c.S;
   {
     const double *rel = d.pc + PC_REL;
     for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
-      const double g1 = d.F[l * NE + 1] * rel[0] / rel[1] * rel[2];
-      d.gh[2 * l] = g1;
-      d.gh[2 * l + 1] = g1 * rel[3] / rel[0] * rel[4];
+      double *row = d.F + l * S;
+      const double g1 = row[2] * rel[0] / rel[1] * rel[2];
+      row[1] = g1;
+      row[0] = g1 * rel[3] / rel[0] * rel[4];
     }
     if (rec)
       for (int k = threadIdx.x; k < NE; k += blockDim.x) d.cmx[k] = 0ull;
   }
   __syncthreads();
   const double *k8 = d.pc + PC_KIJ;
-  const double drd1 = k8[KIJ_DRD1], dpd1 = k8[KIJ_DPD1], bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double rlz = c.rlz, rrlz = c.rrlz, beta = c.beta;
+  const double drd1 = uniform(k8[KIJ_DRD1]), dpd1 = uniform(k8[KIJ_DPD1]);
+  const double bij = uniform(k8[KIJ_BIJ]), rbij = uniform(k8[KIJ_RBIJ]);
+  const double rlz = uniform(c.rlz), rrlz = uniform(c.rrlz), beta = uniform(c.beta);
   const int k = c.ke;
   // QS = +-1 folded into the prefix: the chain of correctly rounded
   // quotients is odd, so the sign comes out exactly as the reference's /QS
   const double ekp = d.kt[KT_EKP * NE + k] * c.qs, edot = d.kt[KT_EDOT * NE + k];
   const double de = d.kt[KT_DE * NE + k], rde = d.kt[KT_RDE * NE + k];
   const double we = d.kt[KT_WE * NE + k], rwe = d.kt[KT_RWE * NE + k];
-  const bool top = k + 1 >= NE;
+  const bool top = k + 1 >= NE;  // its upper term goes to a lane that does not use it
   const double we1 = top ? 1.0 : d.kt[KT_WE * NE + k + 1], rwe1 = top ? 1.0 : d.kt[KT_RWE * NE + k + 1];
-  double O[MAXC];
+  const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int l0 = fresh(c.l0);
   double cmax = 0.0;
-  // stencil slots as LDS offsets from F: the cell itself, its ghost F(1)
-  // for K=1, and the one cell the upwind side adds (F(K+2), or F(K-1) /
-  // the ghosts F(0), F(1)); reads past a row land in the plane or in H and
-  // are masked
-  const int gho = (int)(d.gh - d.F);
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
-    ROW_CELLS(n);
-    const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz);
-    cmax = fmax(cmax, fmax(fabs(ce), 1E-10));
-    const bool neg = ce < 0;
-    const double fk = d.F[cc];
-    const double f0 = (k == 0) ? d.F[gho + 2 * l] : fk;
-    const double fp1 = top ? 0.0 : d.F[cc + 1];
-    const int up = (k >= 2) ? cc - 1 : gho + 2 * l + (k == 0 ? 1 : 0);
-    const double a = d.F[neg ? cc + 2 : up];
-    const double fn = neg ? ((k + 2 < NE) ? a : 0.0) : f0;
-    const double fnm1 = neg ? fp1 : a;
-    const double fb = face_value_sel(f0, fp1, fn, fnm1, ce, div_rn(ce, de, rde), beta);
-    O[n] = fk - div_rn(ce, we, rwe) * fb;
-    d.H[cc] = top ? 0.0 : div_rn(ce, we1, rwe1) * fb;
-  }
-  if (rec && c.eact) atomic_max_pos(&d.cmx[k], cmax);
-  __syncthreads();
-  if (k >= 1) {
+  for (int p0 = 0; p0 < MAXC; p0 += kPassRows) {
+    if (p0 >= c.R) break;
+    // the terms lane 63 hands to the next wave's lane 0, per pass and wave
+    double *xw = d.X + ((p0 / kPassRows) * nw + (threadIdx.x >> 6)) * kPassRows;
+    double v[kPassRows];
 #pragma unroll
-    for (int n = 0; n < MAXC; ++n) {
-      ROW_CELLS(n);
-      double v = O[n] + d.H[cc - 1];
-      if (v < 0) v = 1E-15;
-      d.F[cc] = v;
+    for (int n = 0; n < kPassRows; ++n) {
+      if (p0 + n >= c.nr) break;
+      const int l = l0 + p0 + n;
+      const double *p = d.F + l * S + k;  // positions K-1, K, K+1, K+2
+      const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz);
+      cmax = fmax(cmax, fabs(ce));
+      const bool neg = ce < 0;
+      const double f0 = p[1], fp1 = p[2];
+      const double a = p[neg ? 3 : 0];
+      const double fn = neg ? a : f0, fnm1 = neg ? fp1 : a;
+      const double fb = face_value_sel(f0, fp1, fn, fnm1, ce, div_rn(ce, de, rde), beta);
+      const double h = div_rn(ce, we1, rwe1) * fb;
+      // the lower neighbour is lane-1 of the same chunk (same rows), or for
+      // lane 0 the last lane of the wave before, whose term comes after the
+      // barrier
+      v[n] = settle((f0 - div_rn(ce, we, rwe) * fb) + from_lane_below(h));
+      if (lane == 63) xw[n] = h;
+    }
+    __syncthreads();
+    if (k >= 1) {
+      const bool first = lane == 0;  // its lower neighbour is the last lane of the wave before
+#pragma unroll
+      for (int n = 0; n < kPassRows; ++n) {
+        const int j = p0 + n;
+        if (j >= c.nr) break;
+        double w = v[n];
+        if (first) w = w + xw[n - kPassRows];
+        if (w < 0) w = 1E-15;
+        c.col[(l0 + j) * S] = w;
+      }
     }
   }
   if (rec) {
+    if (c.nr > 0) atomic_max_pos(&d.cmx[k], fmax(cmax, 1E-10));
     __syncthreads();
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
     for (int kk = threadIdx.x; kk < NE; kk += blockDim.x) atomicMax(&g[2 + kk], d.cmx[kk]);
@@ -827,65 +975,97 @@ __device__ void plane_e(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool
 // DRIFTMU on the plane (src/ModRamDrift.f90:413-466).  F(1)=F(2) for the
 // stencil; limiter faces L=2..NPA-2; CDriftMu(1)=0, FBND(1)=0,
 // FBND(NPA-1)=F(NPA); top cell F2(NPA)=F2(NPA-1)*FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1).
-// Same ownership as plane_e (consecutive lanes on consecutive K of a row);
-// the per-L terms are read from d.cm and d.lt (one address per row).
+// A thread walks its rows upwards with the stencil in registers; the
+// per-L terms are read from d.cm and d.lt.  The rows around the chunk are
+// read before any row is rewritten; the chunk's first row takes the term
+// of the chunk below after the last pass, and the top row is set last
+// from the finished row below it (the same thread's, by plane_chunks_for).
 template <int MAXC>
-__device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, bool rec) {
+__device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA;
   if (rec) {
     for (int l = threadIdx.x; l < kMuSpread * NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
     __syncthreads();
   }
   const double *k8 = d.pc + PC_KIJ;
-  const double drm1 = k8[KIJ_DRM1], dpm1 = k8[KIJ_DPM1], db2 = k8[KIJ_DB2];
-  const double bij = k8[KIJ_BIJ], rbij = k8[KIJ_RBIJ];
-  const double rlz = c.rlz, rrlz = c.rrlz, beta = c.beta;
+  const double drm1 = uniform(k8[KIJ_DRM1]), dpm1 = uniform(k8[KIJ_DPM1]), db2 = uniform(k8[KIJ_DB2]);
+  const double bij = uniform(k8[KIJ_BIJ]), rbij = uniform(k8[KIJ_RBIJ]);
+  const double rlz = uniform(c.rlz), rrlz = uniform(c.rrlz), beta = uniform(c.beta);
   const int k = c.ke;
   const double mkp = d.kt[KT_MKP * NE + k] * c.qs;  // QS folded in, as in plane_e
-  double O[MAXC];
+  double *col = c.col;
+  const int cst = c.cst, l0 = fresh(c.l0), R = c.R;
+  // reads past the plane repeat the top row; they feed no kept result
+  auto ld = [&](int l) { return col[min(l, NPA - 1) * cst]; };
+  // stencil window F(L-1), F(L), F(L+1) of row L = l0+j+1, and the two rows
+  // above the chunk
+  const int nr = c.nr;
+  double fm = ld(max(l0 - 1, 0)), f0 = ld(l0), f1 = ld(l0 + 1);
+  const double up0 = ld(l0 + nr), up1 = ld(l0 + nr + 1);
+  double hprev = -0.0, v0 = 0.0;
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
-    ROW_CELLS(n);
-    const int L1 = l + 1;
-    // every row forms the face value; rows 1 and NPA keep 0 (face 1:
-    // CDriftMu(1)=0, FBND(1)=0, so its term is 0/WMU(2)*0 = 0; row NPA is
-    // the top cell).  Reads past the plane land in H and are masked.
-    const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz);
-    // the row's maximum in kMuSpread slots (by K), so that fewer lanes of a
-    // wave update one LDS address
-    if (rec && L1 >= 2) atomic_max_pos(&d.cmx[NE + kMuSpread * l + (k & (kMuSpread - 1))], fmax(fabs(cm), 1E-32));
-    const bool neg = cm < 0;
-    const double f0 = d.F[cc];
-    const double fp1 = d.F[cc + NE];
-    const double a = d.F[neg ? cc + 2 * NE : (L1 <= 2 ? cc : cc - NE)];  // F(1) = F(2)
-    const double fn = neg ? a : f0, fnm1 = neg ? fp1 : a;
-    const int lw = (L1 <= NPA - 1) ? l + 1 : l;  // WMU(L+1) of the face's upper cell (in range)
-    const double fl = face_value_sel(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
-    const double fb = (L1 == NPA - 1) ? fp1 : fl;  // FBND(NPA-1) = F(NPA)
-    const bool face = L1 >= 2 && L1 <= NPA - 1;
-    const double on = face ? div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb : 0.0;
-    const double h = face ? div_rn(cm, d.lt[LT_WMU * NPA + lw], d.lt[LT_RWMU * NPA + lw]) * fb : 0.0;
-    O[n] = f0 - on;
-    d.H[cc] = h;
-  }
-  __syncthreads();
+  for (int p0 = 0; p0 < MAXC; p0 += kPassRows) {
+    if (p0 >= R) break;
+    double v[kPassRows];
 #pragma unroll
-  for (int n = 0; n < MAXC; ++n) {
-    ROW_CELLS(n);
-    if (l >= 1 && l <= NPA - 2) {
-      double v = O[n] + d.H[cc - NE];
-      if (v < 0) v = 1E-15;
-      d.F[cc] = v;
+    for (int n = 0; n < kPassRows; ++n) {
+      const int j = p0 + n;
+      if (j >= nr) break;
+      const int l = l0 + j, L1 = l + 1;
+      // row j+2 of the chunk is not yet rewritten; above the chunk, the
+      // rows read at the start
+      const double f2 = (j + 2 < nr) ? ld(l + 2) : (j + 2 == nr ? up0 : up1);
+      const double cm = c_mu(d.cm + l * 8, mkp, drm1, dpm1, db2, bij, rbij, rlz, rrlz);
+      // the row's CFL maximum in kMuSpread slots (by K), so that fewer lanes
+      // of a wave update one LDS address
+      if (rec && l >= 1)
+        atomic_max_pos(&d.cmx[NE + kMuSpread * l + (k & (kMuSpread - 1))], fmax(fabs(cm), 1E-32));
+      const bool neg = cm < 0;
+      const double a = neg ? f2 : (L1 <= 2 ? f0 : fm);  // F(1) = F(2)
+      const double fn = neg ? a : f0, fnm1 = neg ? f1 : a;
+      const int lw = (L1 <= NPA - 1) ? l + 1 : l;  // WMU(L+1) of the face's upper cell (in range)
+      const double fl = face_value_sel(f0, f1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
+      const double fb = (L1 == NPA - 1) ? f1 : fl;  // FBND(NPA-1) = F(NPA)
+      // faces 1 and NPA carry nothing: face 1 has CDriftMu(1)=0, FBND(1)=0
+      // (its term 0/WMU(2)*0 = 0); row NPA is the top cell
+      const bool face = L1 >= 2 && L1 <= NPA - 1;
+      const double on = div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * fb;
+      const double up = div_rn(cm, d.lt[LT_WMU * NPA + lw], d.lt[LT_RWMU * NPA + lw]) * fb;
+      v[n] = settle((f0 - (face ? on : 0.0)) + hprev);
+      hprev = face ? up : 0.0;
+      fm = f0;
+      f0 = f1;
+      f1 = f2;
+    }
+    if (p0 == 0) v0 = v[0];
+    if (p0 + kPassRows >= R) d.X[threadIdx.x] = hprev;  // the chunk's last row's term, for the next chunk
+    __syncthreads();
+    // the pass's rows but the chunk's first and the top row
+#pragma unroll
+    for (int n = 0; n < kPassRows; ++n) {
+      const int j = p0 + n, l = l0 + j;
+      if (j >= nr) break;
+      if (j >= 1 && l <= NPA - 2) {
+        double w = v[n];
+        if (w < 0) w = 1E-15;
+        col[l * cst] = w;
+      }
     }
   }
-  __syncthreads();
-  {
+  // the chunk's first row with the term of the chunk below (row 1 of the
+  // plane, the first chunk's, is not updated)
+  if (c.nr > 0 && c.q >= 1) {
+    double w = v0 + d.X[threadIdx.x - NE];
+    if (w < 0) w = 1E-15;
+    col[l0 * cst] = w;
+  }
+  if (c.nr > 0 && NPA - 1 < l0 + c.nr) {  // this thread holds the top row and the row below it
     const double hN = d.pc[PC_HN], hN1 = d.pc[PC_HN1];
     const double muN = d.pc[PC_MUN], muN1 = d.pc[PC_MUN1];
-    for (int kk = threadIdx.x; kk < NE; kk += blockDim.x)
-      d.F[(NPA - 1) * NE + kk] = d.F[(NPA - 2) * NE + kk] * hN * muN / hN1 / muN1;
+    col[(NPA - 1) * cst] = col[(NPA - 2) * cst] * hN * muN / hN1 / muN1;
   }
   if (rec) {
+    __syncthreads();
     unsigned long long *g = t.cflmax + (size_t)c.s * (2 + NE + NPA);
     for (int ll = 1 + threadIdx.x; ll < NPA; ll += blockDim.x) {
       unsigned long long m = d.cmx[NE + kMuSpread * ll];
@@ -903,9 +1083,10 @@ __device__ void plane_mu(const Tab &t, const PlaneCtx &c, const PlaneLds &d, boo
 // leaves its cells and so its slot's sum bit for bit those of the slot before.
 // WAVELO's factor is one per thread (it depends on I, J, K only), so a
 // thread's partial sums after it are its sums before it times the factor
-// (to rounding: exactly when the factor is 1).
+// (to rounding: exactly when the factor is 1).  Each thread rewrites only
+// its own cells.
 template <int MAXC>
-__device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, int b) {
+__device__ void plane_loss(const Tab &t, const FwdCtx &c, const PlaneFwd &d, int b) {
   const int NE = c.NE, NPA = c.NPA, NT = c.NT;
   const bool cex = c.cex != 0;
   const int upa = c.upa;
@@ -914,17 +1095,19 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   const double sigv = d.kt[KT_SIGV * NE + k], latl = d.kt[KT_LATL * NE + k];
   const double wk = d.kt[KT_WE * NE + k], ek = d.kt[KT_EKEV * NE + k];
   const double wave = d.kt[KT_WAVE * NE + k];  // 1 for K=1 and without WAVELO
-  const double wave0 = (c.le0 == 0) ? 1.0 : wave;  // the first cell may be row L=1, which WAVELO skips
   const double dts = d.pc[PC_DTS];
   const double ekw = ek * wk;
   double a[5] = {0, 0, 0, 0, 0}, tot[7];
+  const int l0 = fresh(c.l0);
 #pragma unroll
   for (int n = 0; n < MAXC; ++n) {
-    ROW_CELLS(n);
+    if (n >= c.nr) break;
+    const int l = l0 + n;
     const bool w_on = sum_row && k >= 1 && l >= 1;
     const bool cx = cex && k >= 1 && l >= 1;
     const bool atm = k >= 1 && l + 1 >= upa;
-    double f = d.F[cc];
+    double *pf = &c.col[l * c.cst];
+    double f = *pf;
     // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49);
     // ATLOS**(1/FNHS) (:501) as exp(log(ATLOS)*(1/FNHS)); WAVELO's
     // exp(-DTs/TAU) on L>=2 (src/ModRamWPI.f90:630); a factor of 1 leaves a
@@ -932,7 +1115,7 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
     // it out of the sums
     const double cxf = cx ? exp_tab(-(sigv * d.cl[2 * l] * dts), d.ex) : 1.0;
     const double af = atm ? exp_tab(latl * d.cl[2 * l + 1], d.ex) : 1.0;
-    const double wf = (n == 0) ? wave0 : wave;
+    const double wf = (l == 0) ? 1.0 : wave;
     const double q = w_on ? ekw * d.lt[LT_WMU * NPA + l] : 0.0;  // EKEV*WE*WMU
     a[0] = fma(f, q, a[0]);
     f = f * wf;
@@ -944,7 +1127,7 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
     a[3] = fma(f, q, a[3]);
     f = f * cxf;
     a[4] = fma(f, q, a[4]);
-    d.F[cc] = f * wf;
+    *pf = f * wf;
   }
   // slots: before WAVELO, after it, after CEX, ATMOL, ATMOL, CEX, after WAVELO
   double s7[7] = {a[0], a[0] * wave, a[1], a[2], a[3], a[4], a[4] * wave};
@@ -957,12 +1140,37 @@ __device__ void plane_loss(const Tab &t, const PlaneCtx &c, const PlaneLds &d, i
   }
 }
 
-// MAXC bounds the rows a thread owns; the register budget of each bound
-// (4 or 2 waves per SIMD; blocks up to 1024 or 512 threads) fits the block
-// sizes the host pairs it with.
+// One of the loss operators alone on this thread's cells: CHAREXCHANGE
+// (src/ModRamLoss.f90:467-475: K>=2, L>=2), ATMOL (:495-503: K>=2,
+// L>=int(UPA(I))) or WAVELO (src/ModRamWPI.f90:612-633: K>=2, L>=2; the
+// factor is 1 for K=1).
 template <int SEQ, int MAXC>
-__global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads, (MAXC <= 8) ? 4 : 2)
-    k_plane(Launch L, int rec) {
+__device__ void plane_one_loss(const FwdCtx &c, const PlaneFwd &d) {
+  const int NE = c.NE, k = c.ke;
+  if (k < 1) return;
+  if (SEQ == SEQ_CEX && c.cex == 0) return;
+#pragma unroll
+  for (int n = 0; n < MAXC; ++n) {
+    if (n >= c.nr) break;
+    const int l = c.l0 + n;
+    {
+      double &f = c.col[l * c.cst];
+      if constexpr (SEQ == SEQ_CEX) {
+        if (l >= 1) f = f * cex_factor(d, NE, k, l, d.pc[PC_DTS]);
+      } else if constexpr (SEQ == SEQ_ATM) {
+        if (l + 1 >= c.upa) f = f * atm_factor(d, NE, k, l);
+      } else {
+        if (l >= 1) f = f * d.kt[KT_WAVE * NE + k];
+      }
+    }
+  }
+}
+
+// MAXC (8 or 16) bounds the rows a thread owns; the sweeps take them in
+// passes of kPassRows, so both bounds fit the register budget of 4 waves
+// per SIMD with blocks of up to 1024 threads.
+template <int SEQ, int MAXC>
+__global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec) {
   extern __shared__ double sm[];
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
@@ -977,11 +1185,12 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
     }
     constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS ||
                                SEQ == SEQ_TAPE || SEQ == SEQ_WAVE;
-    PlaneCtx c;
-    PlaneLds d;
     constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE;
-    plane_setup<8>(t, u, sm, launch_src(L, t), need_loss, need_kl, c, d);
-    double *gF = launch_dst(L, t) + (size_t)c.s * f2_species_stride(t) + c.ij * c.P;
+    FwdCtx c;
+    PlaneFwd d;
+    fwd_setup<MAXC>(t, u, sm, launch_src(L, t), need_loss, need_kl, c, d);
+    const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * ((size_t)NE * NPA);
+    double *gF = launch_dst(L, t) + po;
     const bool inside = d.pc[PC_KIJ + KIJ_MG] == 0.0;
     const bool rec_here = rec && inside;
 
@@ -989,35 +1198,8 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       plane_e<MAXC>(t, c, d, rec_here);
     } else if constexpr (SEQ == SEQ_MU) {
       plane_mu<MAXC>(t, c, d, rec_here);
-    } else if constexpr (SEQ == SEQ_CEX) {
-      if (c.cex != 0 && c.ke >= 1) {  // CHAREXCHANGE, src/ModRamLoss.f90:467-475: K>=2, L>=2
-#pragma unroll
-        for (int n = 0; n < MAXC; ++n) {
-          ROW_CELLS(n);
-          if (l >= 1) d.F[cc] = d.F[cc] * cex_factor(d, NE, c.ke, l, d.pc[PC_DTS]);
-        }
-      }
-      __syncthreads();
-    } else if constexpr (SEQ == SEQ_ATM) {
-      // ATMOL, src/ModRamLoss.f90:495-503: K>=2, L>=int(UPA(I))
-      const int upa = c.upa;
-      if (c.ke >= 1) {
-#pragma unroll
-        for (int n = 0; n < MAXC; ++n) {
-          ROW_CELLS(n);
-          if (l + 1 >= upa) d.F[cc] = d.F[cc] * atm_factor(d, NE, c.ke, l);
-        }
-      }
-      __syncthreads();
-    } else if constexpr (SEQ == SEQ_WAVE) {
-      // WAVELO, src/ModRamWPI.f90:612-633: K>=2, L>=2 (the factor is 1 for K=1)
-      const double wf = d.kt[KT_WAVE * NE + c.ke];
-#pragma unroll
-      for (int n = 0; n < MAXC; ++n) {
-        ROW_CELLS(n);
-        if (l >= 1) d.F[cc] = d.F[cc] * wf;
-      }
-      __syncthreads();
+    } else if constexpr (SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_WAVE) {
+      plane_one_loss<SEQ, MAXC>(c, d);
     } else if constexpr (SEQ == SEQ_SUM) {
       // SUMRC over I>=2, K>=2, L>=2, J<=NT-1 (src/ModRamRun.f90:246-255), slot 0
       double a[1] = {0.0}, tot[1];
@@ -1025,9 +1207,10 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       if (c.j <= NT - 2 && k >= 1) {
 #pragma unroll
         for (int n = 0; n < MAXC; ++n) {
-          ROW_CELLS(n);
+          if (n >= c.nr) break;
+          const int l = c.l0 + n;
           if (l >= 1)
-            a[0] += d.kt[KT_EKEV * NE + k] * (d.F[cc] * d.kt[KT_WE * NE + k] * d.lt[LT_WMU * NPA + l]);
+            a[0] += d.kt[KT_EKEV * NE + k] * (c.col[l * c.cst] * d.kt[KT_WE * NE + k] * d.lt[LT_WMU * NPA + l]);
         }
       }
       block_sum_n<1>(a, d.red, tot);
@@ -1036,20 +1219,18 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       plane_loss<MAXC>(t, c, d, u.b);
     } else if constexpr (SEQ == SEQ_TAPE) {
       // src is tape slot 0 (this phase's input); phase m's output, the input
-      // of phase m+1, goes to slot m+1, the last to dst.  The sweeps rewrite
-      // F only after a barrier; the loss chain rewrites its cells at once, so
-      // a barrier separates it from the store before it.
-      const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * c.P, ns = f2_elems(t);
+      // of phase m+1, goes to slot m+1, the last to dst.  A thread stores
+      // its own cells, which only it writes.
+      const size_t ns = f2_elems(t);
       double *slot = const_cast<double *>(launch_src(L, t)) + po;
       plane_e<MAXC>(t, c, d, rec_here);
-      block_copy<4>(slot + ns, d.F, c.P);
+      fwd_store<MAXC>(c, slot + ns);
       plane_mu<MAXC>(t, c, d, rec_here);
-      block_copy<4>(slot + 2 * ns, d.F, c.P);
-      __syncthreads();
+      fwd_store<MAXC>(c, slot + 2 * ns);
       plane_loss<MAXC>(t, c, d, u.b);
-      block_copy<4>(slot + 3 * ns, d.F, c.P);
-      plane_mu<MAXC>(t, c, d, false);
-      block_copy<4>(slot + 4 * ns, d.F, c.P);
+      fwd_store<MAXC>(c, slot + 3 * ns);
+      plane_mu<MAXC>(t, c, d, false);  // the loss chain's block sum ends with a barrier
+      fwd_store<MAXC>(c, slot + 4 * ns);
       plane_e<MAXC>(t, c, d, false);
     } else {  // SEQ_STEP
       plane_e<MAXC>(t, c, d, rec_here);
@@ -1059,7 +1240,7 @@ __global__ void __launch_bounds__((MAXC <= 8) ? kPlaneMaxThreads : kPlaneThreads
       plane_e<MAXC>(t, c, d, false);
     }
 
-    if constexpr (SEQ != SEQ_SUM) block_copy<8>(gF, d.F, c.P);
+    if constexpr (SEQ != SEQ_SUM) fwd_store<MAXC>(c, gF);
     __syncthreads();
   }
 }

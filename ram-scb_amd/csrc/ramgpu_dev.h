@@ -28,7 +28,7 @@ constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chos
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bound, checked at creation
 constexpr int kSumSlots = 8;           // SUMRC partial-sum slots of one ram_run (7 plane, 1 radial)
-constexpr int kMuSpread = 8;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 8)
+constexpr int kMuSpread = 4;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 4)
 constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 
 // packed per-(I,J) terms of the plane sweeps, kij[(i*NT+j)*8 + ..]
@@ -99,10 +99,10 @@ struct Tab {
   double *dtdrift;      // [s][4]
   // float reciprocals for exact small-integer division (fast_div)
   float rnt, rne, rp;   // 1/NT, 1/NE, 1/(NE*NPA)
-  // threads of the forward plane kernels' row mapping (a multiple of 64):
-  // thread n < (T/NE)*NE owns K = n%NE of rows L = n/NE + m*(T/NE); a launch
-  // with a larger block leaves the threads above idle
-  int plane_threads;
+  // row mapping of the forward plane kernels (plane_threads a multiple of
+  // 64): thread n < plane_chunks*NE owns K = n%NE of the rows of chunk
+  // n/NE (plane_chunks_for); the threads above are idle
+  int plane_threads, plane_chunks;
   // state
   double *f2;
   double *partials;  // [slot][s][maxblk]
@@ -189,6 +189,28 @@ inline __host__ __device__ size_t plane_lds_bytes(int ne, int npa, int mu_slots 
   const size_t P = (size_t)ne * npa;
   return (2 * P + 16 * (size_t)npa + 2 * (size_t)npa + 11 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 +
           2 * (size_t)npa + (size_t)ne + (size_t)mu_slots * npa) * sizeof(double);
+}
+// Chunks of rows of the forward plane kernels in blocks of T threads: one
+// chunk of NE threads per floor(T/NE), at most NPA/2 so that every chunk
+// holds two rows or more (the top row L=NPA then never opens a chunk:
+// DRIFTMU sets it from the row below, which the same thread holds).  The
+// NPA rows are shared out as evenly as they go, the first NPA % chunks
+// chunks one row longer.  0 when T < NE.
+inline __host__ __device__ int plane_chunks_for(int ne, int npa, int threads) {
+  const int q = threads / ne;
+  return q < npa / 2 ? q : npa / 2;
+}
+// the longest chunk's rows
+inline __host__ __device__ int plane_max_rows(int npa, int chunks) { return (npa + chunks - 1) / chunks; }
+// LDS of the forward plane kernels (PlaneFwd): NPA rows of NE+3 energy
+// stencil slots, the K=1 column, T exchange slots, the per-L DRIFTE and
+// DRIFTMU terms (16 NPA), HDNS and 1/FNHS (2 NPA), KT_N=11 per-K and LT_N=4
+// per-L tables, 128 reduction slots, the 64-entry exp table, PC_N=24 plane
+// scalars and the CFL slots (NE + kMuSpread*NPA).
+inline __host__ __device__ size_t plane_fwd_lds_bytes(int ne, int npa, int threads) {
+  return ((size_t)npa * (ne + 3) + (size_t)npa + (size_t)threads + 16 * (size_t)npa + 2 * (size_t)npa +
+          11 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 + (size_t)ne + (size_t)kMuSpread * npa) *
+         sizeof(double);
 }
 // the adjoint kernels record no CFL maxima: their adjoint plane takes the
 // place of the CFL slots (NE + mu_slots*NPA doubles)

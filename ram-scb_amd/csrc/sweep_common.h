@@ -516,19 +516,134 @@ __device__ inline double wave_factor(const Tab &t, int s, int i, int j, int k, d
   return exp(-t.dts / tau);
 }
 
-// Block set-up of the plane kernels: context of plane b of species s (I>=2)
-// and the LDS copies of the plane (from src, device layout of one species),
-// of the plane's scalars (PcField), of the per-K / per-L tables, with
-// need_loss of HDNS, 1/FNHS and the exponential's table, and with need_kl of
-// the per-L DRIFTE and DRIFTMU terms.  The loads go in two batches, each
-// issued whole before its LDS stores (two memory latencies per plane; one
-// batch would need more registers than the phases); plane cells beyond
-// KS per thread follow in a further pass (KS sets the registers the set-up
-// takes beside what the kernel keeps live across it).
+// LDS copies of the tables of plane (I,J) = (i,j) of species s, for any
+// plane layout D with the table pointers of PlaneLds: the plane's scalars
+// (PcField), the per-K and per-L tables, with need_loss HDNS, 1/FNHS and the
+// exponential's table, and with need_kl the per-L DRIFTE and DRIFTMU terms.
+// The loads go in two batches, each issued whole before its LDS stores.
+template <class D>
+__device__ inline void plane_tables(const Tab &t, int s, int i, int j, const D &d, bool need_loss, bool need_kl) {
+  const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr;
+  const int tid = threadIdx.x, T = blockDim.x;
+  const size_t ij = (size_t)i * NT + j;
+  const int nkl = 8 * NPA;
+  const double *se = t.kle + ij * nkl, *sm8 = t.klm + ij * nkl;
+
+  // batch A: per-L terms and scalars
+  double vc[2] = {0.0, 0.0}, vm[2] = {0.0, 0.0};
+  if (need_kl) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (tid + m * T < nkl) {
+        vc[m] = se[tid + m * T];
+        vm[m] = sm8[tid + m * T];
+      }
+  }
+  double vp = 0.0;
+  if (tid < PC_N) {
+    const double *a;
+    if (tid < PC_REL) a = t.kij + ij * 8 + tid;
+    else if (tid < PC_HN) a = t.rel + s * 8 + (tid - PC_REL);
+    else if (tid == PC_HN) a = &GEO3(t.fnhs, i + 1, j + 1, NPA);
+    else if (tid == PC_HN1) a = &GEO3(t.fnhs, i + 1, j + 1, NPA - 1);
+    else if (tid == PC_MUN) a = t.mu + NPA - 1;
+    else if (tid == PC_MUN1) a = t.mu + NPA - 2;
+    else a = &t.dts;
+    vp = *a;
+  }
+  if (need_kl) {
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+      if (tid + m * T < nkl) {
+        d.cf[tid + m * T] = vc[m];
+        d.cm[tid + m * T] = vm[m];
+      }
+  }
+  if (tid < PC_N) d.pc[tid] = vp;
+
+  // batch B: per-K and per-L tables
+  const size_t sk = (size_t)s * NE, sik = ((size_t)s * NR + i) * NE;
+  const double *wav = t.wavef + (((size_t)s * NR + i) * NT + j) * NE;
+  const bool wave = need_loss && t.wpi[s];
+  double vk[KT_N];
+  if (tid < NE) {
+    vk[KT_EKP] = t.ekp[sk + tid];
+    vk[KT_MKP] = t.mkp[sk + tid];
+    vk[KT_SIGV] = t.sigv[sk + tid];
+    vk[KT_EDOT] = t.edot[sik + tid];
+    vk[KT_LATL] = t.latl[sik + tid];
+    vk[KT_DE] = t.de[tid];
+    vk[KT_RDE] = t.rde[tid];
+    vk[KT_WE] = t.we[tid];
+    vk[KT_RWE] = t.rwe[tid];
+    vk[KT_EKEV] = t.ekev[tid];
+    // WAVELO's factor of the step (1 for K=1 and without WAVELO)
+    vk[KT_WAVE] = (wave && tid >= 1) ? wav[tid] : 1.0;
+  }
+  double vl[LT_N];
+  double2 hv = make_double2(0.0, 0.0);
+  if (tid < NPA) {
+    vl[LT_DMU] = t.dmu[tid];
+    vl[LT_RDMU] = t.rdmu[tid];
+    vl[LT_WMU] = t.wmu[tid];
+    vl[LT_RWMU] = t.rwmu[tid];
+    if (need_loss) hv = t.kll[ij * NPA + tid];
+  }
+  const double vx = (need_loss && tid < 64) ? t.ex64[tid] : 0.0;
+
+  if (tid < NE)
+#pragma unroll
+    for (int f = 0; f < KT_N; ++f) d.kt[f * NE + tid] = vk[f];
+  if (tid < NPA) {
+#pragma unroll
+    for (int f = 0; f < LT_N; ++f) d.lt[f * NPA + tid] = vl[f];
+    if (need_loss) {
+      d.cl[2 * tid] = hv.x;
+      d.cl[2 * tid + 1] = hv.y;
+    }
+  }
+  if (need_loss && tid < 64) d.ex[tid] = vx;
+
+  // what one pass did not cover (small blocks)
+  if (need_kl && nkl > 2 * T) {
+    block_copy<2>(d.cf + 2 * T, se + 2 * T, nkl - 2 * T);
+    block_copy<2>(d.cm + 2 * T, sm8 + 2 * T, nkl - 2 * T);
+  }
+  for (int k = tid + T; k < NE; k += T) {
+    d.kt[KT_EKP * NE + k] = t.ekp[sk + k];
+    d.kt[KT_MKP * NE + k] = t.mkp[sk + k];
+    d.kt[KT_SIGV * NE + k] = t.sigv[sk + k];
+    d.kt[KT_EDOT * NE + k] = t.edot[sik + k];
+    d.kt[KT_LATL * NE + k] = t.latl[sik + k];
+    d.kt[KT_DE * NE + k] = t.de[k];
+    d.kt[KT_RDE * NE + k] = t.rde[k];
+    d.kt[KT_WE * NE + k] = t.we[k];
+    d.kt[KT_RWE * NE + k] = t.rwe[k];
+    d.kt[KT_EKEV * NE + k] = t.ekev[k];
+    d.kt[KT_WAVE * NE + k] = wave ? wav[k] : 1.0;
+  }
+  for (int l = tid + T; l < NPA; l += T) {
+    d.lt[LT_DMU * NPA + l] = t.dmu[l];
+    d.lt[LT_RDMU * NPA + l] = t.rdmu[l];
+    d.lt[LT_WMU * NPA + l] = t.wmu[l];
+    d.lt[LT_RWMU * NPA + l] = t.rwmu[l];
+    if (need_loss) {
+      const double2 h = t.kll[ij * NPA + l];
+      d.cl[2 * l] = h.x;
+      d.cl[2 * l + 1] = h.y;
+    }
+  }
+}
+
+// Block set-up of the plane adjoint kernels (linear plane layout of
+// PlaneLds): context of plane b of species s (I>=2), the LDS copy of the
+// plane (from src, device layout of one species) and its tables
+// (plane_tables).  The plane's first KS cells per thread are in flight
+// while the tables load; the rest follow in a further pass.
 template <int KS>
 __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, const double *src, bool need_loss,
                                    bool need_kl, PlaneCtx &c, PlaneLds &d) {
-  const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr;
+  const int NE = t.ne, NPA = t.npa, NT = t.nt;
   const int tid = threadIdx.x, T = blockDim.x;
   c.s = u.s;
   c.i = 1 + u.b / NT;  // I = i+1 >= 2
@@ -546,136 +661,32 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.beta = t.betalim;
   c.rne = t.rne;
   c.nmine = (c.P - tid + T - 1) / T;  // cells tid, tid+blockDim, .. < P
-  c.lestep = t.plane_threads / NE;  // the problem's block size sets the rows per thread
+  c.lestep = T / NE;
   c.le0 = fast_div(tid, NE, c.rne);
   c.ke = tid - c.le0 * NE;
   c.eact = tid < c.lestep * NE;
   d = plane_lds(sm, NE, NPA);
-  const int P = c.P, nkl = 8 * NPA;
+  const int P = c.P;
   const double *gF = src + (size_t)c.s * f2_species_stride(t) + c.ij * P;
-  const double *se = t.kle + c.ij * nkl, *sm8 = t.klm + c.ij * nkl;
-
-  // batch A: the plane, its per-L terms and scalars
   double vf[KS];
 #pragma unroll
   for (int m = 0; m < KS; ++m) vf[m] = (tid + m * T < P) ? gF[tid + m * T] : 0.0;
-  double vc[2] = {0.0, 0.0}, vm[2] = {0.0, 0.0};
-  if (need_kl) {
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-      if (tid + m * T < nkl) {
-        vc[m] = se[tid + m * T];
-        vm[m] = sm8[tid + m * T];
-      }
-  }
-  double vp = 0.0;
-  if (tid < PC_N) {
-    const double *a;
-    if (tid < PC_REL) a = t.kij + c.ij * 8 + tid;
-    else if (tid < PC_HN) a = t.rel + c.s * 8 + (tid - PC_REL);
-    else if (tid == PC_HN) a = &GEO3(t.fnhs, c.i + 1, c.j + 1, NPA);
-    else if (tid == PC_HN1) a = &GEO3(t.fnhs, c.i + 1, c.j + 1, NPA - 1);
-    else if (tid == PC_MUN) a = t.mu + NPA - 1;
-    else if (tid == PC_MUN1) a = t.mu + NPA - 2;
-    else a = &t.dts;
-    vp = *a;
-  }
+  plane_tables(t, c.s, c.i, c.j, d, need_loss, need_kl);
 #pragma unroll
   for (int m = 0; m < KS; ++m)
     if (tid + m * T < P) d.F[tid + m * T] = vf[m];
-  if (need_kl) {
-#pragma unroll
-    for (int m = 0; m < 2; ++m)
-      if (tid + m * T < nkl) {
-        d.cf[tid + m * T] = vc[m];
-        d.cm[tid + m * T] = vm[m];
-      }
-  }
-  if (tid < PC_N) d.pc[tid] = vp;
-
-  // batch B: per-K and per-L tables
-  const size_t sk = (size_t)c.s * NE, sik = ((size_t)c.s * NR + c.i) * NE;
-  double vk[KT_N];
-  if (tid < NE) {
-    vk[KT_EKP] = t.ekp[sk + tid];
-    vk[KT_MKP] = t.mkp[sk + tid];
-    vk[KT_SIGV] = t.sigv[sk + tid];
-    vk[KT_EDOT] = t.edot[sik + tid];
-    vk[KT_LATL] = t.latl[sik + tid];
-    vk[KT_DE] = t.de[tid];
-    vk[KT_RDE] = t.rde[tid];
-    vk[KT_WE] = t.we[tid];
-    vk[KT_RWE] = t.rwe[tid];
-    vk[KT_EKEV] = t.ekev[tid];
-    // WAVELO's factor of the step (1 for K=1 and without WAVELO)
-    vk[KT_WAVE] = (need_loss && t.wpi[c.s] && tid >= 1)
-                      ? t.wavef[((((size_t)c.s * t.nr + c.i) * NT + c.j) * NE) + tid]
-                      : 1.0;
-  }
-  double vl[LT_N];
-  double2 hv = make_double2(0.0, 0.0);
-  if (tid < NPA) {
-    vl[LT_DMU] = t.dmu[tid];
-    vl[LT_RDMU] = t.rdmu[tid];
-    vl[LT_WMU] = t.wmu[tid];
-    vl[LT_RWMU] = t.rwmu[tid];
-    if (need_loss) hv = t.kll[c.ij * NPA + tid];
-  }
-  const double vx = (need_loss && tid < 64) ? t.ex64[tid] : 0.0;
-
-  if (tid < NE)
-#pragma unroll
-    for (int f = 0; f < KT_N; ++f) d.kt[f * NE + tid] = vk[f];
-  if (tid < NPA) {
-#pragma unroll
-    for (int f = 0; f < LT_N; ++f) d.lt[f * NPA + tid] = vl[f];
-    if (need_loss) {
-      d.cl[2 * tid] = hv.x;
-      d.cl[2 * tid + 1] = hv.y;
-    }
-  }
-  if (need_loss && tid < 64) d.ex[tid] = vx;
-
-  // what one pass did not cover (large planes, small blocks)
   if (P > KS * T) block_copy<4>(d.F + KS * T, gF + KS * T, P - KS * T);
-  if (need_kl && nkl > 2 * T) {
-    block_copy<2>(d.cf + 2 * T, se + 2 * T, nkl - 2 * T);
-    block_copy<2>(d.cm + 2 * T, sm8 + 2 * T, nkl - 2 * T);
-  }
-  for (int k = tid + T; k < NE; k += T) {
-    d.kt[KT_EKP * NE + k] = t.ekp[sk + k];
-    d.kt[KT_MKP * NE + k] = t.mkp[sk + k];
-    d.kt[KT_SIGV * NE + k] = t.sigv[sk + k];
-    d.kt[KT_EDOT * NE + k] = t.edot[sik + k];
-    d.kt[KT_LATL * NE + k] = t.latl[sik + k];
-    d.kt[KT_DE * NE + k] = t.de[k];
-    d.kt[KT_RDE * NE + k] = t.rde[k];
-    d.kt[KT_WE * NE + k] = t.we[k];
-    d.kt[KT_RWE * NE + k] = t.rwe[k];
-    d.kt[KT_EKEV * NE + k] = t.ekev[k];
-    d.kt[KT_WAVE * NE + k] =
-        (need_loss && t.wpi[c.s]) ? t.wavef[((((size_t)c.s * t.nr + c.i) * NT + c.j) * NE) + k] : 1.0;
-  }
-  for (int l = tid + T; l < NPA; l += T) {
-    d.lt[LT_DMU * NPA + l] = t.dmu[l];
-    d.lt[LT_RDMU * NPA + l] = t.rdmu[l];
-    d.lt[LT_WMU * NPA + l] = t.wmu[l];
-    d.lt[LT_RWMU * NPA + l] = t.rwmu[l];
-    if (need_loss) {
-      const double2 h = t.kll[c.ij * NPA + l];
-      d.cl[2 * l] = h.x;
-      d.cl[2 * l + 1] = h.y;
-    }
-  }
   __syncthreads();
 }
 
 // CHARGE=EXP(-ALPHA), ALPHA=10**Y*V*HDNS*DTs (src/ModRamLoss.f90:47-49)
-__device__ __forceinline__ double cex_factor(const PlaneLds &d, int NE, int k, int l, double dts) {
+template <class D>
+__device__ __forceinline__ double cex_factor(const D &d, int NE, int k, int l, double dts) {
   return exp_tab(-(d.kt[KT_SIGV * NE + k] * d.cl[2 * l] * dts), d.ex);
 }
 // ATLOS(S,I,K)**(1/FNHS(I,J,L)) (src/ModRamLoss.f90:501) as exp(log(ATLOS)*(1/FNHS))
-__device__ __forceinline__ double atm_factor(const PlaneLds &d, int NE, int k, int l) {
+template <class D>
+__device__ __forceinline__ double atm_factor(const D &d, int NE, int k, int l) {
   return exp_tab(d.kt[KT_LATL * NE + k] * d.cl[2 * l + 1], d.ex);
 }
 
