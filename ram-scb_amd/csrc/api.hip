@@ -306,14 +306,14 @@ int run_p(ramgpu_ctx *c, int rec, int s0, int nsp, const double *src, double *ds
 // thread holding one K of the rows of its chunk (plane_chunks_for), the
 // row bound MAXC 8 or 16.  A plane's phases are VALU-bound chains between
 // barriers, so a block's time follows the rows of its busiest SIMD (the
-// longest chunk a wave holds, summed over the waves on that SIMD), over an
-// efficiency that measured (MI355X, 35x72, 48x90 and 64x90 planes) near 1
-// with two or more blocks resident on a CU filling its 16 wave slots (the
-// register budget of 4 waves per SIMD), 0.8 with two or more that do not
-// fill them, and 0.65 with one block, whose barriers stall the whole CU.
-// Residency is limited by the wave slots and by the LDS (lds_cu per CU).
-// Ties go to the smaller block.  A function of (NE, NPA, LDS) only, so a
-// problem keeps its mapping in any launch.
+// longest chunk a wave holds, summed over the waves on that SIMD).  Two
+// blocks resident on a CU sweep two planes at once and keep it busy; a
+// third added nothing, and one block alone, whose barriers stall the whole
+// CU, ran at 0.65 of one of two (MI355X, 35x72, 48x90 and 64x90 planes).
+// Residency is limited by the 16 wave slots of the register budget (4
+// waves per SIMD) and by the LDS (lds_cu per CU).  Ties go to the larger
+// block.  A function of (NE, NPA, LDS) only, so a problem keeps its
+// mapping in any launch.
 PlaneShape plane_shape(int ne, int npa, size_t lds_cu) {
   PlaneShape best{0, 0, 0};
   double bcost = 0;
@@ -330,10 +330,10 @@ PlaneShape plane_shape(int ne, int npa, size_t lds_cu) {
       if (c0 < q) simd[w & 3] += base + (c0 < extra ? 1 : 0);
     }
     const int busiest = std::max(std::max(simd[0], simd[1]), std::max(simd[2], simd[3]));
-    const double eff = res >= 2 ? (res * (T / 64) >= 16 ? 1.0 : 0.8) : 0.65;
-    // planes per CU at a time: res; cost per plane
-    const double cost = busiest / eff / res;
-    if (best.threads == 0 || cost < bcost - 1e-9) {
+    // planes a CU sweeps at a time (two are enough to keep it busy); one
+    // block alone stalls the CU at each barrier
+    const double cost = busiest / (res >= 2 ? 2.0 : 0.65);
+    if (best.threads == 0 || cost <= bcost + 1e-9) {
       best = sh;
       bcost = cost;
     }
@@ -563,7 +563,6 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   // gfx950: the LDS of a CU is the largest a block may use
   c->plane = plane_shape(ne, npa, c->lds_limit);
   if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));
-  t.plane_threads = c->plane.threads;
   t.plane_chunks = plane_chunks_for(ne, npa, c->plane.threads);
 #define A(p, n) \
   if ((rc = dev_alloc(c, &p, n))) return cleanup(rc)
@@ -682,7 +681,6 @@ int ramgpu_set_plane_shape(ramgpu_ctx *c, int threads, int maxc) {
       return fail(RAMGPU_EINVAL, "plane shape %d threads: the plane's LDS exceeds the device's", threads);
     c->plane = {threads, maxc, rows};
   }
-  c->t.plane_threads = c->plane.threads;
   c->t.plane_chunks = plane_chunks_for(c->ne, c->npa, c->plane.threads);
   c->tab_dirty = true;
   return RAMGPU_OK;

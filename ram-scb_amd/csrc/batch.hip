@@ -50,10 +50,8 @@ struct ramgpu_batch {
   size_t plane_lds = 0, radial_lds = 0, azim_lds = 0, ghost_lds = 0;
   int mpts_max = 0;  // most satellite points of a member (ensemble adjoint)
   // forward plane blocks of every member: the largest block a member
-  // chose, each member's chunks of rows in it (plane_chunks[m]) and the
-  // largest row bound among them
+  // chose and the largest row bound among them
   PlaneShape plane{64, 8, 0};
-  std::vector<int> plane_chunks;
   int sum_ns = 0;
   std::vector<int> bs_off;  // first species of member m in a schedule row
   int *d_bsoff = nullptr;
@@ -152,10 +150,10 @@ int sync_members(ramgpu_batch *b) {
   for (size_t m = 0; m < b->m.size(); ++m) {
     ramgpu_ctx *c = b->m[m];
     if ((rc = check_ctx(c)) || (rc = prepare(c))) return rc;
+    if (c->plane.threads > b->plane.threads || c->plane.maxc > b->plane.maxc)
+      return fail(RAMGPU_ESTATE, "member %zu changed its plane shape after joining the ensemble", m);
     HIP_TRY(hipStreamSynchronize(c->stream));
     tabs.push_back(c->t);
-    tabs.back().plane_threads = b->plane.threads;  // the ensemble's plane blocks
-    tabs.back().plane_chunks = b->plane_chunks[m];
   }
   HIP_TRY(hipSetDevice(b->device));
   HIP_TRY(hipMemcpy(b->d_tabs, tabs.data(), tabs.size() * sizeof(Tab), hipMemcpyHostToDevice));
@@ -439,30 +437,22 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->ghost_lds = std::max(b->ghost_lds, (size_t)4 * c->nt * sizeof(double));
     b->plane.threads = std::max(b->plane.threads, c->plane.threads);
   }
-  // one block size for all members: the largest any member chose, else the
-  // next that suits them all
+  // one block size for all members, the largest any member chose: each
+  // member keeps its own rows per thread (its Tab's plane_chunks), so its
+  // planes are swept exactly as when it runs alone; the threads above its
+  // chunks are idle
   {
-    const int want = b->plane.threads;
-    int cands[5] = {want, 1024, 768, 512, 256}, chosen = 0;
-    for (int T : cands) {
-      int maxc = 8;
-      size_t lds = 0;
-      bool ok = true;
-      for (ramgpu_ctx *c : b->m) {
-        const PlaneShape sh = plane_shape_for(c->ne, c->npa, T);
-        if (sh.threads == 0) ok = false;
-        maxc = std::max(maxc, sh.maxc);
-        lds = std::max(lds, plane_fwd_lds_bytes(c->ne, c->npa, T));
-      }
-      if (!ok || lds > b->m[0]->lds_limit) continue;
-      chosen = T;
-      b->plane = {T, maxc, 0};
-      b->plane_lds = lds;
-      break;
+    size_t lds = 0;
+    int maxc = 8;
+    for (ramgpu_ctx *c : b->m) {
+      maxc = std::max(maxc, c->plane.maxc);
+      lds = std::max(lds, plane_fwd_lds_bytes(c->ne, c->npa, b->plane.threads));
     }
-    if (!chosen) return cleanup(fail(RAMGPU_EINVAL, "no plane block size suits every member"));
-    for (ramgpu_ctx *c : b->m) b->plane_chunks.push_back(plane_chunks_for(c->ne, c->npa, chosen));
+    if (lds > b->m[0]->lds_limit) return cleanup(fail(RAMGPU_EINVAL, "the members' planes do not fit one block's LDS"));
+    b->plane = {b->plane.threads, maxc, 0};
+    b->plane_lds = lds;
   }
+
 
   if ((rc = batch_alloc(b, (void **)&b->d_bsoff, nmembers * sizeof(int))) ||
       (rc = batch_alloc(b, (void **)&b->d_step, sizeof(int))))

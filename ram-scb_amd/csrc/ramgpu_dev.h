@@ -99,10 +99,10 @@ struct Tab {
   double *dtdrift;      // [s][4]
   // float reciprocals for exact small-integer division (fast_div)
   float rnt, rne, rp;   // 1/NT, 1/NE, 1/(NE*NPA)
-  // row mapping of the forward plane kernels (plane_threads a multiple of
-  // 64): thread n < plane_chunks*NE owns K = n%NE of the rows of chunk
-  // n/NE (plane_chunks_for); the threads above are idle
-  int plane_threads, plane_chunks;
+  // row mapping of the forward plane kernels: thread n < plane_chunks*NE
+  // owns K = n%NE of the rows of chunk n/NE (plane_chunks_for); the
+  // threads above, in a block of any size, are idle
+  int plane_chunks;
   // state
   double *f2;
   double *partials;  // [slot][s][maxblk]
