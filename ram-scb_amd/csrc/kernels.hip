@@ -737,10 +737,10 @@ __global__ void __launch_bounds__(kLineThreads) k_p(Launch L) {
 // Plane kernel: one block per (species, I>=2, J), holding the (K,L) plane in
 // LDS.  Thread n < Q*NE (Q = plane_chunks) owns column K = n%NE of the
 // consecutive rows of chunk q = n/NE (NPA/Q rows, the first NPA%Q chunks
-// one more); only the owner of a cell ever writes it, so a thread may store
-// its own cells without a barrier.  The reference divides both face fluxes of cell K by
-// that cell's width, CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*
-// FBND(K-1) (src/ModRamDrift.f90:364; :460 for Mu): the owner of face K
+// one more); only the owner of a cell ever writes it.  The reference
+// divides both face fluxes of cell K by that cell's width,
+// CDriftE(K)/WE(K)*FBND(K) and CDriftE(K-1)/WE(K)*FBND(K-1)
+// (src/ModRamDrift.f90:364; :460 for Mu): the owner of face K
 // forms F(K) - C(K)/WE(K)*FBND(K) and the term C(K)/WE(K+1)*FBND(K) its
 // upper neighbour adds.  Along L the neighbour is the same thread (the
 // chunk's last row hands its term to the next chunk through LDS); along K
@@ -873,14 +873,16 @@ __device__ inline void fwd_setup(const Tab &t, const Unit &u, double *sm, const 
   __syncthreads();
 }
 
-// This thread's cells to dst (device layout of one species' plane).
-template <int MAXC>
-__device__ __forceinline__ void fwd_store(const FwdCtx &c, double *dst) {
+// The plane to dst (device layout of one species' plane), read from LDS
+// after a barrier: consecutive threads store consecutive cells, so every
+// cache line of the plane is written whole by one instruction.
+__device__ __forceinline__ void fwd_store(const FwdCtx &c, const PlaneFwd &d, float rne, double *dst) {
   const gptr<double> g = as_global(dst);
-  const int l0 = fresh(c.l0), NE = c.NE, k = c.ke;
-#pragma unroll
-  for (int n = 0; n < MAXC; ++n)
-    if (n < c.nr) g[(unsigned)((l0 + n) * NE + k)] = c.col[(l0 + n) * c.cst];
+  const int NE = c.NE, P = NE * c.NPA, S = c.S;
+  for (int i = threadIdx.x; i < P; i += blockDim.x) {
+    const int l = fast_div(i, NE, rne), k = i - l * NE;
+    g[(unsigned)i] = (k == 0) ? d.F1[l] : d.F[l * S + k + 1];
+  }
 }
 
 // Rows of one pass of the sweeps: a thread's rows go in passes of at most
@@ -924,7 +926,7 @@ __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool r
   const int lane = threadIdx.x & 63, nw = blockDim.x >> 6;
   const int l0 = fresh(c.l0);
   double cmax = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int p0 = 0; p0 < MAXC; p0 += kPassRows) {
     if (p0 >= c.R) break;
     // the terms lane 63 hands to the next wave's lane 0, per pass and wave
@@ -1003,7 +1005,7 @@ __device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool 
   double fm = ld(max(l0 - 1, 0)), f0 = ld(l0), f1 = ld(l0 + 1);
   const double up0 = ld(l0 + nr), up1 = ld(l0 + nr + 1);
   double hprev = -0.0, v0 = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int p0 = 0; p0 < MAXC; p0 += kPassRows) {
     if (p0 >= R) break;
     double v[kPassRows];
@@ -1223,14 +1225,18 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
       // its own cells, which only it writes.
       const size_t ns = f2_elems(t);
       double *slot = const_cast<double *>(launch_src(L, t)) + po;
+      // each phase ends with a barrier, so its output is whole in LDS; the
+      // loss chain rewrites cells at once, so a barrier follows the store
+      // before it (the later phases rewrite cells only after barriers)
       plane_e<MAXC>(t, c, d, rec_here);
-      fwd_store<MAXC>(c, slot + ns);
+      fwd_store(c, d, t.rne, slot + ns);
       plane_mu<MAXC>(t, c, d, rec_here);
-      fwd_store<MAXC>(c, slot + 2 * ns);
+      fwd_store(c, d, t.rne, slot + 2 * ns);
+      __syncthreads();
       plane_loss<MAXC>(t, c, d, u.b);
-      fwd_store<MAXC>(c, slot + 3 * ns);
-      plane_mu<MAXC>(t, c, d, false);  // the loss chain's block sum ends with a barrier
-      fwd_store<MAXC>(c, slot + 4 * ns);
+      fwd_store(c, d, t.rne, slot + 3 * ns);
+      plane_mu<MAXC>(t, c, d, false);
+      fwd_store(c, d, t.rne, slot + 4 * ns);
       plane_e<MAXC>(t, c, d, false);
     } else {  // SEQ_STEP
       plane_e<MAXC>(t, c, d, rec_here);
@@ -1240,7 +1246,10 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
       plane_e<MAXC>(t, c, d, false);
     }
 
-    if constexpr (SEQ != SEQ_SUM) fwd_store<MAXC>(c, gF);
+    if constexpr (SEQ != SEQ_SUM) {
+      if constexpr (SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_WAVE) __syncthreads();  // cells rewritten at once
+      fwd_store(c, d, t.rne, gF);
+    }
     __syncthreads();
   }
 }
