@@ -380,6 +380,7 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
   Launch L = launch_of(c, s0, nsp, blocks_kl(c->t) + (copy ? c->nt : 0));
   L.src = src;
   L.dst = dst;
+  L.ilv = 1;  // a plane's species on one XCD
   const size_t lds = plane_fwd_lds_bytes(c->ne, c->npa, c->plane.threads);
   tbeg(c, RAMGPU_KERNEL_PLANE);
   launch_plane(seq, c->plane, dim3(L.nv), lds, c->stream, L, rec);

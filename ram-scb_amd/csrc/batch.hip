@@ -102,10 +102,19 @@ int build_plan(ramgpu_batch *b, int kind) {
     const ramgpu_ctx *c = b->m[mi];
     const int nsp = per_member(kind) ? 1 : c->ns;
     const int nb = blocks_of(kind, c);
-    for (int s = 0; s < nsp; ++s) {
-      Job j{(int)mi, s, (int)vmap.size(), nb};
-      for (int x = 0; x < nb; ++x) vmap.push_back((int)jobs.size());
-      jobs.push_back(j);
+    if (nb >= (1 << kVmapShift) || jobs.size() + nsp >= (1u << (31 - kVmapShift)))
+      return fail(RAMGPU_EINVAL, "ensemble too large for the launch tables");
+    const int j0 = (int)jobs.size();
+    for (int s = 0; s < nsp; ++s) jobs.push_back(Job{(int)mi, s, (int)vmap.size(), nb});
+    if (kind == BK_PLANE) {
+      // the species of each run of 8 planes in turn: a plane's species land
+      // on one XCD, whose L2 serves them the plane's tables (Launch::ilv)
+      for (int g = 0; g < nb; g += 8)
+        for (int s = 0; s < nsp; ++s)
+          for (int x = g; x < std::min(nb, g + 8); ++x) vmap.push_back(vmap_entry(j0 + s, x));
+    } else {
+      for (int s = 0; s < nsp; ++s)
+        for (int x = 0; x < nb; ++x) vmap.push_back(vmap_entry(j0 + s, x));
     }
   }
   Plan &p = b->plan[kind];

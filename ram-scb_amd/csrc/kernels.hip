@@ -1483,7 +1483,7 @@ __global__ void k_copy_ens(Launch L) {
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
-    const int nb = L.vmap ? L.jobs[L.vmap[vb]].nb : L.nb;
+    const int nb = L.vmap ? L.jobs[L.vmap[vb] >> kVmapShift].nb : L.nb;
     const size_t n = (size_t)t.ns * f2_species_stride(t);
     const double *src = launch_src(L, t);
     double *dst = launch_dst(L, t);

@@ -123,11 +123,18 @@ struct Tab {
 struct Job {
   int tab, s, b0, nb;
 };
+// vmap entries: a virtual block's job and its block within the job
+constexpr int kVmapShift = 16;
+inline __host__ __device__ int vmap_entry(int job, int b) { return (job << kVmapShift) | b; }
 
 // A launch over several units.  With vmap == nullptr the launch covers
-// species s0, s0+1, .. of tabs[0], nb blocks each; otherwise vmap[vb] is the
-// job of virtual block vb.  Blocks loop over virtual blocks with stride
-// gridDim.x, so the grid may be smaller than nv.
+// species s0, s0+1, .. of tabs[0], nb blocks each, species after species,
+// or with ilv the species of each run of 8 blocks in turn (so the species'
+// blocks of one plane land on one XCD, whose L2 then serves the plane's
+// species-independent tables to all of them: workgroups go to the 8 XCDs
+// in turn); otherwise vmap[vb] is the vmap_entry of virtual block vb.
+// Blocks loop over virtual blocks with stride gridDim.x, so the grid may be
+// smaller than nv.
 // src/dst (sweep kernels): read F2 from src and write to dst (device layout
 // of the whole problem) instead of updating tabs[..].f2 in place; null for in
 // place.  ssl/dsl name them per problem instead (ensembles): n > 0 is slot
@@ -137,7 +144,7 @@ struct Launch {
   const Tab *tabs;
   const Job *jobs;
   const int *vmap;
-  int nv, nb, s0;
+  int nv, nb, s0, ilv;
   const double *src;
   double *dst;
   int ssl, dsl;

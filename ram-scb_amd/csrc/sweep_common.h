@@ -159,11 +159,19 @@ struct Unit {
 
 __device__ __forceinline__ Unit unit_of(const Launch &L, int vb) {
   if (!L.vmap) {
+    if (L.ilv) {
+      const int nsp = L.nv / L.nb, span = 8 * nsp;
+      const int g = vb / span, r = vb - g * span;
+      const int rem = min(8, L.nb - 8 * g);
+      const int s = r / rem;
+      return {L.tabs, L.s0 + s, 8 * g + (r - s * rem)};
+    }
     const int n = vb / L.nb;
     return {L.tabs, L.s0 + n, vb - n * L.nb};
   }
-  const Job jb = L.jobs[L.vmap[vb]];
-  return {L.tabs + jb.tab, jb.s, vb - jb.b0};
+  const int e = L.vmap[vb];
+  const Job jb = L.jobs[e >> kVmapShift];
+  return {L.tabs + jb.tab, jb.s, e & ((1 << kVmapShift) - 1)};
 }
 
 // ---------------------------------------------------------------------------
