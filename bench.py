@@ -41,7 +41,7 @@ More objects in the same line:
     every 20, in one launch per kernel, against the ensemble's forward;
   "moments": ANISCH of every species (and with FLUX) on the resident F2.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M] [--multi-only]
 """
 from __future__ import annotations
 
@@ -69,6 +69,7 @@ KERNEL_NAMES = {"r": "k_r (radial sweep, DRIFTR)", "p": "k_p (azimuthal sweep, D
 ALGO_BYTES_PER_CELL_UPDATE = 128  # 8 directional sweeps x (read + write) of F2
 METRIC = "FP64 RAM split-step cell-updates/s + HBM roofline %; batched & adjoint vs plain"
 TRAFFIC_FILE = REPO / "profiles" / "traffic_c3_ensemble.txt"
+TRAFFIC_FILE_MULTI = REPO / "profiles" / "traffic_multi_species.txt"
 # MI355X vector issue: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one
 # wave64 VALU instruction (FP64 included) per 4 cycles per SIMD
 VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
@@ -86,14 +87,14 @@ def schedule(nsteps: int, seed: int, ns: int):
     return kp, bscale
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of a kernel of the headline workload, from the
-    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE,
-    copy-calibrated)."""
-    if not TRAFFIC_FILE.exists():
+def measured_traffic(kernel: str, path: Path = TRAFFIC_FILE):
+    """HBM bytes per launch of a kernel kind ("plane", "r", "p") of a
+    workload, from its committed rocprofv3 counter summary (FETCH_SIZE +
+    WRITE_SIZE, copy-calibrated)."""
+    if not path.exists():
         return None
     key = f"traffic_bytes_per_launch[{kernel}]="
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in path.read_text().splitlines():
         if line.startswith(key):
             return float(line.split("=", 1)[1])
     return None
@@ -155,8 +156,9 @@ def calibrate_copy(nbytes: int = 1 << 29, reps: int = 5):
     print(json.dumps({"calibration_copy_bytes": nbytes, "reps": reps + 1, "copy_GBps": gbps}))
 
 
-def per_kernel_roofline(ktimes: dict, cells: int, cells_kl: int, traffic_kernel: str | None = None):
-    """Per-kernel algorithmic GB/s from summed event times, and the roofline
+def per_kernel_roofline(ktimes: dict, cells: int, cells_kl: int, traffic_path: Path = TRAFFIC_FILE):
+    """Per-kernel algorithmic GB/s from summed event times (and the measured
+    HBM bytes per launch of the workload's counter summary), and the roofline
     object of the dominant kernel (by total device time)."""
     per = {}
     for k in ("r", "p", "plane"):
@@ -166,12 +168,12 @@ def per_kernel_roofline(ktimes: dict, cells: int, cells_kl: int, traffic_kernel:
         avg = ms_tot / n * 1e-3
         nb = SWEEP_BYTES * SWEEPS_PER_LAUNCH[k] * (cells if k == "r" else cells_kl)
         per[k] = {"avg_launch_us": avg * 1e6, "bytes_per_launch": nb, "achieved_GBps": nb / avg / 1e9,
-                  "launches": n, "frac": nb / avg / 1e9 / HBM_PEAK_GBPS}
+                  "launches": n, "frac": nb / avg / 1e9 / HBM_PEAK_GBPS,
+                  "traffic": measured_traffic(k, traffic_path)}
     dom = max(per, key=lambda k: ktimes[k][0])
     d = per[dom]
     roof = {"bound": "hbm", "achieved": d["achieved_GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": d["achieved_GBps"] / HBM_PEAK_GBPS,
-            "traffic": measured_traffic(traffic_kernel or dom),
+            "frac": d["achieved_GBps"] / HBM_PEAK_GBPS, "traffic": d["traffic"],
             "kernel": KERNEL_NAMES[dom], "bytes_per_launch": d["bytes_per_launch"],
             "avg_launch_us": d["avg_launch_us"], "per_kernel": per}
     return roof
@@ -342,7 +344,7 @@ def bench_multi_species(steps: int, warmup: int, seed: int):
     ev_ms = dev.last_run_ms()
     ktimes = dev.kernel_times()
     dev.kernel_timing(False)
-    roof = per_kernel_roofline(ktimes, cells, cells_kl)
+    roof = per_kernel_roofline(ktimes, cells, cells_kl, TRAFFIC_FILE_MULTI)
     roof["traffic"] = None  # the committed counters are the headline's
     value = cells * steps / (dev_ms * 1e-3)
     log(f"bench: multi-species {dev_ms / steps * 1e3:.1f} us/step, {value:.3e} cell-updates/s; plane "
@@ -477,6 +479,8 @@ def main():
     ap.add_argument("--ens-adjoint-members", type=int, default=8,
                     help="members of the high-resolution ensemble adjoint (0: skip)")
     ap.add_argument("--ens-adjoint-steps", type=int, default=120)
+    ap.add_argument("--multi-only", action="store_true",
+                    help="run only the multi-species leg and print its object (profiling)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -489,6 +493,9 @@ def main():
         return
     if args.steps < 1 or args.warmup < 0 or args.members < 2:
         ap.error("--steps must be >= 1, --warmup >= 0, --members >= 2")
+    if args.multi_only:
+        print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed)[0]), flush=True)
+        return
 
     c3 = bench_c3(args.members, args.steps, args.warmup, args.seed)
     value = c3["cells"] * args.steps / (c3["ms"] * 1e-3)

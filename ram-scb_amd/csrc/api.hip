@@ -168,6 +168,22 @@ int sync_tab(ramgpu_ctx *c) {
 }
 
 // species s0..s0+nsp-1 (0-based) of this context, nb blocks each
+// Whether a species of the problem takes WAVELO.
+bool takes_wave(const ramgpu_ctx *c) {
+  for (int s = 0; s < c->ns; ++s)
+    if (c->t.wpi[s]) return true;
+  return false;
+}
+
+// Units of the per-step coefficient pass in blocks of `threads`: the (I,J)
+// terms, then WAVELO's factors (one (S,I,J,K) per thread) when a species
+// takes it.
+int coef_units(const ramgpu_ctx *c, int threads) {
+  int n = (c->nr * c->nt + threads - 1) / threads;
+  if (takes_wave(c)) n += (int)(((long long)c->ns * c->nr * c->nt * c->ne + threads - 1) / threads);
+  return n;
+}
+
 Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb) {
   Launch L{};
   L.tabs = c->d_tab;
@@ -221,7 +237,7 @@ int prepare(ramgpu_ctx *c) {
     const size_t q1b = (size_t)c->npa * c->nt * sizeof(double);
     const int stage_q1 = c->head_static && c->head_static + lds + q1b <= 65536;
     if (stage_q1) lds += q1b;
-    const Launch lc = launch_of(c, 0, 1, (c->nr * c->nt + kFinalizeThreads - 1) / kFinalizeThreads);
+    const Launch lc = launch_of(c, 0, 1, coef_units(c, kFinalizeThreads));
     const Launch lf = launch_of(c, 0, c->fin_mask >= 0 ? c->ns : 0, 1);
     const Launch lg = launch_of(c, 0, c->ns, 1);
     hipLaunchKernelGGL(k_step_head, dim3(lc.nv + lf.nv + lg.nv), dim3(kFinalizeThreads), lds, c->stream, lc,

@@ -52,6 +52,7 @@ struct ramgpu_batch {
   // forward plane blocks of every member: the largest block a member
   // chose and the largest row bound among them
   PlaneShape plane{64, 8, 0};
+  std::vector<char> wave;  // each member's WAVELO at creation (its coefficient units)
   int sum_ns = 0;
   std::vector<int> bs_off;  // first species of member m in a schedule row
   int *d_bsoff = nullptr;
@@ -81,7 +82,7 @@ int batch_alloc(ramgpu_batch *b, void **p, size_t bytes) {
 int blocks_of(int kind, const ramgpu_ctx *c) {
   const int P = c->ne * c->npa;
   switch (kind) {
-    case BK_COEFIJ: return (c->nr * c->nt + 255) / 256;
+    case BK_COEFIJ: return coef_units(c, 256);
     case BK_R: return blocks_r(c->t);
     case BK_P: return blocks_p(c->t);
     case BK_PLANE: return blocks_kl(c->t);
@@ -161,6 +162,8 @@ int sync_members(ramgpu_batch *b) {
     if ((rc = check_ctx(c)) || (rc = prepare(c))) return rc;
     if (c->plane.threads > b->plane.threads || c->plane.maxc > b->plane.maxc)
       return fail(RAMGPU_ESTATE, "member %zu changed its plane shape after joining the ensemble", m);
+    if ((char)takes_wave(c) != b->wave[m])
+      return fail(RAMGPU_ESTATE, "member %zu switched WAVELO after joining the ensemble", m);
     HIP_TRY(hipStreamSynchronize(c->stream));
     tabs.push_back(c->t);
   }
@@ -445,6 +448,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     b->azim_lds = std::max(b->azim_lds, azim_lds_bytes(c->nt, c->ne, c->npa));
     b->ghost_lds = std::max(b->ghost_lds, (size_t)4 * c->nt * sizeof(double));
     b->plane.threads = std::max(b->plane.threads, c->plane.threads);
+    b->wave.push_back((char)takes_wave(c));
   }
   // one block size for all members, the largest any member chose: each
   // member keeps its own rows per thread (its Tab's plane_chunks), so its

@@ -133,6 +133,8 @@ int step_once(ramgpu_ctx *c);
 int step_body(ramgpu_ctx *c, int *cflmask);
 // cells per thread the plane kernels need for an NE x NPA plane (8 or 16)
 PlaneShape plane_shape(int ne, int npa, size_t lds_cu);
+int coef_units(const ramgpu_ctx *c, int threads);
+bool takes_wave(const ramgpu_ctx *c);
 PlaneShape plane_shape_for(int ne, int npa, int threads);
 // launch k_plane<seq, maxc>
 void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);

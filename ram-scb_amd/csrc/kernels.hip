@@ -184,6 +184,18 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
   {
     const Unit u = unit_of(L, vb);
     const Tab &t = *u.t;
+    const int nbc = (t.nr * t.nt + blockDim.x - 1) / blockDim.x;  // units of the (I,J) terms
+    if (u.b >= nbc) {
+      // WAVELO's factors at the step's Kp, one (S,I,J,K) per thread, K >= 2,
+      // I >= 2, for the species that take it (src/ModRamWPI.f90:596-630)
+      const int m = (u.b - nbc) * blockDim.x + threadIdx.x;
+      const int NE = t.ne, NT = t.nt;
+      if (m < t.ns * t.nr * NT * NE) {
+        const int k = m % NE, r = m / NE, j = r % NT, r2 = r / NT, i = r2 % t.nr, s = r2 / t.nr;
+        if (t.wpi[s] && i >= 1 && k >= 1) t.wavef[m] = wave_factor(t, s, i, j, k, kp ? *kp : *t.kpnow);
+      }
+      return;
+    }
     if (bs && u.b == 0)
       for (int s = threadIdx.x; s < t.ns; s += blockDim.x) t.bscale[s] = bs[s];
     if (kp && u.b == 0 && threadIdx.x == 0) *t.kpnow = *kp;  // WAVELO's KP of the step
@@ -236,15 +248,6 @@ __device__ inline void coef_ij_unit(const Launch &L, int vb, const double *kp, c
         double *vt = const_cast<double *>(t.vt);
         GEO2(vt, I, J) = VTV(I, J);
         if (I == t.nr) GEO2(vt, I + 1, J) = VTV(I + 1, J);
-      }
-      // WAVELO's factors of this (I,J) at the step's Kp, K >= 2, for the
-      // species that take it (src/ModRamWPI.f90:596-630)
-      if (I >= 2) {
-        const double kpn = kp ? *kp : *t.kpnow;
-        for (int s = 0; s < t.ns; ++s)
-          if (t.wpi[s])
-            for (int k = 1; k < t.ne; ++k)
-              t.wavef[(((size_t)s * t.nr + i) * NT + j) * t.ne + k] = wave_factor(t, s, i, j, k, kpn);
       }
     }
     if (kp) __syncthreads();  // svt is rewritten for the block's next unit
