@@ -72,7 +72,6 @@ TRAFFIC_FILE = REPO / "profiles" / "traffic_c3_ensemble.txt"
 TRAFFIC_FILE_MULTI = REPO / "profiles" / "traffic_multi_species.txt"
 # MI355X vector issue: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one
 # wave64 VALU instruction (FP64 included) per 4 cycles per SIMD
-VALU_ISSUE_PER_S = 256 * 4 * 2.4e9 / 4
 ORACLE_CFLAGS = "gcc -O2 -fPIC -fopenmp -ffp-contract=off (oracle/Makefile)"
 
 
