@@ -2,7 +2,7 @@
 
 ``Oracle`` drives oracle/libramoracle.so, the C restatement of the RAM split
 step; ``Reference`` drives oracle/_ref/libramref.so, the reference's own
-Fortran drift operators behind oracle/ref_harness.f90.  Both take the same
+Fortran drift and Coulomb operators behind oracle/ref_harness.f90.  Both take the same
 host arrays in the reference's Fortran layout.  Only tests/, the smoke check
 and the bench's CPU-baseline leg use this module.
 """
@@ -34,7 +34,12 @@ class _State(C.Structure):
          ("f2", _dp), ("dtdrift_r", _dp), ("dtdrift_p", _dp), ("dtdrift_e", _dp),
          ("dtdrift_mu", _dp), ("setrc", _dp), ("elorc", _dp), ("lsdr", _dp), ("lscha", _dp),
          ("lsatm", _dp), ("wpi", _ip), ("walos1", _dp), ("walos2", _dp), ("walos3", _dp),
-         ("lz", _dp), ("rlpp", _dp), ("kp", C.c_double), ("lswae", _dp)]
+         ("lz", _dp), ("rlpp", _dp), ("kp", C.c_double), ("lswae", _dp),
+         ("vbnd", _dp), ("necr", _dp), ("time_elapsed", C.c_double),
+         ("coule", _dp), ("couli", _dp), ("ata", _dp), ("gta", _dp), ("cedr", _dp), ("cidr", _dp),
+         ("use_coulomb", C.c_int), ("lscoe", _dp), ("lscsc", _dp)]
+
+COUL_TABLES = ("coule", "couli", "ata", "gta", "cedr", "cidr")
 
 
 class _Drift(C.Structure):
@@ -80,6 +85,8 @@ class Oracle:
             lib.oracle_atmol.argtypes = [sp, C.c_int, C.POINTER(_Loss)]
             lib.oracle_sumrc.argtypes = [sp, C.c_int]
             lib.oracle_wavelo.argtypes = [sp, C.c_int]
+            for name in ("coulpara", "coulen", "coulmu"):
+                getattr(lib, "oracle_" + name).argtypes = [sp, C.c_int]
             lib.oracle_ram_run.argtypes = [sp, C.c_int, _dp]
             lib.oracle_ram_run.restype = C.c_int
             lib.oracle_drift_alloc.argtypes = [sp, C.POINTER(_Drift)]
@@ -126,8 +133,18 @@ class Oracle:
         st.fgeos = d("fgeos", problem.fgeos)
         st.f2 = d("f2", problem.f2.copy(order="F"))
         for n in ("dtdrift_r", "dtdrift_p", "dtdrift_e", "dtdrift_mu", "setrc", "elorc",
-                  "lsdr", "lscha", "lsatm", "lswae"):
+                  "lsdr", "lscha", "lsatm", "lswae", "lscoe", "lscsc"):
             setattr(st, n, d(n, np.zeros(g.ns)))
+        vbnd = getattr(g, "vbnd", None)
+        st.vbnd = d("vbnd", np.zeros((g.ns, g.ne)) if vbnd is None else vbnd)
+        for n in COUL_TABLES:
+            setattr(st, n, d(n, np.zeros((g.ns, g.ne, g.npa))))
+        st.necr = d("necr", np.zeros((g.nr, g.nt)))
+        coul = getattr(problem, "coulomb", None)
+        if coul is not None:
+            keep["necr"][...] = coul.necr
+            st.time_elapsed = float(coul.time_elapsed)
+            st.use_coulomb = 1
         st.lz = d("lz", g.lz)
         wave = getattr(problem, "wave", None)
         if wave is not None:
@@ -151,6 +168,19 @@ class Oracle:
 
     def set_wave_kp(self, kp):
         self.st.kp = float(kp)
+
+    def set_coulomb(self, necr=None, time_elapsed=None, on=True):
+        """Coulomb collisions in ram_run (DoUseCoulomb), the plasmasphere
+        density NECR (NR,NT) and TimeRamElapsed (the COULMU clamp)."""
+        if necr is not None:
+            self._keep["necr"][...] = necr
+        if time_elapsed is not None:
+            self.st.time_elapsed = float(time_elapsed)
+        self.st.use_coulomb = 1 if on else 0
+
+    def coul_table(self, name: str) -> np.ndarray:
+        """A COULPARA table (NS,NE,NPA) as numpy (copy)."""
+        return self._keep[name].copy()
 
     # -- state access -------------------------------------------------------
     @property
@@ -199,8 +229,8 @@ class Oracle:
             getattr(lib, "oracle_" + op)(C.byref(self.st), s, C.byref(self._l(s)))
         elif op == "sumrc":
             lib.oracle_sumrc(C.byref(self.st), s)
-        elif op == "wavelo":
-            lib.oracle_wavelo(C.byref(self.st), s)
+        elif op in ("wavelo", "coulpara", "coulen", "coulmu"):
+            getattr(lib, "oracle_" + op)(C.byref(self.st), s)
         else:
             raise ValueError(op)
 
@@ -273,7 +303,8 @@ class Reference:
     """The reference's own DRIFTPARA/DRIFTR/DRIFTP/DRIFTE/DRIFTMU (Fortran)."""
 
     _lib = None
-    OPS = {"driftpara": 0, "driftr": 1, "driftp": 2, "drifte": 3, "driftmu": 4}
+    OPS = {"driftpara": 0, "driftr": 1, "driftp": 2, "drifte": 3, "driftmu": 4,
+           "coulpara": 5, "coulen": 6, "coulmu": 7}
 
     @classmethod
     def available(cls) -> bool:
@@ -292,6 +323,8 @@ class Reference:
             lib.ref_funt.restype = C.c_double
             lib.ref_funi.argtypes = [C.c_double]
             lib.ref_funi.restype = C.c_double
+            lib.ref_setup_coulomb.argtypes = [_dp, _dp, C.c_double]
+            lib.ref_get_coulomb.argtypes = [_dp] * 6
             cls._lib = lib
         return cls._lib
 
@@ -313,6 +346,17 @@ class Reference:
 
     def call(self, op: str, s: int):
         self.lib().ref_call(self.OPS[op], s)
+
+    def setup_coulomb(self, vbnd, necr, time_elapsed):
+        """VBND (NS,NE), NECR (NR,NT), TimeRamElapsed and RAMSpecies."""
+        a, b = _fd(vbnd), _fd(necr)
+        self.lib().ref_setup_coulomb(_ptr(a), _ptr(b), float(time_elapsed))
+
+    def coulomb_tables(self) -> dict:
+        ns, _, _, ne, npa = self.shape
+        out = {n: np.zeros((ns, ne, npa), order="F") for n in COUL_TABLES}
+        self.lib().ref_get_coulomb(*(_ptr(out[n]) for n in COUL_TABLES))
+        return out
 
     @property
     def f2(self) -> np.ndarray:

@@ -558,6 +558,212 @@ void oracle_wavelo(oracle_state *st, int s) {
 #undef WL
 }
 
+/* ---- Coulomb collisions, src/ModRamCoul.f90 ---------------------------- */
+
+static const double RAM_PI = 3.1415926535897932384626433832795;
+static const double RAM_MP = 1.673E-27;  /* src/ModRamConst.f90:18 */
+static const double RAM_RE = 6.371E6;    /* src/ModRamConst.f90:14 */
+
+/* FUNT and FUNI, Ejiri 1978 (src/ModRamFunctions.f90:90-143), in the
+ * reference's expression order. */
+static double coul_funt(double x) {
+  const double y = sqrt(1 - x * x);
+  const double alpha = 1. + log(2. + sqrt(3.)) / 2. / sqrt(3.);
+  const double beta = alpha / 2. - RAM_PI * sqrt(2.) / 12.;
+  const double a1 = 0.055, a2 = -0.037, a3 = -0.074, a4 = 0.056;
+  return alpha - beta * (y + sqrt(y)) + a1 * pow(y, 1. / 3.) + a2 * pow(y, 2. / 3.) + a3 * y + a4 * pow(y, 4. / 3.);
+}
+static double coul_funi(double x) {
+  double ylog = 0.0;
+  const double y = sqrt(1 - x * x);
+  if (y > 0) ylog = log(y);
+  const double alpha = 1. + log(2. + sqrt(3.)) / 2. / sqrt(3.);
+  const double beta = alpha / 2. - RAM_PI * sqrt(2.) / 12.;
+  const double a1 = 0.055, a2 = -0.037, a3 = -0.074, a4 = 0.056;
+  return 2. * alpha * (1. - y) + 2. * beta * y * ylog + 4. * beta * (y - sqrt(y)) +
+         3. * a1 * (pow(y, 1. / 3.) - y) + 6. * a2 * (pow(y, 2. / 3.) - y) + 6. * a4 * (y - pow(y, 4. / 3.)) -
+         2. * a3 * y * ylog;
+}
+/* Gcoul(x) = (erf(x) - x erf'(x)) / (2 x^2), src/ModRamFunctions.f90:72-87 */
+static double coul_g(double x) {
+  const double g1 = erf(x) - 2. * x / sqrt(RAM_PI) * exp(-x * x);
+  return g1 / 2. / x / x;
+}
+
+/* The plasmasphere's species for the Coulomb terms: RAMSpecies(1..6) of
+ * src/ModRamSpecies.f90:43-131 (mass in MP, charge, plasmasphereRatio);
+ * those with a ratio below 1e-9 are skipped. */
+static const double coul_ps_mass[6] = {5.4462E-4, 1.0, 4.0, 16.0, 14.0, 87.62};
+static const double coul_ps_charge[6] = {-1, 1, 1, 1, 1, 1};
+static const double coul_ps_ratio[6] = {1.0, 0.77, 0.2, 0.03, 0.0, 0.0};
+
+#define CT(A, S, K, L) (A)[((S)-1) + (size_t)st->ns * (((K)-1) + (size_t)st->ne * ((L)-1))]
+
+/* COULPARA(S), src/ModRamCoul.f90:17-125: COULE, COULI, CEDR, CIDR, ATA and
+ * GTA of species S.  The plasmasphere sums CCE, CDE, EDRE, CCI, CDI, EDRI
+ * start at 0 once and are not reset between energies, as in the
+ * reference, so the K-th term holds the sums over K' <= K. */
+void oracle_coulpara(oracle_state *st, int s) {
+  const int ne = st->ne, npa = st->npa;
+  double *coulde = (double *)calloc(npa, sizeof(double)), *couldi = (double *)calloc(npa, sizeof(double));
+  double cce = 0, cde = 0, edre = 0, cci = 0, cdi = 0, edri = 0;
+  const double eps = 8.854E-12, dln = 21.5;
+  const double zt = (double)st->charge[s - 1];
+  const double qe = (RAM_Q * RAM_Q / eps);
+  const double gama = zt * zt * dln / 4. / RAM_PI * qe * 1E6 * qe;
+  const double cco = gama / RAM_Q * st->dts / RAM_Q / 1E3;
+  const double rm = st->rmas[s - 1];
+  const double ccd = gama * st->dts / (rm * rm) / (RAM_CS * RAM_CS * RAM_CS);
+  const double edrco = dln * qe * RAM_RE / rm * qe * 1E9 / RAM_Q;
+  for (int k = 1; k <= ne; ++k) {
+    for (int is = 0; is < 6; ++is) {
+      const double ra = coul_ps_ratio[is];
+      if (ra < 1e-9) continue;
+      const double vf = sqrt(2. * RAM_Q / (RAM_MP * coul_ps_mass[is]));
+      const double zb = coul_ps_charge[is];
+      const double x = SK(st->vbnd, s, k) / vf, xd = SK(st->v, s, k) / vf;
+      if (zb < 0.) {
+        cce = cce + ra * coul_g(x);
+        cde = cde + ra * (erf(xd) - coul_g(xd));
+        edre = edre + ra * coul_g(xd);
+      } else {
+        cci = cci + ra * (zb * zb) * coul_g(x);
+        cdi = cdi + ra * (zb * zb) * (erf(xd) - coul_g(xd));
+        edri = edri + ra * coul_g(xd);
+      }
+    }
+    const double vb = SK(st->vbnd, s, k), gb = SK(st->grbnd, s, k), gr = SK(st->grel, s, k), v = SK(st->v, s, k);
+    CT(st->coule, s, k, 1) = -(cce * vb * cco * (gb * gb));
+    CT(st->couli, s, k, 1) = -(cci * vb * cco * (gb * gb));
+    CT(st->cedr, s, k, 1) = edrco * V1(st->ekev, k) * edre * (gr + 1) / (gr * gr) / (v * v);
+    CT(st->cidr, s, k, 1) = edrco * V1(st->ekev, k) * edri * (gr + 1) / (gr * gr) / (v * v);
+    const double ccde = ccd * cde * gr / pow(gr * gr - 1, 1.5);
+    const double ccdi = ccd * cdi * gr / pow(gr * gr - 1, 1.5);
+    for (int l = 2; l <= npa - 1; ++l) {
+      const double mu = V1(st->mu, l), wmu = V1(st->wmu, l);
+      const double bane = (1. - coul_funi(mu) / 2. / coul_funt(mu)) / (1. - mu * mu);
+      CT(st->coule, s, k, l) = CT(st->coule, s, k, 1);
+      CT(st->couli, s, k, l) = CT(st->couli, s, k, 1);
+      CT(st->cedr, s, k, l) = CT(st->cedr, s, k, 1) * bane * coul_funt(mu) * mu * wmu;
+      CT(st->cidr, s, k, l) = CT(st->cidr, s, k, 1) * bane * coul_funt(mu) * mu * wmu;
+      const double muboun = mu + 0.5 * wmu;
+      const double badif = (1. - muboun * muboun) / muboun / 2.;
+      coulde[l - 1] = ccde * badif;
+      const double afer = coulde[l - 1] / mu / V1(st->dmu, l) / wmu;
+      const double asec = coulde[l - 2] / mu / V1(st->dmu, l - 1) / wmu;
+      couldi[l - 1] = ccdi * badif;
+      const double afir = couldi[l - 1] / mu / V1(st->dmu, l) / wmu;
+      const double asic = couldi[l - 2] / mu / V1(st->dmu, l - 1) / wmu;
+      CT(st->ata, s, k, l) = afir + afer;
+      CT(st->gta, s, k, l) = asic + asec;
+    }
+    CT(st->cedr, s, k, npa) = CT(st->cedr, s, k, npa - 1);
+    CT(st->cidr, s, k, npa) = CT(st->cidr, s, k, npa - 1);
+    CT(st->ata, s, k, npa) = 0;
+  }
+  free(coulde);
+  free(couldi);
+}
+
+/* COULEN(S), src/ModRamCoul.f90:133-221: the energy sweep of the Coulomb
+ * drag, CccolE(K) = (COULE+COULI)(S,K,L) * NECR(I,J)*BANE(L) with BANE the
+ * bounce factor (1-FNIS/2/FNHS)/(1-MU^2), its top 11 rows copied from row
+ * NPA-11; ghosts F(1), F(0) and F(NE+1) = F(NE+2) = 0 as in DRIFTE; K=2..NE
+ * updated and clamped to 1e-15. */
+void oracle_coulen(oracle_state *st, int s) {
+  const int ne = st->ne, npa = st->npa;
+  double *f = (double *)calloc(ne + 3, sizeof(double)), *fbnd = (double *)calloc(ne + 1, sizeof(double));
+  double *cc = (double *)calloc(ne + 1, sizeof(double)), *bane = (double *)calloc(npa + 1, sizeof(double));
+  const double rm = st->rmas[s - 1];
+  const double ezero = V1(st->ekev, 1) - V1(st->we, 1);
+  const double grzero = 1. + ezero * 1000. * RAM_Q / rm / RAM_CS / RAM_CS;
+  const double g1 = SK(st->grel, s, 1), g2 = SK(st->grel, s, 2);
+  f[ne + 1] = 0.;
+  f[ne + 2] = 0.;
+  for (int j = 1; j <= st->nt; ++j)
+    for (int i = 2; i <= st->nr; ++i) {
+      for (int l = 2; l <= npa - 1; ++l) {
+        const double mu = V1(st->mu, l);
+        bane[l] = (1. - G3(st->fnis, i, j, l) / 2. / G3(st->fnhs, i, j, l)) / (1. - mu * mu);
+      }
+      for (int l = npa - 10; l <= npa; ++l) bane[l] = bane[l - 1];
+      for (int l = 2; l <= npa; ++l) {
+        const double xne = st->necr[(i - 1) + (size_t)st->nr * (j - 1)] * bane[l];
+        for (int k = 2; k <= ne; ++k) f[k] = F2(s, i, j, k, l);
+        f[1] = f[2] * g1 / g2 * sqrt((g1 * g1 - 1) / (g2 * g2 - 1));
+        f[0] = f[1] * grzero / g1 * sqrt((grzero * grzero - 1) / (g1 * g1 - 1));
+        for (int k = 1; k <= ne; ++k) {
+          cc[k] = (CT(st->coule, s, k, l) + CT(st->couli, s, k, l)) * xne;
+          const int isign = cc[k] < 0. ? -1 : 1;
+          const double x = f[k + 1] - f[k];
+          const double fup = 0.5 * (f[k] + f[k + 1] - isign * x);
+          if (fabs(x) <= 1.E-27) fbnd[k] = fup;
+          if (fabs(x) > 1.E-27) {
+            const int n = k + 1 - isign;
+            const double r = (f[n] - f[n - 1]) / x;
+            if (r <= 0) fbnd[k] = fup;
+            if (r > 0) {
+              const double corr = -0.5 * (cc[k] / V1(st->de, k) - isign) * x;
+              fbnd[k] = fup + limiter(r, st->betalim) * corr;
+            }
+          }
+        }
+        for (int k = 2; k <= ne; ++k) {
+          F2(s, i, j, k, l) =
+              F2(s, i, j, k, l) - cc[k] / V1(st->we, k) * fbnd[k] + cc[k - 1] / V1(st->we, k) * fbnd[k - 1];
+          if (F2(s, i, j, k, l) < 0.) F2(s, i, j, k, l) = 1E-15;
+        }
+      }
+    }
+  free(f);
+  free(fbnd);
+  free(cc);
+  free(bane);
+}
+
+/* COULMU(S), src/ModRamCoul.f90:229-296: implicit pitch-angle scattering
+ * of every (I,J,K) line, a tridiagonal solve in the reference's recurrences
+ * (RK, RL forward over L=2..NPA-1 with RK(1)=0, RL(1)=-1, then back
+ * substitution), with BASCNE = NECR*BOUNIS/2/BOUNHS; F2 is solved for
+ * F2/FNHS/MU and scaled back; negative cells are set to 1e-15 once
+ * TimeRamElapsed > 0. */
+void oracle_coulmu(oracle_state *st, int s) {
+  const int npa = st->npa;
+  double *rk = (double *)calloc(npa + 1, sizeof(double)), *rl = (double *)calloc(npa + 1, sizeof(double));
+  double *bascne = (double *)calloc(npa + 1, sizeof(double));
+  for (int j = 1; j <= st->nt; ++j)
+    for (int i = 2; i <= st->nr; ++i) {
+      const double xne = st->necr[(i - 1) + (size_t)st->nr * (j - 1)];
+      for (int k = 2; k <= st->ne; ++k) {
+        rk[1] = 0.;
+        rl[1] = -1.;
+        bascne[1] = xne * G3(st->bounis, i, j, 1) / 2. / G3(st->bounhs, i, j, 1);
+        for (int l = 2; l <= npa - 1; ++l) {
+          bascne[l] = xne * G3(st->bounis, i, j, l) / 2. / G3(st->bounhs, i, j, l);
+          const double an = CT(st->ata, s, k, l) * bascne[l] / G3(st->fnhs, i, j, l) * G3(st->bounhs, i, j, l);
+          const double gn =
+              CT(st->gta, s, k, l) * bascne[l - 1] / G3(st->fnhs, i, j, l) * G3(st->bounhs, i, j, l - 1);
+          const double bn = an + gn;
+          const double rp = F2(s, i, j, k, l) / G3(st->fnhs, i, j, l) / V1(st->mu, l);
+          const double denom = bn + gn * rl[l - 1] + 1;
+          rk[l] = (rp + gn * rk[l - 1]) / denom;
+          rl[l] = -an / denom;
+        }
+        F2(s, i, j, k, npa - 1) = rk[npa - 1] / (1 + rl[npa - 1]);
+        for (int l = npa - 2; l >= 1; --l) F2(s, i, j, k, l) = rk[l] - rl[l] * F2(s, i, j, k, l + 1);
+        F2(s, i, j, k, npa) = F2(s, i, j, k, npa - 1);
+        for (int l = 1; l <= npa; ++l) {
+          F2(s, i, j, k, l) = F2(s, i, j, k, l) * G3(st->fnhs, i, j, l) * V1(st->mu, l);
+          if (st->time_elapsed > 0. && F2(s, i, j, k, l) < 0.) F2(s, i, j, k, l) = 1E-15;
+        }
+      }
+    }
+  free(rk);
+  free(rl);
+  free(bascne);
+}
+#undef CT
+
 /* SUMRC, src/ModRamRun.f90:231-259 (sequential sum in the I,K,L,J order). */
 void oracle_sumrc(oracle_state *st, int s) {
   st->elorc[s - 1] = 0.0;
@@ -652,8 +858,8 @@ void oracle_flux(const oracle_state *st, const double *ffactor, double *flux) {
 #undef FF
 #undef SIJ
 
-/* The per-species body of ram_run, src/ModRamRun.f90:66-184, optional
- * processes off. */
+/* The per-species body of ram_run, src/ModRamRun.f90:66-184: WAVELO and
+ * Coulomb collisions as switched, EMIC, FLC and WPADIF off. */
 static int ram_run_species(oracle_state *st, int s) {
   oracle_drift_scratch d;
   oracle_loss_scratch ls;
@@ -664,14 +870,24 @@ static int ram_run_species(oracle_state *st, int s) {
   }
   const int cex = st->cex_kind[s - 1] != ORACLE_CEX_NONE;
   const int wpi = st->wpi && st->wpi[s - 1];
+  const int coul = st->use_coulomb;
   oracle_cepara(st, s, &ls);
   oracle_driftpara(st, s, &d);
+  if (coul) oracle_coulpara(st, s);
   oracle_driftr(st, s, &d);
   oracle_driftp(st, s, &d);
   oracle_drifte(st, s, &d);
   oracle_driftmu(st, s, &d);
   oracle_sumrc(st, s);
   st->lsdr[s - 1] += st->elorc[s - 1];
+  if (coul) {
+    oracle_coulen(st, s);
+    oracle_sumrc(st, s);
+    st->lscoe[s - 1] += st->elorc[s - 1];
+    oracle_coulmu(st, s);
+    oracle_sumrc(st, s);
+    st->lscsc[s - 1] += st->elorc[s - 1];
+  }
   if (wpi) {
     oracle_wavelo(st, s);
     oracle_sumrc(st, s);
@@ -697,6 +913,14 @@ static int ram_run_species(oracle_state *st, int s) {
     oracle_wavelo(st, s);
     oracle_sumrc(st, s);
     st->lswae[s - 1] += st->elorc[s - 1];
+  }
+  if (coul) {
+    oracle_coulmu(st, s);
+    oracle_sumrc(st, s);
+    st->lscsc[s - 1] += st->elorc[s - 1];
+    oracle_coulen(st, s);
+    oracle_sumrc(st, s);
+    st->lscoe[s - 1] += st->elorc[s - 1];
   }
   oracle_driftmu(st, s, &d);
   oracle_drifte(st, s, &d);

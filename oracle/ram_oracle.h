@@ -59,6 +59,15 @@ typedef struct {
   const double *rlpp;                      /* (NT) plasmapause L of WAVELO  */
   double kp;                               /* KP                            */
   double *lswae;                           /* (NS) LSWAE                    */
+  /* Coulomb collisions (src/ModRamCoul.f90), in ram_run when use_coulomb
+   * (DoUseCoulomb) is set: VBND (NS,NE), the plasmasphere density NECR
+   * (NR,NT), TimeRamElapsed, the tables COULPARA writes, COULE, COULI, ATA,
+   * GTA, CEDR, CIDR (NS,NE,NPA), and the accumulators LSCOE, LSCSC (NS) */
+  const double *vbnd, *necr;
+  double time_elapsed;
+  double *coule, *couli, *ata, *gta, *cedr, *cidr;
+  int use_coulomb;
+  double *lscoe, *lscsc;
 } oracle_state;
 
 /* Per-species drift tables built by DRIFTPARA (src/ModRamDrift.f90:15-19). */
@@ -94,9 +103,17 @@ void oracle_sumrc(oracle_state *st, int s);
 /* WAVELO(S), src/ModRamWPI.f90:580-636, with the plasmapause RLpp(J) of
  * :596-603 given in st->rlpp. */
 void oracle_wavelo(oracle_state *st, int s);
+/* COULPARA(S), COULEN(S), COULMU(S), src/ModRamCoul.f90:17-125,133-221,
+ * 229-296. */
+void oracle_coulpara(oracle_state *st, int s);
+void oracle_coulen(oracle_state *st, int s);
+void oracle_coulmu(oracle_state *st, int s);
 
 /* One ram_run call (src/ModRamRun.f90:64-205) with the optional processes
- * Coulomb, EMIC, FLC and WPADIF off; WAVELO for species with wpi set.  nthreads>1 runs the species
+ * EMIC, FLC and WPADIF off; WAVELO for species with wpi set, Coulomb
+ * collisions with use_coulomb (their tables from COULPARA, and the
+ * COULMU clamp of TimeRamElapsed > 0 from st->time_elapsed, which this call
+ * does not advance).  nthreads>1 runs the species
  * loop with OpenMP as the reference does (src/ModRamRun.f90:64).  Returns
  * DtsNext before the DtsMin floor in *dts_next (may be NULL). */
 int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next);

@@ -1,8 +1,9 @@
 !  ref_harness.f90 -- TEST INFRASTRUCTURE ONLY.
 !
 !  A C-callable harness around the reference's own, unmodified drift
-!  operators (src/ModRamDrift.f90) and dipole bounce functions
-!  (src/ModRamFunctions.f90:90-143), compiled from the reference sources by
+!  operators (src/ModRamDrift.f90), Coulomb operators (src/ModRamCoul.f90)
+!  and dipole bounce functions (src/ModRamFunctions.f90:90-143), compiled
+!  from the reference sources by
 !  oracle/Makefile into oracle/_ref/.  The harness owns nothing but the copy
 !  in/out of the reference's module state; it fills that state the way
 !  ram_allocate does (src/ModRamInit.f90:67-138) and then calls the reference
@@ -15,10 +16,13 @@ module ram_ref_harness
                              BNES, dBdt, VT, EIR, EIP, outsideMGNP, FGEOS, RLZ, &
                              EKEV, EBND, WE, DE, GREL, GRBND, V, RMAS, MU, WMU, &
                              DMU, UPA, species, DtDriftR, DtDriftP, DtDriftE, &
-                             DtDriftMu, MDR, CONF1, CONF2, FracCFL
-  use ModRamTiming,    only: DTs
+                             DtDriftMu, MDR, CONF1, CONF2, FracCFL, VBND, &
+                             COULE, COULI, ATA, GTA, GTAE, GTAI, CEDR, CIDR, NECR, XNE
+  use ModRamTiming,    only: DTs, TimeRamElapsed
   use ModRamParams,    only: BetaLim
+  use ModRamSpecies,   only: DefineSpecies
   use ModRamDrift,     only: DRIFTPARA, DRIFTR, DRIFTP, DRIFTE, DRIFTMU, DRIFTEND
+  use ModRamCoul,      only: COULPARA, COULEN, COULMU
   use ModRamFunctions, only: funt, funi
   implicit none
   logical, save :: para_live = .false.
@@ -33,6 +37,8 @@ contains
          dIbndt, BNES, dBdt, VT, EIR, EIP, outsideMGNP, FGEOS, RLZ, EKEV, EBND, &
          WE, DE, GREL, GRBND, V, RMAS, MU, WMU, DMU, UPA, species, DtDriftR, &
          DtDriftP, DtDriftE, DtDriftMu)
+    if (allocated(COULE)) deallocate(VBND, COULE, COULI, ATA, GTA, GTAE, GTAI, &
+         CEDR, CIDR, NECR, XNE)
   end subroutine release
 
   subroutine ref_setup(ns_, nr_, nt_, ne_, npa_, dts_, betalim_, fraccfl_, mdr_, &
@@ -77,6 +83,30 @@ contains
     DtDriftR = 0; DtDriftP = 0; DtDriftE = 0; DtDriftMu = 0
   end subroutine ref_setup
 
+  ! the Coulomb operators' state, zeroed as ram_allocate does
+  ! (src/ModRamInit.f90:97-151): VBND(NS,NE), NECR(NR,NT), the elapsed time
+  subroutine ref_setup_coulomb(vbnd_, necr_, time_) bind(C, name="ref_setup_coulomb")
+    real(c_double) :: vbnd_(nS, nE), necr_(nR, nT)
+    real(c_double), value :: time_
+    if (allocated(COULE)) deallocate(VBND, COULE, COULI, ATA, GTA, GTAE, GTAI, &
+         CEDR, CIDR, NECR, XNE)
+    allocate(VBND(nS,nE), COULE(nS,nE,nPa), COULI(nS,nE,nPa), ATA(nS,nE,nPa), &
+         GTA(nS,nE,nPa), GTAE(nS,nE,nPa), GTAI(nS,nE,nPa), CEDR(nS,nE,nPa), &
+         CIDR(nS,nE,nPa), NECR(nR,nT), XNE(nR,nT))
+    COULE = 0.0; COULI = 0.0; ATA = 0.0; GTA = 0.0; GTAE = 0.0; GTAI = 0.0
+    CEDR = 0.0; CIDR = 0.0; XNE = 0.0
+    VBND = vbnd_; NECR = necr_; TimeRamElapsed = time_
+    call DefineSpecies
+  end subroutine ref_setup_coulomb
+
+  ! COULE, COULI, ATA, GTA, CEDR, CIDR (NS,NE,NPA) after COULPARA
+  subroutine ref_get_coulomb(coule_, couli_, ata_, gta_, cedr_, cidr_) &
+       bind(C, name="ref_get_coulomb")
+    real(c_double) :: coule_(nS,nE,nPa), couli_(nS,nE,nPa), ata_(nS,nE,nPa), &
+         gta_(nS,nE,nPa), cedr_(nS,nE,nPa), cidr_(nS,nE,nPa)
+    coule_ = COULE; couli_ = COULI; ata_ = ATA; gta_ = GTA; cedr_ = CEDR; cidr_ = CIDR
+  end subroutine ref_get_coulomb
+
   subroutine ref_put_f2(f2_) bind(C, name="ref_put_f2")
     real(c_double) :: f2_(nS, nR, nT, nE, nPA)
     F2 = f2_
@@ -93,7 +123,8 @@ contains
     out(:, 3) = DtDriftE; out(:, 4) = DtDriftMu
   end subroutine ref_get_dtdrift
 
-  ! op: 0 DRIFTPARA, 1 DRIFTR, 2 DRIFTP, 3 DRIFTE, 4 DRIFTMU
+  ! op: 0 DRIFTPARA, 1 DRIFTR, 2 DRIFTP, 3 DRIFTE, 4 DRIFTMU, 5 COULPARA,
+  ! 6 COULEN, 7 COULMU
   subroutine ref_call(op, s) bind(C, name="ref_call")
     integer(c_int), value :: op, s
     select case (op)
@@ -108,6 +139,12 @@ contains
        call DRIFTE(s)
     case (4)
        call DRIFTMU(s)
+    case (5)
+       call COULPARA(s)
+    case (6)
+       call COULEN(s)
+    case (7)
+       call COULMU(s)
     end select
   end subroutine ref_call
 

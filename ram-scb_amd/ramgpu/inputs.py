@@ -204,6 +204,36 @@ def wave_inputs(grid: RamGrid, kp: float, kpmax12: float | None = None, wpi=None
 
 
 @dataclass
+class CoulombInputs:
+    """Inputs of the Coulomb operators (src/ModRamCoul.f90), in ram_run
+    with DoUseCoulomb: the plasmasphere electron density NECR(NR,NT) [cm^-3]
+    and the elapsed time TimeRamElapsed [s] (COULMU clamps negative cells
+    only when it is positive)."""
+    necr: np.ndarray      # (NR,NT)
+    time_elapsed: float
+
+
+def plasmasphere_density(grid: RamGrid, kp: float) -> np.ndarray:
+    """Synthetic NECR(NR,NT) [cm^-3] shaped like the reference's Carpenter
+    model (src/ModRamPlasmasphere.f90:286-299), which is not part of this
+    library: the saturated plasmasphere 10**(-0.3145 L + 3.9043) inside the
+    plasmapause L = 5.6 - 0.46 Kp (bulging by 0.8 at dusk), the trough
+    n_pp (L/L_pp)**-4.5 + 1 - exp((2-L)/10) outside it."""
+    necr = np.zeros((grid.nr, grid.nt))
+    for j in range(grid.nt):
+        mlt = 24.0 * j / (grid.nt - 1)
+        lpp = 5.6 - 0.46 * kp + 0.8 * math.exp(-((mlt - 18.0) / 3.0) ** 2)
+        npp = 10.0 ** (-0.3145 * lpp + 3.9043)
+        for i in range(grid.nr):
+            lv = grid.lz[i]
+            if lv <= lpp:
+                necr[i, j] = 10.0 ** (-0.3145 * lv + 3.9043)
+            else:
+                necr[i, j] = npp * (lv / lpp) ** -4.5 + 1.0 - math.exp((2.0 - lv) / 10.0)
+    return necr
+
+
+@dataclass
 class Problem:
     """One member's full input set for the split step."""
     grid: RamGrid
@@ -217,6 +247,7 @@ class Problem:
     betalim: float = 1.5
     fraccfl: float = 0.8
     wave: WaveInputs | None = None   # WAVELO inputs; None: no species takes it
+    coulomb: CoulombInputs | None = None  # Coulomb inputs; None: DoUseCoulomb off
 
 
 SPECIES4 = (sp.ELECTRON, sp.HYDROGEN, sp.HELIUM, sp.OXYGEN)
@@ -224,16 +255,21 @@ COMPOSITION4 = (1.0, 0.7, 0.05, 0.25)
 
 
 def make_problem(nr=20, nt=25, ne=35, npa=72, species=(sp.HYDROGEN,), composition=(1.0,),
-                 kp=3.0, seed=0, boundary_scale=1.0, dts=5.0, wave=False) -> Problem:
+                 kp=3.0, seed=0, boundary_scale=1.0, dts=5.0, wave=False, coulomb=False,
+                 time_elapsed=0.0) -> Problem:
     """wave: WAVELO for the species whose species%WPI is set (the electron
-    species, as the reference does by default)."""
+    species, as the reference does by default); coulomb: Coulomb collisions
+    on, with the synthetic plasmasphere of Kp and TimeRamElapsed =
+    time_elapsed."""
     grid = build_grid(nr, nt, ne, npa, species)
     geom = dipole_geometry(grid)
     rng = np.random.default_rng(seed)
     f2 = initial_f2(grid, geom, composition, rng)
     fgeos = boundary_fgeos(grid, composition, boundary_scale)
     w = wave_inputs(grid, kp) if wave else None
-    return Problem(grid, geom, volland_stern(grid, kp), fgeos, f2, kp, tuple(composition), dts, wave=w)
+    c = CoulombInputs(plasmasphere_density(grid, kp), float(time_elapsed)) if coulomb else None
+    return Problem(grid, geom, volland_stern(grid, kp), fgeos, f2, kp, tuple(composition), dts, wave=w,
+                   coulomb=c)
 
 
 def plain_h(seed=0) -> Problem:

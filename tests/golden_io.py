@@ -6,11 +6,16 @@ from types import SimpleNamespace
 import numpy as np
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
+COULOMB_TABLES = ("coule", "couli", "ata", "gta", "cedr", "cidr")
 SEQUENCE = ["driftr", "driftp", "drifte", "driftmu", "driftmu", "drifte", "driftp", "driftr"]
 
 
 def fixtures():
     return sorted(GOLDEN.glob("*.npz"))
+
+
+def coulomb_fixtures():
+    return sorted((GOLDEN / "coulomb").glob("*.npz"))
 
 
 def load(path):
@@ -33,5 +38,10 @@ def load(path):
     problem = SimpleNamespace(grid=grid, geom=geom, vt=np.asfortranarray(z["vt"]),
                               fgeos=np.asfortranarray(z["fgeos"]), f2=np.asfortranarray(z["f2"]),
                               dts=dts, betalim=betalim, fraccfl=fraccfl)
-    expected = {k: z[k] for k in z.files if k.startswith("f2_s") or k.startswith("dtdrift_s")}
+    if "vbnd" in z.files:  # a Coulomb fixture
+        grid.vbnd = z["vbnd"]
+        problem.coulomb = SimpleNamespace(necr=np.asfortranarray(z["necr"]),
+                                          time_elapsed=float(z["time_elapsed"]))
+    expected = {k: z[k] for k in z.files if k.startswith("f2_s") or k.startswith("dtdrift_s")
+                or k.split("_s")[0] in COULOMB_TABLES}
     return problem, expected
