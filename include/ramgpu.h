@@ -170,6 +170,40 @@ int ramgpu_wavelo(ramgpu_ctx *ctx, int s);
  * elorc = old - new.  Either may be NULL. */
 int ramgpu_sumrc(ramgpu_ctx *ctx, int s, double *setrc, double *elorc);
 
+/* ---- Coulomb collisions (src/ModRamCoul.f90) ---------------------------- */
+
+/* DoUseCoulomb (src/ModRamParams.f90:110, read by IM_set_parameters.f90:95):
+ * use = 1 switches COULEN and COULMU into ramgpu_ram_run's species loop as
+ * ram_run calls them (src/ModRamRun.f90:81-89,160-167, with LSCOE and LSCSC
+ * in ramgpu_species_out), with vbnd = VBND(NS,NE), the speed at the energy
+ * bin boundaries (src/ModRamInit.f90:448,457); use = 0 switches them off
+ * (vbnd may be NULL).  Needs NPA >= 13 (COULEN copies BANE(NPA-11) into
+ * rows NPA-10..NPA).  The reverse-mode entry points and ensembles refuse a
+ * context with Coulomb collisions on (RAMGPU_ESTATE). */
+int ramgpu_set_coulomb(ramgpu_ctx *ctx, int use, const double *vbnd);
+/* The plasmaspheric electron density NECR(NR,NT) [cm^-3] the Coulomb
+ * operators read (src/ModRamVariables.f90, filled by the reference's
+ * plasmasphere models, src/ModRamPlasmasphere.f90:90,294-299); required
+ * before a step with Coulomb collisions. */
+int ramgpu_set_necr(ramgpu_ctx *ctx, const double *necr);
+/* TimeRamElapsed [s] (src/ModRamTiming.f90:19; 0 after create): COULMU sets
+ * negative cells to 1e-15 only when it is positive (src/ModRamCoul.f90:286).
+ * Each step of ramgpu_ram_run and of the schedules advances it by 2*DTs, as
+ * src/ModRamScbRun.f90:88 does after every ram_run call. */
+int ramgpu_set_time(ramgpu_ctx *ctx, double time_elapsed);
+int ramgpu_get_time(ramgpu_ctx *ctx, double *time_elapsed);
+/* COULPARA(S), src/ModRamCoul.f90:17-125: builds the Coulomb tables (for
+ * every species; cached until the grid, a species, VBND or DTs changes). */
+int ramgpu_coulpara(ramgpu_ctx *ctx, int s);
+/* The tables of species s as COULPARA leaves them: COULE(S,:,:),
+ * COULI, ATA, GTA, CEDR, CIDR, each (NE,NPA); any pointer may be NULL. */
+int ramgpu_get_coulomb_tables(ramgpu_ctx *ctx, int s, double *coule, double *couli, double *ata,
+                              double *gta, double *cedr, double *cidr);
+/* COULEN(S), src/ModRamCoul.f90:133-221, and COULMU(S), :229-296, on the
+ * device-resident F2 (Coulomb collisions must be on). */
+int ramgpu_coulen(ramgpu_ctx *ctx, int s);
+int ramgpu_coulmu(ramgpu_ctx *ctx, int s);
+
 /* ---- moments of the device-resident F2 ---------------------------------- */
 
 /* The moment factors of ARRAYS (src/ModRamInit.f90:560-576): FFACTOR
@@ -197,15 +231,16 @@ int ramgpu_moments(ramgpu_ctx *ctx, double *ppert, double *ppart, double *edent,
 /* ---- the fused split step ----------------------------------------------- */
 
 /* `nsteps` consecutive ram_run calls (src/ModRamRun.f90:64-205): per species
- * R,P,E,Mu drifts, [WAVELO], charge exchange, atmospheric loss twice, charge
- * exchange, [WAVELO], Mu,E,P,R drifts with the SUMRC bookkeeping, then the
- * periodic local-time copy and the magnetopause mask.  WAVELO only for the
- * species given its tables (ramgpu_set_wavelo).  The other optional
- * processes of ram_run -- Coulomb collisions (DoUseCoulomb), WPADIF
- * (DoUseWPI, and EMIC with DoUseEMIC) and FLC scattering (DoUseFLC) -- are not
- * part of this step: a host that switches any of them on must not replace
- * ram_run's species loop with this call.  out (may be NULL) receives NS
- * records for the last step. */
+ * R,P,E,Mu drifts, [COULEN, COULMU], [WAVELO], charge exchange, atmospheric
+ * loss twice, charge exchange, [WAVELO], [COULMU, COULEN], Mu,E,P,R drifts
+ * with the SUMRC bookkeeping, then the periodic local-time copy and the
+ * magnetopause mask.  WAVELO only for the species given its tables
+ * (ramgpu_set_wavelo), the Coulomb operators with ramgpu_set_coulomb.  The
+ * other optional processes of ram_run -- WPADIF (DoUseWPI, and EMIC with
+ * DoUseEMIC) and FLC scattering (DoUseFLC) -- are not part of this step: a
+ * host that switches any of them on must not replace ram_run's species loop
+ * with this call.  out (may be NULL) receives NS records for the last
+ * step. */
 int ramgpu_ram_run(ramgpu_ctx *ctx, int nsteps, ramgpu_species_out *out);
 
 /* The same with a per-step Kp and boundary scale schedule held on the host:

@@ -124,6 +124,8 @@ int ready(ramgpu_ctx *c) {
     if (c->t.wpi[s] && !c->have_wave)
       return fail(RAMGPU_ESTATE, "species %d takes WAVELO: ramgpu_set_wave_params has not been called", s + 1);
   }
+  if (c->t.coul && !c->have_necr)
+    return fail(RAMGPU_ESTATE, "Coulomb collisions are on: ramgpu_set_necr has not been called");
   return RAMGPU_OK;
 }
 
@@ -199,7 +201,7 @@ Launch launch_of(const ramgpu_ctx *c, int s0, int nsp, int nb) {
 // DRIFTR ghost carry when their inputs changed.
 int prepare(ramgpu_ctx *c) {
   int rc = ready(c);
-  if (rc) return rc;
+  if (rc || (c->t.coul && (rc = coul_tables(c)))) return rc;
   Tab &t = c->t;
   // a pending finalize goes with the step's k_coef_ij, or alone before any
   // table changes
@@ -379,6 +381,9 @@ void launch_plane_t(int seq, int threads, dim3 grid, size_t lds, hipStream_t st,
     case SEQ_LOSS: hipLaunchKernelGGL((k_plane<SEQ_LOSS, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_TAPE: hipLaunchKernelGGL((k_plane<SEQ_TAPE, MAXC>), grid, blk, lds, st, L, rec); break;
     case SEQ_WAVE: hipLaunchKernelGGL((k_plane<SEQ_WAVE, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_COULEN: hipLaunchKernelGGL((k_plane<SEQ_COULEN, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_COULMU: hipLaunchKernelGGL((k_plane<SEQ_COULMU, MAXC>), grid, blk, lds, st, L, rec); break;
+    case SEQ_STEPC: hipLaunchKernelGGL((k_plane<SEQ_STEPC, MAXC>), grid, blk, lds, st, L, rec); break;
     default: hipLaunchKernelGGL((k_plane<SEQ_STEP, MAXC>), grid, blk, lds, st, L, rec); break;
   }
 }
@@ -397,7 +402,8 @@ int run_plane(ramgpu_ctx *c, int seq, int rec, int s0, int nsp, const double *sr
   L.src = src;
   L.dst = dst;
   L.ilv = 1;  // a plane's species on one XCD
-  const size_t lds = plane_fwd_lds_bytes(c->ne, c->npa, c->plane.threads);
+  const size_t lds = seq_coul(seq) ? plane_coul_lds_bytes(c->ne, c->npa, c->plane.threads)
+                                    : plane_fwd_lds_bytes(c->ne, c->npa, c->plane.threads);
   tbeg(c, RAMGPU_KERNEL_PLANE);
   launch_plane(seq, c->plane, dim3(L.nv), lds, c->stream, L, rec);
   tend(c);
@@ -461,6 +467,7 @@ int set_plane_lds_limit(size_t limit) {
   const void *fns[] = {
       PLANE_K(SEQ_E), PLANE_K(SEQ_MU), PLANE_K(SEQ_CEX), PLANE_K(SEQ_ATM), PLANE_K(SEQ_SUM),
       PLANE_K(SEQ_STEP), PLANE_K(SEQ_LOSS), PLANE_K(SEQ_TAPE), PLANE_K(SEQ_WAVE),
+      PLANE_K(SEQ_COULEN), PLANE_K(SEQ_COULMU), PLANE_K(SEQ_STEPC),
       (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
       (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>, (const void *)k_p<0>, (const void *)k_p<1>,
       (const void *)k_p_bwd};
@@ -519,8 +526,12 @@ int ramgpu_host::step_body(ramgpu_ctx *c, int *cflmask) {
   if ((rc = prepare(c))) return rc;
   const int rec = c->cfl_valid ? 0 : 1;
   const int ns = c->ns;
+  // with Coulomb collisions the plane kernel also runs COULEN and COULMU,
+  // flag bit 1 being COULMU's clamp (TimeRamElapsed > 0)
+  const int seq = c->t.coul ? SEQ_STEPC : SEQ_STEP;
+  const int flags = rec | ((c->t.coul && c->time_elapsed > 0.) ? 2 : 0);
   if ((rc = run_r(c, rec, 0, ns)) || (rc = run_p(c, rec, 0, ns)) ||
-      (rc = run_plane(c, SEQ_STEP, rec, 0, ns)) || (rc = run_p(c, 0, 0, ns)) || (rc = run_r(c, 2, 0, ns)))
+      (rc = run_plane(c, seq, flags, 0, ns)) || (rc = run_p(c, 0, 0, ns)) || (rc = run_r(c, 2, 0, ns)))
     return rc;
   *cflmask = rec ? 15 : 0;
   c->cfl_valid = true;
@@ -665,6 +676,7 @@ int ramgpu_set_species(ramgpu_ctx *c, int s, int charge, int cex_kind, double rm
   c->t.cex[s - 1] = cex_kind;
   c->t.rmas[s - 1] = rmas;
   c->have_species[s - 1] = true;
+  c->coul_dirty = true;
   c->tables_dirty = true;
   c->tab_dirty = true;
   return RAMGPU_OK;
@@ -677,6 +689,7 @@ int ramgpu_set_params(ramgpu_ctx *c, double dts, double betalim, double fraccfl)
   c->t.dts = dts;
   c->t.betalim = betalim;
   c->t.fraccfl = fraccfl;
+  c->coul_dirty = true;  // COULPARA's tables scale with DTs
   c->tables_dirty = true;
   c->tab_dirty = true;
   return RAMGPU_OK;
@@ -686,7 +699,9 @@ int ramgpu_set_plane_shape(ramgpu_ctx *c, int threads, int maxc) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (threads == 0 && maxc == 0) {  // back to the model's choice
-    c->plane = plane_shape(c->ne, c->npa, c->lds_limit);
+    const PlaneShape sh = plane_shape(c->ne, c->npa, c->lds_limit);
+    if (c->t.coul && (rc = coul_check_lds(c, sh.threads))) return rc;
+    c->plane = sh;
   } else {
     if (threads < 64 || threads % 64 || threads > kPlaneMaxThreads || (maxc != 8 && maxc != 16))
       return fail(RAMGPU_EINVAL, "plane shape %d threads, %d rows: not a built kernel shape", threads, maxc);
@@ -696,6 +711,7 @@ int ramgpu_set_plane_shape(ramgpu_ctx *c, int threads, int maxc) {
     const int rows = sh.rows;
     if (plane_fwd_lds_bytes(c->ne, c->npa, threads) > c->lds_limit)
       return fail(RAMGPU_EINVAL, "plane shape %d threads: the plane's LDS exceeds the device's", threads);
+    if (c->t.coul && (rc = coul_check_lds(c, threads))) return rc;
     c->plane = {threads, maxc, rows};
   }
   c->t.plane_chunks = plane_chunks_for(c->ne, c->npa, c->plane.threads);
@@ -735,6 +751,15 @@ int ramgpu_set_grid(ramgpu_ctx *c, double mdr, double dphi, double conf1, double
   c->t.conf1 = conf1;
   c->t.conf2 = conf2;
   c->mom_dirty = true;  // the moments' WMU/FFACTOR table
+  c->h_ekev.assign(ekev, ekev + ne);
+  c->h_we.assign(we, we + ne);
+  c->h_grel.assign(grel, grel + ns * ne);
+  c->h_grbnd.assign(grbnd, grbnd + ns * ne);
+  c->h_v.assign(v, v + ns * ne);
+  c->h_mu.assign(mu, mu + npa);
+  c->h_wmu.assign(wmu, wmu + npa);
+  c->h_dmu.assign(dmu, dmu + npa);
+  c->coul_dirty = true;
   c->have_grid = true;
   c->tables_dirty = true;
   c->tab_dirty = true;
@@ -1013,8 +1038,10 @@ int ramgpu_ram_run(ramgpu_ctx *c, int nsteps, ramgpu_species_out *out) {
   if (rc || (rc = ready(c))) return rc;
   if (nsteps < 0) return fail(RAMGPU_EINVAL, "nsteps < 0");
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
-  for (int n = 0; n < nsteps; ++n)
+  for (int n = 0; n < nsteps; ++n) {
     if ((rc = step_once(c))) return rc;
+    c->time_elapsed += 2.0 * c->t.dts;  // src/ModRamScbRun.f90:88, after each ram_run
+  }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));
   return fill_out(c, out);
 }
@@ -1045,6 +1072,7 @@ int run_schedule_staged(ramgpu_ctx *c, int nsteps, const double *d_kp, const dou
     if ((rc = step_body(c, &cflmask))) break;
     c->fin_mask = cflmask;
     c->fin_mode = 2;
+    c->time_elapsed += 2.0 * c->t.dts;  // src/ModRamScbRun.f90:88, after each ram_run
   }
   const int rf = flush_finalize(c);
   if (rc == RAMGPU_OK) rc = rf;

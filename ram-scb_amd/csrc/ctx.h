@@ -98,6 +98,13 @@ struct ramgpu_ctx {
   size_t tused = 0;
   bool bwd_lds_set = false;  // LDS attribute of the plane adjoint kernels set
   size_t head_static = 0;    // static LDS of k_step_head (0: unknown, q1 row not staged)
+  // Coulomb collisions: host copies of the grid that COULPARA reads (kept by
+  // ramgpu_set_grid), VBND, the tables of the last COULPARA
+  // ([table][s][l*NE + k]), NECR on the device, TimeRamElapsed
+  std::vector<double> h_ekev, h_we, h_grel, h_grbnd, h_v, h_mu, h_wmu, h_dmu, h_vbnd, h_coul;
+  bool coul_dirty = true, have_necr = false;
+  double *d_necr = nullptr;
+  double time_elapsed = 0.0;
 };
 
 namespace ramgpu_host {
@@ -146,6 +153,13 @@ int plane_bwd_threads(int ne, int npa);
 void set_plane_bwd_lds_limit(size_t limit);
 void launch_plane_bwd(int seq, int threads, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L,
                       const double *x, double *adj);
+// Coulomb collisions: the COULPARA tables (formed when stale), and whether
+// the plane kernels' LDS with them fits blocks of `threads`
+int coul_tables(ramgpu_ctx *c);
+int coul_check_lds(const ramgpu_ctx *c, int threads);
+// RAMGPU_ESTATE for a context with Coulomb collisions on: the reverse mode
+// and ensembles step without them
+int no_coulomb(const ramgpu_ctx *c);
 // host Fortran-layout F2 <-> device-layout buffer, through the context's stage
 int upload_f2(ramgpu_ctx *c, double *dst, const double *host);
 int download_f2(ramgpu_ctx *c, double *host, const double *src);

@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(kLineThreads, 4) k_r(Launch L) {
     if (MODE == 2) {
       double a[1] = {acc}, tot[1];
       block_sum_n<1>(a, red, tot);
-      if (threadIdx.x == 0) t.partials[((size_t)(kSumSlots - 1) * t.ns + s) * t.maxblk + u.b] = tot[0];
+      if (threadIdx.x == 0) t.partials[((size_t)kSumRadial * t.ns + s) * t.maxblk + u.b] = tot[0];
     }
     __syncthreads();  // the face terms of this block are read
   }
@@ -761,6 +761,7 @@ struct PlaneFwd {
   double *X;   // T: terms handed between chunks (DRIFTMU) and waves (DRIFTE)
   double *cf, *cm, *cl, *kt, *lt, *red, *ex, *pc;
   unsigned long long *cmx;
+  double *cx;  // Coulomb kernels only: 5*NPA per-L terms, then COULMU's RL (NE*NPA)
 };
 
 __device__ __forceinline__ PlaneFwd plane_fwd_lds(double *sm, int ne, int npa, int threads) {
@@ -777,6 +778,7 @@ __device__ __forceinline__ PlaneFwd plane_fwd_lds(double *sm, int ne, int npa, i
   d.ex = d.red + 128;         // 64: 2^(j/64) of exp_tab
   d.pc = d.ex + 64;           // PC_N: the plane's scalars (PcField)
   d.cmx = (unsigned long long *)(d.pc + PC_N);  // NE + kMuSpread*NPA
+  d.cx = (double *)(d.cmx + ne + kMuSpread * npa);
   return d;
 }
 static_assert(KT_N == 11 && LT_N == 4 && PC_N == 24, "plane_fwd_lds_bytes counts these tables");
@@ -899,11 +901,17 @@ constexpr int kPassRows = 8;
 // maximum of its cells is one value.  The per-L terms are in d.cf.  Rows
 // are independent, so a pass rewrites its rows while the next pass reads
 // others.
-template <int MAXC>
+//
+// COUL: COULEN instead (src/ModRamCoul.f90:154-217), the same limited
+// upwind sweep with CccolE(K) = (COULE+COULI)(S,K,L)*XNE(L), its own ghost
+// factors (the square roots of the inverse ratios of DRIFTE's) and rows
+// L=2..NPA only.
+template <int MAXC, bool COUL = false>
 __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec) {
   const int NE = c.NE, NPA = c.NPA, S = c.S;
+  if (COUL) rec = false;
   {
-    const double *rel = d.pc + PC_REL;
+    const double *rel = COUL ? t.crel + 5 * c.s : d.pc + PC_REL;
     for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
       double *row = d.F + l * S;
       const double g1 = row[2] * rel[0] / rel[1] * rel[2];
@@ -940,8 +948,9 @@ __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool r
       if (p0 + n >= c.nr) break;
       const int l = l0 + p0 + n;
       const double *p = d.F + l * S + k;  // positions K-1, K, K+1, K+2
-      const double ce = c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz);
-      cmax = fmax(cmax, fabs(ce));
+      const double ce = COUL ? t.cc[((size_t)c.s * NPA + l) * NE + k] * d.cx[l]
+                             : c_energy(d.cf + l * 8, ekp, edot, drd1, dpd1, bij, rbij, rlz, rrlz);
+      if (!COUL) cmax = fmax(cmax, fabs(ce));
       const bool neg = ce < 0;
       const double f0 = p[1], fp1 = p[2];
       const double a = p[neg ? 3 : 0];
@@ -964,7 +973,7 @@ __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool r
         double w = v[n];
         if (first) w = w + xw[n - kPassRows];
         if (w < 0) w = 1E-15;
-        c.col[(l0 + j) * S] = w;
+        if (!COUL || l0 + j >= 1) c.col[(l0 + j) * S] = w;
       }
     }
   }
@@ -1171,6 +1180,114 @@ __device__ void plane_one_loss(const FwdCtx &c, const PlaneFwd &d) {
   }
 }
 
+// ---- Coulomb collisions (src/ModRamCoul.f90) -----------------------------
+// The plane's per-L terms of COULEN and COULMU in d.cx, in the reference's
+// roundings: XNE(L) = NECR(I,J)*BANE(L) with the bounce factor
+// BANE(L) = (1-FNIS/2/FNHS)/(1-MU^2) (:167-172; rows L >= NPA-10 take
+// BANE(NPA-11), the sequential copy of :170-172), BASCNE(L) =
+// NECR(I,J)*BOUNIS/2/BOUNHS (:263), FNHS(L), BOUNHS(L), MU(L).  The caller
+// puts a barrier before they are read.
+__device__ void plane_coul_tables(const Tab &t, const FwdCtx &c, const PlaneFwd &d) {
+  const int NPA = c.NPA, I = c.i + 1, J = c.j + 1;
+  const double necr = t.necr[c.i + (size_t)t.nr * c.j];
+  for (int l = threadIdx.x; l < NPA; l += blockDim.x) {
+    const int L = l + 1, lb = min(L, NPA - 11);
+    const double mub = t.mu[lb - 1];
+    const double bane = (1. - GEO3(t.fnis, I, J, lb) / 2. / GEO3(t.fnhs, I, J, lb)) / (1. - mub * mub);
+    const double bh = GEO3(t.bounhs, I, J, L);
+    d.cx[l] = necr * bane;
+    d.cx[NPA + l] = necr * GEO3(t.bounis, I, J, L) / 2. / bh;
+    d.cx[2 * NPA + l] = GEO3(t.fnhs, I, J, L);
+    d.cx[3 * NPA + l] = bh;
+    d.cx[4 * NPA + l] = t.mu[l];
+  }
+}
+
+// COULMU on the plane (src/ModRamCoul.f90:255-292): for every K = 2..NE the
+// implicit pitch-angle step of the line L = 1..NPA, solved by the
+// reference's recurrences in its order (RK(1)=0, RL(1)=-1; RK, RL forward
+// over L = 2..NPA-1; back substitution; F2(NPA) = F2(NPA-1); the solution
+// is F2/FNHS/MU, scaled back).  A line's recurrences are sequential, so one
+// thread owns a whole column: threads 1..NE-1 (the first chunk's), with RK
+// in the column's own cells and RL in d.cx.  Negative cells become 1e-15
+// when clamp (TimeRamElapsed > 0).  ATA and GTA of the rows ahead are read
+// eight rows at a time, the next eight in flight while these are used.
+__device__ void plane_coulmu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool clamp) {
+  const int NE = c.NE, NPA = c.NPA, S = c.S;
+  const int k = threadIdx.x;
+  if (k >= 1 && k < NE) {
+    double *cell = d.F + k + 1;  // cell (K, row l) is cell[l*S]
+    const double *xb = d.cx + NPA, *fh = d.cx + 2 * NPA, *bh = d.cx + 3 * NPA, *mu = d.cx + 4 * NPA;
+    double *rlv = d.cx + 5 * NPA + k;  // RL of row l at rlv[l*NE]
+    const gptr<const double> ag = as_global((const double *)(t.cag + (size_t)c.s * NPA * NE + k));
+    const int last = NPA - 2;  // row of L = NPA-1
+    double rk = 0.0, rl = -1.0;
+    cell[0] = 0.0;
+    rlv[0] = -1.0;
+    constexpr int B = 4;
+    double2 cur[B], nxt[B];
+#pragma unroll
+    for (int m = 0; m < B; ++m) cur[m] = (1 + m <= last) ? ld_d2(ag, (unsigned)((1 + m) * NE)) : make_double2(0.0, 0.0);
+#pragma unroll 1
+    for (int l0 = 1; l0 <= last; l0 += B) {
+#pragma unroll
+      for (int m = 0; m < B; ++m)
+        nxt[m] = (l0 + B + m <= last) ? ld_d2(ag, (unsigned)((l0 + B + m) * NE)) : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int m = 0; m < B; ++m) {
+        const int l = l0 + m;
+        if (l > last) break;
+        const double an = cur[m].x * xb[l] / fh[l] * bh[l];
+        const double gn = cur[m].y * xb[l - 1] / fh[l] * bh[l - 1];
+        const double bn = an + gn;
+        const double rp = cell[l * S] / fh[l] / mu[l];
+        const double den = bn + gn * rl + 1;
+        rk = (rp + gn * rk) / den;
+        rl = -an / den;
+        cell[l * S] = rk;
+        rlv[l * NE] = rl;
+      }
+#pragma unroll
+      for (int m = 0; m < B; ++m) cur[m] = nxt[m];
+    }
+    double f = rk / (1 + rl);
+    cell[last * S] = f;
+#pragma unroll 4
+    for (int l = last - 1; l >= 0; --l) {
+      f = cell[l * S] - rlv[l * NE] * f;
+      cell[l * S] = f;
+    }
+    cell[(NPA - 1) * S] = cell[last * S];
+#pragma unroll 4
+    for (int l = 0; l < NPA; ++l) {
+      double v = cell[l * S] * fh[l] * mu[l];
+      if (clamp && v < 0.) v = 1E-15;
+      cell[l * S] = v;
+    }
+  }
+  __syncthreads();
+}
+
+// This block's SUMRC partial of the plane (I>=2, K>=2, L>=2, J<=NT-1,
+// src/ModRamRun.f90:246-255) into slot `slot`, with the weights of the loss
+// chain's sums.
+template <int MAXC>
+__device__ void plane_sum_slot(const Tab &t, const FwdCtx &c, const PlaneFwd &d, int b, int slot) {
+  const int NE = c.NE, NPA = c.NPA, k = c.ke;
+  double a[1] = {0.0}, tot[1];
+  if (c.j <= c.NT - 2 && k >= 1) {
+    const double ekw = d.kt[KT_EKEV * NE + k] * d.kt[KT_WE * NE + k];
+#pragma unroll
+    for (int n = 0; n < MAXC; ++n) {
+      if (n >= c.nr) break;
+      const int l = c.l0 + n;
+      if (l >= 1) a[0] = fma(c.col[l * c.cst], ekw * d.lt[LT_WMU * NPA + l], a[0]);
+    }
+  }
+  block_sum_n<1>(a, d.red, tot);
+  if (threadIdx.x == 0) t.partials[((size_t)slot * t.ns + c.s) * t.maxblk + b] = tot[0];
+}
+
 // MAXC (8 or 16) bounds the rows a thread owns; the sweeps take them in
 // passes of kPassRows, so both bounds fit the register budget of 4 waves
 // per SIMD with blocks of up to 1024 threads.
@@ -1189,15 +1306,21 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
       continue;
     }
     constexpr bool need_loss = SEQ == SEQ_CEX || SEQ == SEQ_ATM || SEQ == SEQ_STEP || SEQ == SEQ_LOSS ||
-                               SEQ == SEQ_TAPE || SEQ == SEQ_WAVE;
-    constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE;
+                               SEQ == SEQ_TAPE || SEQ == SEQ_WAVE || SEQ == SEQ_STEPC;
+    constexpr bool need_kl = SEQ == SEQ_E || SEQ == SEQ_MU || SEQ == SEQ_STEP || SEQ == SEQ_TAPE ||
+                             SEQ == SEQ_STEPC;
     FwdCtx c;
     PlaneFwd d;
     fwd_setup<MAXC>(t, u, sm, launch_src(L, t), need_loss, need_kl, c, d);
     const size_t po = (size_t)c.s * f2_species_stride(t) + c.ij * ((size_t)NE * NPA);
     double *gF = launch_dst(L, t) + po;
     const bool inside = d.pc[PC_KIJ + KIJ_MG] == 0.0;
-    const bool rec_here = rec && inside;
+    const bool rec_here = (rec & 1) && inside;  // bit 0: record the CFL maxima
+    const bool clamp = (rec & 2) != 0;          // bit 1: COULMU's clamp (TimeRamElapsed > 0)
+    if constexpr (seq_coul(SEQ)) {
+      plane_coul_tables(t, c, d);
+      __syncthreads();
+    }
 
     if constexpr (SEQ == SEQ_E) {
       plane_e<MAXC>(t, c, d, rec_here);
@@ -1220,6 +1343,26 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
       }
       block_sum_n<1>(a, d.red, tot);
       if (threadIdx.x == 0) t.partials[((size_t)0 * t.ns + c.s) * t.maxblk + u.b] = tot[0];
+    } else if constexpr (SEQ == SEQ_COULEN) {
+      plane_e<MAXC, true>(t, c, d, false);
+    } else if constexpr (SEQ == SEQ_COULMU) {
+      plane_coulmu(t, c, d, clamp);
+    } else if constexpr (SEQ == SEQ_STEPC) {
+      // src/ModRamRun.f90:72-89 and 160-172: the Coulomb operators sit
+      // between the drift halves and the loss chain, with a SUMRC after each
+      plane_e<MAXC>(t, c, d, rec_here);
+      plane_mu<MAXC>(t, c, d, rec_here);
+      plane_sum_slot<MAXC>(t, c, d, u.b, kSumCoul);
+      plane_e<MAXC, true>(t, c, d, false);
+      plane_sum_slot<MAXC>(t, c, d, u.b, kSumCoul + 1);
+      plane_coulmu(t, c, d, clamp);
+      plane_loss<MAXC>(t, c, d, u.b);
+      plane_coulmu(t, c, d, clamp);
+      plane_sum_slot<MAXC>(t, c, d, u.b, kSumCoul + 2);
+      plane_e<MAXC, true>(t, c, d, false);
+      plane_sum_slot<MAXC>(t, c, d, u.b, kSumCoul + 3);
+      plane_mu<MAXC>(t, c, d, false);
+      plane_e<MAXC>(t, c, d, false);
     } else if constexpr (SEQ == SEQ_LOSS) {
       plane_loss<MAXC>(t, c, d, u.b);
     } else if constexpr (SEQ == SEQ_TAPE) {
@@ -1265,8 +1408,8 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
 // use for the next recording.
 // SUMRC bookkeeping (src/ModRamRun.f90:77-175,239-257), summode 1: one SUMRC
 // from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
-// slots 0..5 (slots 0..4 from the plane blocks, slot 5 from the R blocks).
-// One block per (problem, species).
+// the loss chain's slots 0..6 and the radial slot 7, with Coulomb collisions
+// also the slots 8..11 of their operators.  One block per (problem, species).
 constexpr int kFinNV = 2 + kSumSlots;  // the DtDriftE and DtDriftMu minima, then the SUMRC slot sums
 __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int summode, double *red) {
   constexpr int NV = kFinNV;
@@ -1279,13 +1422,18 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
     unsigned long long *m = t.cflmax + (size_t)s * (2 + NE + NPA);
     // every load of the block first: thread 0's scalars, the CFL maxima, the
     // plane and radial blocks' partial sums
-    double e0[4] = {0.0, 0.0, 0.0, 0.0}, lswae = 0.0;
+    double e0[4] = {0.0, 0.0, 0.0, 0.0}, lswae = 0.0, lscoe = 0.0, lscsc = 0.0;
     unsigned long long m01[2] = {0ull, 0ull};
     int cex = 0, wpi = 0;
+    const bool coul = t.coul != 0;
     if (tid == 0) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) e0[q] = t.energy[s * 4 + q];
       lswae = t.lsx[s * 3];
+      if (coul) {
+        lscoe = t.lsx[s * 3 + 1];
+        lscsc = t.lsx[s * 3 + 2];
+      }
       m01[0] = m[0];
       m01[1] = m[1];
       cex = t.cex[s];
@@ -1307,15 +1455,21 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
         if (c > 0) v[1] = fmin(v[1], a * t.dmu[l] / c);
       }
     if (summode != 0) {
-      // slots 0..6 from the plane blocks, the last from the R blocks
-      const int nslot = (summode == 1) ? 1 : kSumSlots;
+      // slots 0..6 (and with Coulomb collisions 8..11) from the plane
+      // blocks, slot 7 from the R blocks
+      const int nslot = (summode == 1) ? 1 : kSumRadial;
       const int nbk = blocks_kl(t), nbr = blocks_r(t);
       const double *pp = t.partials + (size_t)s * t.maxblk;
       const size_t slot = (size_t)t.ns * t.maxblk;
       for (int b = tid; b < nbk; b += T)
-        for (int sl = 0; sl < nslot && sl < kSumSlots - 1; ++sl) v[2 + sl] += pp[sl * slot + b];
-      if (nslot == kSumSlots)
-        for (int b = tid; b < nbr; b += T) v[2 + kSumSlots - 1] += pp[(kSumSlots - 1) * slot + b];
+        for (int sl = 0; sl < nslot; ++sl) v[2 + sl] += pp[sl * slot + b];
+      if (summode == 2) {
+        for (int b = tid; b < nbr; b += T) v[2 + kSumRadial] += pp[kSumRadial * slot + b];
+        if (coul)
+          for (int b = tid; b < nbk; b += T)
+#pragma unroll
+            for (int sl = kSumCoul; sl < kSumCoul + 4; ++sl) v[2 + sl] += pp[sl * slot + b];
+      }
     }
     // one block reduction (fixed order): minima of v[0..1], sums of the rest
     for (int off = 32; off > 0; off >>= 1) {
@@ -1361,7 +1515,15 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
           // slot sums, each operator's only where the reference runs it
           double lsdr = e0[1], lscha = e0[2], lsatm = e0[3];
           double prev = old;
-          lsdr += prev - S[0];
+          if (coul) {  // :77-89
+            lsdr += prev - S[kSumCoul];
+            prev = S[kSumCoul];
+            lscoe += prev - S[kSumCoul + 1];
+            prev = S[kSumCoul + 1];
+            lscsc += prev - S[0];
+          } else {
+            lsdr += prev - S[0];
+          }
           prev = S[0];
           if (wpi) {
             lswae += prev - S[1];
@@ -1383,12 +1545,22 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
             lswae += prev - S[6];
             prev = S[6];
           }
-          lsdr += prev - S[7];
-          e[0] = S[7];
+          if (coul) {  // :160-167
+            lscsc += prev - S[kSumCoul + 2];
+            prev = S[kSumCoul + 2];
+            lscoe += prev - S[kSumCoul + 3];
+            prev = S[kSumCoul + 3];
+          }
+          lsdr += prev - S[kSumRadial];
+          e[0] = S[kSumRadial];
           e[1] = lsdr;
           e[2] = lscha;
           e[3] = lsatm;
           t.lsx[s * 3] = lswae;
+          if (coul) {
+            t.lsx[s * 3 + 1] = lscoe;
+            t.lsx[s * 3 + 2] = lscsc;
+          }
         }
       }
     }
@@ -1560,5 +1732,11 @@ template __global__ void k_plane<SEQ_SUM, 16>(Launch, int);
 template __global__ void k_plane<SEQ_STEP, 16>(Launch, int);
 template __global__ void k_plane<SEQ_LOSS, 16>(Launch, int);
 template __global__ void k_plane<SEQ_TAPE, 16>(Launch, int);
+template __global__ void k_plane<SEQ_COULEN, 8>(Launch, int);
+template __global__ void k_plane<SEQ_COULEN, 16>(Launch, int);
+template __global__ void k_plane<SEQ_COULMU, 8>(Launch, int);
+template __global__ void k_plane<SEQ_COULMU, 16>(Launch, int);
+template __global__ void k_plane<SEQ_STEPC, 8>(Launch, int);
+template __global__ void k_plane<SEQ_STEPC, 16>(Launch, int);
 
 }  // namespace ramgpu

@@ -27,7 +27,13 @@ constexpr int kPlaneThreads = 512;     // plane adjoint kernels: one (K,L) plane
 constexpr int kPlaneMaxThreads = 1024; // forward plane kernels: block size chosen per problem
 constexpr int kMaxCellsPerThread = 16; // cells of one plane per thread, in every mapping
 constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bound, checked at creation
-constexpr int kSumSlots = 8;           // SUMRC partial-sum slots of one ram_run (7 plane, 1 radial)
+// SUMRC partial-sum slots of one ram_run: 0..6 the loss chain's (plane
+// blocks), 7 the last SUMRC (radial blocks), 8..11 the Coulomb operators'
+// (plane blocks: after the first drift half, after COULEN; after the second
+// COULMU, after the second COULEN)
+constexpr int kSumSlots = 12;
+constexpr int kSumRadial = 7;
+constexpr int kSumCoul = 8;
 constexpr int kMuSpread = 4;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 4)
 constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 
@@ -91,6 +97,16 @@ struct Tab {
   double *rlpp;   // [j]
   double *kpnow;  // 1 value
   double *lsx;    // [s][3]: LSWAE, LSCOE, LSCSC
+  // Coulomb collisions (DoUseCoulomb): the COULPARA tables of every species,
+  // (COULE+COULI)(S,K,L) as cc[(s*NPA + l)*NE + k] and {ATA, GTA}(S,K,L) as
+  // cag[(s*NPA + l)*NE + k]; COULEN's ghost factors per species, crel[s*5 +
+  // ..] = GREL(S,1), GREL(S,2), SQRT((GREL1^2-1)/(GREL2^2-1)), GRZERO(S),
+  // SQRT((GRZERO^2-1)/(GREL1^2-1)); NECR (NR,NT) in the reference's layout
+  int coul;
+  double *cc;
+  double2 *cag;
+  double *crel;
+  const double *necr;
   double *ex64;   // [j] 2^(j/64), the table of the loss factors' exponential
   // CFL maxima of max(|C|,floor) over cells inside the magnetopause, as the
   // bit patterns of non-negative doubles (atomicMax order = double order):
@@ -218,6 +234,12 @@ inline __host__ __device__ size_t plane_fwd_lds_bytes(int ne, int npa, int threa
   return ((size_t)npa * (ne + 3) + (size_t)npa + (size_t)threads + 16 * (size_t)npa + 2 * (size_t)npa +
           11 * (size_t)ne + 4 * (size_t)npa + 128 + 64 + 24 + (size_t)ne + (size_t)kMuSpread * npa) *
          sizeof(double);
+}
+// The Coulomb operators' LDS after the forward plane kernels': per L the
+// bounce-averaged density XNE, BASCNE, FNHS, BOUNHS and MU (5 NPA), and
+// COULMU's RL recurrence of every column (NE*NPA).
+inline __host__ __device__ size_t plane_coul_lds_bytes(int ne, int npa, int threads) {
+  return plane_fwd_lds_bytes(ne, npa, threads) + (5 * (size_t)npa + (size_t)ne * npa) * sizeof(double);
 }
 // the adjoint kernels record no CFL maxima: their adjoint plane takes the
 // place of the CFL slots (NE + mu_slots*NPA doubles)

@@ -421,7 +421,15 @@ enum PlaneSeq : int {
   SEQ_LOSS = 6,  // the loss chain alone (WAVELO, CEX, ATMOL, ATMOL, CEX, WAVELO with seven SUMRC partials)
   SEQ_TAPE = 7,  // SEQ_STEP out of place, also writing the input of each later phase to a tape
   SEQ_WAVE = 8,  // WAVELO
+  SEQ_COULEN = 9,   // COULEN
+  SEQ_COULMU = 10,  // COULMU
+  SEQ_STEPC = 11,   // SEQ_STEP with Coulomb collisions: DRIFTE, DRIFTMU, COULEN, COULMU, loss chain,
+                    // COULMU, COULEN, DRIFTMU, DRIFTE with their SUMRC partials
 };
+// plane kernels whose LDS holds the Coulomb operators' tables
+__host__ __device__ constexpr bool seq_coul(int seq) {
+  return seq == SEQ_COULEN || seq == SEQ_COULMU || seq == SEQ_STEPC;
+}
 
 // Elements of one F2-sized buffer (all species) of the problem t.
 __device__ __forceinline__ size_t f2_elems(const Tab &t) { return (size_t)t.ns * f2_species_stride(t); }

@@ -38,6 +38,8 @@ EXPORTS = (
     "ramgpu_ram_run_schedule_noalloc", "ramgpu_set_loss_state", "ramgpu_get_loss_state",
     "ramgpu_set_wavelo", "ramgpu_set_wave_params", "ramgpu_wavelo",
     "ramgpu_batch_kernel_timing", "ramgpu_batch_kernel_times",
+    "ramgpu_set_coulomb", "ramgpu_set_necr", "ramgpu_set_time", "ramgpu_get_time", "ramgpu_coulpara",
+    "ramgpu_get_coulomb_tables", "ramgpu_coulen", "ramgpu_coulmu",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
@@ -105,6 +107,13 @@ def load_library(path: Path | None = None):
     lib.ramgpu_set_wavelo.argtypes = [v, C.c_int, _dp, _dp, _dp]
     lib.ramgpu_set_wave_params.argtypes = [v, C.c_double, C.c_double, _dp, _dp, C.c_int]
     lib.ramgpu_wavelo.argtypes = [v, C.c_int]
+    lib.ramgpu_set_coulomb.argtypes = [v, C.c_int, _dp]
+    lib.ramgpu_set_necr.argtypes = [v, _dp]
+    lib.ramgpu_set_time.argtypes = [v, C.c_double]
+    lib.ramgpu_get_time.argtypes = [v, _dp]
+    for n in ("coulpara", "coulen", "coulmu"):
+        getattr(lib, "ramgpu_" + n).argtypes = [v, C.c_int]
+    lib.ramgpu_get_coulomb_tables.argtypes = [v, C.c_int] + [_dp] * 6
     lib.ramgpu_batch_kernel_timing.argtypes = [v, C.c_int]
     lib.ramgpu_batch_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
@@ -203,7 +212,45 @@ class RamGpu:
             self.set_moment_factors(g)
         if getattr(problem, "wave", None) is not None:
             self.set_wave(problem.wave)
+        if getattr(problem, "coulomb", None) is not None:
+            self.set_coulomb(g.vbnd, problem.coulomb.necr, problem.coulomb.time_elapsed)
         return self
+
+    def set_coulomb(self, vbnd=None, necr=None, time_elapsed=None, on=True):
+        """Coulomb collisions (DoUseCoulomb) with VBND (NS,NE), the
+        plasmasphere density NECR (NR,NT) and TimeRamElapsed."""
+        if on:
+            self._check(self.lib.ramgpu_set_coulomb(self.h, 1, _p(_fd(vbnd, self.ns * self.ne))))
+        else:
+            self._check(self.lib.ramgpu_set_coulomb(self.h, 0, None))
+        if necr is not None:
+            self._check(self.lib.ramgpu_set_necr(self.h, _p(_fd(necr, self.nr * self.nt))))
+        if time_elapsed is not None:
+            self.set_time(time_elapsed)
+
+    def set_time(self, time_elapsed):
+        self._check(self.lib.ramgpu_set_time(self.h, float(time_elapsed)))
+
+    def time(self) -> float:
+        t = C.c_double(0.0)
+        self._check(self.lib.ramgpu_get_time(self.h, C.byref(t)))
+        return t.value
+
+    def coulpara(self, s):
+        self._check(self.lib.ramgpu_coulpara(self.h, s))
+
+    def coulomb_tables(self, s) -> dict:
+        """COULE, COULI, ATA, GTA, CEDR, CIDR (NE,NPA) of species s."""
+        names = ("coule", "couli", "ata", "gta", "cedr", "cidr")
+        out = {n: np.zeros((self.ne, self.npa), order="F") for n in names}
+        self._check(self.lib.ramgpu_get_coulomb_tables(self.h, s, *[_p(out[n]) for n in names]))
+        return out
+
+    def coulen(self, s):
+        self._check(self.lib.ramgpu_coulen(self.h, s))
+
+    def coulmu(self, s):
+        self._check(self.lib.ramgpu_coulmu(self.h, s))
 
     def set_wave(self, wave):
         """WAVELO for the species flagged in wave.wpi (inputs.WaveInputs)."""
