@@ -33,6 +33,9 @@ More objects in the same line:
     reference's ram_run over the Level-1 drop-in does);
   "multi_species": the plain 4-species 20x25x35x72 run (Kp 2->6 ramp) on one
     context, with its own per-kernel roofline;
+  "multi_species_coulomb": the same run for a third of the steps with
+    Coulomb collisions on (COULEN and COULMU twice a step in the plane
+    kernel);
   "adjoint": H+ on 20x25x35x72, 60 steps, virtual-satellite misfit at 8
     points, gradient w.r.t. the initial F2 and every step's FGEOS with
     checkpoints every 10 steps, against the 60-step forward alone;
@@ -41,7 +44,7 @@ More objects in the same line:
     every 20, in one launch per kernel, against the ensemble's forward;
   "moments": ANISCH of every species (and with FLUX) on the resident F2.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M] [--multi-only]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M] [--multi-only [--coulomb]]
 """
 from __future__ import annotations
 
@@ -70,8 +73,6 @@ ALGO_BYTES_PER_CELL_UPDATE = 128  # 8 directional sweeps x (read + write) of F2
 METRIC = "FP64 RAM split-step cell-updates/s + HBM roofline %; batched & adjoint vs plain"
 TRAFFIC_FILE = REPO / "profiles" / "traffic_c3_ensemble.txt"
 TRAFFIC_FILE_MULTI = REPO / "profiles" / "traffic_multi_species.txt"
-# MI355X vector issue: 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock, one
-# wave64 VALU instruction (FP64 included) per 4 cycles per SIMD
 ORACLE_CFLAGS = "gcc -O2 -fPIC -fopenmp -ffp-contract=off (oracle/Makefile)"
 
 
@@ -315,13 +316,16 @@ def bench_c3(members: int, steps: int, warmup: int, seed: int, seq_steps: int = 
             "ktimes": ktimes}
 
 
-def bench_multi_species(steps: int, warmup: int, seed: int):
+def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = False):
     """The plain multi-species run on one context: Kp ramp 2->6, boundary
-    scale LogNormal(0,0.3) per step."""
+    scale LogNormal(0,0.3) per step; with coulomb, Coulomb collisions on
+    (synthetic plasmasphere of Kp=2, TimeRamElapsed from 0)."""
     from ramgpu import inputs
     from ramgpu.capi import RamGpu, copy_probe
 
     problem = inputs.multi_species(seed=seed)
+    if coulomb:
+        problem.coulomb = inputs.CoulombInputs(inputs.plasmasphere_density(problem.grid, 2.0), 0.0)
     g = problem.grid
     ns = g.ns
     cells = ns * g.nr * g.nt * g.ne * g.npa
@@ -349,13 +353,14 @@ def bench_multi_species(steps: int, warmup: int, seed: int):
     log(f"bench: multi-species {dev_ms / steps * 1e3:.1f} us/step, {value:.3e} cell-updates/s; plane "
         f"{roof['per_kernel']['plane']['avg_launch_us']:.1f} us")
     res = {"workload": "e-(WAVELO),H+,He+,O+ (1:0.7:0.05:0.25) on 20x25x35x72, dt=5 s, Kp 2->6 ramp, "
-                       "boundary scale LogNormal(0,0.3) per step",
+                       "boundary scale LogNormal(0,0.3) per step" + (", Coulomb collisions" if coulomb else ""),
            "cells": cells, "steps": steps, "value": value, "unit": "cell-updates/s",
            "ms_per_step": dev_ms / steps, "ms_per_step_with_kernel_events": ev_ms / steps,
            "step_algorithmic_GBps": ALGO_BYTES_PER_CELL_UPDATE * value / 1e9,
            "step_frac": ALGO_BYTES_PER_CELL_UPDATE * value / 1e9 / HBM_PEAK_GBPS, "roofline": roof,
            "copy_peak_GBps": copy_probe(1 << 30, 10)}
-    res["moments"] = bench_moments(dev, cells_kl)
+    if not coulomb:
+        res["moments"] = bench_moments(dev, cells_kl)
     dev.close()
     return res, problem
 
@@ -480,6 +485,8 @@ def main():
     ap.add_argument("--ens-adjoint-steps", type=int, default=120)
     ap.add_argument("--multi-only", action="store_true",
                     help="run only the multi-species leg and print its object (profiling)")
+    ap.add_argument("--coulomb", action="store_true", help="with --multi-only: Coulomb collisions on")
+    ap.add_argument("--no-coulomb", action="store_true", help="skip the multi-species leg with Coulomb collisions")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -493,7 +500,7 @@ def main():
     if args.steps < 1 or args.warmup < 0 or args.members < 2:
         ap.error("--steps must be >= 1, --warmup >= 0, --members >= 2")
     if args.multi_only:
-        print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed)[0]), flush=True)
+        print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed, args.coulomb)[0]), flush=True)
         return
 
     c3 = bench_c3(args.members, args.steps, args.warmup, args.seed)
@@ -532,6 +539,9 @@ def main():
             out["multi_species"], problem = r
         else:
             out["multi_species"] = r
+    if not args.no_coulomb:
+        r = leg("multi_species_coulomb", lambda: bench_multi_species(args.multi_steps // 3, 5, args.seed, True))
+        out["multi_species_coulomb"] = r[0] if isinstance(r, tuple) else r
     if not args.no_adjoint:
         out["adjoint"] = leg("adjoint", lambda: bench_adjoint(args.seed))
         if args.ens_adjoint_members > 0:

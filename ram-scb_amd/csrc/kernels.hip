@@ -761,7 +761,7 @@ struct PlaneFwd {
   double *X;   // T: terms handed between chunks (DRIFTMU) and waves (DRIFTE)
   double *cf, *cm, *cl, *kt, *lt, *red, *ex, *pc;
   unsigned long long *cmx;
-  double *cx;  // Coulomb kernels only: 5*NPA per-L terms, then COULMU's RL (NE*NPA)
+  double *cx;  // Coulomb kernels only: 7*NPA per-L terms, then COULMU's RL (NE*NPA)
 };
 
 __device__ __forceinline__ PlaneFwd plane_fwd_lds(double *sm, int ne, int npa, int threads) {
@@ -1185,8 +1185,9 @@ __device__ void plane_one_loss(const FwdCtx &c, const PlaneFwd &d) {
 // roundings: XNE(L) = NECR(I,J)*BANE(L) with the bounce factor
 // BANE(L) = (1-FNIS/2/FNHS)/(1-MU^2) (:167-172; rows L >= NPA-10 take
 // BANE(NPA-11), the sequential copy of :170-172), BASCNE(L) =
-// NECR(I,J)*BOUNIS/2/BOUNHS (:263), FNHS(L), BOUNHS(L), MU(L).  The caller
-// puts a barrier before they are read.
+// NECR(I,J)*BOUNIS/2/BOUNHS (:263), FNHS(L), BOUNHS(L), MU(L) and the
+// correctly rounded reciprocals of FNHS and MU.  The caller puts a barrier
+// before they are read.
 __device__ void plane_coul_tables(const Tab &t, const FwdCtx &c, const PlaneFwd &d) {
   const int NPA = c.NPA, I = c.i + 1, J = c.j + 1;
   const double necr = t.necr[c.i + (size_t)t.nr * c.j];
@@ -1197,9 +1198,12 @@ __device__ void plane_coul_tables(const Tab &t, const FwdCtx &c, const PlaneFwd 
     const double bh = GEO3(t.bounhs, I, J, L);
     d.cx[l] = necr * bane;
     d.cx[NPA + l] = necr * GEO3(t.bounis, I, J, L) / 2. / bh;
-    d.cx[2 * NPA + l] = GEO3(t.fnhs, I, J, L);
+    const double fh = GEO3(t.fnhs, I, J, L), mu = t.mu[l];
+    d.cx[2 * NPA + l] = fh;
     d.cx[3 * NPA + l] = bh;
-    d.cx[4 * NPA + l] = t.mu[l];
+    d.cx[4 * NPA + l] = mu;
+    d.cx[5 * NPA + l] = 1.0 / fh;  // RN(1/FNHS) and RN(1/MU) for div_rn (1/MU(1) is not used)
+    d.cx[6 * NPA + l] = 1.0 / mu;
   }
 }
 
@@ -1210,15 +1214,19 @@ __device__ void plane_coul_tables(const Tab &t, const FwdCtx &c, const PlaneFwd 
 // is F2/FNHS/MU, scaled back).  A line's recurrences are sequential, so one
 // thread owns a whole column: threads 1..NE-1 (the first chunk's), with RK
 // in the column's own cells and RL in d.cx.  Negative cells become 1e-15
-// when clamp (TimeRamElapsed > 0).  ATA and GTA of the rows ahead are read
-// eight rows at a time, the next eight in flight while these are used.
+// when clamp (TimeRamElapsed > 0).  The quotients are correctly rounded
+// (div_rn): by FNHS and MU from their tabulated reciprocals, and RK and RL
+// from one reciprocal of DENOM per row, so that a row holds one full
+// division.  ATA and GTA of the rows ahead are read four rows at a time, the
+// next four in flight while these are used.
 __device__ void plane_coulmu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool clamp) {
   const int NE = c.NE, NPA = c.NPA, S = c.S;
   const int k = threadIdx.x;
   if (k >= 1 && k < NE) {
     double *cell = d.F + k + 1;  // cell (K, row l) is cell[l*S]
     const double *xb = d.cx + NPA, *fh = d.cx + 2 * NPA, *bh = d.cx + 3 * NPA, *mu = d.cx + 4 * NPA;
-    double *rlv = d.cx + 5 * NPA + k;  // RL of row l at rlv[l*NE]
+    const double *rfh = d.cx + 5 * NPA, *rmu = d.cx + 6 * NPA;
+    double *rlv = d.cx + 7 * NPA + k;  // RL of row l at rlv[l*NE]
     const gptr<const double> ag = as_global((const double *)(t.cag + (size_t)c.s * NPA * NE + k));
     const int last = NPA - 2;  // row of L = NPA-1
     double rk = 0.0, rl = -1.0;
@@ -1237,13 +1245,15 @@ __device__ void plane_coulmu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, b
       for (int m = 0; m < B; ++m) {
         const int l = l0 + m;
         if (l > last) break;
-        const double an = cur[m].x * xb[l] / fh[l] * bh[l];
-        const double gn = cur[m].y * xb[l - 1] / fh[l] * bh[l - 1];
+        const double f = fh[l], rf = rfh[l];
+        const double an = div_rn(cur[m].x * xb[l], f, rf) * bh[l];
+        const double gn = div_rn(cur[m].y * xb[l - 1], f, rf) * bh[l - 1];
         const double bn = an + gn;
-        const double rp = cell[l * S] / fh[l] / mu[l];
+        const double rp = div_rn(div_rn(cell[l * S], f, rf), mu[l], rmu[l]);
         const double den = bn + gn * rl + 1;
-        rk = (rp + gn * rk) / den;
-        rl = -an / den;
+        const double rden = 1.0 / den;
+        rk = div_rn(rp + gn * rk, den, rden);
+        rl = div_rn(-an, den, rden);
         cell[l * S] = rk;
         rlv[l * NE] = rl;
       }

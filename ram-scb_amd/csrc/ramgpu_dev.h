@@ -236,10 +236,10 @@ inline __host__ __device__ size_t plane_fwd_lds_bytes(int ne, int npa, int threa
          sizeof(double);
 }
 // The Coulomb operators' LDS after the forward plane kernels': per L the
-// bounce-averaged density XNE, BASCNE, FNHS, BOUNHS and MU (5 NPA), and
-// COULMU's RL recurrence of every column (NE*NPA).
+// bounce-averaged density XNE, BASCNE, FNHS, BOUNHS, MU, 1/FNHS and 1/MU
+// (7 NPA), and COULMU's RL recurrence of every column (NE*NPA).
 inline __host__ __device__ size_t plane_coul_lds_bytes(int ne, int npa, int threads) {
-  return plane_fwd_lds_bytes(ne, npa, threads) + (5 * (size_t)npa + (size_t)ne * npa) * sizeof(double);
+  return plane_fwd_lds_bytes(ne, npa, threads) + (7 * (size_t)npa + (size_t)ne * npa) * sizeof(double);
 }
 // the adjoint kernels record no CFL maxima: their adjoint plane takes the
 // place of the CFL slots (NE + mu_slots*NPA doubles)

@@ -247,6 +247,60 @@ module ramgpu_mod
        real(c_double), intent(out), optional :: flux(*)  ! absent: NULL, no FLUX
      end function ramgpu_moments
 
+     ! Coulomb collisions (src/ModRamCoul.f90): DoUseCoulomb with VBND(NS,NE),
+     ! NECR(NR,NT), TimeRamElapsed, COULPARA/COULEN/COULMU(S)
+     integer(c_int) function ramgpu_set_coulomb(ctx, use, vbnd) bind(C, name="ramgpu_set_coulomb")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: use
+       real(c_double), intent(in) :: vbnd(*)
+     end function ramgpu_set_coulomb
+
+     integer(c_int) function ramgpu_set_necr(ctx, necr) bind(C, name="ramgpu_set_necr")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(in) :: necr(*)
+     end function ramgpu_set_necr
+
+     integer(c_int) function ramgpu_set_time(ctx, time_elapsed) bind(C, name="ramgpu_set_time")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: time_elapsed
+     end function ramgpu_set_time
+
+     integer(c_int) function ramgpu_get_time(ctx, time_elapsed) bind(C, name="ramgpu_get_time")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: time_elapsed
+     end function ramgpu_get_time
+
+     integer(c_int) function ramgpu_coulpara(ctx, s) bind(C, name="ramgpu_coulpara")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_coulpara
+
+     integer(c_int) function ramgpu_coulen(ctx, s) bind(C, name="ramgpu_coulen")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_coulen
+
+     integer(c_int) function ramgpu_coulmu(ctx, s) bind(C, name="ramgpu_coulmu")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_coulmu
+
+     ! COULE, COULI, ATA, GTA, CEDR, CIDR (NE,NPA) of species s
+     integer(c_int) function ramgpu_get_coulomb_tables(ctx, s, coule, couli, ata, gta, cedr, cidr) &
+          bind(C, name="ramgpu_get_coulomb_tables")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(out) :: coule(*), couli(*), ata(*), gta(*), cedr(*), cidr(*)
+     end function ramgpu_get_coulomb_tables
+
      integer(c_int) function ramgpu_ram_run(ctx, nsteps, out) bind(C, name="ramgpu_ram_run")
        import :: c_int, c_ptr, ramgpu_species_out
        type(c_ptr), value :: ctx
