@@ -204,6 +204,29 @@ int ramgpu_get_coulomb_tables(ramgpu_ctx *ctx, int s, double *coule, double *cou
 int ramgpu_coulen(ramgpu_ctx *ctx, int s);
 int ramgpu_coulmu(ramgpu_ctx *ctx, int s);
 
+/* ---- implicit pitch-angle diffusion: WPADIF, FLCscatter ------------------ */
+
+/* The WPADIF coefficients of species s (src/ModRamWPI.f90:643-714): the
+ * bounce-averaged diffusion coefficient d1 + d2, both (NR,NT,NE,NPA) --
+ * ATAW + ATAC for electrons (hiss and chorus), ATAW_emic_h + ATAW_emic_he
+ * for ions (EMIC) -- d2 may be NULL.  In ramgpu_ram_run the species then
+ * takes WPADIF at the WPI positions in place of WAVELO when at_wpi
+ * (species%WPI and DoUseWPI, src/ModRamRun.f90:91-99,151-159) and at the
+ * EMIC positions when at_emic (species%EMIC and DoUseEMIC, :101-106,
+ * 144-149).  d1 NULL: the species takes no WPADIF. */
+int ramgpu_set_wpadif(ramgpu_ctx *ctx, int s, const double *d1, const double *d2, int at_wpi, int at_emic);
+/* WPADIF(S) on the device-resident F2 with the species' coefficients. */
+int ramgpu_wpadif(ramgpu_ctx *ctx, int s);
+/* FLC_coef(S,:,:,:,:) (NR,NT,NE,NPA) of species s, the field-line-curvature
+ * pitch-angle diffusion coefficient of PARA_FLC (src/ModRamLoss.f90:342),
+ * and Dt_bc (src/ModRamTiming.f90): ramgpu_ram_run then takes FLCscatter
+ * for the species around its atmospheric losses (species%FLC and DoUseFLC,
+ * src/ModRamRun.f90:114-119,130-135); coef NULL: no FLC for the species. */
+int ramgpu_set_flc(ramgpu_ctx *ctx, int s, const double *coef, double dt_bc);
+/* FLCscatter(S), src/ModRamLoss.f90:513-575: nothing while TimeRamElapsed
+ * < Dt_bc. */
+int ramgpu_flcscatter(ramgpu_ctx *ctx, int s);
+
 /* ---- moments of the device-resident F2 ---------------------------------- */
 
 /* The moment factors of ARRAYS (src/ModRamInit.f90:560-576): FFACTOR
@@ -231,16 +254,16 @@ int ramgpu_moments(ramgpu_ctx *ctx, double *ppert, double *ppart, double *edent,
 /* ---- the fused split step ----------------------------------------------- */
 
 /* `nsteps` consecutive ram_run calls (src/ModRamRun.f90:64-205): per species
- * R,P,E,Mu drifts, [COULEN, COULMU], [WAVELO], charge exchange, atmospheric
- * loss twice, charge exchange, [WAVELO], [COULMU, COULEN], Mu,E,P,R drifts
- * with the SUMRC bookkeeping, then the periodic local-time copy and the
- * magnetopause mask.  WAVELO only for the species given its tables
- * (ramgpu_set_wavelo), the Coulomb operators with ramgpu_set_coulomb.  The
- * other optional processes of ram_run -- WPADIF (DoUseWPI, and EMIC with
- * DoUseEMIC) and FLC scattering (DoUseFLC) -- are not part of this step: a
- * host that switches any of them on must not replace ram_run's species loop
- * with this call.  out (may be NULL) receives NS records for the last
- * step. */
+ * R,P,E,Mu drifts, [COULEN, COULMU], [WAVELO or WPADIF], [EMIC WPADIF],
+ * [charge exchange], [FLCscatter], atmospheric loss twice, and the same
+ * back in reverse, Mu,E,P,R drifts with the SUMRC bookkeeping, then the
+ * periodic local-time copy and the magnetopause mask.  Each optional
+ * process as its set-up call switches it on: WAVELO (ramgpu_set_wavelo),
+ * Coulomb collisions (ramgpu_set_coulomb), WPADIF and EMIC
+ * (ramgpu_set_wpadif), FLC scattering (ramgpu_set_flc).  Without WPADIF and
+ * FLC the middle of the step is one fused plane kernel; with them the step
+ * runs one launch per operator, species after species (the same results).
+ * out (may be NULL) receives NS records for the last step. */
 int ramgpu_ram_run(ramgpu_ctx *ctx, int nsteps, ramgpu_species_out *out);
 
 /* The same with a per-step Kp and boundary scale schedule held on the host:
@@ -285,7 +308,8 @@ int ramgpu_last_run_ms(ramgpu_ctx *ctx, double *ms);
 #define RAMGPU_KERNEL_P_BWD 7    /* adjoint of DRIFTP                         */
 #define RAMGPU_KERNEL_PLANE_BWD 8 /* adjoints of DRIFTE, DRIFTMU, losses      */
 #define RAMGPU_KERNEL_MOMENTS 9  /* ANISCH pressure moments and FLUX          */
-#define RAMGPU_KERNEL_KINDS 10
+#define RAMGPU_KERNEL_PITCH 10   /* WPADIF and FLCscatter line solves          */
+#define RAMGPU_KERNEL_KINDS 11
 int ramgpu_kernel_timing(ramgpu_ctx *ctx, int enable);
 /* Measurement support: a plain device copy of `bytes` (read + write) run
  * `reps` times on device `device`; gbps receives the achieved read+write

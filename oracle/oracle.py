@@ -37,7 +37,9 @@ class _State(C.Structure):
          ("lz", _dp), ("rlpp", _dp), ("kp", C.c_double), ("lswae", _dp),
          ("vbnd", _dp), ("necr", _dp), ("time_elapsed", C.c_double),
          ("coule", _dp), ("couli", _dp), ("ata", _dp), ("gta", _dp), ("cedr", _dp), ("cidr", _dp),
-         ("use_coulomb", C.c_int), ("lscoe", _dp), ("lscsc", _dp)]
+         ("use_coulomb", C.c_int), ("lscoe", _dp), ("lscsc", _dp),
+         ("pd1", C.POINTER(_dp)), ("pd2", C.POINTER(_dp)), ("pd_wpi", _ip), ("pd_emic", _ip),
+         ("flc", C.POINTER(_dp)), ("dt_bc", C.c_double)]
 
 COUL_TABLES = ("coule", "couli", "ata", "gta", "cedr", "cidr")
 
@@ -87,6 +89,8 @@ class Oracle:
             lib.oracle_wavelo.argtypes = [sp, C.c_int]
             for name in ("coulpara", "coulen", "coulmu"):
                 getattr(lib, "oracle_" + name).argtypes = [sp, C.c_int]
+            lib.oracle_wpadif.argtypes = [sp, C.c_int, _dp, _dp]
+            lib.oracle_flcscatter.argtypes = [sp, C.c_int, _dp, C.c_double]
             lib.oracle_ram_run.argtypes = [sp, C.c_int, _dp]
             lib.oracle_ram_run.restype = C.c_int
             lib.oracle_drift_alloc.argtypes = [sp, C.POINTER(_Drift)]
@@ -140,6 +144,13 @@ class Oracle:
         for n in COUL_TABLES:
             setattr(st, n, d(n, np.zeros((g.ns, g.ne, g.npa))))
         st.necr = d("necr", np.zeros((g.nr, g.nt)))
+        # WPADIF / FLC tables, per species (null pointers: none)
+        self._pd1 = (_dp * g.ns)()
+        self._pd2 = (_dp * g.ns)()
+        self._flc = (_dp * g.ns)()
+        st.pd1, st.pd2, st.flc = self._pd1, self._pd2, self._flc
+        st.pd_wpi = i("pd_wpi", np.zeros(g.ns, dtype=np.int32))
+        st.pd_emic = i("pd_emic", np.zeros(g.ns, dtype=np.int32))
         coul = getattr(problem, "coulomb", None)
         if coul is not None:
             keep["necr"][...] = coul.necr
@@ -177,6 +188,34 @@ class Oracle:
         if time_elapsed is not None:
             self.st.time_elapsed = float(time_elapsed)
         self.st.use_coulomb = 1 if on else 0
+
+    def set_wpadif(self, s, d1, d2=None, at_wpi=True, at_emic=False):
+        """WPADIF coefficients of species s (D = d1 + d2, (NR,NT,NE,NPA)) and
+        where ram_run takes WPADIF: at the WPI positions (in place of WAVELO),
+        at the EMIC positions."""
+        self._pd1[s - 1] = _ptr(self._store(f"pd1_{s}", d1))
+        self._pd2[s - 1] = _ptr(self._store(f"pd2_{s}", d2)) if d2 is not None else _dp()
+        self._keep["pd_wpi"][s - 1] = 1 if at_wpi else 0
+        self._keep["pd_emic"][s - 1] = 1 if at_emic else 0
+
+    def set_flc(self, s, coef, dt_bc=0.0):
+        """FLC_coef(S,...) of species s (NR,NT,NE,NPA) for FLCscatter in
+        ram_run, with Dt_bc."""
+        self._flc[s - 1] = _ptr(self._store(f"flc_{s}", coef))
+        self.st.dt_bc = float(dt_bc)
+
+    def _store(self, name, a):
+        self._keep[name] = _fd(a)
+        return self._keep[name]
+
+    def wpadif(self, s, d1, d2=None):
+        a = self._store("_op_d1", d1)
+        b = self._store("_op_d2", d2) if d2 is not None else None
+        self.lib().oracle_wpadif(C.byref(self.st), s, _ptr(a), _ptr(b) if b is not None else _dp())
+
+    def flcscatter(self, s, coef, dt_bc=0.0):
+        a = self._store("_op_flc", coef)
+        self.lib().oracle_flcscatter(C.byref(self.st), s, _ptr(a), float(dt_bc))
 
     def coul_table(self, name: str) -> np.ndarray:
         """A COULPARA table (NS,NE,NPA) as numpy (copy)."""

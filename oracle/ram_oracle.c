@@ -764,6 +764,69 @@ void oracle_coulmu(oracle_state *st, int s) {
 }
 #undef CT
 
+/* ---- implicit pitch-angle diffusion: WPADIF and FLCscatter -------------- */
+
+/* The line solve shared by WPADIF(S) (src/ModRamWPI.f90:643-714) and
+ * FLCscatter(S) (src/ModRamLoss.f90:513-575): for J=1..NT, I=2..NR,
+ * K=2..NE, F = F2/FACMU with FACMU(L) = FNHS(I,J,L)*MU(L); RK(1)=0,
+ * RL(1)=-1; for L=2..NPA-1 AN = D(L)/DMU(L)*DTs/FACMU(L)/WMU(L),
+ * GN = D(L-1)/DMU(L-1)*DTs/FACMU(L)/WMU(L), the RK, RL recurrences; back
+ * substitution, F2(NPA) = F2(NPA-1), F2 = F*FACMU.  D(I,J,K,L) is d1+d2
+ * (d2 may be NULL), both (NR,NT,NE,NPA). */
+#define D4(A, I, J, K, L) (A)[((I)-1) + (size_t)st->nr * (((J)-1) + (size_t)st->nt * (((K)-1) + (size_t)st->ne * ((L)-1)))]
+static void pitch_diffusion(oracle_state *st, int s, const double *d1, const double *d2) {
+  const int npa = st->npa;
+  double *rk = (double *)calloc(npa + 1, sizeof(double)), *rl = (double *)calloc(npa + 1, sizeof(double));
+  double *facmu = (double *)calloc(npa + 1, sizeof(double)), *f = (double *)calloc(npa + 1, sizeof(double));
+  for (int j = 1; j <= st->nt; ++j)
+    for (int i = 2; i <= st->nr; ++i)
+      for (int k = 2; k <= st->ne; ++k) {
+        for (int l = 2; l <= npa; ++l) {
+          facmu[l] = G3(st->fnhs, i, j, l) * V1(st->mu, l);
+          f[l] = F2(s, i, j, k, l) / facmu[l];
+        }
+        facmu[1] = G3(st->fnhs, i, j, 1) * V1(st->mu, 1);
+        f[1] = f[2];
+        rk[1] = 0.;
+        rl[1] = -1.;
+        for (int l = 2; l <= npa - 1; ++l) {
+          const double dl = d2 ? D4(d1, i, j, k, l) + D4(d2, i, j, k, l) : D4(d1, i, j, k, l);
+          const double dm = d2 ? D4(d1, i, j, k, l - 1) + D4(d2, i, j, k, l - 1) : D4(d1, i, j, k, l - 1);
+          double an = dl / V1(st->dmu, l);
+          double gn = dm / V1(st->dmu, l - 1);
+          an = an * st->dts / facmu[l] / V1(st->wmu, l);
+          gn = gn * st->dts / facmu[l] / V1(st->wmu, l);
+          const double bn = an + gn;
+          const double rp = f[l];
+          const double denom = bn + gn * rl[l - 1] + 1;
+          rk[l] = (rp + gn * rk[l - 1]) / denom;
+          rl[l] = -an / denom;
+        }
+        F2(s, i, j, k, npa - 1) = rk[npa - 1] / (1 + rl[npa - 1]);
+        for (int l = npa - 2; l >= 1; --l) F2(s, i, j, k, l) = rk[l] - rl[l] * F2(s, i, j, k, l + 1);
+        F2(s, i, j, k, npa) = F2(s, i, j, k, npa - 1);
+        for (int l = 1; l <= npa; ++l) F2(s, i, j, k, l) = F2(s, i, j, k, l) * facmu[l];
+      }
+  free(rk);
+  free(rl);
+  free(facmu);
+  free(f);
+}
+#undef D4
+
+/* WPADIF(S): d1, d2 = ATAW, ATAC for electrons, ATAW_emic_h, ATAW_emic_he
+ * for ions with EMIC (src/ModRamWPI.f90:674-684). */
+void oracle_wpadif(oracle_state *st, int s, const double *d1, const double *d2) {
+  pitch_diffusion(st, s, d1, d2);
+}
+
+/* FLCscatter(S) with FLC_coef(S,:,:,:,:) as coef (NR,NT,NE,NPA); nothing
+ * while TimeRamElapsed < Dt_bc (src/ModRamLoss.f90:526). */
+void oracle_flcscatter(oracle_state *st, int s, const double *coef, double dt_bc) {
+  if (st->time_elapsed < dt_bc) return;
+  pitch_diffusion(st, s, coef, NULL);
+}
+
 /* SUMRC, src/ModRamRun.f90:231-259 (sequential sum in the I,K,L,J order). */
 void oracle_sumrc(oracle_state *st, int s) {
   st->elorc[s - 1] = 0.0;
@@ -858,8 +921,8 @@ void oracle_flux(const oracle_state *st, const double *ffactor, double *flux) {
 #undef FF
 #undef SIJ
 
-/* The per-species body of ram_run, src/ModRamRun.f90:66-184: WAVELO and
- * Coulomb collisions as switched, EMIC, FLC and WPADIF off. */
+/* The per-species body of ram_run, src/ModRamRun.f90:66-184, with the
+ * optional processes as switched. */
 static int ram_run_species(oracle_state *st, int s) {
   oracle_drift_scratch d;
   oracle_loss_scratch ls;
@@ -869,7 +932,11 @@ static int ram_run_species(oracle_state *st, int s) {
     return -1;
   }
   const int cex = st->cex_kind[s - 1] != ORACLE_CEX_NONE;
-  const int wpi = st->wpi && st->wpi[s - 1];
+  const int wave = st->wpi && st->wpi[s - 1];              /* WAVELO */
+  const int wpd = st->pd_wpi && st->pd_wpi[s - 1];         /* WPADIF in its place */
+  const int emic = st->pd_emic && st->pd_emic[s - 1];      /* WPADIF for EMIC */
+  const int flc = st->flc && st->flc[s - 1];
+  const int wpi = wave || wpd;
   const int coul = st->use_coulomb;
   oracle_cepara(st, s, &ls);
   oracle_driftpara(st, s, &d);
@@ -889,7 +956,13 @@ static int ram_run_species(oracle_state *st, int s) {
     st->lscsc[s - 1] += st->elorc[s - 1];
   }
   if (wpi) {
-    oracle_wavelo(st, s);
+    if (wpd) oracle_wpadif(st, s, st->pd1[s - 1], st->pd2[s - 1]);
+    else oracle_wavelo(st, s);
+    oracle_sumrc(st, s);
+    st->lswae[s - 1] += st->elorc[s - 1];
+  }
+  if (emic) {
+    oracle_wpadif(st, s, st->pd1[s - 1], st->pd2[s - 1]);
     oracle_sumrc(st, s);
     st->lswae[s - 1] += st->elorc[s - 1];
   }
@@ -898,19 +971,27 @@ static int ram_run_species(oracle_state *st, int s) {
     oracle_sumrc(st, s);
     st->lscha[s - 1] += st->elorc[s - 1];
   }
+  if (flc) oracle_flcscatter(st, s, st->flc[s - 1], st->dt_bc);
   oracle_atmol(st, s, &ls);
   oracle_sumrc(st, s);
   st->lsatm[s - 1] += st->elorc[s - 1];
   oracle_atmol(st, s, &ls);
   oracle_sumrc(st, s);
   st->lsatm[s - 1] += st->elorc[s - 1];
+  if (flc) oracle_flcscatter(st, s, st->flc[s - 1], st->dt_bc);
   if (cex) {
     oracle_charexchange(st, s, &ls);
     oracle_sumrc(st, s);
     st->lscha[s - 1] += st->elorc[s - 1];
   }
+  if (emic) {
+    oracle_wpadif(st, s, st->pd1[s - 1], st->pd2[s - 1]);
+    oracle_sumrc(st, s);
+    st->lswae[s - 1] += st->elorc[s - 1];
+  }
   if (wpi) {
-    oracle_wavelo(st, s);
+    if (wpd) oracle_wpadif(st, s, st->pd1[s - 1], st->pd2[s - 1]);
+    else oracle_wavelo(st, s);
     oracle_sumrc(st, s);
     st->lswae[s - 1] += st->elorc[s - 1];
   }

@@ -68,6 +68,16 @@ typedef struct {
   double *coule, *couli, *ata, *gta, *cedr, *cidr;
   int use_coulomb;
   double *lscoe, *lscsc;
+  /* WPADIF and FLCscatter in ram_run (src/ModRamRun.f90:91-111,122-150):
+   * per species the WPADIF coefficients pd1[s-1] + pd2[s-1] (NR,NT,NE,NPA;
+   * pd2[s-1] may be NULL), taken at the WPI positions in place of WAVELO
+   * when pd_wpi[s-1] (species%WPI and DoUseWPI) and at the EMIC positions
+   * when pd_emic[s-1] (species%EMIC and DoUseEMIC); FLC_coef(S,...) as
+   * flc[s-1] (NULL: no FLC) with Dt_bc.  NULL arrays: none of them. */
+  const double *const *pd1, *const *pd2;
+  const int *pd_wpi, *pd_emic;
+  const double *const *flc;
+  double dt_bc;
 } oracle_state;
 
 /* Per-species drift tables built by DRIFTPARA (src/ModRamDrift.f90:15-19). */
@@ -108,12 +118,18 @@ void oracle_wavelo(oracle_state *st, int s);
 void oracle_coulpara(oracle_state *st, int s);
 void oracle_coulen(oracle_state *st, int s);
 void oracle_coulmu(oracle_state *st, int s);
+/* WPADIF(S), src/ModRamWPI.f90:643-714, with D = d1 + d2 (NR,NT,NE,NPA):
+ * ATAW + ATAC for electrons, ATAW_emic_h + ATAW_emic_he for ions.
+ * FLCscatter(S), src/ModRamLoss.f90:513-575, with coef = FLC_coef(S,...)
+ * (NR,NT,NE,NPA); no change while st->time_elapsed < dt_bc. */
+void oracle_wpadif(oracle_state *st, int s, const double *d1, const double *d2);
+void oracle_flcscatter(oracle_state *st, int s, const double *coef, double dt_bc);
 
-/* One ram_run call (src/ModRamRun.f90:64-205) with the optional processes
- * EMIC, FLC and WPADIF off; WAVELO for species with wpi set, Coulomb
- * collisions with use_coulomb (their tables from COULPARA, and the
- * COULMU clamp of TimeRamElapsed > 0 from st->time_elapsed, which this call
- * does not advance).  nthreads>1 runs the species
+/* One ram_run call (src/ModRamRun.f90:64-205): WAVELO for species with wpi
+ * set, WPADIF and FLCscatter as pd_wpi, pd_emic and flc give them, Coulomb
+ * collisions with use_coulomb (their tables from COULPARA, and the COULMU
+ * clamp of TimeRamElapsed > 0 from st->time_elapsed, which this call does
+ * not advance).  nthreads>1 runs the species
  * loop with OpenMP as the reference does (src/ModRamRun.f90:64).  Returns
  * DtsNext before the DtsMin floor in *dts_next (may be NULL). */
 int oracle_ram_run(oracle_state *st, int nthreads, double *dts_next);

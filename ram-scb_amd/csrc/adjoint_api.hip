@@ -248,7 +248,7 @@ int ramgpu_atmol_bwd(ramgpu_ctx *c, int s, double *adj) { return op_bwd(c, s, 5,
 
 int ramgpu_ram_run_bwd(ramgpu_ctx *c, double *adj, double *fgeos_adj) {
   int rc = check_ctx(c);
-  if (rc || (rc = ready(c)) || (rc = no_coulomb(c)) || (rc = prepare(c))) return rc;
+  if (rc || (rc = ready(c)) || (rc = forward_only(c)) || (rc = prepare(c))) return rc;
   if (!adj) return fail(RAMGPU_EINVAL, "adj is null");
   const size_t n = nf2(c);
   DevBuf b;
@@ -278,7 +278,7 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
                        int npts, const int *pts, const double *w, const double *target, double *loss,
                        double *grad_f2, double *grad_fgeos) {
   int rc = check_ctx(c);
-  if (rc || (rc = ready(c)) || (rc = no_coulomb(c))) return rc;
+  if (rc || (rc = ready(c)) || (rc = forward_only(c))) return rc;
   if (nsteps < 1 || ckpt_every < 1) return fail(RAMGPU_EINVAL, "nsteps and ckpt_every must be >= 1");
   if (npts < 1 || !pts || !w || !target) return fail(RAMGPU_EINVAL, "misfit points, weights and targets required");
   if (kp && !c->have_lzphi) return fail(RAMGPU_ESTATE, "ramgpu_set_kp(lz, phi) must precede a Kp schedule");

@@ -515,17 +515,25 @@ int ramgpu_host::fill_out(ramgpu_ctx *c, ramgpu_species_out *out) {
 // change and reused until the next one.
 int ramgpu_host::step_once(ramgpu_ctx *c) {
   int cflmask, rc;
-  if ((rc = step_body(c, &cflmask)) || (rc = run_finalize(c, cflmask, 2, 0, c->ns))) return rc;
+  if ((rc = step_body(c, &cflmask))) return rc;
+  if (cflmask >= 0 && (rc = run_finalize(c, cflmask, 2, 0, c->ns))) return rc;
   return RAMGPU_OK;
 }
 
 // The sweeps of one step; its finalize (cflmask, SUMRC mode 2) is the
-// caller's.
+// caller's, except for the composed step (WPADIF or FLC on), which
+// finalizes species by species and returns cflmask -1.
 int ramgpu_host::step_body(ramgpu_ctx *c, int *cflmask) {
   int rc;
   if ((rc = prepare(c))) return rc;
   const int rec = c->cfl_valid ? 0 : 1;
   const int ns = c->ns;
+  if (takes_pitch(c)) {
+    if ((rc = step_composed(c, rec))) return rc;
+    *cflmask = -1;
+    c->cfl_valid = true;
+    return RAMGPU_OK;
+  }
   // with Coulomb collisions the plane kernel also runs COULEN and COULMU,
   // flag bit 1 being COULMU's clamp (TimeRamElapsed > 0)
   const int seq = c->t.coul ? SEQ_STEPC : SEQ_STEP;
@@ -592,6 +600,11 @@ int ramgpu_create(ramgpu_ctx **out, int ns, int nr, int nt, int ne, int npa, int
   c->plane = plane_shape(ne, npa, c->lds_limit);
   if (c->plane.threads == 0) return cleanup(fail(RAMGPU_EINVAL, "no plane kernel shape for %dx%d", ne, npa));
   t.plane_chunks = plane_chunks_for(ne, npa, c->plane.threads);
+  c->d_pdc.assign(ns, nullptr);
+  c->d_flc.assign(ns, nullptr);
+  c->d_flc_mem.assign(ns, nullptr);
+  c->pd_wpi.assign(ns, 0);
+  c->pd_emic.assign(ns, 0);
 #define A(p, n) \
   if ((rc = dev_alloc(c, &p, n))) return cleanup(rc)
   A(c->d_rlz, nr1); A(c->d_ekev, ne); A(c->d_ebnd, ne); A(c->d_we, ne); A(c->d_de, ne);
@@ -1070,8 +1083,10 @@ int run_schedule_staged(ramgpu_ctx *c, int nsteps, const double *d_kp, const dou
     if ((rc = apply_step_inputs(c, d_kp + n, d_bs ? d_bs + (size_t)n * c->ns : nullptr))) break;
     int cflmask = 0;
     if ((rc = step_body(c, &cflmask))) break;
-    c->fin_mask = cflmask;
-    c->fin_mode = 2;
+    if (cflmask >= 0) {
+      c->fin_mask = cflmask;
+      c->fin_mode = 2;
+    }
     c->time_elapsed += 2.0 * c->t.dts;  // src/ModRamScbRun.f90:88, after each ram_run
   }
   const int rf = flush_finalize(c);

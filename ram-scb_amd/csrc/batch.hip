@@ -159,7 +159,7 @@ int sync_members(ramgpu_batch *b) {
   std::vector<Tab> tabs;
   for (size_t m = 0; m < b->m.size(); ++m) {
     ramgpu_ctx *c = b->m[m];
-    if ((rc = check_ctx(c)) || (rc = no_coulomb(c)) || (rc = prepare(c))) return rc;
+    if ((rc = check_ctx(c)) || (rc = forward_only(c)) || (rc = prepare(c))) return rc;
     if (c->plane.threads > b->plane.threads || c->plane.maxc > b->plane.maxc)
       return fail(RAMGPU_ESTATE, "member %zu changed its plane shape after joining the ensemble", m);
     if ((char)takes_wave(c) != b->wave[m])
@@ -425,7 +425,7 @@ int ramgpu_batch_create(ramgpu_batch **out, ramgpu_ctx *const *members, int nmem
     if (members[i]->device != dev) return fail(RAMGPU_EINVAL, "member %d is on another device", i);
     for (int j = 0; j < i; ++j)
       if (members[j] == members[i]) return fail(RAMGPU_EINVAL, "member %d appears twice", i);
-    const int rc = no_coulomb(members[i]);
+    const int rc = forward_only(members[i]);
     if (rc) return rc;
   }
   HIP_TRY(hipSetDevice(dev));

@@ -200,13 +200,6 @@ int coul_check_lds(const ramgpu_ctx *c, int threads) {
   return RAMGPU_OK;
 }
 
-int no_coulomb(const ramgpu_ctx *c) {
-  if (c->t.coul)
-    return fail(RAMGPU_ESTATE,
-                "Coulomb collisions are on: the reverse mode and ensembles do not take them (ramgpu_set_coulomb(ctx, 0, NULL))");
-  return RAMGPU_OK;
-}
-
 }  // namespace ramgpu_host
 
 using namespace ramgpu_host;

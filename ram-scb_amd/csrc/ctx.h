@@ -17,6 +17,7 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
 __global__ void k_tick(int *step);
 __global__ void k_setstep(int *step, int v);
 __global__ void k_copy_ens(Launch L);
+__global__ void k_pitch_diffusion(const Tab *tp, int s, const double *coef, double *rlbuf);
 template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
 __global__ void k_p_bwd(Launch L, const double *xin, double *adj);
 template <int SEQ, int BT> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
@@ -105,6 +106,12 @@ struct ramgpu_ctx {
   bool coul_dirty = true, have_necr = false;
   double *d_necr = nullptr;
   double time_elapsed = 0.0;
+  // WPADIF and FLCscatter: per species the WPADIF table (device layout) and
+  // whether ram_run takes it at the WPI positions (in place of WAVELO) and at
+  // the EMIC positions; the FLC table (null: no FLC) and its allocation; Dt_bc
+  std::vector<double *> d_pdc, d_flc, d_flc_mem;
+  std::vector<char> pd_wpi, pd_emic;
+  double dt_bc = 0.0;
 };
 
 namespace ramgpu_host {
@@ -157,9 +164,13 @@ void launch_plane_bwd(int seq, int threads, dim3 grid, size_t lds, hipStream_t s
 // the plane kernels' LDS with them fits blocks of `threads`
 int coul_tables(ramgpu_ctx *c);
 int coul_check_lds(const ramgpu_ctx *c, int threads);
-// RAMGPU_ESTATE for a context with Coulomb collisions on: the reverse mode
-// and ensembles step without them
-int no_coulomb(const ramgpu_ctx *c);
+// RAMGPU_ESTATE for a context with Coulomb collisions, WPADIF or FLC
+// scattering on: the reverse mode and ensembles step without them
+int forward_only(const ramgpu_ctx *c);
+// whether a species takes WPADIF or FLCscatter in ram_run, and the step that
+// then runs: one launch per operator, species after species, each finalized
+int takes_pitch(const ramgpu_ctx *c);
+int step_composed(ramgpu_ctx *c, int rec);
 // host Fortran-layout F2 <-> device-layout buffer, through the context's stage
 int upload_f2(ramgpu_ctx *c, double *dst, const double *host);
 int download_f2(ramgpu_ctx *c, double *host, const double *src);

@@ -1417,7 +1417,9 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
 // (E,Mu) as the reference's (:115,223,308,404).  The maxima are cleared after
 // use for the next recording.
 // SUMRC bookkeeping (src/ModRamRun.f90:77-175,239-257), summode 1: one SUMRC
-// from slot 0 (SETRC:=S, ELORC=old-S); summode 2: the full ram_run chain from
+// from slot 0, or with kSumFromRadial from the radial slot (SETRC:=S,
+// ELORC=old-S, and with an accumulator in the bits at kSumAccShift that
+// LS* += ELORC); summode 2: the full ram_run chain from
 // the loss chain's slots 0..6 and the radial slot 7, with Coulomb collisions
 // also the slots 8..11 of their operators.  One block per (problem, species).
 constexpr int kFinNV = 2 + kSumSlots;  // the DtDriftE and DtDriftMu minima, then the SUMRC slot sums
@@ -1440,10 +1442,8 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
 #pragma unroll
       for (int q = 0; q < 4; ++q) e0[q] = t.energy[s * 4 + q];
       lswae = t.lsx[s * 3];
-      if (coul) {
-        lscoe = t.lsx[s * 3 + 1];
-        lscsc = t.lsx[s * 3 + 2];
-      }
+      lscoe = t.lsx[s * 3 + 1];
+      lscsc = t.lsx[s * 3 + 2];
       m01[0] = m[0];
       m01[1] = m[1];
       cex = t.cex[s];
@@ -1464,16 +1464,22 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
         const double c = __longlong_as_double((long long)m[2 + NE + l]);
         if (c > 0) v[1] = fmin(v[1], a * t.dmu[l] / c);
       }
-    if (summode != 0) {
+    const int mode = summode & kSumModeMask;
+    if (mode != 0) {
       // slots 0..6 (and with Coulomb collisions 8..11) from the plane
-      // blocks, slot 7 from the R blocks
-      const int nslot = (summode == 1) ? 1 : kSumRadial;
+      // blocks, slot 7 from the R blocks; a single SUMRC reads slot 0, or
+      // the radial slot
       const int nbk = blocks_kl(t), nbr = blocks_r(t);
       const double *pp = t.partials + (size_t)s * t.maxblk;
       const size_t slot = (size_t)t.ns * t.maxblk;
-      for (int b = tid; b < nbk; b += T)
-        for (int sl = 0; sl < nslot; ++sl) v[2 + sl] += pp[sl * slot + b];
-      if (summode == 2) {
+      if (mode == 1 && (summode & kSumFromRadial)) {
+        for (int b = tid; b < nbr; b += T) v[2] += pp[kSumRadial * slot + b];
+      } else {
+        const int nslot = (mode == 1) ? 1 : kSumRadial;
+        for (int b = tid; b < nbk; b += T)
+          for (int sl = 0; sl < nslot; ++sl) v[2 + sl] += pp[sl * slot + b];
+      }
+      if (mode == 2) {
         for (int b = tid; b < nbr; b += T) v[2 + kSumRadial] += pp[kSumRadial * slot + b];
         if (coul)
           for (int b = tid; b < nbk; b += T)
@@ -1513,13 +1519,20 @@ __device__ inline void finalize_unit(const Launch &L, int vb, int cflmask, int s
       }
       if (cflmask & 4) t.dtdrift[s * 4 + 2] = r[0];
       if (cflmask & 8) t.dtdrift[s * 4 + 3] = r[1];
-      if (summode != 0) {
+      if (mode != 0) {
         const double *S = r + 2;
         double *e = t.energy + s * 4;
         const double old = e0[0];
-        if (summode == 1) {
+        if (mode == 1) {
+          const double elorc = old - S[0];
           e[0] = S[0];
-          t.partials[((size_t)kSumSlots * t.ns + s) * t.maxblk] = old - S[0];  // ELORC
+          t.partials[((size_t)kSumSlots * t.ns + s) * t.maxblk] = elorc;  // ELORC
+          // LS* += ELORC as ram_run books it after the operator
+          const int acc = (summode >> kSumAccShift) & 15;
+          if (acc >= SUM_LSDR && acc <= SUM_LSATM) e[acc] = e0[acc] + elorc;
+          else if (acc == SUM_LSWAE) t.lsx[s * 3] = lswae + elorc;
+          else if (acc == SUM_LSCOE) t.lsx[s * 3 + 1] = lscoe + elorc;
+          else if (acc == SUM_LSCSC) t.lsx[s * 3 + 2] = lscsc + elorc;
         } else {
           // the accumulators of src/ModRamRun.f90:77-175 from consecutive
           // slot sums, each operator's only where the reference runs it
@@ -1664,6 +1677,96 @@ __global__ void k_setstep(int *step, int v) {
 // Whole-F2 copy of every problem of an ensemble launch, source to
 // destination as the launch names them (tape slots or F2); each unit is one
 // problem, its nb blocks striding over it.
+// ---------------------------------------------------------------------------
+// Implicit pitch-angle diffusion of species s: WPADIF (src/ModRamWPI.f90:
+// 643-714) and FLCscatter (src/ModRamLoss.f90:513-575) are the same solve
+// with different coefficients D = coef (NR,NT,NE,NPA in F2's device layout
+// [I][J][L][K]).  One thread per (I>=2, J, K>=2) line, consecutive threads
+// on consecutive K, so each row of the lines of a wave is one contiguous
+// run.  In the reference's order: F = F2/FACMU with FACMU(L) =
+// FNHS(I,J,L)*MU(L); RK(1)=0, RL(1)=-1; for L = 2..NPA-1
+// AN = D(L)/DMU(L)*DTs/FACMU(L)/WMU(L), GN the same of D(L-1)/DMU(L-1),
+// DENOM = AN+GN+GN*RL(L-1)+1, RK = (F+GN*RK(L-1))/DENOM, RL = -AN/DENOM;
+// back substitution, F2(NPA) = F2(NPA-1), F2 = F*FACMU.  Every quotient is
+// correctly rounded: by DMU and WMU from their tabulated reciprocals, by
+// FACMU and DENOM from one reciprocal each per row (div_rn).  RK goes to
+// the line's own F2 cells and RL to rlbuf (F2's layout), both read back by
+// the substitution; D, F and FACMU of four rows are loaded ahead.
+__global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, const double *__restrict__ coef,
+                                                        double *__restrict__ rlbuf) {
+  const Tab &t = *tp;
+  const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr, nk = NE - 1;
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (NR - 1) * NT * nk) return;
+  const int ij = n / nk, k = 1 + (n - ij * nk);
+  const int i = 1 + ij / NT, j = ij - (i - 1) * NT;
+  const size_t po = ((size_t)i * NT + j) * ((size_t)NE * NPA) + k;
+  const gptr<double> f = as_global(t.f2 + (size_t)s * f2_species_stride(t) + po);
+  const gptr<const double> d = as_global(coef + po);
+  const gptr<double> rlv = as_global(rlbuf + po);
+  const gptr<const double> fh = as_global(&GEO3(t.fnhs, i + 1, j + 1, 1));
+  const size_t fst = (size_t)(NR + 1) * NT;  // FNHS's stride along L
+  const double dts = t.dts;
+  const int last = NPA - 2;                  // row of L = NPA-1
+  constexpr int B = 4;
+  double cf[B], cd[B], ch[B], nf[B], nd[B], nh[B];
+#pragma unroll
+  for (int m = 0; m < B; ++m) {
+    const int l = min(1 + m, last);
+    cf[m] = f[(unsigned)(l * NE)];
+    cd[m] = d[(unsigned)(l * NE)];
+    ch[m] = fh[l * fst];
+  }
+  double pprev = div_rn(d[0], t.dmu[0], t.rdmu[0]);  // D(1)/DMU(1)
+  double rk = 0.0, rl = -1.0;
+  f[0] = 0.0;
+  rlv[0] = -1.0;
+#pragma unroll 1
+  for (int l0 = 1; l0 <= last; l0 += B) {
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+      const int l = min(l0 + B + m, last);
+      nf[m] = f[(unsigned)(l * NE)];
+      nd[m] = d[(unsigned)(l * NE)];
+      nh[m] = fh[l * fst];
+    }
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+      const int l = l0 + m;
+      if (l > last) break;
+      const double fac = ch[m] * t.mu[l], rfac = 1.0 / fac;
+      const double wmu = t.wmu[l], rwmu = t.rwmu[l];
+      const double p = div_rn(cd[m], t.dmu[l], t.rdmu[l]);
+      const double an = div_rn(div_rn(p * dts, fac, rfac), wmu, rwmu);
+      const double gn = div_rn(div_rn(pprev * dts, fac, rfac), wmu, rwmu);
+      const double bn = an + gn;
+      const double rp = div_rn(cf[m], fac, rfac);
+      const double den = bn + gn * rl + 1;
+      const double rden = 1.0 / den;
+      rk = div_rn(rp + gn * rk, den, rden);
+      rl = div_rn(-an, den, rden);
+      f[(unsigned)(l * NE)] = rk;
+      rlv[(unsigned)(l * NE)] = rl;
+      pprev = p;
+    }
+#pragma unroll
+    for (int m = 0; m < B; ++m) {
+      cf[m] = nf[m];
+      cd[m] = nd[m];
+      ch[m] = nh[m];
+    }
+  }
+  // back substitution on the unscaled values, each row stored scaled
+  double v = rk / (1 + rl);
+  f[(unsigned)(last * NE)] = v * (fh[last * fst] * t.mu[last]);
+  f[(unsigned)((NPA - 1) * NE)] = v * (fh[(NPA - 1) * fst] * t.mu[NPA - 1]);
+#pragma unroll 4
+  for (int l = last - 1; l >= 0; --l) {
+    v = f[(unsigned)(l * NE)] - rlv[(unsigned)(l * NE)] * v;
+    f[(unsigned)(l * NE)] = v * (fh[l * fst] * t.mu[l]);
+  }
+}
+
 __global__ void k_copy_ens(Launch L) {
   for (int vb = blockIdx.x; vb < L.nv; vb += gridDim.x) {
     const Unit u = unit_of(L, vb);

@@ -34,6 +34,14 @@ constexpr int kMaxPlaneCells = kPlaneThreads * kMaxCellsPerThread;  // NE*NPA bo
 constexpr int kSumSlots = 12;
 constexpr int kSumRadial = 7;
 constexpr int kSumCoul = 8;
+// k_finalize's summode: bits 0-1 the mode (0 none, 1 one SUMRC, 2 the
+// fused step's chain); with mode 1, kSumFromRadial reads the radial slot
+// and bits kSumAccShift.. name the accumulator the ELORC goes to
+constexpr int kSumModeMask = 3;
+constexpr int kSumFromRadial = 1 << 2;
+constexpr int kSumAccShift = 4;
+enum SumAcc { SUM_NONE = 0, SUM_LSDR = 1, SUM_LSCHA = 2, SUM_LSATM = 3, SUM_LSWAE = 4, SUM_LSCOE = 5,
+              SUM_LSCSC = 6 };
 constexpr int kMuSpread = 4;           // LDS slots per pitch-angle row of DRIFTMU's CFL maximum (by K mod 4)
 constexpr int kFinalizeThreads = 1024;  // one block per (problem, species) of k_finalize (and of k_ghost)
 

@@ -301,6 +301,35 @@ module ramgpu_mod
        real(c_double), intent(out) :: coule(*), couli(*), ata(*), gta(*), cedr(*), cidr(*)
      end function ramgpu_get_coulomb_tables
 
+     ! WPADIF (D = d1 + d2, (NR,NT,NE,NPA)) and FLCscatter (FLC_coef(S,...))
+     integer(c_int) function ramgpu_set_wpadif(ctx, s, d1, d2, at_wpi, at_emic) &
+          bind(C, name="ramgpu_set_wpadif")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s, at_wpi, at_emic
+       real(c_double), intent(in) :: d1(*), d2(*)
+     end function ramgpu_set_wpadif
+
+     integer(c_int) function ramgpu_wpadif(ctx, s) bind(C, name="ramgpu_wpadif")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_wpadif
+
+     integer(c_int) function ramgpu_set_flc(ctx, s, coef, dt_bc) bind(C, name="ramgpu_set_flc")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+       real(c_double), intent(in) :: coef(*)
+       real(c_double), value :: dt_bc
+     end function ramgpu_set_flc
+
+     integer(c_int) function ramgpu_flcscatter(ctx, s) bind(C, name="ramgpu_flcscatter")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: s
+     end function ramgpu_flcscatter
+
      integer(c_int) function ramgpu_ram_run(ctx, nsteps, out) bind(C, name="ramgpu_ram_run")
        import :: c_int, c_ptr, ramgpu_species_out
        type(c_ptr), value :: ctx

@@ -40,10 +40,11 @@ EXPORTS = (
     "ramgpu_batch_kernel_timing", "ramgpu_batch_kernel_times",
     "ramgpu_set_coulomb", "ramgpu_set_necr", "ramgpu_set_time", "ramgpu_get_time", "ramgpu_coulpara",
     "ramgpu_get_coulomb_tables", "ramgpu_coulen", "ramgpu_coulmu",
+    "ramgpu_set_wpadif", "ramgpu_wpadif", "ramgpu_set_flc", "ramgpu_flcscatter",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
-                "moments")
+                "moments", "pitch")
 
 
 class SpeciesOut(C.Structure):
@@ -114,6 +115,10 @@ def load_library(path: Path | None = None):
     for n in ("coulpara", "coulen", "coulmu"):
         getattr(lib, "ramgpu_" + n).argtypes = [v, C.c_int]
     lib.ramgpu_get_coulomb_tables.argtypes = [v, C.c_int] + [_dp] * 6
+    lib.ramgpu_set_wpadif.argtypes = [v, C.c_int, _dp, _dp, C.c_int, C.c_int]
+    lib.ramgpu_wpadif.argtypes = [v, C.c_int]
+    lib.ramgpu_set_flc.argtypes = [v, C.c_int, _dp, C.c_double]
+    lib.ramgpu_flcscatter.argtypes = [v, C.c_int]
     lib.ramgpu_batch_kernel_timing.argtypes = [v, C.c_int]
     lib.ramgpu_batch_kernel_times.argtypes = [v, _dp, C.POINTER(C.c_int)]
     lib.ramgpu_last_run_ms.argtypes = [v, _dp]
@@ -248,6 +253,26 @@ class RamGpu:
 
     def coulen(self, s):
         self._check(self.lib.ramgpu_coulen(self.h, s))
+
+    def set_wpadif(self, s, d1, d2=None, at_wpi=True, at_emic=False):
+        """WPADIF coefficients of species s, D = d1 + d2 (NR,NT,NE,NPA);
+        d1 None switches WPADIF off for the species."""
+        n = self.nr * self.nt * self.ne * self.npa
+        a = _fd(d1, n) if d1 is not None else None
+        b = _fd(d2, n) if d2 is not None else None
+        self._check(self.lib.ramgpu_set_wpadif(self.h, s, _p(a) if a is not None else None,
+                                               _p(b) if b is not None else None, int(bool(at_wpi)),
+                                               int(bool(at_emic))))
+
+    def wpadif(self, s):
+        self._check(self.lib.ramgpu_wpadif(self.h, s))
+
+    def set_flc(self, s, coef, dt_bc=0.0):
+        a = _fd(coef, self.nr * self.nt * self.ne * self.npa) if coef is not None else None
+        self._check(self.lib.ramgpu_set_flc(self.h, s, _p(a) if a is not None else None, float(dt_bc)))
+
+    def flcscatter(self, s):
+        self._check(self.lib.ramgpu_flcscatter(self.h, s))
 
     def coulmu(self, s):
         self._check(self.lib.ramgpu_coulmu(self.h, s))

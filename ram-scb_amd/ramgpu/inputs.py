@@ -233,6 +233,22 @@ def plasmasphere_density(grid: RamGrid, kp: float) -> np.ndarray:
     return necr
 
 
+def pitch_diffusion_coef(grid: RamGrid, geom, scale: float = 1e-5, seed: int = 0) -> np.ndarray:
+    """A synthetic bounce-averaged pitch-angle diffusion coefficient
+    D(NR,NT,NE,NPA) of the form WPADIF and FLCscatter take (ATAW, ATAC,
+    ATAW_emic_*, FLC_coef; the reference reads or derives them from wave and
+    field-line-curvature models that are not part of this library):
+    scale * (1 - MU^2) * FNHS * MU * E^-1/2 (E in keV), with a seeded
+    log-normal texture, so that DTs*D/(DMU*FNHS*MU*WMU) is of order 0.1-1."""
+    rng = np.random.default_rng(seed)
+    mu = grid.mu[None, None, None, :]
+    fh = geom.fnhs[:grid.nr, :, None, :]
+    e = grid.ekev[None, None, :, None]
+    d = scale * (1.0 - mu * mu) * fh * mu / np.sqrt(e) * np.exp(0.3 * rng.standard_normal(
+        (grid.nr, grid.nt, grid.ne, grid.npa)))
+    return np.asfortranarray(d)
+
+
 @dataclass
 class Problem:
     """One member's full input set for the split step."""
