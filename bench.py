@@ -36,6 +36,8 @@ More objects in the same line:
   "multi_species_coulomb": the same run for a third of the steps with
     Coulomb collisions on (COULEN and COULMU twice a step in the plane
     kernel);
+  "multi_species_all_processes": a sixth of the steps with every optional
+    process on (Coulomb, e- WPADIF, H+ EMIC, He+ FLC): the composed step;
   "adjoint": H+ on 20x25x35x72, 60 steps, virtual-satellite misfit at 8
     points, gradient w.r.t. the initial F2 and every step's FGEOS with
     checkpoints every 10 steps, against the 60-step forward alone;
@@ -316,15 +318,17 @@ def bench_c3(members: int, steps: int, warmup: int, seed: int, seq_steps: int = 
             "ktimes": ktimes}
 
 
-def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = False):
+def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = False, pitch: bool = False):
     """The plain multi-species run on one context: Kp ramp 2->6, boundary
     scale LogNormal(0,0.3) per step; with coulomb, Coulomb collisions on
-    (synthetic plasmasphere of Kp=2, TimeRamElapsed from 0)."""
+    (synthetic plasmasphere of Kp=2, TimeRamElapsed from 0); with pitch also
+    WPADIF for e- in place of WAVELO, EMIC WPADIF for H+ and FLC scattering
+    for He+ (synthetic coefficients), which takes the composed step."""
     from ramgpu import inputs
     from ramgpu.capi import RamGpu, copy_probe
 
     problem = inputs.multi_species(seed=seed)
-    if coulomb:
+    if coulomb or pitch:
         problem.coulomb = inputs.CoulombInputs(inputs.plasmasphere_density(problem.grid, 2.0), 0.0)
     g = problem.grid
     ns = g.ns
@@ -332,6 +336,13 @@ def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = Fals
     cells_kl = ns * (g.nr - 1) * g.nt * g.ne * g.npa
     dev = RamGpu.from_problem(problem)
     dev.set_kp(2.0)
+    if pitch:
+        gr, geo = problem.grid, problem.geom
+        dev.set_wpadif(1, inputs.pitch_diffusion_coef(gr, geo, seed=seed + 11),
+                       inputs.pitch_diffusion_coef(gr, geo, scale=3e-6, seed=seed + 12), at_wpi=True)
+        dev.set_wpadif(2, inputs.pitch_diffusion_coef(gr, geo, scale=2e-6, seed=seed + 13), None,
+                       at_wpi=False, at_emic=True)
+        dev.set_flc(3, inputs.pitch_diffusion_coef(gr, geo, scale=1e-6, seed=seed + 14), 0.0)
     if warmup:
         kpw, bsw = schedule(warmup, seed + 100, ns)
         dev.ram_run_schedule(kpw, bsw)
@@ -347,6 +358,18 @@ def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = Fals
     ev_ms = dev.last_run_ms()
     ktimes = dev.kernel_times()
     dev.kernel_timing(False)
+    if pitch:  # one launch per operator: device time per step by kernel kind instead of a roofline
+        value = cells * steps / (dev_ms * 1e-3)
+        per = {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps} for k, v in ktimes.items() if v[1]}
+        log(f"bench: all processes {dev_ms / steps * 1e3:.1f} us/step, {value:.3e} cell-updates/s")
+        res = {"workload": "e-(WPADIF),H+(EMIC),He+(FLC),O+ (1:0.7:0.05:0.25) on 20x25x35x72 with Coulomb "
+                           "collisions, dt=5 s, Kp 2->6 ramp, boundary scale LogNormal(0,0.3) per step; the "
+                           "composed step (one launch per operator)",
+               "cells": cells, "steps": steps, "value": value, "unit": "cell-updates/s",
+               "ms_per_step": dev_ms / steps, "ms_per_step_with_kernel_events": ev_ms / steps,
+               "kernels": per}
+        dev.close()
+        return res, problem
     roof = per_kernel_roofline(ktimes, cells, cells_kl, TRAFFIC_FILE_MULTI)
     roof["traffic"] = None  # the committed counters are the headline's
     value = cells * steps / (dev_ms * 1e-3)
@@ -359,7 +382,7 @@ def bench_multi_species(steps: int, warmup: int, seed: int, coulomb: bool = Fals
            "step_algorithmic_GBps": ALGO_BYTES_PER_CELL_UPDATE * value / 1e9,
            "step_frac": ALGO_BYTES_PER_CELL_UPDATE * value / 1e9 / HBM_PEAK_GBPS, "roofline": roof,
            "copy_peak_GBps": copy_probe(1 << 30, 10)}
-    if not coulomb:
+    if not (coulomb or pitch):
         res["moments"] = bench_moments(dev, cells_kl)
     dev.close()
     return res, problem
@@ -486,6 +509,8 @@ def main():
     ap.add_argument("--multi-only", action="store_true",
                     help="run only the multi-species leg and print its object (profiling)")
     ap.add_argument("--coulomb", action="store_true", help="with --multi-only: Coulomb collisions on")
+    ap.add_argument("--all-processes", action="store_true",
+                    help="with --multi-only: every optional process on (the composed step)")
     ap.add_argument("--no-coulomb", action="store_true", help="skip the multi-species leg with Coulomb collisions")
     args = ap.parse_args()
 
@@ -500,7 +525,8 @@ def main():
     if args.steps < 1 or args.warmup < 0 or args.members < 2:
         ap.error("--steps must be >= 1, --warmup >= 0, --members >= 2")
     if args.multi_only:
-        print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed, args.coulomb)[0]), flush=True)
+        print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed, args.coulomb,
+                                             args.all_processes)[0]), flush=True)
         return
 
     c3 = bench_c3(args.members, args.steps, args.warmup, args.seed)
@@ -542,6 +568,9 @@ def main():
     if not args.no_coulomb:
         r = leg("multi_species_coulomb", lambda: bench_multi_species(args.multi_steps // 3, 5, args.seed, True))
         out["multi_species_coulomb"] = r[0] if isinstance(r, tuple) else r
+        r = leg("multi_species_all_processes",
+                lambda: bench_multi_species(args.multi_steps // 6, 5, args.seed, False, True))
+        out["multi_species_all_processes"] = r[0] if isinstance(r, tuple) else r
     if not args.no_adjoint:
         out["adjoint"] = leg("adjoint", lambda: bench_adjoint(args.seed))
         if args.ens_adjoint_members > 0:

@@ -262,7 +262,7 @@ int ramgpu_moments(ramgpu_ctx *ctx, double *ppert, double *ppart, double *edent,
  * Coulomb collisions (ramgpu_set_coulomb), WPADIF and EMIC
  * (ramgpu_set_wpadif), FLC scattering (ramgpu_set_flc).  Without WPADIF and
  * FLC the middle of the step is one fused plane kernel; with them the step
- * runs one launch per operator, species after species (the same results).
+ * runs one launch per operator for all species at once (the same results).
  * out (may be NULL) receives NS records for the last step. */
 int ramgpu_ram_run(ramgpu_ctx *ctx, int nsteps, ramgpu_species_out *out);
 

@@ -470,7 +470,7 @@ int set_plane_lds_limit(size_t limit) {
       PLANE_K(SEQ_COULEN), PLANE_K(SEQ_COULMU), PLANE_K(SEQ_STEPC),
       (const void *)k_r<0>, (const void *)k_r<1>, (const void *)k_r<2>,
       (const void *)k_r_bwd<0>, (const void *)k_r_bwd<1>, (const void *)k_p<0>, (const void *)k_p<1>,
-      (const void *)k_p_bwd};
+      (const void *)k_p_bwd, (const void *)k_pitch_diffusion};
 #undef PLANE_K
   for (const void *f : fns) {
     hipFuncAttributes fa;

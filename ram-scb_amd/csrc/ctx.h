@@ -17,7 +17,7 @@ __global__ void k_sched(Launch L, const double *kp, const double *bscale, const 
 __global__ void k_tick(int *step);
 __global__ void k_setstep(int *step, int v);
 __global__ void k_copy_ens(Launch L);
-__global__ void k_pitch_diffusion(const Tab *tp, int s, const double *coef, double *rlbuf);
+__global__ void k_pitch_diffusion(const Tab *tp, PitchJob job);
 template <int FINAL> __global__ void k_r_bwd(Launch L, const double *xin, double *adj, double *bbar);
 __global__ void k_p_bwd(Launch L, const double *xin, double *adj);
 template <int SEQ, int BT> __global__ void k_plane_bwd(Launch L, const double *xin, double *adj);
@@ -168,7 +168,7 @@ int coul_check_lds(const ramgpu_ctx *c, int threads);
 // scattering on: the reverse mode and ensembles step without them
 int forward_only(const ramgpu_ctx *c);
 // whether a species takes WPADIF or FLCscatter in ram_run, and the step that
-// then runs: one launch per operator, species after species, each finalized
+// then runs: one launch per operator for all species, finalized at its end
 int takes_pitch(const ramgpu_ctx *c);
 int step_composed(ramgpu_ctx *c, int rec);
 // host Fortran-layout F2 <-> device-layout buffer, through the context's stage

@@ -1678,23 +1678,28 @@ __global__ void k_setstep(int *step, int v) {
 // destination as the launch names them (tape slots or F2); each unit is one
 // problem, its nb blocks striding over it.
 // ---------------------------------------------------------------------------
-// Implicit pitch-angle diffusion of species s: WPADIF (src/ModRamWPI.f90:
-// 643-714) and FLCscatter (src/ModRamLoss.f90:513-575) are the same solve
-// with different coefficients D = coef (NR,NT,NE,NPA in F2's device layout
-// [I][J][L][K]).  One thread per (I>=2, J, K>=2) line, consecutive threads
-// on consecutive K, so each row of the lines of a wave is one contiguous
-// run.  In the reference's order: F = F2/FACMU with FACMU(L) =
-// FNHS(I,J,L)*MU(L); RK(1)=0, RL(1)=-1; for L = 2..NPA-1
+// Implicit pitch-angle diffusion of the species of job (blockIdx.y picks
+// one): WPADIF (src/ModRamWPI.f90:643-714) and FLCscatter
+// (src/ModRamLoss.f90:513-575) are the same solve with different
+// coefficients D = coef (NR,NT,NE,NPA in F2's device layout [I][J][L][K]).
+// One thread per (I>=2, J, K>=2) line, blocks of one wave,
+// consecutive threads on consecutive K, so each row of the lines of a wave
+// is one contiguous run.  In the reference's order: F = F2/FACMU with
+// FACMU(L) = FNHS(I,J,L)*MU(L); RK(1)=0, RL(1)=-1; for L = 2..NPA-1
 // AN = D(L)/DMU(L)*DTs/FACMU(L)/WMU(L), GN the same of D(L-1)/DMU(L-1),
 // DENOM = AN+GN+GN*RL(L-1)+1, RK = (F+GN*RK(L-1))/DENOM, RL = -AN/DENOM;
 // back substitution, F2(NPA) = F2(NPA-1), F2 = F*FACMU.  Every quotient is
 // correctly rounded: by DMU and WMU from their tabulated reciprocals, by
-// FACMU and DENOM from one reciprocal each per row (div_rn).  RK goes to
-// the line's own F2 cells and RL to rlbuf (F2's layout), both read back by
-// the substitution; D, F and FACMU of four rows are loaded ahead.
-__global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, const double *__restrict__ coef,
-                                                        double *__restrict__ rlbuf) {
+// FACMU and DENOM from one reciprocal each per row (div_rn).  RK and RL of
+// the line stay in LDS (2*NPA doubles per thread) for the substitution; F,
+// D and FNHS of the next eight rows are in flight while eight are solved:
+// the recurrence is serial, so its speed is the latency of each row.
+constexpr int kPitchAhead = 8;
+__global__ void __launch_bounds__(kPitchThreads) k_pitch_diffusion(const Tab *tp, PitchJob job) {
+  extern __shared__ double sm[];
   const Tab &t = *tp;
+  const int s = job.s[blockIdx.y];
+  const double *__restrict__ coef = job.coef[blockIdx.y];
   const int NE = t.ne, NPA = t.npa, NT = t.nt, NR = t.nr, nk = NE - 1;
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (NR - 1) * NT * nk) return;
@@ -1703,12 +1708,12 @@ __global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, co
   const size_t po = ((size_t)i * NT + j) * ((size_t)NE * NPA) + k;
   const gptr<double> f = as_global(t.f2 + (size_t)s * f2_species_stride(t) + po);
   const gptr<const double> d = as_global(coef + po);
-  const gptr<double> rlv = as_global(rlbuf + po);
   const gptr<const double> fh = as_global(&GEO3(t.fnhs, i + 1, j + 1, 1));
   const size_t fst = (size_t)(NR + 1) * NT;  // FNHS's stride along L
+  double *rkv = sm + threadIdx.x, *rlv = sm + (size_t)NPA * kPitchThreads + threadIdx.x;  // row l at [l*64]
   const double dts = t.dts;
   const int last = NPA - 2;                  // row of L = NPA-1
-  constexpr int B = 4;
+  constexpr int B = kPitchAhead;
   double cf[B], cd[B], ch[B], nf[B], nd[B], nh[B];
 #pragma unroll
   for (int m = 0; m < B; ++m) {
@@ -1719,7 +1724,7 @@ __global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, co
   }
   double pprev = div_rn(d[0], t.dmu[0], t.rdmu[0]);  // D(1)/DMU(1)
   double rk = 0.0, rl = -1.0;
-  f[0] = 0.0;
+  rkv[0] = 0.0;
   rlv[0] = -1.0;
 #pragma unroll 1
   for (int l0 = 1; l0 <= last; l0 += B) {
@@ -1745,8 +1750,8 @@ __global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, co
       const double rden = 1.0 / den;
       rk = div_rn(rp + gn * rk, den, rden);
       rl = div_rn(-an, den, rden);
-      f[(unsigned)(l * NE)] = rk;
-      rlv[(unsigned)(l * NE)] = rl;
+      rkv[l * kPitchThreads] = rk;
+      rlv[l * kPitchThreads] = rl;
       pprev = p;
     }
 #pragma unroll
@@ -1760,9 +1765,9 @@ __global__ void __launch_bounds__(64) k_pitch_diffusion(const Tab *tp, int s, co
   double v = rk / (1 + rl);
   f[(unsigned)(last * NE)] = v * (fh[last * fst] * t.mu[last]);
   f[(unsigned)((NPA - 1) * NE)] = v * (fh[(NPA - 1) * fst] * t.mu[NPA - 1]);
-#pragma unroll 4
+#pragma unroll 8
   for (int l = last - 1; l >= 0; --l) {
-    v = f[(unsigned)(l * NE)] - rlv[(unsigned)(l * NE)] * v;
+    v = rkv[l * kPitchThreads] - rlv[l * kPitchThreads] * v;
     f[(unsigned)(l * NE)] = v * (fh[l * fst] * t.mu[l]);
   }
 }

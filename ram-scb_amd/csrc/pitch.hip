@@ -1,8 +1,8 @@
 // pitch.hip -- the implicit pitch-angle diffusions of the split step, WPADIF
 // (src/ModRamWPI.f90:643-714) and FLCscatter (src/ModRamLoss.f90:513-575),
 // and the step that takes them: ram_run's species loop as one launch per
-// operator (src/ModRamRun.f90:66-183) where the fused plane kernel does not
-// hold the operator.
+// operator (src/ModRamRun.f90:66-183), for the operators the fused plane
+// kernel does not hold.
 #include <hip/hip_runtime.h>
 
 #include <vector>
@@ -19,6 +19,8 @@ using namespace ramgpu_host;
 // (Fortran (NR,NT,NE,NPA), d2 may be null), allocated on first use
 int upload_coef(ramgpu_ctx *c, double **slot, const double *d1, const double *d2) {
   const size_t n = (size_t)c->nr * c->nt * c->ne * c->npa;
+  if (pitch_lds_bytes(c->npa) > c->lds_limit)
+    return fail(RAMGPU_EINVAL, "NPA=%d: the pitch-angle line solve's LDS exceeds the device's", c->npa);
   if (!*slot) {
     void *p = nullptr;
     if (hipMalloc(&p, n * sizeof(double)) != hipSuccess)
@@ -38,20 +40,31 @@ int upload_coef(ramgpu_ctx *c, double **slot, const double *d1, const double *d2
   return RAMGPU_OK;
 }
 
-// the line solve of species s (0-based) with coefficient table coef
-int run_pitch(ramgpu_ctx *c, int s, const double *coef) {
+// the line solves of the species of job in one launch
+int run_pitch(ramgpu_ctx *c, const PitchJob &job) {
+  if (job.n == 0) return RAMGPU_OK;
   const int lines = (c->nr - 1) * c->nt * (c->ne - 1);
   tbeg(c, RAMGPU_KERNEL_PITCH);
-  hipLaunchKernelGGL(k_pitch_diffusion, dim3((lines + 63) / 64), dim3(64), 0, c->stream, c->d_tab, s, coef,
-                     c->d_stage);
+  hipLaunchKernelGGL(k_pitch_diffusion, dim3((lines + kPitchThreads - 1) / kPitchThreads, job.n),
+                     dim3(kPitchThreads), pitch_lds_bytes(c->npa), c->stream, c->d_tab, job);
   tend(c);
   return launch_check("k_pitch_diffusion");
 }
 
-// SUMRC of species s, its ELORC booked to accumulator acc
-int sumrc_into(ramgpu_ctx *c, int s, int acc) {
-  int rc = run_plane(c, SEQ_SUM, 0, s, 1);
-  return rc ? rc : run_finalize(c, 0, 1 | (acc << kSumAccShift), s, 1);
+int run_pitch(ramgpu_ctx *c, int s, const double *coef) {
+  PitchJob job{};
+  job.n = 1;
+  job.s[0] = s;
+  job.coef[0] = coef;
+  return run_pitch(c, job);
+}
+
+// SUMRC of every species, each ELORC booked to accumulator acc (a species
+// whose F2 the operator before left alone books exactly 0, its sum being
+// the same as at its previous SUMRC)
+int sumrc_into(ramgpu_ctx *c, int acc) {
+  int rc = run_plane(c, SEQ_SUM, 0, 0, c->ns);
+  return rc ? rc : run_finalize(c, 0, 1 | (acc << kSumAccShift), 0, c->ns);
 }
 
 }  // namespace
@@ -72,68 +85,89 @@ int forward_only(const ramgpu_ctx *c) {
   return RAMGPU_OK;
 }
 
-// One ram_run step with one launch per operator, species after species,
-// in the order of src/ModRamRun.f90:66-183 and with its SUMRC bookkeeping;
-// each species ends with DRIFTR's SUMRC, seam copy and magnetopause mask
-// and its DtDrift* (the CFL maxima recorded by its first sweeps when rec).
+// One ram_run step with one launch per operator (src/ModRamRun.f90:66-183):
+// the species' chains are independent, so each operator runs for every
+// species that takes it at once, in the reference's order, and each SUMRC
+// for every species (booking 0 where the operator did not run).  The chain
+// ends with DRIFTR's SUMRC, the seam copy, the magnetopause mask and
+// DtDrift* (the CFL maxima recorded by the first sweeps when rec).
 int step_composed(ramgpu_ctx *c, int rec) {
   int rc = RAMGPU_OK;
+  const int ns = c->ns;
   const bool coul = c->t.coul != 0;
   const int clamp = (coul && c->time_elapsed > 0.) ? 2 : 0;
+  PitchJob wpd{}, emic{}, flc{};
+  bool wave[kMaxSpecies] = {}, any_wave = false, any_cex = false;
+  for (int s = 0; s < ns; ++s) {
+    if (c->pd_wpi[s]) {
+      wpd.s[wpd.n] = s;
+      wpd.coef[wpd.n++] = c->d_pdc[s];
+    } else if (c->t.wpi[s]) {
+      wave[s] = any_wave = true;
+    }
+    if (c->pd_emic[s]) {
+      emic.s[emic.n] = s;
+      emic.coef[emic.n++] = c->d_pdc[s];
+    }
+    if (c->d_flc[s] && c->time_elapsed >= c->dt_bc) {
+      flc.s[flc.n] = s;
+      flc.coef[flc.n++] = c->d_flc[s];
+    }
+    any_cex = any_cex || c->t.cex[s] != 0;
+  }
 #define TRY(x) \
   if ((rc = (x))) return rc
-  for (int s = 0; s < c->ns; ++s) {
-    const bool wave = c->t.wpi[s] != 0, wpd = c->pd_wpi[s] != 0, emic = c->pd_emic[s] != 0;
-    const bool cex = c->t.cex[s] != 0, flc = c->d_flc[s] && c->time_elapsed >= c->dt_bc;
-    // [WAVELO or WPADIF], [EMIC], [CEX], [FLC]: the loss half, once forward
-    // and once reversed around the two ATMOLs
-    auto half = [&](bool forward) -> int {
-      for (int q = 0; q < 4; ++q) {
-        const int op = forward ? q : 3 - q;
-        if (op == 0 && (wave || wpd)) {
-          TRY(wpd ? run_pitch(c, s, c->d_pdc[s]) : run_plane(c, SEQ_WAVE, 0, s, 1));
-          TRY(sumrc_into(c, s, SUM_LSWAE));
-        } else if (op == 1 && emic) {
-          TRY(run_pitch(c, s, c->d_pdc[s]));
-          TRY(sumrc_into(c, s, SUM_LSWAE));
-        } else if (op == 2 && cex) {
-          TRY(run_plane(c, SEQ_CEX, 0, s, 1));
-          TRY(sumrc_into(c, s, SUM_LSCHA));
-        } else if (op == 3 && flc) {
-          TRY(run_pitch(c, s, c->d_flc[s]));  // no SUMRC after FLCscatter
-        }
+  // [WAVELO or WPADIF], [EMIC], [CEX], [FLC]: the loss half, forward and
+  // then reversed around the two ATMOLs; CHAREXCHANGE leaves species
+  // without CEX alone, so it runs for all
+  auto half = [&](bool forward) -> int {
+    for (int q = 0; q < 4; ++q) {
+      const int op = forward ? q : 3 - q;
+      if (op == 0 && (any_wave || wpd.n)) {
+        for (int s = 0; s < ns; ++s)
+          if (wave[s]) TRY(run_plane(c, SEQ_WAVE, 0, s, 1));
+        TRY(run_pitch(c, wpd));
+        TRY(sumrc_into(c, SUM_LSWAE));
+      } else if (op == 1 && emic.n) {
+        TRY(run_pitch(c, emic));
+        TRY(sumrc_into(c, SUM_LSWAE));
+      } else if (op == 2 && any_cex) {
+        TRY(run_plane(c, SEQ_CEX, 0, 0, ns));
+        TRY(sumrc_into(c, SUM_LSCHA));
+      } else if (op == 3 && flc.n) {
+        TRY(run_pitch(c, flc));  // no SUMRC after FLCscatter
       }
-      return RAMGPU_OK;
-    };
-    TRY(run_r(c, rec, s, 1));
-    TRY(run_p(c, rec, s, 1));
-    TRY(run_plane(c, SEQ_E, rec, s, 1));
-    TRY(run_plane(c, SEQ_MU, rec, s, 1));
-    TRY(sumrc_into(c, s, SUM_LSDR));
-    if (coul) {
-      TRY(run_plane(c, SEQ_COULEN, 0, s, 1));
-      TRY(sumrc_into(c, s, SUM_LSCOE));
-      TRY(run_plane(c, SEQ_COULMU, clamp, s, 1));
-      TRY(sumrc_into(c, s, SUM_LSCSC));
     }
-    TRY(half(true));
-    for (int a = 0; a < 2; ++a) {
-      TRY(run_plane(c, SEQ_ATM, 0, s, 1));
-      TRY(sumrc_into(c, s, SUM_LSATM));
-    }
-    TRY(half(false));
-    if (coul) {
-      TRY(run_plane(c, SEQ_COULMU, clamp, s, 1));
-      TRY(sumrc_into(c, s, SUM_LSCSC));
-      TRY(run_plane(c, SEQ_COULEN, 0, s, 1));
-      TRY(sumrc_into(c, s, SUM_LSCOE));
-    }
-    TRY(run_plane(c, SEQ_MU, 0, s, 1));
-    TRY(run_plane(c, SEQ_E, 0, s, 1));
-    TRY(run_p(c, 0, s, 1));
-    TRY(run_r(c, 2, s, 1));
-    TRY(run_finalize(c, rec ? 15 : 0, 1 | kSumFromRadial | (SUM_LSDR << kSumAccShift), s, 1));
+    return RAMGPU_OK;
+  };
+  TRY(run_r(c, rec, 0, ns));
+  TRY(run_p(c, rec, 0, ns));
+  TRY(run_plane(c, SEQ_E, rec, 0, ns));
+  TRY(run_plane(c, SEQ_MU, rec, 0, ns));
+  TRY(sumrc_into(c, SUM_LSDR));
+  if (coul) {
+    TRY(run_plane(c, SEQ_COULEN, 0, 0, ns));
+    TRY(sumrc_into(c, SUM_LSCOE));
+    TRY(run_plane(c, SEQ_COULMU, clamp, 0, ns));
+    TRY(sumrc_into(c, SUM_LSCSC));
   }
+  TRY(half(true));
+  for (int a = 0; a < 2; ++a) {
+    TRY(run_plane(c, SEQ_ATM, 0, 0, ns));
+    TRY(sumrc_into(c, SUM_LSATM));
+  }
+  TRY(half(false));
+  if (coul) {
+    TRY(run_plane(c, SEQ_COULMU, clamp, 0, ns));
+    TRY(sumrc_into(c, SUM_LSCSC));
+    TRY(run_plane(c, SEQ_COULEN, 0, 0, ns));
+    TRY(sumrc_into(c, SUM_LSCOE));
+  }
+  TRY(run_plane(c, SEQ_MU, 0, 0, ns));
+  TRY(run_plane(c, SEQ_E, 0, 0, ns));
+  TRY(run_p(c, 0, 0, ns));
+  TRY(run_r(c, 2, 0, ns));
+  TRY(run_finalize(c, rec ? 15 : 0, 1 | kSumFromRadial | (SUM_LSDR << kSumAccShift), 0, ns));
 #undef TRY
   return RAMGPU_OK;
 }

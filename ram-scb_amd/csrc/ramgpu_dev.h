@@ -249,6 +249,16 @@ inline __host__ __device__ size_t plane_fwd_lds_bytes(int ne, int npa, int threa
 inline __host__ __device__ size_t plane_coul_lds_bytes(int ne, int npa, int threads) {
   return plane_fwd_lds_bytes(ne, npa, threads) + (7 * (size_t)npa + (size_t)ne * npa) * sizeof(double);
 }
+// LDS of the pitch-angle line solve (k_pitch_diffusion): RK and RL of each
+// of its 64 lines
+constexpr int kPitchThreads = 64;
+// the species of one k_pitch_diffusion launch and their coefficient tables
+struct PitchJob {
+  int n;
+  int s[kMaxSpecies];
+  const double *coef[kMaxSpecies];
+};
+inline __host__ __device__ size_t pitch_lds_bytes(int npa) { return (size_t)2 * npa * kPitchThreads * sizeof(double); }
 // the adjoint kernels record no CFL maxima: their adjoint plane takes the
 // place of the CFL slots (NE + mu_slots*NPA doubles)
 inline __host__ __device__ size_t plane_bwd_lds_bytes(int ne, int npa) {

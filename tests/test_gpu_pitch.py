@@ -6,7 +6,7 @@ The line solve is built of correctly rounded operations in the reference's
 order (quotients through div_rn and exact reciprocals), so the operators
 equal the oracle bit for bit.  ram_run with WPADIF (electrons, at the WPI
 positions), EMIC WPADIF (H+), FLC scattering (He+) and Coulomb collisions
-runs one launch per operator; it matches the oracle's ram_run to the stated
+runs one launch per operator for all species; it matches the oracle's ram_run to the stated
 1e-12 (the loss factors' table-driven exponential), step after step, with
 the LS* accumulators to 1e-12 of SETRC.  A context whose FLC never acts
 (Dt_bc beyond the run) takes that composed step and must give the fused
