@@ -356,6 +356,32 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 // (CELL_KL_T).  The adjoint of the input is assembled in H and written back
 // over adj.
 
+// The partial scatter of the plane adjoints.  Each face sends partials to
+// four cells at offsets -1..+2 (times stride) from its own cell base:
+// part[o] for offset o-1, summed per face in slot order (add_partial).
+// scatter_rounds adds them into H in four rounds, one offset per round,
+// with a barrier after each: within a round every face adds into a distinct
+// cell, so no atomics are needed and each cell's sum comes in a fixed order
+// (by the owners' face groups, then by offset), the same on every run.
+// base < 0: no face.  Every thread of the block runs the rounds.
+constexpr int kScatterFaces = 1;  // faces per thread per group of rounds
+__device__ __forceinline__ void add_partial(double (&part)[4], int o, double v) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (o == q) part[q] += v;
+}
+template <int G>
+__device__ __forceinline__ void scatter_rounds(double *H, const double (&part)[G][4], const int (&base)[G],
+                                               int stride) {
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (base[g] >= 0 && part[g][o] != 0.0) H[base[g] + (o - 1) * stride] += part[g][o];
+    __syncthreads();
+  }
+}
+
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)
 // or f0, fnm1 = fp1 (C<0) or F(K-1) / ghosts F(0), F(1).
@@ -448,25 +474,36 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
+  // face K's partials go to cells K-1..K+2 of its row (the ghosts' to cell
+  // 2): scatter_rounds adds them without atomics, in a fixed order
 #pragma unroll 1
-  for (int n = 0; n < 8; ++n) {
-    if (n < c.nmine) {
-      CELL_KL_T(n, BT);
-      double ce;
-      FaceD D;
-      int tg[4];
-      double gf[4];
-      eface(k, l, ce, D, tg, gf);
-      double fbb = (k >= 1) ? -div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * Y[cc] : 0.0;
-      if (k + 1 < NE) fbb += div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * Y[cc + 1];
-      if (fbb != 0.0) {
+  for (int n0 = 0; n0 < c.nmax; n0 += kScatterFaces) {
+    double part[kScatterFaces][4];
+    int base[kScatterFaces];
+#pragma unroll
+    for (int g = 0; g < kScatterFaces; ++g) {
+      const int n = n0 + g;
+      base[g] = -1;
+#pragma unroll
+      for (int o = 0; o < 4; ++o) part[g][o] = 0.0;
+      if (n < c.nmine) {
+        CELL_KL_T(n, BT);
+        double ce;
+        FaceD D;
+        int tg[4];
+        double gf[4];
+        eface(k, l, ce, D, tg, gf);
+        double fbb = (k >= 1) ? -div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * Y[cc] : 0.0;
+        if (k + 1 < NE) fbb += div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * Y[cc + 1];
+        base[g] = cc;
         const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
+#pragma unroll
         for (int m = 0; m < 4; ++m)
-          if (tg[m] >= 0 && pd[m] != 0.0) atomicAdd(&d.H[l * NE + tg[m]], pd[m] * gf[m] * fbb);
+          if (tg[m] >= 0) add_partial(part[g], tg[m] - k + 1, pd[m] * gf[m] * fbb);
       }
     }
+    scatter_rounds<kScatterFaces>(d.H, part, base, 1);
   }
-  __syncthreads();
 }
 
 // DRIFTMU adjoint (forward: plane_mu).  Updated cells L=2..NPA-1 from faces
@@ -553,26 +590,38 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
+  // face L's partials go to rows L-1..L+2 of its column: scatter_rounds
+  // adds them without atomics, in a fixed order
 #pragma unroll 1
-  for (int n = 0; n < 8; ++n) {
-    if (n < c.nmine) {
-      CELL_KL_T(n, BT);
-      if (l >= 1 && l <= NPA - 2) {
-        double cm;
-        FaceD D;
-        int tg[4];
-        mface(k, l, cm, D, tg);
-        double fbb = -div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * Y[cc];
-        if (l + 1 <= NPA - 2) fbb += div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * Y[cc + NE];
-        if (fbb != 0.0) {
+  for (int n0 = 0; n0 < c.nmax; n0 += kScatterFaces) {
+    double part[kScatterFaces][4];
+    int base[kScatterFaces];
+#pragma unroll
+    for (int g = 0; g < kScatterFaces; ++g) {
+      const int n = n0 + g;
+      base[g] = -1;
+#pragma unroll
+      for (int o = 0; o < 4; ++o) part[g][o] = 0.0;
+      if (n < c.nmine) {
+        CELL_KL_T(n, BT);
+        if (l >= 1 && l <= NPA - 2) {
+          double cm;
+          FaceD D;
+          int tg[4];
+          mface(k, l, cm, D, tg);
+          double fbb = -div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * Y[cc];
+          if (l + 1 <= NPA - 2)
+            fbb += div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * Y[cc + NE];
+          base[g] = cc;
           const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
+#pragma unroll
           for (int m = 0; m < 4; ++m)
-            if (tg[m] >= 0 && pd[m] != 0.0) atomicAdd(&d.H[tg[m] * NE + k], pd[m] * fbb);
+            if (tg[m] >= 0) add_partial(part[g], tg[m] - l + 1, pd[m] * fbb);
         }
       }
     }
+    scatter_rounds<kScatterFaces>(d.H, part, base, NE);
   }
-  __syncthreads();
 }
 
 // Loss operators: F2 *= factor on their cell ranges, so the adjoint is

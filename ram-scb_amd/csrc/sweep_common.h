@@ -468,7 +468,7 @@ __device__ __forceinline__ PlaneLds plane_lds(double *sm, int ne, int npa) {
 }
 
 struct PlaneCtx {
-  int s, i, j, NE, NPA, NT, P, nmine;
+  int s, i, j, NE, NPA, NT, P, nmine, nmax;  // nmax: the most cells any thread of the block owns
   int cex, upa;  // the species' charge-exchange kind, UPA(I)
   size_t ij;
   double rlz, rrlz, qs, beta;
@@ -677,6 +677,7 @@ __device__ inline void plane_setup(const Tab &t, const Unit &u, double *sm, cons
   c.beta = t.betalim;
   c.rne = t.rne;
   c.nmine = (c.P - tid + T - 1) / T;  // cells tid, tid+blockDim, .. < P
+  c.nmax = (c.P + T - 1) / T;
   c.lestep = T / NE;
   c.le0 = fast_div(tid, NE, c.rne);
   c.ke = tid - c.le0 * NE;
