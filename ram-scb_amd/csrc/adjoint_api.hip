@@ -103,6 +103,48 @@ struct DevBuf {
   }
 };
 
+// an allocation the caller can do without (no error is recorded when it fails)
+template <class T>
+bool try_alloc(DevBuf &b, T **out, size_t n) {
+  void *q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  b.p.push_back(q);
+  *out = (T *)q;
+  return true;
+}
+
+// The second stream of ramgpu_adjoint_run and the events that order the two
+// streams: fwd (the forward pass done), rec[g%2] (segment g recomputed),
+// bwd[g%2] (segment g's adjoint done).  Declared after the run's buffers, so
+// that on every exit the second stream has drained before they are freed.
+struct Overlap {
+  ramgpu_ctx *c;
+  hipEvent_t fwd = nullptr, rec[2] = {nullptr, nullptr}, bwd[2] = {nullptr, nullptr};
+  explicit Overlap(ramgpu_ctx *cc) : c(cc) {}
+  bool init() {
+    if (!c->stream2 && hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) {
+      c->stream2 = nullptr;
+      (void)hipGetLastError();
+      return false;
+    }
+    for (hipEvent_t *e : {&fwd, &rec[0], &rec[1], &bwd[0], &bwd[1]})
+      if (hipEventCreateWithFlags(e, hipEventDisableTiming) != hipSuccess) {
+        *e = nullptr;
+        (void)hipGetLastError();
+        return false;
+      }
+    return true;
+  }
+  ~Overlap() {
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    for (hipEvent_t e : {fwd, rec[0], rec[1], bwd[0], bwd[1]})
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
 int fill(ramgpu_ctx *c, double *y, double v, size_t n) {
   hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, c->stream, y, v, n);
   return launch_check("k_fill");
@@ -291,11 +333,12 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   const size_t n = nf2(c), ng = nfg(c);
   const int nseg = (nsteps + ckpt_every - 1) / ckpt_every;
   const int seglen = std::min(ckpt_every, nsteps);
+  const size_t tape_n = (size_t)seglen * T_N * n;
   DevBuf b;
   double *ckpt, *tape, *lam, *bbar = nullptr, *dw, *dtarget, *lossv, *scratch_loss, *final_f2, *final_e;
   double *d_kp = nullptr, *d_bs = nullptr;
   int *dpts;
-  if ((rc = b.alloc(&ckpt, (size_t)nseg * n)) || (rc = b.alloc(&tape, (size_t)seglen * T_N * n)) ||
+  if ((rc = b.alloc(&ckpt, (size_t)nseg * n)) || (rc = b.alloc(&tape, tape_n)) ||
       (rc = b.alloc(&lam, n)) || (rc = b.alloc(&dw, (size_t)npts * c->npa)) ||
       (rc = b.alloc(&dtarget, (size_t)nsteps * npts)) || (rc = b.alloc(&lossv, (size_t)nsteps * npts)) ||
       (rc = b.alloc(&scratch_loss, npts)) || (rc = b.alloc(&final_f2, n)) ||
@@ -320,9 +363,25 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
                        lm);
     return launch_check("k_misfit");
   };
-  // the input of step s of segment g's tape (s in the segment) and the output of step s
-  auto tape_in = [&](int s, int n0) { return tape + (size_t)(s - n0) * T_N * n + (size_t)T_R1 * n; };
-  auto step_out = [&](int s, int n0, int n1) { return (s == n1 - 1) ? c->t.f2 : tape_in(s + 1, n0); };
+  // Two tapes when the step tables stay fixed over the run (no Kp or
+  // boundary-scale schedule): segment g-1 is then recomputed into one of
+  // them on the second stream while segment g's adjoint runs from the
+  // other.  The two streams share only the tables, which neither writes,
+  // and run the kernels of the serial order on the same data, so the
+  // results are the same bits.  Without the memory for a second tape, or
+  // with a schedule, the segments go one after the other on one stream.
+  Overlap ov(c);
+  double *tapes[2] = {tape, tape}, *yend[2] = {nullptr, nullptr};
+  const bool overlap = !kp && !bscale && nseg >= 2 && ov.init() && try_alloc(b, &tapes[1], tape_n) &&
+                       try_alloc(b, &yend[0], n) && try_alloc(b, &yend[1], n);
+  auto buf = [&](int g) { return overlap ? g % 2 : 0; };
+  // step s of segment g (steps n0..n1-1): its tape, the input it holds, its output
+  auto tape_at = [&](int s, int g) { return tapes[buf(g)] + (size_t)(s - g * ckpt_every) * T_N * n; };
+  auto tape_in = [&](int s, int g) { return tape_at(s, g) + (size_t)T_R1 * n; };
+  auto step_out = [&](int s, int g, int n1) -> double * {
+    if (s < n1 - 1) return tape_in(s + 1, g);
+    return (overlap && g < nseg - 1) ? yend[buf(g)] : c->t.f2;  // the next segment's checkpoint
+  };
   const int nlast0 = (nseg - 1) * ckpt_every;
   HIP_TRY(hipEventRecord(c->ev0, c->stream));
   // forward, keeping a checkpoint at the start of every segment; the last
@@ -342,10 +401,10 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
         c->fin_mode = 2;
         if ((r = misfit(c->t.f2, s, lossv + (size_t)s * npts, nullptr))) return r;
       } else {
-        if (s == nlast0 && (r = copy_d2d(c, tape_in(s, nlast0), c->t.f2, n))) return r;
-        double *y = step_out(s, nlast0, nsteps);
-        if ((r = taped_step(c, tape + (size_t)(s - nlast0) * T_N * n, y)) ||
-            (r = misfit(y, s, lossv + (size_t)s * npts, nullptr)))
+        const int g = nseg - 1;
+        if (s == nlast0 && (r = copy_d2d(c, tape_in(s, g), c->t.f2, n))) return r;
+        double *y = step_out(s, g, nsteps);
+        if ((r = taped_step(c, tape_at(s, g), y)) || (r = misfit(y, s, lossv + (size_t)s * npts, nullptr)))
           return r;
       }
     }
@@ -356,25 +415,53 @@ int ramgpu_adjoint_run(ramgpu_ctx *c, int nsteps, int ckpt_every, const double *
   if (rc == RAMGPU_OK) rc = rf;
   if (rc || (rc = copy_d2d(c, final_f2, c->t.f2, n)) || (rc = copy_d2d(c, final_e, c->t.energy, (size_t)c->ns * 4)))
     return rc;
-  // backward, segment by segment: recompute the segment with its tape, then
-  // walk its steps in reverse
+  // backward, segment by segment: recompute the segment with its tape (on
+  // the context's current stream), then walk its steps in reverse
   if ((rc = fill(c, lam, 0.0, n))) return rc;
   if (bbar && (rc = fill(c, bbar, 0.0, (size_t)nsteps * ng))) return rc;
-  for (int g = nseg - 1; g >= 0; --g) {
+  auto recompute = [&](int g) -> int {
     const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
-    if (g < nseg - 1) {  // recompute the segment with its tape
-      if ((rc = copy_d2d(c, tape, ckpt + (size_t)g * n, n))) return rc;  // slot T_R1 of the first step
-      for (int s = n0; s < n1; ++s)
-        if ((rc = set_step_inputs(c, d_kp, d_bs, s)) ||
-            (rc = taped_step(c, tape + (size_t)(s - n0) * T_N * n, step_out(s, n0, n1), false)))
-          return rc;
-    }
-    for (int s = n1 - 1; s >= n0; --s) {
-      const double *y = step_out(s, n0, n1);
-      if ((rc = misfit(y, s, scratch_loss, lam)) || (rc = set_step_inputs(c, d_kp, d_bs, s)) ||
-          (rc = prepare(c)) ||
-          (rc = step_bwd(c, tape + (size_t)(s - n0) * T_N * n, lam, bbar ? bbar + (size_t)s * ng : nullptr)))
-        return rc;
+    int r;
+    if ((r = copy_d2d(c, tape_in(n0, g), ckpt + (size_t)g * n, n))) return r;
+    for (int s = n0; s < n1; ++s)
+      if ((r = set_step_inputs(c, d_kp, d_bs, s)) || (r = taped_step(c, tape_at(s, g), step_out(s, g, n1), false)))
+        return r;
+    return RAMGPU_OK;
+  };
+  auto backward = [&](int g) -> int {
+    const int n0 = g * ckpt_every, n1 = std::min(nsteps, n0 + ckpt_every);
+    int r;
+    for (int s = n1 - 1; s >= n0; --s)
+      if ((r = misfit(step_out(s, g, n1), s, scratch_loss, lam)) || (r = set_step_inputs(c, d_kp, d_bs, s)) ||
+          (r = prepare(c)) || (r = step_bwd(c, tape_at(s, g), lam, bbar ? bbar + (size_t)s * ng : nullptr)))
+        return r;
+    return RAMGPU_OK;
+  };
+  if (!overlap) {
+    for (int g = nseg - 1; g >= 0; --g)
+      if ((g < nseg - 1 && (rc = recompute(g))) || (rc = backward(g))) return rc;
+  } else {
+    // recompute(g) on the second stream once `after` has passed; its tape is
+    // that of segment g+2, whose adjoint must be done
+    auto on_side = [&](int g, hipEvent_t after) -> int {
+      HIP_TRY(hipStreamWaitEvent(c->stream2, after, 0));
+      hipStream_t main = c->stream;
+      c->stream = c->stream2;
+      const int r = recompute(g);
+      c->stream = main;
+      if (r) return r;
+      HIP_TRY(hipEventRecord(ov.rec[g % 2], c->stream2));
+      return RAMGPU_OK;
+    };
+    HIP_TRY(hipEventRecord(ov.fwd, c->stream));
+    if ((rc = on_side(nseg - 2, ov.fwd))) return rc;
+    for (int g = nseg - 1; g >= 0; --g) {
+      if (g < nseg - 1) HIP_TRY(hipStreamWaitEvent(c->stream, ov.rec[g % 2], 0));
+      if ((rc = backward(g))) return rc;
+      if (g >= 2) {
+        HIP_TRY(hipEventRecord(ov.bwd[g % 2], c->stream));
+        if ((rc = on_side(g - 2, ov.bwd[g % 2]))) return rc;
+      }
     }
   }
   HIP_TRY(hipEventRecord(c->ev1, c->stream));

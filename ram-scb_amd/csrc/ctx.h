@@ -55,6 +55,10 @@ struct ramgpu_ctx {
   int device = 0;
   int ns = 0, nr = 0, nt = 0, ne = 0, npa = 0;
   hipStream_t stream = nullptr;
+  // a second stream (default priority) on which ramgpu_adjoint_run
+  // recomputes a segment while the previous one's adjoint runs; created by
+  // its first use
+  hipStream_t stream2 = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
   size_t lds_limit = 65536;

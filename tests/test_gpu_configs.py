@@ -95,6 +95,25 @@ def _c4_problem(nsteps, seed=70):
     return p, pts, w, target, kp, bs
 
 
+def test_adjoint_60_steps_fixed_field_overlapped_segments():
+    # C4 as the bench runs it (fixed field and boundary): the segments are
+    # recomputed on a second stream while the adjoint of the next runs, and
+    # every checkpoint interval gives the bits of the one-segment run
+    nsteps = 60
+    p, pts, w, target, _, _ = _c4_problem(nsteps, seed=72)
+    g = RamGpu.from_problem(p)
+    res = {}
+    for ck in (60, 10, 7):
+        g.put_f2(p.f2)
+        res[ck] = g.adjoint_run(nsteps, pts, w, target, ckpt_every=ck)
+        print(f"ckpt {ck}: loss {res[ck][0]:.15e}", flush=True)
+    for ck in (10, 7):
+        assert res[ck][0] == res[60][0], ck
+        assert np.array_equal(res[ck][1], res[60][1]), ck
+        assert np.array_equal(res[ck][2], res[60][2]), ck
+    g.close()
+
+
 def test_adjoint_one_step_dot_product_at_20x25x35x72():
     # <ybar, J v> by central differences of the oracle's split step against
     # <ram_run_bwd(ybar), v>, normalised by |ybar| |J v|; the step is
