@@ -506,6 +506,8 @@ def main():
     ap.add_argument("--ens-adjoint-members", type=int, default=8,
                     help="members of the high-resolution ensemble adjoint (0: skip)")
     ap.add_argument("--ens-adjoint-steps", type=int, default=120)
+    ap.add_argument("--ens-adjoint-only", action="store_true",
+                    help="only the ensemble adjoint leg (--ens-adjoint-members, --ens-adjoint-steps), its JSON")
     ap.add_argument("--multi-only", action="store_true",
                     help="run only the multi-species leg and print its object (profiling)")
     ap.add_argument("--coulomb", action="store_true", help="with --multi-only: Coulomb collisions on")
@@ -524,6 +526,10 @@ def main():
         return
     if args.steps < 1 or args.warmup < 0 or args.members < 2:
         ap.error("--steps must be >= 1, --warmup >= 0, --members >= 2")
+    if args.ens_adjoint_only:
+        r = bench_batched_adjoint(max(args.ens_adjoint_members, 1), args.ens_adjoint_steps, 20, args.seed)
+        print(json.dumps({k: v for k, v in r.items() if k != "loss"}), flush=True)
+        return
     if args.multi_only:
         print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed, args.coulomb,
                                              args.all_processes)[0]), flush=True)
