@@ -357,30 +357,11 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 // over adj.
 
 // The partial scatter of the plane adjoints.  Each face sends partials to
-// four cells at offsets -1..+2 (times stride) from its own cell base:
-// part[o] for offset o-1, summed per face in slot order (add_partial).
-// scatter_rounds adds them into H in four rounds, one offset per round,
-// with a barrier after each: within a round every face adds into a distinct
-// cell, so no atomics are needed and each cell's sum comes in a fixed order
-// (by the owners' face groups, then by offset), the same on every run.
-// base < 0: no face.  Every thread of the block runs the rounds.
-constexpr int kScatterFaces = 1;  // faces per thread per group of rounds
-__device__ __forceinline__ void add_partial(double (&part)[4], int o, double v) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q)
-    if (o == q) part[q] += v;
-}
-template <int G>
-__device__ __forceinline__ void scatter_rounds(double *H, const double (&part)[G][4], const int (&base)[G],
-                                               int stride) {
-#pragma unroll
-  for (int o = 0; o < 4; ++o) {
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (base[g] >= 0 && part[g][o] != 0.0) H[base[g] + (o - 1) * stride] += part[g][o];
-    __syncthreads();
-  }
-}
+// the cells at offsets -1..+2 along its sweep.  The faces go in four passes
+// by their index mod 4 along the sweep, with a barrier after each: two faces
+// of a pass are four or more cells apart, so their targets are disjoint and
+// each adds into H without atomics, and every cell takes its partials in the
+// order of the passes (the same on every run and for every launch shape).
 
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)
@@ -475,34 +456,33 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
   // face K's partials go to cells K-1..K+2 of its row (the ghosts' to cell
-  // 2): scatter_rounds adds them without atomics, in a fixed order
+  // 2).  The faces go in four passes by K mod 4, each ending in a barrier:
+  // the targets of two faces of a pass lie in disjoint spans of four cells,
+  // so no two faces add to one cell at once, and every cell takes its
+  // partials in the order of the passes (no atomics; the same bits for any
+  // launch shape)
 #pragma unroll 1
-  for (int n0 = 0; n0 < c.nmax; n0 += kScatterFaces) {
-    double part[kScatterFaces][4];
-    int base[kScatterFaces];
+  for (int r = 0; r < 4; ++r) {
+    const int nk = (NE - r + 3) / 4;  // K = r, r+4, .. < NE (0-based)
+    const int cnt = NPA * nk;
+    const float rnk = 1.0f / (float)max(nk, 1);
+#pragma unroll 1
+    for (int j = threadIdx.x; j < cnt; j += BT) {
+      const int l = fast_div(j, nk, rnk), k = r + 4 * (j - l * nk);
+      const int cc = l * NE + k;
+      double ce;
+      FaceD D;
+      int tg[4];
+      double gf[4];
+      eface(k, l, ce, D, tg, gf);
+      double fbb = (k >= 1) ? -div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * Y[cc] : 0.0;
+      if (k + 1 < NE) fbb += div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * Y[cc + 1];
+      const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
 #pragma unroll
-    for (int g = 0; g < kScatterFaces; ++g) {
-      const int n = n0 + g;
-      base[g] = -1;
-#pragma unroll
-      for (int o = 0; o < 4; ++o) part[g][o] = 0.0;
-      if (n < c.nmine) {
-        CELL_KL_T(n, BT);
-        double ce;
-        FaceD D;
-        int tg[4];
-        double gf[4];
-        eface(k, l, ce, D, tg, gf);
-        double fbb = (k >= 1) ? -div_rn(ce, d.kt[KT_WE * NE + k], d.kt[KT_RWE * NE + k]) * Y[cc] : 0.0;
-        if (k + 1 < NE) fbb += div_rn(ce, d.kt[KT_WE * NE + k + 1], d.kt[KT_RWE * NE + k + 1]) * Y[cc + 1];
-        base[g] = cc;
-        const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-          if (tg[m] >= 0) add_partial(part[g], tg[m] - k + 1, pd[m] * gf[m] * fbb);
-      }
+      for (int m = 0; m < 4; ++m)
+        if (tg[m] >= 0) d.H[l * NE + tg[m]] += pd[m] * gf[m] * fbb;
     }
-    scatter_rounds<kScatterFaces>(d.H, part, base, 1);
+    __syncthreads();
   }
 }
 
@@ -590,37 +570,29 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
-  // face L's partials go to rows L-1..L+2 of its column: scatter_rounds
-  // adds them without atomics, in a fixed order
+  // face L's partials go to rows L-1..L+2 of its column, in four passes by
+  // L mod 4 as DRIFTE's
 #pragma unroll 1
-  for (int n0 = 0; n0 < c.nmax; n0 += kScatterFaces) {
-    double part[kScatterFaces][4];
-    int base[kScatterFaces];
+  for (int r = 0; r < 4; ++r) {
+    const int nl = (NPA - r + 3) / 4;  // rows r, r+4, .. < NPA (0-based)
+    const int cnt = nl * NE;
+#pragma unroll 1
+    for (int j = threadIdx.x; j < cnt; j += BT) {
+      const int ll = fast_div(j, NE, c.rne), k = j - ll * NE, l = r + 4 * ll;
+      if (l < 1 || l > NPA - 2) continue;
+      const int cc = l * NE + k;
+      double cm;
+      FaceD D;
+      int tg[4];
+      mface(k, l, cm, D, tg);
+      double fbb = -div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * Y[cc];
+      if (l + 1 <= NPA - 2) fbb += div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * Y[cc + NE];
+      const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
 #pragma unroll
-    for (int g = 0; g < kScatterFaces; ++g) {
-      const int n = n0 + g;
-      base[g] = -1;
-#pragma unroll
-      for (int o = 0; o < 4; ++o) part[g][o] = 0.0;
-      if (n < c.nmine) {
-        CELL_KL_T(n, BT);
-        if (l >= 1 && l <= NPA - 2) {
-          double cm;
-          FaceD D;
-          int tg[4];
-          mface(k, l, cm, D, tg);
-          double fbb = -div_rn(cm, d.lt[LT_WMU * NPA + l], d.lt[LT_RWMU * NPA + l]) * Y[cc];
-          if (l + 1 <= NPA - 2)
-            fbb += div_rn(cm, d.lt[LT_WMU * NPA + l + 1], d.lt[LT_RWMU * NPA + l + 1]) * Y[cc + NE];
-          base[g] = cc;
-          const double pd[4] = {D.d0, D.d1, D.dn, D.dm};
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-            if (tg[m] >= 0) add_partial(part[g], tg[m] - l + 1, pd[m] * fbb);
-        }
-      }
+      for (int m = 0; m < 4; ++m)
+        if (tg[m] >= 0) d.H[tg[m] * NE + k] += pd[m] * fbb;
     }
-    scatter_rounds<kScatterFaces>(d.H, part, base, NE);
+    __syncthreads();
   }
 }
 
