@@ -366,8 +366,13 @@ __global__ void __launch_bounds__(kLineThreads) k_p_bwd(Launch L, const double *
 // DRIFTE adjoint (forward: plane_e).  Slots of face K (0-based k): f0 = F(K)
 // or the ghost F(1) (from F(2)), fp1 = F(K+1) or 0, fn = F(K+2) or 0 (C<0)
 // or f0, fnm1 = fp1 (C<0) or F(K-1) / ghosts F(0), F(1).
-template <int BT>
-__device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
+// TAPED: the forward's clamp decisions of the plane's cells are in cb (one
+// byte per cell, [L][K], written by k_plane<SEQ_TAPE>), so the faces are
+// formed once, with their derivatives; otherwise they are formed first for
+// the decisions.
+template <int BT, bool TAPED = false>
+__device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y,
+                            const unsigned char *cb = nullptr) {
   const int NE = c.NE, NPA = c.NPA;
   __syncthreads();
   const double *k8 = d.pc + PC_KIJ;  // the per-L terms are in d.cf (plane_setup)
@@ -418,6 +423,24 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
     }
     D = face_d(f0, fp1, fn, fnm1, ce, div_rn(ce, d.kt[KT_DE * NE + k], d.kt[KT_RDE * NE + k]), beta);
   };
+  if constexpr (TAPED) {
+    // flux weights w (unclamped updated cells) into Y, pass-through terms into H
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      if (n < c.nmine) {
+        CELL_KL_T(n, BT);
+        const double yb = Y[cc];
+        if (k >= 1) {
+          const double w = cb[cc] ? 0.0 : yb;
+          Y[cc] = w;
+          d.H[cc] = w;
+        } else {
+          Y[cc] = 0.0;
+          d.H[cc] = yb;  // cell 1 is not updated
+        }
+      }
+    __syncthreads();
+  } else {
   double O[8];
 #pragma unroll
   for (int n = 0; n < 8; ++n) {
@@ -455,6 +478,7 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
+  }
   // face K's partials go to cells K-1..K+2 of its row (the ghosts' to cell
   // 2).  The faces go in four passes by K mod 4, each ending in a barrier:
   // the targets of two faces of a pass lie in disjoint spans of four cells,
@@ -490,8 +514,10 @@ __device__ void plane_e_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, 
 // L and L-1 (face 1 carries nothing); the output F2(NPA) is F2(NPA-1) times
 // FNHS(NPA)*MU(NPA)/FNHS(NPA-1)/MU(NPA-1), so its adjoint joins cell NPA-1's;
 // the input F(NPA) only feeds FBND(NPA-1) = F(NPA) and stencils.
-template <int BT>
-__device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y) {
+// TAPED: as plane_e_bwd's.
+template <int BT, bool TAPED = false>
+__device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d, double *Y,
+                             const unsigned char *cb = nullptr) {
   const int NE = c.NE, NPA = c.NPA;
   const double hN = d.pc[PC_HN], hN1 = d.pc[PC_HN1];
   const double ratio = hN * d.pc[PC_MUN] / hN1 / d.pc[PC_MUN1];
@@ -529,6 +555,31 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
     }
     D = face_d(f0, fp1, fn, fnm1, cm, div_rn(cm, d.lt[LT_DMU * NPA + l], d.lt[LT_RDMU * NPA + l]), beta);
   };
+  if constexpr (TAPED) {
+    // the top cell's output adjoint joins row NPA-1's, before any row's Y changes
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      if (n < c.nmine) {
+        CELL_KL_T(n, BT);
+        if (l == NPA - 2) Y[cc] = Y[cc] + ratio * Y[cc + NE];
+      }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < 8; ++n)
+      if (n < c.nmine) {
+        CELL_KL_T(n, BT);
+        const double yb = Y[cc];
+        if (l >= 1 && l <= NPA - 2) {
+          const double w = cb[cc] ? 0.0 : yb;
+          Y[cc] = w;
+          d.H[cc] = w;
+        } else {
+          Y[cc] = 0.0;
+          d.H[cc] = (l == 0) ? yb : 0.0;  // L=1 passes; the input F(NPA) is overwritten
+        }
+      }
+    __syncthreads();
+  } else {
   double O[8];
 #pragma unroll
   for (int n = 0; n < 8; ++n) {
@@ -570,6 +621,7 @@ __device__ void plane_mu_bwd(const Tab &t, const PlaneCtx &c, const PlaneLds &d,
   for (int n = 0; n < 8; ++n)
     if (n < c.nmine) d.H[threadIdx.x + n * BT] = O[n];
   __syncthreads();
+  }
   // face L's partials go to rows L-1..L+2 of its column, in four passes by
   // L mod 4 as DRIFTE's
 #pragma unroll 1
@@ -663,20 +715,23 @@ __global__ void __launch_bounds__(BT, 4) k_plane_bwd(Launch L, const double *xin
       // phases alternate the two adjoint planes: A holds the output adjoint
       // of a phase, B receives the adjoint of its input
       const double *x = x0 + po + c.ij * c.P;
+      // the clamp decisions of E1, MU1, MU2, E2: byte planes of the slot 7
+      // after slot 0 (k_plane<SEQ_TAPE>)
+      const unsigned char *cb = reinterpret_cast<const unsigned char *>(x0 + 7 * ns) + po + c.ij * c.P;
       double *const A = Y, *const B = d.H;
       PlaneLds d2 = d;
       d2.H = A;
-      plane_e_bwd<BT>(t, c, d, A);  // E2: A -> B
+      plane_e_bwd<BT, true>(t, c, d, A, cb + 3 * ns);  // E2: A -> B
       block_copy<4>(d.F, x + 3 * ns, c.P);
       __syncthreads();
-      plane_mu_bwd<BT>(t, c, d2, B);  // MU2: B -> A
+      plane_mu_bwd<BT, true>(t, c, d2, B, cb + 2 * ns);  // MU2: B -> A
       plane_loss_bwd<BT>(t, c, d, A, 2);  // loss chain: A -> B (needs only its factors)
       block_copy<4>(d.F, x + ns, c.P);
       __syncthreads();
-      plane_mu_bwd<BT>(t, c, d2, B);  // MU1: B -> A
+      plane_mu_bwd<BT, true>(t, c, d2, B, cb + ns);  // MU1: B -> A
       block_copy<4>(d.F, x, c.P);
       __syncthreads();
-      plane_e_bwd<BT>(t, c, d, A);  // E1: A -> B
+      plane_e_bwd<BT, true>(t, c, d, A, cb);  // E1: A -> B
     }
     block_copy<4>(ga, d.H, c.P);
     __syncthreads();

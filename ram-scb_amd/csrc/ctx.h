@@ -158,7 +158,9 @@ PlaneShape plane_shape_for(int ne, int npa, int threads);
 void launch_plane(int seq, PlaneShape sh, dim3 grid, size_t lds, hipStream_t st, const ramgpu::Launch &L, int rec);
 // tape slots: the inputs of the operators of one ram_run step, in order
 // (T_E1..T_E2 consecutive: the plane kernels address them from T_E1)
-enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_N };
+// T_CLAMP: the plane phases' clamp decisions, one byte plane per drift
+// phase (E1, MU1, MU2, E2) in the slot's bytes
+enum TapeSlot { T_R1, T_P1, T_E1, T_MU1, T_LOSS, T_MU2, T_E2, T_P2, T_R2, T_CLAMP, T_N };
 // plane adjoint kernels
 int plane_bwd_threads(int ne, int npa);
 void set_plane_bwd_lds_limit(size_t limit);

@@ -906,8 +906,11 @@ constexpr int kPassRows = 8;
 // upwind sweep with CccolE(K) = (COULE+COULI)(S,K,L)*XNE(L), its own ghost
 // factors (the square roots of the inverse ratios of DRIFTE's) and rows
 // L=2..NPA only.
-template <int MAXC, bool COUL = false>
-__device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec) {
+//
+// TAPE: also write each updated cell's clamp decision (1: the forward
+// clamped it) to cb (this plane's bytes, [L][K]), for the adjoint.
+template <int MAXC, bool COUL = false, bool TAPE = false>
+__device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec, unsigned char *cb = nullptr) {
   const int NE = c.NE, NPA = c.NPA, S = c.S;
   if (COUL) rec = false;
   {
@@ -972,6 +975,7 @@ __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool r
         if (j >= c.nr) break;
         double w = v[n];
         if (first) w = w + xw[n - kPassRows];
+        if (TAPE) cb[(l0 + j) * NE + k] = (w < 0) ? 1 : 0;
         if (w < 0) w = 1E-15;
         if (!COUL || l0 + j >= 1) c.col[(l0 + j) * S] = w;
       }
@@ -994,8 +998,9 @@ __device__ void plane_e(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool r
 // read before any row is rewritten; the chunk's first row takes the term
 // of the chunk below after the last pass, and the top row is set last
 // from the finished row below it (the same thread's, by plane_chunks_for).
-template <int MAXC>
-__device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec) {
+// TAPE: as plane_e's.
+template <int MAXC, bool TAPE = false>
+__device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool rec, unsigned char *cb = nullptr) {
   const int NE = c.NE, NPA = c.NPA;
   if (rec) {
     for (int l = threadIdx.x; l < kMuSpread * NPA; l += blockDim.x) d.cmx[NE + l] = 0ull;
@@ -1061,6 +1066,7 @@ __device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool 
       if (j >= nr) break;
       if (j >= 1 && l <= NPA - 2) {
         double w = v[n];
+        if (TAPE) cb[l * NE + k] = (w < 0) ? 1 : 0;
         if (w < 0) w = 1E-15;
         col[l * cst] = w;
       }
@@ -1070,6 +1076,7 @@ __device__ void plane_mu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool 
   // plane, the first chunk's, is not updated)
   if (c.nr > 0 && c.q >= 1) {
     double w = v0 + d.X[threadIdx.x - NE];
+    if (TAPE) cb[l0 * NE + k] = (w < 0) ? 1 : 0;
     if (w < 0) w = 1E-15;
     col[l0 * cst] = w;
   }
@@ -1379,21 +1386,24 @@ __global__ void __launch_bounds__(kPlaneMaxThreads, 4) k_plane(Launch L, int rec
       // src is tape slot 0 (this phase's input); phase m's output, the input
       // of phase m+1, goes to slot m+1, the last to dst.  A thread stores
       // its own cells, which only it writes.
+      // The clamp decisions of the four drift phases go to the bytes of the
+      // tape slot 7 after slot 0, one byte plane per phase.
       const size_t ns = f2_elems(t);
       double *slot = const_cast<double *>(launch_src(L, t)) + po;
+      unsigned char *cb = reinterpret_cast<unsigned char *>(const_cast<double *>(launch_src(L, t)) + 7 * ns) + po;
       // each phase ends with a barrier, so its output is whole in LDS; the
       // loss chain rewrites cells at once, so a barrier follows the store
       // before it (the later phases rewrite cells only after barriers)
-      plane_e<MAXC>(t, c, d, rec_here);
+      plane_e<MAXC, false, true>(t, c, d, rec_here, cb);
       fwd_store(c, d, t.rne, slot + ns);
-      plane_mu<MAXC>(t, c, d, rec_here);
+      plane_mu<MAXC, true>(t, c, d, rec_here, cb + ns);
       fwd_store(c, d, t.rne, slot + 2 * ns);
       __syncthreads();
       plane_loss<MAXC>(t, c, d, u.b);
       fwd_store(c, d, t.rne, slot + 3 * ns);
-      plane_mu<MAXC>(t, c, d, false);
+      plane_mu<MAXC, true>(t, c, d, false, cb + 2 * ns);
       fwd_store(c, d, t.rne, slot + 4 * ns);
-      plane_e<MAXC>(t, c, d, false);
+      plane_e<MAXC, false, true>(t, c, d, false, cb + 3 * ns);
     } else {  // SEQ_STEP
       plane_e<MAXC>(t, c, d, rec_here);
       plane_mu<MAXC>(t, c, d, rec_here);
