@@ -100,6 +100,33 @@ int ramgpu_set_geometry(ramgpu_ctx *ctx, const double *fnhs, const double *fnis,
                         const double *dbdt, const double *eir, const double *eip,
                         const int *outside_mgnp);
 
+/* The geometry of the context back to the host, in the shapes of
+ * ramgpu_set_geometry (ModRamVariables' FNHS ... EIP, outsideMGNP); any
+ * pointer may be null. */
+int ramgpu_get_geometry(ramgpu_ctx *ctx, double *fnhs, double *fnis, double *bounhs,
+                        double *bounis, double *hdns, double *didt, double *dibndt,
+                        double *bnes, double *dbdt, double *eir, double *eip,
+                        int *outside_mgnp);
+
+/* computehI (src/ModRamScb.f90:144-643) after its field-line step: from the
+ * field lines on the RAM grid, x, y, z [RE] and B [units of bnormal], each
+ * (nthe, NR, NT) as xRAM(k,i,j) with i = 1..NR the rows I = 2..NR+1, the line
+ * coordinate chi(nthe) (chiVal, strictly increasing) and the equator sample
+ * ieq (nThetaEquator, 1-based): the h and I integrals and the Rairden density
+ * bounce average of every line with outside_mgnp(NR,NT) = 0, the rows from
+ * scale_at(NT) (ScaleAt, 0 none; may be null) scaled as the reference does
+ * beyond the SCB domain, its corrections, Steffen interpolation to pabn(NPA)
+ * over pa(NPA) [deg], the Gaussian smoothing when smooth != 0 (needs NR, NT
+ * >= 9), and FNHS, FNIS, BOUNHS, BOUNIS, HDNS, BNES, dIdt, dIbndt, dBdt with
+ * DthI = TimeRamElapsed - TOld, written into the context's geometry;
+ * EIR(1,:) = EIP(1,:) = 0; outsideMGNP taken from outside_mgnp; TOld set to
+ * TimeRamElapsed.  lz(NR+1) is LZ.  The integrals are exact for the
+ * piecewise-linear B the reference's quadrature samples (GSL cquad, 1e-3). */
+int ramgpu_computehI(ramgpu_ctx *ctx, int nthe, int ieq, const double *x, const double *y,
+                     const double *z, const double *b, const double *chi, const double *pa,
+                     const double *pabn, const double *lz, const int *outside_mgnp,
+                     const int *scale_at, double bnormal, int smooth);
+
 /* Convection potential VT(NR+1,NT) [V] (src/ModRamRun.f90:45-54). */
 int ramgpu_set_efield(ramgpu_ctx *ctx, const double *vt);
 

@@ -1,0 +1,531 @@
+// bounce.hip -- the SCB->RAM bounce integrals, computehI
+// (src/ModRamScb.f90:144-643), from field lines already on the RAM grid:
+// the h and I integrals and the bounce-averaged geocoronal density of every
+// line, the reference's corrections, the interpolation to the boundary
+// pitch angles, the optional Gaussian smoothing, and the RAM geometry with
+// its time derivatives, written straight into the context's geometry.
+//
+// The reference integrates with GSL's cquad at 1e-3 tolerance an integrand
+// built on GSL's linear interpolation of B (src/RamGSL.c:326-602).  Here the
+// same integrands are integrated exactly: on a segment between two samples
+// B and the density are linear in the line coordinate, so sqrt(Bm-B),
+// 1/sqrt(Bm-B) and v/sqrt(Bm-B) have closed-form integrals, summed segment
+// by segment along the line.  The pitch-angle interpolation is Steffen's
+// monotone cubic (GSL's gsl_interp_steffen; Steffen 1990, A&A 239:443).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace ramgpu_host;
+
+namespace {
+
+constexpr int kHiThreads = 64;     // one wave per field line
+constexpr int kHiPostThreads = 1024;
+constexpr int kSmoothR = 4;        // Gaussian kernel of sigma 1, truncated at 4 sigma
+constexpr int kSmoothW = 2 * kSmoothR + 1;
+
+struct HiArgs {
+  int nthe, nr, nt, npa, ieq;  // ieq 0-based
+  const double *x, *y, *z, *b;  // (NTHE, NR, NT)
+  const double *chi, *mu, *pa, *pabn, *lz, *w;
+  const int *mgnp, *scale_at;
+  double bnormal, dthi;
+  // line values (NR, NT, NPA) at [i + NR*(j + NT*L)], bz (NR, NT)
+  double *hC, *iC, *dC, *bz, *hB, *iB;
+  // their smoothed copies
+  double *sh, *si, *shb, *sib, *sd;
+  // the context's geometry, (NR+1, NT[, NPA]) Fortran layout
+  double *fnhs, *fnis, *bounhs, *bounis, *hdns, *didt, *dibndt, *bnes, *dbdt, *eir, *eip;
+  int *mgnp_dst;
+  int smooth;
+};
+
+// Geocoronal H density [m^-3] at d [RE], the Rairden profile
+// (src/ModRamScb.f90:363-370)
+__device__ inline double rairden(double d) {
+  const double d2 = d * d;
+  const double e = (((13.326 + -3.6908 * d) + 1.1362 * d2) + -0.16984 * (d2 * d)) + 0.009553 * (d2 * d2);
+  return pow(10.0, e);
+}
+
+// Integrals over one segment [c0, c1] of sqrt(m-B), 1/sqrt(m-B) and
+// v/sqrt(m-B) (zero where B >= m), B and v linear between the samples.
+// With s, e the square roots of m-B at the ends of the part below m, of
+// length dl: 2dl/(s+e), 2dl(s^2+se+e^2)/(3(s+e)), and
+// vs*2dl/(s+e) + (ve-vs)*2dl(2s+e)/(3(s+e)^2).  False if no part is below m.
+__device__ inline bool segment(double c0, double c1, double b0, double b1, double v0, double v1, double m,
+                               double &ji, double &jh, double &jd) {
+  const double u0 = m - b0, u1 = m - b1, d = c1 - c0;
+  const bool p0 = u0 > 0, p1 = u1 > 0;
+  if (!p0 && !p1) return false;
+  double dl, us, ue, vs, ve;
+  if (p0 && p1) {
+    dl = d; us = u0; ue = u1; vs = v0; ve = v1;
+  } else if (p0) {  // B reaches m inside the segment
+    const double t = u0 / (u0 - u1);
+    dl = d * t; us = u0; ue = 0.0; vs = v0; ve = v0 + (v1 - v0) * t;
+  } else {
+    const double t = u1 / (u1 - u0);
+    dl = d * t; us = 0.0; ue = u1; vs = v1 + (v0 - v1) * t; ve = v1;
+  }
+  const double s = sqrt(us), e = sqrt(ue), den = s + e;
+  jh = (2.0 * dl) / den;
+  ji = ((2.0 * dl) * ((us + s * e) + ue)) / (3.0 * den);
+  jd = vs * jh + (ve - vs) * (((2.0 * dl) * (2.0 * s + e)) / ((3.0 * den) * den));
+  return true;
+}
+
+// One field line per block: integrator_c and bounceaverage_c
+// (src/RamGSL.c:480-602) and the line's I, h, density
+// (src/ModRamScb.f90:376-404).
+__global__ void __launch_bounds__(kHiThreads) k_hi_lines(HiArgs a) {
+  extern __shared__ double hs[];
+  const int nthe = a.nthe, npa = a.npa, tid = threadIdx.x;
+  const int line = blockIdx.x, i = line % a.nr, j = line / a.nr;
+  const size_t cell0 = (size_t)i + (size_t)a.nr * j, plane = (size_t)a.nr * a.nt;
+  if (a.mgnp[cell0] != 0) {  // outside the magnetopause: no integrals
+    for (int L = tid; L < npa; L += blockDim.x) {
+      a.hC[cell0 + plane * L] = 0.0;
+      a.iC[cell0 + plane * L] = 0.0;
+      a.dC[cell0 + plane * L] = 0.0;
+    }
+    if (tid == 0) a.bz[cell0] = 0.0;
+    return;
+  }
+  double *bs = hs, *cs = bs + nthe, *vs = cs + nthe, *ms = vs + nthe;
+  double *yI = ms + npa, *yH = yI + npa, *yD = yH + npa, *sc = yD + npa;  // sc: length, r0
+  int *code = (int *)(sc + 2);
+  const size_t base = (size_t)nthe * cell0;
+  for (int k = tid; k < nthe; k += blockDim.x) {
+    const double x = a.x[base + k], y = a.y[base + k], z = a.z[base + k];
+    bs[k] = a.b[base + k];
+    cs[k] = a.chi[k];
+    vs[k] = rairden(sqrt((x * x + y * y) + z * z));
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double length = 0.0, bmin = bs[0];
+    for (int k = 1; k < nthe; ++k) {
+      const double dx = a.x[base + k] - a.x[base + k - 1], dy = a.y[base + k] - a.y[base + k - 1],
+                   dz = a.z[base + k] - a.z[base + k - 1];
+      length += sqrt((dx * dx + dy * dy) + dz * dz);
+      bmin = fmin(bmin, bs[k]);
+    }
+    const double xe = a.x[base + a.ieq], ye = a.y[base + a.ieq];
+    sc[0] = length;
+    sc[1] = sqrt(xe * xe + ye * ye);
+    const double be = bs[a.ieq];
+    if (fabs(be - bmin) > 1e-9) bs[a.ieq] = (2.0 * bmin - be > 0) ? 2.0 * bmin - be : bmin - 0.01;
+  }
+  __syncthreads();
+  const double beq = bs[a.ieq];
+  for (int L = tid; L < npa; L += blockDim.x) {
+    const double m = (L < npa - 1) ? beq / (1.0 - a.mu[L] * a.mu[L]) : bs[nthe - 1];
+    ms[L] = m;
+    int cd = 0, lh = 0, rh = nthe - 1;
+    if (L == 0) {
+      cd = 3;  // 90 degrees: set from L=1 below
+    } else if (L < npa - 1) {
+      double av = 0.0, bv = 0.0;
+      lh = rh = 0;
+      for (int k = 1; k < nthe - 1; ++k)
+        if (m <= bs[k - 1] && m >= bs[k]) {
+          av = cs[k - 1];
+          lh = k - 1;
+          break;
+        }
+      for (int k = nthe - 2; k > 0; --k)
+        if (m >= bs[k - 1] && m <= bs[k]) {
+          bv = cs[k];
+          rh = k;
+          break;
+        }
+      if (m >= bs[1] || av == 0) av = cs[0];
+      if (m >= bs[nthe - 1] || bv == 0) bv = cs[nthe - 1];
+      if (av <= cs[0] || bv >= cs[nthe - 1])
+        cd = 1;  // mirrors off the line: the last pitch angle's values
+      else if (rh - lh <= 4)
+        cd = 2;  // too few samples: the next pitch angle's values
+    }
+    double si = 0.0, sh = 0.0, sd = 0.0;
+    if (cd == 0)
+      for (int k = lh; k < rh; ++k) {
+        double ji, jh, jd;
+        if (segment(cs[k], cs[k + 1], bs[k], bs[k + 1], vs[k], vs[k + 1], m, ji, jh, jd)) {
+          si += ji;
+          sh += jh;
+          sd += jd;
+        }
+      }
+    yI[L] = si;
+    yH[L] = sh;
+    yD[L] = sd;
+    code[L] = cd;
+  }
+  __syncthreads();
+  if (tid == 0) {  // the reference's fallbacks, from the loss cone down (src/RamGSL.c:566-594)
+    const int t = npa - 1;
+    for (int L = npa - 2; L > 0; --L) {
+      if (code[L] == 1) {
+        ms[L] = ms[t];
+        yI[L] = yI[t]; yH[L] = yH[t]; yD[L] = yD[t];
+      } else if (code[L] == 2) {
+        yI[L] = yI[L + 1]; yH[L] = yH[L + 1]; yD[L] = yD[L + 1];
+      } else {
+        if (yI[L] <= 0) yI[L] = yI[L + 1];
+        if (yH[L] <= 0) yH[L] = yH[L + 1];
+        if (yD[L] <= 0) yD[L] = yD[L + 1];
+      }
+    }
+    yI[0] = 0.0;
+    yH[0] = yH[1];
+    yD[0] = yD[1];
+  }
+  __syncthreads();
+  const double length = sc[0], r0 = sc[1];
+  for (int L = tid; L < npa; L += blockDim.x) {
+    const double sm = sqrt(ms[L]);
+    a.iC[cell0 + plane * L] = (length / (M_PI * r0)) * yI[L] / sm;
+    a.hC[cell0 + plane * L] = (length / (M_PI * 2.0 * r0)) * yH[L] * sm;
+    a.dC[cell0 + plane * L] = yD[L] / yH[L];
+  }
+  if (tid == 0) a.bz[cell0] = beq * a.bnormal;
+}
+
+// Steffen's interpolant of one line's X over the pitch angles, at PAbn(L)
+// for L = 2..NPA-1, the ends copied (src/ModRamScb.f90:520-533).  The
+// reference interpolates over PA(NPA:1:-1), so xa and fa are read reversed;
+// slopes by Steffen's eq. 11 inside and the end secants at the ends,
+// coefficients by eqs. 2-5, Horner evaluation on xa(r) <= x < xa(r+1), and
+// linear extrapolation outside (src/RamGSL.c:158-166).
+__device__ void steffen_line(const HiArgs &a, const double *X, double *Y, size_t cell0) {
+  const int n = a.npa;
+  const size_t plane = (size_t)a.nr * a.nt;
+  auto XA = [&](int r) { return a.pa[n - 1 - r]; };
+  auto FA = [&](int r) { return X[cell0 + plane * (n - 1 - r)]; };
+  auto slope = [&](int r) -> double {
+    if (r == 0) return (FA(1) - FA(0)) / (XA(1) - XA(0));
+    if (r == n - 1) return (FA(n - 1) - FA(n - 2)) / (XA(n - 1) - XA(n - 2));
+    const double hi = XA(r + 1) - XA(r), him1 = XA(r) - XA(r - 1);
+    const double si = (FA(r + 1) - FA(r)) / hi, sim1 = (FA(r) - FA(r - 1)) / him1;
+    const double p = (sim1 * hi + si * him1) / (him1 + hi);
+    const double sg = (sim1 < 0 ? -1.0 : 1.0) + (si < 0 ? -1.0 : 1.0);
+    return sg * fmin(fabs(sim1), fmin(fabs(si), 0.5 * fabs(p)));
+  };
+  for (int L = 1; L <= n - 2; ++L) {
+    const double x = a.pabn[L];
+    double v;
+    if (x <= XA(0)) {
+      v = FA(0) + (x - XA(0)) / (XA(1) - XA(0)) * (FA(1) - FA(0));
+    } else if (x >= XA(n - 1)) {
+      v = FA(n - 1) + (x - XA(n - 1)) / (XA(n - 2) - XA(n - 1)) * (FA(n - 2) - FA(n - 1));
+    } else {
+      int r = 0;
+      while (r + 1 < n - 1 && XA(r + 1) <= x) ++r;
+      const double y0 = slope(r), y1 = slope(r + 1);
+      const double h = XA(r + 1) - XA(r);
+      const double s = (FA(r + 1) - FA(r)) / h;
+      const double ca = (y0 + y1 - 2.0 * s) / h / h;
+      const double cb = (3.0 * s - 2.0 * y0 - y1) / h;
+      const double dx = x - XA(r);
+      v = FA(r) + dx * (y0 + dx * (cb + dx * ca));
+    }
+    Y[cell0 + plane * L] = v;
+  }
+  Y[cell0 + plane * (n - 1)] = Y[cell0 + plane * (n - 2)];
+  Y[cell0] = Y[cell0 + plane];
+}
+
+// The corrections of src/ModRamScb.f90:406-533 over all lines, in the
+// reference's order, in one block (a few thousand lines): rows beyond the
+// SCB domain, MLT continuity, the near-90-degree values, negative values,
+// monotonicity in pitch angle, then the interpolation to PAbn.
+__global__ void __launch_bounds__(kHiPostThreads) k_hi_post(HiArgs a) {
+  const int nr = a.nr, nt = a.nt, npa = a.npa, tid = threadIdx.x, T = blockDim.x;
+  const size_t plane = (size_t)nr * nt;
+  double *const XS[3] = {a.iC, a.hC, a.dC};
+  auto at = [&](int i, int j, int L) { return (size_t)i + (size_t)nr * j + plane * L; };
+  // rows from ScaleAt(j) on (:413-461), for j = 2..NT, L = 2..NPA
+  for (int t = tid; t < nt * npa; t += T) {
+    const int j = t % nt, L = t / nt, ii = a.scale_at[j];
+    if (j == 0 || L == 0 || ii == 0) continue;
+    const int a0 = ii - 1, a1 = ii - 2, a2 = ii - 3;
+    const double fr = (a.lz[ii - 1] - a.lz[ii - 2]) / (a.lz[ii - 3] - a.lz[ii - 2]);
+    double s[3];
+    for (int q = 0; q < 3; ++q) {
+      const double *X = XS[q];
+      const double tmp = X[at(a1, j, L)] + fr * (X[at(a2, j, L)] - X[at(a1, j, L)]);
+      s[q] = (tmp <= 0) ? X[at(a1, j, L)] / X[at(a0, j, L)] : tmp / X[at(a0, j, L)];
+    }
+    for (int i = a0; i < nr; ++i) {
+      for (int q = 0; q < 3; ++q) {
+        double *X = XS[q];
+        X[at(i, j, L)] = (a.mgnp[i + (size_t)nr * j] == 0) ? X[at(i, j, L)] * s[q] : X[at(i - 1, j, L)];
+      }
+      if (L == 1) a.bz[i + (size_t)nr * j] = a.bz[i - 1 + (size_t)nr * j];
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < nr * npa; t += T) {  // continuity across MLT 0 (:464-467)
+    const int i = t % nr, L = t / nr;
+    for (int q = 0; q < 3; ++q) XS[q][at(i, 0, L)] = XS[q][at(i, nt - 1, L)];
+    if (L == 0) a.bz[i] = a.bz[i + (size_t)nr * (nt - 1)];
+  }
+  __syncthreads();
+  for (int t = tid; t < nr * nt; t += T) {  // near 90 degrees (:470-478)
+    const size_t c = t;
+    double *I = a.iC, *H = a.hC, *D = a.dC;
+    I[c + 2 * plane] = 0.50 * I[c + 3 * plane];
+    I[c + plane] = 0.20 * I[c + 2 * plane];
+    I[c] = 0.0;
+    H[c + 2 * plane] = 0.99 * H[c + 3 * plane];
+    H[c + plane] = 0.99 * H[c + 2 * plane];
+    H[c] = 0.99 * H[c + plane];
+    D[c + 2 * plane] = 0.999 * D[c + 3 * plane];
+    D[c + plane] = 0.999 * D[c + 2 * plane];
+    D[c] = 0.999 * D[c + plane];
+  }
+  __syncthreads();
+  for (int t = tid; t < nt * npa; t += T) {  // negative values take the row below's (:481-500)
+    const int j = t % nt, L = t / nt;
+    for (int i = 1; i < nr; ++i)
+      for (int q = 0; q < 3; ++q) {
+        double *X = XS[q];
+        if (X[at(i, j, L)] < 0) X[at(i, j, L)] = X[at(i - 1, j, L)];
+      }
+  }
+  __syncthreads();
+  for (int t = tid; t < nr * nt; t += T) {  // no value above the next pitch angle's (:502-510)
+    const size_t c = t;
+    const double f[3] = {0.99, 0.99, 0.999};
+    for (int L = npa - 2; L >= 0; --L)
+      for (int q = 0; q < 3; ++q) {
+        double *X = XS[q];
+        if (X[c + plane * L] > X[c + plane * (L + 1)]) X[c + plane * L] = f[q] * X[c + plane * (L + 1)];
+      }
+  }
+  __syncthreads();
+  for (int t = tid; t < nr * nt; t += T) {
+    steffen_line(a, a.hC, a.hB, t);
+    steffen_line(a, a.iC, a.iB, t);
+  }
+}
+
+__device__ inline int reflect(int p, int n) { return p < 0 ? -1 - p : (p >= n ? 2 * n - 1 - p : p); }
+
+// Gaussian smoothing of the five tables over (I,J) at each L >= 2
+// (src/ModRamScb.f90:539-562 with srcExternal/gaussian_filter.f90:104-178):
+// edges by reflection, the window summed in column-major order.
+__global__ void k_hi_smooth(HiArgs a) {
+  const size_t plane = (size_t)a.nr * a.nt, n = plane * a.npa;
+  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const int L = (int)(c / plane), ij = (int)(c - plane * L), j = ij / a.nr, i = ij - j * a.nr;
+  const double *src[5] = {a.hC, a.iC, a.hB, a.iB, a.dC};
+  double *dst[5] = {a.sh, a.si, a.shb, a.sib, a.sd};
+  for (int q = 0; q < 5; ++q) {
+    if (L == 0) {
+      dst[q][c] = src[q][c];
+      continue;
+    }
+    const double *X = src[q] + plane * L;
+    double acc = 0.0;
+    for (int dj = -kSmoothR; dj <= kSmoothR; ++dj) {
+      const int jj = reflect(j + dj, a.nt);
+      for (int di = -kSmoothR; di <= kSmoothR; ++di)
+        acc = acc + a.w[(di + kSmoothR) + kSmoothW * (dj + kSmoothR)] * X[reflect(i + di, a.nr) + (size_t)a.nr * jj];
+    }
+    dst[q][c] = acc;
+  }
+}
+
+// The RAM variables and their time derivatives (src/ModRamScb.f90:564-627):
+// one thread per (J, L) walks I, so the NaN repair takes the finished row
+// below.
+__global__ void k_hi_fill(HiArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nt * a.npa) return;
+  const int J = t % a.nt, L = t / a.nt, nr = a.nr, nr1 = nr + 1;
+  const size_t plane = (size_t)nr * a.nt;
+  const bool sm = a.smooth != 0;
+  const double *H = sm ? a.sh : a.hC, *I = sm ? a.si : a.iC, *HB = sm ? a.shb : a.hB, *IB = sm ? a.sib : a.iB,
+               *D = sm ? a.sd : a.dC;
+  const bool dt = fabs(a.dthi) > 1e-9;
+  auto g = [&](int i) { return (size_t)i + (size_t)nr1 * J + (size_t)nr1 * a.nt * L; };
+  for (int i = 1; i <= nr; ++i) {
+    const size_t s = (size_t)(i - 1) + (size_t)nr * J + plane * L, d = g(i);
+    const double fi = I[s], bi = IB[s];
+    a.didt[d] = dt ? (fi - a.fnis[d]) / a.dthi : 0.0;
+    a.dibndt[d] = dt ? (bi - a.bounis[d]) / a.dthi : 0.0;
+    a.fnhs[d] = H[s];
+    a.fnis[d] = fi;
+    a.hdns[d] = D[s];
+    a.bounhs[d] = HB[s];
+    a.bounis[d] = bi;
+  }
+  if (L == 0) {
+    for (int i = 1; i <= nr; ++i) {
+      const size_t d = (size_t)i + (size_t)nr1 * J;
+      const double bn = a.bz[(i - 1) + (size_t)nr * J] / 1e9;
+      a.dbdt[d] = dt ? (bn - a.bnes[d]) / a.dthi : 0.0;
+      a.bnes[d] = bn;
+    }
+    const double l1 = a.lz[0];
+    const size_t d = (size_t)nr1 * J;
+    a.bnes[d] = 0.32 / (l1 * l1 * l1) / 1.e4;
+    a.dbdt[d] = 0.0;
+    a.eir[d] = 0.0;
+    a.eip[d] = 0.0;
+  }
+  {
+    const size_t d0 = g(0), d1 = g(1);
+    a.fnhs[d0] = a.fnhs[d1];
+    a.fnis[d0] = a.fnis[d1];
+    a.bounhs[d0] = a.bounhs[d1];
+    a.bounis[d0] = a.bounis[d1];
+    a.hdns[d0] = a.hdns[d1];
+    a.didt[d0] = 0.0;
+    a.dibndt[d0] = 0.0;
+  }
+  for (int i = 1; i <= nr; ++i) {
+    const size_t d = g(i), p = g(i - 1);
+    if (isnan(a.fnis[d])) a.fnis[d] = a.fnis[p];
+    if (isnan(a.fnhs[d])) a.fnhs[d] = a.fnhs[p];
+    if (isnan(a.bounis[d])) a.bounis[d] = a.bounis[p];
+    if (isnan(a.bounhs[d])) a.bounhs[d] = a.bounhs[p];
+    if (isnan(a.hdns[d])) a.hdns[d] = a.hdns[p];
+    if (isnan(a.didt[d])) a.didt[d] = 0.0;
+    if (isnan(a.dibndt[d])) a.dibndt[d] = 0.0;
+  }
+  for (int i = t; i < nr * a.nt; i += a.nt * a.npa) a.mgnp_dst[i] = a.mgnp[i];
+}
+
+size_t hi_lds_bytes(int nthe, int npa) { return (3 * (size_t)nthe + 4 * (size_t)npa + 2) * 8 + (size_t)npa * 4; }
+
+}  // namespace
+
+extern "C" {
+
+int ramgpu_computehI(ramgpu_ctx *c, int nthe, int ieq, const double *x, const double *y, const double *z,
+                     const double *b, const double *chi, const double *pa, const double *pabn, const double *lz,
+                     const int *outside_mgnp, const int *scale_at, double bnormal, int smooth) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->have_grid) return fail(RAMGPU_ESTATE, "ramgpu_set_grid has not been called (MU is needed)");
+  const int nr = c->nr, nt = c->nt, npa = c->npa;
+  if (!x || !y || !z || !b || !chi || !pa || !pabn || !lz || !outside_mgnp)
+    return fail(RAMGPU_EINVAL, "computehI: a required array is null");
+  if (nthe < 3) return fail(RAMGPU_EINVAL, "computehI: NTHE=%d, at least 3 samples per line", nthe);
+  if (ieq < 1 || ieq > nthe) return fail(RAMGPU_EINVAL, "computehI: equator index %d outside 1..%d", ieq, nthe);
+  if (npa < 4) return fail(RAMGPU_EINVAL, "computehI: NPA=%d, the corrections need NPA >= 4", npa);
+  if (hi_lds_bytes(nthe, npa) > 65536) return fail(RAMGPU_EINVAL, "computehI: NTHE=%d too large", nthe);
+  if (smooth && (nr < kSmoothW || nt < kSmoothW))
+    return fail(RAMGPU_EINVAL, "computehI: smoothing needs NR and NT of at least %d (grid %dx%d)", kSmoothW, nr, nt);
+  for (int l = 1; l < npa; ++l)
+    if (!(pa[l] < pa[l - 1])) return fail(RAMGPU_EINVAL, "computehI: PA must decrease strictly");
+  for (int k = 1; k < nthe; ++k)
+    if (!(chi[k] > chi[k - 1])) return fail(RAMGPU_EINVAL, "computehI: the line coordinate must increase strictly");
+  if (scale_at)
+    for (int j = 1; j < nt; ++j)
+      if (scale_at[j] != 0 && (scale_at[j] < 3 || scale_at[j] > nr))
+        return fail(RAMGPU_EINVAL, "computehI: ScaleAt(%d)=%d outside 3..NR", j + 1, scale_at[j]);
+  // Gaussian weights on the host (src: srcExternal/gaussian_filter.f90:18-56)
+  double w[kSmoothW * kSmoothW], tot = 0.0;
+  for (int dj = -kSmoothR; dj <= kSmoothR; ++dj)
+    for (int di = -kSmoothR; di <= kSmoothR; ++di)
+      w[(di + kSmoothR) + kSmoothW * (dj + kSmoothR)] = 2.0 * std::exp((-0.5 * (double)(di * di + dj * dj)) / 1.0);
+  for (int q = 0; q < kSmoothW * kSmoothW; ++q) tot += w[q];
+  for (int q = 0; q < kSmoothW * kSmoothW; ++q) w[q] = w[q] / tot;
+  // one workspace for the call: inputs, line values, smoothed copies
+  const size_t nl = (size_t)nthe * nr * nt, n3 = (size_t)nr * nt * npa, n2 = (size_t)nr * nt;
+  const size_t nd = 4 * nl + nthe + 2 * (size_t)npa + (nr + 1) + kSmoothW * kSmoothW + 10 * n3 + n2;
+  const size_t ni = n2 + nt;
+  char *ws = nullptr;
+  if (hipMalloc(&ws, nd * sizeof(double) + ni * sizeof(int)) != hipSuccess)
+    return fail(RAMGPU_ENOMEM, "computehI: workspace allocation failed");
+  struct Free {
+    char *p;
+    ramgpu_ctx *c;
+    ~Free() {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(p);
+    }
+  } guard{ws, c};
+  double *d = (double *)ws;
+  HiArgs a{};
+  a.nthe = nthe; a.nr = nr; a.nt = nt; a.npa = npa; a.ieq = ieq - 1;
+  double *p = d;
+  auto take = [&](size_t n) { double *q = p; p += n; return q; };
+  double *dx = take(nl), *dy = take(nl), *dz = take(nl), *db = take(nl), *dchi = take(nthe);
+  double *dpa = take(npa), *dpabn = take(npa), *dlz = take(nr + 1), *dw = take(kSmoothW * kSmoothW);
+  a.hC = take(n3); a.iC = take(n3); a.dC = take(n3); a.hB = take(n3); a.iB = take(n3);
+  a.sh = take(n3); a.si = take(n3); a.shb = take(n3); a.sib = take(n3); a.sd = take(n3);
+  a.bz = take(n2);
+  int *im = (int *)p, *isa = im + n2;
+  std::vector<int> sa(nt, 0);
+  if (scale_at)
+    for (int j = 1; j < nt; ++j) sa[j] = scale_at[j];
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(dx, x, nl * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dy, y, nl * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dz, z, nl * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(db, b, nl * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dchi, chi, (size_t)nthe * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dpa, pa, (size_t)npa * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dpabn, pabn, (size_t)npa * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dlz, lz, (size_t)(nr + 1) * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(dw, w, sizeof(w), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(im, outside_mgnp, n2 * sizeof(int), hipMemcpyHostToDevice, s));
+  HIP_TRY(hipMemcpyAsync(isa, sa.data(), (size_t)nt * sizeof(int), hipMemcpyHostToDevice, s));
+  a.x = dx; a.y = dy; a.z = dz; a.b = db; a.chi = dchi; a.mu = c->d_mu; a.pa = dpa; a.pabn = dpabn;
+  a.lz = dlz; a.w = dw; a.mgnp = im; a.scale_at = isa;
+  a.bnormal = bnormal;
+  a.dthi = c->time_elapsed - c->told;
+  a.fnhs = c->d_fnhs; a.fnis = c->d_fnis; a.bounhs = c->d_bounhs; a.bounis = c->d_bounis; a.hdns = c->d_hdnsf;
+  a.didt = c->d_didt; a.dibndt = c->d_dibndt; a.bnes = c->d_bnes; a.dbdt = c->d_dbdt; a.eir = c->d_eir;
+  a.eip = c->d_eip; a.mgnp_dst = c->d_mgnp;
+  a.smooth = smooth ? 1 : 0;
+  hipLaunchKernelGGL(k_hi_lines, dim3((unsigned)n2), dim3(kHiThreads), hi_lds_bytes(nthe, npa), s, a);
+  if ((rc = launch_check("k_hi_lines"))) return rc;
+  hipLaunchKernelGGL(k_hi_post, dim3(1), dim3(kHiPostThreads), 0, s, a);
+  if ((rc = launch_check("k_hi_post"))) return rc;
+  if (smooth) {
+    hipLaunchKernelGGL(k_hi_smooth, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, a);
+    if ((rc = launch_check("k_hi_smooth"))) return rc;
+  }
+  hipLaunchKernelGGL(k_hi_fill, dim3((unsigned)((nt * npa + 255) / 256)), dim3(256), 0, s, a);
+  if ((rc = launch_check("k_hi_fill"))) return rc;
+  HIP_TRY(hipStreamSynchronize(s));
+  c->told = c->time_elapsed;  // TOld = TimeRamElapsed (src/ModRamScb.f90:612)
+  c->have_geom = true;
+  c->tables_dirty = true;
+  return RAMGPU_OK;
+}
+
+int ramgpu_get_geometry(ramgpu_ctx *c, double *fnhs, double *fnis, double *bounhs, double *bounis, double *hdns,
+                        double *didt, double *dibndt, double *bnes, double *dbdt, double *eir, double *eip,
+                        int *outside_mgnp) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const size_t g3 = (size_t)(c->nr + 1) * c->nt * c->npa, g2 = (size_t)(c->nr + 1) * c->nt;
+  const struct {
+    double *h;
+    const double *d;
+    size_t n;
+  } v[] = {{fnhs, c->d_fnhs, g3}, {fnis, c->d_fnis, g3}, {bounhs, c->d_bounhs, g3}, {bounis, c->d_bounis, g3},
+           {hdns, c->d_hdnsf, g3}, {didt, c->d_didt, g3}, {dibndt, c->d_dibndt, g3}, {bnes, c->d_bnes, g2},
+           {dbdt, c->d_dbdt, g2}, {eir, c->d_eir, g2}, {eip, c->d_eip, g2}};
+  for (const auto &e : v)
+    if (e.h) HIP_TRY(hipMemcpyAsync(e.h, e.d, e.n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (outside_mgnp)
+    HIP_TRY(hipMemcpyAsync(outside_mgnp, c->d_mgnp, (size_t)c->nr * c->nt * sizeof(int), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RAMGPU_OK;
+}
+
+}  // extern "C"

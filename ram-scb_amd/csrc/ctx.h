@@ -110,6 +110,7 @@ struct ramgpu_ctx {
   bool coul_dirty = true, have_necr = false;
   double *d_necr = nullptr;
   double time_elapsed = 0.0;
+  double told = 0.0;  // TimeRamElapsed at the last ramgpu_computehI (TOld)
   // WPADIF and FLCscatter: per species the WPADIF table (device layout) and
   // whether ram_run takes it at the WPI positions (in place of WAVELO) and at
   // the EMIC positions; the FLC table (null: no FLC) and its allocation; Dt_bc

@@ -17,7 +17,7 @@ INCLUDE = REPO / "include"
 LIB = PKG / "libramgpu.so"
 
 HIP_SOURCES = ["kernels.hip", "api.hip", "batch.hip", "adjoint.hip", "adjoint_api.hip", "moments.hip",
-               "coulomb.hip", "pitch.hip"]
+               "coulomb.hip", "pitch.hip", "bounce.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",

@@ -41,6 +41,7 @@ EXPORTS = (
     "ramgpu_set_coulomb", "ramgpu_set_necr", "ramgpu_set_time", "ramgpu_get_time", "ramgpu_coulpara",
     "ramgpu_get_coulomb_tables", "ramgpu_coulen", "ramgpu_coulmu",
     "ramgpu_set_wpadif", "ramgpu_wpadif", "ramgpu_set_flc", "ramgpu_flcscatter",
+    "ramgpu_get_geometry", "ramgpu_computehI",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
@@ -82,6 +83,8 @@ def load_library(path: Path | None = None):
     lib.ramgpu_set_params.argtypes = [v, C.c_double, C.c_double, C.c_double]
     lib.ramgpu_set_grid.argtypes = [v] + [C.c_double] * 4 + [_dp] * 12
     lib.ramgpu_set_geometry.argtypes = [v] + [_dp] * 11 + [_ip]
+    lib.ramgpu_get_geometry.argtypes = [v] + [_dp] * 11 + [_ip]
+    lib.ramgpu_computehI.argtypes = [v, C.c_int, C.c_int] + [_dp] * 8 + [_ip, _ip, C.c_double, C.c_int]
     lib.ramgpu_set_efield.argtypes = [v, _dp]
     lib.ramgpu_set_kp.argtypes = [v, C.c_double, _dp, _dp]
     lib.ramgpu_set_boundary.argtypes = [v, _dp]
@@ -330,6 +333,36 @@ class RamGpu:
             raise ValueError("outside_mgnp must be (NR,NT)")
         self._check(self.lib.ramgpu_set_geometry(self.h, *[_p(a) for a in arrs],
                                                  mg.ctypes.data_as(_ip)))
+
+    def get_geometry(self):
+        """The context's geometry as a dict of Fortran-ordered arrays in the
+        shapes of set_geometry (keys as Geometry's fields)."""
+        ns, nr, nt, ne, npa = self.dims
+        g3, g2 = (nr + 1, nt, npa), (nr + 1, nt)
+        out = {k: np.zeros(g3, order="F") for k in ("fnhs", "fnis", "bounhs", "bounis", "hdns", "didt",
+                                                      "dibndt")}
+        out.update({k: np.zeros(g2, order="F") for k in ("bnes", "dbdt", "eir", "eip")})
+        mg = np.zeros((nr, nt), dtype=np.int32, order="F")
+        keys = ("fnhs", "fnis", "bounhs", "bounis", "hdns", "didt", "dibndt", "bnes", "dbdt", "eir", "eip")
+        self._check(self.lib.ramgpu_get_geometry(self.h, *[_p(out[k]) for k in keys], mg.ctypes.data_as(_ip)))
+        out["outside_mgnp"] = mg
+        return out
+
+    def computehI(self, lines, pa, pabn, lz, outside_mgnp=None, scale_at=None, smooth=True):
+        """computehI from field lines on the RAM grid (``fieldlines.Lines``:
+        x, y, z, b as (nthe, NR, NT), chi, ieq 1-based, bnormal): the
+        context's geometry from their bounce integrals."""
+        ns, nr, nt, ne, npa = self.dims
+        n = lines.b.size
+        x, y, z, b = (_fd(a, n) for a in (lines.x, lines.y, lines.z, lines.b))
+        nthe = lines.b.shape[0]
+        mg = np.asfortranarray(np.zeros((nr, nt)) if outside_mgnp is None else outside_mgnp, dtype=np.int32)
+        sa = None if scale_at is None else np.ascontiguousarray(scale_at, dtype=np.int32)
+        chi, pa_, pb, lz_ = (np.ascontiguousarray(a, dtype=np.float64) for a in (lines.chi, pa, pabn, lz))
+        self._check(self.lib.ramgpu_computehI(
+            self.h, int(nthe), int(lines.ieq), _p(x), _p(y), _p(z), _p(b), _p(chi), _p(pa_), _p(pb), _p(lz_),
+            mg.ctypes.data_as(_ip), None if sa is None else sa.ctypes.data_as(_ip), float(lines.bnormal),
+            1 if smooth else 0))
 
     def set_efield(self, vt):
         a = _fd(vt, (self.nr + 1) * self.nt)

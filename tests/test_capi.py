@@ -15,7 +15,7 @@ HEADER = Path(__file__).resolve().parent.parent / "include" / "ramgpu.h"
 
 def declared():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"\b(ramgpu_[a-z_0-9]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(ramgpu_[A-Za-z_0-9]+)\s*\(", text)))
 
 
 def test_header_declares_the_operator_set():
