@@ -44,7 +44,11 @@ More objects in the same line:
   "batched_adjoint": the same misfit gradient for an ensemble of 4-species
     40x96x64x90 members (Kp ~ U(1,8) per member), 120 steps with checkpoints
     every 20, in one launch per kernel, against the ensemble's forward;
-  "moments": ANISCH of every species (and with FLUX) on the resident F2.
+  "moments": ANISCH of every species (and with FLUX) on the resident F2;
+  "computehI": the SCB->RAM bounce integrals from 101-sample field lines
+    (dipole lines with 3% noise in B, smoothing on) on the 20x25x35x72 and
+    40x96x64x90 grids, per call (host lines in, geometry left on the
+    device), beside the oracle's time on the smaller grid.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--members M] [--multi-only [--coulomb]]
 """
@@ -490,6 +494,50 @@ def bench_moments(dev, cells_kl: int, reps: int = 50):
     return res
 
 
+def bench_computehI(seed: int, reps: int = 20):
+    """ramgpu_computehI per call, wall clock around the call (it uploads the
+    lines, runs four kernels and synchronises), on two grids."""
+    import numpy as np
+
+    from ramgpu import fieldlines, inputs
+    from ramgpu.capi import RamGpu
+    res = {}
+    for tag, dims in (("20x25x35x72", (20, 25, 35, 72)), ("40x96x64x90", (40, 96, 64, 90))):
+        p = inputs.make_problem(*dims, (inputs.SPECIES4[1],), (1.0,), kp=4.0, seed=seed)
+        g = p.grid
+        lines = fieldlines.dipole_lines(g.lz, g.nt)
+        lines.b = lines.b * np.random.default_rng(seed).uniform(0.97, 1.03, lines.b.shape)
+        dev = RamGpu.from_problem(p)
+        dev.computehI(lines, g.pa, g.pabn, g.lz)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dev.computehI(lines, g.pa, g.pabn, g.lz)
+        ms = (time.perf_counter() - t0) / reps * 1e3
+        dev.close()
+        res[tag] = {"ms_per_call": ms, "lines": g.nr * g.nt, "nthe": lines.b.shape[0], "npa": g.npa}
+        log(f"bench: computehI {tag} {ms:.3f} ms per call")
+    return res
+
+
+def cpu_computehI(seed: int):
+    """The oracle's computehI on the 20x25x35x72 lines of bench_computehI
+    (numpy, one thread): its CPU baseline."""
+    import numpy as np
+
+    from oracle import bounce as ob
+    from ramgpu import fieldlines, inputs
+    p = inputs.make_problem(20, 25, 35, 72, (inputs.SPECIES4[1],), (1.0,), kp=4.0, seed=seed)
+    g = p.grid
+    lines = fieldlines.dipole_lines(g.lz, g.nt)
+    lines.b = lines.b * np.random.default_rng(seed).uniform(0.97, 1.03, lines.b.shape)
+    prev = {k: getattr(p.geom, k) for k in ("fnhs", "fnis", "bounhs", "bounis", "bnes")}
+    t0 = time.perf_counter()
+    ob.computehI(lines.x, lines.y, lines.z, lines.b, lines.chi, lines.ieq - 1, g.mu, g.pa, g.pabn, g.lz,
+                 np.zeros((g.nr, g.nt), int), np.zeros(g.nt, int), lines.bnormal, 0.0, True, prev)
+    return {"ms_per_call": (time.perf_counter() - t0) * 1e3, "kind": "port", "cores": 1,
+            "sample": "one call on the 20x25x35x72 lines (numpy restatement)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -514,6 +562,7 @@ def main():
     ap.add_argument("--all-processes", action="store_true",
                     help="with --multi-only: every optional process on (the composed step)")
     ap.add_argument("--no-coulomb", action="store_true", help="skip the multi-species leg with Coulomb collisions")
+    ap.add_argument("--hi-only", action="store_true", help="only the computehI leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -529,6 +578,9 @@ def main():
     if args.ens_adjoint_only:
         r = bench_batched_adjoint(max(args.ens_adjoint_members, 1), args.ens_adjoint_steps, 20, args.seed)
         print(json.dumps({k: v for k, v in r.items() if k != "loss"}), flush=True)
+        return
+    if args.hi_only:
+        print(json.dumps(bench_computehI(args.seed)), flush=True)
         return
     if args.multi_only:
         print(json.dumps(bench_multi_species(args.multi_steps, 20, args.seed, args.coulomb,
@@ -582,10 +634,13 @@ def main():
         if args.ens_adjoint_members > 0:
             out["batched_adjoint"] = leg("batched_adjoint", lambda: bench_batched_adjoint(
                 args.ens_adjoint_members, args.ens_adjoint_steps, 20, args.seed))
+    out["computehI"] = leg("computehI", lambda: bench_computehI(args.seed))
     if not args.no_cpu_baseline:
         from ramgpu import inputs
         log("bench: timing the CPU baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(problem or inputs.multi_species(seed=args.seed))
+        if isinstance(out.get("computehI"), dict) and "error" not in out["computehI"]:
+            out["computehI"]["cpu_oracle_20x25x35x72"] = cpu_computehI(args.seed)
         log(f"bench: CPU baseline {out['cpu_baseline']['value']:.3e} cell-updates/s")
     else:
         out["cpu_baseline"] = None

@@ -23,8 +23,8 @@ using namespace ramgpu_host;
 
 namespace {
 
-constexpr int kHiThreads = 64;     // one wave per field line
-constexpr int kHiPostThreads = 1024;
+constexpr int kHiThreads = 64;      // one wave per field line
+constexpr int kHiLineThreads = 128; // two waves per line for the integrals (one pitch angle each up to 128)
 constexpr int kSmoothR = 4;        // Gaussian kernel of sigma 1, truncated at 4 sigma
 constexpr int kSmoothW = 2 * kSmoothR + 1;
 
@@ -79,10 +79,10 @@ __device__ inline bool segment(double c0, double c1, double b0, double b1, doubl
   return true;
 }
 
-// One field line per block: integrator_c and bounceaverage_c
+// One field line per block of two waves: integrator_c and bounceaverage_c
 // (src/RamGSL.c:480-602) and the line's I, h, density
 // (src/ModRamScb.f90:376-404).
-__global__ void __launch_bounds__(kHiThreads) k_hi_lines(HiArgs a) {
+__global__ void __launch_bounds__(kHiLineThreads) k_hi_lines(HiArgs a) {
   extern __shared__ double hs[];
   const int nthe = a.nthe, npa = a.npa, tid = threadIdx.x;
   const int line = blockIdx.x, i = line % a.nr, j = line / a.nr;
@@ -98,21 +98,24 @@ __global__ void __launch_bounds__(kHiThreads) k_hi_lines(HiArgs a) {
   }
   double *bs = hs, *cs = bs + nthe, *vs = cs + nthe, *ms = vs + nthe;
   double *yI = ms + npa, *yH = yI + npa, *yD = yH + npa, *sc = yD + npa;  // sc: length, r0
-  int *code = (int *)(sc + 2);
+  double *sg = sc + 2;  // segment lengths
+  int *code = (int *)(sg + nthe);
   const size_t base = (size_t)nthe * cell0;
   for (int k = tid; k < nthe; k += blockDim.x) {
     const double x = a.x[base + k], y = a.y[base + k], z = a.z[base + k];
     bs[k] = a.b[base + k];
     cs[k] = a.chi[k];
     vs[k] = rairden(sqrt((x * x + y * y) + z * z));
+    if (k >= 1) {
+      const double dx = x - a.x[base + k - 1], dy = y - a.y[base + k - 1], dz = z - a.z[base + k - 1];
+      sg[k] = sqrt((dx * dx + dy * dy) + dz * dz);
+    }
   }
   __syncthreads();
-  if (tid == 0) {
+  if (tid == 0) {  // the line's length, summed along it
     double length = 0.0, bmin = bs[0];
     for (int k = 1; k < nthe; ++k) {
-      const double dx = a.x[base + k] - a.x[base + k - 1], dy = a.y[base + k] - a.y[base + k - 1],
-                   dz = a.z[base + k] - a.z[base + k - 1];
-      length += sqrt((dx * dx + dy * dy) + dz * dz);
+      length += sg[k];
       bmin = fmin(bmin, bs[k]);
     }
     const double xe = a.x[base + a.ieq], ye = a.y[base + a.ieq];
@@ -196,17 +199,15 @@ __global__ void __launch_bounds__(kHiThreads) k_hi_lines(HiArgs a) {
   if (tid == 0) a.bz[cell0] = beq * a.bnormal;
 }
 
-// Steffen's interpolant of one line's X over the pitch angles, at PAbn(L)
-// for L = 2..NPA-1, the ends copied (src/ModRamScb.f90:520-533).  The
-// reference interpolates over PA(NPA:1:-1), so xa and fa are read reversed;
+// Steffen's interpolant at x of one line's f (in L order) over the pitch
+// angles pa.  The reference interpolates over PA(NPA:1:-1)
+// (src/ModRamScb.f90:520-533), so xa and fa are pa and f read reversed;
 // slopes by Steffen's eq. 11 inside and the end secants at the ends,
 // coefficients by eqs. 2-5, Horner evaluation on xa(r) <= x < xa(r+1), and
 // linear extrapolation outside (src/RamGSL.c:158-166).
-__device__ void steffen_line(const HiArgs &a, const double *X, double *Y, size_t cell0) {
-  const int n = a.npa;
-  const size_t plane = (size_t)a.nr * a.nt;
-  auto XA = [&](int r) { return a.pa[n - 1 - r]; };
-  auto FA = [&](int r) { return X[cell0 + plane * (n - 1 - r)]; };
+__device__ double steffen_at(const double *pa, const double *f, int n, double x) {
+  auto XA = [&](int r) { return pa[n - 1 - r]; };
+  auto FA = [&](int r) { return f[n - 1 - r]; };
   auto slope = [&](int r) -> double {
     if (r == 0) return (FA(1) - FA(0)) / (XA(1) - XA(0));
     if (r == n - 1) return (FA(n - 1) - FA(n - 2)) / (XA(n - 1) - XA(n - 2));
@@ -216,102 +217,135 @@ __device__ void steffen_line(const HiArgs &a, const double *X, double *Y, size_t
     const double sg = (sim1 < 0 ? -1.0 : 1.0) + (si < 0 ? -1.0 : 1.0);
     return sg * fmin(fabs(sim1), fmin(fabs(si), 0.5 * fabs(p)));
   };
-  for (int L = 1; L <= n - 2; ++L) {
-    const double x = a.pabn[L];
-    double v;
-    if (x <= XA(0)) {
-      v = FA(0) + (x - XA(0)) / (XA(1) - XA(0)) * (FA(1) - FA(0));
-    } else if (x >= XA(n - 1)) {
-      v = FA(n - 1) + (x - XA(n - 1)) / (XA(n - 2) - XA(n - 1)) * (FA(n - 2) - FA(n - 1));
-    } else {
-      int r = 0;
-      while (r + 1 < n - 1 && XA(r + 1) <= x) ++r;
-      const double y0 = slope(r), y1 = slope(r + 1);
-      const double h = XA(r + 1) - XA(r);
-      const double s = (FA(r + 1) - FA(r)) / h;
-      const double ca = (y0 + y1 - 2.0 * s) / h / h;
-      const double cb = (3.0 * s - 2.0 * y0 - y1) / h;
-      const double dx = x - XA(r);
-      v = FA(r) + dx * (y0 + dx * (cb + dx * ca));
-    }
-    Y[cell0 + plane * L] = v;
-  }
-  Y[cell0 + plane * (n - 1)] = Y[cell0 + plane * (n - 2)];
-  Y[cell0] = Y[cell0 + plane];
+  if (x <= XA(0)) return FA(0) + (x - XA(0)) / (XA(1) - XA(0)) * (FA(1) - FA(0));
+  if (x >= XA(n - 1)) return FA(n - 1) + (x - XA(n - 1)) / (XA(n - 2) - XA(n - 1)) * (FA(n - 2) - FA(n - 1));
+  int r = 0;
+  while (r + 1 < n - 1 && XA(r + 1) <= x) ++r;
+  const double y0 = slope(r), y1 = slope(r + 1);
+  const double h = XA(r + 1) - XA(r);
+  const double s = (FA(r + 1) - FA(r)) / h;
+  const double ca = (y0 + y1 - 2.0 * s) / h / h;
+  const double cb = (3.0 * s - 2.0 * y0 - y1) / h;
+  const double dx = x - XA(r);
+  return FA(r) + dx * (y0 + dx * (cb + dx * ca));
 }
 
-// The corrections of src/ModRamScb.f90:406-533 over all lines, in the
-// reference's order, in one block (a few thousand lines): rows beyond the
-// SCB domain, MLT continuity, the near-90-degree values, negative values,
-// monotonicity in pitch angle, then the interpolation to PAbn.
-__global__ void __launch_bounds__(kHiPostThreads) k_hi_post(HiArgs a) {
-  const int nr = a.nr, nt = a.nt, npa = a.npa, tid = threadIdx.x, T = blockDim.x;
-  const size_t plane = (size_t)nr * nt;
+// The corrections of src/ModRamScb.f90:406-533, in the reference's order,
+// one launch per stage: rows beyond the SCB domain, MLT continuity with the
+// near-90-degree values, negative values, then monotonicity in pitch angle
+// with the interpolation to PAbn.
+
+// Rows from ScaleAt(j) on (:413-461): one thread per (j, L), j = 2..NT,
+// L = 2..NPA, walking the rows.
+__global__ void k_hi_scale(HiArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nt * a.npa) return;
+  const int nr = a.nr, j = t % a.nt, L = t / a.nt, ii = a.scale_at[j];
+  if (j == 0 || L == 0 || ii == 0) return;
+  const size_t plane = (size_t)nr * a.nt;
   double *const XS[3] = {a.iC, a.hC, a.dC};
-  auto at = [&](int i, int j, int L) { return (size_t)i + (size_t)nr * j + plane * L; };
-  // rows from ScaleAt(j) on (:413-461), for j = 2..NT, L = 2..NPA
-  for (int t = tid; t < nt * npa; t += T) {
-    const int j = t % nt, L = t / nt, ii = a.scale_at[j];
-    if (j == 0 || L == 0 || ii == 0) continue;
-    const int a0 = ii - 1, a1 = ii - 2, a2 = ii - 3;
-    const double fr = (a.lz[ii - 1] - a.lz[ii - 2]) / (a.lz[ii - 3] - a.lz[ii - 2]);
-    double s[3];
+  auto at = [&](int i) { return (size_t)i + (size_t)nr * j + plane * L; };
+  const int a0 = ii - 1, a1 = ii - 2, a2 = ii - 3;
+  const double fr = (a.lz[ii - 1] - a.lz[ii - 2]) / (a.lz[ii - 3] - a.lz[ii - 2]);
+  double s[3];
+  for (int q = 0; q < 3; ++q) {
+    const double *X = XS[q];
+    const double tmp = X[at(a1)] + fr * (X[at(a2)] - X[at(a1)]);
+    s[q] = (tmp <= 0) ? X[at(a1)] / X[at(a0)] : tmp / X[at(a0)];
+  }
+  for (int i = a0; i < nr; ++i) {
     for (int q = 0; q < 3; ++q) {
-      const double *X = XS[q];
-      const double tmp = X[at(a1, j, L)] + fr * (X[at(a2, j, L)] - X[at(a1, j, L)]);
-      s[q] = (tmp <= 0) ? X[at(a1, j, L)] / X[at(a0, j, L)] : tmp / X[at(a0, j, L)];
+      double *X = XS[q];
+      X[at(i)] = (a.mgnp[i + (size_t)nr * j] == 0) ? X[at(i)] * s[q] : X[at(i - 1)];
     }
-    for (int i = a0; i < nr; ++i) {
-      for (int q = 0; q < 3; ++q) {
-        double *X = XS[q];
-        X[at(i, j, L)] = (a.mgnp[i + (size_t)nr * j] == 0) ? X[at(i, j, L)] * s[q] : X[at(i - 1, j, L)];
-      }
-      if (L == 1) a.bz[i + (size_t)nr * j] = a.bz[i - 1 + (size_t)nr * j];
+    if (L == 1) a.bz[i + (size_t)nr * j] = a.bz[i - 1 + (size_t)nr * j];
+  }
+}
+
+// Continuity across MLT 0 (:464-467) and the near-90-degree values
+// (:470-478): one thread per line.  Column 1 takes column NT's L >= 4 (the
+// corrections rewrite L = 1..3 of every line from its L = 4, so copying
+// first and correcting after equals the reference's order).
+__global__ void k_hi_near90(HiArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, nr = a.nr, nt = a.nt;
+  if (t >= nr * nt) return;
+  const size_t plane = (size_t)nr * nt, c = t;
+  double *I = a.iC, *H = a.hC, *D = a.dC;
+  if (t < nr) {
+    const size_t src = c + (size_t)nr * (nt - 1);
+    for (int L = 3; L < a.npa; ++L) {
+      I[c + plane * L] = I[src + plane * L];
+      H[c + plane * L] = H[src + plane * L];
+      D[c + plane * L] = D[src + plane * L];
     }
+    a.bz[c] = a.bz[src];
+  }
+  I[c + 2 * plane] = 0.50 * I[c + 3 * plane];
+  I[c + plane] = 0.20 * I[c + 2 * plane];
+  I[c] = 0.0;
+  H[c + 2 * plane] = 0.99 * H[c + 3 * plane];
+  H[c + plane] = 0.99 * H[c + 2 * plane];
+  H[c] = 0.99 * H[c + plane];
+  D[c + 2 * plane] = 0.999 * D[c + 3 * plane];
+  D[c + plane] = 0.999 * D[c + 2 * plane];
+  D[c] = 0.999 * D[c + plane];
+}
+
+// Negative values take the row below's (:481-500): one thread per (j, L).
+__global__ void k_hi_negative(HiArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.nt * a.npa) return;
+  const int nr = a.nr, j = t % a.nt, L = t / a.nt;
+  const size_t b0 = (size_t)nr * j + (size_t)nr * a.nt * L;
+  double *const XS[3] = {a.hC, a.iC, a.dC};
+  for (int i = 1; i < nr; ++i)
+    for (int q = 0; q < 3; ++q) {
+      double *X = XS[q] + b0;
+      if (X[i] < 0) X[i] = X[i - 1];
+    }
+}
+
+// No value above the next pitch angle's (:502-510), then the interpolation
+// to PAbn (:520-533): one wave per line, the line's tables in LDS; one lane
+// per table walks the pitch angles down, then the lanes interpolate one L
+// each, the ends copied.
+__global__ void __launch_bounds__(kHiThreads) k_hi_pitch(HiArgs a) {
+  extern __shared__ double ls[];
+  const int n = a.npa, tid = threadIdx.x;
+  const size_t plane = (size_t)a.nr * a.nt, c = blockIdx.x;
+  double *I = ls, *H = I + n, *D = H + n, *P = D + n;
+  for (int L = tid; L < n; L += blockDim.x) {
+    I[L] = a.iC[c + plane * L];
+    H[L] = a.hC[c + plane * L];
+    D[L] = a.dC[c + plane * L];
+    P[L] = a.pa[L];
   }
   __syncthreads();
-  for (int t = tid; t < nr * npa; t += T) {  // continuity across MLT 0 (:464-467)
-    const int i = t % nr, L = t / nr;
-    for (int q = 0; q < 3; ++q) XS[q][at(i, 0, L)] = XS[q][at(i, nt - 1, L)];
-    if (L == 0) a.bz[i] = a.bz[i + (size_t)nr * (nt - 1)];
+  if (tid < 3) {
+    double *X = (tid == 0) ? I : (tid == 1 ? H : D);
+    const double f = (tid == 2) ? 0.999 : 0.99;
+    for (int L = n - 2; L >= 0; --L)
+      if (X[L] > X[L + 1]) X[L] = f * X[L + 1];
   }
   __syncthreads();
-  for (int t = tid; t < nr * nt; t += T) {  // near 90 degrees (:470-478)
-    const size_t c = t;
-    double *I = a.iC, *H = a.hC, *D = a.dC;
-    I[c + 2 * plane] = 0.50 * I[c + 3 * plane];
-    I[c + plane] = 0.20 * I[c + 2 * plane];
-    I[c] = 0.0;
-    H[c + 2 * plane] = 0.99 * H[c + 3 * plane];
-    H[c + plane] = 0.99 * H[c + 2 * plane];
-    H[c] = 0.99 * H[c + plane];
-    D[c + 2 * plane] = 0.999 * D[c + 3 * plane];
-    D[c + plane] = 0.999 * D[c + 2 * plane];
-    D[c] = 0.999 * D[c + plane];
+  for (int L = tid; L < n; L += blockDim.x) {
+    a.iC[c + plane * L] = I[L];
+    a.hC[c + plane * L] = H[L];
+    a.dC[c + plane * L] = D[L];
   }
-  __syncthreads();
-  for (int t = tid; t < nt * npa; t += T) {  // negative values take the row below's (:481-500)
-    const int j = t % nt, L = t / nt;
-    for (int i = 1; i < nr; ++i)
-      for (int q = 0; q < 3; ++q) {
-        double *X = XS[q];
-        if (X[at(i, j, L)] < 0) X[at(i, j, L)] = X[at(i - 1, j, L)];
-      }
-  }
-  __syncthreads();
-  for (int t = tid; t < nr * nt; t += T) {  // no value above the next pitch angle's (:502-510)
-    const size_t c = t;
-    const double f[3] = {0.99, 0.99, 0.999};
-    for (int L = npa - 2; L >= 0; --L)
-      for (int q = 0; q < 3; ++q) {
-        double *X = XS[q];
-        if (X[c + plane * L] > X[c + plane * (L + 1)]) X[c + plane * L] = f[q] * X[c + plane * (L + 1)];
-      }
-  }
-  __syncthreads();
-  for (int t = tid; t < nr * nt; t += T) {
-    steffen_line(a, a.hC, a.hB, t);
-    steffen_line(a, a.iC, a.iB, t);
+  for (int L = 1 + tid; L <= n - 2; L += blockDim.x) {
+    const double x = a.pabn[L];
+    const double hb = steffen_at(P, H, n, x), ib = steffen_at(P, I, n, x);
+    a.hB[c + plane * L] = hb;
+    a.iB[c + plane * L] = ib;
+    if (L == n - 2) {
+      a.hB[c + plane * (n - 1)] = hb;
+      a.iB[c + plane * (n - 1)] = ib;
+    }
+    if (L == 1) {
+      a.hB[c] = hb;
+      a.iB[c] = ib;
+    }
   }
 }
 
@@ -319,61 +353,69 @@ __device__ inline int reflect(int p, int n) { return p < 0 ? -1 - p : (p >= n ? 
 
 // Gaussian smoothing of the five tables over (I,J) at each L >= 2
 // (src/ModRamScb.f90:539-562 with srcExternal/gaussian_filter.f90:104-178):
-// edges by reflection, the window summed in column-major order.
-__global__ void k_hi_smooth(HiArgs a) {
-  const size_t plane = (size_t)a.nr * a.nt, n = plane * a.npa;
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  const int L = (int)(c / plane), ij = (int)(c - plane * L), j = ij / a.nr, i = ij - j * a.nr;
+// edges by reflection, the window summed in column-major order.  One block
+// per (L, table) with the (I,J) plane in LDS; L = 1 is copied.
+__global__ void __launch_bounds__(256) k_hi_smooth(HiArgs a) {
+  extern __shared__ double pl[];
+  const int nr = a.nr, nt = a.nt, L = blockIdx.x, q = blockIdx.y;
+  const size_t plane = (size_t)nr * nt;
   const double *src[5] = {a.hC, a.iC, a.hB, a.iB, a.dC};
   double *dst[5] = {a.sh, a.si, a.shb, a.sib, a.sd};
-  for (int q = 0; q < 5; ++q) {
-    if (L == 0) {
-      dst[q][c] = src[q][c];
-      continue;
-    }
-    const double *X = src[q] + plane * L;
+  const double *X = src[q] + plane * L;
+  double *Y = dst[q] + plane * L;
+  if (L == 0) {
+    for (size_t c = threadIdx.x; c < plane; c += blockDim.x) Y[c] = X[c];
+    return;
+  }
+  double *w = pl + plane;
+  for (size_t c = threadIdx.x; c < plane; c += blockDim.x) pl[c] = X[c];
+  for (int c = threadIdx.x; c < kSmoothW * kSmoothW; c += blockDim.x) w[c] = a.w[c];
+  __syncthreads();
+  for (size_t c = threadIdx.x; c < plane; c += blockDim.x) {
+    const int j = (int)(c / nr), i = (int)(c - (size_t)j * nr);
     double acc = 0.0;
     for (int dj = -kSmoothR; dj <= kSmoothR; ++dj) {
-      const int jj = reflect(j + dj, a.nt);
+      const double *col = pl + (size_t)nr * reflect(j + dj, nt);
       for (int di = -kSmoothR; di <= kSmoothR; ++di)
-        acc = acc + a.w[(di + kSmoothR) + kSmoothW * (dj + kSmoothR)] * X[reflect(i + di, a.nr) + (size_t)a.nr * jj];
+        acc = acc + w[(di + kSmoothR) + kSmoothW * (dj + kSmoothR)] * col[reflect(i + di, nr)];
     }
-    dst[q][c] = acc;
+    Y[c] = acc;
   }
 }
 
-// The RAM variables and their time derivatives (src/ModRamScb.f90:564-627):
-// one thread per (J, L) walks I, so the NaN repair takes the finished row
-// below.
+// The RAM variables and their time derivatives (src/ModRamScb.f90:564-596):
+// one thread per cell of the rows I = 2..NR+1.
 __global__ void k_hi_fill(HiArgs a) {
+  const int nr = a.nr, nt = a.nt;
+  const size_t plane = (size_t)nr * nt, t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= plane * a.npa) return;
+  const int L = (int)(t / plane), ij = (int)(t - plane * L), J = ij / nr, i = ij - J * nr + 1;
+  const bool sm = a.smooth != 0, dt = fabs(a.dthi) > 1e-9;
+  const size_t d = (size_t)i + (size_t)(nr + 1) * J + (size_t)(nr + 1) * nt * L;
+  const double fi = (sm ? a.si : a.iC)[t], bi = (sm ? a.sib : a.iB)[t];
+  a.didt[d] = dt ? (fi - a.fnis[d]) / a.dthi : 0.0;
+  a.dibndt[d] = dt ? (bi - a.bounis[d]) / a.dthi : 0.0;
+  a.fnhs[d] = (sm ? a.sh : a.hC)[t];
+  a.fnis[d] = fi;
+  a.hdns[d] = (sm ? a.sd : a.dC)[t];
+  a.bounhs[d] = (sm ? a.shb : a.hB)[t];
+  a.bounis[d] = bi;
+  if (L == 0) {
+    const size_t e = (size_t)i + (size_t)(nr + 1) * J;
+    const double bn = a.bz[ij] / 1e9;
+    a.dbdt[e] = dt ? (bn - a.bnes[e]) / a.dthi : 0.0;
+    a.bnes[e] = bn;
+  }
+}
+
+// Row I = 1 (src/ModRamScb.f90:598-611) and the NaN repair (:614-626): one
+// thread per (J, L) walks I, so a repaired cell takes the finished row below.
+__global__ void k_hi_rows(HiArgs a) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.nt * a.npa) return;
   const int J = t % a.nt, L = t / a.nt, nr = a.nr, nr1 = nr + 1;
-  const size_t plane = (size_t)nr * a.nt;
-  const bool sm = a.smooth != 0;
-  const double *H = sm ? a.sh : a.hC, *I = sm ? a.si : a.iC, *HB = sm ? a.shb : a.hB, *IB = sm ? a.sib : a.iB,
-               *D = sm ? a.sd : a.dC;
-  const bool dt = fabs(a.dthi) > 1e-9;
   auto g = [&](int i) { return (size_t)i + (size_t)nr1 * J + (size_t)nr1 * a.nt * L; };
-  for (int i = 1; i <= nr; ++i) {
-    const size_t s = (size_t)(i - 1) + (size_t)nr * J + plane * L, d = g(i);
-    const double fi = I[s], bi = IB[s];
-    a.didt[d] = dt ? (fi - a.fnis[d]) / a.dthi : 0.0;
-    a.dibndt[d] = dt ? (bi - a.bounis[d]) / a.dthi : 0.0;
-    a.fnhs[d] = H[s];
-    a.fnis[d] = fi;
-    a.hdns[d] = D[s];
-    a.bounhs[d] = HB[s];
-    a.bounis[d] = bi;
-  }
   if (L == 0) {
-    for (int i = 1; i <= nr; ++i) {
-      const size_t d = (size_t)i + (size_t)nr1 * J;
-      const double bn = a.bz[(i - 1) + (size_t)nr * J] / 1e9;
-      a.dbdt[d] = dt ? (bn - a.bnes[d]) / a.dthi : 0.0;
-      a.bnes[d] = bn;
-    }
     const double l1 = a.lz[0];
     const size_t d = (size_t)nr1 * J;
     a.bnes[d] = 0.32 / (l1 * l1 * l1) / 1.e4;
@@ -404,7 +446,8 @@ __global__ void k_hi_fill(HiArgs a) {
   for (int i = t; i < nr * a.nt; i += a.nt * a.npa) a.mgnp_dst[i] = a.mgnp[i];
 }
 
-size_t hi_lds_bytes(int nthe, int npa) { return (3 * (size_t)nthe + 4 * (size_t)npa + 2) * 8 + (size_t)npa * 4; }
+size_t hi_lds_bytes(int nthe, int npa) { return (4 * (size_t)nthe + 4 * (size_t)npa + 2) * 8 + (size_t)npa * 4; }
+size_t hi_smooth_lds_bytes(int nr, int nt) { return ((size_t)nr * nt + kSmoothW * kSmoothW) * 8; }
 
 }  // namespace
 
@@ -425,6 +468,8 @@ int ramgpu_computehI(ramgpu_ctx *c, int nthe, int ieq, const double *x, const do
   if (hi_lds_bytes(nthe, npa) > 65536) return fail(RAMGPU_EINVAL, "computehI: NTHE=%d too large", nthe);
   if (smooth && (nr < kSmoothW || nt < kSmoothW))
     return fail(RAMGPU_EINVAL, "computehI: smoothing needs NR and NT of at least %d (grid %dx%d)", kSmoothW, nr, nt);
+  if (smooth && hi_smooth_lds_bytes(nr, nt) > 65536)
+    return fail(RAMGPU_EINVAL, "computehI: smoothing of a %dx%d plane exceeds the LDS of one block", nr, nt);
   for (int l = 1; l < npa; ++l)
     if (!(pa[l] < pa[l - 1])) return fail(RAMGPU_EINVAL, "computehI: PA must decrease strictly");
   for (int k = 1; k < nthe; ++k)
@@ -489,16 +534,27 @@ int ramgpu_computehI(ramgpu_ctx *c, int nthe, int ieq, const double *x, const do
   a.didt = c->d_didt; a.dibndt = c->d_dibndt; a.bnes = c->d_bnes; a.dbdt = c->d_dbdt; a.eir = c->d_eir;
   a.eip = c->d_eip; a.mgnp_dst = c->d_mgnp;
   a.smooth = smooth ? 1 : 0;
-  hipLaunchKernelGGL(k_hi_lines, dim3((unsigned)n2), dim3(kHiThreads), hi_lds_bytes(nthe, npa), s, a);
+  hipLaunchKernelGGL(k_hi_lines, dim3((unsigned)n2), dim3(kHiLineThreads), hi_lds_bytes(nthe, npa), s, a);
   if ((rc = launch_check("k_hi_lines"))) return rc;
-  hipLaunchKernelGGL(k_hi_post, dim3(1), dim3(kHiPostThreads), 0, s, a);
-  if ((rc = launch_check("k_hi_post"))) return rc;
+  const unsigned gjl = (unsigned)((nt * npa + 255) / 256), gij = (unsigned)((n2 + 255) / 256);
+  if (scale_at) {
+    hipLaunchKernelGGL(k_hi_scale, dim3(gjl), dim3(256), 0, s, a);
+    if ((rc = launch_check("k_hi_scale"))) return rc;
+  }
+  hipLaunchKernelGGL(k_hi_near90, dim3(gij), dim3(256), 0, s, a);
+  if ((rc = launch_check("k_hi_near90"))) return rc;
+  hipLaunchKernelGGL(k_hi_negative, dim3(gjl), dim3(256), 0, s, a);
+  if ((rc = launch_check("k_hi_negative"))) return rc;
+  hipLaunchKernelGGL(k_hi_pitch, dim3((unsigned)n2), dim3(kHiThreads), 4 * (size_t)npa * sizeof(double), s, a);
+  if ((rc = launch_check("k_hi_pitch"))) return rc;
   if (smooth) {
-    hipLaunchKernelGGL(k_hi_smooth, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_hi_smooth, dim3((unsigned)npa, 5), dim3(256), hi_smooth_lds_bytes(nr, nt), s, a);
     if ((rc = launch_check("k_hi_smooth"))) return rc;
   }
-  hipLaunchKernelGGL(k_hi_fill, dim3((unsigned)((nt * npa + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(k_hi_fill, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, a);
   if ((rc = launch_check("k_hi_fill"))) return rc;
+  hipLaunchKernelGGL(k_hi_rows, dim3(gjl), dim3(256), 0, s, a);
+  if ((rc = launch_check("k_hi_rows"))) return rc;
   HIP_TRY(hipStreamSynchronize(s));
   c->told = c->time_elapsed;  // TOld = TimeRamElapsed (src/ModRamScb.f90:612)
   c->have_geom = true;

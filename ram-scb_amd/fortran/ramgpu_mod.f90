@@ -72,6 +72,27 @@ module ramgpu_mod
        integer(c_int), intent(in) :: outside_mgnp(*)
      end function ramgpu_set_geometry
 
+     integer(c_int) function ramgpu_get_geometry(ctx, fnhs, fnis, bounhs, bounis, hdns, &
+          didt, dibndt, bnes, dbdt, eir, eip, outside_mgnp) bind(C, name="ramgpu_get_geometry")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: fnhs(*), fnis(*), bounhs(*), bounis(*), hdns(*), &
+            didt(*), dibndt(*), bnes(*), dbdt(*), eir(*), eip(*)
+       integer(c_int), intent(out) :: outside_mgnp(*)
+     end function ramgpu_get_geometry
+
+     ! computehI (src/ModRamScb.f90:144-643) from xRAM, yRAM, zRAM, bRAM on
+     ! the RAM grid; scale_at is ScaleAt (zeros: no row beyond the SCB domain)
+     integer(c_int) function ramgpu_computehI(ctx, nthe, ieq, x, y, z, b, chi, pa, pabn, lz, &
+          outside_mgnp, scale_at, bnormal, smooth) bind(C, name="ramgpu_computehI")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: nthe, ieq, smooth
+       real(c_double), intent(in) :: x(*), y(*), z(*), b(*), chi(*), pa(*), pabn(*), lz(*)
+       integer(c_int), intent(in) :: outside_mgnp(*), scale_at(*)
+       real(c_double), value :: bnormal
+     end function ramgpu_computehI
+
      integer(c_int) function ramgpu_set_efield(ctx, vt) bind(C, name="ramgpu_set_efield")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx
