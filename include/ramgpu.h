@@ -127,6 +127,12 @@ int ramgpu_computehI(ramgpu_ctx *ctx, int nthe, int ieq, const double *x, const 
                      const double *pabn, const double *lz, const int *outside_mgnp,
                      const int *scale_at, double bnormal, int smooth);
 
+/* TOld (src/ModRamTiming.f90:20), the TimeRamElapsed of the last
+ * computehI: set by ramgpu_computehI, restored from a restart file
+ * (src/ModRamRestart.f90 writes and reads it). */
+int ramgpu_set_told(ramgpu_ctx *ctx, double told);
+int ramgpu_get_told(ramgpu_ctx *ctx, double *told);
+
 /* Convection potential VT(NR+1,NT) [V] (src/ModRamRun.f90:45-54). */
 int ramgpu_set_efield(ramgpu_ctx *ctx, const double *vt);
 

@@ -562,6 +562,21 @@ int ramgpu_computehI(ramgpu_ctx *c, int nthe, int ieq, const double *x, const do
   return RAMGPU_OK;
 }
 
+int ramgpu_set_told(ramgpu_ctx *c, double told) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  c->told = told;
+  return RAMGPU_OK;
+}
+
+int ramgpu_get_told(ramgpu_ctx *c, double *told) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!told) return fail(RAMGPU_EINVAL, "told is null");
+  *told = c->told;
+  return RAMGPU_OK;
+}
+
 int ramgpu_get_geometry(ramgpu_ctx *c, double *fnhs, double *fnis, double *bounhs, double *bounis, double *hdns,
                         double *didt, double *dibndt, double *bnes, double *dbdt, double *eir, double *eip,
                         int *outside_mgnp) {

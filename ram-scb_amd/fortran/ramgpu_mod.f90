@@ -289,6 +289,18 @@ module ramgpu_mod
        real(c_double), value :: time_elapsed
      end function ramgpu_set_time
 
+     integer(c_int) function ramgpu_set_told(ctx, told) bind(C, name="ramgpu_set_told")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), value :: told
+     end function ramgpu_set_told
+
+     integer(c_int) function ramgpu_get_told(ctx, told) bind(C, name="ramgpu_get_told")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       real(c_double), intent(out) :: told
+     end function ramgpu_get_told
+
      integer(c_int) function ramgpu_get_time(ctx, time_elapsed) bind(C, name="ramgpu_get_time")
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: ctx

@@ -41,7 +41,7 @@ EXPORTS = (
     "ramgpu_set_coulomb", "ramgpu_set_necr", "ramgpu_set_time", "ramgpu_get_time", "ramgpu_coulpara",
     "ramgpu_get_coulomb_tables", "ramgpu_coulen", "ramgpu_coulmu",
     "ramgpu_set_wpadif", "ramgpu_wpadif", "ramgpu_set_flc", "ramgpu_flcscatter",
-    "ramgpu_get_geometry", "ramgpu_computehI",
+    "ramgpu_get_geometry", "ramgpu_computehI", "ramgpu_set_told", "ramgpu_get_told",
 )
 
 KERNEL_KINDS = ("prepare", "r", "plane", "finalize", "vs", "p", "r_bwd", "p_bwd", "plane_bwd",
@@ -84,6 +84,8 @@ def load_library(path: Path | None = None):
     lib.ramgpu_set_grid.argtypes = [v] + [C.c_double] * 4 + [_dp] * 12
     lib.ramgpu_set_geometry.argtypes = [v] + [_dp] * 11 + [_ip]
     lib.ramgpu_get_geometry.argtypes = [v] + [_dp] * 11 + [_ip]
+    lib.ramgpu_set_told.argtypes = [v, C.c_double]
+    lib.ramgpu_get_told.argtypes = [v, _dp]
     lib.ramgpu_computehI.argtypes = [v, C.c_int, C.c_int] + [_dp] * 8 + [_ip, _ip, C.c_double, C.c_int]
     lib.ramgpu_set_efield.argtypes = [v, _dp]
     lib.ramgpu_set_kp.argtypes = [v, C.c_double, _dp, _dp]
@@ -347,6 +349,14 @@ class RamGpu:
         self._check(self.lib.ramgpu_get_geometry(self.h, *[_p(out[k]) for k in keys], mg.ctypes.data_as(_ip)))
         out["outside_mgnp"] = mg
         return out
+
+    def set_told(self, told):
+        self._check(self.lib.ramgpu_set_told(self.h, float(told)))
+
+    def told(self) -> float:
+        x = C.c_double()
+        self._check(self.lib.ramgpu_get_told(self.h, C.byref(x)))
+        return x.value
 
     def computehI(self, lines, pa, pabn, lz, outside_mgnp=None, scale_at=None, smooth=True):
         """computehI from field lines on the RAM grid (``fieldlines.Lines``:

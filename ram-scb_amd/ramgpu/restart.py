@@ -138,7 +138,7 @@ GEOM_KEYS = {"FNHS": "fnhs", "BOUNHS": "bounhs", "FNIS": "fnis", "BOUNIS": "boun
 
 
 def save(path, dev, problem, niter=0, dts_next=0.0, ppart=None, pperT=None):
-    """Write the restart of a device context: its F2 and TimeRamElapsed, the
+    """Write the restart of a device context: its F2, TimeRamElapsed and TOld, the
     problem's grid, field, geometry and plasmasphere density, and the
     caller's DtNext and (NS,NR,NT) PPART/PPERT (from ``RamGpu.moments``,
     which the caller runs where ram_run's ANISCH block runs, as it rewrites
@@ -155,13 +155,13 @@ def save(path, dev, problem, niter=0, dts_next=0.0, ppart=None, pperT=None):
     write(path, f2=dev.get_f2(), ekev=g.ekev, pa=g.pa, vt=problem.vt,
           geometry={k: getattr(geo, v) for k, v in GEOM_KEYS.items()},
           necr=coul.necr if coul is not None else None, ppart=four(ppart), pperT=four(pperT),
-          dts_next=dts_next, time_elapsed=dev.time(), niter=niter)
+          dts_next=dts_next, told=dev.told(), time_elapsed=dev.time(), niter=niter)
 
 
 def load_into(path, dev, problem):
     """Resume a device context from a restart: F2, the field and geometry
     rows I = 2..NR+1 (row 1 kept from the problem), NECR when Coulomb
-    collisions are on, and TimeRamElapsed."""
+    collisions are on, TimeRamElapsed and TOld."""
     r = read(path)
     geo = problem.geom
     g = {}
@@ -179,4 +179,5 @@ def load_into(path, dev, problem):
     if getattr(problem, "coulomb", None) is not None:
         dev.set_coulomb(problem.grid.vbnd, r["necr"], None)
     dev.set_time(r["time_elapsed"])
+    dev.set_told(r["TOld"])
     return r

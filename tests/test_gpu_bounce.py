@@ -75,6 +75,14 @@ def test_computehI_matches_oracle(dims, smooth, beyond):
     want2 = _oracle(p, lines, got, 300.0, smooth, outside, scale_at)
     _compare(got2, want2, f"{dims} smooth={smooth} second call")
     assert np.abs(got2["didt"]).max() > 0
+    assert g.told() == 300.0
+    # TOld as a restart restores it: DthI = 600 - 500
+    g.set_time(600.0)
+    g.set_told(500.0)
+    lines.b = lines.b * 0.995
+    g.computehI(lines, p.grid.pa, p.grid.pabn, p.grid.lz, outside, scale_at, smooth)
+    _compare(g.get_geometry(), _oracle(p, lines, got2, 100.0, smooth, outside, scale_at),
+             f"{dims} smooth={smooth} after set_told")
     g.close()
 
 
