@@ -1228,14 +1228,29 @@ __device__ void plane_coul_tables(const Tab &t, const FwdCtx &c, const PlaneFwd 
 // next four in flight while these are used.
 __device__ void plane_coulmu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, bool clamp) {
   const int NE = c.NE, NPA = c.NPA, S = c.S;
+  const double *xb = d.cx + NPA, *fh = d.cx + 2 * NPA, *bh = d.cx + 3 * NPA, *mu = d.cx + 4 * NPA;
+  const double *rfh = d.cx + 5 * NPA, *rmu = d.cx + 6 * NPA;
+  const int last = NPA - 2;  // row of L = NPA-1
+  {
+    // the block first forms each cell's GN (into its RL slot, which the
+    // recurrence reads before it writes RL there) and RP (in place); they do
+    // not depend on the recurrence, which is left AN, DENOM, RK and RL
+    const double2 *cg = t.cag + (size_t)c.s * NPA * NE;
+    const int w = NE - 1, ncell = w * last;
+    for (int e = threadIdx.x; e < ncell; e += blockDim.x) {
+      const int q = e / w, l = 1 + q, kk = 1 + (e - q * w);
+      const double f = fh[l], rf = rfh[l];
+      d.cx[7 * NPA + l * NE + kk] = div_rn(cg[l * NE + kk].y * xb[l - 1], f, rf) * bh[l - 1];
+      double *cl = d.F + kk + 1 + l * S;
+      *cl = div_rn(div_rn(*cl, f, rf), mu[l], rmu[l]);
+    }
+  }
+  __syncthreads();
   const int k = threadIdx.x;
   if (k >= 1 && k < NE) {
-    double *cell = d.F + k + 1;  // cell (K, row l) is cell[l*S]
-    const double *xb = d.cx + NPA, *fh = d.cx + 2 * NPA, *bh = d.cx + 3 * NPA, *mu = d.cx + 4 * NPA;
-    const double *rfh = d.cx + 5 * NPA, *rmu = d.cx + 6 * NPA;
-    double *rlv = d.cx + 7 * NPA + k;  // RL of row l at rlv[l*NE]
+    double *cell = d.F + k + 1;  // cell (K, row l) is cell[l*S]; RP there on entry
+    double *rlv = d.cx + 7 * NPA + k;  // GN, then RL, of row l at rlv[l*NE]
     const gptr<const double> ag = as_global((const double *)(t.cag + (size_t)c.s * NPA * NE + k));
-    const int last = NPA - 2;  // row of L = NPA-1
     double rk = 0.0, rl = -1.0;
     cell[0] = 0.0;
     rlv[0] = -1.0;
@@ -1252,11 +1267,10 @@ __device__ void plane_coulmu(const Tab &t, const FwdCtx &c, const PlaneFwd &d, b
       for (int m = 0; m < B; ++m) {
         const int l = l0 + m;
         if (l > last) break;
-        const double f = fh[l], rf = rfh[l];
-        const double an = div_rn(cur[m].x * xb[l], f, rf) * bh[l];
-        const double gn = div_rn(cur[m].y * xb[l - 1], f, rf) * bh[l - 1];
+        const double an = div_rn(cur[m].x * xb[l], fh[l], rfh[l]) * bh[l];
+        const double gn = rlv[l * NE];
         const double bn = an + gn;
-        const double rp = div_rn(div_rn(cell[l * S], f, rf), mu[l], rmu[l]);
+        const double rp = cell[l * S];
         const double den = bn + gn * rl + 1;
         const double rden = 1.0 / den;
         rk = div_rn(rp + gn * rk, den, rden);
